@@ -1,0 +1,799 @@
+// KIWI oracle: a plain C++ (CPU, FP64) restatement of the reference's kernel-integral mapping.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP path in kiwi_amd/csrc is compared
+// against; it is imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+// and never by the product path (which fails loudly when its HIP library is missing).
+//
+// Each function restates one reference function and cites it.  Arithmetic order follows the
+// reference closely on purpose (the point of an oracle is to reproduce the reference's numbers to
+// the ulp level), but the code, data layout and names are this project's own.
+//
+// Parity pins (how each part is checked; see tests/test_oracle.py):
+//   m4 tiers / face walk  : oracle/_ref (the reference built from source) on random inputs, closed forms
+//                           (enclosing box = 1, partition additivity), brute-force quadrature
+//   generic tiers         : oracle/_ref, quadrature, normalisation, generic == cubic for M4
+//   sph-to-grid loop      : golden fixtures written by the reference CLI (tests/golden), mass conservation
+//   Voronoi BFS           : the lattice cross-check (Voronoi cells of a cubic lattice are the Cartesian
+//                           cells) and the reference CLI's own `map` output on a small fixture
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../kiwi_amd/csrc/kernel_tables.h"
+
+namespace {
+
+const KiwiKernelTable kKiwiKernelTables[KIWI_TAB_NKERNELS] = {KIWI_KERNEL_TABLE_DATA};
+
+// ------------------------------------------------------------------------------------------------
+// Cubic spline (M4), analytic: the three tiers of src/kernel_integral.cpp:222-400.
+// ------------------------------------------------------------------------------------------------
+
+struct M4Wall {  // include/kiwi/kernel_integral.hpp:209-214
+    bool ok = false;
+    double h = 0, r0 = 0, h2 = 0, r03 = 0, q2 = 0, q3 = 0, qi2 = 0, qi3 = 0;
+    double B1 = 0, B2 = 0, B3 = 0;
+};
+
+struct M4Edge {  // include/kiwi/kernel_integral.hpp:216-221
+    bool ok = false;
+    double R0 = 0, a = 0, a2 = 0, D2 = 0, D3 = 0;
+};
+
+// The I-series at angle phi (src/kernel_integral.cpp:278-294, 367-386; identical in all three places).
+struct M4Series {
+    double I0, Im2, Im4, I1, Im1, Im3, Im5;
+};
+
+inline void sincos_pair(double x, double& s, double& c) {
+#if defined(__GLIBC__)
+    ::sincos(x, &s, &c);  // what the reference uses on glibc (src/kernel_integral.cpp:200-211)
+#else
+    s = std::sin(x);
+    c = std::cos(x);
+#endif
+}
+
+M4Series m4_series(double phi, double a, double a2) {
+    double sn, cs;
+    sincos_pair(phi, sn, cs);
+    const double c2 = cs * cs;
+    const double mu = cs / a / std::sqrt(1. + c2 / a2);
+    const double tn = sn / cs;
+    M4Series s;
+    s.I0 = phi;
+    s.Im2 = phi + a2 * tn;
+    s.Im4 = phi + 2 * a2 * tn + 1. / 3. * a2 * a2 * tn * (2. + 1. / c2);
+    const double u = sn * std::sqrt(1 - mu * mu);
+    const double lg = std::log(1 + u) - std::log(1 - u);
+    s.I1 = std::atan(u / a);
+    s.Im1 = a / 2. * lg + s.I1;
+    s.Im3 = s.Im1 + a * (1. + a2) / 4. * (2 * u / (1 - u * u) + lg);
+    s.Im5 = s.Im3 + a * (1. + a2) * (1. + a2) / 16. * ((10 * u - 6 * u * u * u) / (1 - u * u) / (1 - u * u) + 3. * lg);
+    return s;
+}
+
+// compute_wall_constants, src/kernel_integral.cpp:222-247
+M4Wall m4_wall(double h, double r0) {
+    M4Wall w;
+    w.h = h;
+    w.r0 = r0;
+    if (r0 == 0.0) return w;
+    w.ok = true;
+    w.h2 = h * h;
+    w.r03 = r0 * r0 * r0;
+    w.q2 = r0 / h * r0 / h;
+    w.q3 = w.q2 * r0 / h;
+    w.qi2 = h / r0 * h / r0;
+    w.qi3 = w.qi2 * h / r0;
+    if (r0 >= 2.0 * h) {
+        w.B3 = w.h2 * h / 4.;
+    } else if (r0 > h) {
+        const double common = -4. / 3. + (r0 / h) - 0.3 * w.q2 + 1. / 30. * w.q3 - 1. / 15. * w.qi3;
+        w.B3 = w.r03 / 4. * (common + 8. / 5. * w.qi2);
+        w.B2 = w.r03 / 4. * common;
+    } else {
+        const double common = -2. / 3. + 0.3 * w.q2 - 0.1 * w.q3;
+        w.B3 = w.r03 / 4. * (common + 7. / 5. * w.qi2);
+        w.B2 = w.r03 / 4. * (common - 1. / 5. * w.qi2);
+        w.B1 = w.r03 / 4. * common;
+    }
+    return w;
+}
+
+// The three per-region vertex forms inside the r0h3/pi bracket (src/kernel_integral.cpp:389-397),
+// without the D offsets; the edge tier takes differences of them at the crossing angles.
+inline double m4_inner(const M4Wall& w, const M4Series& s) {
+    return 1. / 6. * s.Im2 - 3. / 40. * w.q2 * s.Im4 + 1. / 40. * w.q3 * s.Im5 + w.B1 / w.r03 * s.I0;
+}
+
+// compute_edge_constants, src/kernel_integral.cpp:253-350
+M4Edge m4_edge(const M4Wall& w, double R0) {
+    M4Edge e;
+    e.R0 = R0;
+    if (!w.ok || R0 == 0.0) return e;
+    e.ok = true;
+    const double h = w.h, r0 = w.r0;
+    e.a = R0 / r0;
+    e.a2 = e.a * e.a;
+    const double line2 = r0 * r0 + R0 * R0;
+    double D2 = 0.0, D3 = 0.0;
+    if (line2 <= w.h2) {
+        const double p1 = std::acos(R0 / std::sqrt(h * h - r0 * r0));
+        const M4Series s = m4_series(p1, e.a, e.a2);
+        D2 = -1. / 6. * s.Im2 + 0.25 * (r0 / h) * s.Im3 - 0.15 * w.q2 * s.Im4 + 1. / 30. * w.q3 * s.Im5 -
+             1. / 60. * w.qi3 * s.I1 + (w.B1 - w.B2) / w.r03 * s.I0;
+    }
+    if (line2 <= 4.0 * w.h2) {
+        const double p2 = std::acos(R0 / std::sqrt(4.0 * h * h - r0 * r0));
+        const M4Series s = m4_series(p2, e.a, e.a2);
+        D3 = 1. / 3. * s.Im2 - 0.25 * (r0 / h) * s.Im3 + 3. / 40. * w.q2 * s.Im4 - 1. / 120. * w.q3 * s.Im5 +
+             4. / 15. * w.qi3 * s.I1 + (w.B2 - w.B3) / w.r03 * s.I0 + D2;
+    }
+    e.D2 = D2;
+    e.D3 = D3;
+    return e;
+}
+
+// vertex_integral, src/kernel_integral.cpp:356-400
+double m4_vertex(const M4Wall& w, const M4Edge& e, double phi) {
+    if (!w.ok || !e.ok) return 0.0;
+    if (phi == 0.0) return 0.0;
+    double sn, cs;
+    sincos_pair(phi, sn, cs);
+    const double R = e.R0 / cs;
+    const double r2 = w.r0 * w.r0 + R * R;
+    const M4Series s = m4_series(phi, e.a, e.a2);
+    const double r0 = w.r0, h = w.h;
+    if (r2 < w.h2) return w.q3 / M_PI * m4_inner(w, s);
+    if (r2 < 4.0 * w.h2)
+        return w.q3 / M_PI *
+               (0.25 * (4. / 3. * s.Im2 - (r0 / h) * s.Im3 + 0.3 * w.q2 * s.Im4 - 1. / 30. * w.q3 * s.Im5 +
+                        1. / 15. * w.qi3 * s.I1) +
+                w.B2 / w.r03 * s.I0 + e.D2);
+    return w.q3 / M_PI * (-0.25 * w.qi3 * s.I1 + w.B3 / w.r03 * s.I0 + e.D3);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Tabulated cubic path: generate_density_table + gridded_integral, src/kernel_integral.cpp:57-190.
+// ------------------------------------------------------------------------------------------------
+
+struct M4Table {
+    int nr01 = 50, nr02 = 199, nR01 = 150, nR02 = 150, nph1 = 150, nph2 = 150;
+    double rl = 0.1, mul = 0.98, cphil = 0.98;
+    int nr0 = 249, nR0 = 300, nph = 300;
+    std::vector<double> v;  // (nr0+2) x (nR0+1) x (nph+1)
+    double& at(int i, int j, int k) { return v[(static_cast<size_t>(i) * (nR0 + 1) + j) * (nph + 1) + k]; }
+    double at(int i, int j, int k) const { return v[(static_cast<size_t>(i) * (nR0 + 1) + j) * (nph + 1) + k]; }
+};
+
+double m4_full(double h, double phi, double r0, double R0);
+
+const M4Table& m4_table() {
+    static const M4Table t = [] {
+        M4Table t;
+        t.v.assign(static_cast<size_t>(t.nr0 + 2) * (t.nR0 + 1) * (t.nph + 1), 0.0);
+        const int n = t.nR01;
+        for (int i = 1; i <= t.nr0; ++i) {
+            const double r0 = (i <= t.nr01) ? (t.rl / t.nr01) * i : t.rl + ((2.0 - t.rl) / t.nr02) * (i - t.nr01);
+            for (int j = 0; j < t.nR0; ++j) {
+                const double mu0 = (j < n) ? (t.mul / (n - 1)) * j : t.mul + ((1.0 - t.mul) / n) * (j - n + 1);
+                const double R0 = r0 * std::sqrt(1.0 - mu0 * mu0) / mu0;  // j = 0 gives +inf on purpose
+                for (int k = 0; k < t.nph; ++k) {
+                    const double phi = (k < n) ? std::acos((t.cphil / (n - 1)) * k)
+                                               : std::acos(t.cphil + ((1.0 - t.cphil) / n) * (k - n + 1));
+                    t.at(i, j, k) = m4_full(1.0, phi, r0, R0);
+                }
+            }
+        }
+        for (int j = 0; j < t.nR0; ++j)
+            for (int k = 0; k < t.nph; ++k) t.at(t.nr0 + 1, j, k) = t.at(t.nr0, j, k);
+        return t;
+    }();
+    return t;
+}
+
+int clamp_idx(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// gridded_integral, src/kernel_integral.cpp:136-190 (the clamp warning of :125-133 is not printed)
+double m4_gridded(const M4Table& t, double h, double phi, double r0_in, double R0_in) {
+    if (r0_in == 0.0 || R0_in == 0.0 || phi == 0.0) return 0.0;
+    double r0 = r0_in / h, R0 = R0_in / h;
+    const double cphi = std::cos(phi);
+    if (r0 < 0.0) r0 = 0.0;
+    if (R0 < 0.0) R0 = 0.0;
+    const double mu0 = r0 / std::sqrt(r0 * r0 + R0 * R0);
+    if (r0 > 2.0) r0 = 2.0;
+    int i, j, k;
+    if (r0 < t.rl) i = static_cast<int>(r0 / (t.rl / t.nr01));
+    else i = t.nr01 + static_cast<int>((r0 - t.rl) / ((2.0 - t.rl) / t.nr02));
+    if (mu0 < t.mul) j = static_cast<int>(mu0 / (t.mul / (t.nR01 - 1)));
+    else j = t.nR01 - 1 + static_cast<int>((mu0 - t.mul) / ((1.0 - t.mul) / t.nR02));
+    if (cphi < t.cphil) k = static_cast<int>(cphi / (t.cphil / (t.nph1 - 1)));
+    else k = t.nph1 - 1 + static_cast<int>((cphi - t.cphil) / ((1.0 - t.cphil) / t.nph2));
+    i = clamp_idx(i, 0, t.nr0);
+    j = clamp_idx(j, 0, t.nR0 - 1);
+    k = clamp_idx(k, 0, t.nph - 1);
+    double fr;
+    if (r0 < t.rl) fr = (r0 - (t.rl / t.nr01) * i) / (t.rl / t.nr01);
+    else fr = (r0 - t.rl - ((2.0 - t.rl) / t.nr02) * (i - t.nr01)) / ((2.0 - t.rl) / t.nr02);
+    const double fx1 = fr * t.at(i + 1, j, k) + (1 - fr) * t.at(i, j, k);
+    const double fx2 = fr * t.at(i + 1, j + 1, k) + (1 - fr) * t.at(i, j + 1, k);
+    const double fx3 = fr * t.at(i + 1, j, k + 1) + (1 - fr) * t.at(i, j, k + 1);
+    const double fx4 = fr * t.at(i + 1, j + 1, k + 1) + (1 - fr) * t.at(i, j + 1, k + 1);
+    if (mu0 < t.mul) fr = (mu0 - t.mul / (t.nR01 - 1) * j) / (t.mul / (t.nR01 - 1));
+    else fr = (mu0 - t.mul - (1.0 - t.mul) / t.nR02 * (j - t.nR01 + 1)) / ((1.0 - t.mul) / t.nR02);
+    const double fy1 = fr * fx2 + (1 - fr) * fx1;
+    const double fy2 = fr * fx4 + (1 - fr) * fx3;
+    if (cphi < t.cphil) fr = (cphi - t.cphil / (t.nph1 - 1) * k) / (t.cphil / (t.nph1 - 1));
+    else fr = (cphi - t.cphil - (1.0 - t.cphil) / t.nph2 * (k - t.nph1 + 1)) / ((1.0 - t.cphil) / t.nph2);
+    double fz = fr * fy2 + (1 - fr) * fy1;
+    if (j == t.nR01 + t.nR02 - 1 || k == t.nph1 + t.nph2 - 1) fz = 0.0;
+    return fz;
+}
+
+// full_integral_new, src/kernel_integral.cpp:410-418
+double m4_full(double h, double phi, double r0, double R0) {
+    if (r0 == 0.0 || R0 == 0.0 || phi == 0.0) return 0.0;
+    const M4Wall w = m4_wall(h, r0);
+    const M4Edge e = m4_edge(w, R0);
+    return m4_vertex(w, e, phi);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Generic piecewise-polynomial kernels: src/kernel_integral_generic.cpp:19-163, driven by the
+// coefficient tables of kiwi_amd/csrc/kernel_tables.h (our own exact re-derivation).
+// ------------------------------------------------------------------------------------------------
+
+constexpr int NI = KIWI_TAB_NI;
+constexpr int NREG = KIWI_TAB_MAXREG;
+
+struct GenWall {  // include/kiwi/kernel_integral_generic.hpp:27-35
+    bool ok = false;
+    const KiwiKernelTable* K = nullptr;
+    double H = 0, r0 = 0, q0 = 0;
+    int j = 0;
+    double coef[NREG][NI];
+    double bphi[NREG];
+};
+
+struct GenEdge {  // include/kiwi/kernel_integral_generic.hpp:37-44
+    bool ok = false;
+    double R0 = 0, a = 0, a2 = 0;
+    int nseg = 0;
+    int region[NREG];
+    double phi_end[NREG];
+    double offset[NREG];
+};
+
+// basis_integrals, src/kernel_integral_generic.cpp:19-66: I_i(phi) at out[1-i] for i = 1 .. imin.
+void gen_basis(double a, double a2, double phi, int imin, double* out) {
+    const double sn = std::sin(phi), cs = std::cos(phi);
+    const double den = a2 + cs * cs;
+    const double u = a * sn / std::sqrt(den);
+    const double omu2 = cs * cs * (1.0 + a2) / den;
+    out[0] = std::atan(u / a);
+    out[1] = phi;
+    const int mmax = (-imin - 1) / 2;
+    double Iodd = out[0];
+    double K = std::atanh(u);
+    double p1a2 = 1.0;
+    for (int m = 0; m <= mmax && -2 * m - 1 >= imin; ++m) {
+        if (m > 0) {
+            K = u / (2.0 * m * std::pow(omu2, m)) + (2.0 * m - 1.0) / (2.0 * m) * K;
+            p1a2 *= (1.0 + a2);
+        }
+        Iodd += a * p1a2 * K;
+        out[2 + 2 * m] = Iodd;
+    }
+    const int kmax = -imin / 2;
+    if (kmax >= 1) {
+        double J[NI];
+        const double t = sn / cs, sec = 1.0 / cs;
+        J[0] = phi;
+        J[2] = t;
+        double sp = 1.0;
+        for (int l = 2; l <= kmax; ++l) {
+            const int m = 2 * l;
+            sp *= sec * sec;
+            J[m] = sp * t / (m - 1.0) + (m - 2.0) / (m - 1.0) * J[m - 2];
+        }
+        for (int k = 1; k <= kmax; ++k) {
+            double sum = J[0], ap = 1.0, b = 1.0;
+            for (int l = 1; l <= k; ++l) {
+                b = b * (k - l + 1) / l;
+                ap *= a2;
+                sum += b * ap * J[2 * l];
+            }
+            out[1 + 2 * k] = sum;
+        }
+    }
+}
+
+// region_of, src/kernel_integral_generic.cpp:68-72 (1-based)
+int gen_region(const KiwiKernelTable& K, double q) {
+    for (int k = 0; k < K.nreg - 1; ++k)
+        if (q <= K.c[k]) return k + 1;
+    return K.nreg;
+}
+
+// imin_of, src/kernel_integral_generic.cpp:75-79
+int gen_imin(const GenWall& w, int k) {
+    for (int idx = NI - 1; idx >= 0; --idx)
+        if (w.coef[k - 1][idx] != 0.0) return 1 - idx;
+    return 1;
+}
+
+// G, src/kernel_integral_generic.cpp:82-92
+double gen_G(const GenWall& w, const GenEdge& e, int k, double phi) {
+    double I[NI];
+    gen_basis(e.a, e.a2, phi, gen_imin(w, k), I);
+    double sum = w.bphi[k - 1] * phi;
+    for (int idx = 0; idx < NI; ++idx) {
+        const double c = w.coef[k - 1][idx];
+        if (c != 0.0) sum += c * I[idx];
+    }
+    return sum;
+}
+
+// generic_wall_constants, src/kernel_integral_generic.cpp:96-117
+GenWall gen_wall(const KiwiKernelTable& K, double H, double r0) {
+    GenWall w;
+    w.K = &K;
+    w.H = H;
+    w.r0 = r0;
+    if (r0 == 0.0) return w;
+    w.ok = true;
+    w.q0 = r0 / H;
+    w.j = gen_region(K, w.q0);
+    const double A3 = K.A3_pi / M_PI;
+    for (int k = 1; k <= K.nreg; ++k) {
+        for (int idx = 0; idx < NI; ++idx) {
+            const double av = K.a[k - 1][idx];
+            w.coef[k - 1][idx] = (av == 0.0) ? 0.0 : A3 * av * std::pow(w.q0, idx);
+        }
+        double b = 0.0;
+        if (k >= w.j)
+            for (int p = KIWI_TAB_NB - 1; p >= 0; --p) b = b * w.q0 + K.B[w.j - 1][k - 1][p];
+        w.bphi[k - 1] = A3 * b;
+    }
+    return w;
+}
+
+// generic_edge_constants, src/kernel_integral_generic.cpp:119-148
+GenEdge gen_edge(const GenWall& w, double R0) {
+    GenEdge e;
+    e.R0 = R0;
+    if (!w.ok || R0 == 0.0) return e;
+    e.ok = true;
+    e.a = R0 / w.r0;
+    e.a2 = e.a * e.a;
+    const KiwiKernelTable& K = *w.K;
+    const double qR = R0 / w.H;
+    const double rs = std::sqrt(w.q0 * w.q0 + qR * qR);
+    int k = gen_region(K, rs);
+    e.nseg = 0;
+    for (; k <= K.nreg; ++k) {
+        e.region[e.nseg] = k;
+        if (k == K.nreg) {
+            e.phi_end[e.nseg] = 0.5 * M_PI;
+        } else {
+            const double c = K.c[k - 1];
+            e.phi_end[e.nseg] = std::acos(std::fmin(1.0, qR / std::sqrt(c * c - w.q0 * w.q0)));
+        }
+        ++e.nseg;
+    }
+    e.offset[0] = 0.0;
+    for (int s = 0; s + 1 < e.nseg; ++s) {
+        const double pb = e.phi_end[s];
+        e.offset[s + 1] = e.offset[s] + gen_G(w, e, e.region[s], pb) - gen_G(w, e, e.region[s + 1], pb);
+    }
+    return e;
+}
+
+// generic_vertex_integral, src/kernel_integral_generic.cpp:150-156
+double gen_vertex(const GenWall& w, const GenEdge& e, double phi) {
+    if (!w.ok || !e.ok) return 0.0;
+    if (phi == 0.0) return 0.0;
+    int s = 0;
+    while (s + 1 < e.nseg && phi > e.phi_end[s]) ++s;
+    return e.offset[s] + gen_G(w, e, e.region[s], phi);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Face walk: face_integral_impl, src/kernel_integral.cpp:471-647, over an evaluator that supplies
+// the wall / edge / vertex tiers (src/kernel_integral.cpp:433-467).
+// ------------------------------------------------------------------------------------------------
+
+struct EvalM4 {
+    double h;
+    const M4Table* table;  // non-null: tabulated path
+    M4Wall w;
+    M4Edge e;
+    double ar0 = 0, R0 = 0;
+    void wall(double r) {
+        ar0 = r;
+        if (!table) w = m4_wall(h, r);
+    }
+    void edge(double R) {
+        R0 = R;
+        if (!table) e = m4_edge(w, R);
+    }
+    double at(double phi) const { return table ? m4_gridded(*table, h, phi, ar0, R0) : m4_vertex(w, e, phi); }
+};
+
+struct EvalGen {
+    const KiwiKernelTable* K;
+    double H;
+    GenWall w;
+    GenEdge e;
+    void wall(double r) { w = gen_wall(*K, H, r); }
+    void edge(double R) { e = gen_edge(w, R); }
+    double at(double phi) const { return gen_vertex(w, e, phi); }
+};
+
+template <class Ev>
+double face_walk(double x0, double y0, double z0, double support, const double* v, int n, int* out_vert, Ev& ev) {
+    if (n < 3) {
+        if (out_vert) *out_vert = 0;
+        return 0.0;
+    }
+    const double x1 = v[0], y1 = v[1], z1 = v[2];
+    const double x2s = v[3], y2s = v[4], z2s = v[5];
+    const double x3s = v[6], y3s = v[7], z3s = v[8];
+    // plane through the first three vertices (:495-506)
+    const double A = (y2s - y1) * (z3s - z1) - (y3s - y1) * (z2s - z1);
+    const double B = (z2s - z1) * (x3s - x1) - (z3s - z1) * (x2s - x1);
+    const double C = (x2s - x1) * (y3s - y1) - (x3s - x1) * (y2s - y1);
+    const double D = -A * x1 - B * y1 - C * z1;
+    const double nrm = std::sqrt(A * A + B * B + C * C);
+    const double r0 = (A * x0 + B * y0 + C * z0 + D) / nrm;
+    const double ar0 = std::fabs(r0);
+    const double xp = x0 - r0 * A / nrm, yp = y0 - r0 * B / nrm, zp = z0 - r0 * C / nrm;
+    // winding sign reference, triple product about the particle (:525-529)
+    const double ax = x1 - x0, ay = y1 - y0, az = z1 - z0;
+    const double bx = x2s - x0, by = y2s - y0, bz = z2s - z0;
+    const double cx = x3s - x0, cy = y3s - y0, cz = z3s - z0;
+    const double s2 = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
+    ev.wall(ar0);
+
+    bool inside = true, have_sign = false;
+    double sign_ref = 0.0;
+    double min_edge = std::numeric_limits<double>::infinity();
+    double total = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const double x2 = v[3 * j], y2 = v[3 * j + 1], z2 = v[3 * j + 2];
+        const int jn = (j == n - 1) ? 0 : j + 1;
+        const double x3 = v[3 * jn], y3 = v[3 * jn + 1], z3 = v[3 * jn + 2];
+        // edge geometry (:573-583)
+        const double r23 = std::sqrt((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2) + (z3 - z2) * (z3 - z2));
+        const double r12 = std::sqrt((xp - x2) * (xp - x2) + (yp - y2) * (yp - y2) + (zp - z2) * (zp - z2));
+        const double r13 = std::sqrt((xp - x3) * (xp - x3) + (yp - y3) * (yp - y3) + (zp - z3) * (zp - z3));
+        const double cosa = ((x3 - x2) * (xp - x2) + (y3 - y2) * (yp - y2) + (z3 - z2) * (zp - z2)) / r12 / r23;
+        const double R0 = (std::fabs(cosa) < 1.0) ? r12 * std::sqrt(1 - cosa * cosa) : 0.0;
+        // s1 about the particle (:588-591)
+        const double px = xp - x0, py = yp - y0, pz = zp - z0;
+        const double ux = x2 - x0, uy = y2 - y0, uz = z2 - z0;
+        const double wx = x3 - x0, wy = y3 - y0, wz = z3 - z0;
+        const double s1 = px * (uy * wz - uz * wy) + py * (uz * wx - ux * wz) + pz * (ux * wy - uy * wx);
+        const double phi1 = (R0 < r12) ? std::acos(R0 / r12) : 0.0;
+        const double phi2 = (R0 < r13) ? std::acos(R0 / r13) : 0.0;
+        // inside-polygon side test (:600-611)
+        const double ex = x3 - x2, ey = y3 - y2, ez = z3 - z2;
+        const double tx = xp - x2, ty = yp - y2, tz = zp - z2;
+        const double side = (ey * tz - ez * ty) * A + (ez * tx - ex * tz) * B + (ex * ty - ey * tx) * C;
+        const double side_eps = 1e-9 * nrm * r23 * std::max({std::fabs(tx), std::fabs(ty), std::fabs(tz), 1.0});
+        if (std::fabs(side) > side_eps) {
+            if (!have_sign) {
+                sign_ref = side;
+                have_sign = true;
+            } else if ((side > 0) != (sign_ref > 0)) {
+                inside = false;
+            }
+        }
+        const bool beyond = (r12 * std::sin(phi1) >= r23) || (r13 * std::sin(phi2) >= r23);
+        min_edge = std::min(min_edge, beyond ? std::min(r12, r13) : R0);
+        double M = (s1 * s2 * r0 <= 0) ? -1.0 : 1.0;
+        ev.edge(R0);
+        if (beyond) {
+            M = (phi1 >= phi2) ? M * (ev.at(phi1) - ev.at(phi2)) : M * (ev.at(phi2) - ev.at(phi1));
+        } else {
+            M = M * (ev.at(phi1) + ev.at(phi2));
+        }
+        total += M;
+    }
+    // reachability of the face (:642-645)
+    const double fd = inside ? ar0 : std::sqrt(r0 * r0 + min_edge * min_edge);
+    if (out_vert) *out_vert = (fd >= support) ? n : 0;
+    return total;
+}
+
+// box_cell_faces, src/cartesian_grid.cpp:54-72: the 6 faces of a box, in the winding the face walk needs.
+void box_faces(double cx, double cy, double cz, double hx, double hy, double hz, double f[6][12]) {
+    const double xm = cx - hx, xp = cx + hx, ym = cy - hy, yp = cy + hy, zm = cz - hz, zp = cz + hz;
+    const double F[6][12] = {{xp, ym, zp, xp, yp, zp, xp, yp, zm, xp, ym, zm},
+                             {xm, yp, zm, xm, yp, zp, xm, ym, zp, xm, ym, zm},
+                             {xp, yp, zm, xp, yp, zp, xm, yp, zp, xm, yp, zm},
+                             {xm, ym, zp, xp, ym, zp, xp, ym, zm, xm, ym, zm},
+                             {xm, yp, zp, xp, yp, zp, xp, ym, zp, xm, ym, zp},
+                             {xp, ym, zm, xp, yp, zm, xm, yp, zm, xm, ym, zm}};
+    std::memcpy(f, F, sizeof(F));
+}
+
+// KernelIntegrator::face (include/kiwi/kernel_integrator.hpp:126-130)
+struct Integrator {
+    const KiwiKernelTable* gen = nullptr;  // null: cubic fast path
+    double support_in_h = 2.0;
+    bool tabulated = false;
+    double face(double x0, double y0, double z0, double h, const double* v, int n, int* out_vert) const {
+        if (gen) {
+            EvalGen ev{gen, support_in_h * h, {}, {}};
+            return face_walk(x0, y0, z0, support_in_h * h, v, n, out_vert, ev);
+        }
+        EvalM4 ev{h, tabulated ? &m4_table() : nullptr, {}, {}};
+        return face_walk(x0, y0, z0, 2 * h, v, n, out_vert, ev);
+    }
+};
+
+}  // namespace
+
+// ================================================================================================
+// C ABI (ctypes)
+// ================================================================================================
+
+extern "C" {
+
+struct OracleKernel {  // mirrors kiwi_kernel in include/kiwi_hip.h
+    int kernel_id;        // KIWI_KERNEL_M4 .. KIWI_KERNEL_WC6
+    double support_in_h;  // 0: the kernel's default radkern
+    int generic_path;     // force the generic path
+    int tabulated;        // cubic tabulated path
+};
+
+struct OracleStats {  // mirrors kiwi_stats
+    long long skipped_outside, n_negative, n_negative_warn, pairs_bbox, pairs_nonzero;
+    double v_mean[3];
+};
+
+int oracle_abi_version(void) { return 1; }
+
+double oracle_m4_vertex(double h, double phi, double r0, double R0) { return m4_full(h, phi, r0, R0); }
+
+double oracle_m4_gridded(double h, double phi, double r0, double R0) {
+    return m4_gridded(m4_table(), h, phi, r0, R0);
+}
+
+double oracle_generic_vertex(int kernel_id, double H, double phi, double r0, double R0) {
+    if (kernel_id < 0 || kernel_id >= KIWI_TAB_NKERNELS) return std::nan("");
+    if (r0 == 0.0 || R0 == 0.0 || phi == 0.0) return 0.0;
+    const GenWall w = gen_wall(kKiwiKernelTables[kernel_id], H, r0);
+    const GenEdge e = gen_edge(w, R0);
+    return gen_vertex(w, e, phi);
+}
+
+double oracle_face(const OracleKernel* k, double x0, double y0, double z0, double h, const double* verts,
+                   int nverts, int* out_vert) {
+    Integrator ki;
+    const KiwiKernelTable& K = kKiwiKernelTables[k->kernel_id];
+    ki.support_in_h = k->support_in_h > 0 ? k->support_in_h : K.radkern;
+    // Config::use_generic_kernel_path, src/config.cpp:275-277
+    if (k->generic_path || k->kernel_id != 0 || ki.support_in_h != 2.0) ki.gen = &K;
+    else ki.tabulated = k->tabulated != 0;
+    return ki.face(x0, y0, z0, h, verts, nverts, out_vert);
+}
+
+// The sph-to-grid particle loop (src/cartesian_grid.cpp:184-301): outputs are the UN-normalised sums
+// mass = sum Msum*m and p = sum Msum*m*v per cell, x-major idx = (ix*ny+iy)*nz+iz.  With threads > 1
+// the per-thread accumulators are reduced in thread order, as the reference does; threads == 1
+// reproduces the reference's OMP_NUM_THREADS=1 summation order exactly.
+int oracle_sph_to_grid(long long n, const double* x, const double* y, const double* z, const double* vx,
+                       const double* vy, const double* vz, const double* m, const double* h,
+                       const double* domain, int nx, int ny, int nz, const OracleKernel* k, int threads,
+                       double* mass, double* px, double* py, double* pz, OracleStats* st) {
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !k || k->kernel_id < 0 || k->kernel_id >= KIWI_TAB_NKERNELS) return -1;
+    const KiwiKernelTable& K = kKiwiKernelTables[k->kernel_id];
+    Integrator ki;
+    ki.support_in_h = k->support_in_h > 0 ? k->support_in_h : K.radkern;
+    if (k->generic_path || k->kernel_id != 0 || ki.support_in_h != 2.0) ki.gen = &K;
+    else ki.tabulated = k->tabulated != 0;
+    if (ki.tabulated) (void)m4_table();
+
+    const double xmin = domain[0], ymin = domain[2], zmin = domain[4];
+    const double dpx = (domain[1] - domain[0]) / nx, dpy = (domain[3] - domain[2]) / ny,
+                 dpz = (domain[5] - domain[4]) / nz;
+    const double hpx = 0.5 * dpx, hpy = 0.5 * dpy, hpz = 0.5 * dpz;
+    const size_t nc = static_cast<size_t>(nx) * ny * nz;
+    int nt = threads > 0 ? threads : 1;
+#ifndef _OPENMP
+    nt = 1;
+#endif
+    std::vector<std::vector<double>> acc(static_cast<size_t>(nt), std::vector<double>(4 * nc, 0.0));
+    double vmx = 0, vmy = 0, vmz = 0;
+    long long skipped = 0, nneg = 0, nwarn = 0, pbox = 0, pnz = 0;
+
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 256) \
+    reduction(+ : vmx, vmy, vmz, skipped, nneg, nwarn, pbox, pnz)
+#endif
+    for (long long i = 0; i < n; ++i) {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double* A = acc[static_cast<size_t>(tid)].data();
+        const double x0 = x[i], y0 = y[i], z0 = z[i], hh = h[i], mm = m[i];
+        vmx += vx[i];
+        vmy += vy[i];
+        vmz += vz[i];
+        if (hh <= 0.0) continue;
+        const double rad = ki.support_in_h * hh;
+        int ix0 = static_cast<int>(std::floor((x0 - rad - xmin) / dpx));
+        int ix1 = static_cast<int>(std::floor((x0 + rad - xmin) / dpx));
+        int iy0 = static_cast<int>(std::floor((y0 - rad - ymin) / dpy));
+        int iy1 = static_cast<int>(std::floor((y0 + rad - ymin) / dpy));
+        int iz0 = static_cast<int>(std::floor((z0 - rad - zmin) / dpz));
+        int iz1 = static_cast<int>(std::floor((z0 + rad - zmin) / dpz));
+        ix0 = std::max(ix0, 0);
+        ix1 = std::min(ix1, nx - 1);
+        iy0 = std::max(iy0, 0);
+        iy1 = std::min(iy1, ny - 1);
+        iz0 = std::max(iz0, 0);
+        iz1 = std::min(iz1, nz - 1);
+        if (ix0 > ix1 || iy0 > iy1 || iz0 > iz1) {
+            ++skipped;
+            continue;
+        }
+        pbox += static_cast<long long>(ix1 - ix0 + 1) * (iy1 - iy0 + 1) * (iz1 - iz0 + 1);
+        for (int ix = ix0; ix <= ix1; ++ix) {
+            const double cx = xmin + (ix + 0.5) * dpx;
+            for (int iy = iy0; iy <= iy1; ++iy) {
+                const double cy = ymin + (iy + 0.5) * dpy;
+                for (int iz = iz0; iz <= iz1; ++iz) {
+                    const double cz = zmin + (iz + 0.5) * dpz;
+                    double f[6][12];
+                    box_faces(cx, cy, cz, hpx, hpy, hpz, f);
+                    double Ms = 0.0;
+                    for (int q = 0; q < 6; ++q) Ms += ki.face(x0, y0, z0, hh, f[q], 4, nullptr);
+                    if (Ms < 0.0) {
+                        ++nneg;
+                        if (Ms <= -0.01) ++nwarn;
+                        Ms = 0.0;
+                    }
+                    if (Ms == 0.0) continue;
+                    ++pnz;
+                    const size_t c = (static_cast<size_t>(ix) * ny + iy) * nz + iz;
+                    A[c] += Ms * mm;
+                    A[nc + c] += Ms * mm * vx[i];
+                    A[2 * nc + c] += Ms * mm * vy[i];
+                    A[3 * nc + c] += Ms * mm * vz[i];
+                }
+            }
+        }
+    }
+    for (size_t c = 0; c < nc; ++c) {
+        double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        for (int t = 0; t < nt; ++t) {
+            const double* A = acc[static_cast<size_t>(t)].data();
+            s0 += A[c];
+            s1 += A[nc + c];
+            s2 += A[2 * nc + c];
+            s3 += A[3 * nc + c];
+        }
+        mass[c] = s0;
+        px[c] = s1;
+        py[c] = s2;
+        pz[c] = s3;
+    }
+    if (st) {
+        st->skipped_outside = skipped;
+        st->n_negative = nneg;
+        st->n_negative_warn = nwarn;
+        st->pairs_bbox = pbox;
+        st->pairs_nonzero = pnz;
+        st->v_mean[0] = n > 0 ? vmx / static_cast<double>(n) : 0.0;
+        st->v_mean[1] = n > 0 ? vmy / static_cast<double>(n) : 0.0;
+        st->v_mean[2] = n > 0 ? vmz / static_cast<double>(n) : 0.0;
+    }
+    return 0;
+}
+
+// Voronoi mapping BFS (src/mapping.cpp:105-206, 400-460) over flattened cell geometry:
+//   cell_face_ptr[c] .. cell_face_ptr[c+1]   faces of cell c (in voro++ face order)
+//   face_vert_ptr[f] .. face_vert_ptr[f+1]   absolute vertex coordinates (xyz triples) of face f
+//   face_nbr[f]                              neighbouring cell index, or -1 (domain wall / unresolved)
+//   home[i]                                  home cell of particle i, or -1 (unresolved -> skipped)
+//   pid[i]                                   the particle id used for the BFS visit stamps
+// Accumulates the same un-normalised sums as oracle_sph_to_grid, per cell.
+int oracle_voronoi_map(long long n, const double* x, const double* y, const double* z, const double* vx,
+                       const double* vy, const double* vz, const double* m, const double* h, const int* pid,
+                       const int* home, long long ncells, const long long* cell_face_ptr,
+                       const long long* face_vert_ptr, const double* face_xyz, const int* face_nbr,
+                       const OracleKernel* k, double* mass, double* px, double* py, double* pz, long long* counts) {
+    if (!k || k->kernel_id < 0 || k->kernel_id >= KIWI_TAB_NKERNELS) return -1;
+    const KiwiKernelTable& K = kKiwiKernelTables[k->kernel_id];
+    Integrator ki;
+    ki.support_in_h = k->support_in_h > 0 ? k->support_in_h : K.radkern;
+    if (k->generic_path || k->kernel_id != 0 || ki.support_in_h != 2.0) ki.gen = &K;
+    else ki.tabulated = k->tabulated != 0;
+
+    std::vector<int> stamp(static_cast<size_t>(ncells), 0), queue(static_cast<size_t>(ncells), 0);
+    std::unordered_map<int64_t, std::pair<int, double>> cache;
+    for (long long c = 0; c < ncells; ++c) mass[c] = px[c] = py[c] = pz[c] = 0.0;
+    long long unresolved = 0, unmapped = 0, nneg = 0, nwarn = 0, visited = 0, face_evals = 0;
+    for (long long i = 0; i < n; ++i) {
+        const double x0 = x[i], y0 = y[i], z0 = z[i], hh = h[i], mm = m[i];
+        const int ind = pid[i];
+        if (hh <= 0.0) continue;
+        const int hc = home[i];
+        if (hc < 0) {
+            ++unresolved;
+            continue;
+        }
+        int top = 1, bottom = 0;
+        queue[0] = hc;
+        stamp[static_cast<size_t>(hc)] = ind;
+        cache.clear();
+        while (bottom < top) {
+            const int cell = queue[static_cast<size_t>(bottom)];
+            double Ms = 0.0;
+            for (long long f = cell_face_ptr[cell]; f < cell_face_ptr[cell + 1]; ++f) {
+                const int nb = face_nbr[f];
+                int64_t key = 0;
+                bool cacheable = false;
+                if (nb >= 0) {
+                    const int lo = std::min(cell, nb), hi = std::max(cell, nb);
+                    key = (static_cast<int64_t>(lo) << 32) | static_cast<uint32_t>(hi);
+                    cacheable = true;
+                    auto it = cache.find(key);
+                    if (it != cache.end()) {
+                        Ms += (it->second.first == cell) ? it->second.second : -it->second.second;
+                        continue;
+                    }
+                }
+                const long long v0 = face_vert_ptr[f], nv = face_vert_ptr[f + 1] - v0;
+                int ov = 0;
+                const double fm = ki.face(x0, y0, z0, hh, face_xyz + 3 * v0, static_cast<int>(nv), &ov);
+                ++face_evals;
+                Ms += fm;
+                if (cacheable) cache.emplace(key, std::make_pair(cell, fm));
+                if (ov != nv && nb >= 0 && stamp[static_cast<size_t>(nb)] != ind) {
+                    queue[static_cast<size_t>(top++)] = nb;
+                    stamp[static_cast<size_t>(nb)] = ind;
+                }
+            }
+            if (Ms < 0.0) {
+                ++nneg;
+                if (Ms <= -0.01) ++nwarn;
+                Ms = 0.0;
+            }
+            mass[cell] += Ms * mm;
+            px[cell] += Ms * mm * vx[i];
+            py[cell] += Ms * mm * vy[i];
+            pz[cell] += Ms * mm * vz[i];
+            ++bottom;
+        }
+        visited += top;
+        if (top == 1) ++unmapped;
+    }
+    if (counts) {
+        counts[0] = unresolved;
+        counts[1] = unmapped;
+        counts[2] = nneg;
+        counts[3] = nwarn;
+        counts[4] = visited;
+        counts[5] = face_evals;
+    }
+    return 0;
+}
+
+}  // extern "C"

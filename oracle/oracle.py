@@ -1,0 +1,147 @@
+"""ctypes bindings for the CPU oracle (oracle/libkiwi_oracle.so) and, when it has been built here,
+for the reference itself (oracle/_ref/libkiwi_ref.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "libkiwi_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libkiwi_ref.so")
+REF_CLI = os.path.join(HERE, "_ref", "kiwi")
+
+KERNELS = ["M4", "M5", "M6", "WC2", "WC4", "WC6"]
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_ll = ctypes.c_longlong
+_pd = ctypes.POINTER(ctypes.c_double)
+_pi = ctypes.POINTER(ctypes.c_int)
+_pll = ctypes.POINTER(ctypes.c_longlong)
+
+
+class OracleKernel(ctypes.Structure):
+    _fields_ = [("kernel_id", _i), ("support_in_h", _d), ("generic_path", _i), ("tabulated", _i)]
+
+
+class OracleStats(ctypes.Structure):
+    _fields_ = [("skipped_outside", _ll), ("n_negative", _ll), ("n_negative_warn", _ll),
+                ("pairs_bbox", _ll), ("pairs_nonzero", _ll), ("v_mean", _d * 3)]
+
+
+def build():
+    """Compile the oracle (and nothing else) in-tree."""
+    import subprocess
+    root = os.path.dirname(HERE)
+    subprocess.run(["make", "-s", "-f", "oracle/Makefile", "oracle/libkiwi_oracle.so"], cwd=root, check=True)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle library not built: {ORACLE_SO} (run make -f oracle/Makefile)")
+        L = ctypes.CDLL(ORACLE_SO)
+        L.oracle_m4_vertex.restype = _d
+        L.oracle_m4_vertex.argtypes = [_d, _d, _d, _d]
+        L.oracle_m4_gridded.restype = _d
+        L.oracle_m4_gridded.argtypes = [_d, _d, _d, _d]
+        L.oracle_generic_vertex.restype = _d
+        L.oracle_generic_vertex.argtypes = [_i, _d, _d, _d, _d]
+        L.oracle_face.restype = _d
+        L.oracle_face.argtypes = [ctypes.POINTER(OracleKernel), _d, _d, _d, _d, _pd, _i, _pi]
+        L.oracle_sph_to_grid.restype = _i
+        L.oracle_sph_to_grid.argtypes = [_ll] + [_pd] * 8 + [_pd, _i, _i, _i, ctypes.POINTER(OracleKernel), _i,
+                                                              _pd, _pd, _pd, _pd, ctypes.POINTER(OracleStats)]
+        L.oracle_voronoi_map.restype = _i
+        L.oracle_voronoi_map.argtypes = [_ll] + [_pd] * 8 + [_pi, _pi, _ll, _pll, _pll, _pd, _pi,
+                                                              ctypes.POINTER(OracleKernel), _pd, _pd, _pd, _pd, _pll]
+        _lib = L
+    return _lib
+
+
+def have_ref():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    """The reference's own kernel math (built from /root/reference by oracle/ref.mk), or None."""
+    global _ref
+    if _ref is None:
+        if not have_ref():
+            return None
+        R = ctypes.CDLL(REF_SO)
+        R.ref_m4_vertex.restype = _d
+        R.ref_m4_vertex.argtypes = [_d, _d, _d, _d]
+        R.ref_m4_face.restype = _d
+        R.ref_m4_face.argtypes = [_d, _d, _d, _d, _pd, _i, _pi]
+        R.ref_generic_vertex.restype = _d
+        R.ref_generic_vertex.argtypes = [_i, _d, _d, _d, _d]
+        R.ref_generic_face.restype = _d
+        R.ref_generic_face.argtypes = [_i, _d, _d, _d, _d, _pd, _i, _pi]
+        R.ref_kernel_spec.restype = _i
+        R.ref_kernel_spec.argtypes = [_i, _pi, _pd, _pd, _pd, _pd]
+        R.ref_m4_gridded.restype = _d
+        R.ref_m4_gridded.argtypes = [_d, _d, _d, _d]
+        _ref = R
+    return _ref
+
+
+def _p(a):
+    return a.ctypes.data_as(_pd)
+
+
+def kernel(name="M4", support_in_h=0.0, generic=False, tabulated=False):
+    return OracleKernel(KERNELS.index(name), support_in_h, int(generic), int(tabulated))
+
+
+def face(k, p, h, verts):
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1)
+    ov = ctypes.c_int(0)
+    m = lib().oracle_face(ctypes.byref(k), p[0], p[1], p[2], h, _p(v), len(v) // 3, ctypes.byref(ov))
+    return m, ov.value
+
+
+def sph_to_grid(parts, domain, nx, ny, nz, k=None, threads=1):
+    """parts: dict of float64 arrays x y z vx vy vz m h.  Returns (mass, px, py, pz, stats) un-normalised."""
+    k = k or kernel()
+    n = len(parts["x"])
+    cols = [np.ascontiguousarray(parts[c], dtype=np.float64) for c in ("x", "y", "z", "vx", "vy", "vz", "m", "h")]
+    dom = np.ascontiguousarray(domain, dtype=np.float64)
+    nc = nx * ny * nz
+    out = [np.zeros(nc) for _ in range(4)]
+    st = OracleStats()
+    rc = lib().oracle_sph_to_grid(n, *[_p(c) for c in cols], _p(dom), nx, ny, nz, ctypes.byref(k), threads,
+                                  *[_p(o) for o in out], ctypes.byref(st))
+    if rc != 0:
+        raise ValueError(f"oracle_sph_to_grid failed ({rc})")
+    return out[0], out[1], out[2], out[3], st
+
+
+def voronoi_map(parts, pid, home, cell_face_ptr, face_vert_ptr, face_xyz, face_nbr, k=None):
+    k = k or kernel()
+    n = len(parts["x"])
+    cols = [np.ascontiguousarray(parts[c], dtype=np.float64) for c in ("x", "y", "z", "vx", "vy", "vz", "m", "h")]
+    pid = np.ascontiguousarray(pid, dtype=np.int32)
+    home = np.ascontiguousarray(home, dtype=np.int32)
+    cfp = np.ascontiguousarray(cell_face_ptr, dtype=np.int64)
+    fvp = np.ascontiguousarray(face_vert_ptr, dtype=np.int64)
+    fxyz = np.ascontiguousarray(face_xyz, dtype=np.float64)
+    fnb = np.ascontiguousarray(face_nbr, dtype=np.int32)
+    ncells = len(cfp) - 1
+    out = [np.zeros(ncells) for _ in range(4)]
+    counts = np.zeros(6, dtype=np.int64)
+    rc = lib().oracle_voronoi_map(n, *[_p(c) for c in cols], pid.ctypes.data_as(_pi), home.ctypes.data_as(_pi),
+                                  ncells, cfp.ctypes.data_as(_pll), fvp.ctypes.data_as(_pll), _p(fxyz),
+                                  fnb.ctypes.data_as(_pi), ctypes.byref(k), *[_p(o) for o in out],
+                                  counts.ctypes.data_as(_pll))
+    if rc != 0:
+        raise ValueError(f"oracle_voronoi_map failed ({rc})")
+    return out[0], out[1], out[2], out[3], counts

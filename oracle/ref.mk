@@ -1,0 +1,46 @@
+# Builds the reference KIWI (read-only sources under $(REF)) into oracle/_ref/ so that the oracle
+# can be checked against the real thing.  Only plain g++ on the reference's own source files; the
+# reference's CMake build is not used.  Outputs land in oracle/_ref/ only (git-ignored).
+#
+#   make -f oracle/ref.mk            -> oracle/_ref/kiwi (the reference CLI)
+#                                       oracle/_ref/libkiwi_ref.so (reference kernel math + our C shim)
+#
+# Nothing here runs on the GPU box: the reference never leaves this container.
+
+REF      ?= /root/reference
+OUT      ?= oracle/_ref
+CXX      ?= g++
+CXXFLAGS ?= -O2 -std=c++17 -fopenmp -fPIC -w
+INC      := -I$(REF)/include -I$(REF)/third_party -I$(REF)/third_party/voro/src
+DEFS     := -DKIWI_VERSION=\"0.1.0\" -DKIWI_SOURCE_DIR=\"$(REF)\"
+
+VORO_SRC := cell.cc common.cc container.cc unitcell.cc v_compute.cc c_loops.cc v_base.cc wall.cc \
+            pre_container.cc container_prd.cc
+KIWI_SRC := $(wildcard $(REF)/src/*.cpp)
+MATH_SRC := $(REF)/src/kernel_integral.cpp $(REF)/src/kernel_integral_generic.cpp \
+            $(REF)/src/kernel_spec_data.cpp
+
+VORO_OBJ := $(patsubst %.cc,$(OUT)/obj/voro_%.o,$(VORO_SRC))
+KIWI_OBJ := $(patsubst $(REF)/src/%.cpp,$(OUT)/obj/kiwi_%.o,$(KIWI_SRC))
+MATH_OBJ := $(patsubst $(REF)/src/%.cpp,$(OUT)/obj/kiwi_%.o,$(MATH_SRC))
+
+all: $(OUT)/kiwi $(OUT)/libkiwi_ref.so
+
+$(OUT)/obj/voro_%.o: $(REF)/third_party/voro/src/%.cc
+	@mkdir -p $(OUT)/obj
+	$(CXX) $(CXXFLAGS) $(INC) -c $< -o $@
+
+$(OUT)/obj/kiwi_%.o: $(REF)/src/%.cpp
+	@mkdir -p $(OUT)/obj
+	$(CXX) $(CXXFLAGS) $(INC) $(DEFS) -c $< -o $@
+
+$(OUT)/kiwi: $(KIWI_OBJ) $(VORO_OBJ)
+	$(CXX) $(CXXFLAGS) $^ -o $@
+
+$(OUT)/libkiwi_ref.so: oracle/ref_shim.cpp $(MATH_OBJ)
+	$(CXX) $(CXXFLAGS) $(INC) -shared $^ -o $@
+
+clean:
+	rm -rf $(OUT)
+
+.PHONY: all clean
