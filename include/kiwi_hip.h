@@ -1,0 +1,128 @@
+/* libkiwi_hip: MI355X (gfx950) implementation of KIWI's exact kernel-integral mapping.
+ *
+ * Plain C ABI: no exceptions cross it, every struct is plain old data, pointers and sizes only.
+ * It replaces the per-(particle, cell) loops of the reference's two mapping stages:
+ *
+ *   kiwi_hip_sph_to_grid   <- the particle loop of run_sph_to_grid_stage
+ *                             (src/cartesian_grid.cpp:184-301; include/kiwi/cartesian_grid.hpp:88),
+ *                             i.e. the 6 x KernelIntegrator::face calls per cell
+ *                             (include/kiwi/kernel_integrator.hpp:22-27).
+ *
+ * The host keeps what the reference keeps outside those loops: configuration, snapshot reading,
+ * normalisation (src/cartesian_grid.cpp:294-318, src/mapping.cpp:473-502), file output and caching.
+ *
+ * Outputs are the UN-normalised per-cell sums the reference accumulates:
+ *   mass[c] = sum_p Msum(p,c) m_p,   p*[c] = sum_p Msum(p,c) m_p v*_p
+ * Cartesian cells are x-major: c = (ix*ny + iy)*nz + iz (include/kiwi/cartesian_grid.hpp:62-66).
+ *
+ * Return codes: KIWI_OK (0) or a negative KIWI_ERR_*; kiwi_hip_last_error() then holds the message.
+ * A context owns one HIP stream and its device scratch; it must not be used from two threads at once.
+ * Calls are synchronous: outputs are complete when the call returns.
+ */
+#ifndef KIWI_HIP_H
+#define KIWI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KIWI_HIP_ABI_VERSION 1
+
+#define KIWI_OK 0
+#define KIWI_ERR_INVALID_ARG (-1)
+#define KIWI_ERR_DEVICE (-2)
+#define KIWI_ERR_OUT_OF_MEMORY (-3)
+#define KIWI_ERR_UNSUPPORTED (-4)
+#define KIWI_ERR_LIMIT (-5)
+
+/* arithmetic of the per-pair kernel integrals */
+#define KIWI_FP64 0  /* FP64 throughout (parity mode) */
+#define KIWI_MIXED 1 /* FP32 bulk, FP64 accumulation */
+
+/* kernels, in the reference's order (src/kernel_spec_data.cpp:12-163) */
+#define KIWI_KERNEL_M4 0
+#define KIWI_KERNEL_M5 1
+#define KIWI_KERNEL_M6 2
+#define KIWI_KERNEL_WC2 3
+#define KIWI_KERNEL_WC4 4
+#define KIWI_KERNEL_WC6 5
+
+typedef struct kiwi_hip_ctx kiwi_hip_ctx;
+
+/* SPHSnapshot's columns (include/kiwi/snapshot.hpp:70-77), structure of arrays, n particles. */
+typedef struct kiwi_particles {
+    int64_t n;
+    const double *x, *y, *z;
+    const double *vx, *vy, *vz;
+    const double *m, *h;
+    int on_device; /* 1: device pointers (already in HBM), 0: host pointers */
+} kiwi_particles;
+
+/* Config::domain plus the resolved per-axis cell counts (src/cartesian_grid.cpp:89-97). */
+typedef struct kiwi_grid {
+    double x_min, x_max, y_min, y_max, z_min, z_max;
+    int nx, ny, nz;
+} kiwi_grid;
+
+/* KernelIntegrator settings (include/kiwi/kernel_integrator.hpp:118-122, src/config.cpp:267-277). */
+typedef struct kiwi_kernel {
+    int kernel_id;       /* KIWI_KERNEL_* */
+    double support_in_h; /* kernel.support_radius_in_h; 0 = the kernel's default */
+    int generic_path;    /* kernel.generic_path */
+    int tabulated;       /* *.use_tabulated_integral (cubic default-support path only) */
+} kiwi_kernel;
+
+/* n_cells doubles each; on_device selects device or host pointers. */
+typedef struct kiwi_cell_sums {
+    double *mass, *px, *py, *pz;
+    int on_device;
+} kiwi_cell_sums;
+
+#define KIWI_MAX_WARN_RECORDS 16
+
+/* One Msum <= -0.01 event (the reference's warning line, src/cartesian_grid.cpp:264-269). */
+typedef struct kiwi_warn_record {
+    int64_t particle; /* 0-based particle index */
+    int64_t cell;     /* flattened cell index */
+    double value;
+} kiwi_warn_record;
+
+typedef struct kiwi_stats {
+    int64_t skipped_outside;  /* particles whose support box misses the grid */
+    int64_t n_negative;       /* pairs with Msum < 0, clamped to 0 */
+    int64_t n_negative_warn;  /* of which Msum <= -0.01 */
+    int64_t pairs_bbox;       /* (particle, cell) pairs in the clamped bounding boxes */
+    int64_t pairs_sphere;     /* pairs whose cell intersects the support sphere (the throughput unit) */
+    int64_t particle_blocks;  /* (particle, cell-block) work items */
+    double v_mean[3];         /* mean velocity over all particles */
+    double device_ms_prep;    /* binning + sort */
+    double device_ms_gather;  /* the cell-integral kernel */
+    int64_t lattice_points[3]; /* lattice values evaluated, by radial region (inside h, 2h, outside) */
+    int64_t line_evals[3];     /* edge-line constants with 0, 1 or 2 crossing evaluations */
+    int64_t plane_evals;       /* wall-plane constants evaluated */
+    int n_warn_records;
+    kiwi_warn_record warn[KIWI_MAX_WARN_RECORDS];
+} kiwi_stats;
+
+int kiwi_hip_abi_version(void);
+
+/* One context per host thread; binds device_ordinal and creates a stream. */
+int kiwi_hip_create(kiwi_hip_ctx** out, int device_ordinal);
+void kiwi_hip_destroy(kiwi_hip_ctx* ctx);
+const char* kiwi_hip_last_error(const kiwi_hip_ctx* ctx);
+/* The context's HIP stream (hipStream_t), for callers that time or order work around the calls. */
+void* kiwi_hip_stream(kiwi_hip_ctx* ctx);
+
+/* Direct SPH -> Cartesian grid, replacing src/cartesian_grid.cpp:184-301.
+ * precision: KIWI_FP64 or KIWI_MIXED.  out: n_cells = nx*ny*nz per array.  stats may be NULL. */
+int kiwi_hip_sph_to_grid(kiwi_hip_ctx* ctx, const kiwi_particles* particles, const kiwi_grid* grid,
+                         const kiwi_kernel* kernel, int precision, const kiwi_cell_sums* out,
+                         kiwi_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KIWI_HIP_H */

@@ -1,0 +1,54 @@
+"""Builds libkiwi_hip.so (gfx950) and the host CLI in-tree with hipcc / g++.
+
+    python -m kiwi_amd.build          # library + CLI
+"""
+import os
+import subprocess
+import sys
+
+from . import CLI_PATH, LIB_PATH, PKG_DIR, ROOT_DIR
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# Packed FP32 instructions are switched off for all device code.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+             "-packed-fp32-ops", "-Wall", "-Wno-unused-result"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, cwd=ROOT_DIR, capture_output=True, text=True)
+    noise = "is not a recognized feature for this target"
+    err = "\n".join(l for l in r.stderr.splitlines() if noise not in l and l.strip())
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{err}")
+    if err:
+        sys.stderr.write(err + "\n")
+
+
+def build_lib():
+    src = os.path.join(PKG_DIR, "csrc", "kiwi_hip.hip")
+    tmp = LIB_PATH + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT_DIR, "include"), src, "-o", tmp])
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+def build_cli():
+    host = os.path.join(PKG_DIR, "host")
+    srcs = sorted(os.path.join(host, f) for f in os.listdir(host) if f.endswith(".cpp"))
+    if not srcs:
+        return None
+    tmp = CLI_PATH + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT_DIR, "include"), *srcs, "-o", tmp,
+          f"-L{PKG_DIR}", "-lkiwi_hip", f"-Wl,-rpath,{PKG_DIR}", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, CLI_PATH)
+    return CLI_PATH
+
+
+def build_all():
+    build_lib()
+    build_cli()
+
+
+if __name__ == "__main__":
+    build_all()

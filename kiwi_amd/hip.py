@@ -1,0 +1,202 @@
+"""ctypes view of libkiwi_hip (include/kiwi_hip.h).
+
+This mirrors the reference's stage interface for the mapping loop: `sph_to_grid` takes the particle
+columns of SPHSnapshot (include/kiwi/snapshot.hpp:70-77), the domain and resolved resolution
+(src/cartesian_grid.cpp:89-97) and the kernel settings (include/kiwi/kernel_integrator.hpp:118-122),
+and returns the un-normalised per-cell sums; `normalise` then applies src/cartesian_grid.cpp:294-318.
+Errors from the library raise KiwiError carrying the library's message (the reference throws
+std::runtime_error / ConfigError at the same points).
+
+There is no fallback: if libkiwi_hip.so is missing or no GPU is present, loading raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import LIB_PATH
+
+FP64, MIXED = 0, 1
+KERNELS = ["M4", "M5", "M6", "WC2", "WC4", "WC6"]
+ALIASES = {"cubic": "M4", "cubic_spline": "M4", "m4": "M4", "quartic": "M5", "quartic_spline": "M5", "m5": "M5",
+           "quintic": "M6", "quintic_spline": "M6", "m6": "M6", "wendland_c2": "WC2", "wendlandc2": "WC2",
+           "wc2": "WC2", "wendland_c4": "WC4", "wendlandc4": "WC4", "wc4": "WC4", "wendland_c6": "WC6",
+           "wendlandc6": "WC6", "wc6": "WC6"}
+
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+_i = ctypes.c_int
+_pd = ctypes.POINTER(ctypes.c_double)
+
+EXPORTED = ["kiwi_hip_abi_version", "kiwi_hip_create", "kiwi_hip_destroy", "kiwi_hip_last_error",
+            "kiwi_hip_stream", "kiwi_hip_sph_to_grid"]
+
+
+class KiwiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[kiwi] error: {msg} (code {code})")
+        self.code = code
+
+
+class Particles(ctypes.Structure):
+    _fields_ = [("n", _i64), ("x", _pd), ("y", _pd), ("z", _pd), ("vx", _pd), ("vy", _pd), ("vz", _pd),
+                ("m", _pd), ("h", _pd), ("on_device", _i)]
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("x_min", _d), ("x_max", _d), ("y_min", _d), ("y_max", _d), ("z_min", _d), ("z_max", _d),
+                ("nx", _i), ("ny", _i), ("nz", _i)]
+
+
+class Kernel(ctypes.Structure):
+    _fields_ = [("kernel_id", _i), ("support_in_h", _d), ("generic_path", _i), ("tabulated", _i)]
+
+
+class CellSums(ctypes.Structure):
+    _fields_ = [("mass", _pd), ("px", _pd), ("py", _pd), ("pz", _pd), ("on_device", _i)]
+
+
+class WarnRecord(ctypes.Structure):
+    _fields_ = [("particle", _i64), ("cell", _i64), ("value", _d)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("skipped_outside", _i64), ("n_negative", _i64), ("n_negative_warn", _i64), ("pairs_bbox", _i64),
+                ("pairs_sphere", _i64), ("particle_blocks", _i64), ("v_mean", _d * 3), ("device_ms_prep", _d),
+                ("device_ms_gather", _d), ("lattice_points", _i64 * 3), ("line_evals", _i64 * 3),
+                ("plane_evals", _i64), ("n_warn_records", _i), ("warn", WarnRecord * 16)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_
+             if f not in ("v_mean", "warn", "lattice_points", "line_evals")}
+        d["v_mean"] = list(self.v_mean)
+        d["lattice_points"] = list(self.lattice_points)
+        d["line_evals"] = list(self.line_evals)
+        d["warn"] = [(w.particle, w.cell, w.value) for w in self.warn[: self.n_warn_records]]
+        return d
+
+
+_lib = None
+
+
+def lib_path():
+    return LIB_PATH
+
+
+def load():
+    """Load libkiwi_hip.so.  Raises if it has not been built (python -m kiwi_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libkiwi_hip.so not built at {LIB_PATH}: run python -m kiwi_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        L.kiwi_hip_abi_version.restype = _i
+        L.kiwi_hip_create.restype = _i
+        L.kiwi_hip_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), _i]
+        L.kiwi_hip_destroy.restype = None
+        L.kiwi_hip_destroy.argtypes = [ctypes.c_void_p]
+        L.kiwi_hip_last_error.restype = ctypes.c_char_p
+        L.kiwi_hip_last_error.argtypes = [ctypes.c_void_p]
+        L.kiwi_hip_stream.restype = ctypes.c_void_p
+        L.kiwi_hip_stream.argtypes = [ctypes.c_void_p]
+        L.kiwi_hip_sph_to_grid.restype = _i
+        L.kiwi_hip_sph_to_grid.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.POINTER(Grid),
+                                           ctypes.POINTER(Kernel), _i, ctypes.POINTER(CellSums),
+                                           ctypes.POINTER(Stats)]
+        _lib = L
+    return _lib
+
+
+def kernel_spec(name="M4", support_in_h=0.0, generic_path=False, tabulated=False):
+    key = ALIASES.get(str(name).lower().replace("-", "_").replace(" ", "_"), str(name).upper())
+    if key not in KERNELS:
+        raise ValueError(f"unknown kernel '{name}'")
+    return Kernel(KERNELS.index(key), float(support_in_h), int(bool(generic_path)), int(bool(tabulated)))
+
+
+def _ptr(a):
+    """Host numpy float64 array or device tensor -> (double*, on_device)."""
+    if isinstance(a, np.ndarray):
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data_as(_pd), 0
+    # torch tensor (device memory plumbing only)
+    assert str(a.dtype) == "torch.float64" and a.is_contiguous()
+    return ctypes.cast(ctypes.c_void_p(a.data_ptr()), _pd), int(a.is_cuda)
+
+
+class Context:
+    """One libkiwi_hip context (device + stream).  Not thread-safe, like the C context."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        h = ctypes.c_void_p()
+        rc = self.L.kiwi_hip_create(ctypes.byref(h), int(device))
+        if rc != 0:
+            raise KiwiError(rc, f"kiwi_hip_create(device={device}) failed")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.kiwi_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def stream(self):
+        return self.L.kiwi_hip_stream(self.h)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise KiwiError(rc, self.L.kiwi_hip_last_error(self.h).decode())
+
+    def sph_to_grid(self, parts, domain, res, kernel=None, precision=FP64, out=None):
+        """parts: dict of x y z vx vy vz m h (numpy float64 or device float64 tensors, all the same kind).
+        domain: (x_min, x_max, y_min, y_max, z_min, z_max); res: (nx, ny, nz).
+        out: None (host numpy results) or a tuple of 4 device float64 tensors of nx*ny*nz.
+        Returns (mass, px, py, pz, Stats)."""
+        kernel = kernel or kernel_spec()
+        cols = [parts[c] for c in ("x", "y", "z", "vx", "vy", "vz", "m", "h")]
+        ptrs = [_ptr(c) for c in cols]
+        on_dev = {p[1] for p in ptrs}
+        if len(on_dev) != 1:
+            raise ValueError("particle columns must all be host arrays or all device tensors")
+        n = len(cols[0])
+        P = Particles(n, *[p[0] for p in ptrs], on_dev.pop())
+        nx, ny, nz = (int(v) for v in res)
+        G = Grid(*[float(v) for v in domain], nx, ny, nz)
+        nc = nx * ny * nz
+        if out is None:
+            arrs = [np.zeros(nc) for _ in range(4)]
+            O = CellSums(*[a.ctypes.data_as(_pd) for a in arrs], 0)
+        else:
+            arrs = list(out)
+            O = CellSums(*[_ptr(a)[0] for a in arrs], 1)
+        st = Stats()
+        self._check(self.L.kiwi_hip_sph_to_grid(self.h, ctypes.byref(P), ctypes.byref(G), ctypes.byref(kernel),
+                                                int(precision), ctypes.byref(O), ctypes.byref(st)))
+        return arrs[0], arrs[1], arrs[2], arrs[3], st
+
+
+def normalise(mass, px, py, pz, domain, res, v_mean):
+    """density = mass / V and v = p / mass, or the mean velocity where no kernel reached the cell
+    (src/cartesian_grid.cpp:303-318)."""
+    nx, ny, nz = res
+    V = ((domain[1] - domain[0]) / nx) * ((domain[3] - domain[2]) / ny) * ((domain[5] - domain[4]) / nz)
+    dens = mass / V
+    pos = mass > 0
+    safe = np.where(pos, mass, 1.0)
+    vx = np.where(pos, px / safe, v_mean[0])
+    vy = np.where(pos, py / safe, v_mean[1])
+    vz = np.where(pos, pz / safe, v_mean[2])
+    return dens, vx, vy, vz

@@ -29,6 +29,8 @@
 #include <omp.h>
 #endif
 
+#include <quadmath.h>
+
 #include "../kiwi_amd/csrc/kernel_tables.h"
 
 namespace {
@@ -39,21 +41,29 @@ const KiwiKernelTable kKiwiKernelTables[KIWI_TAB_NKERNELS] = {KIWI_KERNEL_TABLE_
 // Cubic spline (M4), analytic: the three tiers of src/kernel_integral.cpp:222-400.
 // ------------------------------------------------------------------------------------------------
 
-struct M4Wall {  // include/kiwi/kernel_integral.hpp:209-214
-    bool ok = false;
-    double h = 0, r0 = 0, h2 = 0, r03 = 0, q2 = 0, q3 = 0, qi2 = 0, qi3 = 0;
-    double B1 = 0, B2 = 0, B3 = 0;
-};
+// The cubic path is written once for a scalar type T: T = double is the oracle proper (the
+// reference's arithmetic, bit for bit); T = __float128 evaluates the same algorithm in binary128 so
+// that cells where the double result carries visible rounding error can be arbitrated.  Decimal
+// constants are written as ratios C(n, d) = T(n) / T(d): in double that is exactly the literal the
+// reference uses (the nearest double to n/d), in binary128 it is the exact rational to 113 bits.
+template <typename T>
+inline T C(int n, int d) { return T(n) / T(d); }
 
-struct M4Edge {  // include/kiwi/kernel_integral.hpp:216-221
-    bool ok = false;
-    double R0 = 0, a = 0, a2 = 0, D2 = 0, D3 = 0;
-};
-
-// The I-series at angle phi (src/kernel_integral.cpp:278-294, 367-386; identical in all three places).
-struct M4Series {
-    double I0, Im2, Im4, I1, Im1, Im3, Im5;
-};
+inline double q_sqrt(double x) { return std::sqrt(x); }
+inline double q_acos(double x) { return std::acos(x); }
+inline double q_log(double x) { return std::log(x); }
+inline double q_atan(double x) { return std::atan(x); }
+inline double q_sin(double x) { return std::sin(x); }
+inline double q_fabs(double x) { return std::fabs(x); }
+inline __float128 q_sqrt(__float128 x) { return sqrtq(x); }
+inline __float128 q_acos(__float128 x) { return acosq(x); }
+inline __float128 q_log(__float128 x) { return logq(x); }
+inline __float128 q_atan(__float128 x) { return atanq(x); }
+inline __float128 q_sin(__float128 x) { return sinq(x); }
+inline __float128 q_fabs(__float128 x) { return fabsq(x); }
+template <typename T> inline T q_pi();
+template <> inline double q_pi<double>() { return M_PI; }
+template <> inline __float128 q_pi<__float128>() { return M_PIq; }
 
 inline void sincos_pair(double x, double& s, double& c) {
 #if defined(__GLIBC__)
@@ -63,32 +73,54 @@ inline void sincos_pair(double x, double& s, double& c) {
     c = std::cos(x);
 #endif
 }
+inline void sincos_pair(__float128 x, __float128& s, __float128& c) { sincosq(x, &s, &c); }
 
-M4Series m4_series(double phi, double a, double a2) {
-    double sn, cs;
+template <typename T>
+struct M4Wall {  // include/kiwi/kernel_integral.hpp:209-214
+    bool ok = false;
+    T h = 0, r0 = 0, h2 = 0, r03 = 0, q2 = 0, q3 = 0, qi2 = 0, qi3 = 0;
+    T B1 = 0, B2 = 0, B3 = 0;
+};
+
+template <typename T>
+struct M4Edge {  // include/kiwi/kernel_integral.hpp:216-221
+    bool ok = false;
+    T R0 = 0, a = 0, a2 = 0, D2 = 0, D3 = 0;
+};
+
+// The I-series at angle phi (src/kernel_integral.cpp:278-294, 367-386; identical in all three places).
+template <typename T>
+struct M4Series {
+    T I0, Im2, Im4, I1, Im1, Im3, Im5;
+};
+
+template <typename T>
+M4Series<T> m4_series(T phi, T a, T a2) {
+    T sn, cs;
     sincos_pair(phi, sn, cs);
-    const double c2 = cs * cs;
-    const double mu = cs / a / std::sqrt(1. + c2 / a2);
-    const double tn = sn / cs;
-    M4Series s;
+    const T c2 = cs * cs;
+    const T mu = cs / a / q_sqrt(T(1) + c2 / a2);
+    const T tn = sn / cs;
+    M4Series<T> s;
     s.I0 = phi;
     s.Im2 = phi + a2 * tn;
-    s.Im4 = phi + 2 * a2 * tn + 1. / 3. * a2 * a2 * tn * (2. + 1. / c2);
-    const double u = sn * std::sqrt(1 - mu * mu);
-    const double lg = std::log(1 + u) - std::log(1 - u);
-    s.I1 = std::atan(u / a);
-    s.Im1 = a / 2. * lg + s.I1;
-    s.Im3 = s.Im1 + a * (1. + a2) / 4. * (2 * u / (1 - u * u) + lg);
-    s.Im5 = s.Im3 + a * (1. + a2) * (1. + a2) / 16. * ((10 * u - 6 * u * u * u) / (1 - u * u) / (1 - u * u) + 3. * lg);
+    s.Im4 = phi + 2 * a2 * tn + C<T>(1, 3) * a2 * a2 * tn * (T(2) + T(1) / c2);
+    const T u = sn * q_sqrt(1 - mu * mu);
+    const T lg = q_log(1 + u) - q_log(1 - u);
+    s.I1 = q_atan(u / a);
+    s.Im1 = a / T(2) * lg + s.I1;
+    s.Im3 = s.Im1 + a * (T(1) + a2) / T(4) * (2 * u / (1 - u * u) + lg);
+    s.Im5 = s.Im3 + a * (T(1) + a2) * (T(1) + a2) / T(16) * ((10 * u - 6 * u * u * u) / (1 - u * u) / (1 - u * u) + T(3) * lg);
     return s;
 }
 
 // compute_wall_constants, src/kernel_integral.cpp:222-247
-M4Wall m4_wall(double h, double r0) {
-    M4Wall w;
+template <typename T>
+M4Wall<T> m4_wall(T h, T r0) {
+    M4Wall<T> w;
     w.h = h;
     w.r0 = r0;
-    if (r0 == 0.0) return w;
+    if (r0 == T(0)) return w;
     w.ok = true;
     w.h2 = h * h;
     w.r03 = r0 * r0 * r0;
@@ -96,49 +128,50 @@ M4Wall m4_wall(double h, double r0) {
     w.q3 = w.q2 * r0 / h;
     w.qi2 = h / r0 * h / r0;
     w.qi3 = w.qi2 * h / r0;
-    if (r0 >= 2.0 * h) {
-        w.B3 = w.h2 * h / 4.;
+    if (r0 >= T(2) * h) {
+        w.B3 = w.h2 * h / T(4);
     } else if (r0 > h) {
-        const double common = -4. / 3. + (r0 / h) - 0.3 * w.q2 + 1. / 30. * w.q3 - 1. / 15. * w.qi3;
-        w.B3 = w.r03 / 4. * (common + 8. / 5. * w.qi2);
-        w.B2 = w.r03 / 4. * common;
+        const T common = C<T>(-4, 3) + (r0 / h) - C<T>(3, 10) * w.q2 + C<T>(1, 30) * w.q3 - C<T>(1, 15) * w.qi3;
+        w.B3 = w.r03 / T(4) * (common + C<T>(8, 5) * w.qi2);
+        w.B2 = w.r03 / T(4) * common;
     } else {
-        const double common = -2. / 3. + 0.3 * w.q2 - 0.1 * w.q3;
-        w.B3 = w.r03 / 4. * (common + 7. / 5. * w.qi2);
-        w.B2 = w.r03 / 4. * (common - 1. / 5. * w.qi2);
-        w.B1 = w.r03 / 4. * common;
+        const T common = C<T>(-2, 3) + C<T>(3, 10) * w.q2 - C<T>(1, 10) * w.q3;
+        w.B3 = w.r03 / T(4) * (common + C<T>(7, 5) * w.qi2);
+        w.B2 = w.r03 / T(4) * (common - C<T>(1, 5) * w.qi2);
+        w.B1 = w.r03 / T(4) * common;
     }
     return w;
 }
 
-// The three per-region vertex forms inside the r0h3/pi bracket (src/kernel_integral.cpp:389-397),
-// without the D offsets; the edge tier takes differences of them at the crossing angles.
-inline double m4_inner(const M4Wall& w, const M4Series& s) {
-    return 1. / 6. * s.Im2 - 3. / 40. * w.q2 * s.Im4 + 1. / 40. * w.q3 * s.Im5 + w.B1 / w.r03 * s.I0;
+// The region-1 vertex form inside the r0h3/pi bracket (src/kernel_integral.cpp:389-390).
+template <typename T>
+inline T m4_inner(const M4Wall<T>& w, const M4Series<T>& s) {
+    return C<T>(1, 6) * s.Im2 - C<T>(3, 40) * w.q2 * s.Im4 + C<T>(1, 40) * w.q3 * s.Im5 + w.B1 / w.r03 * s.I0;
 }
 
 // compute_edge_constants, src/kernel_integral.cpp:253-350
-M4Edge m4_edge(const M4Wall& w, double R0) {
-    M4Edge e;
+template <typename T>
+M4Edge<T> m4_edge(const M4Wall<T>& w, T R0) {
+    M4Edge<T> e;
     e.R0 = R0;
-    if (!w.ok || R0 == 0.0) return e;
+    if (!w.ok || R0 == T(0)) return e;
     e.ok = true;
-    const double h = w.h, r0 = w.r0;
+    const T h = w.h, r0 = w.r0;
     e.a = R0 / r0;
     e.a2 = e.a * e.a;
-    const double line2 = r0 * r0 + R0 * R0;
-    double D2 = 0.0, D3 = 0.0;
+    const T line2 = r0 * r0 + R0 * R0;
+    T D2 = 0, D3 = 0;
     if (line2 <= w.h2) {
-        const double p1 = std::acos(R0 / std::sqrt(h * h - r0 * r0));
-        const M4Series s = m4_series(p1, e.a, e.a2);
-        D2 = -1. / 6. * s.Im2 + 0.25 * (r0 / h) * s.Im3 - 0.15 * w.q2 * s.Im4 + 1. / 30. * w.q3 * s.Im5 -
-             1. / 60. * w.qi3 * s.I1 + (w.B1 - w.B2) / w.r03 * s.I0;
+        const T p1 = q_acos(R0 / q_sqrt(h * h - r0 * r0));
+        const M4Series<T> s = m4_series(p1, e.a, e.a2);
+        D2 = C<T>(-1, 6) * s.Im2 + C<T>(1, 4) * (r0 / h) * s.Im3 - C<T>(3, 20) * w.q2 * s.Im4 +
+             C<T>(1, 30) * w.q3 * s.Im5 - C<T>(1, 60) * w.qi3 * s.I1 + (w.B1 - w.B2) / w.r03 * s.I0;
     }
-    if (line2 <= 4.0 * w.h2) {
-        const double p2 = std::acos(R0 / std::sqrt(4.0 * h * h - r0 * r0));
-        const M4Series s = m4_series(p2, e.a, e.a2);
-        D3 = 1. / 3. * s.Im2 - 0.25 * (r0 / h) * s.Im3 + 3. / 40. * w.q2 * s.Im4 - 1. / 120. * w.q3 * s.Im5 +
-             4. / 15. * w.qi3 * s.I1 + (w.B2 - w.B3) / w.r03 * s.I0 + D2;
+    if (line2 <= T(4) * w.h2) {
+        const T p2 = q_acos(R0 / q_sqrt(T(4) * h * h - r0 * r0));
+        const M4Series<T> s = m4_series(p2, e.a, e.a2);
+        D3 = C<T>(1, 3) * s.Im2 - C<T>(1, 4) * (r0 / h) * s.Im3 + C<T>(3, 40) * w.q2 * s.Im4 -
+             C<T>(1, 120) * w.q3 * s.Im5 + C<T>(4, 15) * w.qi3 * s.I1 + (w.B2 - w.B3) / w.r03 * s.I0 + D2;
     }
     e.D2 = D2;
     e.D3 = D3;
@@ -146,22 +179,23 @@ M4Edge m4_edge(const M4Wall& w, double R0) {
 }
 
 // vertex_integral, src/kernel_integral.cpp:356-400
-double m4_vertex(const M4Wall& w, const M4Edge& e, double phi) {
-    if (!w.ok || !e.ok) return 0.0;
-    if (phi == 0.0) return 0.0;
-    double sn, cs;
+template <typename T>
+T m4_vertex(const M4Wall<T>& w, const M4Edge<T>& e, T phi) {
+    if (!w.ok || !e.ok) return T(0);
+    if (phi == T(0)) return T(0);
+    T sn, cs;
     sincos_pair(phi, sn, cs);
-    const double R = e.R0 / cs;
-    const double r2 = w.r0 * w.r0 + R * R;
-    const M4Series s = m4_series(phi, e.a, e.a2);
-    const double r0 = w.r0, h = w.h;
-    if (r2 < w.h2) return w.q3 / M_PI * m4_inner(w, s);
-    if (r2 < 4.0 * w.h2)
-        return w.q3 / M_PI *
-               (0.25 * (4. / 3. * s.Im2 - (r0 / h) * s.Im3 + 0.3 * w.q2 * s.Im4 - 1. / 30. * w.q3 * s.Im5 +
-                        1. / 15. * w.qi3 * s.I1) +
+    const T R = e.R0 / cs;
+    const T r2 = w.r0 * w.r0 + R * R;
+    const M4Series<T> s = m4_series(phi, e.a, e.a2);
+    const T r0 = w.r0, h = w.h;
+    if (r2 < w.h2) return w.q3 / q_pi<T>() * m4_inner(w, s);
+    if (r2 < T(4) * w.h2)
+        return w.q3 / q_pi<T>() *
+               (C<T>(1, 4) * (C<T>(4, 3) * s.Im2 - (r0 / h) * s.Im3 + C<T>(3, 10) * w.q2 * s.Im4 -
+                              C<T>(1, 30) * w.q3 * s.Im5 + C<T>(1, 15) * w.qi3 * s.I1) +
                 w.B2 / w.r03 * s.I0 + e.D2);
-    return w.q3 / M_PI * (-0.25 * w.qi3 * s.I1 + w.B3 / w.r03 * s.I0 + e.D3);
+    return w.q3 / q_pi<T>() * (C<T>(-1, 4) * w.qi3 * s.I1 + w.B3 / w.r03 * s.I0 + e.D3);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -245,8 +279,8 @@ double m4_gridded(const M4Table& t, double h, double phi, double r0_in, double R
 // full_integral_new, src/kernel_integral.cpp:410-418
 double m4_full(double h, double phi, double r0, double R0) {
     if (r0 == 0.0 || R0 == 0.0 || phi == 0.0) return 0.0;
-    const M4Wall w = m4_wall(h, r0);
-    const M4Edge e = m4_edge(w, R0);
+    const M4Wall<double> w = m4_wall(h, r0);
+    const M4Edge<double> e = m4_edge(w, R0);
     return m4_vertex(w, e, phi);
 }
 
@@ -415,24 +449,35 @@ double gen_vertex(const GenWall& w, const GenEdge& e, double phi) {
 // the wall / edge / vertex tiers (src/kernel_integral.cpp:433-467).
 // ------------------------------------------------------------------------------------------------
 
+template <typename T>
 struct EvalM4 {
-    double h;
-    const M4Table* table;  // non-null: tabulated path
-    M4Wall w;
-    M4Edge e;
-    double ar0 = 0, R0 = 0;
-    void wall(double r) {
+    using scalar = T;
+    T h;
+    const M4Table* table;  // non-null: tabulated path (double only)
+    M4Wall<T> w;
+    M4Edge<T> e;
+    T ar0 = 0, R0 = 0;
+    void wall(T r) {
         ar0 = r;
         if (!table) w = m4_wall(h, r);
     }
-    void edge(double R) {
+    void edge(T R) {
         R0 = R;
         if (!table) e = m4_edge(w, R);
     }
-    double at(double phi) const { return table ? m4_gridded(*table, h, phi, ar0, R0) : m4_vertex(w, e, phi); }
+    T at(T phi) const;
 };
+template <>
+double EvalM4<double>::at(double phi) const {
+    return table ? m4_gridded(*table, h, phi, ar0, R0) : m4_vertex(w, e, phi);
+}
+template <>
+__float128 EvalM4<__float128>::at(__float128 phi) const {
+    return m4_vertex(w, e, phi);
+}
 
 struct EvalGen {
+    using scalar = double;
     const KiwiKernelTable* K;
     double H;
     GenWall w;
@@ -443,57 +488,60 @@ struct EvalGen {
 };
 
 template <class Ev>
-double face_walk(double x0, double y0, double z0, double support, const double* v, int n, int* out_vert, Ev& ev) {
+typename Ev::scalar face_walk(typename Ev::scalar x0, typename Ev::scalar y0, typename Ev::scalar z0,
+                              typename Ev::scalar support, const double* vd, int n, int* out_vert, Ev& ev) {
+    using T = typename Ev::scalar;
     if (n < 3) {
         if (out_vert) *out_vert = 0;
-        return 0.0;
+        return T(0);
     }
-    const double x1 = v[0], y1 = v[1], z1 = v[2];
-    const double x2s = v[3], y2s = v[4], z2s = v[5];
-    const double x3s = v[6], y3s = v[7], z3s = v[8];
+    auto v = [&](int i) { return T(vd[i]); };
+    const T x1 = v(0), y1 = v(1), z1 = v(2);
+    const T x2s = v(3), y2s = v(4), z2s = v(5);
+    const T x3s = v(6), y3s = v(7), z3s = v(8);
     // plane through the first three vertices (:495-506)
-    const double A = (y2s - y1) * (z3s - z1) - (y3s - y1) * (z2s - z1);
-    const double B = (z2s - z1) * (x3s - x1) - (z3s - z1) * (x2s - x1);
-    const double C = (x2s - x1) * (y3s - y1) - (x3s - x1) * (y2s - y1);
-    const double D = -A * x1 - B * y1 - C * z1;
-    const double nrm = std::sqrt(A * A + B * B + C * C);
-    const double r0 = (A * x0 + B * y0 + C * z0 + D) / nrm;
-    const double ar0 = std::fabs(r0);
-    const double xp = x0 - r0 * A / nrm, yp = y0 - r0 * B / nrm, zp = z0 - r0 * C / nrm;
+    const T A = (y2s - y1) * (z3s - z1) - (y3s - y1) * (z2s - z1);
+    const T B = (z2s - z1) * (x3s - x1) - (z3s - z1) * (x2s - x1);
+    const T C = (x2s - x1) * (y3s - y1) - (x3s - x1) * (y2s - y1);
+    const T D = -A * x1 - B * y1 - C * z1;
+    const T nrm = q_sqrt(A * A + B * B + C * C);
+    const T r0 = (A * x0 + B * y0 + C * z0 + D) / nrm;
+    const T ar0 = q_fabs(r0);
+    const T xp = x0 - r0 * A / nrm, yp = y0 - r0 * B / nrm, zp = z0 - r0 * C / nrm;
     // winding sign reference, triple product about the particle (:525-529)
-    const double ax = x1 - x0, ay = y1 - y0, az = z1 - z0;
-    const double bx = x2s - x0, by = y2s - y0, bz = z2s - z0;
-    const double cx = x3s - x0, cy = y3s - y0, cz = z3s - z0;
-    const double s2 = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
+    const T ax = x1 - x0, ay = y1 - y0, az = z1 - z0;
+    const T bx = x2s - x0, by = y2s - y0, bz = z2s - z0;
+    const T cx = x3s - x0, cy = y3s - y0, cz = z3s - z0;
+    const T s2 = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
     ev.wall(ar0);
 
     bool inside = true, have_sign = false;
-    double sign_ref = 0.0;
-    double min_edge = std::numeric_limits<double>::infinity();
-    double total = 0.0;
+    T sign_ref = 0;
+    T min_edge = T(std::numeric_limits<double>::infinity());
+    T total = 0;
     for (int j = 0; j < n; ++j) {
-        const double x2 = v[3 * j], y2 = v[3 * j + 1], z2 = v[3 * j + 2];
+        const T x2 = v(3 * j), y2 = v(3 * j + 1), z2 = v(3 * j + 2);
         const int jn = (j == n - 1) ? 0 : j + 1;
-        const double x3 = v[3 * jn], y3 = v[3 * jn + 1], z3 = v[3 * jn + 2];
+        const T x3 = v(3 * jn), y3 = v(3 * jn + 1), z3 = v(3 * jn + 2);
         // edge geometry (:573-583)
-        const double r23 = std::sqrt((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2) + (z3 - z2) * (z3 - z2));
-        const double r12 = std::sqrt((xp - x2) * (xp - x2) + (yp - y2) * (yp - y2) + (zp - z2) * (zp - z2));
-        const double r13 = std::sqrt((xp - x3) * (xp - x3) + (yp - y3) * (yp - y3) + (zp - z3) * (zp - z3));
-        const double cosa = ((x3 - x2) * (xp - x2) + (y3 - y2) * (yp - y2) + (z3 - z2) * (zp - z2)) / r12 / r23;
-        const double R0 = (std::fabs(cosa) < 1.0) ? r12 * std::sqrt(1 - cosa * cosa) : 0.0;
+        const T r23 = q_sqrt((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2) + (z3 - z2) * (z3 - z2));
+        const T r12 = q_sqrt((xp - x2) * (xp - x2) + (yp - y2) * (yp - y2) + (zp - z2) * (zp - z2));
+        const T r13 = q_sqrt((xp - x3) * (xp - x3) + (yp - y3) * (yp - y3) + (zp - z3) * (zp - z3));
+        const T cosa = ((x3 - x2) * (xp - x2) + (y3 - y2) * (yp - y2) + (z3 - z2) * (zp - z2)) / r12 / r23;
+        const T R0 = (q_fabs(cosa) < T(1)) ? r12 * q_sqrt(1 - cosa * cosa) : T(0);
         // s1 about the particle (:588-591)
-        const double px = xp - x0, py = yp - y0, pz = zp - z0;
-        const double ux = x2 - x0, uy = y2 - y0, uz = z2 - z0;
-        const double wx = x3 - x0, wy = y3 - y0, wz = z3 - z0;
-        const double s1 = px * (uy * wz - uz * wy) + py * (uz * wx - ux * wz) + pz * (ux * wy - uy * wx);
-        const double phi1 = (R0 < r12) ? std::acos(R0 / r12) : 0.0;
-        const double phi2 = (R0 < r13) ? std::acos(R0 / r13) : 0.0;
+        const T px = xp - x0, py = yp - y0, pz = zp - z0;
+        const T ux = x2 - x0, uy = y2 - y0, uz = z2 - z0;
+        const T wx = x3 - x0, wy = y3 - y0, wz = z3 - z0;
+        const T s1 = px * (uy * wz - uz * wy) + py * (uz * wx - ux * wz) + pz * (ux * wy - uy * wx);
+        const T phi1 = (R0 < r12) ? q_acos(R0 / r12) : T(0);
+        const T phi2 = (R0 < r13) ? q_acos(R0 / r13) : T(0);
         // inside-polygon side test (:600-611)
-        const double ex = x3 - x2, ey = y3 - y2, ez = z3 - z2;
-        const double tx = xp - x2, ty = yp - y2, tz = zp - z2;
-        const double side = (ey * tz - ez * ty) * A + (ez * tx - ex * tz) * B + (ex * ty - ey * tx) * C;
-        const double side_eps = 1e-9 * nrm * r23 * std::max({std::fabs(tx), std::fabs(ty), std::fabs(tz), 1.0});
-        if (std::fabs(side) > side_eps) {
+        const T ex = x3 - x2, ey = y3 - y2, ez = z3 - z2;
+        const T tx = xp - x2, ty = yp - y2, tz = zp - z2;
+        const T side = (ey * tz - ez * ty) * A + (ez * tx - ex * tz) * B + (ex * ty - ey * tx) * C;
+        const T side_eps = T(1e-9) * nrm * r23 * std::max({q_fabs(tx), q_fabs(ty), q_fabs(tz), T(1)});
+        if (q_fabs(side) > side_eps) {
             if (!have_sign) {
                 sign_ref = side;
                 have_sign = true;
@@ -501,9 +549,9 @@ double face_walk(double x0, double y0, double z0, double support, const double* 
                 inside = false;
             }
         }
-        const bool beyond = (r12 * std::sin(phi1) >= r23) || (r13 * std::sin(phi2) >= r23);
+        const bool beyond = (r12 * q_sin(phi1) >= r23) || (r13 * q_sin(phi2) >= r23);
         min_edge = std::min(min_edge, beyond ? std::min(r12, r13) : R0);
-        double M = (s1 * s2 * r0 <= 0) ? -1.0 : 1.0;
+        T M = (s1 * s2 * r0 <= 0) ? T(-1) : T(1);
         ev.edge(R0);
         if (beyond) {
             M = (phi1 >= phi2) ? M * (ev.at(phi1) - ev.at(phi2)) : M * (ev.at(phi2) - ev.at(phi1));
@@ -513,7 +561,7 @@ double face_walk(double x0, double y0, double z0, double support, const double* 
         total += M;
     }
     // reachability of the face (:642-645)
-    const double fd = inside ? ar0 : std::sqrt(r0 * r0 + min_edge * min_edge);
+    const T fd = inside ? ar0 : q_sqrt(r0 * r0 + min_edge * min_edge);
     if (out_vert) *out_vert = (fd >= support) ? n : 0;
     return total;
 }
@@ -540,7 +588,7 @@ struct Integrator {
             EvalGen ev{gen, support_in_h * h, {}, {}};
             return face_walk(x0, y0, z0, support_in_h * h, v, n, out_vert, ev);
         }
-        EvalM4 ev{h, tabulated ? &m4_table() : nullptr, {}, {}};
+        EvalM4<double> ev{h, tabulated ? &m4_table() : nullptr, {}, {}};
         return face_walk(x0, y0, z0, 2 * h, v, n, out_vert, ev);
     }
 };
@@ -703,6 +751,60 @@ int oracle_sph_to_grid(long long n, const double* x, const double* y, const doub
         st->v_mean[0] = n > 0 ? vmx / static_cast<double>(n) : 0.0;
         st->v_mean[1] = n > 0 ? vmy / static_cast<double>(n) : 0.0;
         st->v_mean[2] = n > 0 ? vmz / static_cast<double>(n) : 0.0;
+    }
+    return 0;
+}
+
+// The reference's cubic sph-to-grid sums for a list of cells, with the kernel mathematics of
+// src/kernel_integral.cpp evaluated in binary128 (the geometry, i.e. the bounding boxes and the cell
+// face coordinates, is formed in double exactly as src/cartesian_grid.cpp:220-247 does).  Used to
+// arbitrate cells where two double-precision evaluations of the same integral disagree.
+int oracle_sph_to_grid_cells_q(long long n, const double* x, const double* y, const double* z, const double* vx,
+                               const double* vy, const double* vz, const double* m, const double* h,
+                               const double* domain, int nx, int ny, int nz, long long ncells,
+                               const long long* cells, double* out4) {
+    if (nx <= 0 || ny <= 0 || nz <= 0) return -1;
+    const double xmin = domain[0], ymin = domain[2], zmin = domain[4];
+    const double dpx = (domain[1] - domain[0]) / nx, dpy = (domain[3] - domain[2]) / ny,
+                 dpz = (domain[5] - domain[4]) / nz;
+    const double hpx = 0.5 * dpx, hpy = 0.5 * dpy, hpz = 0.5 * dpz;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (long long q = 0; q < ncells; ++q) {
+        const long long c = cells[q];
+        const int ix = static_cast<int>(c / (static_cast<long long>(ny) * nz));
+        const int iy = static_cast<int>((c / nz) % ny);
+        const int iz = static_cast<int>(c % nz);
+        const double cx = xmin + (ix + 0.5) * dpx, cy = ymin + (iy + 0.5) * dpy, cz = zmin + (iz + 0.5) * dpz;
+        double f[6][12];
+        box_faces(cx, cy, cz, hpx, hpy, hpz, f);
+        __float128 acc[4] = {0, 0, 0, 0};
+        for (long long i = 0; i < n; ++i) {
+            const double hh = h[i];
+            if (hh <= 0.0) continue;
+            const double rad = 2.0 * hh;
+            const int ix0 = std::max(static_cast<int>(std::floor((x[i] - rad - xmin) / dpx)), 0);
+            const int ix1 = std::min(static_cast<int>(std::floor((x[i] + rad - xmin) / dpx)), nx - 1);
+            const int iy0 = std::max(static_cast<int>(std::floor((y[i] - rad - ymin) / dpy)), 0);
+            const int iy1 = std::min(static_cast<int>(std::floor((y[i] + rad - ymin) / dpy)), ny - 1);
+            const int iz0 = std::max(static_cast<int>(std::floor((z[i] - rad - zmin) / dpz)), 0);
+            const int iz1 = std::min(static_cast<int>(std::floor((z[i] + rad - zmin) / dpz)), nz - 1);
+            if (ix < ix0 || ix > ix1 || iy < iy0 || iy > iy1 || iz < iz0 || iz > iz1) continue;
+            __float128 Ms = 0;
+            for (int k = 0; k < 6; ++k) {
+                EvalM4<__float128> ev{hh, nullptr, {}, {}};
+                Ms += face_walk<EvalM4<__float128>>(x[i], y[i], z[i], 2 * static_cast<__float128>(hh), f[k], 4,
+                                                     nullptr, ev);
+            }
+            if (Ms < 0) Ms = 0;
+            const __float128 t = Ms * m[i];
+            acc[0] += t;
+            acc[1] += t * vx[i];
+            acc[2] += t * vy[i];
+            acc[3] += t * vz[i];
+        }
+        for (int k = 0; k < 4; ++k) out4[4 * q + k] = static_cast<double>(acc[k]);
     }
     return 0;
 }

@@ -1,0 +1,82 @@
+"""The C-ABI library: it loads without a GPU, exports every function include/kiwi_hip.h declares, and
+its struct layouts agree with the Python view (no compute calls here)."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+from kiwi_amd import LIB_PATH, hip  # noqa: E402
+
+HEADER = os.path.join(ROOT, "include", "kiwi_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char|void\*)\s*\*?\s*(kiwi_hip_\w+)\s*\(", text, flags=re.M)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB_PATH):
+        pytest.fail("libkiwi_hip.so is not built (python -m kiwi_amd.build)")
+    return ctypes.CDLL(LIB_PATH)
+
+
+def test_exports_every_declared_symbol(lib):
+    names = declared_functions()
+    assert "kiwi_hip_sph_to_grid" in names and len(names) >= 6
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(hip.EXPORTED)
+
+
+def test_abi_version(lib):
+    lib.kiwi_hip_abi_version.restype = ctypes.c_int
+    text = open(HEADER).read()
+    want = int(re.search(r"#define KIWI_HIP_ABI_VERSION (\d+)", text).group(1))
+    assert lib.kiwi_hip_abi_version() == want
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Compile a tiny C program against the header and compare sizeof/offsetof with ctypes."""
+    src = tmp_path / "layout.c"
+    fields = {"kiwi_particles": [f for f, _ in hip.Particles._fields_],
+              "kiwi_grid": [f for f, _ in hip.Grid._fields_],
+              "kiwi_kernel": [f for f, _ in hip.Kernel._fields_],
+              "kiwi_cell_sums": [f for f, _ in hip.CellSums._fields_],
+              "kiwi_stats": [f for f, _ in hip.Stats._fields_]}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
+    for st, fs in fields.items():
+        lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
+        for f in fs:
+            lines.append(f'printf("{st}.{f} %zu\\n", offsetof({st}, {f}));')
+    lines.append("return 0;}")
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", str(src), "-o", str(exe)], check=True)
+    out = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                          check=True).stdout.split("\n") if l)
+    for st, cls in (("kiwi_particles", hip.Particles), ("kiwi_grid", hip.Grid), ("kiwi_kernel", hip.Kernel),
+                    ("kiwi_cell_sums", hip.CellSums), ("kiwi_stats", hip.Stats)):
+        assert int(out[st]) == ctypes.sizeof(cls), st
+        for f, _ in cls._fields_:
+            assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, (st, f)
+
+
+def test_context_creation_fails_cleanly_without_gpu():
+    """On a host without a GPU the library reports an error code instead of crashing."""
+    try:
+        import torch
+        if torch.cuda.device_count() > 0:
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    with pytest.raises(hip.KiwiError):
+        hip.Context(0)

@@ -1,0 +1,235 @@
+"""The oracle pinned: against the reference's own numbers (golden fixtures written by the reference built
+from source; oracle/_ref when it is built here), against closed forms and against brute-force quadrature.
+
+These run on the CPU (no GPU needed)."""
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+sys.path.insert(0, ROOT)
+
+from oracle import oracle as O  # noqa: E402
+
+KEYS = ["M4", "M5", "M6", "WC2", "WC4", "WC6"]
+RADKERN = {"M4": 2.0, "M5": 2.5, "M6": 3.0, "WC2": 2.0, "WC4": 2.0, "WC6": 2.0}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    if not os.path.exists(O.ORACLE_SO):
+        O.build()
+
+
+def box_faces(lo, hi):
+    """The 6 faces of an axis-aligned box, in the winding the face walk needs
+    (src/cartesian_grid.cpp:54-72)."""
+    xm, ym, zm = lo
+    xp, yp, zp = hi
+    return [[xp, ym, zp, xp, yp, zp, xp, yp, zm, xp, ym, zm], [xm, yp, zm, xm, yp, zp, xm, ym, zp, xm, ym, zm],
+            [xp, yp, zm, xp, yp, zp, xm, yp, zp, xm, yp, zm], [xm, ym, zp, xp, ym, zp, xp, ym, zm, xm, ym, zm],
+            [xm, yp, zp, xp, yp, zp, xp, ym, zp, xm, ym, zp], [xp, ym, zm, xp, yp, zm, xm, yp, zm, xm, ym, zm]]
+
+
+def box_mass(k, p, h, lo, hi):
+    return sum(O.face(k, p, h, np.array(f))[0] for f in box_faces(lo, hi))
+
+
+# ------------------------------------------------------------------ golden vectors of the reference
+def test_m4_vertex_golden():
+    g = np.load(os.path.join(GOLD, "m4_vertex.npz"))
+    L = O.lib()
+    got = np.array([L.oracle_m4_vertex(*t) for t in zip(g["h"], g["phi"], g["r0"], g["R0"])])
+    assert np.array_equal(got, g["value"])  # bit-exact (same libm on this host)
+
+
+def test_generic_vertex_golden():
+    g = np.load(os.path.join(GOLD, "generic_vertex.npz"))
+    L = O.lib()
+    got = np.array([L.oracle_generic_vertex(int(k), *t)
+                    for k, t in zip(g["kernel"], zip(g["H"], g["phi"], g["r0"], g["R0"]))])
+    np.testing.assert_allclose(got, g["value"], rtol=1e-13, atol=1e-15)
+
+
+def test_m4_gridded_golden():
+    g = np.load(os.path.join(GOLD, "m4_gridded.npz"))
+    L = O.lib()
+    got = np.array([L.oracle_m4_gridded(*t) for t in zip(g["h"], g["phi"], g["r0"], g["R0"])])
+    np.testing.assert_allclose(got, g["value"], rtol=1e-13, atol=1e-16)
+
+
+def test_face_golden():
+    g = np.load(os.path.join(GOLD, "face.npz"))
+    verts = g["verts"]
+    off = 0
+    for i, nv in enumerate(g["nverts"]):
+        v = verts[off:off + 3 * nv]
+        off += 3 * nv
+        p, h = g["p"][i], g["h"][i]
+        m, ov = O.face(O.kernel("M4"), p, h, v)
+        assert abs(m - g["m4"][i]) <= 1e-13 + 1e-12 * abs(g["m4"][i])
+        assert ov == g["m4_out"][i]
+        key = KEYS[g["kernel"][i]]
+        mg, ovg = O.face(O.kernel(key), p, h, v)
+        assert abs(mg - g["generic"][i]) <= 1e-12 + 1e-11 * abs(g["generic"][i])
+        assert ovg == g["generic_out"][i]
+
+
+def test_c1_grid_matches_reference_cli():
+    """The oracle's sph-to-grid loop against the reference CLI's cartesian_grid.dat, 2,000 particles ->
+    32^3 (fixture written with OMP_NUM_THREADS=1): identical to the last bit."""
+    s = np.load(os.path.join(GOLD, "c1_snapshot.npz"))
+    g = np.load(os.path.join(GOLD, "c1_ref_grid.npz"))
+    parts = {k: s[k] for k in ("x", "y", "z", "vx", "vy", "vz", "m", "h")}
+    dom = s["domain"]
+    res = tuple(int(v) for v in g["shape"])
+    mass, px, py, pz, st = O.sph_to_grid(parts, dom, *res, threads=1)
+    V = ((dom[1] - dom[0]) / res[0]) * ((dom[3] - dom[2]) / res[1]) * ((dom[5] - dom[4]) / res[2])
+    assert np.array_equal(mass / V, g["density"])
+    pos = mass > 0
+    vx = np.where(pos, px / np.where(pos, mass, 1), st.v_mean[0])
+    assert np.array_equal(vx, g["vx"])
+
+
+# ------------------------------------------------------------------ closed forms
+@pytest.mark.parametrize("key", KEYS)
+def test_enclosing_box_is_one_and_partitions_add(key):
+    """A box enclosing the support has mass 1; two halves add up to the whole, wherever the particle is
+    (tools/test_generic_kernel.cpp:261-292 states the same invariants)."""
+    rng = np.random.default_rng(KEYS.index(key))
+    k = O.kernel(key)
+    for _ in range(5):
+        p = rng.uniform(-0.5, 0.5, 3)
+        h = rng.uniform(0.2, 0.8)
+        H = RADKERN[key] * h
+        lo, hi = p - H - rng.uniform(0.01, 1, 3), p + H + rng.uniform(0.01, 1, 3)
+        assert abs(box_mass(k, p, h, lo, hi) - 1.0) <= 1e-10
+        # split along a random axis at a random plane (possibly away from the particle)
+        ax = rng.integers(0, 3)
+        cut = rng.uniform(lo[ax], hi[ax])
+        hi1, lo2 = hi.copy(), lo.copy()
+        hi1[ax] = cut
+        lo2[ax] = cut
+        q = rng.uniform(-1.5, 1.5, 3)  # particle anywhere, also outside the box
+        whole = box_mass(k, q, h, lo, hi)
+        parts = box_mass(k, q, h, lo, hi1) + box_mass(k, q, h, lo2, hi)
+        assert abs(whole - parts) <= 1e-10
+
+
+def test_guards_are_exact_zeros():
+    """r0 = 0, R0 = 0 or phi = 0 give exactly 0 (src/kernel_integral.cpp:357-358, 411-413)."""
+    L = O.lib()
+    assert L.oracle_m4_vertex(1.0, 0.3, 0.0, 0.5) == 0.0
+    assert L.oracle_m4_vertex(1.0, 0.3, 0.5, 0.0) == 0.0
+    assert L.oracle_m4_vertex(1.0, 0.0, 0.5, 0.5) == 0.0
+
+
+def _gl(n):
+    x, w = np.polynomial.legendre.leggauss(n)
+    return x, w
+
+
+def _w_shape(key, q):
+    p = lambda x, n: np.where(x > 0, np.abs(x) ** n, 0.0)  # noqa: E731
+    q = np.asarray(q, dtype=float)
+    if key == "M4":
+        w = p(1 - q, 3) - 4 * p(0.5 - q, 3)
+    elif key == "M5":
+        w = p(1 - q, 4) - 5 * p(0.6 - q, 4) + 10 * p(0.2 - q, 4)
+    elif key == "M6":
+        w = p(1 - q, 5) - 6 * p(2 / 3 - q, 5) + 15 * p(1 / 3 - q, 5)
+    elif key == "WC2":
+        w = p(1 - q, 4) * (1 + 4 * q)
+    elif key == "WC4":
+        w = p(1 - q, 6) * (1 + 6 * q + 35 / 3 * q * q)
+    else:
+        w = p(1 - q, 8) * (1 + 8 * q + 25 * q * q + 32 * q ** 3)
+    return np.where(q < 1, w, 0.0)
+
+
+def _pyramid_quadrature(key, H, r0, R0, phi, n=160):
+    """Brute-force kernel mass of the vertex pyramid: integrate W over the triangle (foot, edge foot,
+    vertex at azimuth phi) swept from the particle, in spherical shells: mass = int dOmega m(r_b),
+    with m(r) = int_0^r W s^2 ds, by Gauss-Legendre in (theta, s)."""
+    A3 = {"M4": 16.0, "M5": 30.517578125, "M6": 54.675, "WC2": 10.5, "WC4": 15.46875, "WC6": 21.328125}[key] / np.pi
+    xg, wg = _gl(n)
+    # azimuth in the wall plane, 0 .. phi
+    th = 0.5 * phi * (xg + 1)
+    wth = 0.5 * phi * wg
+    total = 0.0
+    for t, wt in zip(th, wth):
+        Rmax = R0 / np.cos(t)  # in-plane distance from the foot to the edge line along azimuth t
+        # integrate over in-plane radius rho in [0, Rmax]: dOmega = r0 rho drho / r^3, mass in cone = m(r)
+        rho = 0.5 * Rmax * (xg + 1)
+        wr = 0.5 * Rmax * wg
+        r = np.sqrt(r0 * r0 + rho * rho)
+        # m(r) = A3 int_0^{r/H} w(q) q^2 dq
+        qq = np.minimum(r / H, 1.0)
+        s = 0.5 * qq[:, None] * (xg[None, :] + 1)
+        ws = 0.5 * qq[:, None] * wg[None, :]
+        m = A3 * np.sum(ws * _w_shape(key, s) * s * s, axis=1)
+        total += wt * np.sum(wr * r0 * rho / r ** 3 * m)
+    return total
+
+
+@pytest.mark.parametrize("key", KEYS)
+def test_vertex_integral_vs_quadrature(key):
+    """Generic (and, for M4, cubic) vertex integrals against brute-force quadrature of the kernel taken
+    straight from its definition (the method of tools/test_generic_kernel.cpp:82-112)."""
+    rng = np.random.default_rng(100 + KEYS.index(key))
+    L = O.lib()
+    kid = KEYS.index(key)
+    for _ in range(4):
+        H = 1.0
+        r0 = rng.uniform(0.15, 0.9)
+        R0 = rng.uniform(0.15, 0.9)
+        phi = rng.uniform(0.2, 1.3)
+        ref = _pyramid_quadrature(key, H, r0, R0, phi)
+        got = L.oracle_generic_vertex(kid, H, phi, r0, R0)
+        assert abs(got - ref) <= 2e-7 * max(abs(ref), 1e-3), (key, r0, R0, phi, got, ref)
+        if key == "M4":
+            assert abs(L.oracle_m4_vertex(H / 2, phi, r0, R0) - got) <= 1e-12
+
+
+def test_kernel_tables_regenerate_exactly():
+    """kernel_tables.h is exactly what the derivation script writes."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "kiwi_amd", "csrc", "gen_kernel_tables.py"), "--check"])
+    assert r.returncode == 0
+
+
+# ------------------------------------------------------------------ the reference itself, when built here
+@pytest.mark.skipif(not O.have_ref(), reason="oracle/_ref not built (needs the reference sources)")
+def test_oracle_vs_reference_random():
+    R = O.ref()
+    L = O.lib()
+    rng = np.random.default_rng(7)
+    for _ in range(3000):
+        h = rng.uniform(0.05, 3)
+        r0, R0 = h * rng.uniform(0, 3, 2)
+        phi = rng.uniform(0, np.pi / 2)
+        assert L.oracle_m4_vertex(h, phi, r0, R0) == R.ref_m4_vertex(h, phi, r0, R0)
+        kid = int(rng.integers(0, 6))
+        assert L.oracle_generic_vertex(kid, 2 * h, phi, r0, R0) == R.ref_generic_vertex(kid, 2 * h, phi, r0, R0)
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="oracle/_ref not built (needs the reference sources)")
+def test_derived_tables_equal_reference_tables():
+    """Our exact re-derivation of the kernel tables equals the reference's generated tables."""
+    R = O.ref()
+    sys.path.insert(0, os.path.join(ROOT, "kiwi_amd", "csrc"))
+    import gen_kernel_tables as G
+    for kid, (key, name, (breaks, pieces), radkern, hfact) in enumerate(G.KERNELS):
+        nreg = ctypes.c_int()
+        c4, n4, a, b = np.zeros(4), np.zeros(4), np.zeros(60), np.zeros(256)
+        R.ref_kernel_spec(kid, ctypes.byref(nreg), O._p(c4), O._p(n4), O._p(a), O._p(b))
+        nr, n2, n3, A, B = G.derive(breaks, pieces)
+        assert nr == nreg.value
+        np.testing.assert_array_equal(np.array([float(v) for v in breaks] + [0.0] * (4 - len(breaks))), c4)
+        np.testing.assert_array_equal([float(n2), float(n3), radkern, hfact], n4)
+        np.testing.assert_array_equal(np.array([[float(v) for v in row] for row in A]).reshape(-1), a)
+        np.testing.assert_array_equal(np.array([[[float(v) for v in r] for r in blk] for blk in B]).reshape(-1), b)
