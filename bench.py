@@ -139,6 +139,8 @@ def main():
         "mixed_field_err_over_max": field_err,
         "pairs_sphere": pairs, "pairs_bbox": st32.pairs_bbox, "particle_blocks": st32.particle_blocks,
         "gather_ms": g32, "prep_ms": st32.device_ms_prep,
+        "work": {"lattice_points": list(st32.lattice_points), "line_evals": list(st32.line_evals),
+                 "plane_evals": st32.plane_evals},
         "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_TFLOPS["mixed"], "unit": "TFLOP/s",
                      "frac": ach / PEAK_TFLOPS["mixed"], "traffic": traffic,
                      "flops_per_launch": fl32, "kernel": "k_gather_m4<float>"},

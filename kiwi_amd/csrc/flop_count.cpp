@@ -41,6 +41,19 @@ inline CF fabs(CF a) { return CF(std::fabs(a.v)); }
 
 using namespace kiwi_hip;
 
+// the counting scalar follows the double-precision arithmetic (one division per reciprocal)
+namespace kiwi_hip {
+template <>
+struct KMath<CF> {
+    static CF rcp(CF x) { return CF(1.0) / x; }
+    static CF div(CF a, CF b) { return a / b; }
+    static CF sqrt_(CF x) { return sqrt(x); }
+    static CF log_(CF x) { return log(x); }
+    static CF atan_pos(CF x) { return atan(x); }
+    static CF atan2_pos(CF y, CF x) { return atan2(y, x); }
+};
+}  // namespace kiwi_hip
+
 struct Count {
     long f, t;
 };
@@ -60,7 +73,7 @@ int main() {
         pt[r] = measure([&] {
             CF ax(pts[r][0]), ay(pts[r][1]), az(pts[r][2]);
             CF r2 = ax * ax + ay * ay + az * az;
-            CF rr = sqrt(r2);
+            CF rr = KMath<CF>::sqrt_(r2);
             int k = m4_region(r2);
             CF csum = CF(0.1) + CF(0.2) + CF(0.3);
             CF dsum = (k == 1) ? CF(0) : CF(0.1) + CF(0.1) + CF(0.1) + CF(0.1) + CF(0.1) + CF(0.1);

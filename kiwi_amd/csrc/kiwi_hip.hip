@@ -278,7 +278,7 @@ __global__ __launch_bounds__(THREADS) void k_gather_m4(
             if (dx != R(0) && dy != R(0) && dz != R(0)) {
                 const R ax = fabs(dx), ay = fabs(dy), az = fabs(dz);
                 const R r2 = ax * ax + ay * ay + az * az;
-                const R r = sqrt(r2);
+                const R r = KMath<R>::sqrt_(r2);
                 const int reg = m4_region(r2);
                 ++c_pts[reg - 1];
                 const R csum = L.pc[0][i][reg - 1] + L.pc[1][j][reg - 1] + L.pc[2][k][reg - 1];
