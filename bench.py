@@ -47,13 +47,16 @@ def run_precision(ctx, hip, dev, dom, res, out, prec, steps, warmup):
     for _ in range(warmup):
         ctx.sph_to_grid(dev, dom, res, precision=prec, out=out)
     torch.cuda.synchronize()
-    gms, stats = [], None
+    gms, lms, pms, stats = [], [], [], None
     t0 = time.perf_counter()
     for _ in range(steps):
         *_, stats = ctx.sph_to_grid(dev, dom, res, precision=prec, out=out)
         gms.append(stats.device_ms_gather)
+        lms.append(stats.device_ms_lines)
+        pms.append(stats.device_ms_prep)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    log(f"  device ms per step: prep {np.mean(pms):.2f}, lines {np.mean(lms):.2f}, gather {np.mean(gms):.2f}")
     return (t1 - t0) * 1e3 / steps, float(np.mean(gms)), stats
 
 
@@ -138,7 +141,7 @@ def main():
         "mixed_total_mass_rel_err": mass_err,
         "mixed_field_err_over_max": field_err,
         "pairs_sphere": pairs, "pairs_bbox": st32.pairs_bbox, "particle_blocks": st32.particle_blocks,
-        "gather_ms": g32, "prep_ms": st32.device_ms_prep,
+        "gather_ms": g32, "prep_ms": st32.device_ms_prep, "lines_ms": st32.device_ms_lines,
         "work": {"lattice_points": list(st32.lattice_points), "line_evals": list(st32.line_evals),
                  "plane_evals": st32.plane_evals},
         "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_TFLOPS["mixed"], "unit": "TFLOP/s",

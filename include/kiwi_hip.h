@@ -97,7 +97,8 @@ typedef struct kiwi_stats {
     int64_t pairs_sphere;     /* pairs whose cell intersects the support sphere (the throughput unit) */
     int64_t particle_blocks;  /* (particle, cell-block) work items */
     double v_mean[3];         /* mean velocity over all particles */
-    double device_ms_prep;    /* binning + sort */
+    double device_ms_prep;    /* binning, sorts, block order */
+    double device_ms_lines;   /* per-particle edge-line constants */
     double device_ms_gather;  /* the cell-integral kernel */
     int64_t lattice_points[3]; /* lattice values evaluated, by radial region (inside h, 2h, outside) */
     int64_t line_evals[3];     /* edge-line constants with 0, 1 or 2 crossing evaluations */
