@@ -11,9 +11,10 @@ The metric is cell-integrals/s: (particle, cell) pairs whose cell intersects the
 sphere, divided by the step time.  `value` is the mixed-precision rate; the FP64 rate, the speed ratio
 and the mixed-vs-FP64 relative total-mass error are reported in the same line.
 
-roofline: the gather kernel's algorithmic FLOPs per launch (work counters of the launch times the
-per-unit counts of kiwi_amd/flops_per_unit.json, produced by kiwi_amd/csrc/flop_count.cpp) divided by
-its average duration, measured with HIP events on the stream the kernel runs on.
+roofline: the cell-integral kernel (k_particles: lattice values and per-pair cell sums) -- its
+algorithmic FLOPs per launch (work counters of the launch times the per-unit counts of
+kiwi_amd/flops_per_unit.json, produced by kiwi_amd/csrc/flop_count.cpp) divided by its average duration,
+measured with HIP events on the stream the kernel runs on.
 cpu_baseline: the CPU oracle (a bit-exact restatement of the reference's loop) on a bounded sample of
 the same workload (every 128th particle, same grid), on this host's cores.
 """
@@ -36,10 +37,14 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def gather_flops(st, fpu):
+def cell_kernel_flops(st, fpu):
+    """Algorithmic FLOPs of one k_particles launch: lattice points by region, wall planes, cell stencils."""
     pts = sum(st.lattice_points[r] * fpu["point"][r] for r in range(3))
-    lines = sum(st.line_evals[r] * fpu["line"][r] for r in range(3))
-    return pts + lines + st.plane_evals * fpu["plane"] + st.pairs_sphere * fpu["cell"]
+    return pts + st.plane_evals * fpu["plane"] + st.pairs_sphere * fpu["cell"]
+
+
+def line_kernel_flops(st, fpu):
+    return sum(st.line_evals[r] * fpu["line"][r] for r in range(3))
 
 
 def run_precision(ctx, hip, dev, dom, res, out, prec, steps, warmup):
@@ -47,16 +52,18 @@ def run_precision(ctx, hip, dev, dom, res, out, prec, steps, warmup):
     for _ in range(warmup):
         ctx.sph_to_grid(dev, dom, res, precision=prec, out=out)
     torch.cuda.synchronize()
-    gms, lms, pms, stats = [], [], [], None
+    gms, lms, pms, ams, stats = [], [], [], [], None
     t0 = time.perf_counter()
     for _ in range(steps):
         *_, stats = ctx.sph_to_grid(dev, dom, res, precision=prec, out=out)
         gms.append(stats.device_ms_gather)
         lms.append(stats.device_ms_lines)
         pms.append(stats.device_ms_prep)
+        ams.append(stats.device_ms_accum)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    log(f"  device ms per step: prep {np.mean(pms):.2f}, lines {np.mean(lms):.2f}, gather {np.mean(gms):.2f}")
+    log(f"  device ms per step: prep {np.mean(pms):.2f}, lines {np.mean(lms):.2f}, cells {np.mean(gms):.2f}, "
+        f"accum {np.mean(ams):.2f}")
     return (t1 - t0) * 1e3 / steps, float(np.mean(gms)), stats
 
 
@@ -104,9 +111,9 @@ def main():
     ctx = hip.Context(0)
 
     ms64, g64, st64 = run_precision(ctx, hip, dev, dom, res, out64, hip.FP64, args.steps, args.warmup)
-    log(f"fp64: {ms64:.2f} ms/step, gather {g64:.2f} ms, pairs {st64.pairs_sphere}")
+    log(f"fp64: {ms64:.2f} ms/step, cell kernel {g64:.2f} ms, pairs {st64.pairs_sphere}")
     ms32, g32, st32 = run_precision(ctx, hip, dev, dom, res, out32, hip.MIXED, args.steps, args.warmup)
-    log(f"mixed: {ms32:.2f} ms/step, gather {g32:.2f} ms")
+    log(f"mixed: {ms32:.2f} ms/step, cell kernel {g32:.2f} ms")
 
     m64 = out64[0].cpu().numpy()
     m32 = out32[0].cpu().numpy()
@@ -118,11 +125,11 @@ def main():
 
     pairs = st32.pairs_sphere
     value = pairs / (ms32 * 1e-3)
-    fl32 = gather_flops(st32, fpu)
-    fl64 = gather_flops(st64, fpu)
+    fl32 = cell_kernel_flops(st32, fpu)
+    fl64 = cell_kernel_flops(st64, fpu)
     ach = fl32 / (g32 * 1e-3) / 1e12
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "gather_m4_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "cells_m4_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("mixed_bytes_per_launch")
@@ -135,18 +142,20 @@ def main():
         "config": {"workload": f"sph-to-grid, {args.particles} log-normal clustered particles -> "
                                f"{res[0]}^3, M4, h 0.5-50 cells", "particles": args.particles,
                    "grid": list(res), "kernel": "M4", "parallelism": "single GPU"},
-        "fp64": {"value": pairs / (ms64 * 1e-3), "ms_per_step": ms64, "gather_ms": g64,
+        "fp64": {"value": pairs / (ms64 * 1e-3), "ms_per_step": ms64, "cells_ms": g64,
                  "roofline_frac": fl64 / (g64 * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"]},
         "mixed_over_fp64": ms64 / ms32,
         "mixed_total_mass_rel_err": mass_err,
         "mixed_field_err_over_max": field_err,
         "pairs_sphere": pairs, "pairs_bbox": st32.pairs_bbox, "particle_blocks": st32.particle_blocks,
-        "gather_ms": g32, "prep_ms": st32.device_ms_prep, "lines_ms": st32.device_ms_lines,
+        "chunk_tasks": st32.chunk_tasks, "cells_ms": g32, "prep_ms": st32.device_ms_prep,
+        "lines_ms": st32.device_ms_lines, "accum_ms": st32.device_ms_accum,
+        "lines_tflops": line_kernel_flops(st32, fpu) / max(st32.device_ms_lines * 1e-3, 1e-12) / 1e12,
         "work": {"lattice_points": list(st32.lattice_points), "line_evals": list(st32.line_evals),
                  "plane_evals": st32.plane_evals},
         "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_TFLOPS["mixed"], "unit": "TFLOP/s",
                      "frac": ach / PEAK_TFLOPS["mixed"], "traffic": traffic,
-                     "flops_per_launch": fl32, "kernel": "k_gather_m4<float>"},
+                     "flops_per_launch": fl32, "kernel": "k_particles<float>"},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

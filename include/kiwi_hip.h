@@ -95,11 +95,13 @@ typedef struct kiwi_stats {
     int64_t n_negative_warn;  /* of which Msum <= -0.01 */
     int64_t pairs_bbox;       /* (particle, cell) pairs in the clamped bounding boxes */
     int64_t pairs_sphere;     /* pairs whose cell intersects the support sphere (the throughput unit) */
-    int64_t particle_blocks;  /* (particle, cell-block) work items */
+    int64_t particle_blocks;  /* (particle, 8^3 cell-block) accumulation items */
+    int64_t chunk_tasks;      /* (particle, chunk of <= 8^3 cells) cell-integral tasks */
     double v_mean[3];         /* mean velocity over all particles */
     double device_ms_prep;    /* binning, sorts, block order */
     double device_ms_lines;   /* per-particle edge-line constants */
-    double device_ms_gather;  /* the cell-integral kernel */
+    double device_ms_gather;  /* cell integrals: lattice values and per-pair cell sums */
+    double device_ms_accum;   /* accumulation of the per-pair sums into the grid */
     int64_t lattice_points[3]; /* lattice values evaluated, by radial region (inside h, 2h, outside) */
     int64_t line_evals[3];     /* edge-line constants with 0, 1 or 2 crossing evaluations */
     int64_t plane_evals;       /* wall-plane constants evaluated */

@@ -2,24 +2,27 @@
 //
 // sph-to-grid pipeline (one call of kiwi_hip_sph_to_grid, all on the context's stream):
 //   1. k_prep        one thread per particle: the reference's bounding box of cells
-//                    (src/cartesian_grid.cpp:209-238, FP64 floor, clamped), the number of cell blocks it
-//                    overlaps and the size of its edge-line table; skipped / pair counters.
-//   2. scans         exclusive prefix sums of both (hipcub).
-//   3. k_emit        one 64-bit key (block << 32 | particle) per (particle, cell block) work item.
+//                    (src/cartesian_grid.cpp:209-238, FP64 floor, clamped) and the sizes of the
+//                    particle's work: cell blocks overlapped, edge lines, chunks, cells; counters.
+//   2. scans         exclusive prefix sums of those sizes (hipcub).
+//   3. k_emit        one 64-bit key (block << 32 | particle) per (particle, cell block) pair;
+//      k_emit_tasks  one task per (particle, chunk of at most 8^3 cells of its own bounding box).
 //   4. sort          radix sort of the keys: every block's particle list comes out in particle order,
 //                    so each cell adds its particles in the reference's single-thread order.
 //   5. k_ranges      [begin, end) of every block's list; k_block_work + a second sort give the
 //                    blocks in order of decreasing work (longest-processing-time-first).
 //   6. k_lines       one wavefront per particle: the region-matching constants (D2, D3) of every
 //                    edge line of the particle's lattice, computed once per particle into HBM.
-//   7. k_gather      one workgroup per cell block, blocks taken in work order.  Particles of the
-//                    block's list are processed in batches: the batch's wall-plane constants and its
-//                    slice of the line tables are staged in LDS, the lattice values W (m4_lattice.h)
-//                    of all batch particles are evaluated as one flat task list, then every thread
-//                    folds the 8-corner stencil of each batch particle into the cells it owns.  Each
-//                    cell is owned by exactly one thread of one workgroup: no float atomics, and the
-//                    results are bit-reproducible.
-//   8. k_vsum        deterministic mean velocity over all particles (the empty-cell fallback).
+//   7. k_particles   one wavefront per chunk task: wall-plane constants and the lattice values W
+//                    (m4_lattice.h) of the chunk are built in the wave's LDS, then the 8-corner
+//                    stencil gives Msum of every cell of the chunk, written to the particle's slab
+//                    of cell sums in HBM (x-major over its bounding box).  Everything a wave touches
+//                    is its own: no workgroup barriers.
+//   8. k_accum       one workgroup per 8^3 cell block, blocks in work order: every thread owns two
+//                    cells and adds Msum m and Msum m v of the block's particles, read from their
+//                    slabs, in list order.  Each cell is owned by exactly one thread: no float
+//                    atomics, and the results are bit-reproducible.
+//   9. k_vsum        deterministic mean velocity over all particles (the empty-cell fallback).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -39,6 +42,11 @@ using namespace kiwi_hip;
 namespace {
 
 constexpr int THREADS = 256;
+constexpr int BLK = 8;                  // accumulation block edge (cells)
+constexpr int CH = 8;                   // particle chunk edge (cells)
+constexpr int CHL = CH + 1;             // lattice points per chunk edge
+constexpr int CH_PTS = CHL * CHL * CHL;
+constexpr int CH_MAXAXIS = 1024;        // chunk index range per axis (grids up to 8192 cells per axis)
 
 struct DevStats {
     unsigned long long skipped_outside, n_negative, n_negative_warn, pairs_bbox, pairs_sphere;
@@ -50,8 +58,7 @@ struct DevStats {
 struct GridGeom {
     double x_min, y_min, z_min, dx, dy, dz;
     int nx, ny, nz;
-    int bsx, bsy, bsz;  // cell-block edge lengths
-    int nbx, nby, nbz;  // blocks per axis
+    int nbx, nby, nbz;  // accumulation blocks per axis
 };
 
 template <typename R>
@@ -65,8 +72,30 @@ __device__ __forceinline__ T sel3(int a, T v0, T v1, T v2) { return a == 0 ? v0 
 __device__ __forceinline__ double gmin(const GridGeom& g, int a) { return sel3(a, g.x_min, g.y_min, g.z_min); }
 __device__ __forceinline__ double gstep(const GridGeom& g, int a) { return sel3(a, g.dx, g.dy, g.dz); }
 
+// u / d for small non-negative integers through a float reciprocal, corrected to be exact
+__device__ __forceinline__ int small_div(int u, int d, float inv_d) {
+    int q = static_cast<int>((static_cast<float>(u) + 0.5f) * inv_d);
+    const int r = u - q * d;
+    q += (r >= d) ? 1 : 0;
+    q -= (r < 0) ? 1 : 0;
+    return q;
+}
+
+// distance component of the interval [lo, hi] (units of h) from the particle
+template <typename R>
+__device__ __forceinline__ R gap(R lo, R hi) {
+    return (lo > R(0)) ? lo : ((hi < R(0)) ? -hi : R(0));
+}
+
+// LDS writes of one lane made visible to the other lanes of the same wavefront
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ------------------------------------------------------------------------------------------------
-// 1. per-particle bounding box
+// 1. per-particle bounding box and work sizes
 // ------------------------------------------------------------------------------------------------
 // floor(v) as a cell index, saturated to [-1, n] before the conversion so that huge supports and
 // non-finite coordinates cannot overflow it (NaN lands on n, outside the grid).  Within range this is
@@ -80,15 +109,17 @@ __device__ __forceinline__ int cell_floor(double v, int n) {
 __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                        const double* __restrict__ z, const double* __restrict__ h, double support_in_h,
                        GridGeom g, int* __restrict__ bbox, int64_t* __restrict__ nblk, int64_t* __restrict__ nlines,
-                       DevStats* st) {
+                       int64_t* __restrict__ ntask, int64_t* __restrict__ ncell, DevStats* st) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     unsigned long long skipped = 0, pairs = 0;
     if (i == n) {  // trailing zeros of the exclusive scans
         nblk[n] = 0;
         nlines[n] = 0;
+        ntask[n] = 0;
+        ncell[n] = 0;
     }
     if (i < n) {
-        int64_t nb = 0, nl = 0;
+        int64_t nb = 0, nl = 0, nt = 0, nc = 0;
         int* bb = bbox + 6 * i;
         bb[0] = 1;
         bb[1] = 0;  // empty unless set below
@@ -117,15 +148,19 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
                 bb[3] = iy1;
                 bb[4] = iz0;
                 bb[5] = iz1;
-                pairs = static_cast<unsigned long long>(ix1 - ix0 + 1) * (iy1 - iy0 + 1) * (iz1 - iz0 + 1);
-                nb = static_cast<int64_t>(ix1 / g.bsx - ix0 / g.bsx + 1) * (iy1 / g.bsy - iy0 / g.bsy + 1) *
-                     (iz1 / g.bsz - iz0 / g.bsz + 1);
-                const int64_t Lx = ix1 - ix0 + 2, Ly = iy1 - iy0 + 2, Lz = iz1 - iz0 + 2;
-                nl = 2 * (Lx * Ly + Lx * Lz + Ly * Lz);
+                const int64_t Nx = ix1 - ix0 + 1, Ny = iy1 - iy0 + 1, Nz = iz1 - iz0 + 1;
+                nc = Nx * Ny * Nz;
+                pairs = static_cast<unsigned long long>(nc);
+                nb = static_cast<int64_t>(ix1 / BLK - ix0 / BLK + 1) * (iy1 / BLK - iy0 / BLK + 1) *
+                     (iz1 / BLK - iz0 / BLK + 1);
+                nl = 2 * ((Nx + 1) * (Ny + 1) + (Nx + 1) * (Nz + 1) + (Ny + 1) * (Nz + 1));
+                nt = ((Nx + CH - 1) / CH) * ((Ny + CH - 1) / CH) * ((Nz + CH - 1) / CH);
             }
         }
         nblk[i] = nb;
         nlines[i] = nl;
+        ntask[i] = nt;
+        ncell[i] = nc;
     }
     for (int off = 32; off > 0; off >>= 1) {
         skipped += __shfl_down(skipped, off);
@@ -138,7 +173,7 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
 }
 
 // ------------------------------------------------------------------------------------------------
-// 3. keys
+// 3. keys and chunk tasks
 // ------------------------------------------------------------------------------------------------
 __global__ void k_emit(int64_t n, const int* __restrict__ bbox, const int64_t* __restrict__ offs, GridGeom g,
                        uint64_t* __restrict__ keys) {
@@ -147,12 +182,26 @@ __global__ void k_emit(int64_t n, const int* __restrict__ bbox, const int64_t* _
     const int* bb = bbox + 6 * i;
     if (bb[0] > bb[1]) return;
     int64_t o = offs[i];
-    for (int bx = bb[0] / g.bsx; bx <= bb[1] / g.bsx; ++bx)
-        for (int by = bb[2] / g.bsy; by <= bb[3] / g.bsy; ++by)
-            for (int bz = bb[4] / g.bsz; bz <= bb[5] / g.bsz; ++bz) {
+    for (int bx = bb[0] / BLK; bx <= bb[1] / BLK; ++bx)
+        for (int by = bb[2] / BLK; by <= bb[3] / BLK; ++by)
+            for (int bz = bb[4] / BLK; bz <= bb[5] / BLK; ++bz) {
                 const uint64_t b = (static_cast<uint64_t>(bx) * g.nby + by) * g.nbz + bz;
                 keys[o++] = (b << 32) | static_cast<uint64_t>(i);
             }
+}
+
+// task = (particle, chunk): chunk indices packed 10 bits per axis
+__global__ void k_emit_tasks(int64_t n, const int* __restrict__ bbox, const int64_t* __restrict__ toffs,
+                             int2* __restrict__ tasks) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int* bb = bbox + 6 * i;
+    if (bb[0] > bb[1]) return;
+    const int cx = (bb[1] - bb[0] + CH) / CH, cy = (bb[3] - bb[2] + CH) / CH, cz = (bb[5] - bb[4] + CH) / CH;
+    int64_t o = toffs[i];
+    for (int a = 0; a < cx; ++a)
+        for (int b = 0; b < cy; ++b)
+            for (int c = 0; c < cz; ++c) tasks[o++] = make_int2(static_cast<int>(i), (a << 20) | (b << 10) | c);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -182,8 +231,9 @@ __global__ void k_ranges(int64_t npairs, const uint64_t* __restrict__ keys, int*
     if (i == npairs - 1 || static_cast<uint32_t>(keys[i + 1] >> 32) != b) end[b] = static_cast<int>(i + 1);
 }
 
-// lattice points of a (particle, block) work item: the work estimate for the block order.  The keys
-// are sorted by block, so equal blocks form runs of lanes; each run adds its total with one atomic.
+// cells of a (particle, block) pair plus a fixed per-pair cost: the work estimate for the block
+// order.  The keys are sorted by block, so equal blocks form runs of lanes; each run adds its total
+// with one atomic.
 __global__ void k_block_work(int64_t npairs, const uint64_t* __restrict__ keys, const int* __restrict__ bbox,
                              GridGeom g, unsigned long long* __restrict__ work) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -196,12 +246,11 @@ __global__ void k_block_work(int64_t npairs, const uint64_t* __restrict__ keys, 
         const int p = static_cast<int>(keys[i] & 0xffffffffull);
         const int bz = b % g.nbz, by = (b / g.nbz) % g.nby, bx = b / (g.nbz * g.nby);
         const int* bb = bbox + 6 * p;
-        const int nx = min(bb[1], bx * g.bsx + g.bsx - 1) - max(bb[0], bx * g.bsx) + 2;
-        const int ny = min(bb[3], by * g.bsy + g.bsy - 1) - max(bb[2], by * g.bsy) + 2;
-        const int nz = min(bb[5], bz * g.bsz + g.bsz - 1) - max(bb[4], bz * g.bsz) + 2;
-        w = static_cast<unsigned long long>(nx * ny * nz + 64);
+        const int nx = min(bb[1], bx * BLK + BLK - 1) - max(bb[0], bx * BLK) + 1;
+        const int ny = min(bb[3], by * BLK + BLK - 1) - max(bb[2], by * BLK) + 1;
+        const int nz = min(bb[5], bz * BLK + BLK - 1) - max(bb[4], bz * BLK) + 1;
+        w = static_cast<unsigned long long>(nx * ny * nz + 32);
     }
-    // inclusive segmented scan (segments = runs of equal b) towards higher lanes
     for (int off = 1; off < 64; off <<= 1) {
         const unsigned long long wv = __shfl_up(w, off);
         const uint32_t bv = __shfl_up(b, off);
@@ -272,263 +321,105 @@ __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __re
 }
 
 // ------------------------------------------------------------------------------------------------
-// 7. the gather: one workgroup per cell block
+// 7. lattice values and cell sums, one wavefront per (particle, chunk)
 // ------------------------------------------------------------------------------------------------
-template <int BX, int BY, int BZ>
-struct Shape {
-    static constexpr int NX = BX + 1, NY = BY + 1, NZ = BZ + 1;      // lattice points per block edge
-    static constexpr int CELLS = BX * BY * BZ;
-    static constexpr int CPT = CELLS / THREADS;                      // cells per thread
-    static constexpr int COLS = BY * BZ;                             // (y, z) columns of the block
-    static constexpr int TPC = THREADS / COLS;                       // threads per column
-    static constexpr int QMAX = 16;                                  // particles per batch
-    static constexpr int PMAX = NX * NY * NZ > 1024 ? NX * NY * NZ : 1024;            // lattice values per batch
-    static constexpr int LMAX_ONE = 2 * (NX * NY + NX * NZ + NY * NZ);
-    static constexpr int LNMAX = LMAX_ONE > 1024 ? LMAX_ONE : 1024;                  // line entries per batch
-    static constexpr int PLMAX = QMAX * (NX + NY + NZ);                              // plane entries per batch
-    static_assert(CELLS % THREADS == 0, "block cells must be a multiple of the workgroup size");
-    static_assert(THREADS % COLS == 0 && TPC * CPT == BX, "each thread owns a run of cells along x");
-};
-
-template <typename R, class S>
-struct GatherLds {
-    R U[S::PMAX];              // signed lattice values s * W, per batch particle, (i, j, k) k fastest
-    R sd[S::PLMAX];            // signed offsets (lattice - particle) / h, per batch particle: x..., y..., z...
-    union {
-        struct {
-            R pc[3][S::PLMAX];     // plane constants c1, c2, c3 of |sd|        (phases A, B)
-            Pair2<R> ln[S::LNMAX]; // the batch's slices of the line tables     (phases A, B)
-        } ab;
-        double Ms[S::PMAX];        // Msum of every (batch particle, intersection cell)   (phases C1, C2)
-    } u;
-    int q_p[S::QMAX];
-    int q_lo[3][S::QMAX];      // intersection low cell, block-local
-    int q_n[3][S::QMAX];       // intersection lattice points per axis (cells + 1)
-    int q_g[3][S::QMAX];       // intersection low lattice coordinate relative to the particle's bbox
-    int q_L[3][S::QMAX];       // particle bbox lattice extents
-    int q_pt[S::QMAX + 1], q_pl[S::QMAX + 1], q_ln[S::QMAX + 1], q_cl[S::QMAX + 1];
-    float q_inv_nyz[S::QMAX], q_inv_nz[S::QMAX], q_inv_cyz[S::QMAX], q_inv_cz[S::QMAX];
-    long long q_lbase[S::QMAX];
-    double q_m[S::QMAX], q_v[3][S::QMAX], q_pos[3][S::QMAX], q_ih[S::QMAX];
-    int s_nq;
-};
-
-// index of the batch particle owning flat task t, given the batch prefix offsets off[0..nq]
-template <int QMAX>
-__device__ __forceinline__ int owner(const int* off, int nq, int t) {
-    int q = 0;
-#pragma unroll
-    for (int s = QMAX / 2; s > 0; s >>= 1)
-        if (q + s < nq && off[q + s] <= t) q += s;
-    return q;
-}
-
-// u / d for small non-negative integers through a float reciprocal, corrected to be exact
-__device__ __forceinline__ int small_div(int u, int d, float inv_d) {
-    int q = static_cast<int>((static_cast<float>(u) + 0.5f) * inv_d);
-    const int r = u - q * d;
-    q += (r >= d) ? 1 : 0;
-    q -= (r < 0) ? 1 : 0;
-    return q;
-}
-
-// distance component of the interval [lo, hi] (units of h) from the particle
 template <typename R>
-__device__ __forceinline__ R gap(R lo, R hi) {
-    return (lo > R(0)) ? lo : ((hi < R(0)) ? -hi : R(0));
-}
+struct WaveLds {
+    R sd[3 * CHL];     // signed offsets (lattice - particle) / h: x..., y..., z...
+    R pc[3][3 * CHL];  // plane constants c1, c2, c3 of |sd|
+    R U[CH_PTS];       // signed lattice values s * W, (i, j, k) k fastest
+};
 
-template <typename R, int BX, int BY, int BZ>
-__global__ __launch_bounds__(THREADS) void k_gather(
-    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
-    const double* __restrict__ m, const double* __restrict__ h, const int* __restrict__ bbox,
-    const uint64_t* __restrict__ keys, const int* __restrict__ blk_begin, const int* __restrict__ blk_end,
-    const int* __restrict__ order, const int64_t* __restrict__ loffs, const Pair2<R>* __restrict__ lines, GridGeom g,
-    double* __restrict__ out_mass, double* __restrict__ out_px, double* __restrict__ out_py,
-    double* __restrict__ out_pz, DevStats* st, int clamp_pairs) {
-    using S = Shape<BX, BY, BZ>;
-    __shared__ GatherLds<R, S> L;
-    const int b = order[blockIdx.x];
-    const int bz = b % g.nbz;
-    const int by = (b / g.nbz) % g.nby;
-    const int bx = b / (g.nbz * g.nby);
-    const int o[3] = {bx * BX, by * BY, bz * BZ};
-    const int tid = threadIdx.x;
-    // owned cells: the column (cy, cz) and a run of CPT cells along x starting at cx0
-    const int col = tid % S::COLS;
-    const int own_cy = col / BZ, own_cz = col % BZ;
-    const int own_cx0 = (tid / S::COLS) * S::CPT;
+template <typename R>
+__global__ __launch_bounds__(THREADS) void k_particles(
+    int64_t ntasks, const int2* __restrict__ tasks, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ h, const int* __restrict__ bbox,
+    const int64_t* __restrict__ loffs, const int64_t* __restrict__ coffs, GridGeom g,
+    const Pair2<R>* __restrict__ lines, R* __restrict__ ms, DevStats* st, int clamp_pairs) {
+    __shared__ WaveLds<R> WL[THREADS / 64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WaveLds<R>& L = WL[wave];
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (THREADS / 64);
+    unsigned long long n_neg = 0, n_warn = 0, n_sphere = 0, c_planes = 0;
+    unsigned long long c_pts[3] = {0, 0, 0};
 
-    double acc[S::CPT][4];
-#pragma unroll
-    for (int s = 0; s < S::CPT; ++s) acc[s][0] = acc[s][1] = acc[s][2] = acc[s][3] = 0.0;
-    unsigned long long n_neg = 0, n_warn = 0, n_sphere = 0;
-    unsigned long long c_pts[3] = {0, 0, 0}, c_planes = 0;
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * (THREADS / 64) + wave; t < ntasks; t += nwaves) {
+        const int2 tk = tasks[t];
+        const int p = tk.x;
+        const int* bb = bbox + 6 * p;
+        const int bx0 = bb[0], by0 = bb[2], bz0 = bb[4];
+        const int Nx = bb[1] - bb[0] + 1, Ny = bb[3] - bb[2] + 1, Nz = bb[5] - bb[4] + 1;  // particle cells
+        const int Lx = Nx + 1, Ly = Ny + 1, Lz = Nz + 1;                                     // particle lattice
+        // the particle's box is cut into ceil(N / CH) near-equal chunks per axis
+        const int kx = (Nx + CH - 1) / CH, ky = (Ny + CH - 1) / CH, kz = (Nz + CH - 1) / CH;
+        const int ax = (tk.y >> 20) & 1023, ay = (tk.y >> 10) & 1023, az = tk.y & 1023;
+        const int c0x = ax * Nx / kx, c0y = ay * Ny / ky, c0z = az * Nz / kz;
+        const int mx = (ax + 1) * Nx / kx - c0x, my = (ay + 1) * Ny / ky - c0y, mz = (az + 1) * Nz / kz - c0z;
+        const int nx = mx + 1, ny = my + 1, nz = mz + 1;                                     // chunk lattice
+        const double inv_h = 1.0 / h[p];
+        const double px = x[p], py = y[p], pz = z[p];
+        const Pair2<R>* ltab = lines + loffs[p];
+        const int64_t cbase = coffs[p];
 
-    const int q1 = blk_end[b];
-    int cursor = blk_begin[b];
-    while (cursor < q1) {
-        // ---- phase 0: the next batch (wave 0) ----
-        if (tid < 64) {
-            const int lane = tid;
-            const int idx = cursor + lane;
-            const bool valid = lane < S::QMAX && idx < q1;
-            constexpr int BIG = 1 << 20;
-            int npt = BIG, nln = BIG, npl = 0, ncl = 0, p = 0;
-            int lo[3] = {0, 0, 0}, nn[3] = {0, 0, 0}, gg[3] = {0, 0, 0}, LL[3] = {0, 0, 0};
-            if (valid) {
-                p = static_cast<int>(keys[idx] & 0xffffffffull);
-                const int* bb = bbox + 6 * p;
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const int blo = bb[2 * a], bhi = bb[2 * a + 1];
-                    const int l0 = max(blo - o[a], 0), l1 = min(bhi - o[a], sel3(a, BX, BY, BZ) - 1);
-                    lo[a] = l0;
-                    nn[a] = l1 - l0 + 2;
-                    gg[a] = o[a] + l0 - blo;
-                    LL[a] = bhi - blo + 2;
-                }
-                npt = nn[0] * nn[1] * nn[2];
-                nln = 2 * (nn[0] * nn[1] + nn[0] * nn[2] + nn[1] * nn[2]);
-                npl = nn[0] + nn[1] + nn[2];
-                ncl = (nn[0] - 1) * (nn[1] - 1) * (nn[2] - 1);
-            }
-            int ipt = npt, iln = nln, ipl = npl, icl = ncl;
-            for (int off = 1; off < 64; off <<= 1) {
-                const int a = __shfl_up(ipt, off), c = __shfl_up(iln, off), d = __shfl_up(ipl, off),
-                          e = __shfl_up(icl, off);
-                if (lane >= off) {
-                    ipt += a;
-                    iln += c;
-                    ipl += d;
-                    icl += e;
+        // ---- plane constants of the chunk's lattice coordinates ----
+        if (lane < nx + ny + nz) {
+            int a = 0, u = lane;
+            if (u >= nx) {
+                u -= nx;
+                a = 1;
+                if (u >= ny) {
+                    u -= ny;
+                    a = 2;
                 }
             }
-            const bool take = valid && ipt <= S::PMAX && iln <= S::LNMAX;
-            const int nq = __popcll(__ballot(take));
-            if (take) {
-                L.q_p[lane] = p;
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    L.q_lo[a][lane] = lo[a];
-                    L.q_n[a][lane] = nn[a];
-                    L.q_g[a][lane] = gg[a];
-                    L.q_L[a][lane] = LL[a];
-                }
-                L.q_pt[lane] = ipt - npt;
-                L.q_ln[lane] = iln - nln;
-                L.q_pl[lane] = ipl - npl;
-                L.q_cl[lane] = icl - ncl;
-                L.q_inv_nyz[lane] = 1.0f / static_cast<float>(nn[1] * nn[2]);
-                L.q_inv_nz[lane] = 1.0f / static_cast<float>(nn[2]);
-                L.q_inv_cyz[lane] = 1.0f / static_cast<float>((nn[1] - 1) * (nn[2] - 1));
-                L.q_inv_cz[lane] = 1.0f / static_cast<float>(nn[2] - 1);
-                L.q_lbase[lane] = loffs[p];
-                L.q_m[lane] = m[p];
-                L.q_v[0][lane] = vx[p];
-                L.q_v[1][lane] = vy[p];
-                L.q_v[2][lane] = vz[p];
-                L.q_pos[0][lane] = x[p];
-                L.q_pos[1][lane] = y[p];
-                L.q_pos[2][lane] = z[p];
-                L.q_ih[lane] = 1.0 / h[p];
-                if (lane == nq - 1) {
-                    L.q_pt[nq] = ipt;
-                    L.q_ln[nq] = iln;
-                    L.q_pl[nq] = ipl;
-                    L.q_cl[nq] = icl;
-                }
-            }
-            if (lane == 0) L.s_nq = nq;
+            const int gi = sel3(a, bx0 + c0x, by0 + c0y, bz0 + c0z) + u;
+            const double d = (gmin(g, a) + gi * gstep(g, a) - sel3(a, px, py, pz)) * inv_h;
+            const R dr = static_cast<R>(d);
+            const M4Plane<R> pl = m4_plane<R>(dr < R(0) ? -dr : dr);
+            L.sd[lane] = dr;
+            L.pc[0][lane] = pl.c1;
+            L.pc[1][lane] = pl.c2;
+            L.pc[2][lane] = pl.c3;
+            ++c_planes;
         }
-        __syncthreads();
-        const int nq = L.s_nq;
-        cursor += nq;
+        wave_sync();
 
-        // ---- phase A: plane constants and the line-table slices ----
-        const int npl_all = L.q_pl[nq];
-        const int nln_all = L.q_ln[nq];
-        for (int t = tid; t < npl_all + nln_all; t += THREADS) {
-            if (t < npl_all) {
-                const int q = owner<S::QMAX>(L.q_pl, nq, t);
-                int u = t - L.q_pl[q], a = 0;
-                if (u >= L.q_n[0][q]) {
-                    u -= L.q_n[0][q];
-                    a = 1;
-                    if (u >= L.q_n[1][q]) {
-                        u -= L.q_n[1][q];
-                        a = 2;
-                    }
-                }
-                const int gi = sel3(a, o[0], o[1], o[2]) + L.q_lo[a][q] + u;
-                const double d = (gmin(g, a) + gi * gstep(g, a) - L.q_pos[a][q]) * L.q_ih[q];
-                const R dr = static_cast<R>(d);
-                const M4Plane<R> pl = m4_plane<R>(dr < R(0) ? -dr : dr);
-                L.sd[t] = dr;
-                L.u.ab.pc[0][t] = pl.c1;
-                L.u.ab.pc[1][t] = pl.c2;
-                L.u.ab.pc[2][t] = pl.c3;
-                ++c_planes;
-            } else {
-                const int tl = t - npl_all;
-                const int q = owner<S::QMAX>(L.q_ln, nq, tl);
-                int u = tl - L.q_ln[q];
-                const int n0 = L.q_n[0][q], n1 = L.q_n[1][q], n2 = L.q_n[2][q];
-                const long long L0 = L.q_L[0][q], L1 = L.q_L[1][q], L2 = L.q_L[2][q];
-                // pair blocks (x,y) (x,z) (y,x) (y,z) (z,x) (z,y): slice sizes n_a n_b, full-table sizes L_a L_b
-                int a, bq;
-                long long gbase;
-                if (u < n0 * n1) { a = 0; bq = 1; gbase = 0; }
-                else if ((u -= n0 * n1) < n0 * n2) { a = 0; bq = 2; gbase = L0 * L1; }
-                else if ((u -= n0 * n2) < n1 * n0) { a = 1; bq = 0; gbase = L0 * L1 + L0 * L2; }
-                else if ((u -= n1 * n0) < n1 * n2) { a = 1; bq = 2; gbase = 2 * L0 * L1 + L0 * L2; }
-                else if ((u -= n1 * n2) < n2 * n0) { a = 2; bq = 0; gbase = 2 * L0 * L1 + L0 * L2 + L1 * L2; }
-                else { u -= n2 * n0; a = 2; bq = 1; gbase = 2 * L0 * L1 + 2 * L0 * L2 + L1 * L2; }
-                const int nb_ = sel3(bq, n0, n1, n2);
-                const int ia = u / nb_, ib = u - ia * nb_;
-                const long long gidx = L.q_lbase[q] + gbase +
-                                       static_cast<long long>(L.q_g[a][q] + ia) * sel3(bq, L0, L1, L2) +
-                                       (L.q_g[bq][q] + ib);
-                L.u.ab.ln[tl] = lines[gidx];
-            }
-        }
-        __syncthreads();
-
-        // ---- phase B: lattice values of all batch particles ----
-        const int npt_all = L.q_pt[nq];
-        for (int t = tid; t < npt_all; t += THREADS) {
-            const int q = owner<S::QMAX>(L.q_pt, nq, t);
-            const int u = t - L.q_pt[q];
-            const int nx = L.q_n[0][q], ny = L.q_n[1][q], nz = L.q_n[2][q];
-            const int i = small_div(u, ny * nz, L.q_inv_nyz[q]);
-            const int rem = u - i * (ny * nz);
-            const int j = small_div(rem, nz, L.q_inv_nz[q]);
+        // ---- lattice values ----
+        const int nyz = ny * nz;
+        const float inv_nyz = 1.0f / static_cast<float>(nyz), inv_nz = 1.0f / static_cast<float>(nz);
+        // global line-table bases of the six pair blocks
+        const int64_t pxy = 0, pxz = pxy + static_cast<int64_t>(Lx) * Ly, pyx = pxz + static_cast<int64_t>(Lx) * Lz,
+                      pyz = pyx + static_cast<int64_t>(Ly) * Lx, pzx = pyz + static_cast<int64_t>(Ly) * Lz,
+                      pzy = pzx + static_cast<int64_t>(Lz) * Lx;
+        const int npts = nx * nyz;
+        for (int u = lane; u < npts; u += 64) {
+            const int i = small_div(u, nyz, inv_nyz);
+            const int rem = u - i * nyz;
+            const int j = small_div(rem, nz, inv_nz);
             const int k = rem - j * nz;
-            const int pb = L.q_pl[q];
-            const R dx = L.sd[pb + i], dy = L.sd[pb + nx + j], dz = L.sd[pb + nx + ny + k];
+            const R dx = L.sd[i], dy = L.sd[nx + j], dz = L.sd[nx + ny + k];
             R U = R(0);
             if (dx != R(0) && dy != R(0) && dz != R(0)) {
-                // skip points that no cell of this intersection reaching the support sphere uses
-                const R ex = gap(L.sd[pb + max(i - 1, 0)], L.sd[pb + min(i + 1, nx - 1)]);
-                const R ey = gap(L.sd[pb + nx + max(j - 1, 0)], L.sd[pb + nx + min(j + 1, ny - 1)]);
-                const R ez = gap(L.sd[pb + nx + ny + max(k - 1, 0)], L.sd[pb + nx + ny + min(k + 1, nz - 1)]);
+                // skip points that no cell of this chunk reaching the support sphere uses
+                const R ex = gap(L.sd[max(i - 1, 0)], L.sd[min(i + 1, nx - 1)]);
+                const R ey = gap(L.sd[nx + max(j - 1, 0)], L.sd[nx + min(j + 1, ny - 1)]);
+                const R ez = gap(L.sd[nx + ny + max(k - 1, 0)], L.sd[nx + ny + min(k + 1, nz - 1)]);
                 if (ex * ex + ey * ey + ez * ez < R(4)) {
                     const R ax = dx < R(0) ? -dx : dx, ay = dy < R(0) ? -dy : dy, az = dz < R(0) ? -dz : dz;
                     const R r2 = ax * ax + ay * ay + az * az;
                     const R r = KMath<R>::sqrt_(r2);
                     const int reg = m4_region(r2);
                     ++c_pts[reg - 1];
-                    const R csum = L.u.ab.pc[reg - 1][pb + i] + L.u.ab.pc[reg - 1][pb + nx + j] +
-                                   L.u.ab.pc[reg - 1][pb + nx + ny + k];
+                    const R csum = L.pc[reg - 1][i] + L.pc[reg - 1][nx + j] + L.pc[reg - 1][nx + ny + k];
                     R dsum = R(0);
                     if (reg != 1) {
-                        const int lb = L.q_ln[q];
-                        const int pxy = lb, pxz = pxy + nx * ny, pyx = pxz + nx * nz, pyz = pyx + ny * nx,
-                                  pzx = pyz + ny * nz, pzy = pzx + nz * nx;
-                        const Pair2<R> l0 = L.u.ab.ln[pxy + i * ny + j], l1 = L.u.ab.ln[pxz + i * nz + k];
-                        const Pair2<R> l2 = L.u.ab.ln[pyx + j * nx + i], l3 = L.u.ab.ln[pyz + j * nz + k];
-                        const Pair2<R> l4 = L.u.ab.ln[pzx + k * nx + i], l5 = L.u.ab.ln[pzy + k * ny + j];
+                        const int gi = c0x + i, gj = c0y + j, gk = c0z + k;  // particle-lattice coordinates
+                        const Pair2<R> l0 = ltab[pxy + static_cast<int64_t>(gi) * Ly + gj];
+                        const Pair2<R> l1 = ltab[pxz + static_cast<int64_t>(gi) * Lz + gk];
+                        const Pair2<R> l2 = ltab[pyx + static_cast<int64_t>(gj) * Lx + gi];
+                        const Pair2<R> l3 = ltab[pyz + static_cast<int64_t>(gj) * Lz + gk];
+                        const Pair2<R> l4 = ltab[pzx + static_cast<int64_t>(gk) * Lx + gi];
+                        const Pair2<R> l5 = ltab[pzy + static_cast<int64_t>(gk) * Ly + gj];
                         dsum = (reg == 2) ? (((l0.a + l1.a) + (l2.a + l3.a)) + (l4.a + l5.a))
                                           : (((l0.b + l1.b) + (l2.b + l3.b)) + (l4.b + l5.b));
                     }
@@ -537,86 +428,160 @@ __global__ __launch_bounds__(THREADS) void k_gather(
                     U = neg ? -W : W;
                 }
             }
-            L.U[t] = U;
+            L.U[u] = U;
         }
-        __syncthreads();
+        wave_sync();
 
-        // ---- phase C1: Msum of every (batch particle, intersection cell), a flat task list ----
-        const int ncl_all = L.q_cl[nq];
-        for (int t = tid; t < ncl_all; t += THREADS) {
-            const int q = owner<S::QMAX>(L.q_cl, nq, t);
-            const int u = t - L.q_cl[q];
-            const int nx = L.q_n[0][q], ny = L.q_n[1][q], nz = L.q_n[2][q];
-            const int cyz = (ny - 1) * (nz - 1);
-            const int i = small_div(u, cyz, L.q_inv_cyz[q]);
-            const int rem = u - i * cyz;
-            const int j = small_div(rem, nz - 1, L.q_inv_cz[q]);
-            const int k = rem - j * (nz - 1);
-            const int pb = L.q_pl[q];
-            const R ex = gap(L.sd[pb + i], L.sd[pb + i + 1]);
-            const R ey = gap(L.sd[pb + nx + j], L.sd[pb + nx + j + 1]);
-            const R ez = gap(L.sd[pb + nx + ny + k], L.sd[pb + nx + ny + k + 1]);
+        // ---- cell sums: the 8-corner stencil, written to the particle's slab ----
+        const int myz = my * mz;
+        const float inv_myz = 1.0f / static_cast<float>(myz), inv_mz = 1.0f / static_cast<float>(mz);
+        const int ncells = mx * myz;
+        for (int u = lane; u < ncells; u += 64) {
+            const int i = small_div(u, myz, inv_myz);
+            const int rem = u - i * myz;
+            const int j = small_div(rem, mz, inv_mz);
+            const int k = rem - j * mz;
+            const R ex = gap(L.sd[i], L.sd[i + 1]);
+            const R ey = gap(L.sd[nx + j], L.sd[nx + j + 1]);
+            const R ez = gap(L.sd[nx + ny + k], L.sd[nx + ny + k + 1]);
             double Ms = 0.0;
             if (ex * ex + ey * ey + ez * ez < R(4)) {
                 ++n_sphere;
-                const R* U0 = &L.U[L.q_pt[q] + (i * ny + j) * nz + k];
-                const int sx = ny * nz, sy = nz;
-                Ms = static_cast<double>(U0[sx]) + static_cast<double>(U0[sy]) + static_cast<double>(U0[1]) +
-                     static_cast<double>(U0[sx + sy + 1]) - static_cast<double>(U0[0]) -
-                     static_cast<double>(U0[sx + sy]) - static_cast<double>(U0[sx + 1]) -
-                     static_cast<double>(U0[sy + 1]);
+                const R* U0 = &L.U[(i * ny + j) * nz + k];
+                Ms = static_cast<double>(U0[nyz]) + static_cast<double>(U0[nz]) + static_cast<double>(U0[1]) +
+                     static_cast<double>(U0[nyz + nz + 1]) - static_cast<double>(U0[0]) -
+                     static_cast<double>(U0[nyz + nz]) - static_cast<double>(U0[nyz + 1]) -
+                     static_cast<double>(U0[nz + 1]);
                 if (clamp_pairs && Ms < 0.0) {
                     ++n_neg;
                     if (Ms <= -0.01) {
                         ++n_warn;
                         const unsigned long long slot = atomicAdd(&st->n_warn, 1ull);
                         if (slot < KIWI_MAX_WARN_RECORDS) {
-                            st->warn[slot].particle = L.q_p[q];
-                            st->warn[slot].cell =
-                                (static_cast<int64_t>(o[0] + L.q_lo[0][q] + i) * g.ny + (o[1] + L.q_lo[1][q] + j)) * g.nz +
-                                (o[2] + L.q_lo[2][q] + k);
+                            st->warn[slot].particle = p;
+                            st->warn[slot].cell = (static_cast<int64_t>(bx0 + c0x + i) * g.ny + (by0 + c0y + j)) * g.nz +
+                                                  (bz0 + c0z + k);
                             st->warn[slot].value = Ms;
                         }
                     }
                     Ms = 0.0;
                 }
             }
-            L.u.Ms[t] = Ms;
+            ms[cbase + (static_cast<int64_t>(c0x + i) * Ny + (c0y + j)) * Nz + (c0z + k)] = static_cast<R>(Ms);
+        }
+        wave_sync();
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        n_neg += __shfl_down(n_neg, off);
+        n_warn += __shfl_down(n_warn, off);
+        n_sphere += __shfl_down(n_sphere, off);
+        c_planes += __shfl_down(c_planes, off);
+        for (int r = 0; r < 3; ++r) c_pts[r] += __shfl_down(c_pts[r], off);
+    }
+    if (lane == 0) {
+        if (n_neg) atomicAdd(&st->n_negative, n_neg);
+        if (n_warn) atomicAdd(&st->n_negative_warn, n_warn);
+        if (n_sphere) atomicAdd(&st->pairs_sphere, n_sphere);
+        if (c_planes) atomicAdd(&st->planes, c_planes);
+        for (int r = 0; r < 3; ++r)
+            if (c_pts[r]) atomicAdd(&st->pts[r], c_pts[r]);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// 8. accumulation: one workgroup per 8^3 cell block
+// ------------------------------------------------------------------------------------------------
+constexpr int ACC_CPT = BLK * BLK * BLK / THREADS;  // 2 cells per thread, along x
+constexpr int ACC_GROUP = 8;                         // particles whose sums are loaded before adding
+
+struct AccLds {
+    int bb[6][THREADS];
+    long long cb[THREADS];
+    double m[THREADS], v[3][THREADS];
+};
+
+template <typename R>
+__global__ __launch_bounds__(THREADS) void k_accum(
+    const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
+    const double* __restrict__ m, const int* __restrict__ bbox, const uint64_t* __restrict__ keys,
+    const int* __restrict__ blk_begin, const int* __restrict__ blk_end, const int* __restrict__ order,
+    const int64_t* __restrict__ coffs, const R* __restrict__ ms, GridGeom g, double* __restrict__ out_mass,
+    double* __restrict__ out_px, double* __restrict__ out_py, double* __restrict__ out_pz, int clamp_pairs) {
+    __shared__ AccLds A;
+    const int b = order[blockIdx.x];
+    const int bz = b % g.nbz;
+    const int by = (b / g.nbz) % g.nby;
+    const int bx = b / (g.nbz * g.nby);
+    const int tid = threadIdx.x;
+    // owned cells: column (cy, cz) = tid % 64, x = 2 * (tid / 64) + s
+    const int gy = by * BLK + (tid & 63) / BLK, gz = bz * BLK + (tid & 7);
+    const int gx0 = bx * BLK + (tid >> 6) * ACC_CPT;
+
+    double acc[ACC_CPT][4];
+#pragma unroll
+    for (int s = 0; s < ACC_CPT; ++s) acc[s][0] = acc[s][1] = acc[s][2] = acc[s][3] = 0.0;
+
+    const int q0 = blk_begin[b], q1 = blk_end[b];
+    for (int base = q0; base < q1; base += THREADS) {
+        const int cnt = min(THREADS, q1 - base);
+        __syncthreads();
+        if (tid < cnt) {
+            const int p = static_cast<int>(keys[base + tid] & 0xffffffffull);
+            const int* pb = bbox + 6 * p;
+#pragma unroll
+            for (int a = 0; a < 6; ++a) A.bb[a][tid] = pb[a];
+            A.cb[tid] = coffs[p];
+            A.m[tid] = m[p];
+            A.v[0][tid] = vx[p];
+            A.v[1][tid] = vy[p];
+            A.v[2][tid] = vz[p];
         }
         __syncthreads();
-
-        // ---- phase C2: every thread adds its owned cells' Msum, batch particles in list order ----
-        for (int q = 0; q < nq; ++q) {
-            const int ly0 = L.q_lo[1][q], lz0 = L.q_lo[2][q];
-            const int jy = own_cy - ly0, kz = own_cz - lz0;
-            const int ny1 = L.q_n[1][q] - 1, nz1 = L.q_n[2][q] - 1;
-            if (jy < 0 || jy >= ny1 || kz < 0 || kz >= nz1) continue;
-            const int ix0 = own_cx0 - L.q_lo[0][q];
-            const int nx1 = L.q_n[0][q] - 1;
-            const double* Mq = &L.u.Ms[L.q_cl[q] + jy * nz1 + kz];
-            const int sxc = ny1 * nz1;
-            const double mm = L.q_m[q], v0 = L.q_v[0][q], v1 = L.q_v[1][q], v2 = L.q_v[2][q];
+        for (int g0 = 0; g0 < cnt; g0 += ACC_GROUP) {
+            // load the cell sums of a group of particles first (independent loads in flight) ...
+            double val[ACC_GROUP][ACC_CPT];
 #pragma unroll
-            for (int s = 0; s < S::CPT; ++s) {
-                const int i = ix0 + s;
-                if (i < 0 || i >= nx1) continue;
-                const double Ms = Mq[i * sxc];
-                if (Ms == 0.0) continue;
-                const double t = Ms * mm;
-                acc[s][0] += t;
-                acc[s][1] += t * v0;
-                acc[s][2] += t * v1;
-                acc[s][3] += t * v2;
+            for (int u = 0; u < ACC_GROUP; ++u) {
+                const int q = g0 + u;
+#pragma unroll
+                for (int s = 0; s < ACC_CPT; ++s) val[u][s] = 0.0;
+                if (q >= cnt) continue;
+                const int x0 = A.bb[0][q], x1 = A.bb[1][q], y0 = A.bb[2][q], y1 = A.bb[3][q], z0 = A.bb[4][q],
+                          z1 = A.bb[5][q];
+                if (gy < y0 || gy > y1 || gz < z0 || gz > z1) continue;
+                const int Ny = y1 - y0 + 1, Nz = z1 - z0 + 1;
+                const long long row = A.cb[q] + static_cast<long long>(gy - y0) * Nz + (gz - z0);
+#pragma unroll
+                for (int s = 0; s < ACC_CPT; ++s) {
+                    const int gx = gx0 + s;
+                    if (gx >= x0 && gx <= x1) val[u][s] = static_cast<double>(ms[row + static_cast<long long>(gx - x0) * Ny * Nz]);
+                }
+            }
+            // ... then add them in list order
+#pragma unroll
+            for (int u = 0; u < ACC_GROUP; ++u) {
+                const int q = g0 + u;
+                if (q >= cnt) break;
+                const double mm = A.m[q], v0 = A.v[0][q], v1 = A.v[1][q], v2 = A.v[2][q];
+#pragma unroll
+                for (int s = 0; s < ACC_CPT; ++s) {
+                    const double Ms = val[u][s];
+                    if (Ms == 0.0) continue;
+                    const double t = Ms * mm;
+                    acc[s][0] += t;
+                    acc[s][1] += t * v0;
+                    acc[s][2] += t * v1;
+                    acc[s][3] += t * v2;
+                }
             }
         }
-        __syncthreads();
     }
 
 #pragma unroll
-    for (int s = 0; s < S::CPT; ++s) {
-        const int ix = o[0] + own_cx0 + s, iy = o[1] + own_cy, iz = o[2] + own_cz;
-        if (ix >= g.nx || iy >= g.ny || iz >= g.nz) continue;
-        const int64_t c = (static_cast<int64_t>(ix) * g.ny + iy) * g.nz + iz;
+    for (int s = 0; s < ACC_CPT; ++s) {
+        const int ix = gx0 + s;
+        if (ix >= g.nx || gy >= g.ny || gz >= g.nz) continue;
+        const int64_t c = (static_cast<int64_t>(ix) * g.ny + gy) * g.nz + gz;
         double mass = acc[s][0], p0 = acc[s][1], p1 = acc[s][2], p2 = acc[s][3];
         if (!clamp_pairs && mass < 0.0) {
             // mixed mode clamps the cell's total instead of every pair: a negative total is rounding
@@ -628,25 +593,10 @@ __global__ __launch_bounds__(THREADS) void k_gather(
         out_py[c] = p1;
         out_pz[c] = p2;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        n_neg += __shfl_down(n_neg, off);
-        n_warn += __shfl_down(n_warn, off);
-        n_sphere += __shfl_down(n_sphere, off);
-        c_planes += __shfl_down(c_planes, off);
-        for (int r = 0; r < 3; ++r) c_pts[r] += __shfl_down(c_pts[r], off);
-    }
-    if ((tid & 63) == 0) {
-        if (n_neg) atomicAdd(&st->n_negative, n_neg);
-        if (n_warn) atomicAdd(&st->n_negative_warn, n_warn);
-        if (n_sphere) atomicAdd(&st->pairs_sphere, n_sphere);
-        if (c_planes) atomicAdd(&st->planes, c_planes);
-        for (int r = 0; r < 3; ++r)
-            if (c_pts[r]) atomicAdd(&st->pts[r], c_pts[r]);
-    }
 }
 
 // ------------------------------------------------------------------------------------------------
-// 8. deterministic velocity sums: fixed per-thread strides, fixed tree order
+// 9. deterministic velocity sums: fixed per-thread strides, fixed tree order
 // ------------------------------------------------------------------------------------------------
 constexpr int VSUM_BLOCKS = 256;
 
@@ -683,6 +633,7 @@ __global__ void k_vsum_final(const double* partial, double* out) {
     }
 }
 
+
 inline int grid1d(int64_t n) { return static_cast<int>((n + THREADS - 1) / THREADS); }
 
 }  // namespace
@@ -699,11 +650,10 @@ struct DevBuf {
 struct kiwi_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     std::string err;
-    int block_shape = 1;  // 0: 8x8x8 cells, 1: 16x16x8 cells (KIWI_HIP_BLOCK=8 or 16 overrides)
-    DevBuf parts, bbox, nblk, nlin, offs, loffs, keys_a, keys_b, cub_tmp, ranges, work, work_sorted, order_in,
-        order, lines, stats, outbuf, vsum;
+    DevBuf parts, bbox, sizes, offsets, keys_a, keys_b, cub_tmp, ranges, work, work_sorted, order_in, order, tasks,
+        lines, ms, stats, outbuf, vsum;
 };
 
 namespace {
@@ -739,14 +689,35 @@ hipError_t reserve(DevBuf& b, size_t bytes) {
     return hipSuccess;
 }
 
-template <typename R, int BX, int BY, int BZ>
-void launch_gather(int nblocks, hipStream_t s, const double* const* dp, const int* bbox, const uint64_t* keys,
-                   const int* blk_begin, const int* blk_end, const int* order, const int64_t* loffs, const void* lines,
-                   const GridGeom& g, double* const* om, DevStats* dst, int clamp) {
-    k_gather<R, BX, BY, BZ><<<nblocks, THREADS, 0, s>>>(dp[0], dp[1], dp[2], dp[3], dp[4], dp[5], dp[6], dp[7], bbox,
-                                                         keys, blk_begin, blk_end, order, loffs,
-                                                         static_cast<const Pair2<R>*>(lines), g, om[0], om[1], om[2],
-                                                         om[3], dst, clamp);
+// persistent grid: enough workgroups to fill every CU several times over, fewer for small jobs
+int persistent_grid(int64_t items, int per_block, int cap) {
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((items + per_block - 1) / per_block, cap)));
+}
+
+template <typename R>
+hipError_t launch_cells(hipStream_t s, int64_t n, int64_t ntasks, const double* const* dp, const int* bbox,
+                        const int64_t* loffs, const int64_t* coffs, const int2* tasks,
+                        const uint64_t* keys, const int* blk_begin, const int* blk_end, const int* order,
+                        int nblocks, const GridGeom& g, void* lines, void* ms, double* const* om, DevStats* dst,
+                        int clamp, hipEvent_t ev_lines, hipEvent_t ev_cells, hipEvent_t ev_accum) {
+    if (n > 0) {
+        k_lines<R><<<persistent_grid(n, THREADS / 64, 2048), THREADS, 0, s>>>(
+            n, dp[0], dp[1], dp[2], dp[7], bbox, loffs, g, static_cast<Pair2<R>*>(lines), dst);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if ((e = hipEventRecord(ev_lines, s)) != hipSuccess) return e;
+    if (ntasks > 0) {
+        k_particles<R><<<persistent_grid(ntasks, THREADS / 64, 4096), THREADS, 0, s>>>(
+            ntasks, tasks, dp[0], dp[1], dp[2], dp[7], bbox, loffs, coffs, g, static_cast<const Pair2<R>*>(lines),
+            static_cast<R*>(ms), dst, clamp);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipEventRecord(ev_cells, s)) != hipSuccess) return e;
+    k_accum<R><<<nblocks, THREADS, 0, s>>>(dp[3], dp[4], dp[5], dp[6], bbox, keys, blk_begin, blk_end, order, coffs,
+                                          static_cast<const R*>(ms), g, om[0], om[1], om[2], om[3], clamp);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return hipEventRecord(ev_accum, s);
 }
 
 }  // namespace
@@ -763,7 +734,6 @@ int kiwi_hip_create(kiwi_hip_ctx** out, int device_ordinal) {
     if (device_ordinal < 0 || device_ordinal >= ndev) return KIWI_ERR_INVALID_ARG;
     kiwi_hip_ctx* c = new kiwi_hip_ctx();
     c->device = device_ordinal;
-    if (const char* e = std::getenv("KIWI_HIP_BLOCK")) c->block_shape = (std::atoi(e) == 8) ? 0 : 1;
     if (hipSetDevice(device_ordinal) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -783,9 +753,9 @@ void kiwi_hip_destroy(kiwi_hip_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf* b : {&c->parts, &c->bbox, &c->nblk, &c->nlin, &c->offs, &c->loffs, &c->keys_a, &c->keys_b,
-                      &c->cub_tmp, &c->ranges, &c->work, &c->work_sorted, &c->order_in, &c->order, &c->lines,
-                      &c->stats, &c->outbuf, &c->vsum})
+    for (DevBuf* b : {&c->parts, &c->bbox, &c->sizes, &c->offsets, &c->keys_a, &c->keys_b, &c->cub_tmp, &c->ranges,
+                      &c->work, &c->work_sorted, &c->order_in, &c->order, &c->tasks, &c->lines, &c->ms, &c->stats,
+                      &c->outbuf, &c->vsum})
         if (b->p) (void)hipFree(b->p);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -816,11 +786,12 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     if (K->kernel_id != KIWI_KERNEL_M4 || K->generic_path || sup != 2.0 || K->tabulated)
         return fail(c, KIWI_ERR_UNSUPPORTED, "this build implements the analytic cubic-spline (M4, support 2h) path only");
     if (P->n > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 particles in one call");
+    if (G->nx > CH * CH_MAXAXIS || G->ny > CH * CH_MAXAXIS || G->nz > CH * CH_MAXAXIS)
+        return fail(c, KIWI_ERR_LIMIT, "more than 8192 cells along a grid axis");
 
     KIWI_HIP_CHECK(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const int64_t n = P->n;
-    const int bsx = c->block_shape ? 16 : 8, bsy = c->block_shape ? 16 : 8, bsz = 8;
     GridGeom g;
     g.x_min = G->x_min;
     g.y_min = G->y_min;
@@ -831,15 +802,11 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     g.nx = G->nx;
     g.ny = G->ny;
     g.nz = G->nz;
-    g.bsx = bsx;
-    g.bsy = bsy;
-    g.bsz = bsz;
-    g.nbx = (G->nx + bsx - 1) / bsx;
-    g.nby = (G->ny + bsy - 1) / bsy;
-    g.nbz = (G->nz + bsz - 1) / bsz;
+    g.nbx = (G->nx + BLK - 1) / BLK;
+    g.nby = (G->ny + BLK - 1) / BLK;
+    g.nbz = (G->nz + BLK - 1) / BLK;
     const int64_t nblocks = static_cast<int64_t>(g.nbx) * g.nby * g.nbz;
     const int64_t ncells = static_cast<int64_t>(G->nx) * G->ny * G->nz;
-    if (nblocks > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "grid too large");
 
     // particle arrays on the device
     const double* dp[8];
@@ -865,13 +832,13 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
         for (int k = 0; k < 4; ++k) om[k] = static_cast<double*>(c->outbuf.p) + k * ncells;
     }
 
+    // per-particle sizes (blocks, lines, tasks, cells) and their exclusive scans, (n + 1) each
+    const int64_t n1 = n + 1;
     KIWI_HIP_CHECK(c, reserve(c->stats, sizeof(DevStats)));
     DevStats* dst = static_cast<DevStats*>(c->stats.p);
     KIWI_HIP_CHECK(c, reserve(c->bbox, 6 * sizeof(int) * std::max<int64_t>(n, 1)));
-    KIWI_HIP_CHECK(c, reserve(c->nblk, sizeof(int64_t) * (n + 1)));
-    KIWI_HIP_CHECK(c, reserve(c->nlin, sizeof(int64_t) * (n + 1)));
-    KIWI_HIP_CHECK(c, reserve(c->offs, sizeof(int64_t) * (n + 1)));
-    KIWI_HIP_CHECK(c, reserve(c->loffs, sizeof(int64_t) * (n + 1)));
+    KIWI_HIP_CHECK(c, reserve(c->sizes, 4 * sizeof(int64_t) * n1));
+    KIWI_HIP_CHECK(c, reserve(c->offsets, 4 * sizeof(int64_t) * n1));
     KIWI_HIP_CHECK(c, reserve(c->ranges, 2 * sizeof(int) * nblocks));
     KIWI_HIP_CHECK(c, reserve(c->work, sizeof(unsigned long long) * nblocks));
     KIWI_HIP_CHECK(c, reserve(c->work_sorted, sizeof(unsigned long long) * nblocks));
@@ -879,10 +846,10 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     KIWI_HIP_CHECK(c, reserve(c->order, sizeof(int) * nblocks));
     KIWI_HIP_CHECK(c, reserve(c->vsum, sizeof(double) * (3 * VSUM_BLOCKS + 3)));
     int* bbox = static_cast<int*>(c->bbox.p);
-    int64_t* nblk = static_cast<int64_t*>(c->nblk.p);
-    int64_t* nlin = static_cast<int64_t*>(c->nlin.p);
-    int64_t* offs = static_cast<int64_t*>(c->offs.p);
-    int64_t* loffs = static_cast<int64_t*>(c->loffs.p);
+    int64_t* sz = static_cast<int64_t*>(c->sizes.p);
+    int64_t* of = static_cast<int64_t*>(c->offsets.p);
+    int64_t *nblk = sz, *nlin = sz + n1, *ntask = sz + 2 * n1, *ncell = sz + 3 * n1;
+    int64_t *offs = of, *loffs = of + n1, *toffs = of + 2 * n1, *coffs = of + 3 * n1;
     int* blk_begin = static_cast<int*>(c->ranges.p);
     int* blk_end = blk_begin + nblocks;
     unsigned long long* work = static_cast<unsigned long long*>(c->work.p);
@@ -894,33 +861,39 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     KIWI_HIP_CHECK(c, hipMemsetAsync(dst, 0, sizeof(DevStats), s));
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[0], s));
     const double support = 2.0;
-    k_prep<<<grid1d(n + 1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], support, g, bbox, nblk, nlin, dst);
+    k_prep<<<grid1d(n1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], support, g, bbox, nblk, nlin, ntask, ncell,
+                                         dst);
     KIWI_HIP_CHECK(c, hipGetLastError());
-    size_t tmp_bytes = 0, tmp2 = 0;
-    KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, nblk, offs, static_cast<int>(n + 1), s));
-    KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, nlin, loffs, static_cast<int>(n + 1), s));
-    KIWI_HIP_CHECK(c, reserve(c->cub_tmp, std::max(tmp_bytes, tmp2)));
-    tmp_bytes = c->cub_tmp.cap;
-    KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, tmp_bytes, nblk, offs, static_cast<int>(n + 1), s));
-    tmp_bytes = c->cub_tmp.cap;
-    KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, tmp_bytes, nlin, loffs, static_cast<int>(n + 1), s));
-    int64_t totals[2] = {0, 0};
-    KIWI_HIP_CHECK(c, hipMemcpyAsync(&totals[0], offs + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    KIWI_HIP_CHECK(c, hipMemcpyAsync(&totals[1], loffs + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    {
+        size_t tmp_bytes = 0;
+        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, nblk, offs, static_cast<int>(n1), s));
+        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, tmp_bytes));
+        for (int k = 0; k < 4; ++k) {
+            size_t tb = c->cub_tmp.cap;
+            KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, tb, sz + k * n1, of + k * n1,
+                                                               static_cast<int>(n1), s));
+        }
+    }
+    int64_t totals[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k)
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(&totals[k], of + k * n1 + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     KIWI_HIP_CHECK(c, hipStreamSynchronize(s));
-    const int64_t npairs = totals[0], nlines_total = totals[1];
+    const int64_t npairs = totals[0], nlines_total = totals[1], ntasks = totals[2], ncell_total = totals[3];
     if (npairs > 0x7fffffffLL)
         return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 (particle, cell-block) work items in one call");
 
-    const size_t line_bytes = (precision == KIWI_FP64 ? sizeof(Pair2<double>) : sizeof(Pair2<float>)) *
-                              static_cast<size_t>(std::max<int64_t>(nlines_total, 1));
-    KIWI_HIP_CHECK(c, reserve(c->lines, line_bytes));
+    const size_t rsz = (precision == KIWI_FP64) ? sizeof(double) : sizeof(float);
+    KIWI_HIP_CHECK(c, reserve(c->lines, 2 * rsz * static_cast<size_t>(std::max<int64_t>(nlines_total, 1))));
+    KIWI_HIP_CHECK(c, reserve(c->ms, rsz * static_cast<size_t>(std::max<int64_t>(ncell_total, 1))));
+    KIWI_HIP_CHECK(c, reserve(c->tasks, sizeof(int2) * static_cast<size_t>(std::max<int64_t>(ntasks, 1))));
     KIWI_HIP_CHECK(c, reserve(c->keys_a, sizeof(uint64_t) * std::max<int64_t>(npairs, 1)));
     KIWI_HIP_CHECK(c, reserve(c->keys_b, sizeof(uint64_t) * std::max<int64_t>(npairs, 1)));
     uint64_t* keys_a = static_cast<uint64_t*>(c->keys_a.p);
     uint64_t* keys_b = static_cast<uint64_t*>(c->keys_b.p);
+    int2* tasks = static_cast<int2*>(c->tasks.p);
     if (n > 0) {
         k_emit<<<grid1d(n), THREADS, 0, s>>>(n, bbox, offs, g, keys_a);
+        k_emit_tasks<<<grid1d(n), THREADS, 0, s>>>(n, bbox, toffs, tasks);
         KIWI_HIP_CHECK(c, hipGetLastError());
     }
     int end_bit = 32;
@@ -953,36 +926,16 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
                                                                         order, static_cast<int>(nblocks), 0, 64, s));
     }
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[1], s));
-    const int lgrid = static_cast<int>(std::min<int64_t>((n + 3) / 4, 2048));
-    if (n > 0) {
-        if (precision == KIWI_FP64)
-            k_lines<double><<<lgrid, THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], bbox, loffs, g,
-                                                      static_cast<Pair2<double>*>(c->lines.p), dst);
-        else
-            k_lines<float><<<lgrid, THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], bbox, loffs, g,
-                                                     static_cast<Pair2<float>*>(c->lines.p), dst);
-        KIWI_HIP_CHECK(c, hipGetLastError());
-    }
-    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[2], s));
-    const int nb = static_cast<int>(nblocks);
     const int clamp = (precision == KIWI_FP64) ? 1 : 0;
-    if (precision == KIWI_FP64) {
-        if (c->block_shape)
-            launch_gather<double, 16, 16, 8>(nb, s, dp, bbox, keys_b, blk_begin, blk_end, order, loffs, c->lines.p, g,
-                                              om, dst, clamp);
-        else
-            launch_gather<double, 8, 8, 8>(nb, s, dp, bbox, keys_b, blk_begin, blk_end, order, loffs, c->lines.p, g,
-                                            om, dst, clamp);
-    } else {
-        if (c->block_shape)
-            launch_gather<float, 16, 16, 8>(nb, s, dp, bbox, keys_b, blk_begin, blk_end, order, loffs, c->lines.p, g,
-                                             om, dst, clamp);
-        else
-            launch_gather<float, 8, 8, 8>(nb, s, dp, bbox, keys_b, blk_begin, blk_end, order, loffs, c->lines.p, g,
-                                           om, dst, clamp);
-    }
-    KIWI_HIP_CHECK(c, hipGetLastError());
-    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[3], s));
+    const int nb = static_cast<int>(nblocks);
+    if (precision == KIWI_FP64)
+        KIWI_HIP_CHECK(c, launch_cells<double>(s, n, ntasks, dp, bbox, loffs, coffs, tasks, keys_b, blk_begin,
+                                               blk_end, order, nb, g, c->lines.p, c->ms.p, om, dst, clamp, c->ev[2],
+                                               c->ev[3], c->ev[4]));
+    else
+        KIWI_HIP_CHECK(c, launch_cells<float>(s, n, ntasks, dp, bbox, loffs, coffs, tasks, keys_b, blk_begin,
+                                              blk_end, order, nb, g, c->lines.p, c->ms.p, om, dst, clamp, c->ev[2],
+                                              c->ev[3], c->ev[4]));
     k_vsum_partial<<<VSUM_BLOCKS, THREADS, 0, s>>>(n, dp[3], dp[4], dp[5], vpart);
     k_vsum_final<<<1, 64, 0, s>>>(vpart, vpart + 3 * VSUM_BLOCKS);
     KIWI_HIP_CHECK(c, hipGetLastError());
@@ -1005,14 +958,17 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
         stats->pairs_bbox = static_cast<int64_t>(hs.pairs_bbox);
         stats->pairs_sphere = static_cast<int64_t>(hs.pairs_sphere);
         stats->particle_blocks = npairs;
+        stats->chunk_tasks = ntasks;
         for (int k = 0; k < 3; ++k) stats->v_mean[k] = n > 0 ? vs[k] / static_cast<double>(n) : 0.0;
-        float ms_prep = 0, ms_lines = 0, ms_gather = 0;
+        float ms_prep = 0, ms_lines = 0, ms_cells = 0, ms_accum = 0;
         (void)hipEventElapsedTime(&ms_prep, c->ev[0], c->ev[1]);
         (void)hipEventElapsedTime(&ms_lines, c->ev[1], c->ev[2]);
-        (void)hipEventElapsedTime(&ms_gather, c->ev[2], c->ev[3]);
+        (void)hipEventElapsedTime(&ms_cells, c->ev[2], c->ev[3]);
+        (void)hipEventElapsedTime(&ms_accum, c->ev[3], c->ev[4]);
         stats->device_ms_prep = ms_prep;
         stats->device_ms_lines = ms_lines;
-        stats->device_ms_gather = ms_gather;
+        stats->device_ms_gather = ms_cells;
+        stats->device_ms_accum = ms_accum;
         for (int r = 0; r < 3; ++r) {
             stats->lattice_points[r] = static_cast<int64_t>(hs.pts[r]);
             stats->line_evals[r] = static_cast<int64_t>(hs.lines[r]);
