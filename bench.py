@@ -148,7 +148,7 @@ def main():
         "mixed_total_mass_rel_err": mass_err,
         "mixed_field_err_over_max": field_err,
         "pairs_sphere": pairs, "pairs_bbox": st32.pairs_bbox, "particle_blocks": st32.particle_blocks,
-        "chunk_tasks": st32.chunk_tasks, "cells_ms": g32, "prep_ms": st32.device_ms_prep,
+        "cells_ms": g32, "prep_ms": st32.device_ms_prep,
         "lines_ms": st32.device_ms_lines, "accum_ms": st32.device_ms_accum,
         "lines_tflops": line_kernel_flops(st32, fpu) / max(st32.device_ms_lines * 1e-3, 1e-12) / 1e12,
         "work": {"lattice_points": list(st32.lattice_points), "line_evals": list(st32.line_evals),

@@ -95,8 +95,7 @@ typedef struct kiwi_stats {
     int64_t n_negative_warn;  /* of which Msum <= -0.01 */
     int64_t pairs_bbox;       /* (particle, cell) pairs in the clamped bounding boxes */
     int64_t pairs_sphere;     /* pairs whose cell intersects the support sphere (the throughput unit) */
-    int64_t particle_blocks;  /* (particle, 8^3 cell-block) accumulation items */
-    int64_t chunk_tasks;      /* (particle, chunk of <= 8^3 cells) cell-integral tasks */
+    int64_t particle_blocks;  /* (particle, 8^3 cell block) work items */
     double v_mean[3];         /* mean velocity over all particles */
     double device_ms_prep;    /* binning, sorts, block order */
     double device_ms_lines;   /* per-particle edge-line constants */

@@ -2,26 +2,30 @@
 //
 // sph-to-grid pipeline (one call of kiwi_hip_sph_to_grid, all on the context's stream):
 //   1. k_prep        one thread per particle: the reference's bounding box of cells
-//                    (src/cartesian_grid.cpp:209-238, FP64 floor, clamped) and the sizes of the
-//                    particle's work: cell blocks overlapped, edge lines, chunks, cells; counters.
+//                    (src/cartesian_grid.cpp:209-238, FP64 floor, clamped), the number of 8^3 cell
+//                    blocks it overlaps and of edge lines of its lattice; counters.
 //   2. scans         exclusive prefix sums of those sizes (hipcub).
-//   3. k_emit        one 64-bit key (block << 32 | particle) per (particle, cell block) pair;
-//      k_emit_tasks  one task per (particle, chunk of at most 8^3 cells of its own bounding box).
-//   4. sort          radix sort of the keys: every block's particle list comes out in particle order,
-//                    so each cell adds its particles in the reference's single-thread order.
-//   5. k_ranges      [begin, end) of every block's list; k_block_work + a second sort give the
-//                    blocks in order of decreasing work (longest-processing-time-first).
-//   6. k_lines       one wavefront per particle: the region-matching constants (D2, D3) of every
-//                    edge line of the particle's lattice, computed once per particle into HBM.
-//   7. k_particles   one wavefront per chunk task: wall-plane constants and the lattice values W
-//                    (m4_lattice.h) of the chunk are built in the wave's LDS, then the 8-corner
-//                    stencil gives Msum of every cell of the chunk, written to the particle's slab
-//                    of cell sums in HBM (x-major over its bounding box).  Everything a wave touches
-//                    is its own: no workgroup barriers.
+//   3. k_emit        one item per (particle, cell block) pair, as the 64-bit key block << 32 | particle,
+//                    written in particle order.
+//   4. sort          radix sort of the keys by block (hipcub, stable): every block's item list comes
+//                    out in particle order, so each cell adds its particles in the reference's
+//                    single-thread order.  The sort carries the item's particle-order position along.
+//   5. k_item_cells  cells of every item (its box = particle box intersected with the block); their
+//                    prefix sum in block order places each item's cell sums in one contiguous slab per
+//                    block; k_scatter hands every item its slab offset in particle order.
+//      k_ranges, k_block_work + sort: [begin, end) of every block's items and the blocks in order of
+//                    decreasing work (longest-processing-time-first) for the accumulation.
+//   6. k_lines       one wavefront per particle: the region-matching constants (D2, D3) of every edge
+//                    line of the particle's lattice, once per particle, into HBM.
+//   7. k_particles   wavefronts take items in particle order from a shared cursor; per item: wall-plane
+//                    constants and per-axis distance gaps in the wave's LDS, a compacted list of the
+//                    lattice points any in-sphere cell needs, their lattice values W (m4_lattice.h),
+//                    then the 8-corner stencil gives Msum of every cell, stored to the item's slab.
+//                    Everything a wave touches in LDS is its own: no workgroup barriers.
 //   8. k_accum       one workgroup per 8^3 cell block, blocks in work order: every thread owns two
-//                    cells and adds Msum m and Msum m v of the block's particles, read from their
-//                    slabs, in list order.  Each cell is owned by exactly one thread: no float
-//                    atomics, and the results are bit-reproducible.
+//                    cells and adds Msum m and Msum m v of the block's items, read from the block's
+//                    contiguous slab, in list order.  Each cell is owned by exactly one thread: no
+//                    float atomics, and the results are bit-reproducible.
 //   9. k_vsum        deterministic mean velocity over all particles (the empty-cell fallback).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -42,23 +46,24 @@ using namespace kiwi_hip;
 namespace {
 
 constexpr int THREADS = 256;
-constexpr int BLK = 8;                  // accumulation block edge (cells)
-constexpr int CH = 8;                   // particle chunk edge (cells)
-constexpr int CHL = CH + 1;             // lattice points per chunk edge
-constexpr int CH_PTS = CHL * CHL * CHL;
-constexpr int CH_MAXAXIS = 1024;        // chunk index range per axis (grids up to 8192 cells per axis)
+constexpr int WAVES = THREADS / 64;
+constexpr int BLK = 8;                        // cell block edge (cells)
+constexpr int BLK_PTS = (BLK + 1) * (BLK + 1) * (BLK + 1);
+constexpr int MAX_AXIS = 8192;                // grid cells per axis
+constexpr unsigned ITEM_GRAB = 8;             // items a wavefront takes from the cursor at once
 
 struct DevStats {
     unsigned long long skipped_outside, n_negative, n_negative_warn, pairs_bbox, pairs_sphere;
     unsigned long long n_warn;  // number of warn records claimed
     unsigned long long pts[3], lines[3], planes;
+    unsigned long long cursor;  // next unclaimed item of k_particles
     kiwi_warn_record warn[KIWI_MAX_WARN_RECORDS];
 };
 
 struct GridGeom {
     double x_min, y_min, z_min, dx, dy, dz;
     int nx, ny, nz;
-    int nbx, nby, nbz;  // accumulation blocks per axis
+    int nbx, nby, nbz;  // cell blocks per axis
 };
 
 template <typename R>
@@ -109,17 +114,15 @@ __device__ __forceinline__ int cell_floor(double v, int n) {
 __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                        const double* __restrict__ z, const double* __restrict__ h, double support_in_h,
                        GridGeom g, int* __restrict__ bbox, int64_t* __restrict__ nblk, int64_t* __restrict__ nlines,
-                       int64_t* __restrict__ ntask, int64_t* __restrict__ ncell, DevStats* st) {
+                       DevStats* st) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     unsigned long long skipped = 0, pairs = 0;
     if (i == n) {  // trailing zeros of the exclusive scans
         nblk[n] = 0;
         nlines[n] = 0;
-        ntask[n] = 0;
-        ncell[n] = 0;
     }
     if (i < n) {
-        int64_t nb = 0, nl = 0, nt = 0, nc = 0;
+        int64_t nb = 0, nl = 0;
         int* bb = bbox + 6 * i;
         bb[0] = 1;
         bb[1] = 0;  // empty unless set below
@@ -149,18 +152,14 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
                 bb[4] = iz0;
                 bb[5] = iz1;
                 const int64_t Nx = ix1 - ix0 + 1, Ny = iy1 - iy0 + 1, Nz = iz1 - iz0 + 1;
-                nc = Nx * Ny * Nz;
-                pairs = static_cast<unsigned long long>(nc);
+                pairs = static_cast<unsigned long long>(Nx * Ny * Nz);
                 nb = static_cast<int64_t>(ix1 / BLK - ix0 / BLK + 1) * (iy1 / BLK - iy0 / BLK + 1) *
                      (iz1 / BLK - iz0 / BLK + 1);
                 nl = 2 * ((Nx + 1) * (Ny + 1) + (Nx + 1) * (Nz + 1) + (Ny + 1) * (Nz + 1));
-                nt = ((Nx + CH - 1) / CH) * ((Ny + CH - 1) / CH) * ((Nz + CH - 1) / CH);
             }
         }
         nblk[i] = nb;
         nlines[i] = nl;
-        ntask[i] = nt;
-        ncell[i] = nc;
     }
     for (int off = 32; off > 0; off >>= 1) {
         skipped += __shfl_down(skipped, off);
@@ -173,7 +172,7 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
 }
 
 // ------------------------------------------------------------------------------------------------
-// 3. keys and chunk tasks
+// 3. items: (particle, cell block) keys in particle order
 // ------------------------------------------------------------------------------------------------
 __global__ void k_emit(int64_t n, const int* __restrict__ bbox, const int64_t* __restrict__ offs, GridGeom g,
                        uint64_t* __restrict__ keys) {
@@ -190,23 +189,48 @@ __global__ void k_emit(int64_t n, const int* __restrict__ bbox, const int64_t* _
             }
 }
 
-// task = (particle, chunk): chunk indices packed 10 bits per axis
-__global__ void k_emit_tasks(int64_t n, const int* __restrict__ bbox, const int64_t* __restrict__ toffs,
-                             int2* __restrict__ tasks) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int* bb = bbox + 6 * i;
-    if (bb[0] > bb[1]) return;
-    const int cx = (bb[1] - bb[0] + CH) / CH, cy = (bb[3] - bb[2] + CH) / CH, cz = (bb[5] - bb[4] + CH) / CH;
-    int64_t o = toffs[i];
-    for (int a = 0; a < cx; ++a)
-        for (int b = 0; b < cy; ++b)
-            for (int c = 0; c < cz; ++c) tasks[o++] = make_int2(static_cast<int>(i), (a << 20) | (b << 10) | c);
+// ------------------------------------------------------------------------------------------------
+// 5. item boxes, slab offsets, block ranges, work per block
+// ------------------------------------------------------------------------------------------------
+struct ItemBox {
+    int x0, y0, z0;  // first cell of the item (grid coordinates)
+    int mx, my, mz;  // cells per axis, 1..8
+};
+
+// the item's cells: the particle's box intersected with the block
+__device__ __forceinline__ ItemBox item_box(const int* bb, uint32_t b, const GridGeom& g) {
+    const int bz = static_cast<int>(b % static_cast<uint32_t>(g.nbz));
+    const uint32_t bxy = b / static_cast<uint32_t>(g.nbz);
+    const int by = static_cast<int>(bxy % static_cast<uint32_t>(g.nby));
+    const int bx = static_cast<int>(bxy / static_cast<uint32_t>(g.nby));
+    ItemBox r;
+    r.x0 = max(bb[0], bx * BLK);
+    r.y0 = max(bb[2], by * BLK);
+    r.z0 = max(bb[4], bz * BLK);
+    r.mx = min(bb[1], bx * BLK + BLK - 1) - r.x0 + 1;
+    r.my = min(bb[3], by * BLK + BLK - 1) - r.y0 + 1;
+    r.mz = min(bb[5], bz * BLK + BLK - 1) - r.z0 + 1;
+    return r;
 }
 
-// ------------------------------------------------------------------------------------------------
-// 5. block ranges in the sorted key array; work per block for the processing order
-// ------------------------------------------------------------------------------------------------
+// cells of every item in block order (the sizes whose prefix sum gives the slab offsets)
+__global__ void k_item_cells(int64_t npairs, const uint64_t* __restrict__ keys, const int* __restrict__ bbox,
+                             GridGeom g, int64_t* __restrict__ cells) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i == npairs) cells[npairs] = 0;
+    if (i >= npairs) return;
+    const uint64_t key = keys[i];
+    const ItemBox r = item_box(bbox + 6 * (key & 0xffffffffull), static_cast<uint32_t>(key >> 32), g);
+    cells[i] = static_cast<int64_t>(r.mx * r.my * r.mz);
+}
+
+// slab offset of every item at its particle-order position
+__global__ void k_scatter(int64_t npairs, const int* __restrict__ pos, const int64_t* __restrict__ off_sorted,
+                          int64_t* __restrict__ off_by_particle) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < npairs) off_by_particle[pos[i]] = off_sorted[i];
+}
+
 __global__ void k_fill_int(int* p, int64_t n, int v) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -231,11 +255,10 @@ __global__ void k_ranges(int64_t npairs, const uint64_t* __restrict__ keys, int*
     if (i == npairs - 1 || static_cast<uint32_t>(keys[i + 1] >> 32) != b) end[b] = static_cast<int>(i + 1);
 }
 
-// cells of a (particle, block) pair plus a fixed per-pair cost: the work estimate for the block
-// order.  The keys are sorted by block, so equal blocks form runs of lanes; each run adds its total
-// with one atomic.
-__global__ void k_block_work(int64_t npairs, const uint64_t* __restrict__ keys, const int* __restrict__ bbox,
-                             GridGeom g, unsigned long long* __restrict__ work) {
+// cells of an item plus a fixed per-item cost: the work estimate for the block order.  The keys are
+// sorted by block, so equal blocks form runs of lanes; each run adds its total with one atomic.
+__global__ void k_block_work(int64_t npairs, const uint64_t* __restrict__ keys, const int64_t* __restrict__ cells,
+                             unsigned long long* __restrict__ work) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool valid = i < npairs;
@@ -243,13 +266,7 @@ __global__ void k_block_work(int64_t npairs, const uint64_t* __restrict__ keys, 
     unsigned long long w = 0;
     if (valid) {
         b = static_cast<uint32_t>(keys[i] >> 32);
-        const int p = static_cast<int>(keys[i] & 0xffffffffull);
-        const int bz = b % g.nbz, by = (b / g.nbz) % g.nby, bx = b / (g.nbz * g.nby);
-        const int* bb = bbox + 6 * p;
-        const int nx = min(bb[1], bx * BLK + BLK - 1) - max(bb[0], bx * BLK) + 1;
-        const int ny = min(bb[3], by * BLK + BLK - 1) - max(bb[2], by * BLK) + 1;
-        const int nz = min(bb[5], bz * BLK + BLK - 1) - max(bb[4], bz * BLK) + 1;
-        w = static_cast<unsigned long long>(nx * ny * nz + 32);
+        w = static_cast<unsigned long long>(cells[i] + 32);
     }
     for (int off = 1; off < 64; off <<= 1) {
         const unsigned long long wv = __shfl_up(w, off);
@@ -321,155 +338,192 @@ __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __re
 }
 
 // ------------------------------------------------------------------------------------------------
-// 7. lattice values and cell sums, one wavefront per (particle, chunk)
+// 7. lattice values and cell sums, one wavefront per item
 // ------------------------------------------------------------------------------------------------
+// Per-wave LDS.  Axis arrays hold the item's lattice coordinates x..., y..., z... (n_x + n_y + n_z
+// <= 27 entries, n = cells + 1 per axis).
 template <typename R>
 struct WaveLds {
-    R sd[3 * CHL];     // signed offsets (lattice - particle) / h: x..., y..., z...
-    R pc[3][3 * CHL];  // plane constants c1, c2, c3 of |sd|
-    R U[CH_PTS];       // signed lattice values s * W, (i, j, k) k fastest
+    R sd[3 * (BLK + 1)];      // signed offsets (lattice - particle) / h
+    R pc[3][3 * (BLK + 1)];   // plane constants c1, c2, c3 of |sd|
+    R pg2[3 * (BLK + 1)];     // squared gap to the cells on either side of a lattice plane (point test)
+    R cg2[3 * (BLK + 1)];     // squared gap to the cell slab [u, u+1] (cell test; last entry unused)
+    uint32_t list[BLK_PTS];   // needed points: u | i << 10 | j << 14 | k << 18; inside 2h from the
+                              // front, outside from the back
+    R U[BLK_PTS];             // signed lattice values s * W, (i, j, k) k fastest
 };
+
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 template <typename R>
 __global__ __launch_bounds__(THREADS) void k_particles(
-    int64_t ntasks, const int2* __restrict__ tasks, const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ z, const double* __restrict__ h, const int* __restrict__ bbox,
-    const int64_t* __restrict__ loffs, const int64_t* __restrict__ coffs, GridGeom g,
+    uint32_t npairs, const uint64_t* __restrict__ keys, const int64_t* __restrict__ slab,
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ h, const int* __restrict__ bbox, const int64_t* __restrict__ loffs, GridGeom g,
     const Pair2<R>* __restrict__ lines, R* __restrict__ ms, DevStats* st, int clamp_pairs) {
-    __shared__ WaveLds<R> WL[THREADS / 64];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ WaveLds<R> WL[WAVES];
+    const int wave = static_cast<int>(wave_uniform(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
     WaveLds<R>& L = WL[wave];
-    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (THREADS / 64);
     unsigned long long n_neg = 0, n_warn = 0, n_sphere = 0, c_planes = 0;
     unsigned long long c_pts[3] = {0, 0, 0};
+    const unsigned long long lane_lt = (1ull << lane) - 1ull;
 
-    for (int64_t t = static_cast<int64_t>(blockIdx.x) * (THREADS / 64) + wave; t < ntasks; t += nwaves) {
-        const int2 tk = tasks[t];
-        const int p = tk.x;
-        const int* bb = bbox + 6 * p;
-        const int bx0 = bb[0], by0 = bb[2], bz0 = bb[4];
-        const int Nx = bb[1] - bb[0] + 1, Ny = bb[3] - bb[2] + 1, Nz = bb[5] - bb[4] + 1;  // particle cells
-        const int Lx = Nx + 1, Ly = Ny + 1, Lz = Nz + 1;                                     // particle lattice
-        // the particle's box is cut into ceil(N / CH) near-equal chunks per axis
-        const int kx = (Nx + CH - 1) / CH, ky = (Ny + CH - 1) / CH, kz = (Nz + CH - 1) / CH;
-        const int ax = (tk.y >> 20) & 1023, ay = (tk.y >> 10) & 1023, az = tk.y & 1023;
-        const int c0x = ax * Nx / kx, c0y = ay * Ny / ky, c0z = az * Nz / kz;
-        const int mx = (ax + 1) * Nx / kx - c0x, my = (ay + 1) * Ny / ky - c0y, mz = (az + 1) * Nz / kz - c0z;
-        const int nx = mx + 1, ny = my + 1, nz = mz + 1;                                     // chunk lattice
-        const double inv_h = 1.0 / h[p];
-        const double px = x[p], py = y[p], pz = z[p];
-        const Pair2<R>* ltab = lines + loffs[p];
-        const int64_t cbase = coffs[p];
+    uint32_t cur = 0;
+    if (lane == 0) cur = static_cast<uint32_t>(atomicAdd(&st->cursor, static_cast<unsigned long long>(ITEM_GRAB)));
+    cur = wave_uniform(cur);
+    while (cur < npairs) {
+        uint32_t nxt = 0;
+        if (lane == 0) nxt = static_cast<uint32_t>(atomicAdd(&st->cursor, static_cast<unsigned long long>(ITEM_GRAB)));
+        const uint32_t tend = min(cur + ITEM_GRAB, npairs);
+        for (uint32_t t = cur; t < tend; ++t) {
+            const uint64_t key = keys[t];
+            const uint32_t p = static_cast<uint32_t>(key & 0xffffffffull);
+            const int* bb = bbox + 6 * static_cast<size_t>(p);
+            const ItemBox ib = item_box(bb, static_cast<uint32_t>(key >> 32), g);
+            const int mx = ib.mx, my = ib.my, mz = ib.mz;
+            const int nx = mx + 1, ny = my + 1, nz = mz + 1;  // item lattice
+            const int Lx = bb[1] - bb[0] + 2, Ly = bb[3] - bb[2] + 2, Lz = bb[5] - bb[4] + 2;  // particle lattice
+            const int ox = ib.x0 - bb[0], oy = ib.y0 - bb[2], oz = ib.z0 - bb[4];
+            const double inv_h = 1.0 / h[p];
+            const double px = x[p], py = y[p], pz = z[p];
+            const Pair2<R>* ltab = lines + loffs[p];
+            R* out = ms + slab[t];
 
-        // ---- plane constants of the chunk's lattice coordinates ----
-        if (lane < nx + ny + nz) {
-            int a = 0, u = lane;
-            if (u >= nx) {
-                u -= nx;
-                a = 1;
-                if (u >= ny) {
-                    u -= ny;
-                    a = 2;
-                }
-            }
-            const int gi = sel3(a, bx0 + c0x, by0 + c0y, bz0 + c0z) + u;
-            const double d = (gmin(g, a) + gi * gstep(g, a) - sel3(a, px, py, pz)) * inv_h;
-            const R dr = static_cast<R>(d);
-            const M4Plane<R> pl = m4_plane<R>(dr < R(0) ? -dr : dr);
-            L.sd[lane] = dr;
-            L.pc[0][lane] = pl.c1;
-            L.pc[1][lane] = pl.c2;
-            L.pc[2][lane] = pl.c3;
-            ++c_planes;
-        }
-        wave_sync();
-
-        // ---- lattice values ----
-        const int nyz = ny * nz;
-        const float inv_nyz = 1.0f / static_cast<float>(nyz), inv_nz = 1.0f / static_cast<float>(nz);
-        // global line-table bases of the six pair blocks
-        const int64_t pxy = 0, pxz = pxy + static_cast<int64_t>(Lx) * Ly, pyx = pxz + static_cast<int64_t>(Lx) * Lz,
-                      pyz = pyx + static_cast<int64_t>(Ly) * Lx, pzx = pyz + static_cast<int64_t>(Ly) * Lz,
-                      pzy = pzx + static_cast<int64_t>(Lz) * Lx;
-        const int npts = nx * nyz;
-        for (int u = lane; u < npts; u += 64) {
-            const int i = small_div(u, nyz, inv_nyz);
-            const int rem = u - i * nyz;
-            const int j = small_div(rem, nz, inv_nz);
-            const int k = rem - j * nz;
-            const R dx = L.sd[i], dy = L.sd[nx + j], dz = L.sd[nx + ny + k];
-            R U = R(0);
-            if (dx != R(0) && dy != R(0) && dz != R(0)) {
-                // skip points that no cell of this chunk reaching the support sphere uses
-                const R ex = gap(L.sd[max(i - 1, 0)], L.sd[min(i + 1, nx - 1)]);
-                const R ey = gap(L.sd[nx + max(j - 1, 0)], L.sd[nx + min(j + 1, ny - 1)]);
-                const R ez = gap(L.sd[nx + ny + max(k - 1, 0)], L.sd[nx + ny + min(k + 1, nz - 1)]);
-                if (ex * ex + ey * ey + ez * ez < R(4)) {
-                    const R ax = dx < R(0) ? -dx : dx, ay = dy < R(0) ? -dy : dy, az = dz < R(0) ? -dz : dz;
-                    const R r2 = ax * ax + ay * ay + az * az;
-                    const R r = KMath<R>::sqrt_(r2);
-                    const int reg = m4_region(r2);
-                    ++c_pts[reg - 1];
-                    const R csum = L.pc[reg - 1][i] + L.pc[reg - 1][nx + j] + L.pc[reg - 1][nx + ny + k];
-                    R dsum = R(0);
-                    if (reg != 1) {
-                        const int gi = c0x + i, gj = c0y + j, gk = c0z + k;  // particle-lattice coordinates
-                        const Pair2<R> l0 = ltab[pxy + static_cast<int64_t>(gi) * Ly + gj];
-                        const Pair2<R> l1 = ltab[pxz + static_cast<int64_t>(gi) * Lz + gk];
-                        const Pair2<R> l2 = ltab[pyx + static_cast<int64_t>(gj) * Lx + gi];
-                        const Pair2<R> l3 = ltab[pyz + static_cast<int64_t>(gj) * Lz + gk];
-                        const Pair2<R> l4 = ltab[pzx + static_cast<int64_t>(gk) * Lx + gi];
-                        const Pair2<R> l5 = ltab[pzy + static_cast<int64_t>(gk) * Ly + gj];
-                        dsum = (reg == 2) ? (((l0.a + l1.a) + (l2.a + l3.a)) + (l4.a + l5.a))
-                                          : (((l0.b + l1.b) + (l2.b + l3.b)) + (l4.b + l5.b));
+            // ---- per-axis constants of the item's lattice planes ----
+            if (lane < nx + ny + nz) {
+                int a = 0, u = lane, n_a = nx;
+                if (u >= nx) {
+                    u -= nx;
+                    a = 1;
+                    n_a = ny;
+                    if (u >= ny) {
+                        u -= ny;
+                        a = 2;
+                        n_a = nz;
                     }
-                    const R W = m4_lattice_free<R>(ax, ay, az, r, reg) * M4Consts<R>::inv_pi + R(0.5) * csum + dsum;
-                    const bool neg = (dx < R(0)) != ((dy < R(0)) != (dz < R(0)));
-                    U = neg ? -W : W;
                 }
+                // offsets formed as the reference forms vertex coordinates: (x_min + i dx) - x_p
+                const int g0 = sel3(a, ib.x0, ib.y0, ib.z0);
+                const double lo = gmin(g, a), pa = sel3(a, px, py, pz), st_a = gstep(g, a);
+                const double d = (lo + (g0 + u) * st_a - pa) * inv_h;
+                const double dm = (lo + (g0 + max(u - 1, 0)) * st_a - pa) * inv_h;
+                const double dp = (lo + (g0 + min(u + 1, n_a - 1)) * st_a - pa) * inv_h;
+                const double dn = (lo + (g0 + u + 1) * st_a - pa) * inv_h;
+                const R dr = static_cast<R>(d);
+                const M4Plane<R> pl = m4_plane<R>(dr < R(0) ? -dr : dr);
+                const R gp = static_cast<R>(gap(dm, dp)), gc = static_cast<R>(gap(d, dn));
+                L.sd[lane] = dr;
+                L.pc[0][lane] = pl.c1;
+                L.pc[1][lane] = pl.c2;
+                L.pc[2][lane] = pl.c3;
+                L.pg2[lane] = gp * gp;
+                L.cg2[lane] = gc * gc;
+                ++c_planes;
             }
-            L.U[u] = U;
-        }
-        wave_sync();
+            wave_sync();
 
-        // ---- cell sums: the 8-corner stencil, written to the particle's slab ----
-        const int myz = my * mz;
-        const float inv_myz = 1.0f / static_cast<float>(myz), inv_mz = 1.0f / static_cast<float>(mz);
-        const int ncells = mx * myz;
-        for (int u = lane; u < ncells; u += 64) {
-            const int i = small_div(u, myz, inv_myz);
-            const int rem = u - i * myz;
-            const int j = small_div(rem, mz, inv_mz);
-            const int k = rem - j * mz;
-            const R ex = gap(L.sd[i], L.sd[i + 1]);
-            const R ey = gap(L.sd[nx + j], L.sd[nx + j + 1]);
-            const R ez = gap(L.sd[nx + ny + k], L.sd[nx + ny + k + 1]);
-            double Ms = 0.0;
-            if (ex * ex + ey * ey + ez * ez < R(4)) {
-                ++n_sphere;
-                const R* U0 = &L.U[(i * ny + j) * nz + k];
-                Ms = static_cast<double>(U0[nyz]) + static_cast<double>(U0[nz]) + static_cast<double>(U0[1]) +
-                     static_cast<double>(U0[nyz + nz + 1]) - static_cast<double>(U0[0]) -
-                     static_cast<double>(U0[nyz + nz]) - static_cast<double>(U0[nyz + 1]) -
-                     static_cast<double>(U0[nz + 1]);
-                if (clamp_pairs && Ms < 0.0) {
-                    ++n_neg;
-                    if (Ms <= -0.01) {
-                        ++n_warn;
-                        const unsigned long long slot = atomicAdd(&st->n_warn, 1ull);
-                        if (slot < KIWI_MAX_WARN_RECORDS) {
-                            st->warn[slot].particle = p;
-                            st->warn[slot].cell = (static_cast<int64_t>(bx0 + c0x + i) * g.ny + (by0 + c0y + j)) * g.nz +
-                                                  (bz0 + c0z + k);
-                            st->warn[slot].value = Ms;
+            // ---- classify the lattice points: zero planes, not needed, inside 2h, outside 2h ----
+            const int nyz = ny * nz, npts = nx * nyz;
+            const float inv_nyz = 1.0f / static_cast<float>(nyz), inv_nz = 1.0f / static_cast<float>(nz);
+            uint32_t nA = 0, nB = 0;
+            for (int u0 = 0; u0 < npts; u0 += 64) {
+                const int u = u0 + lane;
+                const int i = small_div(u, nyz, inv_nyz);
+                const int rem = u - i * nyz;
+                const int j = small_div(rem, nz, inv_nz);
+                const int k = rem - j * nz;
+                bool isA = false, isB = false;
+                if (u < npts) {
+                    const R dx = L.sd[i], dy = L.sd[nx + j], dz = L.sd[nx + ny + k];
+                    if (dx == R(0) || dy == R(0) || dz == R(0)) {
+                        L.U[u] = R(0);  // every pyramid anchored in a plane through the particle is flat
+                    } else if (L.pg2[i] + L.pg2[nx + j] + L.pg2[nx + ny + k] < R(4)) {
+                        const R r2 = dx * dx + dy * dy + dz * dz;
+                        isA = r2 < R(4);
+                        isB = !isA;
+                    }
+                }
+                const unsigned long long bA = __ballot(isA), bB = __ballot(isB);
+                const uint32_t code = static_cast<uint32_t>(u) | (static_cast<uint32_t>(i) << 10) |
+                                      (static_cast<uint32_t>(j) << 14) | (static_cast<uint32_t>(k) << 18);
+                if (isA) L.list[nA + __popcll(bA & lane_lt)] = code;
+                if (isB) L.list[BLK_PTS - 1 - (nB + __popcll(bB & lane_lt))] = code;
+                nA += __popcll(bA);
+                nB += __popcll(bB);
+            }
+            wave_sync();
+
+            // ---- lattice values of the needed points (inside 2h first, then outside) ----
+            const int pxy = 0, pxz = Lx * Ly, pyx = pxz + Lx * Lz, pyz = pyx + Ly * Lx, pzx = pyz + Ly * Lz,
+                      pzy = pzx + Lz * Lx;
+            const uint32_t ntot = nA + nB;
+            for (uint32_t v = lane; v < ntot; v += 64) {
+                const uint32_t code = (v < nA) ? L.list[v] : L.list[BLK_PTS - 1 - (v - nA)];
+                const int u = static_cast<int>(code & 1023u);
+                const int i = static_cast<int>((code >> 10) & 15u), j = static_cast<int>((code >> 14) & 15u),
+                          k = static_cast<int>((code >> 18) & 15u);
+                const R dx = L.sd[i], dy = L.sd[nx + j], dz = L.sd[nx + ny + k];
+                const R ax = dx < R(0) ? -dx : dx, ay = dy < R(0) ? -dy : dy, az = dz < R(0) ? -dz : dz;
+                const R r2 = ax * ax + ay * ay + az * az;
+                const R r = KMath<R>::sqrt_(r2);
+                const int reg = m4_region(r2);
+                ++c_pts[reg - 1];
+                const R csum = L.pc[reg - 1][i] + L.pc[reg - 1][nx + j] + L.pc[reg - 1][nx + ny + k];
+                R dsum = R(0);
+                if (reg != 1) {
+                    const int gi = ox + i, gj = oy + j, gk = oz + k;  // particle-lattice coordinates
+                    const Pair2<R> l0 = ltab[static_cast<uint32_t>(pxy + gi * Ly + gj)];
+                    const Pair2<R> l1 = ltab[static_cast<uint32_t>(pxz + gi * Lz + gk)];
+                    const Pair2<R> l2 = ltab[static_cast<uint32_t>(pyx + gj * Lx + gi)];
+                    const Pair2<R> l3 = ltab[static_cast<uint32_t>(pyz + gj * Lz + gk)];
+                    const Pair2<R> l4 = ltab[static_cast<uint32_t>(pzx + gk * Lx + gi)];
+                    const Pair2<R> l5 = ltab[static_cast<uint32_t>(pzy + gk * Ly + gj)];
+                    dsum = (reg == 2) ? (((l0.a + l1.a) + (l2.a + l3.a)) + (l4.a + l5.a))
+                                      : (((l0.b + l1.b) + (l2.b + l3.b)) + (l4.b + l5.b));
+                }
+                const R W = m4_lattice_free<R>(ax, ay, az, r, reg) * M4Consts<R>::inv_pi + R(0.5) * csum + dsum;
+                const bool neg = (dx < R(0)) != ((dy < R(0)) != (dz < R(0)));
+                L.U[u] = neg ? -W : W;
+            }
+            wave_sync();
+
+            // ---- cell sums: the 8-corner stencil, stored to the item's slab ----
+            const int myz = my * mz, ncells = mx * myz;
+            const float inv_myz = 1.0f / static_cast<float>(myz), inv_mz = 1.0f / static_cast<float>(mz);
+            for (int u = lane; u < ncells; u += 64) {
+                const int i = small_div(u, myz, inv_myz);
+                const int rem = u - i * myz;
+                const int j = small_div(rem, mz, inv_mz);
+                const int k = rem - j * mz;
+                double Ms = 0.0;
+                if (L.cg2[i] + L.cg2[nx + j] + L.cg2[nx + ny + k] < R(4)) {
+                    ++n_sphere;
+                    const R* U0 = &L.U[(i * ny + j) * nz + k];
+                    Ms = static_cast<double>(U0[nyz]) + static_cast<double>(U0[nz]) + static_cast<double>(U0[1]) +
+                         static_cast<double>(U0[nyz + nz + 1]) - static_cast<double>(U0[0]) -
+                         static_cast<double>(U0[nyz + nz]) - static_cast<double>(U0[nyz + 1]) -
+                         static_cast<double>(U0[nz + 1]);
+                    if (clamp_pairs && Ms < 0.0) {
+                        ++n_neg;
+                        if (Ms <= -0.01) {
+                            ++n_warn;
+                            const unsigned long long slot = atomicAdd(&st->n_warn, 1ull);
+                            if (slot < KIWI_MAX_WARN_RECORDS) {
+                                st->warn[slot].particle = p;
+                                st->warn[slot].cell =
+                                    (static_cast<int64_t>(ib.x0 + i) * g.ny + (ib.y0 + j)) * g.nz + (ib.z0 + k);
+                                st->warn[slot].value = Ms;
+                            }
                         }
+                        Ms = 0.0;
                     }
-                    Ms = 0.0;
                 }
+                out[u] = static_cast<R>(Ms);
             }
-            ms[cbase + (static_cast<int64_t>(c0x + i) * Ny + (c0y + j)) * Nz + (c0z + k)] = static_cast<R>(Ms);
+            wave_sync();
         }
-        wave_sync();
+        cur = wave_uniform(nxt);
     }
     for (int off = 32; off > 0; off >>= 1) {
         n_neg += __shfl_down(n_neg, off);
@@ -492,11 +546,11 @@ __global__ __launch_bounds__(THREADS) void k_particles(
 // 8. accumulation: one workgroup per 8^3 cell block
 // ------------------------------------------------------------------------------------------------
 constexpr int ACC_CPT = BLK * BLK * BLK / THREADS;  // 2 cells per thread, along x
-constexpr int ACC_GROUP = 8;                         // particles whose sums are loaded before adding
+constexpr int ACC_GROUP = 8;                         // items whose sums are loaded before adding
 
 struct AccLds {
-    int bb[6][THREADS];
-    long long cb[THREADS];
+    long long off[THREADS];  // item slab offsets
+    uint32_t box[THREADS];   // block-local origin and extents, 4 bits each: ox oy oz mx my mz
     double m[THREADS], v[3][THREADS];
 };
 
@@ -504,8 +558,8 @@ template <typename R>
 __global__ __launch_bounds__(THREADS) void k_accum(
     const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
     const double* __restrict__ m, const int* __restrict__ bbox, const uint64_t* __restrict__ keys,
-    const int* __restrict__ blk_begin, const int* __restrict__ blk_end, const int* __restrict__ order,
-    const int64_t* __restrict__ coffs, const R* __restrict__ ms, GridGeom g, double* __restrict__ out_mass,
+    const int64_t* __restrict__ slab, const int* __restrict__ blk_begin, const int* __restrict__ blk_end,
+    const int* __restrict__ order, const R* __restrict__ ms, GridGeom g, double* __restrict__ out_mass,
     double* __restrict__ out_px, double* __restrict__ out_py, double* __restrict__ out_pz, int clamp_pairs) {
     __shared__ AccLds A;
     const int b = order[blockIdx.x];
@@ -513,9 +567,10 @@ __global__ __launch_bounds__(THREADS) void k_accum(
     const int by = (b / g.nbz) % g.nby;
     const int bx = b / (g.nbz * g.nby);
     const int tid = threadIdx.x;
-    // owned cells: column (cy, cz) = tid % 64, x = 2 * (tid / 64) + s
-    const int gy = by * BLK + (tid & 63) / BLK, gz = bz * BLK + (tid & 7);
-    const int gx0 = bx * BLK + (tid >> 6) * ACC_CPT;
+    const int wave = static_cast<int>(wave_uniform(threadIdx.x >> 6));
+    // owned cells (block-local): x = 2 * wave + s, y = (tid % 64) / 8, z = tid % 8
+    const int cy = (tid & 63) >> 3, cz = tid & 7;
+    const int cx0 = wave * ACC_CPT;
 
     double acc[ACC_CPT][4];
 #pragma unroll
@@ -526,11 +581,13 @@ __global__ __launch_bounds__(THREADS) void k_accum(
         const int cnt = min(THREADS, q1 - base);
         __syncthreads();
         if (tid < cnt) {
-            const int p = static_cast<int>(keys[base + tid] & 0xffffffffull);
-            const int* pb = bbox + 6 * p;
-#pragma unroll
-            for (int a = 0; a < 6; ++a) A.bb[a][tid] = pb[a];
-            A.cb[tid] = coffs[p];
+            const uint64_t key = keys[base + tid];
+            const int p = static_cast<int>(key & 0xffffffffull);
+            const ItemBox ib = item_box(bbox + 6 * p, static_cast<uint32_t>(b), g);
+            const uint32_t ox = ib.x0 - bx * BLK, oy = ib.y0 - by * BLK, oz = ib.z0 - bz * BLK;
+            A.box[tid] = ox | (oy << 4) | (oz << 8) | (static_cast<uint32_t>(ib.mx) << 12) |
+                         (static_cast<uint32_t>(ib.my) << 16) | (static_cast<uint32_t>(ib.mz) << 20);
+            A.off[tid] = slab[base + tid];
             A.m[tid] = m[p];
             A.v[0][tid] = vx[p];
             A.v[1][tid] = vy[p];
@@ -538,7 +595,7 @@ __global__ __launch_bounds__(THREADS) void k_accum(
         }
         __syncthreads();
         for (int g0 = 0; g0 < cnt; g0 += ACC_GROUP) {
-            // load the cell sums of a group of particles first (independent loads in flight) ...
+            // load the cell sums of a group of items first (independent loads in flight) ...
             double val[ACC_GROUP][ACC_CPT];
 #pragma unroll
             for (int u = 0; u < ACC_GROUP; ++u) {
@@ -546,15 +603,17 @@ __global__ __launch_bounds__(THREADS) void k_accum(
 #pragma unroll
                 for (int s = 0; s < ACC_CPT; ++s) val[u][s] = 0.0;
                 if (q >= cnt) continue;
-                const int x0 = A.bb[0][q], x1 = A.bb[1][q], y0 = A.bb[2][q], y1 = A.bb[3][q], z0 = A.bb[4][q],
-                          z1 = A.bb[5][q];
-                if (gy < y0 || gy > y1 || gz < z0 || gz > z1) continue;
-                const int Ny = y1 - y0 + 1, Nz = z1 - z0 + 1;
-                const long long row = A.cb[q] + static_cast<long long>(gy - y0) * Nz + (gz - z0);
+                // item geometry is the same for every lane: scalar registers
+                const uint32_t bx_ = wave_uniform(A.box[q]);
+                const int ox = bx_ & 15, oy = (bx_ >> 4) & 15, oz = (bx_ >> 8) & 15;
+                const int mx = (bx_ >> 12) & 15, my = (bx_ >> 16) & 15, mz = (bx_ >> 20) & 15;
+                const int ly = cy - oy, lz = cz - oz;
+                if (ly < 0 || ly >= my || lz < 0 || lz >= mz) continue;
+                const R* row = ms + A.off[q] + (ly * mz + lz);
 #pragma unroll
                 for (int s = 0; s < ACC_CPT; ++s) {
-                    const int gx = gx0 + s;
-                    if (gx >= x0 && gx <= x1) val[u][s] = static_cast<double>(ms[row + static_cast<long long>(gx - x0) * Ny * Nz]);
+                    const int lx = cx0 + s - ox;
+                    if (lx >= 0 && lx < mx) val[u][s] = static_cast<double>(row[lx * my * mz]);
                 }
             }
             // ... then add them in list order
@@ -577,11 +636,12 @@ __global__ __launch_bounds__(THREADS) void k_accum(
         }
     }
 
+    const int iy = by * BLK + cy, iz = bz * BLK + cz;
 #pragma unroll
     for (int s = 0; s < ACC_CPT; ++s) {
-        const int ix = gx0 + s;
-        if (ix >= g.nx || gy >= g.ny || gz >= g.nz) continue;
-        const int64_t c = (static_cast<int64_t>(ix) * g.ny + gy) * g.nz + gz;
+        const int ix = bx * BLK + cx0 + s;
+        if (ix >= g.nx || iy >= g.ny || iz >= g.nz) continue;
+        const int64_t c = (static_cast<int64_t>(ix) * g.ny + iy) * g.nz + iz;
         double mass = acc[s][0], p0 = acc[s][1], p1 = acc[s][2], p2 = acc[s][3];
         if (!clamp_pairs && mass < 0.0) {
             // mixed mode clamps the cell's total instead of every pair: a negative total is rounding
@@ -649,11 +709,12 @@ struct DevBuf {
 
 struct kiwi_hip_ctx {
     int device = 0;
+    int n_cu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     std::string err;
-    DevBuf parts, bbox, sizes, offsets, keys_a, keys_b, cub_tmp, ranges, work, work_sorted, order_in, order, tasks,
-        lines, ms, stats, outbuf, vsum;
+    DevBuf parts, bbox, sizes, offsets, keys_a, keys_b, pos_a, pos_b, cells, slab_sorted, slab, cub_tmp, ranges, work,
+        work_sorted, order_in, order, lines, ms, stats, outbuf, vsum;
 };
 
 namespace {
@@ -689,33 +750,40 @@ hipError_t reserve(DevBuf& b, size_t bytes) {
     return hipSuccess;
 }
 
-// persistent grid: enough workgroups to fill every CU several times over, fewer for small jobs
-int persistent_grid(int64_t items, int per_block, int cap) {
-    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((items + per_block - 1) / per_block, cap)));
+// workgroups of a persistent kernel: as many as are resident on the device at once (every wave
+// loops until the work runs out), fewer for small jobs
+template <typename K>
+int resident_grid(const kiwi_hip_ctx* c, K kernel, int64_t items_per_wave_round, int64_t items) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, THREADS, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    const int64_t full = static_cast<int64_t>(per_cu) * std::max(c->n_cu, 1);
+    const int64_t need = (items + WAVES * items_per_wave_round - 1) / (WAVES * items_per_wave_round);
+    return static_cast<int>(std::max<int64_t>(1, std::min(full, need)));
 }
 
 template <typename R>
-hipError_t launch_cells(hipStream_t s, int64_t n, int64_t ntasks, const double* const* dp, const int* bbox,
-                        const int64_t* loffs, const int64_t* coffs, const int2* tasks,
-                        const uint64_t* keys, const int* blk_begin, const int* blk_end, const int* order,
-                        int nblocks, const GridGeom& g, void* lines, void* ms, double* const* om, DevStats* dst,
-                        int clamp, hipEvent_t ev_lines, hipEvent_t ev_cells, hipEvent_t ev_accum) {
-    if (n > 0) {
-        k_lines<R><<<persistent_grid(n, THREADS / 64, 2048), THREADS, 0, s>>>(
+hipError_t launch_cells(const kiwi_hip_ctx* c, hipStream_t s, int64_t n, int64_t npairs, const double* const* dp,
+                        const int* bbox, const int64_t* loffs, const uint64_t* keys_particle_order,
+                        const int64_t* slab, const uint64_t* keys_block_order, const int64_t* slab_sorted,
+                        const int* blk_begin, const int* blk_end, const int* order, int nblocks, const GridGeom& g,
+                        void* lines, void* ms, double* const* om, DevStats* dst, int clamp, hipEvent_t ev_lines,
+                        hipEvent_t ev_cells, hipEvent_t ev_accum) {
+    if (n > 0)
+        k_lines<R><<<resident_grid(c, k_lines<R>, 1, n), THREADS, 0, s>>>(
             n, dp[0], dp[1], dp[2], dp[7], bbox, loffs, g, static_cast<Pair2<R>*>(lines), dst);
-    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if ((e = hipEventRecord(ev_lines, s)) != hipSuccess) return e;
-    if (ntasks > 0) {
-        k_particles<R><<<persistent_grid(ntasks, THREADS / 64, 4096), THREADS, 0, s>>>(
-            ntasks, tasks, dp[0], dp[1], dp[2], dp[7], bbox, loffs, coffs, g, static_cast<const Pair2<R>*>(lines),
-            static_cast<R*>(ms), dst, clamp);
-    }
+    if (npairs > 0)
+        k_particles<R><<<resident_grid(c, k_particles<R>, ITEM_GRAB, npairs), THREADS, 0, s>>>(
+            static_cast<uint32_t>(npairs), keys_particle_order, slab, dp[0], dp[1], dp[2], dp[7], bbox, loffs, g,
+            static_cast<const Pair2<R>*>(lines), static_cast<R*>(ms), dst, clamp);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipEventRecord(ev_cells, s)) != hipSuccess) return e;
-    k_accum<R><<<nblocks, THREADS, 0, s>>>(dp[3], dp[4], dp[5], dp[6], bbox, keys, blk_begin, blk_end, order, coffs,
-                                          static_cast<const R*>(ms), g, om[0], om[1], om[2], om[3], clamp);
+    k_accum<R><<<nblocks, THREADS, 0, s>>>(dp[3], dp[4], dp[5], dp[6], bbox, keys_block_order, slab_sorted, blk_begin,
+                                          blk_end, order, static_cast<const R*>(ms), g, om[0], om[1], om[2], om[3],
+                                          clamp);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return hipEventRecord(ev_accum, s);
 }
@@ -735,6 +803,7 @@ int kiwi_hip_create(kiwi_hip_ctx** out, int device_ordinal) {
     kiwi_hip_ctx* c = new kiwi_hip_ctx();
     c->device = device_ordinal;
     if (hipSetDevice(device_ordinal) != hipSuccess ||
+        hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device_ordinal) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return KIWI_ERR_DEVICE;
@@ -753,9 +822,9 @@ void kiwi_hip_destroy(kiwi_hip_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf* b : {&c->parts, &c->bbox, &c->sizes, &c->offsets, &c->keys_a, &c->keys_b, &c->cub_tmp, &c->ranges,
-                      &c->work, &c->work_sorted, &c->order_in, &c->order, &c->tasks, &c->lines, &c->ms, &c->stats,
-                      &c->outbuf, &c->vsum})
+    for (DevBuf* b : {&c->parts, &c->bbox, &c->sizes, &c->offsets, &c->keys_a, &c->keys_b, &c->pos_a, &c->pos_b,
+                      &c->cells, &c->slab_sorted, &c->slab, &c->cub_tmp, &c->ranges, &c->work, &c->work_sorted,
+                      &c->order_in, &c->order, &c->lines, &c->ms, &c->stats, &c->outbuf, &c->vsum})
         if (b->p) (void)hipFree(b->p);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -786,7 +855,7 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     if (K->kernel_id != KIWI_KERNEL_M4 || K->generic_path || sup != 2.0 || K->tabulated)
         return fail(c, KIWI_ERR_UNSUPPORTED, "this build implements the analytic cubic-spline (M4, support 2h) path only");
     if (P->n > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 particles in one call");
-    if (G->nx > CH * CH_MAXAXIS || G->ny > CH * CH_MAXAXIS || G->nz > CH * CH_MAXAXIS)
+    if (G->nx > MAX_AXIS || G->ny > MAX_AXIS || G->nz > MAX_AXIS)
         return fail(c, KIWI_ERR_LIMIT, "more than 8192 cells along a grid axis");
 
     KIWI_HIP_CHECK(c, hipSetDevice(c->device));
@@ -832,13 +901,13 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
         for (int k = 0; k < 4; ++k) om[k] = static_cast<double*>(c->outbuf.p) + k * ncells;
     }
 
-    // per-particle sizes (blocks, lines, tasks, cells) and their exclusive scans, (n + 1) each
+    // per-particle sizes (items, lines) and their exclusive scans, (n + 1) each
     const int64_t n1 = n + 1;
     KIWI_HIP_CHECK(c, reserve(c->stats, sizeof(DevStats)));
     DevStats* dst = static_cast<DevStats*>(c->stats.p);
     KIWI_HIP_CHECK(c, reserve(c->bbox, 6 * sizeof(int) * std::max<int64_t>(n, 1)));
-    KIWI_HIP_CHECK(c, reserve(c->sizes, 4 * sizeof(int64_t) * n1));
-    KIWI_HIP_CHECK(c, reserve(c->offsets, 4 * sizeof(int64_t) * n1));
+    KIWI_HIP_CHECK(c, reserve(c->sizes, 2 * sizeof(int64_t) * n1));
+    KIWI_HIP_CHECK(c, reserve(c->offsets, 2 * sizeof(int64_t) * n1));
     KIWI_HIP_CHECK(c, reserve(c->ranges, 2 * sizeof(int) * nblocks));
     KIWI_HIP_CHECK(c, reserve(c->work, sizeof(unsigned long long) * nblocks));
     KIWI_HIP_CHECK(c, reserve(c->work_sorted, sizeof(unsigned long long) * nblocks));
@@ -848,8 +917,8 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     int* bbox = static_cast<int*>(c->bbox.p);
     int64_t* sz = static_cast<int64_t*>(c->sizes.p);
     int64_t* of = static_cast<int64_t*>(c->offsets.p);
-    int64_t *nblk = sz, *nlin = sz + n1, *ntask = sz + 2 * n1, *ncell = sz + 3 * n1;
-    int64_t *offs = of, *loffs = of + n1, *toffs = of + 2 * n1, *coffs = of + 3 * n1;
+    int64_t *nblk = sz, *nlin = sz + n1;
+    int64_t *offs = of, *loffs = of + n1;
     int* blk_begin = static_cast<int*>(c->ranges.p);
     int* blk_end = blk_begin + nblocks;
     unsigned long long* work = static_cast<unsigned long long*>(c->work.p);
@@ -861,59 +930,79 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     KIWI_HIP_CHECK(c, hipMemsetAsync(dst, 0, sizeof(DevStats), s));
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[0], s));
     const double support = 2.0;
-    k_prep<<<grid1d(n1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], support, g, bbox, nblk, nlin, ntask, ncell,
-                                         dst);
+    k_prep<<<grid1d(n1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], support, g, bbox, nblk, nlin, dst);
     KIWI_HIP_CHECK(c, hipGetLastError());
     {
         size_t tmp_bytes = 0;
         KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, nblk, offs, static_cast<int>(n1), s));
         KIWI_HIP_CHECK(c, reserve(c->cub_tmp, tmp_bytes));
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < 2; ++k) {
             size_t tb = c->cub_tmp.cap;
             KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, tb, sz + k * n1, of + k * n1,
                                                                static_cast<int>(n1), s));
         }
     }
-    int64_t totals[4] = {0, 0, 0, 0};
-    for (int k = 0; k < 4; ++k)
+    int64_t totals[2] = {0, 0};
+    unsigned long long pairs_bbox = 0;
+    for (int k = 0; k < 2; ++k)
         KIWI_HIP_CHECK(c, hipMemcpyAsync(&totals[k], of + k * n1 + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    KIWI_HIP_CHECK(c, hipMemcpyAsync(&pairs_bbox, &dst->pairs_bbox, sizeof(pairs_bbox), hipMemcpyDeviceToHost, s));
     KIWI_HIP_CHECK(c, hipStreamSynchronize(s));
-    const int64_t npairs = totals[0], nlines_total = totals[1], ntasks = totals[2], ncell_total = totals[3];
+    const int64_t npairs = totals[0], nlines_total = totals[1];
     if (npairs > 0x7fffffffLL)
-        return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 (particle, cell-block) work items in one call");
+        return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 (particle, cell-block) items in one call");
 
     const size_t rsz = (precision == KIWI_FP64) ? sizeof(double) : sizeof(float);
+    const int64_t np1 = npairs + 1;
     KIWI_HIP_CHECK(c, reserve(c->lines, 2 * rsz * static_cast<size_t>(std::max<int64_t>(nlines_total, 1))));
-    KIWI_HIP_CHECK(c, reserve(c->ms, rsz * static_cast<size_t>(std::max<int64_t>(ncell_total, 1))));
-    KIWI_HIP_CHECK(c, reserve(c->tasks, sizeof(int2) * static_cast<size_t>(std::max<int64_t>(ntasks, 1))));
-    KIWI_HIP_CHECK(c, reserve(c->keys_a, sizeof(uint64_t) * std::max<int64_t>(npairs, 1)));
-    KIWI_HIP_CHECK(c, reserve(c->keys_b, sizeof(uint64_t) * std::max<int64_t>(npairs, 1)));
-    uint64_t* keys_a = static_cast<uint64_t*>(c->keys_a.p);
-    uint64_t* keys_b = static_cast<uint64_t*>(c->keys_b.p);
-    int2* tasks = static_cast<int2*>(c->tasks.p);
+    KIWI_HIP_CHECK(c, reserve(c->ms, rsz * static_cast<size_t>(std::max<unsigned long long>(pairs_bbox, 1))));
+    KIWI_HIP_CHECK(c, reserve(c->keys_a, sizeof(uint64_t) * np1));
+    KIWI_HIP_CHECK(c, reserve(c->keys_b, sizeof(uint64_t) * np1));
+    KIWI_HIP_CHECK(c, reserve(c->pos_a, sizeof(int) * np1));
+    KIWI_HIP_CHECK(c, reserve(c->pos_b, sizeof(int) * np1));
+    KIWI_HIP_CHECK(c, reserve(c->cells, sizeof(int64_t) * np1));
+    KIWI_HIP_CHECK(c, reserve(c->slab_sorted, sizeof(int64_t) * np1));
+    KIWI_HIP_CHECK(c, reserve(c->slab, sizeof(int64_t) * np1));
+    uint64_t* keys_a = static_cast<uint64_t*>(c->keys_a.p);  // particle order
+    uint64_t* keys_b = static_cast<uint64_t*>(c->keys_b.p);  // block order
+    int* pos_a = static_cast<int*>(c->pos_a.p);
+    int* pos_b = static_cast<int*>(c->pos_b.p);  // particle-order position of each block-order item
+    int64_t* cells = static_cast<int64_t*>(c->cells.p);
+    int64_t* slab_sorted = static_cast<int64_t*>(c->slab_sorted.p);
+    int64_t* slab = static_cast<int64_t*>(c->slab.p);
     if (n > 0) {
         k_emit<<<grid1d(n), THREADS, 0, s>>>(n, bbox, offs, g, keys_a);
-        k_emit_tasks<<<grid1d(n), THREADS, 0, s>>>(n, bbox, toffs, tasks);
         KIWI_HIP_CHECK(c, hipGetLastError());
     }
-    int end_bit = 32;
+    int end_bit = 33;  // at least one key bit above the particle index
     while ((1LL << (end_bit - 32)) < nblocks) ++end_bit;
     if (npairs > 0) {
+        k_iota<<<grid1d(npairs), THREADS, 0, s>>>(pos_a, npairs);
         size_t sort_bytes = 0;
-        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, keys_a, keys_b,
-                                                             static_cast<int>(npairs), 0, end_bit, s));
+        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys_a, keys_b, pos_a, pos_b,
+                                                              static_cast<int>(npairs), 32, end_bit, s));
         KIWI_HIP_CHECK(c, reserve(c->cub_tmp, sort_bytes));
         sort_bytes = c->cub_tmp.cap;
-        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortKeys(c->cub_tmp.p, sort_bytes, keys_a, keys_b,
-                                                             static_cast<int>(npairs), 0, end_bit, s));
+        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairs(c->cub_tmp.p, sort_bytes, keys_a, keys_b, pos_a, pos_b,
+                                                              static_cast<int>(npairs), 32, end_bit, s));
+    }
+    k_item_cells<<<grid1d(np1), THREADS, 0, s>>>(npairs, keys_b, bbox, g, cells);
+    {
+        size_t tb = 0;
+        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cells, slab_sorted, static_cast<int>(np1), s));
+        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, tb));
+        tb = c->cub_tmp.cap;
+        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, tb, cells, slab_sorted, static_cast<int>(np1),
+                                                           s));
     }
     k_fill_int<<<grid1d(2 * nblocks), THREADS, 0, s>>>(blk_begin, 2 * nblocks, 0);
     k_fill_u64<<<grid1d(nblocks), THREADS, 0, s>>>(work, nblocks);
     k_iota<<<grid1d(nblocks), THREADS, 0, s>>>(order_in, nblocks);
     KIWI_HIP_CHECK(c, hipGetLastError());
     if (npairs > 0) {
+        k_scatter<<<grid1d(npairs), THREADS, 0, s>>>(npairs, pos_b, slab_sorted, slab);
         k_ranges<<<grid1d(npairs), THREADS, 0, s>>>(npairs, keys_b, blk_begin, blk_end);
-        k_block_work<<<grid1d(npairs), THREADS, 0, s>>>(npairs, keys_b, bbox, g, work);
+        k_block_work<<<grid1d(npairs), THREADS, 0, s>>>(npairs, keys_b, cells, work);
         KIWI_HIP_CHECK(c, hipGetLastError());
     }
     {
@@ -929,13 +1018,13 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     const int clamp = (precision == KIWI_FP64) ? 1 : 0;
     const int nb = static_cast<int>(nblocks);
     if (precision == KIWI_FP64)
-        KIWI_HIP_CHECK(c, launch_cells<double>(s, n, ntasks, dp, bbox, loffs, coffs, tasks, keys_b, blk_begin,
-                                               blk_end, order, nb, g, c->lines.p, c->ms.p, om, dst, clamp, c->ev[2],
-                                               c->ev[3], c->ev[4]));
+        KIWI_HIP_CHECK(c, launch_cells<double>(c, s, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b, slab_sorted,
+                                               blk_begin, blk_end, order, nb, g, c->lines.p, c->ms.p, om, dst, clamp,
+                                               c->ev[2], c->ev[3], c->ev[4]));
     else
-        KIWI_HIP_CHECK(c, launch_cells<float>(s, n, ntasks, dp, bbox, loffs, coffs, tasks, keys_b, blk_begin,
-                                              blk_end, order, nb, g, c->lines.p, c->ms.p, om, dst, clamp, c->ev[2],
-                                              c->ev[3], c->ev[4]));
+        KIWI_HIP_CHECK(c, launch_cells<float>(c, s, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b, slab_sorted,
+                                              blk_begin, blk_end, order, nb, g, c->lines.p, c->ms.p, om, dst, clamp,
+                                              c->ev[2], c->ev[3], c->ev[4]));
     k_vsum_partial<<<VSUM_BLOCKS, THREADS, 0, s>>>(n, dp[3], dp[4], dp[5], vpart);
     k_vsum_final<<<1, 64, 0, s>>>(vpart, vpart + 3 * VSUM_BLOCKS);
     KIWI_HIP_CHECK(c, hipGetLastError());
@@ -958,7 +1047,6 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
         stats->pairs_bbox = static_cast<int64_t>(hs.pairs_bbox);
         stats->pairs_sphere = static_cast<int64_t>(hs.pairs_sphere);
         stats->particle_blocks = npairs;
-        stats->chunk_tasks = ntasks;
         for (int k = 0; k < 3; ++k) stats->v_mean[k] = n > 0 ? vs[k] / static_cast<double>(n) : 0.0;
         float ms_prep = 0, ms_lines = 0, ms_cells = 0, ms_accum = 0;
         (void)hipEventElapsedTime(&ms_prep, c->ev[0], c->ev[1]);

@@ -52,9 +52,10 @@ struct M4Consts {
 
 // atan on [0, 1] in single precision: t * P(t^2), P of degree 8.  The coefficients are our own fit
 // (iteratively reweighted least squares for minimum relative error on 6000 Chebyshev nodes of
-// [0, 1], i.e. Lawson's algorithm towards the minimax polynomial); the polynomial's relative error
-// is 1.5e-8 and the single-precision evaluation's 1.3e-7 (tests/test_lattice_math.py checks it
-// against double-precision atan).  Arguments above 1 use atan(x) = pi/2 - atan(1/x).
+// [0, 1], i.e. Lawson's algorithm towards the minimax polynomial), rounded to 10 digits.  With these
+// coefficients the polynomial's relative error is 3.6e-8 (largest at t = 1) and the single-precision
+// evaluation's 1.3e-7 (tests/test_lattice_math.py checks both against double-precision atan).
+// Arguments above 1 use atan(x) = pi/2 - atan(1/x).
 template <typename R>
 KIWI_HD R atan_poly01(R t) {
     const R u = t * t;
@@ -102,11 +103,10 @@ struct KMath<float> {
         const float a = atan_poly01(t);
         return big ? M4Consts<float>::half_pi - a : a;
     }
+    // one reciprocal: the smaller over the larger argument
     static KIWI_HD float atan2_pos(float y, float x) {
-        const bool big = y > x;
-        const float t = big ? div(x, y) : div(y, x);
-        const float a = atan_poly01(t);
-        return big ? M4Consts<float>::half_pi - a : a;
+        const float a = atan_poly01(fminf(y, x) * rcp(fmaxf(y, x)));
+        return (y > x) ? M4Consts<float>::half_pi - a : a;
     }
 };
 
@@ -264,14 +264,17 @@ KIWI_HD R m4_lattice_free(R ax, R ay, R az, R r, int k) {
         T1 = M::atan_pos(axz * axz * inv);
         T2 = M::atan_pos(axy * axy * inv);
     } else {
-        const R ir = M::rcp(r);
-        T0 = M::atan_pos(ayz * ir * M::rcp(ax));
-        T1 = M::atan_pos(axz * ir * M::rcp(ay));
-        T2 = M::atan_pos(axy * ir * M::rcp(az));
+        // single precision: one reciprocal per angle (atan2 of the ratio's two sides)
+        T0 = M::atan2_pos(ayz, ax * r);
+        T1 = M::atan2_pos(axz, ay * r);
+        T2 = M::atan2_pos(axy, az * r);
     }
     if (k == 3) return R(-0.5) * (T0 + T1 + T2);
     // L_c: c is the along-edge axis, log((r + d_c)^2 / (r^2 - d_c^2)) with r^2 - d_c^2 formed exactly
     const R sx = ay * ay + az * az, sy = ax * ax + az * az, sz = ax * ax + ay * ay;
+    // one reciprocal for the three denominators.  In single precision the product is kept above
+    // 1e-30 (it is smaller only when all three offsets are below ~1e-5; the L terms then carry weights
+    // below 1e-15, so a capped logarithm changes nothing), which keeps the reciprocal finite.
     R ix, iy, iz;
     if (std::is_same<R, double>::value) {
         const R inv = M::rcp(sx * sy * sz);

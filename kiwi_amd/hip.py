@@ -62,7 +62,7 @@ class WarnRecord(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [("skipped_outside", _i64), ("n_negative", _i64), ("n_negative_warn", _i64), ("pairs_bbox", _i64),
-                ("pairs_sphere", _i64), ("particle_blocks", _i64), ("chunk_tasks", _i64), ("v_mean", _d * 3),
+                ("pairs_sphere", _i64), ("particle_blocks", _i64), ("v_mean", _d * 3),
                 ("device_ms_prep", _d), ("device_ms_lines", _d), ("device_ms_gather", _d), ("device_ms_accum", _d),
                 ("lattice_points", _i64 * 3), ("line_evals", _i64 * 3),
                 ("plane_evals", _i64), ("n_warn_records", _i), ("warn", WarnRecord * 16)]

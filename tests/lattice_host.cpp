@@ -1,0 +1,71 @@
+// Host (g++) instantiations of kiwi_amd/csrc/m4_lattice.h for tests/test_lattice_math.py.
+// The device kernels compile the same header; here it runs in double and, with the host's libm in
+// place of the device's hardware reciprocal / square root / log2, in single precision.
+#include <cmath>
+
+#include "../kiwi_amd/csrc/m4_lattice.h"
+
+using namespace kiwi_hip;
+
+namespace {
+
+// signed lattice value U = s * W at the offset (dx, dy, dz) (units of h)
+template <typename R>
+R lattice_U(R dx, R dy, R dz) {
+    if (dx == R(0) || dy == R(0) || dz == R(0)) return R(0);
+    const R d[3] = {std::fabs(dx), std::fabs(dy), std::fabs(dz)};
+    M4Plane<R> pl[3];
+    R D2[3][3] = {}, D3[3][3] = {};
+    for (int a = 0; a < 3; ++a) pl[a] = m4_plane<R>(d[a]);
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            if (a != b) m4_line<R>(d[a], d[b], D2[a][b], D3[a][b]);
+    const R W = m4_lattice_W<R>(d[0], d[1], d[2], pl, D2, D3);
+    const bool neg = (dx < R(0)) != ((dy < R(0)) != (dz < R(0)));
+    return neg ? -W : W;
+}
+
+// Msum of the box [lo, hi] for a particle at p with smoothing length h: the 8-corner stencil
+template <typename R>
+double lattice_box(const double* p, double h, const double* lo, const double* hi) {
+    double m = 0.0;
+    for (int di = 0; di < 2; ++di)
+        for (int dj = 0; dj < 2; ++dj)
+            for (int dk = 0; dk < 2; ++dk) {
+                const double cx = di ? hi[0] : lo[0], cy = dj ? hi[1] : lo[1], cz = dk ? hi[2] : lo[2];
+                const double e = ((di + dj + dk) % 2 == 1) ? 1.0 : -1.0;
+                m += e * static_cast<double>(lattice_U<R>(static_cast<R>((cx - p[0]) / h),
+                                                          static_cast<R>((cy - p[1]) / h),
+                                                          static_cast<R>((cz - p[2]) / h)));
+            }
+    return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+// closed-form 6-pyramid sum and the literal sum of the 6 pyramids, both in double
+void lat_free_both(double ax, double ay, double az, int k, double* out) {
+    const double r = std::sqrt(ax * ax + ay * ay + az * az);
+    out[0] = m4_lattice_free<double>(ax, ay, az, r, k);
+    out[1] = m4_lattice_free_by_pyramid<double>(ax, ay, az, r, k);
+}
+
+// shared-polynomial line constants and the difference-of-vertex-integrals form: D2, D3, D2', D3'
+void lat_line_both(double da, double db, double* out) {
+    m4_line<double>(da, db, out[0], out[1]);
+    m4_line_by_vertex<double>(da, db, out[2], out[3]);
+}
+
+double lat_box(const double* p, double h, const double* lo, const double* hi, int single) {
+    return single ? lattice_box<float>(p, h, lo, hi) : lattice_box<double>(p, h, lo, hi);
+}
+
+double lat_U(double dx, double dy, double dz) { return lattice_U<double>(dx, dy, dz); }
+
+double lat_atan_poly_d(double t) { return atan_poly01<double>(t); }
+float lat_atan_poly_f(float t) { return atan_poly01<float>(t); }
+float lat_atan2_pos_f(float y, float x) { return KMath<float>::atan2_pos(y, x); }
+
+}  // extern "C"

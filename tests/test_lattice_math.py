@@ -1,0 +1,149 @@
+"""The lattice restatement of the M4 cell integral (kiwi_amd/csrc/m4_lattice.h), checked on the host.
+
+The HIP kernels evaluate Msum(cell) as an 8-corner stencil of lattice values W.  These tests pin that
+restatement, compiled for the host from the same header (tests/lattice_host.cpp), against:
+  - the reference's cubic algorithm in binary128 (the oracle's quad path, itself the oracle's double
+    path instantiated in higher precision, which the golden-vector tests in test_oracle.py pin),
+  - closed forms: a box that contains the whole support holds exactly the particle's mass, and
+    boxes that tile it add up to it,
+  - its own unregrouped forms (the literal sum of the 6 pyramids; the line constants as differences
+    of vertex integrals),
+  - the single-precision atan against double-precision atan.
+"""
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_pd = ctypes.POINTER(ctypes.c_double)
+
+
+@pytest.fixture(scope="module")
+def lat(tmp_path_factory):
+    so = str(tmp_path_factory.mktemp("lattice") / "liblattice_host.so")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", os.path.join(HERE, "lattice_host.cpp"),
+                    "-o", so], check=True)
+    L = ctypes.CDLL(so)
+    L.lat_free_both.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, _pd]
+    L.lat_line_both.argtypes = [ctypes.c_double, ctypes.c_double, _pd]
+    L.lat_box.restype = ctypes.c_double
+    L.lat_box.argtypes = [_pd, ctypes.c_double, _pd, _pd, ctypes.c_int]
+    L.lat_U.restype = ctypes.c_double
+    L.lat_U.argtypes = [ctypes.c_double] * 3
+    L.lat_atan_poly_d.restype = ctypes.c_double
+    L.lat_atan_poly_d.argtypes = [ctypes.c_double]
+    L.lat_atan_poly_f.restype = ctypes.c_float
+    L.lat_atan_poly_f.argtypes = [ctypes.c_float]
+    L.lat_atan2_pos_f.restype = ctypes.c_float
+    L.lat_atan2_pos_f.argtypes = [ctypes.c_float, ctypes.c_float]
+    return L
+
+
+def _box(L, p, h, lo, hi, single=False):
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (p, lo, hi)]
+    return L.lat_box(a[0].ctypes.data_as(_pd), h, a[1].ctypes.data_as(_pd), a[2].ctypes.data_as(_pd), int(single))
+
+
+def _region(r2):
+    return 1 if r2 < 1 else (2 if r2 < 4 else 3)
+
+
+def test_closed_form_equals_sum_of_pyramids(lat):
+    rng = np.random.default_rng(5)
+    pts = list(rng.uniform(0.01, 2.6, size=(400, 3)))
+    pts += [(1e-4, 0.7, 0.9), (0.3, 1e-5, 1.2), (1.9, 0.05, 0.04), (0.5, 0.5, 0.5), (1.0, 1.0, 1.0)]
+    out = (ctypes.c_double * 2)()
+    for ax, ay, az in pts:
+        k = _region(ax * ax + ay * ay + az * az)
+        lat.lat_free_both(ax, ay, az, k, out)
+        assert abs(out[0] - out[1]) <= 1e-12 * max(1.0, abs(out[1])), (ax, ay, az, k, out[0], out[1])
+
+
+def test_line_constants_shared_polynomials(lat):
+    rng = np.random.default_rng(6)
+    pairs = list(rng.uniform(0.001, 2.1, size=(600, 2))) + [(0.999, 0.01), (0.01, 0.999), (1.41, 1.41)]
+    out = (ctypes.c_double * 4)()
+    for da, db in pairs:
+        lat.lat_line_both(da, db, out)
+        assert abs(out[0] - out[2]) <= 1e-13 and abs(out[1] - out[3]) <= 1e-13, (da, db, list(out))
+        if da * da + db * db > 4:
+            assert out[0] == 0.0 and out[1] == 0.0
+
+
+@pytest.mark.parametrize("p,h", [((0.13, -0.41, 0.27), 0.8), ((1.77, 0.52, -1.05), 0.35), ((0.0, 0.5, 0.25), 0.6),
+                                 ((0.5, 0.5, 0.5), 0.45), ((-0.2, 0.3, 0.71), 1.1)])
+def test_box_sums_equal_binary128_reference_algorithm(lat, p, h):
+    """Every cell of the particle's bounding box, against the reference's cubic algorithm evaluated in
+    binary128.  The cases include a particle on a grid plane (x = 0) and one on a lattice point."""
+    dom = np.array([-3.0, 3.0, -2.5, 3.5, -3.0, 3.0])
+    res = (24, 24, 24)
+    d = (dom[1::2] - dom[::2]) / np.array(res)
+    parts = {k: np.array([v]) for k, v in zip(("x", "y", "z"), p)}
+    parts.update(vx=np.zeros(1), vy=np.zeros(1), vz=np.zeros(1), m=np.ones(1), h=np.array([h]))
+    lo_i = np.maximum(np.floor((np.array(p) - 2 * h - dom[::2]) / d).astype(int), 0)
+    hi_i = np.minimum(np.floor((np.array(p) + 2 * h - dom[::2]) / d).astype(int), np.array(res) - 1)
+    cells, lat_vals = [], []
+    for i in range(lo_i[0], hi_i[0] + 1):
+        for j in range(lo_i[1], hi_i[1] + 1):
+            for k in range(lo_i[2], hi_i[2] + 1):
+                lo = dom[::2] + np.array([i, j, k]) * d
+                cells.append((i * res[1] + j) * res[2] + k)
+                lat_vals.append(_box(lat, p, h, lo, lo + d))
+    q = O.sph_to_grid_cells_quad(parts, dom, *res, np.array(cells))[:, 0]
+    lat_vals = np.maximum(np.array(lat_vals), 0.0)  # the reference clamps Msum < 0 to 0
+    err = np.abs(lat_vals - q)
+    assert err.max() <= 2e-14, (err.max(), cells[int(np.argmax(err))])
+    assert abs(q.sum() - 1.0) < 1e-12  # the bounding box holds the whole support
+
+
+def test_enclosing_box_holds_the_mass_and_tiles_add_up(lat):
+    rng = np.random.default_rng(8)
+    for _ in range(20):
+        p = rng.uniform(-1, 1, 3)
+        h = rng.uniform(0.2, 2.0)
+        lo = p - 2 * h - rng.uniform(0.01, 1.0, 3)
+        hi = p + 2 * h + rng.uniform(0.01, 1.0, 3)
+        assert abs(_box(lat, p, h, lo, hi) - 1.0) < 1e-13
+        # split the box at random planes through the support: the 8 parts add up to the whole
+        cut = p + rng.uniform(-1.5, 1.5, 3) * h
+        tot = 0.0
+        for sx in range(2):
+            for sy in range(2):
+                for sz in range(2):
+                    a = np.where([sx, sy, sz], cut, lo)
+                    b = np.where([sx, sy, sz], hi, cut)
+                    tot += _box(lat, p, h, a, b)
+        assert abs(tot - 1.0) < 1e-13
+
+
+def test_single_precision_lattice_close_to_double(lat):
+    rng = np.random.default_rng(9)
+    worst = 0.0
+    for _ in range(300):
+        p = rng.uniform(-1, 1, 3)
+        h = rng.uniform(0.3, 1.5)
+        lo = p + rng.uniform(-2.2, 1.8, 3) * h
+        hi = lo + rng.uniform(0.05, 0.8, 3) * h
+        worst = max(worst, abs(_box(lat, p, h, lo, hi, single=True) - _box(lat, p, h, lo, hi)))
+    assert worst < 2e-6, worst
+
+
+def test_atan_polynomial_error_bounds(lat):
+    t = np.linspace(0.0, 1.0, 200001)[1:]
+    exact = np.arctan(t)
+    pd = np.array([lat.lat_atan_poly_d(float(v)) for v in t[::10]])
+    assert np.max(np.abs(pd - exact[::10]) / exact[::10]) < 4e-8
+    t32 = t[::10].astype(np.float32)
+    pf = np.array([lat.lat_atan_poly_f(float(v)) for v in t32], dtype=np.float64)
+    exact32 = np.arctan(t32.astype(np.float64))  # the float argument's own atan
+    assert np.max(np.abs(pf - exact32) / exact32) < 1.5e-7
+    rng = np.random.default_rng(10)
+    for y, x in rng.uniform(1e-4, 10.0, size=(2000, 2)):
+        y32, x32 = float(np.float32(y)), float(np.float32(x))
+        assert abs(lat.lat_atan2_pos_f(y32, x32) - math.atan2(y32, x32)) <= 3e-7 * math.atan2(y32, x32)
