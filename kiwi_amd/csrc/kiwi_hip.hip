@@ -16,7 +16,8 @@
 //      k_ranges, k_block_work + sort: [begin, end) of every block's items and the blocks in order of
 //                    decreasing work (longest-processing-time-first) for the accumulation.
 //   6. k_lines       one wavefront per particle: the region-matching constants (D2, D3) of every edge
-//                    line of the particle's lattice, once per particle, into HBM.
+//                    line of the particle's lattice, summed over the two lines through each pair of
+//                    lattice coordinates, once per particle, into HBM.
 //   7. k_particles   wavefronts take items in particle order from a shared cursor; per item: wall-plane
 //                    constants and per-axis distance gaps in the wave's LDS, a compacted list of the
 //                    lattice points any in-sphere cell needs, their lattice values W (m4_lattice.h),
@@ -77,7 +78,8 @@ __device__ __forceinline__ T sel3(int a, T v0, T v1, T v2) { return a == 0 ? v0 
 __device__ __forceinline__ double gmin(const GridGeom& g, int a) { return sel3(a, g.x_min, g.y_min, g.z_min); }
 __device__ __forceinline__ double gstep(const GridGeom& g, int a) { return sel3(a, g.dx, g.dy, g.dz); }
 
-// u / d for small non-negative integers through a float reciprocal, corrected to be exact
+// u / d for non-negative integers through a float reciprocal inv_d ~ 1/d (within a few ulp), corrected
+// by one step either way to be exact; valid while u / d < 2^22
 __device__ __forceinline__ int small_div(int u, int d, float inv_d) {
     int q = static_cast<int>((static_cast<float>(u) + 0.5f) * inv_d);
     const int r = u - q * d;
@@ -91,6 +93,9 @@ template <typename R>
 __device__ __forceinline__ R gap(R lo, R hi) {
     return (lo > R(0)) ? lo : ((hi < R(0)) ? -hi : R(0));
 }
+
+// a value that is the same in every lane, moved to a scalar register
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // LDS writes of one lane made visible to the other lanes of the same wavefront
 __device__ __forceinline__ void wave_sync() {
@@ -155,7 +160,7 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
                 pairs = static_cast<unsigned long long>(Nx * Ny * Nz);
                 nb = static_cast<int64_t>(ix1 / BLK - ix0 / BLK + 1) * (iy1 / BLK - iy0 / BLK + 1) *
                      (iz1 / BLK - iz0 / BLK + 1);
-                nl = 2 * ((Nx + 1) * (Ny + 1) + (Nx + 1) * (Nz + 1) + (Ny + 1) * (Nz + 1));
+                nl = (Nx + 1) * (Ny + 1) + (Nx + 1) * (Nz + 1) + (Ny + 1) * (Nz + 1);
             }
         }
         nblk[i] = nb;
@@ -281,9 +286,11 @@ __global__ void k_block_work(int64_t npairs, const uint64_t* __restrict__ keys, 
 // ------------------------------------------------------------------------------------------------
 // 6. edge-line constants, once per particle
 // ------------------------------------------------------------------------------------------------
-// Table layout per particle (lattice extents L = bbox cells + 1 per axis), six ordered (wall a,
-// edge-normal b) blocks in the order (x,y) (x,z) (y,x) (y,z) (z,x) (z,y), each La x Lb with the
-// a-index major: entry (ia, ib) holds (D2, D3) of the line at lattice coordinates (lo_a + ia, lo_b + ib).
+// Table layout per particle (lattice extents L = bbox cells + 1 per axis): three unordered axis pairs
+// (x,y) (x,z) (y,z), each La x Lb with the a-index major.  Entry (ia, ib) of pair {a, b} holds the sums
+//   E2 = D2(d_a, d_b) + D2(d_b, d_a),   E3 = D3(d_a, d_b) + D3(d_b, d_a)
+// of the two edge lines through lattice coordinates (lo_a + ia, lo_b + ib): wall a with edge normal b,
+// and wall b with edge normal a.  A lattice point in region k adds E_k of its three axis pairs.
 template <typename R>
 __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                                                    const double* __restrict__ z, const double* __restrict__ h,
@@ -291,38 +298,45 @@ __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __re
                                                    GridGeom g, Pair2<R>* __restrict__ table, DevStats* st) {
     // grid-stride over particles, one wavefront per particle at a time; counters stay in registers
     const int lane = threadIdx.x & 63;
-    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (THREADS / 64);
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * WAVES;
     unsigned long long cnt0 = 0, cnt1 = 0, cnt2 = 0;
-    for (int64_t p = static_cast<int64_t>(blockIdx.x) * (THREADS / 64) + (threadIdx.x >> 6); p < n; p += nwaves) {
+    for (int64_t p = static_cast<int64_t>(blockIdx.x) * WAVES + wave_uniform(threadIdx.x >> 6); p < n; p += nwaves) {
         const int* bb = bbox + 6 * p;
         if (bb[0] > bb[1]) continue;
-        const int lo0 = bb[0], lo1 = bb[2], lo2 = bb[4];
         const int L0 = bb[1] - bb[0] + 2, L1 = bb[3] - bb[2] + 2, L2 = bb[5] - bb[4] + 2;
         const int c01 = L0 * L1, c02 = L0 * L2, c12 = L1 * L2;
-        const int nl = 2 * (c01 + c02 + c12);
+        const int nl = c01 + c02 + c12;
+        const float inv1 = __builtin_amdgcn_rcpf(static_cast<float>(L1)), inv2 = __builtin_amdgcn_rcpf(static_cast<float>(L2));
         const double inv_h = 1.0 / h[p];
         const double p0 = x[p], p1 = y[p], p2 = z[p];
         Pair2<R>* out = table + loffs[p];
         for (int t = lane; t < nl; t += 64) {
-            // pair blocks (x,y) (x,z) (y,x) (y,z) (z,x) (z,y) of sizes c01 c02 c01 c12 c02 c12
-            int u = t, a, b;
-            if (u < c01) { a = 0; b = 1; }
-            else if ((u -= c01) < c02) { a = 0; b = 2; }
-            else if ((u -= c02) < c01) { a = 1; b = 0; }
-            else if ((u -= c01) < c12) { a = 1; b = 2; }
-            else if ((u -= c12) < c02) { a = 2; b = 0; }
-            else { u -= c02; a = 2; b = 1; }
-            const int Lb = sel3(b, L0, L1, L2);
-            const int ia = u / Lb, ib = u - ia * Lb;
-            const double da = fabs((gmin(g, a) + (sel3(a, lo0, lo1, lo2) + ia) * gstep(g, a) - sel3(a, p0, p1, p2)) * inv_h);
-            const double db = fabs((gmin(g, b) + (sel3(b, lo0, lo1, lo2) + ib) * gstep(g, b) - sel3(b, p0, p1, p2)) * inv_h);
-            R D2, D3;
-            m4_line<R>(static_cast<R>(da), static_cast<R>(db), D2, D3);
+            // pair blocks (x,y) (x,z) (y,z) of sizes c01 c02 c12
+            int u = t, a = 0, b = 1, Lb = L1;
+            float inv_b = inv1;
+            if (u >= c01) {
+                u -= c01;
+                b = 2;
+                Lb = L2;
+                inv_b = inv2;
+                if (u >= c02) {
+                    u -= c02;
+                    a = 1;
+                }
+            }
+            const int ia = small_div(u, Lb, inv_b), ib = u - ia * Lb;
+            const double da = fabs((gmin(g, a) + (sel3(a, bb[0], bb[2], bb[4]) + ia) * gstep(g, a) -
+                                    sel3(a, p0, p1, p2)) * inv_h);
+            const double db = fabs((gmin(g, b) + (sel3(b, bb[0], bb[2], bb[4]) + ib) * gstep(g, b) -
+                                    sel3(b, p0, p1, p2)) * inv_h);
+            R D2ab, D3ab, D2ba, D3ba;
+            m4_line<R>(static_cast<R>(da), static_cast<R>(db), D2ab, D3ab);
+            m4_line<R>(static_cast<R>(db), static_cast<R>(da), D2ba, D3ba);
             const double l2 = da * da + db * db;
-            if (l2 <= 1.0) ++cnt2;
-            else if (l2 <= 4.0) ++cnt1;
-            else ++cnt0;
-            out[t] = Pair2<R>{D2, D3};
+            if (l2 <= 1.0) cnt2 += 2;
+            else if (l2 <= 4.0) cnt1 += 2;
+            else cnt0 += 2;
+            out[t] = Pair2<R>{D2ab + D2ba, D3ab + D3ba};
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -344,16 +358,46 @@ __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __re
 // <= 27 entries, n = cells + 1 per axis).
 template <typename R>
 struct WaveLds {
-    R sd[3 * (BLK + 1)];      // signed offsets (lattice - particle) / h
-    R pc[3][3 * (BLK + 1)];   // plane constants c1, c2, c3 of |sd|
-    R pg2[3 * (BLK + 1)];     // squared gap to the cells on either side of a lattice plane (point test)
-    R cg2[3 * (BLK + 1)];     // squared gap to the cell slab [u, u+1] (cell test; last entry unused)
-    uint32_t list[BLK_PTS];   // needed points: u | i << 10 | j << 14 | k << 18; inside 2h from the
-                              // front, outside from the back
-    R U[BLK_PTS];             // signed lattice values s * W, (i, j, k) k fastest
+    Pair2<R> sg[3 * (BLK + 1)];  // a: signed offset (lattice - particle) / h; b: squared gap to the
+                                 // cells on either side of the lattice plane (point test)
+    R pc[3][3 * (BLK + 1)];      // plane constants c1, c2, c3 of |offset|
+    R cg2[3 * (BLK + 1)];        // squared gap to the cell slab [u, u+1] (cell test; last entry unused)
+    uint32_t list[BLK_PTS];      // needed points: u | i << 10 | j << 14 | k << 18; inside 2h from the
+                                 // front, outside from the back
+    R U[BLK_PTS];                // signed lattice values s * W, (i, j, k) k fastest
 };
 
-__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// Lane-strided walk over an n0 x n1 x n2 box (index u = (i n1 + j) n2 + k): the lane's first point
+// and the (i, j, k) step of 64 points, so that each iteration advances with adds and compares.
+struct BoxWalk {
+    int i, j, k;
+    int di, dj, dk;
+    __device__ __forceinline__ BoxWalk(int lane, int n1, int n2) {
+        const int n12 = n1 * n2;
+        const float r12 = __builtin_amdgcn_rcpf(static_cast<float>(n12)), r2 = __builtin_amdgcn_rcpf(static_cast<float>(n2));
+        i = small_div(lane, n12, r12);
+        const int r = lane - i * n12;
+        j = small_div(r, n2, r2);
+        k = r - j * n2;
+        di = small_div(64, n12, r12);
+        const int s = 64 - di * n12;
+        dj = small_div(s, n2, r2);
+        dk = s - dj * n2;
+    }
+    __device__ __forceinline__ void next(int n1, int n2) {
+        k += dk;
+        j += dj;
+        if (k >= n2) {
+            k -= n2;
+            ++j;
+        }
+        i += di;
+        if (j >= n1) {
+            j -= n1;
+            ++i;
+        }
+    }
+};
 
 template <typename R>
 __global__ __launch_bounds__(THREADS) void k_particles(
@@ -387,7 +431,7 @@ __global__ __launch_bounds__(THREADS) void k_particles(
             const int ox = ib.x0 - bb[0], oy = ib.y0 - bb[2], oz = ib.z0 - bb[4];
             const double inv_h = 1.0 / h[p];
             const double px = x[p], py = y[p], pz = z[p];
-            const Pair2<R>* ltab = lines + loffs[p];
+            const R* etab = reinterpret_cast<const R*>(lines + loffs[p]);
             R* out = ms + slab[t];
 
             // ---- per-axis constants of the item's lattice planes ----
@@ -413,11 +457,10 @@ __global__ __launch_bounds__(THREADS) void k_particles(
                 const R dr = static_cast<R>(d);
                 const M4Plane<R> pl = m4_plane<R>(dr < R(0) ? -dr : dr);
                 const R gp = static_cast<R>(gap(dm, dp)), gc = static_cast<R>(gap(d, dn));
-                L.sd[lane] = dr;
+                L.sg[lane] = Pair2<R>{dr, gp * gp};
                 L.pc[0][lane] = pl.c1;
                 L.pc[1][lane] = pl.c2;
                 L.pc[2][lane] = pl.c3;
-                L.pg2[lane] = gp * gp;
                 L.cg2[lane] = gc * gc;
                 ++c_planes;
             }
@@ -425,28 +468,24 @@ __global__ __launch_bounds__(THREADS) void k_particles(
 
             // ---- classify the lattice points: zero planes, not needed, inside 2h, outside 2h ----
             const int nyz = ny * nz, npts = nx * nyz;
-            const float inv_nyz = 1.0f / static_cast<float>(nyz), inv_nz = 1.0f / static_cast<float>(nz);
             uint32_t nA = 0, nB = 0;
-            for (int u0 = 0; u0 < npts; u0 += 64) {
+            BoxWalk w(lane, ny, nz);
+            for (int u0 = 0; u0 < npts; u0 += 64, w.next(ny, nz)) {
                 const int u = u0 + lane;
-                const int i = small_div(u, nyz, inv_nyz);
-                const int rem = u - i * nyz;
-                const int j = small_div(rem, nz, inv_nz);
-                const int k = rem - j * nz;
                 bool isA = false, isB = false;
                 if (u < npts) {
-                    const R dx = L.sd[i], dy = L.sd[nx + j], dz = L.sd[nx + ny + k];
-                    if (dx == R(0) || dy == R(0) || dz == R(0)) {
+                    const Pair2<R> ex = L.sg[w.i], ey = L.sg[nx + w.j], ez = L.sg[nx + ny + w.k];
+                    if (ex.a == R(0) || ey.a == R(0) || ez.a == R(0)) {
                         L.U[u] = R(0);  // every pyramid anchored in a plane through the particle is flat
-                    } else if (L.pg2[i] + L.pg2[nx + j] + L.pg2[nx + ny + k] < R(4)) {
-                        const R r2 = dx * dx + dy * dy + dz * dz;
+                    } else if (ex.b + ey.b + ez.b < R(4)) {
+                        const R r2 = ex.a * ex.a + ey.a * ey.a + ez.a * ez.a;
                         isA = r2 < R(4);
                         isB = !isA;
                     }
                 }
                 const unsigned long long bA = __ballot(isA), bB = __ballot(isB);
-                const uint32_t code = static_cast<uint32_t>(u) | (static_cast<uint32_t>(i) << 10) |
-                                      (static_cast<uint32_t>(j) << 14) | (static_cast<uint32_t>(k) << 18);
+                const uint32_t code = static_cast<uint32_t>(u) | (static_cast<uint32_t>(w.i) << 10) |
+                                      (static_cast<uint32_t>(w.j) << 14) | (static_cast<uint32_t>(w.k) << 18);
                 if (isA) L.list[nA + __popcll(bA & lane_lt)] = code;
                 if (isB) L.list[BLK_PTS - 1 - (nB + __popcll(bB & lane_lt))] = code;
                 nA += __popcll(bA);
@@ -455,15 +494,15 @@ __global__ __launch_bounds__(THREADS) void k_particles(
             wave_sync();
 
             // ---- lattice values of the needed points (inside 2h first, then outside) ----
-            const int pxy = 0, pxz = Lx * Ly, pyx = pxz + Lx * Lz, pyz = pyx + Ly * Lx, pzx = pyz + Ly * Lz,
-                      pzy = pzx + Lz * Lx;
+            // line-pair sums: blocks (x,y) (x,z) (y,z) of the particle's table; component 0 is E2, 1 is E3
+            const int exz = Lx * Ly, eyz = exz + Lx * Lz;
             const uint32_t ntot = nA + nB;
             for (uint32_t v = lane; v < ntot; v += 64) {
                 const uint32_t code = (v < nA) ? L.list[v] : L.list[BLK_PTS - 1 - (v - nA)];
                 const int u = static_cast<int>(code & 1023u);
                 const int i = static_cast<int>((code >> 10) & 15u), j = static_cast<int>((code >> 14) & 15u),
                           k = static_cast<int>((code >> 18) & 15u);
-                const R dx = L.sd[i], dy = L.sd[nx + j], dz = L.sd[nx + ny + k];
+                const R dx = L.sg[i].a, dy = L.sg[nx + j].a, dz = L.sg[nx + ny + k].a;
                 const R ax = dx < R(0) ? -dx : dx, ay = dy < R(0) ? -dy : dy, az = dz < R(0) ? -dz : dz;
                 const R r2 = ax * ax + ay * ay + az * az;
                 const R r = KMath<R>::sqrt_(r2);
@@ -473,14 +512,10 @@ __global__ __launch_bounds__(THREADS) void k_particles(
                 R dsum = R(0);
                 if (reg != 1) {
                     const int gi = ox + i, gj = oy + j, gk = oz + k;  // particle-lattice coordinates
-                    const Pair2<R> l0 = ltab[static_cast<uint32_t>(pxy + gi * Ly + gj)];
-                    const Pair2<R> l1 = ltab[static_cast<uint32_t>(pxz + gi * Lz + gk)];
-                    const Pair2<R> l2 = ltab[static_cast<uint32_t>(pyx + gj * Lx + gi)];
-                    const Pair2<R> l3 = ltab[static_cast<uint32_t>(pyz + gj * Lz + gk)];
-                    const Pair2<R> l4 = ltab[static_cast<uint32_t>(pzx + gk * Lx + gi)];
-                    const Pair2<R> l5 = ltab[static_cast<uint32_t>(pzy + gk * Ly + gj)];
-                    dsum = (reg == 2) ? (((l0.a + l1.a) + (l2.a + l3.a)) + (l4.a + l5.a))
-                                      : (((l0.b + l1.b) + (l2.b + l3.b)) + (l4.b + l5.b));
+                    const uint32_t c = static_cast<uint32_t>(reg - 2);
+                    dsum = (etab[2u * static_cast<uint32_t>(gi * Ly + gj) + c] +
+                            etab[2u * static_cast<uint32_t>(exz + gi * Lz + gk) + c]) +
+                           etab[2u * static_cast<uint32_t>(eyz + gj * Lz + gk) + c];
                 }
                 const R W = m4_lattice_free<R>(ax, ay, az, r, reg) * M4Consts<R>::inv_pi + R(0.5) * csum + dsum;
                 const bool neg = (dx < R(0)) != ((dy < R(0)) != (dz < R(0)));
@@ -489,13 +524,10 @@ __global__ __launch_bounds__(THREADS) void k_particles(
             wave_sync();
 
             // ---- cell sums: the 8-corner stencil, stored to the item's slab ----
-            const int myz = my * mz, ncells = mx * myz;
-            const float inv_myz = 1.0f / static_cast<float>(myz), inv_mz = 1.0f / static_cast<float>(mz);
-            for (int u = lane; u < ncells; u += 64) {
-                const int i = small_div(u, myz, inv_myz);
-                const int rem = u - i * myz;
-                const int j = small_div(rem, mz, inv_mz);
-                const int k = rem - j * mz;
+            const int ncells = mx * my * mz;
+            BoxWalk wc(lane, my, mz);
+            for (int u = lane; u < ncells; u += 64, wc.next(my, mz)) {
+                const int i = wc.i, j = wc.j, k = wc.k;
                 double Ms = 0.0;
                 if (L.cg2[i] + L.cg2[nx + j] + L.cg2[nx + ny + k] < R(4)) {
                     ++n_sphere;
@@ -595,25 +627,27 @@ __global__ __launch_bounds__(THREADS) void k_accum(
         }
         __syncthreads();
         for (int g0 = 0; g0 < cnt; g0 += ACC_GROUP) {
-            // load the cell sums of a group of items first (independent loads in flight) ...
-            double val[ACC_GROUP][ACC_CPT];
+            // issue the loads of a group of items first (independent loads in flight) ...
+            // Every load is issued unconditionally from an in-bounds address (coordinates outside the
+            // item are clamped into it) and masked afterwards: no branch separates a load from its use,
+            // so the group's loads stay in flight together.
+            R raw[ACC_GROUP][ACC_CPT];
+            bool ok[ACC_GROUP][ACC_CPT];
 #pragma unroll
             for (int u = 0; u < ACC_GROUP; ++u) {
-                const int q = g0 + u;
-#pragma unroll
-                for (int s = 0; s < ACC_CPT; ++s) val[u][s] = 0.0;
-                if (q >= cnt) continue;
+                const int q = min(g0 + u, cnt - 1);
                 // item geometry is the same for every lane: scalar registers
-                const uint32_t bx_ = wave_uniform(A.box[q]);
-                const int ox = bx_ & 15, oy = (bx_ >> 4) & 15, oz = (bx_ >> 8) & 15;
-                const int mx = (bx_ >> 12) & 15, my = (bx_ >> 16) & 15, mz = (bx_ >> 20) & 15;
+                const uint32_t bq = wave_uniform(A.box[q]);
+                const int ox = bq & 15, oy = (bq >> 4) & 15, oz = (bq >> 8) & 15;
+                const int mx = (bq >> 12) & 15, my = (bq >> 16) & 15, mz = (bq >> 20) & 15;
                 const int ly = cy - oy, lz = cz - oz;
-                if (ly < 0 || ly >= my || lz < 0 || lz >= mz) continue;
-                const R* row = ms + A.off[q] + (ly * mz + lz);
+                const bool in_yz = (g0 + u < cnt) && ly >= 0 && ly < my && lz >= 0 && lz < mz;
+                const R* row = ms + A.off[q] + (min(max(ly, 0), my - 1) * mz + min(max(lz, 0), mz - 1));
 #pragma unroll
                 for (int s = 0; s < ACC_CPT; ++s) {
                     const int lx = cx0 + s - ox;
-                    if (lx >= 0 && lx < mx) val[u][s] = static_cast<double>(row[lx * my * mz]);
+                    ok[u][s] = in_yz && lx >= 0 && lx < mx;
+                    raw[u][s] = row[min(max(lx, 0), mx - 1) * my * mz];
                 }
             }
             // ... then add them in list order
@@ -624,7 +658,7 @@ __global__ __launch_bounds__(THREADS) void k_accum(
                 const double mm = A.m[q], v0 = A.v[0][q], v1 = A.v[1][q], v2 = A.v[2][q];
 #pragma unroll
                 for (int s = 0; s < ACC_CPT; ++s) {
-                    const double Ms = val[u][s];
+                    const double Ms = ok[u][s] ? static_cast<double>(raw[u][s]) : 0.0;
                     if (Ms == 0.0) continue;
                     const double t = Ms * mm;
                     acc[s][0] += t;
