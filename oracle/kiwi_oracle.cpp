@@ -566,6 +566,54 @@ typename Ev::scalar face_walk(typename Ev::scalar x0, typename Ev::scalar y0, ty
     return total;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Operation count of the cubic path (measurement only): the same templates over a scalar that
+// tallies what it computes.  Convention: every +, -, *, /, sqrt and transcendental call counts one
+// operation (sincos counts two); comparisons, fabs and selects count none; constants such as 3/40
+// are literals in the reference and cost nothing.
+// ------------------------------------------------------------------------------------------------
+struct OpTally {
+    long long add = 0, mul = 0, div = 0, sqrt = 0, trans = 0;
+};
+thread_local OpTally g_tally;
+
+struct OpCount {
+    double v;
+    OpCount() : v(0) {}
+    OpCount(double x) : v(x) {}
+};
+inline OpCount operator+(OpCount a, OpCount b) { ++g_tally.add; return OpCount(a.v + b.v); }
+inline OpCount operator-(OpCount a, OpCount b) { ++g_tally.add; return OpCount(a.v - b.v); }
+inline OpCount operator*(OpCount a, OpCount b) { ++g_tally.mul; return OpCount(a.v * b.v); }
+inline OpCount operator/(OpCount a, OpCount b) { ++g_tally.div; return OpCount(a.v / b.v); }
+inline OpCount operator-(OpCount a) { return OpCount(-a.v); }
+inline OpCount& operator+=(OpCount& a, OpCount b) { return a = a + b; }
+inline bool operator<(OpCount a, OpCount b) { return a.v < b.v; }
+inline bool operator<=(OpCount a, OpCount b) { return a.v <= b.v; }
+inline bool operator>(OpCount a, OpCount b) { return a.v > b.v; }
+inline bool operator>=(OpCount a, OpCount b) { return a.v >= b.v; }
+inline bool operator==(OpCount a, OpCount b) { return a.v == b.v; }
+inline bool operator!=(OpCount a, OpCount b) { return a.v != b.v; }
+inline OpCount q_sqrt(OpCount x) { ++g_tally.sqrt; return OpCount(std::sqrt(x.v)); }
+inline OpCount q_acos(OpCount x) { ++g_tally.trans; return OpCount(std::acos(x.v)); }
+inline OpCount q_log(OpCount x) { ++g_tally.trans; return OpCount(std::log(x.v)); }
+inline OpCount q_atan(OpCount x) { ++g_tally.trans; return OpCount(std::atan(x.v)); }
+inline OpCount q_sin(OpCount x) { ++g_tally.trans; return OpCount(std::sin(x.v)); }
+inline OpCount q_fabs(OpCount x) { return OpCount(std::fabs(x.v)); }
+template <> inline OpCount q_pi<OpCount>() { return OpCount(M_PI); }
+template <> inline OpCount C<OpCount>(int n, int d) { return OpCount(static_cast<double>(n) / d); }
+inline void sincos_pair(OpCount x, OpCount& s, OpCount& c) {
+    g_tally.trans += 2;
+    double a, b;
+    sincos_pair(x.v, a, b);
+    s = OpCount(a);
+    c = OpCount(b);
+}
+template <>
+OpCount EvalM4<OpCount>::at(OpCount phi) const {
+    return m4_vertex(w, e, phi);
+}
+
 // box_cell_faces, src/cartesian_grid.cpp:54-72: the 6 faces of a box, in the winding the face walk needs.
 void box_faces(double cx, double cy, double cz, double hx, double hy, double hz, double f[6][12]) {
     const double xm = cx - hx, xp = cx + hx, ym = cy - hy, yp = cy + hy, zm = cz - hz, zp = cz + hz;
@@ -895,6 +943,84 @@ int oracle_voronoi_map(long long n, const double* x, const double* y, const doub
         counts[4] = visited;
         counts[5] = face_evals;
     }
+    return 0;
+}
+
+// One cubic face integral evaluated over the counting scalar: its value (which must equal
+// oracle_face's, so the counted path is the computed path) and its operations (add, mul, div, sqrt,
+// transcendental).
+double oracle_face_counted(double x0, double y0, double z0, double h, const double* verts, int nverts,
+                           long long* counts) {
+    g_tally = OpTally();
+    EvalM4<OpCount> ev{OpCount(h), nullptr, {}, {}};
+    const OpCount v = face_walk(OpCount(x0), OpCount(y0), OpCount(z0), OpCount(2.0 * h), verts, nverts, nullptr, ev);
+    counts[0] = g_tally.add;
+    counts[1] = g_tally.mul;
+    counts[2] = g_tally.div;
+    counts[3] = g_tally.sqrt;
+    counts[4] = g_tally.trans;
+    return v.v;
+}
+
+// Operations of the reference's cubic cell loop (src/cartesian_grid.cpp:239-279 with the face
+// integrals of src/kernel_integral.cpp:471-647) over every bounding-box pair of the given particles,
+// FP64 path.  out: pairs, add, mul, div, sqrt, transcendental.  Per pair it also counts the 6 box-face
+// coordinates (6 adds), the cell-centre z coordinate (2 adds, 1 mul) and the four accumulations of
+// Msum m and Msum m v (4 adds, 4 muls).
+int oracle_count_m4(long long n, const double* x, const double* y, const double* z, const double* h,
+                    const double* domain, int nx, int ny, int nz, int threads, long long* out) {
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !out) return -1;
+    const double xmin = domain[0], ymin = domain[2], zmin = domain[4];
+    const double dpx = (domain[1] - domain[0]) / nx, dpy = (domain[3] - domain[2]) / ny,
+                 dpz = (domain[5] - domain[4]) / nz;
+    const double hpx = 0.5 * dpx, hpy = 0.5 * dpy, hpz = 0.5 * dpz;
+    long long pairs = 0, add = 0, mul = 0, dv = 0, sq = 0, tr = 0;
+    int nt = threads > 0 ? threads : 1;
+#ifndef _OPENMP
+    nt = 1;
+#endif
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1) reduction(+ : pairs, add, mul, dv, sq, tr)
+#endif
+    for (long long i = 0; i < n; ++i) {
+        const double x0 = x[i], y0 = y[i], z0 = z[i], hh = h[i];
+        if (hh <= 0.0) continue;
+        const double rad = 2.0 * hh;
+        const int ix0 = std::max(static_cast<int>(std::floor((x0 - rad - xmin) / dpx)), 0);
+        const int ix1 = std::min(static_cast<int>(std::floor((x0 + rad - xmin) / dpx)), nx - 1);
+        const int iy0 = std::max(static_cast<int>(std::floor((y0 - rad - ymin) / dpy)), 0);
+        const int iy1 = std::min(static_cast<int>(std::floor((y0 + rad - ymin) / dpy)), ny - 1);
+        const int iz0 = std::max(static_cast<int>(std::floor((z0 - rad - zmin) / dpz)), 0);
+        const int iz1 = std::min(static_cast<int>(std::floor((z0 + rad - zmin) / dpz)), nz - 1);
+        if (ix0 > ix1 || iy0 > iy1 || iz0 > iz1) continue;
+        g_tally = OpTally();
+        long long np = 0;
+        for (int ix = ix0; ix <= ix1; ++ix)
+            for (int iy = iy0; iy <= iy1; ++iy)
+                for (int iz = iz0; iz <= iz1; ++iz) {
+                    double f[6][12];
+                    box_faces(xmin + (ix + 0.5) * dpx, ymin + (iy + 0.5) * dpy, zmin + (iz + 0.5) * dpz, hpx, hpy,
+                              hpz, f);
+                    OpCount Ms = 0.0;
+                    for (int q = 0; q < 6; ++q) {
+                        EvalM4<OpCount> ev{OpCount(hh), nullptr, {}, {}};
+                        Ms += face_walk(OpCount(x0), OpCount(y0), OpCount(z0), OpCount(2.0 * hh), f[q], 4, nullptr, ev);
+                    }
+                    ++np;
+                }
+        pairs += np;
+        add += g_tally.add + np * (6 + 2 + 4);
+        mul += g_tally.mul + np * (1 + 4);
+        dv += g_tally.div;
+        sq += g_tally.sqrt;
+        tr += g_tally.trans;
+    }
+    out[0] = pairs;
+    out[1] = add;
+    out[2] = mul;
+    out[3] = dv;
+    out[4] = sq;
+    out[5] = tr;
     return 0;
 }
 

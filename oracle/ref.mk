@@ -5,7 +5,7 @@
 #   make -f oracle/ref.mk            -> oracle/_ref/kiwi (the reference CLI)
 #                                       oracle/_ref/libkiwi_ref.so (reference kernel math + our C shim)
 #
-# Nothing here runs on the GPU box: the reference never leaves this container.
+# The reference build stays on the build machine: no GPU test, smoke test or benchmark loads it.
 
 REF      ?= /root/reference
 OUT      ?= oracle/_ref

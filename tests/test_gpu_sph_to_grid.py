@@ -9,8 +9,8 @@ Tolerances (FP64 mode):
       unweighted velocity comparison is ill-posed in cells whose mass is a small residue: there the
       velocity is a ratio of two tiny sums, and the reference's own sums carry absolute errors up to
       ~1e-11 of a particle for particles within ~1e-3 h of a grid plane (acos of arguments near 1,
-      src/kernel_integral.cpp:593-594; tests/test_lattice_math.py pins such a case against 40-digit
-      arithmetic, where the HIP path's value is the accurate one).
+      src/kernel_integral.cpp:593-594; tests/test_lattice_math.py checks the lattice restatement
+      against the reference algorithm evaluated in binary128, particles on grid planes included).
 Mixed mode: relative total-mass error <= 1e-6 against FP64, per-field error reported.
 """
 import os

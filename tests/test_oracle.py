@@ -233,3 +233,35 @@ def test_derived_tables_equal_reference_tables():
         np.testing.assert_array_equal([float(n2), float(n3), radkern, hfact], n4)
         np.testing.assert_array_equal(np.array([[float(v) for v in row] for row in A]).reshape(-1), a)
         np.testing.assert_array_equal(np.array([[[float(v) for v in r] for r in blk] for blk in B]).reshape(-1), b)
+
+
+def test_operation_count_follows_the_computed_path():
+    """The counting instantiation behind the reference operation count (oracle/flops_per_pair.py)
+    evaluates the same face integrals as the oracle, bit for bit, and counts a plausible number of
+    operations per cell (SURVEY-style static estimate: 7.9k-12.1k per pair)."""
+    L = O.lib()
+    L.oracle_face_counted.restype = ctypes.c_double
+    L.oracle_face_counted.argtypes = [ctypes.c_double] * 4 + [ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                                              ctypes.POINTER(ctypes.c_longlong)]
+    k = O.kernel("M4")
+    rng = np.random.default_rng(11)
+    per_cell = []
+    for _ in range(200):
+        p = rng.uniform(-1, 1, 3)
+        h = rng.uniform(0.3, 1.5)
+        lo = p + rng.uniform(-2.5, 2.0, 3) * h
+        hi = lo + rng.uniform(0.1, 0.6, 3) * h
+        tot = 0
+        for f in box_faces(lo, hi):
+            v = np.ascontiguousarray(f, dtype=np.float64)
+            cnt = (ctypes.c_longlong * 5)()
+            val = L.oracle_face_counted(p[0], p[1], p[2], h, O._p(v), 4, cnt)
+            assert val == O.face(k, p, h, v)[0]
+            tot += sum(cnt)
+        per_cell.append(tot)
+    assert 7000 < np.mean(per_cell) < 13000, np.mean(per_cell)
+    # a particle in the face plane: every pyramid of the face is flat and the tiers are skipped
+    v = np.ascontiguousarray([0.5, -0.5, 0.0, 0.5, 0.5, 0.0, -0.5, 0.5, 0.0, -0.5, -0.5, 0.0])
+    cnt = (ctypes.c_longlong * 5)()
+    assert L.oracle_face_counted(0.1, 0.2, 0.0, 1.0, O._p(v), 4, cnt) == 0.0
+    assert cnt[4] == 16  # only the face walk's own calls remain: 2 acos + 2 sin per edge, 4 edges
