@@ -40,6 +40,7 @@
 #include <string>
 
 #include "../../include/kiwi_hip.h"
+#include "gen_lattice.h"
 #include "m4_lattice.h"
 
 using namespace kiwi_hip;
@@ -284,18 +285,105 @@ __global__ void k_block_work(int64_t npairs, const uint64_t* __restrict__ keys, 
 }
 
 // ------------------------------------------------------------------------------------------------
+// Kernel policies: what the line and cell kernels need to know about the kernel function
+// ------------------------------------------------------------------------------------------------
+// M4 analytic (m4_lattice.h): lattice offsets in units of h, support 2; regions inside h, inside 2h,
+// outside; plane constants c1..c3; the line table holds the region-2 and region-3 offsets (region 1
+// has none).  R = double is the parity mode, R = float the mixed mode.
+template <typename R>
+struct PolicyM4 {
+    using Real = R;
+    static constexpr int NPL = 3;  // plane values per lattice coordinate
+    static constexpr int NLN = 2;  // line-table values per axis pair
+    struct Ctx {
+        double support_in_h;  // support / h = 2
+    };
+    static __device__ __forceinline__ double unit_in_h(const Ctx&) { return 1.0; }  // lattice unit: h
+    static __device__ __forceinline__ R support2() { return R(4); }
+    static __device__ __forceinline__ int region(const Ctx&, R r2) { return m4_region(r2); }
+    static __device__ __forceinline__ void plane(const Ctx&, R ad, R* v) {
+        const M4Plane<R> pl = m4_plane<R>(ad);
+        v[0] = pl.c1;
+        v[1] = pl.c2;
+        v[2] = pl.c3;
+    }
+    // line-table slot of region k, -1: none
+    static __device__ __forceinline__ int line_slot(int k) { return k - 2; }
+    static __device__ __forceinline__ void line(const Ctx&, R da, R db, R* v) { m4_line<R>(da, db, v[0], v[1]); }
+    static __device__ __forceinline__ int line_kind(double l2) { return l2 <= 1.0 ? 2 : (l2 <= 4.0 ? 1 : 0); }
+    // the lattice value without its plane and line terms, and the weight of the plane sum
+    static __device__ __forceinline__ R free_part(const Ctx&, R ax, R ay, R az, R r, int k) {
+        return m4_lattice_free<R>(ax, ay, az, r, k) * M4Consts<R>::inv_pi;
+    }
+    static __device__ __forceinline__ R plane_weight() { return R(0.5); }
+};
+
+// Generic tabulated-coefficient kernels (gen_lattice.h), double precision: lattice offsets in units of
+// the support H, support 1; regions of the kernel table; plane values Phi_k; the line table holds
+// the offsets of every region.
+struct GenTab {  // the fields gen_lattice.h reads, copied from a KiwiKernelTable
+    int nreg;
+    double c[KIWI_TAB_MAXREG];
+    double A3_pi;
+    double a[KIWI_TAB_MAXREG][KIWI_TAB_NI];
+    double B[KIWI_TAB_MAXREG][KIWI_TAB_MAXREG][KIWI_TAB_NB];
+};
+
+struct PolicyGen {
+    using Real = double;
+    static constexpr int NPL = GEN_NREG;
+    static constexpr int NLN = GEN_NREG;
+    struct Ctx {
+        double support_in_h;  // H / h
+        const GenTab* tab;
+    };
+    static __device__ __forceinline__ double unit_in_h(const Ctx& c) { return c.support_in_h; }  // lattice unit: H
+    static __device__ __forceinline__ double support2() { return 1.0; }
+    static __device__ __forceinline__ int region(const Ctx& c, double r2) { return gen_region_of(*c.tab, sqrt(r2)); }
+    static __device__ __forceinline__ void plane(const Ctx& c, double ad, double* v) { gen_plane_phi(*c.tab, ad, v); }
+    static __device__ __forceinline__ int line_slot(int k) { return k - 1; }
+    static __device__ __forceinline__ void line(const Ctx& c, double da, double db, double* v) { gen_line(*c.tab, da, db, v); }
+    static __device__ __forceinline__ int line_kind(double l2) { return l2 < 1.0 ? 1 : 0; }
+    static __device__ __forceinline__ double free_part(const Ctx& c, double ax, double ay, double az, double r, int k) {
+        const double d[3] = {ax, ay, az};
+        double s = 0.0;
+        // T_b for edge normal b, atanh(d_c / r) for along-edge axis c; the 6 ordered (a, b) pyramids
+        const double T0 = atan(ay * az / (ax * r)), T1 = atan(ax * az / (ay * r)), T2 = atan(ax * ay / (az * r));
+        const double K0 = 0.5 * log((r + ax) * (r + ax) / (ay * ay + az * az));
+        const double K1 = 0.5 * log((r + ay) * (r + ay) / (ax * ax + az * az));
+        const double K2 = 0.5 * log((r + az) * (r + az) / (ax * ax + ay * ay));
+        s += gen_pyramid_nonphi(*c.tab, k, d[0], d[1], d[2], r, T1, K2);  // (x, y, z)
+        s += gen_pyramid_nonphi(*c.tab, k, d[0], d[2], d[1], r, T2, K1);  // (x, z, y)
+        s += gen_pyramid_nonphi(*c.tab, k, d[1], d[0], d[2], r, T0, K2);  // (y, x, z)
+        s += gen_pyramid_nonphi(*c.tab, k, d[1], d[2], d[0], r, T2, K0);  // (y, z, x)
+        s += gen_pyramid_nonphi(*c.tab, k, d[2], d[0], d[1], r, T0, K1);  // (z, x, y)
+        s += gen_pyramid_nonphi(*c.tab, k, d[2], d[1], d[0], r, T1, K0);  // (z, y, x)
+        return s;
+    }
+    static __device__ __forceinline__ double plane_weight() { return 0.5 * GEN_PI; }
+};
+
+template <typename R, int N>
+struct LineVals {
+    R v[N];
+};
+
+// ------------------------------------------------------------------------------------------------
 // 6. edge-line constants, once per particle
 // ------------------------------------------------------------------------------------------------
 // Table layout per particle (lattice extents L = bbox cells + 1 per axis): three unordered axis pairs
-// (x,y) (x,z) (y,z), each La x Lb with the a-index major.  Entry (ia, ib) of pair {a, b} holds the sums
-//   E2 = D2(d_a, d_b) + D2(d_b, d_a),   E3 = D3(d_a, d_b) + D3(d_b, d_a)
-// of the two edge lines through lattice coordinates (lo_a + ia, lo_b + ib): wall a with edge normal b,
-// and wall b with edge normal a.  A lattice point in region k adds E_k of its three axis pairs.
-template <typename R>
+// (x,y) (x,z) (y,z), each La x Lb with the a-index major.  Entry (ia, ib) of pair {a, b} holds, per
+// line-table slot, the sum of the region offsets of the two edge lines through lattice coordinates
+// (lo_a + ia, lo_b + ib): wall a with edge normal b, and wall b with edge normal a.  A lattice point in
+// region k adds slot k of its three axis pairs.
+template <class P>
 __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                                                    const double* __restrict__ z, const double* __restrict__ h,
                                                    const int* __restrict__ bbox, const int64_t* __restrict__ loffs,
-                                                   GridGeom g, Pair2<R>* __restrict__ table, DevStats* st) {
+                                                   GridGeom g, typename P::Ctx ctx,
+                                                   LineVals<typename P::Real, P::NLN>* __restrict__ table,
+                                                   DevStats* st) {
+    using R = typename P::Real;
     // grid-stride over particles, one wavefront per particle at a time; counters stay in registers
     const int lane = threadIdx.x & 63;
     const int64_t nwaves = static_cast<int64_t>(gridDim.x) * WAVES;
@@ -307,9 +395,9 @@ __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __re
         const int c01 = L0 * L1, c02 = L0 * L2, c12 = L1 * L2;
         const int nl = c01 + c02 + c12;
         const float inv1 = __builtin_amdgcn_rcpf(static_cast<float>(L1)), inv2 = __builtin_amdgcn_rcpf(static_cast<float>(L2));
-        const double inv_h = 1.0 / h[p];
+        const double inv_s = 1.0 / (P::unit_in_h(ctx) * h[p]);  // lattice units
         const double p0 = x[p], p1 = y[p], p2 = z[p];
-        Pair2<R>* out = table + loffs[p];
+        LineVals<R, P::NLN>* out = table + loffs[p];
         for (int t = lane; t < nl; t += 64) {
             // pair blocks (x,y) (x,z) (y,z) of sizes c01 c02 c12
             int u = t, a = 0, b = 1, Lb = L1;
@@ -326,17 +414,20 @@ __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __re
             }
             const int ia = small_div(u, Lb, inv_b), ib = u - ia * Lb;
             const double da = fabs((gmin(g, a) + (sel3(a, bb[0], bb[2], bb[4]) + ia) * gstep(g, a) -
-                                    sel3(a, p0, p1, p2)) * inv_h);
+                                    sel3(a, p0, p1, p2)) * inv_s);
             const double db = fabs((gmin(g, b) + (sel3(b, bb[0], bb[2], bb[4]) + ib) * gstep(g, b) -
-                                    sel3(b, p0, p1, p2)) * inv_h);
-            R D2ab, D3ab, D2ba, D3ba;
-            m4_line<R>(static_cast<R>(da), static_cast<R>(db), D2ab, D3ab);
-            m4_line<R>(static_cast<R>(db), static_cast<R>(da), D2ba, D3ba);
-            const double l2 = da * da + db * db;
-            if (l2 <= 1.0) cnt2 += 2;
-            else if (l2 <= 4.0) cnt1 += 2;
-            else cnt0 += 2;
-            out[t] = Pair2<R>{D2ab + D2ba, D3ab + D3ba};
+                                    sel3(b, p0, p1, p2)) * inv_s);
+            R vab[P::NLN], vba[P::NLN];
+            P::line(ctx, static_cast<R>(da), static_cast<R>(db), vab);
+            P::line(ctx, static_cast<R>(db), static_cast<R>(da), vba);
+            const int kind = P::line_kind(da * da + db * db);
+            cnt2 += (kind == 2) ? 2 : 0;
+            cnt1 += (kind == 1) ? 2 : 0;
+            cnt0 += (kind == 0) ? 2 : 0;
+            LineVals<R, P::NLN> e;
+#pragma unroll
+            for (int q = 0; q < P::NLN; ++q) e.v[q] = vab[q] + vba[q];
+            out[t] = e;
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -356,14 +447,14 @@ __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __re
 // ------------------------------------------------------------------------------------------------
 // Per-wave LDS.  Axis arrays hold the item's lattice coordinates x..., y..., z... (n_x + n_y + n_z
 // <= 27 entries, n = cells + 1 per axis).
-template <typename R>
+template <typename R, int NPL>
 struct WaveLds {
-    Pair2<R> sg[3 * (BLK + 1)];  // a: signed offset (lattice - particle) / h; b: squared gap to the
+    Pair2<R> sg[3 * (BLK + 1)];  // a: signed offset (lattice - particle) / scale; b: squared gap to the
                                  // cells on either side of the lattice plane (point test)
-    R pc[3][3 * (BLK + 1)];      // plane constants c1, c2, c3 of |offset|
+    R pc[NPL][3 * (BLK + 1)];    // plane values of |offset| (M4: c1..c3; generic: Phi_1..Phi_4)
     R cg2[3 * (BLK + 1)];        // squared gap to the cell slab [u, u+1] (cell test; last entry unused)
-    uint32_t list[BLK_PTS];      // needed points: u | i << 10 | j << 14 | k << 18; inside 2h from the
-                                 // front, outside from the back
+    uint32_t list[BLK_PTS];      // needed points: u | i << 10 | j << 14 | k << 18; inside the support
+                                 // from the front, outside from the back
     R U[BLK_PTS];                // signed lattice values s * W, (i, j, k) k fastest
 };
 
@@ -399,16 +490,20 @@ struct BoxWalk {
     }
 };
 
-template <typename R>
+template <class P>
 __global__ __launch_bounds__(THREADS) void k_particles(
     uint32_t npairs, const uint64_t* __restrict__ keys, const int64_t* __restrict__ slab,
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     const double* __restrict__ h, const int* __restrict__ bbox, const int64_t* __restrict__ loffs, GridGeom g,
-    const Pair2<R>* __restrict__ lines, R* __restrict__ ms, DevStats* st, int clamp_pairs) {
-    __shared__ WaveLds<R> WL[WAVES];
+    typename P::Ctx ctx, const LineVals<typename P::Real, P::NLN>* __restrict__ lines, typename P::Real* __restrict__ ms,
+    DevStats* st, int clamp_pairs) {
+    using R = typename P::Real;
+    constexpr int NLN = P::NLN;
+    __shared__ WaveLds<R, P::NPL> WL[WAVES];
     const int wave = static_cast<int>(wave_uniform(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
-    WaveLds<R>& L = WL[wave];
+    WaveLds<R, P::NPL>& L = WL[wave];
+    const R S2 = P::support2();
     unsigned long long n_neg = 0, n_warn = 0, n_sphere = 0, c_planes = 0;
     unsigned long long c_pts[3] = {0, 0, 0};
     const unsigned long long lane_lt = (1ull << lane) - 1ull;
@@ -429,7 +524,7 @@ __global__ __launch_bounds__(THREADS) void k_particles(
             const int nx = mx + 1, ny = my + 1, nz = mz + 1;  // item lattice
             const int Lx = bb[1] - bb[0] + 2, Ly = bb[3] - bb[2] + 2, Lz = bb[5] - bb[4] + 2;  // particle lattice
             const int ox = ib.x0 - bb[0], oy = ib.y0 - bb[2], oz = ib.z0 - bb[4];
-            const double inv_h = 1.0 / h[p];
+            const double inv_s = 1.0 / (P::unit_in_h(ctx) * h[p]);  // lattice units
             const double px = x[p], py = y[p], pz = z[p];
             const R* etab = reinterpret_cast<const R*>(lines + loffs[p]);
             R* out = ms + slab[t];
@@ -450,23 +545,23 @@ __global__ __launch_bounds__(THREADS) void k_particles(
                 // offsets formed as the reference forms vertex coordinates: (x_min + i dx) - x_p
                 const int g0 = sel3(a, ib.x0, ib.y0, ib.z0);
                 const double lo = gmin(g, a), pa = sel3(a, px, py, pz), st_a = gstep(g, a);
-                const double d = (lo + (g0 + u) * st_a - pa) * inv_h;
-                const double dm = (lo + (g0 + max(u - 1, 0)) * st_a - pa) * inv_h;
-                const double dp = (lo + (g0 + min(u + 1, n_a - 1)) * st_a - pa) * inv_h;
-                const double dn = (lo + (g0 + u + 1) * st_a - pa) * inv_h;
+                const double d = (lo + (g0 + u) * st_a - pa) * inv_s;
+                const double dm = (lo + (g0 + max(u - 1, 0)) * st_a - pa) * inv_s;
+                const double dp = (lo + (g0 + min(u + 1, n_a - 1)) * st_a - pa) * inv_s;
+                const double dn = (lo + (g0 + u + 1) * st_a - pa) * inv_s;
                 const R dr = static_cast<R>(d);
-                const M4Plane<R> pl = m4_plane<R>(dr < R(0) ? -dr : dr);
+                R pv[P::NPL];
+                P::plane(ctx, dr < R(0) ? -dr : dr, pv);
                 const R gp = static_cast<R>(gap(dm, dp)), gc = static_cast<R>(gap(d, dn));
                 L.sg[lane] = Pair2<R>{dr, gp * gp};
-                L.pc[0][lane] = pl.c1;
-                L.pc[1][lane] = pl.c2;
-                L.pc[2][lane] = pl.c3;
+#pragma unroll
+                for (int q = 0; q < P::NPL; ++q) L.pc[q][lane] = pv[q];
                 L.cg2[lane] = gc * gc;
                 ++c_planes;
             }
             wave_sync();
 
-            // ---- classify the lattice points: zero planes, not needed, inside 2h, outside 2h ----
+            // ---- classify the lattice points: zero planes, not needed, inside the support, outside ----
             const int nyz = ny * nz, npts = nx * nyz;
             uint32_t nA = 0, nB = 0;
             BoxWalk w(lane, ny, nz);
@@ -477,9 +572,9 @@ __global__ __launch_bounds__(THREADS) void k_particles(
                     const Pair2<R> ex = L.sg[w.i], ey = L.sg[nx + w.j], ez = L.sg[nx + ny + w.k];
                     if (ex.a == R(0) || ey.a == R(0) || ez.a == R(0)) {
                         L.U[u] = R(0);  // every pyramid anchored in a plane through the particle is flat
-                    } else if (ex.b + ey.b + ez.b < R(4)) {
+                    } else if (ex.b + ey.b + ez.b < S2) {
                         const R r2 = ex.a * ex.a + ey.a * ey.a + ez.a * ez.a;
-                        isA = r2 < R(4);
+                        isA = r2 < S2;
                         isB = !isA;
                     }
                 }
@@ -493,8 +588,8 @@ __global__ __launch_bounds__(THREADS) void k_particles(
             }
             wave_sync();
 
-            // ---- lattice values of the needed points (inside 2h first, then outside) ----
-            // line-pair sums: blocks (x,y) (x,z) (y,z) of the particle's table; component 0 is E2, 1 is E3
+            // ---- lattice values of the needed points (inside the support first, then outside) ----
+            // line-table blocks (x,y) (x,z) (y,z) of the particle's table
             const int exz = Lx * Ly, eyz = exz + Lx * Lz;
             const uint32_t ntot = nA + nB;
             for (uint32_t v = lane; v < ntot; v += 64) {
@@ -506,18 +601,19 @@ __global__ __launch_bounds__(THREADS) void k_particles(
                 const R ax = dx < R(0) ? -dx : dx, ay = dy < R(0) ? -dy : dy, az = dz < R(0) ? -dz : dz;
                 const R r2 = ax * ax + ay * ay + az * az;
                 const R r = KMath<R>::sqrt_(r2);
-                const int reg = m4_region(r2);
-                ++c_pts[reg - 1];
+                const int reg = P::region(ctx, r2);
+                ++c_pts[min(reg, 3) - 1];
                 const R csum = L.pc[reg - 1][i] + L.pc[reg - 1][nx + j] + L.pc[reg - 1][nx + ny + k];
                 R dsum = R(0);
-                if (reg != 1) {
+                const int slot = P::line_slot(reg);
+                if (slot >= 0) {
                     const int gi = ox + i, gj = oy + j, gk = oz + k;  // particle-lattice coordinates
-                    const uint32_t c = static_cast<uint32_t>(reg - 2);
-                    dsum = (etab[2u * static_cast<uint32_t>(gi * Ly + gj) + c] +
-                            etab[2u * static_cast<uint32_t>(exz + gi * Lz + gk) + c]) +
-                           etab[2u * static_cast<uint32_t>(eyz + gj * Lz + gk) + c];
+                    const uint32_t c = static_cast<uint32_t>(slot);
+                    dsum = (etab[NLN * static_cast<uint32_t>(gi * Ly + gj) + c] +
+                            etab[NLN * static_cast<uint32_t>(exz + gi * Lz + gk) + c]) +
+                           etab[NLN * static_cast<uint32_t>(eyz + gj * Lz + gk) + c];
                 }
-                const R W = m4_lattice_free<R>(ax, ay, az, r, reg) * M4Consts<R>::inv_pi + R(0.5) * csum + dsum;
+                const R W = P::free_part(ctx, ax, ay, az, r, reg) + P::plane_weight() * csum + dsum;
                 const bool neg = (dx < R(0)) != ((dy < R(0)) != (dz < R(0)));
                 L.U[u] = neg ? -W : W;
             }
@@ -529,7 +625,7 @@ __global__ __launch_bounds__(THREADS) void k_particles(
             for (int u = lane; u < ncells; u += 64, wc.next(my, mz)) {
                 const int i = wc.i, j = wc.j, k = wc.k;
                 double Ms = 0.0;
-                if (L.cg2[i] + L.cg2[nx + j] + L.cg2[nx + ny + k] < R(4)) {
+                if (L.cg2[i] + L.cg2[nx + j] + L.cg2[nx + ny + k] < S2) {
                     ++n_sphere;
                     const R* U0 = &L.U[(i * ny + j) * nz + k];
                     Ms = static_cast<double>(U0[nyz]) + static_cast<double>(U0[nz]) + static_cast<double>(U0[1]) +
@@ -748,7 +844,7 @@ struct kiwi_hip_ctx {
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     std::string err;
     DevBuf parts, bbox, sizes, offsets, keys_a, keys_b, pos_a, pos_b, cells, slab_sorted, slab, cub_tmp, ranges, work,
-        work_sorted, order_in, order, lines, ms, stats, outbuf, vsum;
+        work_sorted, order_in, order, lines, ms, stats, outbuf, vsum, tables;
 };
 
 namespace {
@@ -796,23 +892,25 @@ int resident_grid(const kiwi_hip_ctx* c, K kernel, int64_t items_per_wave_round,
     return static_cast<int>(std::max<int64_t>(1, std::min(full, need)));
 }
 
-template <typename R>
-hipError_t launch_cells(const kiwi_hip_ctx* c, hipStream_t s, int64_t n, int64_t npairs, const double* const* dp,
-                        const int* bbox, const int64_t* loffs, const uint64_t* keys_particle_order,
-                        const int64_t* slab, const uint64_t* keys_block_order, const int64_t* slab_sorted,
-                        const int* blk_begin, const int* blk_end, const int* order, int nblocks, const GridGeom& g,
-                        void* lines, void* ms, double* const* om, DevStats* dst, int clamp, hipEvent_t ev_lines,
-                        hipEvent_t ev_cells, hipEvent_t ev_accum) {
+template <class P>
+hipError_t launch_cells(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pctx, int64_t n, int64_t npairs,
+                        const double* const* dp, const int* bbox, const int64_t* loffs,
+                        const uint64_t* keys_particle_order, const int64_t* slab, const uint64_t* keys_block_order,
+                        const int64_t* slab_sorted, const int* blk_begin, const int* blk_end, const int* order,
+                        int nblocks, const GridGeom& g, void* lines, void* ms, double* const* om, DevStats* dst,
+                        int clamp, hipEvent_t ev_lines, hipEvent_t ev_cells, hipEvent_t ev_accum) {
+    using R = typename P::Real;
+    using Line = LineVals<R, P::NLN>;
     if (n > 0)
-        k_lines<R><<<resident_grid(c, k_lines<R>, 1, n), THREADS, 0, s>>>(
-            n, dp[0], dp[1], dp[2], dp[7], bbox, loffs, g, static_cast<Pair2<R>*>(lines), dst);
+        k_lines<P><<<resident_grid(c, k_lines<P>, 1, n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], bbox, loffs,
+                                                                          g, pctx, static_cast<Line*>(lines), dst);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if ((e = hipEventRecord(ev_lines, s)) != hipSuccess) return e;
     if (npairs > 0)
-        k_particles<R><<<resident_grid(c, k_particles<R>, ITEM_GRAB, npairs), THREADS, 0, s>>>(
-            static_cast<uint32_t>(npairs), keys_particle_order, slab, dp[0], dp[1], dp[2], dp[7], bbox, loffs, g,
-            static_cast<const Pair2<R>*>(lines), static_cast<R*>(ms), dst, clamp);
+        k_particles<P><<<resident_grid(c, k_particles<P>, ITEM_GRAB, npairs), THREADS, 0, s>>>(
+            static_cast<uint32_t>(npairs), keys_particle_order, slab, dp[0], dp[1], dp[2], dp[7], bbox, loffs, g, pctx,
+            static_cast<const Line*>(lines), static_cast<R*>(ms), dst, clamp);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipEventRecord(ev_cells, s)) != hipSuccess) return e;
     k_accum<R><<<nblocks, THREADS, 0, s>>>(dp[3], dp[4], dp[5], dp[6], bbox, keys_block_order, slab_sorted, blk_begin,
@@ -821,6 +919,9 @@ hipError_t launch_cells(const kiwi_hip_ctx* c, hipStream_t s, int64_t n, int64_t
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return hipEventRecord(ev_accum, s);
 }
+
+// the kernel tables on the host, in KIWI_KERNEL_M4 .. KIWI_KERNEL_WC6 order
+const KiwiKernelTable kHostTables[KIWI_TAB_NKERNELS] = {KIWI_KERNEL_TABLE_DATA};
 
 }  // namespace
 
@@ -848,6 +949,21 @@ int kiwi_hip_create(kiwi_hip_ctx** out, int device_ordinal) {
             return KIWI_ERR_DEVICE;
         }
     }
+    // the generic kernels' coefficient tables, resident for the context's lifetime
+    GenTab tabs[KIWI_TAB_NKERNELS];
+    for (int k = 0; k < KIWI_TAB_NKERNELS; ++k) {
+        const KiwiKernelTable& T = kHostTables[k];
+        tabs[k].nreg = T.nreg;
+        std::memcpy(tabs[k].c, T.c, sizeof(T.c));
+        tabs[k].A3_pi = T.A3_pi;
+        std::memcpy(tabs[k].a, T.a, sizeof(T.a));
+        std::memcpy(tabs[k].B, T.B, sizeof(T.B));
+    }
+    if (reserve(c->tables, sizeof(tabs)) != hipSuccess ||
+        hipMemcpy(c->tables.p, tabs, sizeof(tabs), hipMemcpyHostToDevice) != hipSuccess) {
+        kiwi_hip_destroy(c);
+        return KIWI_ERR_DEVICE;
+    }
     *out = c;
     return KIWI_OK;
 }
@@ -858,7 +974,7 @@ void kiwi_hip_destroy(kiwi_hip_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->parts, &c->bbox, &c->sizes, &c->offsets, &c->keys_a, &c->keys_b, &c->pos_a, &c->pos_b,
                       &c->cells, &c->slab_sorted, &c->slab, &c->cub_tmp, &c->ranges, &c->work, &c->work_sorted,
-                      &c->order_in, &c->order, &c->lines, &c->ms, &c->stats, &c->outbuf, &c->vsum})
+                      &c->order_in, &c->order, &c->lines, &c->ms, &c->stats, &c->outbuf, &c->vsum, &c->tables})
         if (b->p) (void)hipFree(b->p);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -885,9 +1001,15 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
         return fail(c, KIWI_ERR_INVALID_ARG, "domain min must be < max on every axis");
     if (precision != KIWI_FP64 && precision != KIWI_MIXED) return fail(c, KIWI_ERR_INVALID_ARG, "unknown precision");
     if (K->kernel_id < 0 || K->kernel_id > KIWI_KERNEL_WC6) return fail(c, KIWI_ERR_INVALID_ARG, "unknown kernel id");
-    const double sup = K->support_in_h > 0 ? K->support_in_h : (K->kernel_id == KIWI_KERNEL_M4 ? 2.0 : 0.0);
-    if (K->kernel_id != KIWI_KERNEL_M4 || K->generic_path || sup != 2.0 || K->tabulated)
-        return fail(c, KIWI_ERR_UNSUPPORTED, "this build implements the analytic cubic-spline (M4, support 2h) path only");
+    // dispatch of make_kernel_integrator (include/kiwi/kernel_integrator.hpp:14-46, src/config.cpp:269-277):
+    // the cubic analytic path iff M4 with support 2h and no generic request, the generic path otherwise
+    if (!(K->support_in_h >= 0.0) || !std::isfinite(K->support_in_h))  // src/config.cpp:169-170
+        return fail(c, KIWI_ERR_INVALID_ARG, "kernel.support_radius_in_h must be > 0 (or 0 for the kernel's default)");
+    const double sup = K->support_in_h > 0 ? K->support_in_h : kHostTables[K->kernel_id].radkern;
+    const bool generic = K->generic_path || K->kernel_id != KIWI_KERNEL_M4 || sup != 2.0;
+    if (K->tabulated && generic)
+        return fail(c, KIWI_ERR_INVALID_ARG, "the tabulated integral is only available on the cubic-spline fast path");
+    if (K->tabulated) return fail(c, KIWI_ERR_UNSUPPORTED, "the tabulated cubic-spline integral is not built yet");
     if (P->n > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 particles in one call");
     if (G->nx > MAX_AXIS || G->ny > MAX_AXIS || G->nz > MAX_AXIS)
         return fail(c, KIWI_ERR_LIMIT, "more than 8192 cells along a grid axis");
@@ -963,8 +1085,7 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
 
     KIWI_HIP_CHECK(c, hipMemsetAsync(dst, 0, sizeof(DevStats), s));
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[0], s));
-    const double support = 2.0;
-    k_prep<<<grid1d(n1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], support, g, bbox, nblk, nlin, dst);
+    k_prep<<<grid1d(n1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], sup, g, bbox, nblk, nlin, dst);
     KIWI_HIP_CHECK(c, hipGetLastError());
     {
         size_t tmp_bytes = 0;
@@ -986,9 +1107,13 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     if (npairs > 0x7fffffffLL)
         return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 (particle, cell-block) items in one call");
 
-    const size_t rsz = (precision == KIWI_FP64) ? sizeof(double) : sizeof(float);
+    // the generic path always computes in double precision (mixed applies to the cubic path)
+    const bool fp64 = generic || precision == KIWI_FP64;
+    const size_t rsz = fp64 ? sizeof(double) : sizeof(float);
+    const size_t line_entry = generic ? sizeof(LineVals<double, PolicyGen::NLN>)
+                                      : (fp64 ? sizeof(LineVals<double, 2>) : sizeof(LineVals<float, 2>));
     const int64_t np1 = npairs + 1;
-    KIWI_HIP_CHECK(c, reserve(c->lines, 2 * rsz * static_cast<size_t>(std::max<int64_t>(nlines_total, 1))));
+    KIWI_HIP_CHECK(c, reserve(c->lines, line_entry * static_cast<size_t>(std::max<int64_t>(nlines_total, 1))));
     KIWI_HIP_CHECK(c, reserve(c->ms, rsz * static_cast<size_t>(std::max<unsigned long long>(pairs_bbox, 1))));
     KIWI_HIP_CHECK(c, reserve(c->keys_a, sizeof(uint64_t) * np1));
     KIWI_HIP_CHECK(c, reserve(c->keys_b, sizeof(uint64_t) * np1));
@@ -1049,16 +1174,22 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
                                                                         order, static_cast<int>(nblocks), 0, 64, s));
     }
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[1], s));
-    const int clamp = (precision == KIWI_FP64) ? 1 : 0;
+    const int clamp = fp64 ? 1 : 0;
     const int nb = static_cast<int>(nblocks);
-    if (precision == KIWI_FP64)
-        KIWI_HIP_CHECK(c, launch_cells<double>(c, s, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b, slab_sorted,
-                                               blk_begin, blk_end, order, nb, g, c->lines.p, c->ms.p, om, dst, clamp,
-                                               c->ev[2], c->ev[3], c->ev[4]));
-    else
-        KIWI_HIP_CHECK(c, launch_cells<float>(c, s, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b, slab_sorted,
-                                              blk_begin, blk_end, order, nb, g, c->lines.p, c->ms.p, om, dst, clamp,
-                                              c->ev[2], c->ev[3], c->ev[4]));
+    if (generic) {
+        const PolicyGen::Ctx pc{sup, static_cast<const GenTab*>(c->tables.p) + K->kernel_id};
+        KIWI_HIP_CHECK(c, launch_cells<PolicyGen>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
+                                                  slab_sorted, blk_begin, blk_end, order, nb, g, c->lines.p, c->ms.p, om,
+                                                  dst, clamp, c->ev[2], c->ev[3], c->ev[4]));
+    } else if (fp64) {
+        KIWI_HIP_CHECK(c, launch_cells<PolicyM4<double>>(c, s, {sup}, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
+                                                         slab_sorted, blk_begin, blk_end, order, nb, g, c->lines.p,
+                                                         c->ms.p, om, dst, clamp, c->ev[2], c->ev[3], c->ev[4]));
+    } else {
+        KIWI_HIP_CHECK(c, launch_cells<PolicyM4<float>>(c, s, {sup}, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
+                                                        slab_sorted, blk_begin, blk_end, order, nb, g, c->lines.p,
+                                                        c->ms.p, om, dst, clamp, c->ev[2], c->ev[3], c->ev[4]));
+    }
     k_vsum_partial<<<VSUM_BLOCKS, THREADS, 0, s>>>(n, dp[3], dp[4], dp[5], vpart);
     k_vsum_final<<<1, 64, 0, s>>>(vpart, vpart + 3 * VSUM_BLOCKS);
     KIWI_HIP_CHECK(c, hipGetLastError());

@@ -55,12 +55,20 @@ inline double q_log(double x) { return std::log(x); }
 inline double q_atan(double x) { return std::atan(x); }
 inline double q_sin(double x) { return std::sin(x); }
 inline double q_fabs(double x) { return std::fabs(x); }
+inline double q_cos(double x) { return std::cos(x); }
+inline double q_atanh(double x) { return std::atanh(x); }
+inline double q_pow(double x, int n) { return std::pow(x, n); }
+inline double q_fmin(double a, double b) { return std::fmin(a, b); }
 inline __float128 q_sqrt(__float128 x) { return sqrtq(x); }
 inline __float128 q_acos(__float128 x) { return acosq(x); }
 inline __float128 q_log(__float128 x) { return logq(x); }
 inline __float128 q_atan(__float128 x) { return atanq(x); }
 inline __float128 q_sin(__float128 x) { return sinq(x); }
 inline __float128 q_fabs(__float128 x) { return fabsq(x); }
+inline __float128 q_cos(__float128 x) { return cosq(x); }
+inline __float128 q_atanh(__float128 x) { return atanhq(x); }
+inline __float128 q_pow(__float128 x, int n) { return powq(x, static_cast<__float128>(n)); }
+inline __float128 q_fmin(__float128 a, __float128 b) { return fminq(a, b); }
 template <typename T> inline T q_pi();
 template <> inline double q_pi<double>() { return M_PI; }
 template <> inline __float128 q_pi<__float128>() { return M_PIq; }
@@ -292,60 +300,63 @@ double m4_full(double h, double phi, double r0, double R0) {
 constexpr int NI = KIWI_TAB_NI;
 constexpr int NREG = KIWI_TAB_MAXREG;
 
+template <typename T>
 struct GenWall {  // include/kiwi/kernel_integral_generic.hpp:27-35
     bool ok = false;
     const KiwiKernelTable* K = nullptr;
-    double H = 0, r0 = 0, q0 = 0;
+    T H = 0, r0 = 0, q0 = 0;
     int j = 0;
-    double coef[NREG][NI];
-    double bphi[NREG];
+    T coef[NREG][NI];
+    T bphi[NREG];
 };
 
+template <typename T>
 struct GenEdge {  // include/kiwi/kernel_integral_generic.hpp:37-44
     bool ok = false;
-    double R0 = 0, a = 0, a2 = 0;
+    T R0 = 0, a = 0, a2 = 0;
     int nseg = 0;
     int region[NREG];
-    double phi_end[NREG];
-    double offset[NREG];
+    T phi_end[NREG];
+    T offset[NREG];
 };
 
 // basis_integrals, src/kernel_integral_generic.cpp:19-66: I_i(phi) at out[1-i] for i = 1 .. imin.
-void gen_basis(double a, double a2, double phi, int imin, double* out) {
-    const double sn = std::sin(phi), cs = std::cos(phi);
-    const double den = a2 + cs * cs;
-    const double u = a * sn / std::sqrt(den);
-    const double omu2 = cs * cs * (1.0 + a2) / den;
-    out[0] = std::atan(u / a);
+template <typename T>
+void gen_basis(T a, T a2, T phi, int imin, T* out) {
+    const T sn = q_sin(phi), cs = q_cos(phi);
+    const T den = a2 + cs * cs;
+    const T u = a * sn / q_sqrt(den);
+    const T omu2 = cs * cs * (T(1) + a2) / den;
+    out[0] = q_atan(u / a);
     out[1] = phi;
     const int mmax = (-imin - 1) / 2;
-    double Iodd = out[0];
-    double K = std::atanh(u);
-    double p1a2 = 1.0;
+    T Iodd = out[0];
+    T K = q_atanh(u);
+    T p1a2 = 1;
     for (int m = 0; m <= mmax && -2 * m - 1 >= imin; ++m) {
         if (m > 0) {
-            K = u / (2.0 * m * std::pow(omu2, m)) + (2.0 * m - 1.0) / (2.0 * m) * K;
-            p1a2 *= (1.0 + a2);
+            K = u / (T(2 * m) * q_pow(omu2, m)) + T(2 * m - 1) / T(2 * m) * K;
+            p1a2 *= (T(1) + a2);
         }
         Iodd += a * p1a2 * K;
         out[2 + 2 * m] = Iodd;
     }
     const int kmax = -imin / 2;
     if (kmax >= 1) {
-        double J[NI];
-        const double t = sn / cs, sec = 1.0 / cs;
+        T J[NI];
+        const T t = sn / cs, sec = T(1) / cs;
         J[0] = phi;
         J[2] = t;
-        double sp = 1.0;
+        T sp = 1;
         for (int l = 2; l <= kmax; ++l) {
             const int m = 2 * l;
             sp *= sec * sec;
-            J[m] = sp * t / (m - 1.0) + (m - 2.0) / (m - 1.0) * J[m - 2];
+            J[m] = sp * t / T(m - 1) + T(m - 2) / T(m - 1) * J[m - 2];
         }
         for (int k = 1; k <= kmax; ++k) {
-            double sum = J[0], ap = 1.0, b = 1.0;
+            T sum = J[0], ap = 1, b = 1;
             for (int l = 1; l <= k; ++l) {
-                b = b * (k - l + 1) / l;
+                b = b * T(k - l + 1) / T(l);
                 ap *= a2;
                 sum += b * ap * J[2 * l];
             }
@@ -355,90 +366,96 @@ void gen_basis(double a, double a2, double phi, int imin, double* out) {
 }
 
 // region_of, src/kernel_integral_generic.cpp:68-72 (1-based)
-int gen_region(const KiwiKernelTable& K, double q) {
+template <typename T>
+int gen_region(const KiwiKernelTable& K, T q) {
     for (int k = 0; k < K.nreg - 1; ++k)
-        if (q <= K.c[k]) return k + 1;
+        if (q <= T(K.c[k])) return k + 1;
     return K.nreg;
 }
 
 // imin_of, src/kernel_integral_generic.cpp:75-79
-int gen_imin(const GenWall& w, int k) {
+template <typename T>
+int gen_imin(const GenWall<T>& w, int k) {
     for (int idx = NI - 1; idx >= 0; --idx)
-        if (w.coef[k - 1][idx] != 0.0) return 1 - idx;
+        if (w.coef[k - 1][idx] != T(0)) return 1 - idx;
     return 1;
 }
 
 // G, src/kernel_integral_generic.cpp:82-92
-double gen_G(const GenWall& w, const GenEdge& e, int k, double phi) {
-    double I[NI];
+template <typename T>
+T gen_G(const GenWall<T>& w, const GenEdge<T>& e, int k, T phi) {
+    T I[NI];
     gen_basis(e.a, e.a2, phi, gen_imin(w, k), I);
-    double sum = w.bphi[k - 1] * phi;
+    T sum = w.bphi[k - 1] * phi;
     for (int idx = 0; idx < NI; ++idx) {
-        const double c = w.coef[k - 1][idx];
-        if (c != 0.0) sum += c * I[idx];
+        const T c = w.coef[k - 1][idx];
+        if (c != T(0)) sum += c * I[idx];
     }
     return sum;
 }
 
 // generic_wall_constants, src/kernel_integral_generic.cpp:96-117
-GenWall gen_wall(const KiwiKernelTable& K, double H, double r0) {
-    GenWall w;
+template <typename T>
+GenWall<T> gen_wall(const KiwiKernelTable& K, T H, T r0) {
+    GenWall<T> w;
     w.K = &K;
     w.H = H;
     w.r0 = r0;
-    if (r0 == 0.0) return w;
+    if (r0 == T(0)) return w;
     w.ok = true;
     w.q0 = r0 / H;
     w.j = gen_region(K, w.q0);
-    const double A3 = K.A3_pi / M_PI;
+    const T A3 = T(K.A3_pi) / q_pi<T>();
     for (int k = 1; k <= K.nreg; ++k) {
         for (int idx = 0; idx < NI; ++idx) {
             const double av = K.a[k - 1][idx];
-            w.coef[k - 1][idx] = (av == 0.0) ? 0.0 : A3 * av * std::pow(w.q0, idx);
+            w.coef[k - 1][idx] = (av == 0.0) ? T(0) : A3 * T(av) * q_pow(w.q0, idx);
         }
-        double b = 0.0;
+        T b = 0;
         if (k >= w.j)
-            for (int p = KIWI_TAB_NB - 1; p >= 0; --p) b = b * w.q0 + K.B[w.j - 1][k - 1][p];
+            for (int p = KIWI_TAB_NB - 1; p >= 0; --p) b = b * w.q0 + T(K.B[w.j - 1][k - 1][p]);
         w.bphi[k - 1] = A3 * b;
     }
     return w;
 }
 
 // generic_edge_constants, src/kernel_integral_generic.cpp:119-148
-GenEdge gen_edge(const GenWall& w, double R0) {
-    GenEdge e;
+template <typename T>
+GenEdge<T> gen_edge(const GenWall<T>& w, T R0) {
+    GenEdge<T> e;
     e.R0 = R0;
-    if (!w.ok || R0 == 0.0) return e;
+    if (!w.ok || R0 == T(0)) return e;
     e.ok = true;
     e.a = R0 / w.r0;
     e.a2 = e.a * e.a;
     const KiwiKernelTable& K = *w.K;
-    const double qR = R0 / w.H;
-    const double rs = std::sqrt(w.q0 * w.q0 + qR * qR);
+    const T qR = R0 / w.H;
+    const T rs = q_sqrt(w.q0 * w.q0 + qR * qR);
     int k = gen_region(K, rs);
     e.nseg = 0;
     for (; k <= K.nreg; ++k) {
         e.region[e.nseg] = k;
         if (k == K.nreg) {
-            e.phi_end[e.nseg] = 0.5 * M_PI;
+            e.phi_end[e.nseg] = q_pi<T>() / T(2);
         } else {
-            const double c = K.c[k - 1];
-            e.phi_end[e.nseg] = std::acos(std::fmin(1.0, qR / std::sqrt(c * c - w.q0 * w.q0)));
+            const T c = T(K.c[k - 1]);
+            e.phi_end[e.nseg] = q_acos(q_fmin(T(1), qR / q_sqrt(c * c - w.q0 * w.q0)));
         }
         ++e.nseg;
     }
-    e.offset[0] = 0.0;
+    e.offset[0] = 0;
     for (int s = 0; s + 1 < e.nseg; ++s) {
-        const double pb = e.phi_end[s];
+        const T pb = e.phi_end[s];
         e.offset[s + 1] = e.offset[s] + gen_G(w, e, e.region[s], pb) - gen_G(w, e, e.region[s + 1], pb);
     }
     return e;
 }
 
 // generic_vertex_integral, src/kernel_integral_generic.cpp:150-156
-double gen_vertex(const GenWall& w, const GenEdge& e, double phi) {
-    if (!w.ok || !e.ok) return 0.0;
-    if (phi == 0.0) return 0.0;
+template <typename T>
+T gen_vertex(const GenWall<T>& w, const GenEdge<T>& e, T phi) {
+    if (!w.ok || !e.ok) return T(0);
+    if (phi == T(0)) return T(0);
     int s = 0;
     while (s + 1 < e.nseg && phi > e.phi_end[s]) ++s;
     return e.offset[s] + gen_G(w, e, e.region[s], phi);
@@ -476,15 +493,16 @@ __float128 EvalM4<__float128>::at(__float128 phi) const {
     return m4_vertex(w, e, phi);
 }
 
+template <typename T>
 struct EvalGen {
-    using scalar = double;
+    using scalar = T;
     const KiwiKernelTable* K;
-    double H;
-    GenWall w;
-    GenEdge e;
-    void wall(double r) { w = gen_wall(*K, H, r); }
-    void edge(double R) { e = gen_edge(w, R); }
-    double at(double phi) const { return gen_vertex(w, e, phi); }
+    T H;
+    GenWall<T> w;
+    GenEdge<T> e;
+    void wall(T r) { w = gen_wall(*K, H, r); }
+    void edge(T R) { e = gen_edge(w, R); }
+    T at(T phi) const { return gen_vertex(w, e, phi); }
 };
 
 template <class Ev>
@@ -633,7 +651,7 @@ struct Integrator {
     bool tabulated = false;
     double face(double x0, double y0, double z0, double h, const double* v, int n, int* out_vert) const {
         if (gen) {
-            EvalGen ev{gen, support_in_h * h, {}, {}};
+            EvalGen<double> ev{gen, support_in_h * h, {}, {}};
             return face_walk(x0, y0, z0, support_in_h * h, v, n, out_vert, ev);
         }
         EvalM4<double> ev{h, tabulated ? &m4_table() : nullptr, {}, {}};
@@ -672,8 +690,8 @@ double oracle_m4_gridded(double h, double phi, double r0, double R0) {
 double oracle_generic_vertex(int kernel_id, double H, double phi, double r0, double R0) {
     if (kernel_id < 0 || kernel_id >= KIWI_TAB_NKERNELS) return std::nan("");
     if (r0 == 0.0 || R0 == 0.0 || phi == 0.0) return 0.0;
-    const GenWall w = gen_wall(kKiwiKernelTables[kernel_id], H, r0);
-    const GenEdge e = gen_edge(w, R0);
+    const GenWall<double> w = gen_wall(kKiwiKernelTables[kernel_id], H, r0);
+    const GenEdge<double> e = gen_edge(w, R0);
     return gen_vertex(w, e, phi);
 }
 
@@ -809,9 +827,13 @@ int oracle_sph_to_grid(long long n, const double* x, const double* y, const doub
 // arbitrate cells where two double-precision evaluations of the same integral disagree.
 int oracle_sph_to_grid_cells_q(long long n, const double* x, const double* y, const double* z, const double* vx,
                                const double* vy, const double* vz, const double* m, const double* h,
-                               const double* domain, int nx, int ny, int nz, long long ncells,
-                               const long long* cells, double* out4) {
-    if (nx <= 0 || ny <= 0 || nz <= 0) return -1;
+                               const double* domain, int nx, int ny, int nz, const OracleKernel* k,
+                               long long ncells, const long long* cells, double* out4) {
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !k || k->kernel_id < 0 || k->kernel_id >= KIWI_TAB_NKERNELS) return -1;
+    if (k->tabulated) return -1;  // the tabulated path is double precision by construction
+    const KiwiKernelTable& K = kKiwiKernelTables[k->kernel_id];
+    const double sup = k->support_in_h > 0 ? k->support_in_h : K.radkern;
+    const bool generic = k->generic_path || k->kernel_id != 0 || sup != 2.0;
     const double xmin = domain[0], ymin = domain[2], zmin = domain[4];
     const double dpx = (domain[1] - domain[0]) / nx, dpy = (domain[3] - domain[2]) / ny,
                  dpz = (domain[5] - domain[4]) / nz;
@@ -831,7 +853,7 @@ int oracle_sph_to_grid_cells_q(long long n, const double* x, const double* y, co
         for (long long i = 0; i < n; ++i) {
             const double hh = h[i];
             if (hh <= 0.0) continue;
-            const double rad = 2.0 * hh;
+            const double rad = sup * hh;
             const int ix0 = std::max(static_cast<int>(std::floor((x[i] - rad - xmin) / dpx)), 0);
             const int ix1 = std::min(static_cast<int>(std::floor((x[i] + rad - xmin) / dpx)), nx - 1);
             const int iy0 = std::max(static_cast<int>(std::floor((y[i] - rad - ymin) / dpy)), 0);
@@ -840,10 +862,15 @@ int oracle_sph_to_grid_cells_q(long long n, const double* x, const double* y, co
             const int iz1 = std::min(static_cast<int>(std::floor((z[i] + rad - zmin) / dpz)), nz - 1);
             if (ix < ix0 || ix > ix1 || iy < iy0 || iy > iy1 || iz < iz0 || iz > iz1) continue;
             __float128 Ms = 0;
-            for (int k = 0; k < 6; ++k) {
-                EvalM4<__float128> ev{hh, nullptr, {}, {}};
-                Ms += face_walk<EvalM4<__float128>>(x[i], y[i], z[i], 2 * static_cast<__float128>(hh), f[k], 4,
-                                                     nullptr, ev);
+            for (int q = 0; q < 6; ++q) {
+                const __float128 H = static_cast<__float128>(sup) * hh;
+                if (generic) {
+                    EvalGen<__float128> ev{&K, H, {}, {}};
+                    Ms += face_walk<EvalGen<__float128>>(x[i], y[i], z[i], H, f[q], 4, nullptr, ev);
+                } else {
+                    EvalM4<__float128> ev{hh, nullptr, {}, {}};
+                    Ms += face_walk<EvalM4<__float128>>(x[i], y[i], z[i], H, f[q], 4, nullptr, ev);
+                }
             }
             if (Ms < 0) Ms = 0;
             const __float128 t = Ms * m[i];

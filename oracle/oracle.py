@@ -61,7 +61,8 @@ def lib():
         L.oracle_sph_to_grid.argtypes = [_ll] + [_pd] * 8 + [_pd, _i, _i, _i, ctypes.POINTER(OracleKernel), _i,
                                                               _pd, _pd, _pd, _pd, ctypes.POINTER(OracleStats)]
         L.oracle_sph_to_grid_cells_q.restype = _i
-        L.oracle_sph_to_grid_cells_q.argtypes = [_ll] + [_pd] * 8 + [_pd, _i, _i, _i, _ll, _pll, _pd]
+        L.oracle_sph_to_grid_cells_q.argtypes = [_ll] + [_pd] * 8 + [_pd, _i, _i, _i, ctypes.POINTER(OracleKernel),
+                                                                      _ll, _pll, _pd]
         L.oracle_voronoi_map.restype = _i
         L.oracle_voronoi_map.argtypes = [_ll] + [_pd] * 8 + [_pi, _pi, _ll, _pll, _pll, _pd, _pi,
                                                               ctypes.POINTER(OracleKernel), _pd, _pd, _pd, _pd, _pll]
@@ -127,15 +128,16 @@ def sph_to_grid(parts, domain, nx, ny, nz, k=None, threads=1):
     return out[0], out[1], out[2], out[3], st
 
 
-def sph_to_grid_cells_quad(parts, domain, nx, ny, nz, cells):
-    """Reference cubic sums (mass, px, py, pz) for the listed flat cell indices, with the kernel
-    mathematics in binary128.  Returns an array of shape (len(cells), 4)."""
+def sph_to_grid_cells_quad(parts, domain, nx, ny, nz, cells, k=None):
+    """Reference sums (mass, px, py, pz) for the listed flat cell indices, with the kernel mathematics
+    (cubic or generic path, per k) in binary128.  Returns an array of shape (len(cells), 4)."""
+    k = k or kernel()
     cols = [np.ascontiguousarray(parts[c], dtype=np.float64) for c in ("x", "y", "z", "vx", "vy", "vz", "m", "h")]
     dom = np.ascontiguousarray(domain, dtype=np.float64)
     cells = np.ascontiguousarray(cells, dtype=np.int64)
     out = np.zeros((len(cells), 4))
-    rc = lib().oracle_sph_to_grid_cells_q(len(cols[0]), *[_p(c) for c in cols], _p(dom), nx, ny, nz, len(cells),
-                                          cells.ctypes.data_as(_pll), _p(out))
+    rc = lib().oracle_sph_to_grid_cells_q(len(cols[0]), *[_p(c) for c in cols], _p(dom), nx, ny, nz, ctypes.byref(k),
+                                          len(cells), cells.ctypes.data_as(_pll), _p(out))
     if rc != 0:
         raise ValueError("oracle_sph_to_grid_cells_q failed")
     return out

@@ -10,6 +10,11 @@ fixtures it writes are plain numbers (inputs and the reference's outputs), commi
   generic_vertex.npz    (kernel, H, phi, r0, R0) -> generic_full_integral for all six kernels
   face.npz              random planar faces -> compute_face_integral(_generic) and out_vert
   m4_gridded.npz        (h, phi, r0, R0) -> gridded_integral on the generated lookup table
+  kernel_grids.npz      `kiwi sph-to-grid` (reference CLI, OMP_NUM_THREADS=1) on every 4th C1 particle
+                        onto 20^3 for each kernel on the generic path (M4 forced generic, M4 with
+                        support 2.5h, M5, M6, WC2, WC4, WC6)
+
+  python tests/golden/make_golden.py [--only kernel_grids]
 """
 import ctypes
 import json
@@ -61,9 +66,35 @@ def run_ref_sph_to_grid(parts, domain, res, kernel=None, threads=1, extra=None):
         return shape, dom, body, r.stderr
 
 
+# (label, kernel config) of the generic-path grids
+KERNEL_GRID_CASES = [("M4_generic", {"name": "m4", "generic_path": True}),
+                     ("M4_support2.5", {"name": "m4", "support_radius_in_h": 2.5}),
+                     ("M5", {"name": "m5"}), ("M6", {"name": "m6"}), ("WC2", {"name": "wc2"}),
+                     ("WC4", {"name": "wc4"}), ("WC6", {"name": "wc6"})]
+
+
+def kernel_grids():
+    """Reference grids for the generic path, one per kernel case (plain numbers, committed)."""
+    c1 = np.load(os.path.join(OUT, "c1_snapshot.npz"))
+    parts = {k: np.ascontiguousarray(c1[k][::4]) for k in ("x", "y", "z", "vx", "vy", "vz", "m", "h", "rho")}
+    domain = np.array(c1["domain"])
+    res = (20, 20, 20)
+    out = {"domain": domain, "res": np.array(res), **{f"p_{k}": v for k, v in parts.items()}}
+    for label, kcfg in KERNEL_GRID_CASES:
+        shape, dom, body, _ = run_ref_sph_to_grid(parts, domain, res, kernel=kcfg)
+        assert tuple(shape) == res
+        for i, f in enumerate(("density", "vx", "vy", "vz")):
+            out[f"{label}_{f}"] = body[i]
+    np.savez_compressed(os.path.join(OUT, "kernel_grids.npz"), **out)
+
+
 def main():
     if not O.have_ref() or not os.path.exists(O.REF_CLI):
         sys.exit("oracle/_ref is not built (make -f oracle/ref.mk)")
+    if "--only" in sys.argv:
+        what = sys.argv[sys.argv.index("--only") + 1]
+        {"kernel_grids": kernel_grids}[what]()
+        return
     R = O.ref()
     rng = np.random.default_rng(20251203)
 
@@ -149,6 +180,7 @@ def main():
     phi = rng.uniform(0.0, 0.5 * np.pi, n)
     val = np.array([R.ref_m4_gridded(*t) for t in zip(h, phi, r0, R0)])
     np.savez_compressed(os.path.join(OUT, "m4_gridded.npz"), h=h, phi=phi, r0=r0, R0=R0, value=val)
+    kernel_grids()
     print("golden fixtures written to", OUT)
 
 

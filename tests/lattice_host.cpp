@@ -69,3 +69,37 @@ float lat_atan_poly_f(float t) { return atan_poly01<float>(t); }
 float lat_atan2_pos_f(float y, float x) { return KMath<float>::atan2_pos(y, x); }
 
 }  // extern "C"
+
+// ---- generic kernels (gen_lattice.h) ----
+#include "../kiwi_amd/csrc/gen_lattice.h"
+
+namespace {
+const KiwiKernelTable kTabs[KIWI_TAB_NKERNELS] = {KIWI_KERNEL_TABLE_DATA};
+
+double gen_U(const KiwiKernelTable& K, double dx, double dy, double dz) {
+    if (dx == 0.0 || dy == 0.0 || dz == 0.0) return 0.0;
+    const double W = gen_lattice_W(K, std::fabs(dx), std::fabs(dy), std::fabs(dz));
+    const bool neg = (dx < 0.0) != ((dy < 0.0) != (dz < 0.0));
+    return neg ? -W : W;
+}
+}  // namespace
+
+extern "C" {
+
+// Msum of the box [lo, hi] for kernel kid with support H: the 8-corner stencil of gen_lattice_W
+double lat_gen_box(int kid, const double* p, double H, const double* lo, const double* hi) {
+    const KiwiKernelTable& K = kTabs[kid];
+    double m = 0.0;
+    for (int di = 0; di < 2; ++di)
+        for (int dj = 0; dj < 2; ++dj)
+            for (int dk = 0; dk < 2; ++dk) {
+                const double cx = di ? hi[0] : lo[0], cy = dj ? hi[1] : lo[1], cz = dk ? hi[2] : lo[2];
+                const double e = ((di + dj + dk) % 2 == 1) ? 1.0 : -1.0;
+                m += e * gen_U(K, (cx - p[0]) / H, (cy - p[1]) / H, (cz - p[2]) / H);
+            }
+    return m;
+}
+
+}  // extern "C"
+
+extern "C" double lat_gen_W(int kid, double ax, double ay, double az) { return gen_lattice_W(kTabs[kid], ax, ay, az); }

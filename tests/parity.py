@@ -11,7 +11,7 @@ import numpy as np
 from oracle import oracle as O
 
 
-def assert_cells_match(gpu4, ref4, parts, domain, res, tol=1e-10, quad_tol=1e-11, max_disputed=5000):
+def assert_cells_match(gpu4, ref4, parts, domain, res, tol=1e-10, quad_tol=1e-11, max_disputed=5000, k=None):
     gpu4 = [np.asarray(a) for a in gpu4]
     ref4 = [np.asarray(a) for a in ref4]
     scale = [max(np.max(np.abs(r)), 1e-300) for r in ref4]
@@ -22,13 +22,13 @@ def assert_cells_match(gpu4, ref4, parts, domain, res, tol=1e-10, quad_tol=1e-11
     if len(cells) == 0:
         return 0
     assert len(cells) <= max_disputed, f"{len(cells)} cells outside {tol:g} of the reference"
-    q = O.sph_to_grid_cells_quad(parts, domain, *res, cells)
-    for k in range(4):
-        g = gpu4[k][cells]
-        r = ref4[k][cells]
-        eg = np.abs(g - q[:, k])
-        er = np.abs(r - q[:, k])
-        assert np.all(eg <= quad_tol * scale[k]), (k, cells[np.argmax(eg)], eg.max() / scale[k])
-        worse = (eg > er) & (np.abs(g - r) > tol * scale[k])
-        assert not np.any(worse), (k, cells[worse][:5])
+    q = O.sph_to_grid_cells_quad(parts, domain, *res, cells, k=k)
+    for f in range(4):
+        g = gpu4[f][cells]
+        r = ref4[f][cells]
+        eg = np.abs(g - q[:, f])
+        er = np.abs(r - q[:, f])
+        assert np.all(eg <= quad_tol * scale[f]), (f, cells[np.argmax(eg)], eg.max() / scale[f])
+        worse = (eg > er) & (np.abs(g - r) > tol * scale[f])
+        assert not np.any(worse), (f, cells[worse][:5])
     return len(cells)

@@ -173,3 +173,69 @@ def test_mixed_precision_total_mass(hip_ctx):
     say(f"  total mass rel err {err:.3e}; per-field max err / field max: "
         + ", ".join(f"{rel_max(x, y):.2e}" for x, y in zip(m[:4], f[:4])))
     assert err <= 1e-6
+
+
+# ----------------------------------------------------------------------------- generic kernels
+KERNEL_GRID_CASES = [("M4_generic", "M4", 0.0, True), ("M4_support2.5", "M4", 2.5, False), ("M5", "M5", 0.0, False),
+                     ("M6", "M6", 0.0, False), ("WC2", "WC2", 0.0, False), ("WC4", "WC4", 0.0, False),
+                     ("WC6", "WC6", 0.0, False)]
+
+
+@pytest.mark.parametrize("label,name,support,generic", KERNEL_GRID_CASES)
+def test_generic_kernels_match_reference_cli(hip_ctx, label, name, support, generic):
+    say(f"case: generic path {label}, 500 particles -> 20^3, vs reference CLI output")
+    g = np.load(os.path.join(GOLD, "kernel_grids.npz"))
+    parts = {k: np.ascontiguousarray(g["p_" + k]) for k in COLS}
+    dom = g["domain"]
+    res = tuple(int(v) for v in g["res"])
+    k = hip.kernel_spec(name, support_in_h=support, generic_path=generic)
+    mass, px, py, pz, st = hip_ctx.sph_to_grid(parts, dom, res, kernel=k, precision=hip.FP64)
+    dens, vx, vy, vz = hip.normalise(mass, px, py, pz, dom, res, st.v_mean)
+    ref_d = g[label + "_density"]
+    err = rel_max(dens, ref_d)
+    say(f"  density max err / max {err:.2e}")
+    assert err <= 1e-10
+    V = np.prod((dom[1::2] - dom[0::2]) / np.array(res))
+    for v, f in ((vx, "vx"), (vy, "vy"), (vz, "vz")):
+        rv = g[f"{label}_{f}"]
+        assert rel_max(dens * v * V, ref_d * rv * V) <= 1e-10  # momentum
+        w = ref_d / ref_d.max()
+        assert np.max(np.abs(v - rv) * w) <= 1e-10 * np.max(np.abs(rv))
+    # the generic path computes in double precision whatever the request
+    m2 = hip_ctx.sph_to_grid(parts, dom, res, kernel=k, precision=hip.MIXED)
+    assert np.array_equal(m2[0], mass)
+
+
+@pytest.mark.parametrize("name", ["M6", "WC6"])
+def test_generic_random_vs_oracle_and_conservation(hip_ctx, name):
+    from oracle import oracle as O
+    say(f"case: generic {name}, 60 large particles -> 40^3 vs oracle; padded-domain conservation")
+    parts, dom, res = _random_case(12, 60, (40, 40, 40), (0.3, 1.2))
+    k = hip.kernel_spec(name)
+    mass, px, py, pz, st = hip_ctx.sph_to_grid(parts, dom, res, kernel=k)
+    ok = O.kernel(name, generic=True)
+    om, opx, opy, opz, ost = O.sph_to_grid(parts, dom, *res, k=ok, threads=8)
+    # The generic kernels' monomial expansions cancel heavily for walls near the support edge (WC6: terms
+    # of O(100) for lattice values of O(1e-4)), in the reference's vertex integrals as in the lattice
+    # values here; the lattice keeps up to ~5e-14 of a particle's mass per lattice value there (measured
+    # against 50-digit arithmetic), which reaches ~2e-10 of a sparse field's maximum.  Cells outside that
+    # of the reference must lie within it of the binary128 evaluation and closer to it than the reference.
+    n = assert_cells_match((mass, px, py, pz), (om, opx, opy, opz), parts, dom, res, tol=2e-10, quad_tol=2e-10,
+                           k=ok)
+    say(f"  cells arbitrated in binary128: {n}")
+    assert st.pairs_bbox == ost.pairs_bbox
+    pad = workloads.padded_domain(parts, support_in_h=3.0 if name == "M6" else 2.0)
+    a = hip_ctx.sph_to_grid(parts, pad, (36, 36, 36), kernel=k)
+    M = parts["m"].sum()
+    assert abs(a[0].sum() - M) <= 1e-9 * M
+
+
+def test_kernel_path_errors(hip_ctx):
+    say("case: tabulated / generic argument rules")
+    parts, dom = load_c1()
+    with pytest.raises(hip.KiwiError) as e:  # tabulated is only for the cubic fast path
+        hip_ctx.sph_to_grid(parts, dom, (8, 8, 8), kernel=hip.kernel_spec("WC2", tabulated=True))
+    assert e.value.code == -1
+    with pytest.raises(hip.KiwiError) as e:
+        hip_ctx.sph_to_grid(parts, dom, (8, 8, 8), kernel=hip.kernel_spec("M4", support_in_h=-1.0))
+    assert e.value.code == -1

@@ -147,3 +147,47 @@ def test_atan_polynomial_error_bounds(lat):
     for y, x in rng.uniform(1e-4, 10.0, size=(2000, 2)):
         y32, x32 = float(np.float32(y)), float(np.float32(x))
         assert abs(lat.lat_atan2_pos_f(y32, x32) - math.atan2(y32, x32)) <= 3e-7 * math.atan2(y32, x32)
+
+
+# ---------------------------------------------------------------------- generic kernels (gen_lattice.h)
+KERNELS = ["M4", "M5", "M6", "WC2", "WC4", "WC6"]
+RADKERN = {"M4": 2.0, "M5": 2.5, "M6": 3.0, "WC2": 2.0, "WC4": 2.0, "WC6": 2.0}
+
+
+def _box_faces(lo, hi):
+    xm, ym, zm = lo
+    xp, yp, zp = hi
+    return [[xp, ym, zp, xp, yp, zp, xp, yp, zm, xp, ym, zm], [xm, yp, zm, xm, yp, zp, xm, ym, zp, xm, ym, zm],
+            [xp, yp, zm, xp, yp, zp, xm, yp, zp, xm, yp, zm], [xm, ym, zp, xp, ym, zp, xp, ym, zm, xm, ym, zm],
+            [xm, yp, zp, xp, yp, zp, xp, ym, zp, xm, ym, zp], [xp, ym, zm, xp, yp, zm, xm, yp, zm, xm, ym, zm]]
+
+
+def _gen_box(L, kid, p, H, lo, hi):
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (p, lo, hi)]
+    return L.lat_gen_box(kid, a[0].ctypes.data_as(_pd), H, a[1].ctypes.data_as(_pd), a[2].ctypes.data_as(_pd))
+
+
+@pytest.mark.parametrize("name", KERNELS)
+def test_generic_lattice_boxes_equal_the_generic_face_walk(lat, name):
+    """Generic kernels: 8-corner lattice sums against the oracle's generic face walk (the reference's
+    src/kernel_integral_generic.cpp algorithm, pinned in test_oracle.py), for random boxes in and
+    around the support, and the enclosing box holds the whole mass."""
+    lat.lat_gen_box.restype = ctypes.c_double
+    lat.lat_gen_box.argtypes = [ctypes.c_int, _pd, ctypes.c_double, _pd, _pd]
+    kid = KERNELS.index(name)
+    k = O.kernel(name, generic=True)
+    rng = np.random.default_rng(20 + kid)
+    worst = 0.0
+    for _ in range(150):
+        p = rng.uniform(-1, 1, 3)
+        h = rng.uniform(0.3, 1.2)
+        H = RADKERN[name] * h
+        lo = p + rng.uniform(-1.1, 0.9, 3) * H
+        hi = lo + rng.uniform(0.05, 0.7, 3) * H
+        ref = sum(O.face(k, p, h, np.array(f))[0] for f in _box_faces(lo, hi))
+        got = _gen_box(lat, kid, p, H, lo, hi)
+        worst = max(worst, abs(got - ref))
+    assert worst < 1e-12, worst
+    p = np.array([0.1, -0.2, 0.3])
+    H = RADKERN[name] * 0.7
+    assert abs(_gen_box(lat, kid, p, H, p - 1.3 * H, p + 1.2 * H) - 1.0) < 1e-13

@@ -96,6 +96,27 @@ def test_c1_grid_matches_reference_cli():
     assert np.array_equal(vx, g["vx"])
 
 
+KERNEL_GRID_CASES = [("M4_generic", "M4", 0.0), ("M4_support2.5", "M4", 2.5), ("M5", "M5", 0.0), ("M6", "M6", 0.0),
+                     ("WC2", "WC2", 0.0), ("WC4", "WC4", 0.0), ("WC6", "WC6", 0.0)]
+
+
+@pytest.mark.parametrize("label,name,support", KERNEL_GRID_CASES)
+def test_generic_grids_match_reference_cli(label, name, support):
+    """The oracle's generic path against the reference CLI on every 4th C1 particle onto 20^3, for each
+    kernel (fixture written with OMP_NUM_THREADS=1): identical to the last bit."""
+    g = np.load(os.path.join(GOLD, "kernel_grids.npz"))
+    parts = {k: g["p_" + k] for k in ("x", "y", "z", "vx", "vy", "vz", "m", "h")}
+    dom = g["domain"]
+    res = tuple(int(v) for v in g["res"])
+    k = O.kernel(name, support_in_h=support, generic=True)
+    mass, px, py, pz, st = O.sph_to_grid(parts, dom, *res, k=k, threads=1)
+    V = ((dom[1] - dom[0]) / res[0]) * ((dom[3] - dom[2]) / res[1]) * ((dom[5] - dom[4]) / res[2])
+    assert np.array_equal(mass / V, g[label + "_density"])
+    pos = mass > 0
+    for f, q, i in (("vx", px, 0), ("vy", py, 1), ("vz", pz, 2)):
+        assert np.array_equal(np.where(pos, q / np.where(pos, mass, 1), st.v_mean[i]), g[f"{label}_{f}"])
+
+
 # ------------------------------------------------------------------ closed forms
 @pytest.mark.parametrize("key", KEYS)
 def test_enclosing_box_is_one_and_partitions_add(key):
