@@ -42,6 +42,7 @@
 #include "../../include/kiwi_hip.h"
 #include "gen_lattice.h"
 #include "m4_lattice.h"
+#include "m4_table.h"
 
 using namespace kiwi_hip;
 
@@ -295,6 +296,8 @@ struct PolicyM4 {
     using Real = R;
     static constexpr int NPL = 3;  // plane values per lattice coordinate
     static constexpr int NLN = 2;  // line-table values per axis pair
+    static constexpr bool HAS_LINES = true;
+    static constexpr bool SPHERE_SKIP = true;  // cells and points beyond the support are exact zeros
     struct Ctx {
         double support_in_h;  // support / h = 2
     };
@@ -333,6 +336,8 @@ struct PolicyGen {
     using Real = double;
     static constexpr int NPL = GEN_NREG;
     static constexpr int NLN = GEN_NREG;
+    static constexpr bool HAS_LINES = true;
+    static constexpr bool SPHERE_SKIP = true;
     struct Ctx {
         double support_in_h;  // H / h
         const GenTab* tab;
@@ -362,6 +367,60 @@ struct PolicyGen {
     }
     static __device__ __forceinline__ double plane_weight() { return 0.5 * GEN_PI; }
 };
+
+// Tabulated cubic spline (m4_table.h): the lattice value is the plain sum of the 6 tabulated vertex
+// integrals anchored at the point (no plane or line terms).  The table's interpolation error does not
+// cancel outside the support, so, as in the reference, every bounding-box cell is evaluated.
+struct PolicyTab {
+    using Real = double;
+    static constexpr int NPL = 1;  // unused
+    static constexpr int NLN = 1;  // unused
+    static constexpr bool HAS_LINES = false;
+    static constexpr bool SPHERE_SKIP = false;
+    struct Ctx {
+        double support_in_h;
+        const double* table;
+    };
+    static __device__ __forceinline__ double unit_in_h(const Ctx&) { return 1.0; }
+    static __device__ __forceinline__ double support2() { return 4.0; }
+    static __device__ __forceinline__ int region(const Ctx&, double r2) { return m4_region(r2); }
+    static __device__ __forceinline__ void plane(const Ctx&, double, double* v) { v[0] = 0.0; }
+    static __device__ __forceinline__ int line_slot(int) { return -1; }
+    static __device__ __forceinline__ void line(const Ctx&, double, double, double* v) { v[0] = 0.0; }
+    static __device__ __forceinline__ int line_kind(double) { return 0; }
+    static __device__ __forceinline__ double free_part(const Ctx& c, double ax, double ay, double az, double, int) {
+        // pyramid (wall a, edge normal b, along c): r0 = d_a, R0 = d_b, cos(phi) = d_b / sqrt(d_b^2 + d_c^2)
+        const double rxy = sqrt(ax * ax + ay * ay), rxz = sqrt(ax * ax + az * az), ryz = sqrt(ay * ay + az * az);
+        const double* T = c.table;
+        return ((m4_gridded_value(T, ax, ay, ay / ryz) + m4_gridded_value(T, ax, az, az / ryz)) +
+                (m4_gridded_value(T, ay, ax, ax / rxz) + m4_gridded_value(T, ay, az, az / rxz))) +
+               (m4_gridded_value(T, az, ax, ax / rxy) + m4_gridded_value(T, az, ay, ay / rxy));
+    }
+    static __device__ __forceinline__ double plane_weight() { return 0.0; }
+};
+
+// the lookup table (src/kernel_integral.cpp:57-119): entry (i, j, k), i = 1..nr0, j < nR0, k < nph
+__global__ void k_table_build(double* __restrict__ tab) {
+    using G = M4TableGeom;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t n = static_cast<int64_t>(G::nr0) * G::nR0 * G::nph;
+    if (t >= n) return;
+    const int k = static_cast<int>(t % G::nph);
+    const int j = static_cast<int>((t / G::nph) % G::nR0);
+    const int i = static_cast<int>(t / (static_cast<int64_t>(G::nph) * G::nR0)) + 1;
+    const double r0 = G::r0_of(i), mu0 = G::mu0_of(j);
+    const double R0 = r0 * sqrt(1.0 - mu0 * mu0) / mu0;  // j = 0: +inf, on purpose
+    tab[G::at(i, j, k)] = m4_vertex_value(r0, R0, acos(G::cphi_of(k)));
+}
+
+// row nr0 + 1 repeats row nr0 (the interpolation reads i + 1 of the last row)
+__global__ void k_table_pad(double* __restrict__ tab) {
+    using G = M4TableGeom;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= G::nR0 * G::nph) return;
+    const int j = t / G::nph, k = t % G::nph;
+    tab[G::at(G::nr0 + 1, j, k)] = tab[G::at(G::nr0, j, k)];
+}
 
 template <typename R, int N>
 struct LineVals {
@@ -572,7 +631,7 @@ __global__ __launch_bounds__(THREADS) void k_particles(
                     const Pair2<R> ex = L.sg[w.i], ey = L.sg[nx + w.j], ez = L.sg[nx + ny + w.k];
                     if (ex.a == R(0) || ey.a == R(0) || ez.a == R(0)) {
                         L.U[u] = R(0);  // every pyramid anchored in a plane through the particle is flat
-                    } else if (ex.b + ey.b + ez.b < S2) {
+                    } else if (!P::SPHERE_SKIP || ex.b + ey.b + ez.b < S2) {
                         const R r2 = ex.a * ex.a + ey.a * ey.a + ez.a * ez.a;
                         isA = r2 < S2;
                         isB = !isA;
@@ -625,7 +684,7 @@ __global__ __launch_bounds__(THREADS) void k_particles(
             for (int u = lane; u < ncells; u += 64, wc.next(my, mz)) {
                 const int i = wc.i, j = wc.j, k = wc.k;
                 double Ms = 0.0;
-                if (L.cg2[i] + L.cg2[nx + j] + L.cg2[nx + ny + k] < S2) {
+                if (!P::SPHERE_SKIP || L.cg2[i] + L.cg2[nx + j] + L.cg2[nx + ny + k] < S2) {
                     ++n_sphere;
                     const R* U0 = &L.U[(i * ny + j) * nz + k];
                     Ms = static_cast<double>(U0[nyz]) + static_cast<double>(U0[nz]) + static_cast<double>(U0[1]) +
@@ -844,7 +903,8 @@ struct kiwi_hip_ctx {
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     std::string err;
     DevBuf parts, bbox, sizes, offsets, keys_a, keys_b, pos_a, pos_b, cells, slab_sorted, slab, cub_tmp, ranges, work,
-        work_sorted, order_in, order, lines, ms, stats, outbuf, vsum, tables;
+        work_sorted, order_in, order, lines, ms, stats, outbuf, vsum, tables, m4table;
+    bool m4table_ready = false;
 };
 
 namespace {
@@ -901,7 +961,7 @@ hipError_t launch_cells(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pc
                         int clamp, hipEvent_t ev_lines, hipEvent_t ev_cells, hipEvent_t ev_accum) {
     using R = typename P::Real;
     using Line = LineVals<R, P::NLN>;
-    if (n > 0)
+    if (n > 0 && P::HAS_LINES)
         k_lines<P><<<resident_grid(c, k_lines<P>, 1, n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], bbox, loffs,
                                                                           g, pctx, static_cast<Line*>(lines), dst);
     hipError_t e = hipGetLastError();
@@ -974,7 +1034,8 @@ void kiwi_hip_destroy(kiwi_hip_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->parts, &c->bbox, &c->sizes, &c->offsets, &c->keys_a, &c->keys_b, &c->pos_a, &c->pos_b,
                       &c->cells, &c->slab_sorted, &c->slab, &c->cub_tmp, &c->ranges, &c->work, &c->work_sorted,
-                      &c->order_in, &c->order, &c->lines, &c->ms, &c->stats, &c->outbuf, &c->vsum, &c->tables})
+                      &c->order_in, &c->order, &c->lines, &c->ms, &c->stats, &c->outbuf, &c->vsum, &c->tables,
+                      &c->m4table})
         if (b->p) (void)hipFree(b->p);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1009,7 +1070,6 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     const bool generic = K->generic_path || K->kernel_id != KIWI_KERNEL_M4 || sup != 2.0;
     if (K->tabulated && generic)
         return fail(c, KIWI_ERR_INVALID_ARG, "the tabulated integral is only available on the cubic-spline fast path");
-    if (K->tabulated) return fail(c, KIWI_ERR_UNSUPPORTED, "the tabulated cubic-spline integral is not built yet");
     if (P->n > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 particles in one call");
     if (G->nx > MAX_AXIS || G->ny > MAX_AXIS || G->nz > MAX_AXIS)
         return fail(c, KIWI_ERR_LIMIT, "more than 8192 cells along a grid axis");
@@ -1107,11 +1167,13 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     if (npairs > 0x7fffffffLL)
         return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 (particle, cell-block) items in one call");
 
-    // the generic path always computes in double precision (mixed applies to the cubic path)
-    const bool fp64 = generic || precision == KIWI_FP64;
+    // the generic and tabulated paths always compute in double precision (mixed applies to the cubic path)
+    const bool tabulated = K->tabulated != 0;
+    const bool fp64 = generic || tabulated || precision == KIWI_FP64;
     const size_t rsz = fp64 ? sizeof(double) : sizeof(float);
-    const size_t line_entry = generic ? sizeof(LineVals<double, PolicyGen::NLN>)
-                                      : (fp64 ? sizeof(LineVals<double, 2>) : sizeof(LineVals<float, 2>));
+    const size_t line_entry = tabulated ? 0
+                              : generic ? sizeof(LineVals<double, PolicyGen::NLN>)
+                                        : (fp64 ? sizeof(LineVals<double, 2>) : sizeof(LineVals<float, 2>));
     const int64_t np1 = npairs + 1;
     KIWI_HIP_CHECK(c, reserve(c->lines, line_entry * static_cast<size_t>(std::max<int64_t>(nlines_total, 1))));
     KIWI_HIP_CHECK(c, reserve(c->ms, rsz * static_cast<size_t>(std::max<unsigned long long>(pairs_bbox, 1))));
@@ -1176,7 +1238,22 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[1], s));
     const int clamp = fp64 ? 1 : 0;
     const int nb = static_cast<int>(nblocks);
-    if (generic) {
+    if (tabulated) {
+        if (!c->m4table_ready) {  // built once per context: 22.4M analytic vertex integrals, 182 MB
+            using G = M4TableGeom;
+            KIWI_HIP_CHECK(c, reserve(c->m4table, sizeof(double) * static_cast<size_t>(G::size)));
+            KIWI_HIP_CHECK(c, hipMemsetAsync(c->m4table.p, 0, sizeof(double) * static_cast<size_t>(G::size), s));
+            k_table_build<<<grid1d(static_cast<int64_t>(G::nr0) * G::nR0 * G::nph), THREADS, 0, s>>>(
+                static_cast<double*>(c->m4table.p));
+            k_table_pad<<<grid1d(G::nR0 * G::nph), THREADS, 0, s>>>(static_cast<double*>(c->m4table.p));
+            KIWI_HIP_CHECK(c, hipGetLastError());
+            c->m4table_ready = true;
+        }
+        const PolicyTab::Ctx pc{sup, static_cast<const double*>(c->m4table.p)};
+        KIWI_HIP_CHECK(c, launch_cells<PolicyTab>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
+                                                  slab_sorted, blk_begin, blk_end, order, nb, g, c->lines.p, c->ms.p, om,
+                                                  dst, clamp, c->ev[2], c->ev[3], c->ev[4]));
+    } else if (generic) {
         const PolicyGen::Ctx pc{sup, static_cast<const GenTab*>(c->tables.p) + K->kernel_id};
         KIWI_HIP_CHECK(c, launch_cells<PolicyGen>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
                                                   slab_sorted, blk_begin, blk_end, order, nb, g, c->lines.p, c->ms.p, om,

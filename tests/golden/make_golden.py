@@ -12,7 +12,7 @@ fixtures it writes are plain numbers (inputs and the reference's outputs), commi
   m4_gridded.npz        (h, phi, r0, R0) -> gridded_integral on the generated lookup table
   kernel_grids.npz      `kiwi sph-to-grid` (reference CLI, OMP_NUM_THREADS=1) on every 4th C1 particle
                         onto 20^3 for each kernel on the generic path (M4 forced generic, M4 with
-                        support 2.5h, M5, M6, WC2, WC4, WC6)
+                        support 2.5h, M5, M6, WC2, WC4, WC6) and for the tabulated cubic path
 
   python tests/golden/make_golden.py [--only kernel_grids]
 """
@@ -44,7 +44,7 @@ def read_vgtcgrd2(path):
     return (int(nx), int(ny), int(nz)), dom, body
 
 
-def run_ref_sph_to_grid(parts, domain, res, kernel=None, threads=1, extra=None):
+def run_ref_sph_to_grid(parts, domain, res, kernel=None, threads=1, extra=None, tabulated=False):
     with tempfile.TemporaryDirectory() as td:
         snap = os.path.join(td, "snap.txt")
         workloads.write_plain_ascii(parts, snap)
@@ -55,6 +55,8 @@ def run_ref_sph_to_grid(parts, domain, res, kernel=None, threads=1, extra=None):
                                   "resolution_y": int(res[1]), "resolution_z": int(res[2])}}
         if kernel:
             cfg["kernel"] = kernel
+        if tabulated:
+            cfg["cartesian_grid"]["use_tabulated_integral"] = True
         if extra:
             cfg.update(extra)
         cp = os.path.join(td, "cfg.json")
@@ -80,8 +82,8 @@ def kernel_grids():
     domain = np.array(c1["domain"])
     res = (20, 20, 20)
     out = {"domain": domain, "res": np.array(res), **{f"p_{k}": v for k, v in parts.items()}}
-    for label, kcfg in KERNEL_GRID_CASES:
-        shape, dom, body, _ = run_ref_sph_to_grid(parts, domain, res, kernel=kcfg)
+    for label, kcfg in KERNEL_GRID_CASES + [("M4_tabulated", None)]:
+        shape, dom, body, _ = run_ref_sph_to_grid(parts, domain, res, kernel=kcfg, tabulated=(label == "M4_tabulated"))
         assert tuple(shape) == res
         for i, f in enumerate(("density", "vx", "vy", "vz")):
             out[f"{label}_{f}"] = body[i]

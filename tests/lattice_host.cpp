@@ -103,3 +103,37 @@ double lat_gen_box(int kid, const double* p, double H, const double* lo, const d
 }  // extern "C"
 
 extern "C" double lat_gen_W(int kid, double ax, double ay, double az) { return gen_lattice_W(kTabs[kid], ax, ay, az); }
+
+// ---- tabulated cubic path (m4_table.h) ----
+#include <vector>
+
+#include "../kiwi_amd/csrc/m4_table.h"
+
+namespace {
+std::vector<double>& host_table() {
+    static std::vector<double> t;
+    if (t.empty()) {
+        using G = M4TableGeom;
+        t.assign(static_cast<size_t>(G::size), 0.0);
+        for (int i = 1; i <= G::nr0; ++i)
+            for (int j = 0; j < G::nR0; ++j) {
+                const double r0 = G::r0_of(i), mu0 = G::mu0_of(j);
+                const double R0 = r0 * std::sqrt(1.0 - mu0 * mu0) / mu0;  // j = 0: +inf
+                for (int k = 0; k < G::nph; ++k) t[G::at(i, j, k)] = m4_vertex_value(r0, R0, std::acos(G::cphi_of(k)));
+            }
+        for (int j = 0; j < G::nR0; ++j)
+            for (int k = 0; k < G::nph; ++k) t[G::at(G::nr0 + 1, j, k)] = t[G::at(G::nr0, j, k)];
+    }
+    return t;
+}
+}  // namespace
+
+extern "C" {
+double lat_m4_vertex_value(double h, double phi, double r0, double R0) {
+    return m4_vertex_value(r0 / h, R0 / h, phi);  // dimensionless: the same at every h
+}
+double lat_m4_gridded(double h, double phi, double r0, double R0) {
+    if (phi == 0.0) return 0.0;
+    return m4_gridded_value(host_table().data(), r0 / h, R0 / h, std::cos(phi));
+}
+}  // extern "C"

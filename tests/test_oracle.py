@@ -97,18 +97,19 @@ def test_c1_grid_matches_reference_cli():
 
 
 KERNEL_GRID_CASES = [("M4_generic", "M4", 0.0), ("M4_support2.5", "M4", 2.5), ("M5", "M5", 0.0), ("M6", "M6", 0.0),
-                     ("WC2", "WC2", 0.0), ("WC4", "WC4", 0.0), ("WC6", "WC6", 0.0)]
+                     ("WC2", "WC2", 0.0), ("WC4", "WC4", 0.0), ("WC6", "WC6", 0.0), ("M4_tabulated", "M4", 0.0)]
 
 
 @pytest.mark.parametrize("label,name,support", KERNEL_GRID_CASES)
 def test_generic_grids_match_reference_cli(label, name, support):
-    """The oracle's generic path against the reference CLI on every 4th C1 particle onto 20^3, for each
-    kernel (fixture written with OMP_NUM_THREADS=1): identical to the last bit."""
+    """The oracle's generic and tabulated paths against the reference CLI on every 4th C1 particle onto
+    20^3, for each kernel (fixture written with OMP_NUM_THREADS=1): identical to the last bit."""
     g = np.load(os.path.join(GOLD, "kernel_grids.npz"))
     parts = {k: g["p_" + k] for k in ("x", "y", "z", "vx", "vy", "vz", "m", "h")}
     dom = g["domain"]
     res = tuple(int(v) for v in g["res"])
-    k = O.kernel(name, support_in_h=support, generic=True)
+    tab = label == "M4_tabulated"
+    k = O.kernel(name, support_in_h=support, generic=not tab, tabulated=tab)
     mass, px, py, pz, st = O.sph_to_grid(parts, dom, *res, k=k, threads=1)
     V = ((dom[1] - dom[0]) / res[0]) * ((dom[3] - dom[2]) / res[1]) * ((dom[5] - dom[4]) / res[2])
     assert np.array_equal(mass / V, g[label + "_density"])
