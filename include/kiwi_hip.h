@@ -7,13 +7,16 @@
  *                             (src/cartesian_grid.cpp:184-301; include/kiwi/cartesian_grid.hpp:88),
  *                             i.e. the 6 x KernelIntegrator::face calls per cell
  *                             (include/kiwi/kernel_integrator.hpp:22-27).
+ *   kiwi_hip_voronoi_map   <- the particle loop of run_mapping_stage (src/mapping.cpp:400-460): the BFS
+ *                             over the tessellation with compute_cell_integral (:105-206) per cell.
  *
  * The host keeps what the reference keeps outside those loops: configuration, snapshot reading,
  * normalisation (src/cartesian_grid.cpp:294-318, src/mapping.cpp:473-502), file output and caching.
  *
  * Outputs are the UN-normalised per-cell sums the reference accumulates:
  *   mass[c] = sum_p Msum(p,c) m_p,   p*[c] = sum_p Msum(p,c) m_p v*_p
- * Cartesian cells are x-major: c = (ix*ny + iy)*nz + iz (include/kiwi/cartesian_grid.hpp:62-66).
+ * Cartesian cells are x-major: c = (ix*ny + iy)*nz + iz (include/kiwi/cartesian_grid.hpp:62-66);
+ * Voronoi cells are in the order of the geometry the caller passes (the reference's c_loop_all order).
  *
  * Return codes: KIWI_OK (0) or a negative KIWI_ERR_*; kiwi_hip_last_error() then holds the message.
  * A context owns one HIP stream and its device scratch; it must not be used from two threads at once.
@@ -28,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KIWI_HIP_ABI_VERSION 1
+#define KIWI_HIP_ABI_VERSION 2
 
 #define KIWI_OK 0
 #define KIWI_ERR_INVALID_ARG (-1)
@@ -108,6 +111,37 @@ typedef struct kiwi_stats {
     kiwi_warn_record warn[KIWI_MAX_WARN_RECORDS];
 } kiwi_stats;
 
+/* A Voronoi tessellation flattened to plain arrays (the reference's CellFaceGeometry, src/mapping.cpp:49-54,
+ * resolved to absolute coordinates and cell indices the way compute_cell_integral reads it, :130-175).
+ * Cell c owns faces [cell_face_ptr[c], cell_face_ptr[c+1]); face f owns vertices
+ * [face_vert_ptr[f], face_vert_ptr[f+1]), in voro++'s face_vertices() order, as xyz triples in vert_xyz.
+ * face_neighbor[f] is the cell index across face f, or -1 for a domain wall or an unresolved neighbour. */
+typedef struct kiwi_voronoi_cells {
+    int64_t n_cells, n_faces, n_face_vertices;
+    const int64_t* cell_face_ptr; /* n_cells + 1 */
+    const int64_t* face_vert_ptr; /* n_faces + 1 */
+    const double* vert_xyz;       /* 3 * n_face_vertices */
+    const int* face_neighbor;     /* n_faces */
+    double domain[6];             /* x_min x_max y_min y_max z_min z_max: the tessellated box */
+    int on_device;                /* 1: device pointers, 0: host pointers */
+} kiwi_voronoi_cells;
+
+typedef struct kiwi_map_stats {
+    int64_t unresolved;      /* particles without a home cell (skipped, src/mapping.cpp:436-440) */
+    int64_t unmapped;        /* particles whose kernel reached no cell beyond the home cell (:459) */
+    int64_t n_negative;      /* (particle, cell) sums < 0, clamped to 0 (:198-203) */
+    int64_t n_negative_warn; /* of which <= -0.01 */
+    int64_t pairs_candidate; /* (particle, cell) pairs whose bounding boxes meet the support */
+    int64_t pairs_visited;   /* pairs the reference's BFS visits (home cell, or a face within the support) */
+    int64_t face_evals;      /* face integrals evaluated */
+    double v_mean[3];        /* mean velocity over all particles */
+    double device_ms_prep;   /* geometry, candidate pairs, visits, face lists */
+    double device_ms_faces;  /* face integrals */
+    double device_ms_accum;  /* per-pair sums and per-cell accumulation */
+    int n_warn_records;
+    kiwi_warn_record warn[KIWI_MAX_WARN_RECORDS]; /* cell = the cell index */
+} kiwi_map_stats;
+
 int kiwi_hip_abi_version(void);
 
 /* One context per host thread; binds device_ordinal and creates a stream. */
@@ -122,6 +156,14 @@ void* kiwi_hip_stream(kiwi_hip_ctx* ctx);
 int kiwi_hip_sph_to_grid(kiwi_hip_ctx* ctx, const kiwi_particles* particles, const kiwi_grid* grid,
                          const kiwi_kernel* kernel, int precision, const kiwi_cell_sums* out,
                          kiwi_stats* stats);
+
+/* Voronoi map, replacing the particle loop of run_mapping_stage (src/mapping.cpp:400-460).
+ * home_cell[n]: each particle's cell (voro++ find_voronoi_cell resolved through reverse_id_list,
+ * :434-440), -1 if unresolved; host or device memory as particles->on_device.  Outputs n_cells each.
+ * The face integrals run in double precision whatever `precision` asks for. */
+int kiwi_hip_voronoi_map(kiwi_hip_ctx* ctx, const kiwi_particles* particles, const int* home_cell,
+                         const kiwi_voronoi_cells* cells, const kiwi_kernel* kernel, int precision,
+                         const kiwi_cell_sums* out, kiwi_map_stats* stats);
 
 #ifdef __cplusplus
 }

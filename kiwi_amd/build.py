@@ -26,9 +26,9 @@ def _run(cmd):
 
 
 def build_lib():
-    src = os.path.join(PKG_DIR, "csrc", "kiwi_hip.hip")
+    srcs = [os.path.join(PKG_DIR, "csrc", f) for f in ("kiwi_hip.hip", "kiwi_voronoi.hip")]
     tmp = LIB_PATH + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT_DIR, "include"), src, "-o", tmp])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT_DIR, "include"), *srcs, "-o", tmp])
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

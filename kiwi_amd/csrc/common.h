@@ -1,0 +1,202 @@
+// Shared by the library's translation units: launch shape, small device helpers, and the context
+// (device, stream, events, device buffers that grow and are reused across calls).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+
+#include "../../include/kiwi_hip.h"
+#include "kernel_tables.h"
+
+namespace {
+
+constexpr int THREADS = 256;
+constexpr int WAVES = THREADS / 64;
+
+template <typename R>
+struct Pair2 {
+    R a, b;
+};
+
+// select one of three by a runtime axis index (no indexed private arrays: those would live in scratch)
+template <typename T>
+__device__ __forceinline__ T sel3(int a, T v0, T v1, T v2) { return a == 0 ? v0 : (a == 1 ? v1 : v2); }
+
+// u / d for non-negative integers through a float reciprocal inv_d ~ 1/d (within a few ulp), corrected
+// by one step either way to be exact; valid while u / d < 2^22
+__device__ __forceinline__ int small_div(int u, int d, float inv_d) {
+    int q = static_cast<int>((static_cast<float>(u) + 0.5f) * inv_d);
+    const int r = u - q * d;
+    q += (r >= d) ? 1 : 0;
+    q -= (r < 0) ? 1 : 0;
+    return q;
+}
+
+// distance component of the interval [lo, hi] (units of h) from the particle
+template <typename R>
+__device__ __forceinline__ R gap(R lo, R hi) {
+    return (lo > R(0)) ? lo : ((hi < R(0)) ? -hi : R(0));
+}
+
+// a value that is the same in every lane, moved to a scalar register
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// LDS writes of one lane made visible to the other lanes of the same wavefront
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+inline int grid1d(int64_t n) { return static_cast<int>((n + THREADS - 1) / THREADS); }
+
+}  // namespace
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+struct kiwi_hip_ctx {
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    std::string err;
+    DevBuf parts, bbox, sizes, offsets, keys_a, keys_b, pos_a, pos_b, cells, slab_sorted, slab, cub_tmp, ranges, work,
+        work_sorted, order_in, order, lines, ms, stats, outbuf, vsum, tables, m4table;
+    bool m4table_ready = false;
+    DevBuf vor[16];  // the Voronoi map's buffers
+    template <class F>
+    void for_each_buf(F f) {
+        for (DevBuf* b : {&parts, &bbox, &sizes, &offsets, &keys_a, &keys_b, &pos_a, &pos_b, &cells, &slab_sorted,
+                          &slab, &cub_tmp, &ranges, &work, &work_sorted, &order_in, &order, &lines, &ms, &stats,
+                          &outbuf, &vsum, &tables, &m4table})
+            f(*b);
+        for (DevBuf& b : vor) f(b);
+    }
+};
+
+namespace {
+
+int fail(kiwi_hip_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(kiwi_hip_ctx* c, hipError_t e, const char* what) {
+    return fail(c, e == hipErrorOutOfMemory ? KIWI_ERR_OUT_OF_MEMORY : KIWI_ERR_DEVICE,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define KIWI_HIP_CHECK(ctx, expr)                                \
+    do {                                                         \
+        hipError_t e_ = (expr);                                  \
+        if (e_ != hipSuccess) return hip_fail((ctx), e_, #expr); \
+    } while (0)
+
+hipError_t reserve(DevBuf& b, size_t bytes) {
+    if (bytes <= b.cap) return hipSuccess;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        if (e != hipSuccess) return e;
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    const size_t want = std::max<size_t>(bytes + bytes / 8, 256);
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) return e;
+    b.cap = want;
+    return hipSuccess;
+}
+
+// workgroups of a persistent kernel: as many as are resident on the device at once (every wave
+// loops until the work runs out), fewer for small jobs
+template <typename K>
+int resident_grid(const kiwi_hip_ctx* c, K kernel, int64_t items_per_wave_round, int64_t items) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, THREADS, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    const int64_t full = static_cast<int64_t>(per_cu) * std::max(c->n_cu, 1);
+    const int64_t need = (items + WAVES * items_per_wave_round - 1) / (WAVES * items_per_wave_round);
+    return static_cast<int>(std::max<int64_t>(1, std::min(full, need)));
+}
+
+// deterministic velocity sums over all particles: fixed per-thread strides, fixed tree order
+constexpr int VSUM_BLOCKS = 256;
+
+__global__ __launch_bounds__(THREADS) void k_vsum_partial(int64_t n, const double* __restrict__ vx,
+                                                          const double* __restrict__ vy,
+                                                          const double* __restrict__ vz, double* partial) {
+    __shared__ double s[3][THREADS];
+    double a = 0, b = 0, c = 0;
+    const int64_t stride = static_cast<int64_t>(VSUM_BLOCKS) * THREADS;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x; i < n; i += stride) {
+        a += vx[i];
+        b += vy[i];
+        c += vz[i];
+    }
+    s[0][threadIdx.x] = a;
+    s[1][threadIdx.x] = b;
+    s[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int w = THREADS / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w)
+            for (int k = 0; k < 3; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) partial[3 * blockIdx.x + k] = s[k][0];
+}
+
+__global__ void k_vsum_final(const double* partial, double* out) {
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 3; ++k) {
+        double t = 0;
+        for (int i = 0; i < VSUM_BLOCKS; ++i) t += partial[3 * i + k];
+        out[k] = t;
+    }
+}
+
+}  // namespace
+
+namespace {
+
+struct GenTab {  // the fields gen_lattice.h reads, copied from a KiwiKernelTable
+    int nreg;
+    double c[KIWI_TAB_MAXREG];
+    double A3_pi;
+    double a[KIWI_TAB_MAXREG][KIWI_TAB_NI];
+    double B[KIWI_TAB_MAXREG][KIWI_TAB_MAXREG][KIWI_TAB_NB];
+};
+
+// the kernel tables on the host, in KIWI_KERNEL_M4 .. KIWI_KERNEL_WC6 order
+const KiwiKernelTable kHostTables[KIWI_TAB_NKERNELS] = {KIWI_KERNEL_TABLE_DATA};
+
+// Which integral a kernel request selects: make_kernel_integrator (include/kiwi/kernel_integrator.hpp:
+// 14-46, src/config.cpp:269-277) -- the cubic analytic path iff M4 with support 2h and no generic request,
+// the generic path otherwise; the tabulated integral only on the cubic path (src/config.cpp:174-178).
+struct KernelChoice {
+    double support_in_h;
+    bool generic, tabulated;
+};
+
+inline int resolve_kernel(kiwi_hip_ctx* c, const kiwi_kernel* K, KernelChoice* out) {
+    if (K->kernel_id < 0 || K->kernel_id > KIWI_KERNEL_WC6) return fail(c, KIWI_ERR_INVALID_ARG, "unknown kernel id");
+    if (!(K->support_in_h >= 0.0) || !(K->support_in_h < INFINITY))  // src/config.cpp:169-170
+        return fail(c, KIWI_ERR_INVALID_ARG, "kernel.support_radius_in_h must be > 0 (or 0 for the kernel's default)");
+    out->support_in_h = K->support_in_h > 0 ? K->support_in_h : kHostTables[K->kernel_id].radkern;
+    out->generic = K->generic_path || K->kernel_id != KIWI_KERNEL_M4 || out->support_in_h != 2.0;
+    out->tabulated = K->tabulated != 0;
+    if (out->tabulated && out->generic)
+        return fail(c, KIWI_ERR_INVALID_ARG, "the tabulated integral is only available on the cubic-spline fast path");
+    return KIWI_OK;
+}
+
+}  // namespace
+
+// the tabulated cubic spline's lookup table of the context (built on first use; kiwi_hip.hip)
+hipError_t kiwi_build_m4_table(kiwi_hip_ctx* c, hipStream_t s);

@@ -27,7 +27,7 @@
 //                    cells and adds Msum m and Msum m v of the block's items, read from the block's
 //                    contiguous slab, in list order.  Each cell is owned by exactly one thread: no
 //                    float atomics, and the results are bit-reproducible.
-//   9. k_vsum        deterministic mean velocity over all particles (the empty-cell fallback).
+//   9. k_vsum        deterministic mean velocity over all particles (the empty-cell fallback; common.h).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -39,7 +39,7 @@
 #include <cstring>
 #include <string>
 
-#include "../../include/kiwi_hip.h"
+#include "common.h"
 #include "gen_lattice.h"
 #include "m4_lattice.h"
 #include "m4_table.h"
@@ -48,8 +48,6 @@ using namespace kiwi_hip;
 
 namespace {
 
-constexpr int THREADS = 256;
-constexpr int WAVES = THREADS / 64;
 constexpr int BLK = 8;                        // cell block edge (cells)
 constexpr int BLK_PTS = (BLK + 1) * (BLK + 1) * (BLK + 1);
 constexpr int MAX_AXIS = 8192;                // grid cells per axis
@@ -69,42 +67,8 @@ struct GridGeom {
     int nbx, nby, nbz;  // cell blocks per axis
 };
 
-template <typename R>
-struct Pair2 {
-    R a, b;
-};
-
-// select one of three by a runtime axis index (no indexed private arrays: those would live in scratch)
-template <typename T>
-__device__ __forceinline__ T sel3(int a, T v0, T v1, T v2) { return a == 0 ? v0 : (a == 1 ? v1 : v2); }
 __device__ __forceinline__ double gmin(const GridGeom& g, int a) { return sel3(a, g.x_min, g.y_min, g.z_min); }
 __device__ __forceinline__ double gstep(const GridGeom& g, int a) { return sel3(a, g.dx, g.dy, g.dz); }
-
-// u / d for non-negative integers through a float reciprocal inv_d ~ 1/d (within a few ulp), corrected
-// by one step either way to be exact; valid while u / d < 2^22
-__device__ __forceinline__ int small_div(int u, int d, float inv_d) {
-    int q = static_cast<int>((static_cast<float>(u) + 0.5f) * inv_d);
-    const int r = u - q * d;
-    q += (r >= d) ? 1 : 0;
-    q -= (r < 0) ? 1 : 0;
-    return q;
-}
-
-// distance component of the interval [lo, hi] (units of h) from the particle
-template <typename R>
-__device__ __forceinline__ R gap(R lo, R hi) {
-    return (lo > R(0)) ? lo : ((hi < R(0)) ? -hi : R(0));
-}
-
-// a value that is the same in every lane, moved to a scalar register
-__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// LDS writes of one lane made visible to the other lanes of the same wavefront
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // ------------------------------------------------------------------------------------------------
 // 1. per-particle bounding box and work sizes
@@ -324,13 +288,6 @@ struct PolicyM4 {
 // Generic tabulated-coefficient kernels (gen_lattice.h), double precision: lattice offsets in units of
 // the support H, support 1; regions of the kernel table; plane values Phi_k; the line table holds
 // the offsets of every region.
-struct GenTab {  // the fields gen_lattice.h reads, copied from a KiwiKernelTable
-    int nreg;
-    double c[KIWI_TAB_MAXREG];
-    double A3_pi;
-    double a[KIWI_TAB_MAXREG][KIWI_TAB_NI];
-    double B[KIWI_TAB_MAXREG][KIWI_TAB_MAXREG][KIWI_TAB_NB];
-};
 
 struct PolicyGen {
     using Real = double;
@@ -844,113 +801,9 @@ __global__ __launch_bounds__(THREADS) void k_accum(
     }
 }
 
-// ------------------------------------------------------------------------------------------------
-// 9. deterministic velocity sums: fixed per-thread strides, fixed tree order
-// ------------------------------------------------------------------------------------------------
-constexpr int VSUM_BLOCKS = 256;
-
-__global__ __launch_bounds__(THREADS) void k_vsum_partial(int64_t n, const double* __restrict__ vx,
-                                                          const double* __restrict__ vy,
-                                                          const double* __restrict__ vz, double* partial) {
-    __shared__ double s[3][THREADS];
-    double a = 0, b = 0, c = 0;
-    const int64_t stride = static_cast<int64_t>(VSUM_BLOCKS) * THREADS;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x; i < n; i += stride) {
-        a += vx[i];
-        b += vy[i];
-        c += vz[i];
-    }
-    s[0][threadIdx.x] = a;
-    s[1][threadIdx.x] = b;
-    s[2][threadIdx.x] = c;
-    __syncthreads();
-    for (int w = THREADS / 2; w > 0; w >>= 1) {
-        if (static_cast<int>(threadIdx.x) < w)
-            for (int k = 0; k < 3; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 3; ++k) partial[3 * blockIdx.x + k] = s[k][0];
-}
-
-__global__ void k_vsum_final(const double* partial, double* out) {
-    if (threadIdx.x != 0) return;
-    for (int k = 0; k < 3; ++k) {
-        double t = 0;
-        for (int i = 0; i < VSUM_BLOCKS; ++i) t += partial[3 * i + k];
-        out[k] = t;
-    }
-}
-
-
-inline int grid1d(int64_t n) { return static_cast<int>((n + THREADS - 1) / THREADS); }
-
 }  // namespace
 
-// ================================================================================================
-// context
-// ================================================================================================
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-};
-
-struct kiwi_hip_ctx {
-    int device = 0;
-    int n_cu = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    std::string err;
-    DevBuf parts, bbox, sizes, offsets, keys_a, keys_b, pos_a, pos_b, cells, slab_sorted, slab, cub_tmp, ranges, work,
-        work_sorted, order_in, order, lines, ms, stats, outbuf, vsum, tables, m4table;
-    bool m4table_ready = false;
-};
-
 namespace {
-
-int fail(kiwi_hip_ctx* c, int code, const std::string& msg) {
-    if (c) c->err = msg;
-    return code;
-}
-
-int hip_fail(kiwi_hip_ctx* c, hipError_t e, const char* what) {
-    return fail(c, e == hipErrorOutOfMemory ? KIWI_ERR_OUT_OF_MEMORY : KIWI_ERR_DEVICE,
-                std::string(what) + ": " + hipGetErrorString(e));
-}
-
-#define KIWI_HIP_CHECK(ctx, expr)                                \
-    do {                                                         \
-        hipError_t e_ = (expr);                                  \
-        if (e_ != hipSuccess) return hip_fail((ctx), e_, #expr); \
-    } while (0)
-
-hipError_t reserve(DevBuf& b, size_t bytes) {
-    if (bytes <= b.cap) return hipSuccess;
-    if (b.p) {
-        hipError_t e = hipFree(b.p);
-        if (e != hipSuccess) return e;
-        b.p = nullptr;
-        b.cap = 0;
-    }
-    const size_t want = std::max<size_t>(bytes + bytes / 8, 256);
-    hipError_t e = hipMalloc(&b.p, want);
-    if (e != hipSuccess) return e;
-    b.cap = want;
-    return hipSuccess;
-}
-
-// workgroups of a persistent kernel: as many as are resident on the device at once (every wave
-// loops until the work runs out), fewer for small jobs
-template <typename K>
-int resident_grid(const kiwi_hip_ctx* c, K kernel, int64_t items_per_wave_round, int64_t items) {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, THREADS, 0) != hipSuccess || per_cu < 1)
-        per_cu = 1;
-    const int64_t full = static_cast<int64_t>(per_cu) * std::max(c->n_cu, 1);
-    const int64_t need = (items + WAVES * items_per_wave_round - 1) / (WAVES * items_per_wave_round);
-    return static_cast<int>(std::max<int64_t>(1, std::min(full, need)));
-}
 
 template <class P>
 hipError_t launch_cells(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pctx, int64_t n, int64_t npairs,
@@ -980,10 +833,22 @@ hipError_t launch_cells(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pc
     return hipEventRecord(ev_accum, s);
 }
 
-// the kernel tables on the host, in KIWI_KERNEL_M4 .. KIWI_KERNEL_WC6 order
-const KiwiKernelTable kHostTables[KIWI_TAB_NKERNELS] = {KIWI_KERNEL_TABLE_DATA};
-
 }  // namespace
+
+// the tabulated cubic spline's lookup table, built once per context: 22.4M analytic vertex integrals, 182 MB
+hipError_t kiwi_build_m4_table(kiwi_hip_ctx* c, hipStream_t s) {
+    if (c->m4table_ready) return hipSuccess;
+    using G = M4TableGeom;
+    hipError_t e = reserve(c->m4table, sizeof(double) * static_cast<size_t>(G::size));
+    if (e != hipSuccess) return e;
+    if ((e = hipMemsetAsync(c->m4table.p, 0, sizeof(double) * static_cast<size_t>(G::size), s)) != hipSuccess) return e;
+    k_table_build<<<grid1d(static_cast<int64_t>(G::nr0) * G::nR0 * G::nph), THREADS, 0, s>>>(
+        static_cast<double*>(c->m4table.p));
+    k_table_pad<<<grid1d(G::nR0 * G::nph), THREADS, 0, s>>>(static_cast<double*>(c->m4table.p));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    c->m4table_ready = true;
+    return hipSuccess;
+}
 
 extern "C" {
 
@@ -1032,11 +897,9 @@ void kiwi_hip_destroy(kiwi_hip_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (DevBuf* b : {&c->parts, &c->bbox, &c->sizes, &c->offsets, &c->keys_a, &c->keys_b, &c->pos_a, &c->pos_b,
-                      &c->cells, &c->slab_sorted, &c->slab, &c->cub_tmp, &c->ranges, &c->work, &c->work_sorted,
-                      &c->order_in, &c->order, &c->lines, &c->ms, &c->stats, &c->outbuf, &c->vsum, &c->tables,
-                      &c->m4table})
-        if (b->p) (void)hipFree(b->p);
+    c->for_each_buf([](DevBuf& b) {
+        if (b.p) (void)hipFree(b.p);
+    });
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1062,14 +925,10 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
         return fail(c, KIWI_ERR_INVALID_ARG, "domain min must be < max on every axis");
     if (precision != KIWI_FP64 && precision != KIWI_MIXED) return fail(c, KIWI_ERR_INVALID_ARG, "unknown precision");
     if (K->kernel_id < 0 || K->kernel_id > KIWI_KERNEL_WC6) return fail(c, KIWI_ERR_INVALID_ARG, "unknown kernel id");
-    // dispatch of make_kernel_integrator (include/kiwi/kernel_integrator.hpp:14-46, src/config.cpp:269-277):
-    // the cubic analytic path iff M4 with support 2h and no generic request, the generic path otherwise
-    if (!(K->support_in_h >= 0.0) || !std::isfinite(K->support_in_h))  // src/config.cpp:169-170
-        return fail(c, KIWI_ERR_INVALID_ARG, "kernel.support_radius_in_h must be > 0 (or 0 for the kernel's default)");
-    const double sup = K->support_in_h > 0 ? K->support_in_h : kHostTables[K->kernel_id].radkern;
-    const bool generic = K->generic_path || K->kernel_id != KIWI_KERNEL_M4 || sup != 2.0;
-    if (K->tabulated && generic)
-        return fail(c, KIWI_ERR_INVALID_ARG, "the tabulated integral is only available on the cubic-spline fast path");
+    KernelChoice kc;
+    if (const int rc = resolve_kernel(c, K, &kc)) return rc;
+    const double sup = kc.support_in_h;
+    const bool generic = kc.generic;
     if (P->n > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 particles in one call");
     if (G->nx > MAX_AXIS || G->ny > MAX_AXIS || G->nz > MAX_AXIS)
         return fail(c, KIWI_ERR_LIMIT, "more than 8192 cells along a grid axis");
@@ -1168,7 +1027,7 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
         return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 (particle, cell-block) items in one call");
 
     // the generic and tabulated paths always compute in double precision (mixed applies to the cubic path)
-    const bool tabulated = K->tabulated != 0;
+    const bool tabulated = kc.tabulated;
     const bool fp64 = generic || tabulated || precision == KIWI_FP64;
     const size_t rsz = fp64 ? sizeof(double) : sizeof(float);
     const size_t line_entry = tabulated ? 0
@@ -1239,16 +1098,7 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     const int clamp = fp64 ? 1 : 0;
     const int nb = static_cast<int>(nblocks);
     if (tabulated) {
-        if (!c->m4table_ready) {  // built once per context: 22.4M analytic vertex integrals, 182 MB
-            using G = M4TableGeom;
-            KIWI_HIP_CHECK(c, reserve(c->m4table, sizeof(double) * static_cast<size_t>(G::size)));
-            KIWI_HIP_CHECK(c, hipMemsetAsync(c->m4table.p, 0, sizeof(double) * static_cast<size_t>(G::size), s));
-            k_table_build<<<grid1d(static_cast<int64_t>(G::nr0) * G::nR0 * G::nph), THREADS, 0, s>>>(
-                static_cast<double*>(c->m4table.p));
-            k_table_pad<<<grid1d(G::nR0 * G::nph), THREADS, 0, s>>>(static_cast<double*>(c->m4table.p));
-            KIWI_HIP_CHECK(c, hipGetLastError());
-            c->m4table_ready = true;
-        }
+        KIWI_HIP_CHECK(c, kiwi_build_m4_table(c, s));
         const PolicyTab::Ctx pc{sup, static_cast<const double*>(c->m4table.p)};
         KIWI_HIP_CHECK(c, launch_cells<PolicyTab>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
                                                   slab_sorted, blk_begin, blk_end, order, nb, g, c->lines.p, c->ms.p, om,

@@ -4,6 +4,8 @@ This mirrors the reference's stage interface for the mapping loop: `sph_to_grid`
 columns of SPHSnapshot (include/kiwi/snapshot.hpp:70-77), the domain and resolved resolution
 (src/cartesian_grid.cpp:89-97) and the kernel settings (include/kiwi/kernel_integrator.hpp:118-122),
 and returns the un-normalised per-cell sums; `normalise` then applies src/cartesian_grid.cpp:294-318.
+`voronoi_map` does the same over a flattened Voronoi tessellation (the particle loop of
+run_mapping_stage, src/mapping.cpp:400-460), and `normalise_voronoi` applies src/mapping.cpp:462-506.
 Errors from the library raise KiwiError carrying the library's message (the reference throws
 std::runtime_error / ConfigError at the same points).
 
@@ -29,7 +31,7 @@ _i = ctypes.c_int
 _pd = ctypes.POINTER(ctypes.c_double)
 
 EXPORTED = ["kiwi_hip_abi_version", "kiwi_hip_create", "kiwi_hip_destroy", "kiwi_hip_last_error",
-            "kiwi_hip_stream", "kiwi_hip_sph_to_grid"]
+            "kiwi_hip_stream", "kiwi_hip_sph_to_grid", "kiwi_hip_voronoi_map"]
 
 
 class KiwiError(RuntimeError):
@@ -77,6 +79,29 @@ class Stats(ctypes.Structure):
         return d
 
 
+_pll = ctypes.POINTER(ctypes.c_int64)
+_pi = ctypes.POINTER(ctypes.c_int)
+
+
+class VoronoiCells(ctypes.Structure):
+    _fields_ = [("n_cells", _i64), ("n_faces", _i64), ("n_face_vertices", _i64), ("cell_face_ptr", _pll),
+                ("face_vert_ptr", _pll), ("vert_xyz", _pd), ("face_neighbor", _pi), ("domain", _d * 6),
+                ("on_device", _i)]
+
+
+class MapStats(ctypes.Structure):
+    _fields_ = [("unresolved", _i64), ("unmapped", _i64), ("n_negative", _i64), ("n_negative_warn", _i64),
+                ("pairs_candidate", _i64), ("pairs_visited", _i64), ("face_evals", _i64), ("v_mean", _d * 3),
+                ("device_ms_prep", _d), ("device_ms_faces", _d), ("device_ms_accum", _d),
+                ("n_warn_records", _i), ("warn", WarnRecord * 16)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f not in ("v_mean", "warn")}
+        d["v_mean"] = list(self.v_mean)
+        d["warn"] = [(w.particle, w.cell, w.value) for w in self.warn[: self.n_warn_records]]
+        return d
+
+
 _lib = None
 
 
@@ -104,6 +129,10 @@ def load():
         L.kiwi_hip_sph_to_grid.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.POINTER(Grid),
                                            ctypes.POINTER(Kernel), _i, ctypes.POINTER(CellSums),
                                            ctypes.POINTER(Stats)]
+        L.kiwi_hip_voronoi_map.restype = _i
+        L.kiwi_hip_voronoi_map.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.c_void_p,
+                                           ctypes.POINTER(VoronoiCells), ctypes.POINTER(Kernel), _i,
+                                           ctypes.POINTER(CellSums), ctypes.POINTER(MapStats)]
         _lib = L
     return _lib
 
@@ -115,14 +144,23 @@ def kernel_spec(name="M4", support_in_h=0.0, generic_path=False, tabulated=False
     return Kernel(KERNELS.index(key), float(support_in_h), int(bool(generic_path)), int(bool(tabulated)))
 
 
-def _ptr(a):
-    """Host numpy float64 array or device tensor -> (double*, on_device)."""
+def _ptr(a, ctype=_pd, dtype="float64"):
+    """Host numpy array or device tensor of the given dtype -> (pointer, on_device)."""
     if isinstance(a, np.ndarray):
-        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
-        return a.ctypes.data_as(_pd), 0
+        assert a.dtype == np.dtype(dtype) and a.flags["C_CONTIGUOUS"], (a.dtype, dtype)
+        return a.ctypes.data_as(ctype), 0
     # torch tensor (device memory plumbing only)
-    assert str(a.dtype) == "torch.float64" and a.is_contiguous()
-    return ctypes.cast(ctypes.c_void_p(a.data_ptr()), _pd), int(a.is_cuda)
+    assert str(a.dtype) == f"torch.{dtype}" and a.is_contiguous(), (a.dtype, dtype)
+    return ctypes.cast(ctypes.c_void_p(a.data_ptr()), ctype), int(a.is_cuda)
+
+
+def _particles(parts):
+    cols = [parts[c] for c in ("x", "y", "z", "vx", "vy", "vz", "m", "h")]
+    ptrs = [_ptr(c) for c in cols]
+    on_dev = {p[1] for p in ptrs}
+    if len(on_dev) != 1:
+        raise ValueError("particle columns must all be host arrays or all device tensors")
+    return Particles(len(cols[0]), *[p[0] for p in ptrs], on_dev.pop())
 
 
 class Context:
@@ -167,13 +205,7 @@ class Context:
         out: None (host numpy results) or a tuple of 4 device float64 tensors of nx*ny*nz.
         Returns (mass, px, py, pz, Stats)."""
         kernel = kernel or kernel_spec()
-        cols = [parts[c] for c in ("x", "y", "z", "vx", "vy", "vz", "m", "h")]
-        ptrs = [_ptr(c) for c in cols]
-        on_dev = {p[1] for p in ptrs}
-        if len(on_dev) != 1:
-            raise ValueError("particle columns must all be host arrays or all device tensors")
-        n = len(cols[0])
-        P = Particles(n, *[p[0] for p in ptrs], on_dev.pop())
+        P = _particles(parts)
         nx, ny, nz = (int(v) for v in res)
         G = Grid(*[float(v) for v in domain], nx, ny, nz)
         nc = nx * ny * nz
@@ -187,6 +219,49 @@ class Context:
         self._check(self.L.kiwi_hip_sph_to_grid(self.h, ctypes.byref(P), ctypes.byref(G), ctypes.byref(kernel),
                                                 int(precision), ctypes.byref(O), ctypes.byref(st)))
         return arrs[0], arrs[1], arrs[2], arrs[3], st
+
+    def voronoi_map(self, parts, home, cells, kernel=None, precision=FP64, out=None):
+        """The Voronoi map's per-cell sums.  parts: as sph_to_grid; home: int32 home cell per particle (-1:
+        unresolved), host array or device tensor like the particle columns.  cells: dict with
+        cell_face_ptr (int64, n_cells+1), face_vert_ptr (int64, n_faces+1), vert_xyz (float64, 3 per
+        vertex), face_nbr (int32, n_faces) and domain (6 floats), all host arrays or all device tensors.
+        Returns (mass, px, py, pz, MapStats) with n_cells entries each."""
+        kernel = kernel or kernel_spec()
+        P = _particles(parts)
+        hp, hdev = _ptr(home, _pi, "int32")
+        if P.n and hdev != P.on_device:
+            raise ValueError("home must live where the particle columns do")
+        cfp, d0 = _ptr(cells["cell_face_ptr"], _pll, "int64")
+        fvp, d1 = _ptr(cells["face_vert_ptr"], _pll, "int64")
+        vx, d2 = _ptr(cells["vert_xyz"])
+        fnb, d3 = _ptr(cells["face_nbr"], _pi, "int32")
+        if len({d0, d1, d2, d3}) != 1:
+            raise ValueError("tessellation arrays must all be host arrays or all device tensors")
+        nc = len(cells["cell_face_ptr"]) - 1
+        nf = len(cells["face_vert_ptr"]) - 1
+        nv = int(np.prod(cells["vert_xyz"].shape)) // 3
+        V = VoronoiCells(nc, nf, nv, cfp, fvp, vx, fnb, (_d * 6)(*[float(v) for v in cells["domain"]]), d0)
+        if out is None:
+            arrs = [np.zeros(max(nc, 0)) for _ in range(4)]
+            O = CellSums(*[a.ctypes.data_as(_pd) for a in arrs], 0)
+        else:
+            arrs = list(out)
+            O = CellSums(*[_ptr(a)[0] for a in arrs], 1)
+        st = MapStats()
+        self._check(self.L.kiwi_hip_voronoi_map(self.h, ctypes.byref(P), ctypes.cast(hp, ctypes.c_void_p),
+                                                ctypes.byref(V), ctypes.byref(kernel), int(precision),
+                                                ctypes.byref(O), ctypes.byref(st)))
+        return arrs[0], arrs[1], arrs[2], arrs[3], st
+
+
+def normalise_voronoi(mass, px, py, pz, volume, v_mean):
+    """density = mass / volume (0 where volume <= 0), v = p / mass where mass > 0, else the mean velocity
+    (src/mapping.cpp:489-499)."""
+    pos = mass > 0
+    safe = np.where(pos, mass, 1.0)
+    dens = np.where(volume > 0, mass / np.where(volume > 0, volume, 1.0), 0.0)
+    return (dens, np.where(pos, px / safe, v_mean[0]), np.where(pos, py / safe, v_mean[1]),
+            np.where(pos, pz / safe, v_mean[2]))
 
 
 def normalise(mass, px, py, pz, domain, res, v_mean):

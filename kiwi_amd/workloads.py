@@ -44,7 +44,7 @@ def gaussian_field(n, sigma, seed, slope=-3.0):
     k2[0, 0, 0] = 1.0
     amp = k2 ** (slope / 4.0)  # sqrt(P), P ~ k^slope
     amp[0, 0, 0] = 0.0
-    delta = np.fft.irfftn(fk * amp, s=(n, n, n))
+    delta = np.fft.irfftn(fk * amp, s=(n, n, n), axes=(0, 1, 2))
     delta *= sigma / delta.std()
     return delta
 

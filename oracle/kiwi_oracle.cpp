@@ -706,6 +706,23 @@ double oracle_face(const OracleKernel* k, double x0, double y0, double z0, doubl
     return ki.face(x0, y0, z0, h, verts, nverts, out_vert);
 }
 
+// oracle_face with the kernel mathematics in binary128 (the vertices as given); the analytic cubic or
+// generic path only
+double oracle_face_q(const OracleKernel* k, double x0, double y0, double z0, double h, const double* verts,
+                     int nverts) {
+    if (!k || k->kernel_id < 0 || k->kernel_id >= KIWI_TAB_NKERNELS || k->tabulated) return std::nan("");
+    const KiwiKernelTable& K = kKiwiKernelTables[k->kernel_id];
+    const double sup = k->support_in_h > 0 ? k->support_in_h : K.radkern;
+    const __float128 hh = h;
+    if (k->generic_path || k->kernel_id != 0 || sup != 2.0) {
+        const __float128 H = static_cast<__float128>(sup) * hh;
+        EvalGen<__float128> ev{&K, H, {}, {}};
+        return static_cast<double>(face_walk<EvalGen<__float128>>(x0, y0, z0, H, verts, nverts, nullptr, ev));
+    }
+    EvalM4<__float128> ev{hh, nullptr, {}, {}};
+    return static_cast<double>(face_walk<EvalM4<__float128>>(x0, y0, z0, 2 * hh, verts, nverts, nullptr, ev));
+}
+
 // The sph-to-grid particle loop (src/cartesian_grid.cpp:184-301): outputs are the UN-normalised sums
 // mass = sum Msum*m and p = sum Msum*m*v per cell, x-major idx = (ix*ny+iy)*nz+iz.  With threads > 1
 // the per-thread accumulators are reduced in thread order, as the reference does; threads == 1
@@ -891,29 +908,29 @@ int oracle_sph_to_grid_cells_q(long long n, const double* x, const double* y, co
 //   home[i]                                  home cell of particle i, or -1 (unresolved -> skipped)
 //   pid[i]                                   the particle id used for the BFS visit stamps
 // Accumulates the same un-normalised sums as oracle_sph_to_grid, per cell.
-int oracle_voronoi_map(long long n, const double* x, const double* y, const double* z, const double* vx,
-                       const double* vy, const double* vz, const double* m, const double* h, const int* pid,
-                       const int* home, long long ncells, const long long* cell_face_ptr,
-                       const long long* face_vert_ptr, const double* face_xyz, const int* face_nbr,
-                       const OracleKernel* k, double* mass, double* px, double* py, double* pz, long long* counts) {
-    if (!k || k->kernel_id < 0 || k->kernel_id >= KIWI_TAB_NKERNELS) return -1;
-    const KiwiKernelTable& K = kKiwiKernelTables[k->kernel_id];
-    Integrator ki;
-    ki.support_in_h = k->support_in_h > 0 ? k->support_in_h : K.radkern;
-    if (k->generic_path || k->kernel_id != 0 || ki.support_in_h != 2.0) ki.gen = &K;
-    else ki.tabulated = k->tabulated != 0;
+}  // extern "C"
 
+namespace {
+// The BFS of run_mapping_stage over flattened geometry; visit(i, cell, Ms) is called for every visited
+// (particle, cell) pair with the clamped double-precision cell integral, in the reference's order.
+struct VoronoiCounts {
+    long long unresolved = 0, unmapped = 0, nneg = 0, nwarn = 0, visited = 0, face_evals = 0;
+};
+template <class Visit>
+VoronoiCounts voronoi_bfs(const Integrator& ki, long long n, const double* x, const double* y, const double* z,
+                          const double* h, const int* pid, const int* home, long long ncells,
+                          const long long* cell_face_ptr, const long long* face_vert_ptr, const double* face_xyz,
+                          const int* face_nbr, Visit visit) {
     std::vector<int> stamp(static_cast<size_t>(ncells), 0), queue(static_cast<size_t>(ncells), 0);
     std::unordered_map<int64_t, std::pair<int, double>> cache;
-    for (long long c = 0; c < ncells; ++c) mass[c] = px[c] = py[c] = pz[c] = 0.0;
-    long long unresolved = 0, unmapped = 0, nneg = 0, nwarn = 0, visited = 0, face_evals = 0;
+    VoronoiCounts k;
     for (long long i = 0; i < n; ++i) {
-        const double x0 = x[i], y0 = y[i], z0 = z[i], hh = h[i], mm = m[i];
+        const double x0 = x[i], y0 = y[i], z0 = z[i], hh = h[i];
         const int ind = pid[i];
         if (hh <= 0.0) continue;
         const int hc = home[i];
         if (hc < 0) {
-            ++unresolved;
+            ++k.unresolved;
             continue;
         }
         int top = 1, bottom = 0;
@@ -940,7 +957,7 @@ int oracle_voronoi_map(long long n, const double* x, const double* y, const doub
                 const long long v0 = face_vert_ptr[f], nv = face_vert_ptr[f + 1] - v0;
                 int ov = 0;
                 const double fm = ki.face(x0, y0, z0, hh, face_xyz + 3 * v0, static_cast<int>(nv), &ov);
-                ++face_evals;
+                ++k.face_evals;
                 Ms += fm;
                 if (cacheable) cache.emplace(key, std::make_pair(cell, fm));
                 if (ov != nv && nb >= 0 && stamp[static_cast<size_t>(nb)] != ind) {
@@ -949,19 +966,50 @@ int oracle_voronoi_map(long long n, const double* x, const double* y, const doub
                 }
             }
             if (Ms < 0.0) {
-                ++nneg;
-                if (Ms <= -0.01) ++nwarn;
+                ++k.nneg;
+                if (Ms <= -0.01) ++k.nwarn;
                 Ms = 0.0;
             }
-            mass[cell] += Ms * mm;
-            px[cell] += Ms * mm * vx[i];
-            py[cell] += Ms * mm * vy[i];
-            pz[cell] += Ms * mm * vz[i];
+            visit(i, cell, Ms);
             ++bottom;
         }
-        visited += top;
-        if (top == 1) ++unmapped;
+        k.visited += top;
+        if (top == 1) ++k.unmapped;
     }
+    return k;
+}
+
+bool make_integrator(const OracleKernel* k, Integrator* ki) {
+    if (!k || k->kernel_id < 0 || k->kernel_id >= KIWI_TAB_NKERNELS) return false;
+    const KiwiKernelTable& K = kKiwiKernelTables[k->kernel_id];
+    ki->support_in_h = k->support_in_h > 0 ? k->support_in_h : K.radkern;
+    if (k->generic_path || k->kernel_id != 0 || ki->support_in_h != 2.0) ki->gen = &K;
+    else ki->tabulated = k->tabulated != 0;
+    return true;
+}
+}  // namespace
+
+extern "C" {
+
+int oracle_voronoi_map(long long n, const double* x, const double* y, const double* z, const double* vx,
+                       const double* vy, const double* vz, const double* m, const double* h, const int* pid,
+                       const int* home, long long ncells, const long long* cell_face_ptr,
+                       const long long* face_vert_ptr, const double* face_xyz, const int* face_nbr,
+                       const OracleKernel* k, double* mass, double* px, double* py, double* pz, long long* counts) {
+    Integrator ki;
+    if (!make_integrator(k, &ki)) return -1;
+    for (long long c = 0; c < ncells; ++c) mass[c] = px[c] = py[c] = pz[c] = 0.0;
+    const VoronoiCounts vc =
+        voronoi_bfs(ki, n, x, y, z, h, pid, home, ncells, cell_face_ptr, face_vert_ptr, face_xyz, face_nbr,
+                    [&](long long i, int cell, double Ms) {
+                        const double mm = m[i];
+                        mass[cell] += Ms * mm;
+                        px[cell] += Ms * mm * vx[i];
+                        py[cell] += Ms * mm * vy[i];
+                        pz[cell] += Ms * mm * vz[i];
+                    });
+    const long long unresolved = vc.unresolved, unmapped = vc.unmapped, nneg = vc.nneg, nwarn = vc.nwarn,
+                    visited = vc.visited, face_evals = vc.face_evals;
     if (counts) {
         counts[0] = unresolved;
         counts[1] = unmapped;
@@ -970,6 +1018,63 @@ int oracle_voronoi_map(long long n, const double* x, const double* y, const doub
         counts[4] = visited;
         counts[5] = face_evals;
     }
+    return 0;
+}
+
+// The Voronoi map's sums for a list of cells with the face integrals in binary128: the double-precision
+// BFS above decides which (particle, cell) pairs are visited; each visited pair's cell integral is then
+// redone in binary128 over the cell's faces (vertices as given), clamped at 0 and accumulated in
+// binary128.  Used to arbitrate cells where two double-precision evaluations disagree.
+int oracle_voronoi_map_cells_q(long long n, const double* x, const double* y, const double* z, const double* vx,
+                               const double* vy, const double* vz, const double* m, const double* h, const int* pid,
+                               const int* home, long long ncells, const long long* cell_face_ptr,
+                               const long long* face_vert_ptr, const double* face_xyz, const int* face_nbr,
+                               const OracleKernel* k, long long nlist, const long long* cells, double* out4) {
+    Integrator ki;
+    if (!make_integrator(k, &ki) || ki.tabulated) return -1;
+    std::vector<long long> slot(static_cast<size_t>(ncells), -1);
+    for (long long q = 0; q < nlist; ++q) {
+        if (cells[q] < 0 || cells[q] >= ncells) return -1;
+        slot[static_cast<size_t>(cells[q])] = q;
+    }
+    std::vector<std::vector<long long>> visitors(static_cast<size_t>(nlist));  // particles per listed cell
+    voronoi_bfs(ki, n, x, y, z, h, pid, home, ncells, cell_face_ptr, face_vert_ptr, face_xyz, face_nbr,
+                [&](long long i, int cell, double) {
+                    const long long sl = slot[static_cast<size_t>(cell)];
+                    if (sl >= 0) visitors[static_cast<size_t>(sl)].push_back(i);
+                });
+    std::vector<__float128> acc(static_cast<size_t>(4 * nlist), 0);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (long long sl = 0; sl < nlist; ++sl) {
+        const long long c = cells[sl];
+        __float128* a = &acc[static_cast<size_t>(4 * sl)];
+        for (const long long i : visitors[static_cast<size_t>(sl)]) {  // particle order, as the BFS
+            const __float128 hh = h[i];
+            __float128 Ms = 0;
+            for (long long f = cell_face_ptr[c]; f < cell_face_ptr[c + 1]; ++f) {
+                const long long v0 = face_vert_ptr[f];
+                const int nv = static_cast<int>(face_vert_ptr[f + 1] - v0);
+                if (ki.gen) {
+                    const __float128 H = static_cast<__float128>(ki.support_in_h) * hh;
+                    EvalGen<__float128> ev{ki.gen, H, {}, {}};
+                    Ms += face_walk<EvalGen<__float128>>(x[i], y[i], z[i], H, face_xyz + 3 * v0, nv, nullptr, ev);
+                } else {
+                    EvalM4<__float128> ev{hh, nullptr, {}, {}};
+                    Ms += face_walk<EvalM4<__float128>>(x[i], y[i], z[i], 2 * hh, face_xyz + 3 * v0, nv, nullptr,
+                                                        ev);
+                }
+            }
+            if (Ms < 0) Ms = 0;
+            const __float128 t = Ms * m[i];
+            a[0] += t;
+            a[1] += t * vx[i];
+            a[2] += t * vy[i];
+            a[3] += t * vz[i];
+        }
+    }
+    for (long long q = 0; q < 4 * nlist; ++q) out4[q] = static_cast<double>(acc[static_cast<size_t>(q)]);
     return 0;
 }
 

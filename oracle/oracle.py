@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "libkiwi_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libkiwi_ref.so")
 REF_CLI = os.path.join(HERE, "_ref", "kiwi")
+VORO_SO = os.path.join(HERE, "_ref", "libkiwi_voro.so")
 
 KERNELS = ["M4", "M5", "M6", "WC2", "WC4", "WC6"]
 
@@ -57,6 +58,8 @@ def lib():
         L.oracle_generic_vertex.argtypes = [_i, _d, _d, _d, _d]
         L.oracle_face.restype = _d
         L.oracle_face.argtypes = [ctypes.POINTER(OracleKernel), _d, _d, _d, _d, _pd, _i, _pi]
+        L.oracle_face_q.restype = _d
+        L.oracle_face_q.argtypes = [ctypes.POINTER(OracleKernel), _d, _d, _d, _d, _pd, _i]
         L.oracle_sph_to_grid.restype = _i
         L.oracle_sph_to_grid.argtypes = [_ll] + [_pd] * 8 + [_pd, _i, _i, _i, ctypes.POINTER(OracleKernel), _i,
                                                               _pd, _pd, _pd, _pd, ctypes.POINTER(OracleStats)]
@@ -66,6 +69,9 @@ def lib():
         L.oracle_voronoi_map.restype = _i
         L.oracle_voronoi_map.argtypes = [_ll] + [_pd] * 8 + [_pi, _pi, _ll, _pll, _pll, _pd, _pi,
                                                               ctypes.POINTER(OracleKernel), _pd, _pd, _pd, _pd, _pll]
+        L.oracle_voronoi_map_cells_q.restype = _i
+        L.oracle_voronoi_map_cells_q.argtypes = [_ll] + [_pd] * 8 + [_pi, _pi, _ll, _pll, _pll, _pd, _pi,
+                                                                      ctypes.POINTER(OracleKernel), _ll, _pll, _pd]
         _lib = L
     return _lib
 
@@ -101,8 +107,40 @@ def _p(a):
     return a.ctypes.data_as(_pd)
 
 
+def tessellate(ids, pts, domain, particles):
+    """Voronoi geometry of the points (reference's vendored voro++, oracle/_ref/libkiwi_voro.so) as CSR
+    arrays, plus each particle's home cell.  pts: (n, 3); particles: (m, 3).  Returns a dict."""
+    V = ctypes.CDLL(VORO_SO)
+    V.voro_build.restype = ctypes.c_void_p
+    V.voro_build.argtypes = [_i, _pi, _pd, _pd, _pd, _pd, _ll, _pd, _pd, _pd, _pll]
+    V.voro_fetch.argtypes = [ctypes.c_void_p, _pi, _pd, _pd, _pll, _pll, _pd, _pi, _pi]
+    V.voro_free.argtypes = [ctypes.c_void_p]
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    P = [np.ascontiguousarray(pts[:, k], dtype=np.float64) for k in range(3)]
+    Q = [np.ascontiguousarray(particles[:, k], dtype=np.float64) for k in range(3)]
+    dom = np.ascontiguousarray(domain, dtype=np.float64)
+    sizes = np.zeros(3, dtype=np.int64)
+    h = V.voro_build(len(ids), ids.ctypes.data_as(_pi), *[_p(a) for a in P], _p(dom), len(Q[0]), *[_p(a) for a in Q],
+                     sizes.ctypes.data_as(_pll))
+    nc, nf, nv = (int(v) for v in sizes)
+    g = {"cell_id": np.zeros(nc, np.int32), "site": np.zeros((nc, 3)), "volume": np.zeros(nc),
+         "cell_face_ptr": np.zeros(nc + 1, np.int64), "face_vert_ptr": np.zeros(nf + 1, np.int64),
+         "vert_xyz": np.zeros((nv, 3)), "face_nbr": np.zeros(nf, np.int32), "home": np.zeros(len(Q[0]), np.int32)}
+    V.voro_fetch(h, g["cell_id"].ctypes.data_as(_pi), _p(g["site"]), _p(g["volume"]),
+                 g["cell_face_ptr"].ctypes.data_as(_pll), g["face_vert_ptr"].ctypes.data_as(_pll), _p(g["vert_xyz"]),
+                 g["face_nbr"].ctypes.data_as(_pi), g["home"].ctypes.data_as(_pi))
+    V.voro_free(h)
+    return g
+
+
 def kernel(name="M4", support_in_h=0.0, generic=False, tabulated=False):
     return OracleKernel(KERNELS.index(name), support_in_h, int(generic), int(tabulated))
+
+
+def face_quad(k, p, h, verts):
+    """oracle face integral with the kernel mathematics in binary128 (analytic cubic or generic path)."""
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1)
+    return lib().oracle_face_q(ctypes.byref(k), p[0], p[1], p[2], h, _p(v), len(v) // 3)
 
 
 def face(k, p, h, verts):
@@ -163,3 +201,26 @@ def voronoi_map(parts, pid, home, cell_face_ptr, face_vert_ptr, face_xyz, face_n
     if rc != 0:
         raise ValueError(f"oracle_voronoi_map failed ({rc})")
     return out[0], out[1], out[2], out[3], counts
+
+
+def voronoi_map_cells_quad(parts, pid, home, cell_face_ptr, face_vert_ptr, face_xyz, face_nbr, cells, k=None):
+    """The Voronoi map's (mass, px, py, pz) for the listed cells with the face integrals in binary128 (the
+    visits decided by the double-precision BFS).  Returns an array of shape (len(cells), 4)."""
+    k = k or kernel()
+    n = len(parts["x"])
+    cols = [np.ascontiguousarray(parts[c], dtype=np.float64) for c in ("x", "y", "z", "vx", "vy", "vz", "m", "h")]
+    pid = np.ascontiguousarray(pid, dtype=np.int32)
+    home = np.ascontiguousarray(home, dtype=np.int32)
+    cfp = np.ascontiguousarray(cell_face_ptr, dtype=np.int64)
+    fvp = np.ascontiguousarray(face_vert_ptr, dtype=np.int64)
+    fxyz = np.ascontiguousarray(face_xyz, dtype=np.float64)
+    fnb = np.ascontiguousarray(face_nbr, dtype=np.int32)
+    cells = np.ascontiguousarray(cells, dtype=np.int64)
+    out = np.zeros((len(cells), 4))
+    rc = lib().oracle_voronoi_map_cells_q(n, *[_p(c) for c in cols], pid.ctypes.data_as(_pi),
+                                          home.ctypes.data_as(_pi), len(cfp) - 1, cfp.ctypes.data_as(_pll),
+                                          fvp.ctypes.data_as(_pll), _p(fxyz), fnb.ctypes.data_as(_pi),
+                                          ctypes.byref(k), len(cells), cells.ctypes.data_as(_pll), _p(out))
+    if rc != 0:
+        raise ValueError("oracle_voronoi_map_cells_q failed")
+    return out

@@ -4,6 +4,7 @@
 #
 #   make -f oracle/ref.mk            -> oracle/_ref/kiwi (the reference CLI)
 #                                       oracle/_ref/libkiwi_ref.so (reference kernel math + our C shim)
+#                                       oracle/_ref/libkiwi_voro.so (the vendored voro++ + our C shim)
 #
 # The reference build stays on the build machine: no GPU test, smoke test or benchmark loads it.
 
@@ -24,7 +25,7 @@ VORO_OBJ := $(patsubst %.cc,$(OUT)/obj/voro_%.o,$(VORO_SRC))
 KIWI_OBJ := $(patsubst $(REF)/src/%.cpp,$(OUT)/obj/kiwi_%.o,$(KIWI_SRC))
 MATH_OBJ := $(patsubst $(REF)/src/%.cpp,$(OUT)/obj/kiwi_%.o,$(MATH_SRC))
 
-all: $(OUT)/kiwi $(OUT)/libkiwi_ref.so
+all: $(OUT)/kiwi $(OUT)/libkiwi_ref.so $(OUT)/libkiwi_voro.so
 
 $(OUT)/obj/voro_%.o: $(REF)/third_party/voro/src/%.cc
 	@mkdir -p $(OUT)/obj
@@ -38,6 +39,9 @@ $(OUT)/kiwi: $(KIWI_OBJ) $(VORO_OBJ)
 	$(CXX) $(CXXFLAGS) $^ -o $@
 
 $(OUT)/libkiwi_ref.so: oracle/ref_shim.cpp $(MATH_OBJ)
+	$(CXX) $(CXXFLAGS) $(INC) -shared $^ -o $@
+
+$(OUT)/libkiwi_voro.so: oracle/voro_shim.cpp $(VORO_OBJ)
 	$(CXX) $(CXXFLAGS) $(INC) -shared $^ -o $@
 
 clean:

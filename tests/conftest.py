@@ -14,6 +14,15 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def hip_ctx():
+    # PyTorch-ROCm bundles its own libamdhip64.so.7; the process shares whichever copy loads first, and
+    # torch's device initialisation needs its own.  Tests that stage device tensors with torch therefore
+    # bring torch's GPU up before the library's context.
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except ImportError:
+        pass
     from kiwi_amd import hip
     ctx = hip.Context(0)
     yield ctx

@@ -14,7 +14,12 @@ fixtures it writes are plain numbers (inputs and the reference's outputs), commi
                         onto 20^3 for each kernel on the generic path (M4 forced generic, M4 with
                         support 2.5h, M5, M6, WC2, WC4, WC6) and for the tabulated cubic path
 
-  python tests/golden/make_golden.py [--only kernel_grids]
+  voronoi_map.npz       `kiwi map` (reference CLI, OMP_NUM_THREADS=1, no buffer, no Lloyd step: the sites
+                        are the particles) on every 8th C1 particle, M4: the per-cell volume, density and
+                        velocity of mapping.dat, with the tessellation of the same sites by the reference's
+                        vendored voro++ as CSR arrays and each particle's home cell
+
+  python tests/golden/make_golden.py [--only kernel_grids | --only voronoi]
 """
 import ctypes
 import json
@@ -90,12 +95,47 @@ def kernel_grids():
     np.savez_compressed(os.path.join(OUT, "kernel_grids.npz"), **out)
 
 
+def run_ref_map(parts, domain, threads=1):
+    with tempfile.TemporaryDirectory() as td:
+        snap = os.path.join(td, "snap.txt")
+        workloads.write_plain_ascii(parts, snap)
+        cfg = {"run_name": "golden", "input_snapshot": snap, "input_snapshot_format": "plain_ascii",
+               "work_folder": os.path.join(td, "work"),
+               "domain": dict(zip(["x_min", "x_max", "y_min", "y_max", "z_min", "z_max"], map(float, domain))),
+               "buffer": {"enabled": False}, "lloyd": {"iterations": 0}}
+        cp = os.path.join(td, "cfg.json")
+        with open(cp, "w") as f:
+            json.dump(cfg, f)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        r = subprocess.run([O.REF_CLI, "map", cp], env=env, capture_output=True, text=True, check=True)
+        rows = np.loadtxt(os.path.join(td, "work", "snap", "mapping.dat"))
+        return rows, r.stderr
+
+
+def voronoi():
+    """The reference's Voronoi map on a small site set, with the geometry the GPU path consumes."""
+    c1 = np.load(os.path.join(OUT, "c1_snapshot.npz"))
+    parts = {k: np.ascontiguousarray(c1[k][::8]) for k in ("x", "y", "z", "vx", "vy", "vz", "m", "h", "rho")}
+    domain = np.array(c1["domain"])
+    rows, log = run_ref_map(parts, domain)
+    n = len(parts["x"])
+    pts = np.stack([parts["x"], parts["y"], parts["z"]], 1)
+    ids = np.arange(1, n + 1)  # plain_ascii ids are the 1-based row numbers (src/snapshot.cpp:67-68)
+    g = O.tessellate(ids, pts, domain, pts)
+    by_id = {int(r[0]): r for r in rows}
+    ref = np.array([by_id[int(i)] for i in g["cell_id"]])
+    np.savez_compressed(os.path.join(OUT, "voronoi_map.npz"), domain=domain, pid=ids.astype(np.int32),
+                        **{f"p_{k}": v for k, v in parts.items()}, **g,
+                        ref_volume=ref[:, 4], ref_density=ref[:, 5], ref_vx=ref[:, 6], ref_vy=ref[:, 7],
+                        ref_vz=ref[:, 8], ref_site=ref[:, 1:4])
+
+
 def main():
     if not O.have_ref() or not os.path.exists(O.REF_CLI):
         sys.exit("oracle/_ref is not built (make -f oracle/ref.mk)")
     if "--only" in sys.argv:
         what = sys.argv[sys.argv.index("--only") + 1]
-        {"kernel_grids": kernel_grids}[what]()
+        {"kernel_grids": kernel_grids, "voronoi": voronoi}[what]()
         return
     R = O.ref()
     rng = np.random.default_rng(20251203)
@@ -183,6 +223,7 @@ def main():
     val = np.array([R.ref_m4_gridded(*t) for t in zip(h, phi, r0, R0)])
     np.savez_compressed(os.path.join(OUT, "m4_gridded.npz"), h=h, phi=phi, r0=r0, R0=R0, value=val)
     kernel_grids()
+    voronoi()
     print("golden fixtures written to", OUT)
 
 

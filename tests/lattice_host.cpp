@@ -137,3 +137,27 @@ double lat_m4_gridded(double h, double phi, double r0, double R0) {
     return m4_gridded_value(host_table().data(), r0 / h, R0 / h, std::cos(phi));
 }
 }  // extern "C"
+
+// ---- the polygon face walk of the Voronoi map (face_walk.h) ----
+#include "../kiwi_amd/csrc/face_walk.h"
+
+extern "C" {
+// face integral and reachability of polygon v[0..n) for a particle at p: kind 0 = cubic analytic,
+// 1 = generic kernel kid (support sup * h), 2 = cubic tabulated
+double lat_face_walk(int kind, int kid, double sup, const double* p, double h, const double* v, int n, int* reach) {
+    bool r = false;
+    double m = 0.0;
+    if (kind == 0) {
+        EvM4 ev{1.0 / h};
+        m = face_walk_integral(p[0], p[1], p[2], 2.0 * h, v, n, ev, &r);
+    } else if (kind == 1) {
+        EvGen<KiwiKernelTable> ev{&kTabs[kid], 1.0 / (sup * h)};
+        m = face_walk_integral(p[0], p[1], p[2], sup * h, v, n, ev, &r);
+    } else {
+        EvTab ev{host_table().data(), 1.0 / h};
+        m = face_walk_integral(p[0], p[1], p[2], 2.0 * h, v, n, ev, &r);
+    }
+    *reach = r ? 1 : 0;
+    return m;
+}
+}  // extern "C"

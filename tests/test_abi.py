@@ -51,7 +51,9 @@ def test_struct_layouts_match_header(tmp_path):
               "kiwi_grid": [f for f, _ in hip.Grid._fields_],
               "kiwi_kernel": [f for f, _ in hip.Kernel._fields_],
               "kiwi_cell_sums": [f for f, _ in hip.CellSums._fields_],
-              "kiwi_stats": [f for f, _ in hip.Stats._fields_]}
+              "kiwi_stats": [f for f, _ in hip.Stats._fields_],
+              "kiwi_voronoi_cells": [f for f, _ in hip.VoronoiCells._fields_],
+              "kiwi_map_stats": [f for f, _ in hip.MapStats._fields_]}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for st, fs in fields.items():
         lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
@@ -64,7 +66,8 @@ def test_struct_layouts_match_header(tmp_path):
     out = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
                                                           check=True).stdout.split("\n") if l)
     for st, cls in (("kiwi_particles", hip.Particles), ("kiwi_grid", hip.Grid), ("kiwi_kernel", hip.Kernel),
-                    ("kiwi_cell_sums", hip.CellSums), ("kiwi_stats", hip.Stats)):
+                    ("kiwi_cell_sums", hip.CellSums), ("kiwi_stats", hip.Stats),
+                    ("kiwi_voronoi_cells", hip.VoronoiCells), ("kiwi_map_stats", hip.MapStats)):
         assert int(out[st]) == ctypes.sizeof(cls), st
         for f, _ in cls._fields_:
             assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, (st, f)

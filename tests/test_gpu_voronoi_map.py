@@ -1,0 +1,256 @@
+"""GPU parity of kiwi_hip_voronoi_map (the HIP Voronoi map) against the reference CLI's mapping.dat
+(tests/golden/voronoi_map.npz) and against the CPU oracle's BFS (oracle/kiwi_oracle.cpp,
+oracle_voronoi_map: src/mapping.cpp:87-206, 400-460 restated, pinned in test_oracle.py).
+
+Tolerances (FP64 throughout):
+  vs the CLI: density |gpu - ref| <= 1e-10 * max|ref|; velocities mass-weighted as in the Cartesian
+      tests, 1e-10 of the field maximum.  (The fixture's geometry and the CLI's agree to rounding
+      only -- voro++ cut order -- and mapping.dat holds 15 significant digits.)
+  vs the oracle on the same geometry: per-cell mass and momentum within 1e-12 of the field maximum;
+      cells outside that are recomputed with the face integrals in binary128, and the GPU must lie
+      within 1e-11 of that (tests/parity.py: the reference's I-series error in the analytic cubic
+      vertex integral, and the rounding gap between the two voro++ copies of a shared face, which the
+      reference's face cache and the GPU's lower-index owner rule each take from one side).  The
+      tabulated path has no binary128 form: 1e-10, the two lookup tables coming from different FP64
+      vertex integrals.
+  Counts: unresolved, unmapped, visited pairs and face integrals are equal.  Negative clamps
+      (src/mapping.cpp:198-203) are pairs whose cell integral is within rounding of 0, so their number
+      depends on the rounding of each side; only the warning-level ones (<= -0.01) are compared.
+Random geometry comes from the reference's vendored voro++ through oracle/_ref/libkiwi_voro.so.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+sys.path.insert(0, ROOT)
+
+from kiwi_amd import hip  # noqa: E402
+from parity import assert_voronoi_cells_match  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+COLS = ("x", "y", "z", "vx", "vy", "vz", "m", "h")
+GEOM = ("cell_face_ptr", "face_vert_ptr", "vert_xyz", "face_nbr")
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def oracle():
+    from oracle import oracle as O
+    if not os.path.exists(O.VORO_SO):
+        pytest.skip(f"voro++ shim not built at {O.VORO_SO} (python -c 'import __graft_entry__ as g; g.build()')")
+    return O
+
+
+def golden():
+    g = np.load(os.path.join(GOLD, "voronoi_map.npz"))
+    parts = {k: np.ascontiguousarray(g["p_" + k]) for k in COLS}
+    cells = {k: np.ascontiguousarray(g[k]) for k in GEOM}
+    cells["vert_xyz"] = np.ascontiguousarray(cells["vert_xyz"].reshape(-1))
+    cells["domain"] = g["domain"]
+    return g, parts, cells
+
+
+def make_case(seed, n_sites, n_parts, h_scale, domain=(-1.0, 1.0, -1.0, 1.0, -1.0, 1.0), outside=0, h_zero=0,
+              clustered=False):
+    O = oracle()
+    rng = np.random.default_rng(seed)
+    dom = np.array(domain, dtype=np.float64)
+    lo, hi = dom[0::2], dom[1::2]
+    if clustered:
+        centres = rng.uniform(lo + 0.2 * (hi - lo), hi - 0.2 * (hi - lo), size=(4, 3))
+        pick = rng.integers(4, size=n_sites)
+        sites = centres[pick] + rng.normal(scale=0.12, size=(n_sites, 3)) * (hi - lo)
+        sites = np.clip(sites, lo + 1e-6, hi - 1e-6)
+    else:
+        sites = rng.uniform(lo, hi, size=(n_sites, 3))
+    pts = rng.uniform(lo, hi, size=(n_parts, 3))
+    if outside:
+        pts[:outside] = hi + rng.uniform(0.01, 0.3, size=(outside, 3))
+    spacing = np.prod(hi - lo) ** (1 / 3) / n_sites ** (1 / 3)
+    parts = {"x": pts[:, 0].copy(), "y": pts[:, 1].copy(), "z": pts[:, 2].copy(),
+             "vx": rng.normal(size=n_parts), "vy": rng.normal(size=n_parts), "vz": rng.normal(size=n_parts),
+             "m": rng.uniform(0.5, 1.5, n_parts),
+             "h": spacing * h_scale * rng.uniform(0.5, 2.0, n_parts)}
+    if h_zero:
+        parts["h"][outside:outside + h_zero] = 0.0
+    parts = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in parts.items()}
+    geo = O.tessellate(np.arange(1, n_sites + 1), sites, dom, pts)
+    cells = {k: np.ascontiguousarray(geo[k]) for k in ("cell_face_ptr", "face_vert_ptr", "face_nbr")}
+    cells["vert_xyz"] = np.ascontiguousarray(geo["vert_xyz"].reshape(-1))
+    cells["domain"] = dom
+    return parts, geo["home"].astype(np.int32), cells, geo
+
+
+def pids(parts):
+    return np.arange(1, len(parts["x"]) + 1, dtype=np.int32)  # non-zero ids (stamp 0 means "unvisited")
+
+
+def run_oracle(parts, home, cells, k=None):
+    O = oracle()
+    return O.voronoi_map(parts, pids(parts), home, cells["cell_face_ptr"], cells["face_vert_ptr"],
+                         cells["vert_xyz"], cells["face_nbr"], k=k)
+
+
+def assert_matches_oracle(got, ref, parts, home, cells, k=None, pid=None):
+    pid = pids(parts) if pid is None else pid
+    n = assert_voronoi_cells_match(got, ref, parts, pid, home, cells, k=k)
+    say(f"  cells arbitrated in binary128: {n}")
+
+
+def assert_close(got, ref, tol):
+    for a, b, name in zip(got, ref, ("mass", "px", "py", "pz")):
+        scale = max(np.max(np.abs(b)), 1e-300)
+        err = np.max(np.abs(a - b)) / scale
+        assert err <= tol, (name, err)
+
+
+def test_golden_matches_reference_cli(hip_ctx):
+    say("case: voronoi_map.npz (250 sites = particles, M4) vs reference CLI mapping.dat")
+    g, parts, cells = golden()
+    mass, px, py, pz, st = hip_ctx.voronoi_map(parts, g["home"].astype(np.int32), cells)
+    dens, vx, vy, vz = hip.normalise_voronoi(mass, px, py, pz, g["volume"], st.v_mean)
+    ref = g["ref_density"]
+    assert np.max(np.abs(dens - ref)) <= 1e-10 * np.abs(ref).max()
+    w = ref / np.abs(ref).max()
+    for v, f in ((vx, "ref_vx"), (vy, "ref_vy"), (vz, "ref_vz")):
+        assert np.max(np.abs(v - g[f]) * w) <= 1e-10 * np.abs(g[f]).max(), f
+    assert st.unresolved == 0 and st.n_negative_warn == 0
+    np.testing.assert_allclose(st.v_mean, [parts[c].mean() for c in ("vx", "vy", "vz")], rtol=1e-12, atol=1e-15)
+
+
+def test_golden_matches_oracle_and_counts(hip_ctx):
+    say("case: voronoi_map.npz vs oracle BFS, un-normalised sums and counts")
+    g, parts, cells = golden()
+    home = g["home"].astype(np.int32)
+    got = hip_ctx.voronoi_map(parts, home, cells)
+    O = oracle()
+    ref = O.voronoi_map(parts, g["pid"], home, cells["cell_face_ptr"], cells["face_vert_ptr"], cells["vert_xyz"],
+                        cells["face_nbr"])
+    assert_matches_oracle(got[:4], ref[:4], parts, home, cells, pid=g["pid"])
+    st, counts = got[4], ref[4]
+    assert (st.unresolved, st.unmapped, st.pairs_visited, st.face_evals) == tuple(int(v) for v in
+                                                                                  (counts[0], counts[1], counts[4],
+                                                                                   counts[5]))
+    assert st.pairs_candidate >= st.pairs_visited
+
+
+@pytest.mark.parametrize("seed,n_sites,n_parts,h_scale,clustered", [(1, 400, 300, 1.0, False),
+                                                                     (2, 900, 700, 2.0, False),
+                                                                     (3, 1500, 1200, 1.3, True),
+                                                                     (4, 300, 500, 0.25, False)])
+def test_random_tessellations_match_oracle(hip_ctx, seed, n_sites, n_parts, h_scale, clustered):
+    say(f"case: random sites={n_sites} particles={n_parts} h_scale={h_scale} clustered={clustered} seed={seed}")
+    parts, home, cells, _ = make_case(seed, n_sites, n_parts, h_scale, outside=7, h_zero=5, clustered=clustered)
+    got = hip_ctx.voronoi_map(parts, home, cells)
+    ref = run_oracle(parts, home, cells)
+    assert_matches_oracle(got[:4], ref[:4], parts, home, cells)
+    st, counts = got[4], ref[4]
+    say(f"  candidates {st.pairs_candidate} visited {st.pairs_visited} faces {st.face_evals} "
+        f"unmapped {st.unmapped} negatives {st.n_negative}/{counts[2]}")
+    assert st.unresolved == counts[0] == 7
+    assert st.unmapped == counts[1]
+    assert st.pairs_visited == counts[4]
+    assert st.face_evals == counts[5]
+    assert st.n_negative_warn == counts[3] == 0
+
+
+@pytest.mark.parametrize("name,generic,tabulated", [("M4", False, True), ("WC2", True, False),
+                                                    ("M6", True, False), ("WC6", True, False)])
+def test_kernels_match_oracle(hip_ctx, name, generic, tabulated):
+    say(f"case: random sites=600 particles=400 kernel={name} generic={generic} tabulated={tabulated}")
+    O = oracle()
+    parts, home, cells, _ = make_case(11, 600, 400, 1.2)
+    got = hip_ctx.voronoi_map(parts, home, cells, kernel=hip.kernel_spec(name, generic_path=generic,
+                                                                         tabulated=tabulated))
+    k = O.kernel(name, generic=generic, tabulated=tabulated)
+    ref = run_oracle(parts, home, cells, k=k)
+    if tabulated:
+        # the two lookup tables come from different FP64 vertex integrals (the reference's I-series in
+        # the oracle, the closed form here), entries differing by up to ~2.5e-12
+        assert_close(got[:4], ref[:4], 1e-10)
+    else:
+        assert_matches_oracle(got[:4], ref[:4], parts, home, cells, k=k)
+    assert got[4].pairs_visited == ref[4][4] and got[4].face_evals == ref[4][5]
+
+
+def test_mass_conservation_inside_the_domain(hip_ctx):
+    """Particles whose support lies inside the domain put exactly their mass into the cells (the
+    cells tile the domain and every cell the support meets is visited)."""
+    say("case: conservation, 2000 sites, 800 particles of which ~190 have their support inside")
+    parts, home, cells, geo = make_case(21, 2000, 800, 1.0)
+    sup = 2.0 * parts["h"]
+    p = np.stack([parts[c] for c in ("x", "y", "z")], 1)
+    inside = np.all((p - sup[:, None] > -1.0) & (p + sup[:, None] < 1.0), axis=1)
+    keep = {k: np.ascontiguousarray(v[inside]) for k, v in parts.items()}
+    mass, px, py, pz, st = hip_ctx.voronoi_map(keep, np.ascontiguousarray(home[inside]), cells)
+    assert inside.sum() > 150
+    assert abs(mass.sum() - keep["m"].sum()) <= 1e-12 * keep["m"].sum()
+    assert abs(px.sum() - (keep["m"] * keep["vx"]).sum()) <= 1e-12 * np.abs(keep["m"] * keep["vx"]).sum()
+    assert st.n_negative_warn == 0
+
+
+def test_device_inputs_and_determinism(hip_ctx):
+    import torch
+    say("case: device-resident inputs vs host inputs; two runs bit-identical")
+    parts, home, cells, _ = make_case(5, 700, 500, 1.4)
+    a = hip_ctx.voronoi_map(parts, home, cells)
+    b = hip_ctx.voronoi_map(parts, home, cells)
+    for x, y in zip(a[:4], b[:4]):
+        assert np.array_equal(x, y)
+    dev = torch.device("cuda:0")
+    dparts = {k: torch.from_numpy(v).to(dev) for k, v in parts.items()}
+    dcells = {k: torch.from_numpy(v).to(dev) for k, v in cells.items() if k != "domain"}
+    dcells["domain"] = cells["domain"]
+    nc = len(cells["cell_face_ptr"]) - 1
+    out = tuple(torch.zeros(nc, dtype=torch.float64, device=dev) for _ in range(4))
+    r = hip_ctx.voronoi_map(dparts, torch.from_numpy(home).to(dev), dcells, out=out)
+    torch.cuda.synchronize()
+    for x, y in zip(a[:4], r[:4]):
+        assert np.array_equal(x, y.cpu().numpy())
+
+
+def test_empty_inputs(hip_ctx):
+    say("case: no particles; no cells")
+    parts, home, cells, _ = make_case(6, 50, 10, 1.0)
+    empty = {k: np.zeros(0) for k in COLS}
+    mass, px, py, pz, st = hip_ctx.voronoi_map(empty, np.zeros(0, np.int32), cells)
+    assert np.all(mass == 0) and st.pairs_visited == 0 and st.v_mean[0] == 0.0
+    none = {"cell_face_ptr": np.zeros(1, np.int64), "face_vert_ptr": np.zeros(1, np.int64),
+            "vert_xyz": np.zeros(0), "face_nbr": np.zeros(0, np.int32), "domain": cells["domain"]}
+    unresolved = np.full(len(parts["x"]), -1, np.int32)
+    mass, px, py, pz, st = hip_ctx.voronoi_map(parts, unresolved, none)
+    assert len(mass) == 0 and st.unresolved == len(parts["x"]) - int(np.sum(parts["h"] <= 0))
+
+
+def test_bad_inputs_raise(hip_ctx):
+    say("case: malformed tessellation, home out of range, non-finite h")
+    parts, home, cells, _ = make_case(7, 60, 20, 1.0)
+    bad = dict(cells)
+    bad["face_nbr"] = cells["face_nbr"].copy()
+    bad["face_nbr"][3] = len(cells["cell_face_ptr"]) + 5
+    with pytest.raises(hip.KiwiError):
+        hip_ctx.voronoi_map(parts, home, bad)
+    bad = dict(cells)
+    bad["face_vert_ptr"] = cells["face_vert_ptr"].copy()
+    bad["face_vert_ptr"][-1] += 10
+    with pytest.raises(hip.KiwiError):
+        hip_ctx.voronoi_map(parts, home, bad)
+    h2 = home.copy()
+    h2[0] = len(cells["cell_face_ptr"]) + 1
+    with pytest.raises(hip.KiwiError):
+        hip_ctx.voronoi_map(parts, h2, cells)
+    p2 = dict(parts)
+    p2["h"] = parts["h"].copy()
+    p2["h"][1] = np.nan
+    with pytest.raises(hip.KiwiError):
+        hip_ctx.voronoi_map(p2, home, cells)
+    # the context still works afterwards
+    ok = hip_ctx.voronoi_map(parts, home, cells)
+    assert np.isfinite(ok[0]).all()

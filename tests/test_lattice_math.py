@@ -214,3 +214,52 @@ def test_vertex_value_and_gridded_lookup_match_reference_golden(lat):
     g = np.load(os.path.join(HERE, "golden", "m4_gridded.npz"))
     v = np.array([lat.lat_m4_gridded(*t) for t in zip(g["h"], g["phi"], g["r0"], g["R0"])])
     assert np.max(np.abs(v - g["value"])) < 1e-13
+
+
+# ---------------------------------------------------------------------- Voronoi face walk (face_walk.h)
+@pytest.mark.parametrize("kind,name,generic,tabulated,qtol", [(0, "M4", False, False, 5e-14),
+                                                              (1, "M6", True, False, 5e-14),
+                                                              (1, "WC2", True, False, 5e-14),
+                                                              (1, "WC6", True, False, 3e-13),
+                                                              (2, "M4", False, True, None)])
+def test_polygon_face_walk_matches_oracle_face_walk(lat, kind, name, generic, tabulated, qtol):
+    """face_walk.h (the Voronoi map's per-face integral and reachability test, run by the HIP kernels)
+    against the oracle's face walk (src/kernel_integral.cpp:471-647, pinned by the golden grids in
+    test_oracle.py) on real Voronoi faces: the cells of tests/golden/voronoi_map.npz near randomly
+    chosen particles, with smoothing lengths from 0.5x to 3x the fixture's.  The reachability flag
+    must agree exactly and the values to 1e-12 (the reference's acos of R0 / r12 and its cubic I-series
+    round differently from the restated geometry here); against the same face walk in binary128 the
+    values must lie within `qtol` (WC6: its monomials cancel near the support edge).  The tabulated
+    path has no binary128 form: 1e-13 against the oracle's double."""
+    lat.lat_face_walk.restype = ctypes.c_double
+    lat.lat_face_walk.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _pd, ctypes.c_double, _pd,
+                                  ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    g = np.load(os.path.join(HERE, "golden", "voronoi_map.npz"))
+    cfp, fvp, vx = g["cell_face_ptr"], g["face_vert_ptr"], g["vert_xyz"].reshape(-1, 3)
+    P = np.stack([g["p_x"], g["p_y"], g["p_z"]], 1)
+    k = O.kernel(name, generic=generic, tabulated=tabulated)
+    kid = KERNELS.index(name)
+    rng = np.random.default_rng(40 + kind + kid)
+    worst, worst_q, reach, nfaces = 0.0, 0.0, 0, 0
+    for _ in range(120):
+        i = rng.integers(len(P))
+        p = np.ascontiguousarray(P[i])
+        h = g["p_h"][i] * rng.uniform(0.5, 3.0)
+        near = np.argsort(np.linalg.norm(g["site"] - p, axis=1))[:12]
+        c = int(rng.choice(near))
+        for f in range(cfp[c], cfp[c + 1]):
+            v = np.ascontiguousarray(vx[fvp[f]:fvp[f + 1]]).reshape(-1)
+            r = ctypes.c_int()
+            m = lat.lat_face_walk(kind, kid, RADKERN[name], p.ctypes.data_as(_pd), h, v.ctypes.data_as(_pd),
+                                  len(v) // 3, ctypes.byref(r))
+            mo, ov = O.face(k, p, h, v)
+            assert r.value == int(ov != len(v) // 3), (i, c, f)
+            worst = max(worst, abs(m - mo))
+            if qtol is not None:
+                worst_q = max(worst_q, abs(m - O.face_quad(k, p, h, v)))
+            reach += r.value
+            nfaces += 1
+    assert worst < (1e-13 if tabulated else 1e-12), worst
+    if qtol is not None:
+        assert worst_q < qtol, worst_q
+    assert 0 < reach < nfaces  # both outcomes exercised

@@ -287,3 +287,53 @@ def test_operation_count_follows_the_computed_path():
     cnt = (ctypes.c_longlong * 5)()
     assert L.oracle_face_counted(0.1, 0.2, 0.0, 1.0, O._p(v), 4, cnt) == 0.0
     assert cnt[4] == 16  # only the face walk's own calls remain: 2 acos + 2 sin per edge, 4 edges
+
+
+# ------------------------------------------------------------------ Voronoi map
+def test_voronoi_map_matches_reference_cli():
+    """The oracle's Voronoi map (BFS over the tessellation with the face cache, src/mapping.cpp:87-206,
+    400-460) against the reference CLI's mapping.dat for the same sites (tests/golden/voronoi_map.npz:
+    250 C1 particles as sites, tessellated by the reference's vendored voro++; mapping.dat holds 15
+    significant digits)."""
+    g = np.load(os.path.join(GOLD, "voronoi_map.npz"))
+    parts = {k: g["p_" + k] for k in ("x", "y", "z", "vx", "vy", "vz", "m", "h")}
+    mass, px, py, pz, counts = O.voronoi_map(parts, g["pid"], g["home"], g["cell_face_ptr"], g["face_vert_ptr"],
+                                             g["vert_xyz"].reshape(-1), g["face_nbr"])
+    vol = g["volume"]
+    dens = np.where(vol > 0, mass / np.where(vol > 0, vol, 1), 0.0)
+    # The tessellation here uses voro++'s own block guess, the CLI's a scaled one: the cell cut order
+    # differs, so vertex coordinates agree to rounding only (~1e-16 relative), which moves a few cells'
+    # sums by ~1e-12 of the field maximum.
+    np.testing.assert_allclose(dens, g["ref_density"], rtol=0, atol=1e-11 * np.abs(g["ref_density"]).max())
+    vm = [parts[c].mean() for c in ("vx", "vy", "vz")]
+    w = g["ref_density"] / np.abs(g["ref_density"]).max()
+    for q, f, i in ((px, "vx", 0), (py, "vy", 1), (pz, "vz", 2)):
+        v = np.where(mass > 0, q / np.where(mass > 0, mass, 1), vm[i])
+        assert np.max(np.abs(v - g["ref_" + f]) * w) <= 1e-11 * np.abs(g["ref_" + f]).max()
+    assert counts[0] == 0  # every particle resolved to a cell
+    # the sites tile the domain
+    dom = g["domain"]
+    assert abs(vol.sum() - np.prod(dom[1::2] - dom[::2])) < 1e-10 * np.prod(dom[1::2] - dom[::2])
+
+
+def test_voronoi_binary128_arbiter():
+    """The binary128 cell sums used to arbitrate Voronoi cells (oracle.voronoi_map_cells_quad): the
+    reference's analytic cubic algorithm in binary128 equals the series-free generic path for the same
+    M4 kernel (in binary128 to ~1e-15, in double to ~2e-13), while the analytic algorithm in double
+    carries the I-series error; at 2x the fixture's smoothing lengths that error is ~1e-11 of the
+    field maximum."""
+    g = np.load(os.path.join(GOLD, "voronoi_map.npz"))
+    parts = {k: g["p_" + k] for k in ("x", "y", "z", "vx", "vy", "vz", "m", "h")}
+    parts["h"] = 2.0 * parts["h"]
+    geo = (g["cell_face_ptr"], g["face_vert_ptr"], g["vert_xyz"].reshape(-1), g["face_nbr"])
+    gen = O.kernel("M4", generic=True)
+    a = O.voronoi_map(parts, g["pid"], g["home"], *geo)
+    b = O.voronoi_map(parts, g["pid"], g["home"], *geo, k=gen)
+    cells = np.argsort(np.abs(a[0] - b[0]))[-25:]
+    qa = O.voronoi_map_cells_quad(parts, g["pid"], g["home"], *geo, cells)
+    qb = O.voronoi_map_cells_quad(parts, g["pid"], g["home"], *geo, cells, k=gen)
+    for f in range(4):
+        s = np.abs(b[f]).max()
+        assert np.max(np.abs(qa[:, f] - qb[:, f])) <= 1e-14 * s
+        assert np.max(np.abs(b[f][cells] - qb[:, f])) <= 5e-13 * s
+    assert np.max(np.abs(a[0][cells] - qa[:, 0])) > 1e-12 * np.abs(b[0]).max()  # the series error is real
