@@ -181,10 +181,18 @@ KIWI_FHD double face_walk_integral(double x0, double y0, double z0, double suppo
         min_edge = fmin(min_edge, beyond ? fmin(r12, r13) : R0);
         const double M = (s1 * s2 * r0 <= 0.0) ? -1.0 : 1.0;
         ev.edge(R0);
+        // one call site for the two vertex integrals (the evaluators inline large)
+        double I1 = 0.0, I2 = 0.0;
+#pragma unroll 1
+        for (int e = 0; e < 2; ++e) {
+            const double I = ev.at(e ? phi2 : phi1, e ? t3 : t2);
+            if (e) I2 = I;
+            else I1 = I;
+        }
         if (beyond)
-            total += (phi1 >= phi2) ? M * (ev.at(phi1, t2) - ev.at(phi2, t3)) : M * (ev.at(phi2, t3) - ev.at(phi1, t2));
+            total += (phi1 >= phi2) ? M * (I1 - I2) : M * (I2 - I1);
         else
-            total += M * (ev.at(phi1, t2) + ev.at(phi2, t3));
+            total += M * (I1 + I2);
     }
     // closest point of the face (:642-645); a NaN distance counts as reachable, as there
     const double fd = inside ? ar0 : sqrt(r0 * r0 + min_edge * min_edge);

@@ -78,6 +78,11 @@ __global__ void k_fill_i32(int* p, int64_t n, int v) {
     if (i < n) p[i] = v;
 }
 
+__global__ void k_fill_i64(int64_t* p, int64_t n, int64_t v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // ------------------------------------------------------------------------------------------------
 // 1. geometry: cell boxes, face -> cell, twins, validation
 // ------------------------------------------------------------------------------------------------
@@ -164,15 +169,19 @@ __device__ __forceinline__ bool cell_bucket_range(const BucketGrid& B, const dou
     return !empty;
 }
 
+// entries per cell, and the cell's lowest bucket per axis packed 10 bits each (x | y << 10 | z << 20)
 __global__ __launch_bounds__(THREADS) void k_vor_bucket_count(int64_t nc, BucketGrid B,
                                                               const double* __restrict__ box,
-                                                              int64_t* __restrict__ cnt) {
+                                                              int64_t* __restrict__ cnt, int* __restrict__ cell_blo) {
     const int64_t c = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
     if (c > nc) return;
     int64_t k = 0;
     int blo[3], bhi[3];
-    if (c < nc && cell_bucket_range(B, box, c, blo, bhi))
-        k = static_cast<int64_t>(bhi[0] - blo[0] + 1) * (bhi[1] - blo[1] + 1) * (bhi[2] - blo[2] + 1);
+    if (c < nc) {
+        if (cell_bucket_range(B, box, c, blo, bhi))
+            k = static_cast<int64_t>(bhi[0] - blo[0] + 1) * (bhi[1] - blo[1] + 1) * (bhi[2] - blo[2] + 1);
+        cell_blo[c] = blo[0] | (blo[1] << 10) | (blo[2] << 20);
+    }
     cnt[c] = k;  // cnt[nc] = 0 closes the scan
 }
 
@@ -193,32 +202,42 @@ __global__ __launch_bounds__(THREADS) void k_vor_bucket_emit(int64_t nc, BucketG
             }
 }
 
-// [beg, end) of each bucket in the sorted entries (buckets without entries keep 0, 0)
-__global__ __launch_bounds__(THREADS) void k_vor_bucket_ranges(int64_t ne, const uint64_t* __restrict__ keys,
-                                                               int* __restrict__ beg, int* __restrict__ end) {
+// boff[b] = first sorted entry of bucket b, boff[nbuckets] = ne: entries of bucket b are
+// [boff[b], boff[b+1]), and a run of consecutive buckets is one contiguous range
+__global__ __launch_bounds__(THREADS) void k_vor_bucket_offsets(int64_t ne, int64_t nbuckets,
+                                                                const uint64_t* __restrict__ keys,
+                                                                int* __restrict__ boff) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
     if (t >= ne) return;
-    const uint64_t b = keys[t] >> 32;
-    if (t == 0 || (keys[t - 1] >> 32) != b) beg[b] = static_cast<int>(t);
-    if (t == ne - 1 || (keys[t + 1] >> 32) != b) end[b] = static_cast<int>(t + 1);
+    const int64_t b = static_cast<int64_t>(keys[t] >> 32);
+    const int64_t prev = (t == 0) ? -1 : static_cast<int64_t>(keys[t - 1] >> 32);
+    for (int64_t bb = prev + 1; bb <= b; ++bb) boff[bb] = static_cast<int>(t);
+    if (t == ne - 1)
+        for (int64_t bb = b + 1; bb <= nbuckets; ++bb) boff[bb] = static_cast<int>(ne);
 }
 
 // ------------------------------------------------------------------------------------------------
 // 3. candidate (particle, cell) pairs: cell boxes that meet the support ball, plus the home cell
 // ------------------------------------------------------------------------------------------------
-// Active particles have h > 0 and a home cell (src/mapping.cpp:425, 436-440).  Each cell is taken
-// once per particle: in the lowest bucket, per axis, that both its box and the query box cover.
+// One wavefront per particle.  The query box's buckets are walked as runs along z, each one contiguous
+// range of sorted entries that the 64 lanes test in parallel; accepted cells are compacted in lane
+// order with a ballot, so the count and fill passes produce the same sequence.  Active particles have
+// h > 0 and a home cell (src/mapping.cpp:425, 436-440).  Each cell is taken once per particle: in the
+// lowest bucket, per axis, that both its box and the query box cover.
+__device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
+
 template <bool FILL>
 __global__ __launch_bounds__(THREADS) void k_vor_candidates(
     int64_t n, const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     const double* __restrict__ ph, const int* __restrict__ home, int64_t nc, double sup, BucketGrid B,
-    const double* __restrict__ box, const uint64_t* __restrict__ bkeys, const int* __restrict__ bbeg,
-    const int* __restrict__ bend, int64_t* __restrict__ cnt, const int64_t* __restrict__ off,
+    const double* __restrict__ box, const int* __restrict__ cell_blo, const uint64_t* __restrict__ bkeys,
+    const int* __restrict__ boff, int64_t* __restrict__ cnt, const int64_t* __restrict__ off,
     int* __restrict__ cand_cell, int* __restrict__ cand_part, MapDevStats* st) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const int64_t i = (static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x) >> 6;  // uniform per wave
     if (i > n) return;
     if (i == n) {
-        if (!FILL) cnt[n] = 0;
+        if (!FILL && lane == 0) cnt[n] = 0;
         return;
     }
     const double h = ph[i];
@@ -229,69 +248,77 @@ __global__ __launch_bounds__(THREADS) void k_vor_candidates(
     const bool finite_h = (h <= 0.0) || (h < INFINITY);  // NaN and +inf are rejected by the caller
     if (finite_h && h > 0.0 && hc >= 0 && hc < nc) {
         const double H = sup * h;
-        const double p[3] = {px[i], py[i], pz[i]};
+        const double p0 = px[i], p1 = py[i], p2 = pz[i];
         const double lim = H * SUPPORT_MARGIN;
         const double lim2 = lim * lim;
-        int qlo[3], qhi[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            qlo[a] = bucket_axis(B, a, p[a] - lim);
-            qhi[a] = bucket_axis(B, a, p[a] + lim);
-        }
+        const int q0 = bucket_axis(B, 0, p0 - lim), q0e = bucket_axis(B, 0, p0 + lim);
+        const int q1 = bucket_axis(B, 1, p1 - lim), q1e = bucket_axis(B, 1, p1 + lim);
+        const int q2 = bucket_axis(B, 2, p2 - lim), q2e = bucket_axis(B, 2, p2 + lim);
+        const uint64_t below = lanes_below(lane);
         bool home_seen = false;
-        for (int x = qlo[0]; x <= qhi[0]; ++x)
-            for (int y = qlo[1]; y <= qhi[1]; ++y)
-                for (int z = qlo[2]; z <= qhi[2]; ++z) {
-                    const int b = (x * B.g1 + y) * B.g2 + z;
-                    for (int e = bbeg[b]; e < bend[b]; ++e) {
-                        const int c = static_cast<int>(bkeys[e] & 0xffffffffu);
-                        const double* bx = box + 6 * static_cast<int64_t>(c);
+        for (int x = q0; x <= q0e; ++x)
+            for (int y = q1; y <= q1e; ++y) {
+                const int row = (x * B.g1 + y) * B.g2;
+                const int e0 = boff[row + q2], e1 = boff[row + q2e + 1];
+                for (int base = e0; base < e1; base += 64) {
+                    const int e = base + lane;
+                    bool acc = false, is_home = false;
+                    int c = 0;
+                    if (e < e1) {
+                        const uint64_t key = bkeys[e];
+                        c = static_cast<int>(key & 0xffffffffu);
+                        const int z = static_cast<int>(key >> 32) - row;
+                        const int cb = cell_blo[c];
                         // lowest shared bucket per axis
-                        const int b0 = max(bucket_axis(B, 0, bx[0]), qlo[0]);
-                        const int b1 = max(bucket_axis(B, 1, bx[1]), qlo[1]);
-                        const int b2 = max(bucket_axis(B, 2, bx[2]), qlo[2]);
-                        if (b0 != x || b1 != y || b2 != z) continue;
-                        // rounded operations: the count and fill passes must take identical decisions
-                        double d2 = 0.0;
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            const double g = gap(bx[a] - p[a], bx[3 + a] - p[a]);
-                            d2 = __dadd_rn(d2, __dmul_rn(g, g));
-                        }
-                        const bool is_home = (c == hc);
-                        if (d2 < lim2 || is_home) {
-                            home_seen |= is_home;
-                            if (FILL && o < o_end) {
-                                cand_cell[o] = c;
-                                cand_part[o] = static_cast<int>(i);
-                                ++o;
-                            }
-                            ++k;
+                        if (max(cb & 1023, q0) == x && max((cb >> 10) & 1023, q1) == y && max(cb >> 20, q2) == z) {
+                            const double* bx = box + 6 * static_cast<int64_t>(c);
+                            // rounded operations: the count and fill passes take identical decisions
+                            const double g0 = gap(bx[0] - p0, bx[3] - p0);
+                            const double g1 = gap(bx[1] - p1, bx[4] - p1);
+                            const double g2 = gap(bx[2] - p2, bx[5] - p2);
+                            const double d2 = __dadd_rn(__dadd_rn(__dmul_rn(g0, g0), __dmul_rn(g1, g1)),
+                                                        __dmul_rn(g2, g2));
+                            is_home = (c == hc);
+                            acc = (d2 < lim2) || is_home;
                         }
                     }
+                    const uint64_t m = __ballot(acc);
+                    home_seen |= (__ballot(acc && is_home) != 0ull);
+                    if (FILL && acc) {
+                        const int64_t pos = o + __popcll(m & below);
+                        if (pos < o_end) {
+                            cand_cell[pos] = c;
+                            cand_part[pos] = static_cast<int>(i);
+                        }
+                    }
+                    if (FILL) o += __popcll(m);
+                    k += __popcll(m);
                 }
+            }
         if (!home_seen) {
-            if (FILL && o < o_end) {
+            if (FILL && lane == 0 && o < o_end) {
                 cand_cell[o] = hc;
                 cand_part[o] = static_cast<int>(i);
-                ++o;
             }
+            if (FILL) ++o;
             ++k;
         }
         if (FILL && k != o_end - off[i]) {
             // cannot happen with identical decisions; keep every slot valid and report it
-            for (; o < o_end; ++o) {
-                cand_cell[o] = hc;
-                cand_part[o] = static_cast<int>(i);
+            if (lane == 0) {
+                for (int64_t q = off[i] + (k < o_end - off[i] ? k : o_end - off[i]); q < o_end; ++q) {
+                    cand_cell[q] = hc;
+                    cand_part[q] = static_cast<int>(i);
+                }
+                atomicAdd(&st->fill_mismatch, 1ull);
             }
-            atomicAdd(&st->fill_mismatch, 1ull);
         }
-    } else if (!FILL) {
+    } else if (!FILL && lane == 0) {
         if (!finite_h) atomicAdd(&st->bad_particle, 1ull);
         else if (h > 0.0 && hc >= nc) atomicAdd(&st->bad_home, 1ull);
         else if (h > 0.0) atomicAdd(&st->unresolved, 1ull);  // h <= 0 is skipped silently (:425)
     }
-    if (!FILL) cnt[i] = k;
+    if (!FILL && lane == 0) cnt[i] = k;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -436,6 +463,50 @@ __global__ __launch_bounds__(THREADS) void k_vor_items(int64_t nvis, const uint6
     int64_t slot = slot_off[q], o = own_off[q];
     for (int64_t f = f0; f < f1; ++f, ++slot)
         if (src[slot] >= 0) items[o++] = FaceItem{static_cast<int>(q), static_cast<int>(f)};
+}
+
+// Face items grouped by vertex count (a stable counting sort), so that the lanes of a wavefront walk
+// faces with the same number of edges.  One wavefront per chunk of BIN_CHUNK items; hist[bin][chunk]
+// counts, and after an exclusive scan in (bin, chunk) order gives each chunk's first slot per bin.
+constexpr int NV_BINS = 14;  // 3 .. 15 vertices, and 16 or more
+constexpr int BIN_CHUNK = 512;
+
+__device__ __forceinline__ int nv_bin(const int64_t* __restrict__ fvp, int f) {
+    const int64_t nv = fvp[f + 1] - fvp[f];
+    return nv < 3 ? 0 : (nv > 16 ? NV_BINS - 1 : static_cast<int>(nv) - 3);
+}
+
+template <bool SCATTER>
+__global__ __launch_bounds__(THREADS) void k_vor_item_bins(int64_t nitems, int64_t nchunks,
+                                                           const FaceItem* __restrict__ items,
+                                                           const int64_t* __restrict__ fvp,
+                                                           int64_t* __restrict__ hist, FaceItem* __restrict__ out) {
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const int64_t chunk = (static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x) >> 6;
+    if (chunk >= nchunks) return;
+    const uint64_t below = lanes_below(lane);
+    // per-bin running position (counts in the first pass), one bin per lane 0..NV_BINS-1
+    int64_t run = SCATTER && lane < NV_BINS ? hist[static_cast<int64_t>(lane) * nchunks + chunk] : 0;
+    const int64_t t0 = chunk * BIN_CHUNK;
+    for (int r = 0; r < BIN_CHUNK / 64; ++r) {
+        const int64_t t = t0 + r * 64 + lane;
+        const bool ok = t < nitems;
+        FaceItem it{0, 0};
+        int b = -1;
+        if (ok) {
+            it = items[t];
+            b = nv_bin(fvp, it.face);
+        }
+        int64_t pos = 0;
+        for (int q = 0; q < NV_BINS; ++q) {
+            const uint64_t m = __ballot(b == q);
+            const int64_t base = __shfl(run, q);
+            if (b == q) pos = base + __popcll(m & below);
+            if (lane == q) run += __popcll(m);
+        }
+        if (SCATTER && ok) out[pos] = it;
+    }
+    if (!SCATTER && lane < NV_BINS) hist[static_cast<int64_t>(lane) * nchunks + chunk] = run;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -734,13 +805,15 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* ho
     }
     const int64_t nbuckets = static_cast<int64_t>(B.g0) * B.g1 * B.g2;
     const int bbits = bits_for(nbuckets);
-    KIWI_HIP_CHECK(c, reserve(c->vor[3], bytes_of({sizeof(int64_t) * (nc + 1), sizeof(int64_t) * (nc + 1), 2 * sizeof(int) * nbuckets})));
+    if (nbuckets > 0x7ffffffeLL) return fail(c, KIWI_ERR_LIMIT, "bucket grid too large");
+    KIWI_HIP_CHECK(c, reserve(c->vor[3], bytes_of({sizeof(int64_t) * (nc + 1), sizeof(int64_t) * (nc + 1),
+                                                   sizeof(int) * (nbuckets + 1), sizeof(int) * nc})));
     size_t o3 = 0;
     int64_t* bcnt = at_offset<int64_t>(c->vor[3].p, o3, nc + 1);
     int64_t* boff = at_offset<int64_t>(c->vor[3].p, o3, nc + 1);
-    int* bbeg = at_offset<int>(c->vor[3].p, o3, 2 * nbuckets);
-    int* bend = bbeg + nbuckets;
-    k_vor_bucket_count<<<grid1d(nc + 1), THREADS, 0, s>>>(nc, B, box, bcnt);
+    int* bucket_off = at_offset<int>(c->vor[3].p, o3, nbuckets + 1);
+    int* cell_blo = at_offset<int>(c->vor[3].p, o3, nc);
+    k_vor_bucket_count<<<grid1d(nc + 1), THREADS, 0, s>>>(nc, B, box, bcnt, cell_blo);
     KIWI_HIP_CHECK(c, hipGetLastError());
     KIWI_HIP_CHECK(c, scan_i64(c, bcnt, boff, nc + 1, s));
     int64_t nentries = 0;
@@ -750,12 +823,12 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* ho
     size_t o4 = 0;
     uint64_t* bkeys_in = at_offset<uint64_t>(c->vor[4].p, o4, nentries);
     uint64_t* bkeys = at_offset<uint64_t>(c->vor[4].p, o4, nentries);
-    k_fill_i32<<<grid1d(2 * nbuckets), THREADS, 0, s>>>(bbeg, 2 * nbuckets, 0);  // bbeg and bend are adjacent
+    k_fill_i32<<<grid1d(nbuckets + 1), THREADS, 0, s>>>(bucket_off, nbuckets + 1, 0);
     if (nc > 0) k_vor_bucket_emit<<<grid1d(nc), THREADS, 0, s>>>(nc, B, box, boff, bkeys_in);
     KIWI_HIP_CHECK(c, hipGetLastError());
     if (nentries > 0) {
         KIWI_HIP_CHECK(c, sort_keys(c, bkeys_in, bkeys, nentries, 32, 32 + bbits, s));
-        k_vor_bucket_ranges<<<grid1d(nentries), THREADS, 0, s>>>(nentries, bkeys, bbeg, bend);
+        k_vor_bucket_offsets<<<grid1d(nentries), THREADS, 0, s>>>(nentries, nbuckets, bkeys, bucket_off);
         KIWI_HIP_CHECK(c, hipGetLastError());
     }
 
@@ -766,8 +839,9 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* ho
     int64_t* ccnt = at_offset<int64_t>(c->vor[5].p, o5, n + 1);
     int64_t* coff = at_offset<int64_t>(c->vor[5].p, o5, n + 1);
     int64_t* vis_off = at_offset<int64_t>(c->vor[5].p, o5, n + 1);
-    k_vor_candidates<false><<<grid1d(n + 1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], home, nc, sup, B, box, bkeys,
-                                                               bbeg, bend, ccnt, nullptr, nullptr, nullptr, dst);
+    k_vor_candidates<false><<<grid1d(64 * (n + 1)), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], home, nc, sup, B,
+                                                                      box, cell_blo, bkeys, bucket_off, ccnt, nullptr,
+                                                                      nullptr, nullptr, dst);
     KIWI_HIP_CHECK(c, hipGetLastError());
     KIWI_HIP_CHECK(c, scan_i64(c, ccnt, coff, n + 1, s));
     int64_t ncand = 0;
@@ -785,8 +859,9 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* ho
     int64_t* flag = at_offset<int64_t>(c->vor[6].p, o6, ncand + 1);
     int64_t* flag_off = at_offset<int64_t>(c->vor[6].p, o6, ncand + 1);
     if (n > 0)
-        k_vor_candidates<true><<<grid1d(n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], home, nc, sup, B, box, bkeys,
-                                                             bbeg, bend, nullptr, coff, cand_cell, cand_part, dst);
+        k_vor_candidates<true><<<grid1d(64 * n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], home, nc, sup, B, box,
+                                                                  cell_blo, bkeys, bucket_off, nullptr, coff, cand_cell,
+                                                                  cand_part, dst);
     KIWI_HIP_CHECK(c, hipGetLastError());
     {
         unsigned long long mism = 0;
@@ -836,9 +911,27 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* ho
     int64_t nitems = 0;
     KIWI_HIP_CHECK(c, read_back(&nitems, own_off + nvis, s));
     if (nitems > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 face integrals in one call");
-    KIWI_HIP_CHECK(c, reserve(c->vor[9], bytes_of({sizeof(FaceItem) * nitems})));
-    FaceItem* items = static_cast<FaceItem*>(c->vor[9].p);
-    if (nvis > 0) k_vor_items<<<grid1d(nvis), THREADS, 0, s>>>(nvis, vkeys, cmask, slot_off, own_off, cfp, src, items);
+    const int64_t nchunks = (nitems + BIN_CHUNK - 1) / BIN_CHUNK;
+    KIWI_HIP_CHECK(c, reserve(c->vor[9], bytes_of({sizeof(FaceItem) * nitems, sizeof(FaceItem) * nitems,
+                                                   sizeof(int64_t) * (NV_BINS * nchunks + 1),
+                                                   sizeof(int64_t) * (NV_BINS * nchunks + 1)})));
+    size_t o9 = 0;
+    FaceItem* items_pair = at_offset<FaceItem>(c->vor[9].p, o9, nitems);
+    FaceItem* items = at_offset<FaceItem>(c->vor[9].p, o9, nitems);
+    int64_t* bin_cnt = at_offset<int64_t>(c->vor[9].p, o9, NV_BINS * nchunks + 1);
+    int64_t* bin_off = at_offset<int64_t>(c->vor[9].p, o9, NV_BINS * nchunks + 1);
+    if (nvis > 0)
+        k_vor_items<<<grid1d(nvis), THREADS, 0, s>>>(nvis, vkeys, cmask, slot_off, own_off, cfp, src, items_pair);
+    KIWI_HIP_CHECK(c, hipGetLastError());
+    if (nitems > 0) {
+        k_fill_i64<<<1, 64, 0, s>>>(bin_cnt + NV_BINS * nchunks, 1, 0);
+        k_vor_item_bins<false><<<grid1d(64 * nchunks), THREADS, 0, s>>>(nitems, nchunks, items_pair, fvp, bin_cnt,
+                                                                       nullptr);
+        KIWI_HIP_CHECK(c, hipGetLastError());
+        KIWI_HIP_CHECK(c, scan_i64(c, bin_cnt, bin_off, NV_BINS * nchunks + 1, s));
+        k_vor_item_bins<true><<<grid1d(64 * nchunks), THREADS, 0, s>>>(nitems, nchunks, items_pair, fvp, bin_off,
+                                                                      items);
+    }
     KIWI_HIP_CHECK(c, hipGetLastError());
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[1], s));
 

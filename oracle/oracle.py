@@ -109,11 +109,12 @@ def _p(a):
 
 def tessellate(ids, pts, domain, particles):
     """Voronoi geometry of the points (reference's vendored voro++, oracle/_ref/libkiwi_voro.so) as CSR
-    arrays, plus each particle's home cell.  pts: (n, 3); particles: (m, 3).  Returns a dict."""
+    arrays, each cell's volume and centroid, plus each particle's home cell.  pts: (n, 3); particles:
+    (m, 3).  Returns a dict."""
     V = ctypes.CDLL(VORO_SO)
     V.voro_build.restype = ctypes.c_void_p
     V.voro_build.argtypes = [_i, _pi, _pd, _pd, _pd, _pd, _ll, _pd, _pd, _pd, _pll]
-    V.voro_fetch.argtypes = [ctypes.c_void_p, _pi, _pd, _pd, _pll, _pll, _pd, _pi, _pi]
+    V.voro_fetch.argtypes = [ctypes.c_void_p, _pi, _pd, _pd, _pd, _pll, _pll, _pd, _pi, _pi]
     V.voro_free.argtypes = [ctypes.c_void_p]
     ids = np.ascontiguousarray(ids, dtype=np.int32)
     P = [np.ascontiguousarray(pts[:, k], dtype=np.float64) for k in range(3)]
@@ -123,10 +124,10 @@ def tessellate(ids, pts, domain, particles):
     h = V.voro_build(len(ids), ids.ctypes.data_as(_pi), *[_p(a) for a in P], _p(dom), len(Q[0]), *[_p(a) for a in Q],
                      sizes.ctypes.data_as(_pll))
     nc, nf, nv = (int(v) for v in sizes)
-    g = {"cell_id": np.zeros(nc, np.int32), "site": np.zeros((nc, 3)), "volume": np.zeros(nc),
+    g = {"cell_id": np.zeros(nc, np.int32), "site": np.zeros((nc, 3)), "volume": np.zeros(nc), "centroid": np.zeros((nc, 3)),
          "cell_face_ptr": np.zeros(nc + 1, np.int64), "face_vert_ptr": np.zeros(nf + 1, np.int64),
          "vert_xyz": np.zeros((nv, 3)), "face_nbr": np.zeros(nf, np.int32), "home": np.zeros(len(Q[0]), np.int32)}
-    V.voro_fetch(h, g["cell_id"].ctypes.data_as(_pi), _p(g["site"]), _p(g["volume"]),
+    V.voro_fetch(h, g["cell_id"].ctypes.data_as(_pi), _p(g["site"]), _p(g["volume"]), _p(g["centroid"]),
                  g["cell_face_ptr"].ctypes.data_as(_pll), g["face_vert_ptr"].ctypes.data_as(_pll), _p(g["vert_xyz"]),
                  g["face_nbr"].ctypes.data_as(_pi), g["home"].ctypes.data_as(_pi))
     V.voro_free(h)

@@ -3,8 +3,9 @@
 // mapping stage does (non-periodic container over the domain box; src/mapping.cpp:222-269), flattens
 // every cell's faces to plain CSR arrays with absolute vertex coordinates (f_coord + site, as
 // compute_cell_integral forms them, src/mapping.cpp:168-175) and neighbour cell indices, and finds each
-// particle's home cell with find_voronoi_cell (src/mapping.cpp:434-440).  The Voronoi golden fixture is
-// written from it; nothing here is shipped or run on a GPU.
+// particle's home cell with find_voronoi_cell (src/mapping.cpp:434-440); each cell's centroid gives
+// the Lloyd step (lloyd_step, src/point_set.cpp:275-306).  The Voronoi golden fixture and the Voronoi
+// test and measurement inputs are written from it; nothing here is shipped or run on a GPU.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -15,7 +16,7 @@ namespace {
 
 struct Geometry {
     std::vector<int> cell_id;
-    std::vector<double> site, volume;
+    std::vector<double> site, volume, centroid;
     std::vector<long long> cell_face_ptr{0}, face_vert_ptr{0};
     std::vector<double> vert_xyz;
     std::vector<int> face_nbr, home;
@@ -63,6 +64,11 @@ void* voro_build(int npts, const int* id, const double* x, const double* y, cons
                 f_coord_all.push_back(fc);
                 n_list_all.push_back(nl);
                 g->volume.push_back(c.volume());
+                double cx, cy, cz;
+                c.centroid(cx, cy, cz);
+                g->centroid.push_back(x0 + cx);
+                g->centroid.push_back(y0 + cy);
+                g->centroid.push_back(z0 + cz);
             }
         } while (clo.inc());
 
@@ -102,12 +108,13 @@ void* voro_build(int npts, const int* id, const double* x, const double* y, cons
     return g;
 }
 
-void voro_fetch(void* h, int* cell_id, double* site, double* volume, long long* cell_face_ptr,
+void voro_fetch(void* h, int* cell_id, double* site, double* volume, double* centroid, long long* cell_face_ptr,
                 long long* face_vert_ptr, double* vert_xyz, int* face_nbr, int* home) {
     const Geometry* g = static_cast<const Geometry*>(h);
     std::memcpy(cell_id, g->cell_id.data(), sizeof(int) * g->cell_id.size());
     std::memcpy(site, g->site.data(), sizeof(double) * g->site.size());
     std::memcpy(volume, g->volume.data(), sizeof(double) * g->volume.size());
+    std::memcpy(centroid, g->centroid.data(), sizeof(double) * g->centroid.size());
     std::memcpy(cell_face_ptr, g->cell_face_ptr.data(), sizeof(long long) * g->cell_face_ptr.size());
     std::memcpy(face_vert_ptr, g->face_vert_ptr.data(), sizeof(long long) * g->face_vert_ptr.size());
     std::memcpy(vert_xyz, g->vert_xyz.data(), sizeof(double) * g->vert_xyz.size());
