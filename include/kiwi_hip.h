@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KIWI_HIP_ABI_VERSION 2
+#define KIWI_HIP_ABI_VERSION 3
 
 #define KIWI_OK 0
 #define KIWI_ERR_INVALID_ARG (-1)
@@ -156,6 +156,31 @@ void* kiwi_hip_stream(kiwi_hip_ctx* ctx);
 int kiwi_hip_sph_to_grid(kiwi_hip_ctx* ctx, const kiwi_particles* particles, const kiwi_grid* grid,
                          const kiwi_kernel* kernel, int precision, const kiwi_cell_sums* out,
                          kiwi_stats* stats);
+
+/* Per-particle gradients, n each; on_device selects device or host pointers. */
+typedef struct kiwi_particle_grads {
+    double *m, *x, *y, *z, *h, *vx, *vy, *vz;
+    int on_device;
+} kiwi_particle_grads;
+
+typedef struct kiwi_adjoint_stats {
+    int64_t skipped_outside; /* particles whose support box misses the grid (zero gradients) */
+    int64_t pairs_bbox;      /* (particle, cell) pairs in the bounding boxes */
+    int64_t lattice_points;  /* lattice values and gradients evaluated */
+    int64_t cells;           /* (particle, cell) pairs inside the support */
+    double device_ms_prep;   /* boxes, work order */
+    double device_ms;        /* line constants and the gradient pass */
+} kiwi_adjoint_stats;
+
+/* The adjoint of kiwi_hip_sph_to_grid (no counterpart in the reference): given dL/d(mass, px, py, pz) per
+ * cell (grad_cells, n_cells each, the un-normalised sums the forward returns), the gradients of L with
+ * respect to each particle's m, x, y, z, h, vx, vy, vz.  The kernel integrals are recomputed, not stored.
+ * Analytic cubic spline only (M4, support 2h; other kernels: KIWI_ERR_UNSUPPORTED).  FP64 differentiates
+ * the FP64 forward (pairs it clamps at 0 have zero derivative); KIWI_MIXED evaluates the lattice values and
+ * gradients in FP32 with FP64 accumulation. */
+int kiwi_hip_sph_to_grid_adjoint(kiwi_hip_ctx* ctx, const kiwi_particles* particles, const kiwi_grid* grid,
+                                 const kiwi_kernel* kernel, int precision, const kiwi_cell_sums* grad_cells,
+                                 const kiwi_particle_grads* grad_particles, kiwi_adjoint_stats* stats);
 
 /* Voronoi map, replacing the particle loop of run_mapping_stage (src/mapping.cpp:400-460).
  * home_cell[n]: each particle's cell (voro++ find_voronoi_cell resolved through reverse_id_list,

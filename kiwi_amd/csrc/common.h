@@ -125,10 +125,47 @@ int resident_grid(const kiwi_hip_ctx* c, K kernel, int64_t items_per_wave_round,
     return static_cast<int>(std::max<int64_t>(1, std::min(full, need)));
 }
 
+// ------------------------------------------------------------------------------------------------
+// Cartesian grid geometry shared by the forward and the adjoint
+// ------------------------------------------------------------------------------------------------
+constexpr int BLK = 8;  // cell block edge (cells)
+
+struct GridGeom {
+    double x_min, y_min, z_min, dx, dy, dz;
+    int nx, ny, nz;
+    int nbx, nby, nbz;  // cell blocks per axis
+};
+
+__device__ __forceinline__ double gmin(const GridGeom& g, int a) { return sel3(a, g.x_min, g.y_min, g.z_min); }
+__device__ __forceinline__ double gstep(const GridGeom& g, int a) { return sel3(a, g.dx, g.dy, g.dz); }
+
+// floor(v) as a cell index, saturated to [-1, n] before the conversion so that huge supports and
+// non-finite coordinates cannot overflow it (NaN lands on n, outside the grid).  Within range this is
+// exactly the reference's static_cast<int>(std::floor(v)) (src/cartesian_grid.cpp:220-225).
+__device__ __forceinline__ int cell_floor(double v, int n) {
+    const double f = floor(v);
+    if (f >= -1.0 && f <= static_cast<double>(n)) return static_cast<int>(f);
+    return (f < -1.0) ? -1 : n;
+}
+
+// the particle's cell bounding box [q0, q1] x [q2, q3] x [q4, q5] clamped to the grid
+// (src/cartesian_grid.cpp:216-238); false if it misses the grid.  Requires h > 0.
+__device__ __forceinline__ bool particle_bbox(const GridGeom& g, double support_in_h, double x0, double y0, double z0,
+                                              double hh, int* q) {
+    const double rad = support_in_h * hh;
+    q[0] = max(cell_floor((x0 - rad - g.x_min) / g.dx, g.nx), 0);
+    q[1] = min(cell_floor((x0 + rad - g.x_min) / g.dx, g.nx), g.nx - 1);
+    q[2] = max(cell_floor((y0 - rad - g.y_min) / g.dy, g.ny), 0);
+    q[3] = min(cell_floor((y0 + rad - g.y_min) / g.dy, g.ny), g.ny - 1);
+    q[4] = max(cell_floor((z0 - rad - g.z_min) / g.dz, g.nz), 0);
+    q[5] = min(cell_floor((z0 + rad - g.z_min) / g.dz, g.nz), g.nz - 1);
+    return q[0] <= q[1] && q[2] <= q[3] && q[4] <= q[5];
+}
+
 // deterministic velocity sums over all particles: fixed per-thread strides, fixed tree order
 constexpr int VSUM_BLOCKS = 256;
 
-__global__ __launch_bounds__(THREADS) void k_vsum_partial(int64_t n, const double* __restrict__ vx,
+__global__ __launch_bounds__(THREADS) __attribute__((unused)) void k_vsum_partial(int64_t n, const double* __restrict__ vx,
                                                           const double* __restrict__ vy,
                                                           const double* __restrict__ vz, double* partial) {
     __shared__ double s[3][THREADS];
@@ -152,7 +189,7 @@ __global__ __launch_bounds__(THREADS) void k_vsum_partial(int64_t n, const doubl
         for (int k = 0; k < 3; ++k) partial[3 * blockIdx.x + k] = s[k][0];
 }
 
-__global__ void k_vsum_final(const double* partial, double* out) {
+__global__ __attribute__((unused)) void k_vsum_final(const double* partial, double* out) {
     if (threadIdx.x != 0) return;
     for (int k = 0; k < 3; ++k) {
         double t = 0;

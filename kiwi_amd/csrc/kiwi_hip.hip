@@ -48,7 +48,6 @@ using namespace kiwi_hip;
 
 namespace {
 
-constexpr int BLK = 8;                        // cell block edge (cells)
 constexpr int BLK_PTS = (BLK + 1) * (BLK + 1) * (BLK + 1);
 constexpr int MAX_AXIS = 8192;                // grid cells per axis
 constexpr unsigned ITEM_GRAB = 8;             // items a wavefront takes from the cursor at once
@@ -61,27 +60,9 @@ struct DevStats {
     kiwi_warn_record warn[KIWI_MAX_WARN_RECORDS];
 };
 
-struct GridGeom {
-    double x_min, y_min, z_min, dx, dy, dz;
-    int nx, ny, nz;
-    int nbx, nby, nbz;  // cell blocks per axis
-};
-
-__device__ __forceinline__ double gmin(const GridGeom& g, int a) { return sel3(a, g.x_min, g.y_min, g.z_min); }
-__device__ __forceinline__ double gstep(const GridGeom& g, int a) { return sel3(a, g.dx, g.dy, g.dz); }
-
 // ------------------------------------------------------------------------------------------------
 // 1. per-particle bounding box and work sizes
 // ------------------------------------------------------------------------------------------------
-// floor(v) as a cell index, saturated to [-1, n] before the conversion so that huge supports and
-// non-finite coordinates cannot overflow it (NaN lands on n, outside the grid).  Within range this is
-// exactly the reference's static_cast<int>(std::floor(v)) (src/cartesian_grid.cpp:220-225).
-__device__ __forceinline__ int cell_floor(double v, int n) {
-    const double f = floor(v);
-    if (f >= -1.0 && f <= static_cast<double>(n)) return static_cast<int>(f);
-    return (f < -1.0) ? -1 : n;
-}
-
 __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                        const double* __restrict__ z, const double* __restrict__ h, double support_in_h,
                        GridGeom g, int* __restrict__ bbox, int64_t* __restrict__ nblk, int64_t* __restrict__ nlines,
@@ -100,28 +81,12 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
         const double hh = h[i];
         const double x0 = x[i], y0 = y[i], z0 = z[i];
         if (hh > 0.0) {
-            const double rad = support_in_h * hh;
-            int ix0 = cell_floor((x0 - rad - g.x_min) / g.dx, g.nx);
-            int ix1 = cell_floor((x0 + rad - g.x_min) / g.dx, g.nx);
-            int iy0 = cell_floor((y0 - rad - g.y_min) / g.dy, g.ny);
-            int iy1 = cell_floor((y0 + rad - g.y_min) / g.dy, g.ny);
-            int iz0 = cell_floor((z0 - rad - g.z_min) / g.dz, g.nz);
-            int iz1 = cell_floor((z0 + rad - g.z_min) / g.dz, g.nz);
-            ix0 = max(ix0, 0);
-            ix1 = min(ix1, g.nx - 1);
-            iy0 = max(iy0, 0);
-            iy1 = min(iy1, g.ny - 1);
-            iz0 = max(iz0, 0);
-            iz1 = min(iz1, g.nz - 1);
-            if (ix0 > ix1 || iy0 > iy1 || iz0 > iz1) {
+            int q[6];
+            if (!particle_bbox(g, support_in_h, x0, y0, z0, hh, q)) {
                 skipped = 1;
             } else {
-                bb[0] = ix0;
-                bb[1] = ix1;
-                bb[2] = iy0;
-                bb[3] = iy1;
-                bb[4] = iz0;
-                bb[5] = iz1;
+                const int ix0 = q[0], ix1 = q[1], iy0 = q[2], iy1 = q[3], iz0 = q[4], iz1 = q[5];
+                for (int k = 0; k < 6; ++k) bb[k] = q[k];
                 const int64_t Nx = ix1 - ix0 + 1, Ny = iy1 - iy0 + 1, Nz = iz1 - iz0 + 1;
                 pairs = static_cast<unsigned long long>(Nx * Ny * Nz);
                 nb = static_cast<int64_t>(ix1 / BLK - ix0 / BLK + 1) * (iy1 / BLK - iy0 / BLK + 1) *

@@ -325,9 +325,17 @@ __global__ __launch_bounds__(THREADS) void k_vor_candidates(
 // 4. visits
 // ------------------------------------------------------------------------------------------------
 // the face reachability test of the reference's face walk (out_vert != n, src/kernel_integral.cpp:
-// 478-481, 642-645): a face with fewer than three vertices counts as reachable unless it has none
+// 478-481, 642-645): a face with fewer than three vertices counts as reachable unless it has none.  The
+// face's distance is at least its plane's, so |r0| >= H rejects it before the edge walk.
 __device__ __forceinline__ bool face_reachable(double x0, double y0, double z0, double H, const double* v, int nvf) {
     if (nvf < 3) return nvf != 0;
+    const double x1 = v[0], y1 = v[1], z1 = v[2];
+    const double ax = v[3] - x1, ay = v[4] - y1, az = v[5] - z1;
+    const double bx = v[6] - x1, by = v[7] - y1, bz = v[8] - z1;
+    const double A = ay * bz - by * az, B = az * bx - bz * ax, C = ax * by - bx * ay;
+    const double nrm = sqrt(A * A + B * B + C * C);
+    const double r0 = (A * (x0 - x1) + B * (y0 - y1) + C * (z0 - z1)) / nrm;
+    if (fabs(r0) >= H) return false;
     EvNone ev;
     bool r = false;
     (void)face_walk_integral(x0, y0, z0, H, v, nvf, ev, &r);

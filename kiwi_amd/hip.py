@@ -31,7 +31,7 @@ _i = ctypes.c_int
 _pd = ctypes.POINTER(ctypes.c_double)
 
 EXPORTED = ["kiwi_hip_abi_version", "kiwi_hip_create", "kiwi_hip_destroy", "kiwi_hip_last_error",
-            "kiwi_hip_stream", "kiwi_hip_sph_to_grid", "kiwi_hip_voronoi_map"]
+            "kiwi_hip_stream", "kiwi_hip_sph_to_grid", "kiwi_hip_voronoi_map", "kiwi_hip_sph_to_grid_adjoint"]
 
 
 class KiwiError(RuntimeError):
@@ -102,6 +102,18 @@ class MapStats(ctypes.Structure):
         return d
 
 
+class ParticleGrads(ctypes.Structure):
+    _fields_ = [("m", _pd), ("x", _pd), ("y", _pd), ("z", _pd), ("h", _pd), ("vx", _pd), ("vy", _pd), ("vz", _pd),
+                ("on_device", _i)]
+
+
+class AdjointStats(ctypes.Structure):
+    _fields_ = [("skipped_outside", _i64), ("pairs_bbox", _i64), ("lattice_points", _i64), ("cells", _i64),
+                ("device_ms_prep", _d), ("device_ms", _d)]
+
+
+GRAD_FIELDS = ("m", "x", "y", "z", "h", "vx", "vy", "vz")
+
 _lib = None
 
 
@@ -129,6 +141,10 @@ def load():
         L.kiwi_hip_sph_to_grid.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.POINTER(Grid),
                                            ctypes.POINTER(Kernel), _i, ctypes.POINTER(CellSums),
                                            ctypes.POINTER(Stats)]
+        L.kiwi_hip_sph_to_grid_adjoint.restype = _i
+        L.kiwi_hip_sph_to_grid_adjoint.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.POINTER(Grid),
+                                                   ctypes.POINTER(Kernel), _i, ctypes.POINTER(CellSums),
+                                                   ctypes.POINTER(ParticleGrads), ctypes.POINTER(AdjointStats)]
         L.kiwi_hip_voronoi_map.restype = _i
         L.kiwi_hip_voronoi_map.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.c_void_p,
                                            ctypes.POINTER(VoronoiCells), ctypes.POINTER(Kernel), _i,
@@ -219,6 +235,31 @@ class Context:
         self._check(self.L.kiwi_hip_sph_to_grid(self.h, ctypes.byref(P), ctypes.byref(G), ctypes.byref(kernel),
                                                 int(precision), ctypes.byref(O), ctypes.byref(st)))
         return arrs[0], arrs[1], arrs[2], arrs[3], st
+
+    def sph_to_grid_adjoint(self, parts, domain, res, grad_cells, kernel=None, precision=FP64, out=None):
+        """Gradients of a loss L with respect to the particles' m, x, y, z, h, vx, vy, vz, given grad_cells =
+        (dL/dmass, dL/dpx, dL/dpy, dL/dpz) per cell (numpy float64 or device float64 tensors of nx*ny*nz).
+        out: None (host numpy results) or a dict of 8 device float64 tensors keyed by GRAD_FIELDS.
+        Returns (dict of gradients, AdjointStats)."""
+        kernel = kernel or kernel_spec()
+        P = _particles(parts)
+        nx, ny, nz = (int(v) for v in res)
+        G = Grid(*[float(v) for v in domain], nx, ny, nz)
+        gp = [_ptr(a) for a in grad_cells]
+        if len({q[1] for q in gp}) != 1:
+            raise ValueError("cell gradients must all be host arrays or all device tensors")
+        Gc = CellSums(*[q[0] for q in gp], gp[0][1])
+        if out is None:
+            grads = {f: np.zeros(P.n) for f in GRAD_FIELDS}
+            O = ParticleGrads(*[grads[f].ctypes.data_as(_pd) for f in GRAD_FIELDS], 0)
+        else:
+            grads = dict(out)
+            O = ParticleGrads(*[_ptr(grads[f])[0] for f in GRAD_FIELDS], 1)
+        st = AdjointStats()
+        self._check(self.L.kiwi_hip_sph_to_grid_adjoint(self.h, ctypes.byref(P), ctypes.byref(G),
+                                                        ctypes.byref(kernel), int(precision), ctypes.byref(Gc),
+                                                        ctypes.byref(O), ctypes.byref(st)))
+        return grads, st
 
     def voronoi_map(self, parts, home, cells, kernel=None, precision=FP64, out=None):
         """The Voronoi map's per-cell sums.  parts: as sph_to_grid; home: int32 home cell per particle (-1:

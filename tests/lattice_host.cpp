@@ -161,3 +161,26 @@ double lat_face_walk(int kind, int kid, double sup, const double* p, double h, c
     return m;
 }
 }  // extern "C"
+
+// ---- the adjoint's lattice gradient (m4_adjoint.h) ----
+#include "../kiwi_amd/csrc/m4_adjoint.h"
+
+extern "C" {
+// W, dW/dax, dW/day, dW/daz at absolute offsets (units of h), in double (single = 0) or float
+void lat_adj_point(double ax, double ay, double az, int single, double* out) {
+    if (single) {
+        const float fx = static_cast<float>(ax), fy = static_cast<float>(ay), fz = static_cast<float>(az);
+        const AdjLine<float> lxy = m4_adj_line<float>(fx, fy), lxz = m4_adj_line<float>(fx, fz),
+                             lyz = m4_adj_line<float>(fy, fz);
+        float W, Wx, Wy, Wz;
+        m4_adj_point<float>(fx, fy, fz, lxy, lxz, lyz, W, Wx, Wy, Wz);
+        out[0] = W;
+        out[1] = Wx;
+        out[2] = Wy;
+        out[3] = Wz;
+        return;
+    }
+    const AdjLine<double> lxy = m4_adj_line(ax, ay), lxz = m4_adj_line(ax, az), lyz = m4_adj_line(ay, az);
+    m4_adj_point<double>(ax, ay, az, lxy, lxz, lyz, out[0], out[1], out[2], out[3]);
+}
+}  // extern "C"

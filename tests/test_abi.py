@@ -53,7 +53,9 @@ def test_struct_layouts_match_header(tmp_path):
               "kiwi_cell_sums": [f for f, _ in hip.CellSums._fields_],
               "kiwi_stats": [f for f, _ in hip.Stats._fields_],
               "kiwi_voronoi_cells": [f for f, _ in hip.VoronoiCells._fields_],
-              "kiwi_map_stats": [f for f, _ in hip.MapStats._fields_]}
+              "kiwi_map_stats": [f for f, _ in hip.MapStats._fields_],
+              "kiwi_particle_grads": [f for f, _ in hip.ParticleGrads._fields_],
+              "kiwi_adjoint_stats": [f for f, _ in hip.AdjointStats._fields_]}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for st, fs in fields.items():
         lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
@@ -67,7 +69,8 @@ def test_struct_layouts_match_header(tmp_path):
                                                           check=True).stdout.split("\n") if l)
     for st, cls in (("kiwi_particles", hip.Particles), ("kiwi_grid", hip.Grid), ("kiwi_kernel", hip.Kernel),
                     ("kiwi_cell_sums", hip.CellSums), ("kiwi_stats", hip.Stats),
-                    ("kiwi_voronoi_cells", hip.VoronoiCells), ("kiwi_map_stats", hip.MapStats)):
+                    ("kiwi_voronoi_cells", hip.VoronoiCells), ("kiwi_map_stats", hip.MapStats),
+                    ("kiwi_particle_grads", hip.ParticleGrads), ("kiwi_adjoint_stats", hip.AdjointStats)):
         assert int(out[st]) == ctypes.sizeof(cls), st
         for f, _ in cls._fields_:
             assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, (st, f)

@@ -1,0 +1,213 @@
+"""GPU checks of kiwi_hip_sph_to_grid_adjoint (no reference counterpart): the gradients of a loss of the
+per-cell sums with respect to every particle's m, x, y, z, h, vx, vy, vz.
+
+Pinned by:
+  - central finite differences of the FP64 forward (kiwi_hip_sph_to_grid itself) on random particles
+    of the C1 workload, relative 1e-5;
+  - exact invariants of the forward: on a domain padded by the largest support every particle's cells
+    hold exactly its mass, so for L = total mass dL/dm = 1 and dL/dx = dL/dh = 0, and for L = total
+    x-momentum dL/dvx = m and dL/dm = vx;
+  - the single-precision lattice (KIWI_MIXED) against FP64: 2e-4 of the largest value for the mass and
+    velocity gradients, 3e-2 for the position and h gradients (FP32 rounding, see the test);
+  - tests/test_lattice_math.py checks the lattice gradient itself against quadrature of the face flux.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+from kiwi_amd import hip, workloads  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+COLS = ("x", "y", "z", "vx", "vy", "vz", "m", "h")
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def c1():
+    parts, dom, res = workloads.uniform_cube()
+    return {k: np.ascontiguousarray(parts[k], dtype=np.float64) for k in COLS}, dom, res
+
+
+class Loss:
+    """L = sum_c w_c (mass_c / V - t_c)^2 + sum_c u_c . p_c  and its gradient with respect to the sums.  The
+    momentum weights are scaled so that both terms give gradients of comparable size."""
+
+    def __init__(self, ctx, parts, dom, res, seed=3):
+        rng = np.random.default_rng(seed)
+        self.ctx, self.dom, self.res = ctx, dom, res
+        self.V = np.prod((dom[1::2] - dom[0::2]) / np.array(res))
+        pert = dict(parts)
+        pert["m"] = parts["m"] * (1 + 0.05 * rng.standard_normal(len(parts["m"])))
+        self.t = ctx.sph_to_grid(pert, dom, res)[0] / self.V
+        nc = int(np.prod(res))
+        self.w = rng.uniform(0.5, 1.5, nc)
+        g_mass = 2 * np.abs(self.t).max() * 0.05 / self.V
+        vmax = max(np.abs(parts[c]).max() for c in ("vx", "vy", "vz"))
+        self.u = [rng.standard_normal(nc) * g_mass / vmax for _ in range(3)]
+
+    def terms(self, parts):
+        """per-cell terms of L (a finite difference of L sums the per-cell differences, so cells the
+        perturbed particle does not reach cancel exactly)"""
+        mass, px, py, pz, _ = self.ctx.sph_to_grid(parts, self.dom, self.res)
+        r = mass / self.V - self.t
+        return self.w * r * r + self.u[0] * px + self.u[1] * py + self.u[2] * pz
+
+    def cell_grads(self, parts):
+        mass = self.ctx.sph_to_grid(parts, self.dom, self.res)[0]
+        return (2 * self.w * (mass / self.V - self.t) / self.V, self.u[0], self.u[1], self.u[2])
+
+
+def test_adjoint_matches_finite_differences(hip_ctx):
+    say("case: C1 2000 particles -> 32^3, FP64 adjoint vs central differences on 40 particles")
+    parts, dom, res = c1()
+    L = Loss(hip_ctx, parts, dom, res)
+    g, st = hip_ctx.sph_to_grid_adjoint(parts, dom, res, L.cell_grads(parts))
+    assert st.skipped_outside == 0 and st.cells > 0
+    rng = np.random.default_rng(7)
+    idx = rng.choice(len(parts["x"]), 40, replace=False)
+    for f in ("m", "x", "y", "z", "h", "vx"):
+        scale = np.sqrt(np.mean(g[f] ** 2))
+        worst = 0.0
+        for i in idx:
+            base = parts[f][i]
+            eps = 1e-6 * abs(base) if f in ("m",) else (1e-4 * parts["h"][i] if f in ("x", "y", "z", "h") else 1e-4)
+            lp, lm = dict(parts), dict(parts)
+            lp[f] = parts[f].copy()
+            lm[f] = parts[f].copy()
+            lp[f][i] = base + eps
+            lm[f][i] = base - eps
+            fd = np.sum(L.terms(lp) - L.terms(lm)) / (2 * eps)
+            err = abs(g[f][i] - fd) / max(abs(fd), 1e-2 * scale)
+            worst = max(worst, err)
+        say(f"  d/d{f}: worst relative error {worst:.2e} (rms gradient {scale:.3e})")
+        assert worst < 1e-5, f
+
+
+def test_adjoint_invariants(hip_ctx):
+    say("case: C1, L = total mass and L = total x-momentum")
+    parts, dom, res = c1()
+    nc = int(np.prod(res))
+    one, zero = np.ones(nc), np.zeros(nc)
+    g, _ = hip_ctx.sph_to_grid_adjoint(parts, dom, res, (one, zero, zero, zero))
+    np.testing.assert_allclose(g["m"], 1.0, rtol=0, atol=1e-12)
+    m_over_h = parts["m"] / parts["h"]
+    for f in ("x", "y", "z", "h"):
+        assert np.max(np.abs(g[f]) / m_over_h) < 1e-11, f
+    for f in ("vx", "vy", "vz"):
+        assert np.max(np.abs(g[f])) == 0.0
+    g, _ = hip_ctx.sph_to_grid_adjoint(parts, dom, res, (zero, one, zero, zero))
+    np.testing.assert_allclose(g["vx"], parts["m"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(g["m"], parts["vx"], rtol=0, atol=1e-12 * np.abs(parts["vx"]).max())
+    for f in ("x", "y", "z", "h"):
+        assert np.max(np.abs(g[f]) / (m_over_h * np.abs(parts["vx"]).max())) < 1e-11, f
+
+
+def test_adjoint_mixed_close_to_fp64(hip_ctx):
+    say("case: C1, KIWI_MIXED adjoint vs FP64 adjoint")
+    parts, dom, res = c1()
+    L = Loss(hip_ctx, parts, dom, res)
+    gc = L.cell_grads(parts)
+    g64, _ = hip_ctx.sph_to_grid_adjoint(parts, dom, res, gc)
+    g32, _ = hip_ctx.sph_to_grid_adjoint(parts, dom, res, gc, precision=hip.MIXED)
+    errs = {}
+    for f in hip.GRAD_FIELDS:
+        scale = np.abs(g64[f]).max()
+        errs[f] = np.max(np.abs(g32[f] - g64[f])) / scale
+        say(f"  d/d{f}: max |mixed - fp64| / max|fp64| = {errs[f]:.2e}")
+    # FP32 lattice values and gradients carry ~1e-6 absolute rounding per lattice point (the closed form
+    # cancels internally at radii ~h-2h).  Mass and velocity gradients sum the values themselves and
+    # stay within ~4e-5 of their maximum; position and h gradients are small differences of lattice
+    # gradients and come to ~1e-2.  The FP64 adjoint is the accurate mode (finite differences above).
+    for f in ("m", "vx", "vy", "vz"):
+        assert errs[f] < 2e-4, (f, errs[f])
+    for f in ("x", "y", "z", "h"):
+        assert errs[f] < 3e-2, (f, errs[f])
+
+
+def test_adjoint_deterministic_and_device_outputs(hip_ctx):
+    import torch
+    say("case: C1 adjoint twice (bit-identical), host vs device buffers")
+    parts, dom, res = c1()
+    L = Loss(hip_ctx, parts, dom, res)
+    gc = L.cell_grads(parts)
+    a, _ = hip_ctx.sph_to_grid_adjoint(parts, dom, res, gc)
+    b, _ = hip_ctx.sph_to_grid_adjoint(parts, dom, res, gc)
+    for f in hip.GRAD_FIELDS:
+        assert np.array_equal(a[f], b[f]), f
+    dev = torch.device("cuda:0")
+    dparts = {k: torch.from_numpy(v).to(dev) for k, v in parts.items()}
+    dgc = tuple(torch.from_numpy(np.ascontiguousarray(v)).to(dev) for v in gc)
+    out = {f: torch.zeros(len(parts["x"]), dtype=torch.float64, device=dev) for f in hip.GRAD_FIELDS}
+    hip_ctx.sph_to_grid_adjoint(dparts, dom, res, dgc, out=out)
+    torch.cuda.synchronize()
+    for f in hip.GRAD_FIELDS:
+        assert np.array_equal(a[f], out[f].cpu().numpy()), f
+
+
+def test_autograd_function_and_edge_cases(hip_ctx):
+    import torch
+    from kiwi_amd.autograd import sph_to_grid
+    say("case: torch.autograd.Function backward == direct adjoint; skipped and h <= 0 particles; kernels")
+    parts, dom, res = c1()
+    parts["h"][:3] = 0.0
+    parts["x"][3:6] = dom[1] + 5.0  # support box misses the grid
+    L = Loss(hip_ctx, parts, dom, res)
+    dev = torch.device("cuda:0")
+    t = {k: torch.from_numpy(v).to(dev).requires_grad_(True) for k, v in parts.items()}
+    mass, px, py, pz = sph_to_grid(t["m"], t["x"], t["y"], t["z"], t["h"], t["vx"], t["vy"], t["vz"], hip_ctx, dom,
+                                   res)
+    tt = torch.from_numpy(L.t).to(dev)
+    w = torch.from_numpy(L.w).to(dev)
+    u = [torch.from_numpy(v).to(dev) for v in L.u]
+    loss = torch.sum(w * (mass / L.V - tt) ** 2) + torch.sum(u[0] * px) + torch.sum(u[1] * py) + torch.sum(u[2] * pz)
+    loss.backward()
+    g, st = hip_ctx.sph_to_grid_adjoint(parts, dom, res, L.cell_grads(parts))
+    assert st.skipped_outside == 3
+    for f in hip.GRAD_FIELDS:
+        np.testing.assert_allclose(t[f].grad.cpu().numpy(), g[f], rtol=1e-12, atol=1e-14 * np.abs(g[f]).max())
+        assert np.all(g[f][:6] == 0.0)
+    for kern in (hip.kernel_spec("WC2"), hip.kernel_spec("M4", tabulated=True)):
+        with pytest.raises(hip.KiwiError):
+            hip_ctx.sph_to_grid_adjoint(parts, dom, res, L.cell_grads(parts), kernel=kern)
+
+
+def test_mass_recovery_by_gradient_descent(hip_ctx):
+    """A small version of the fit: recover 5%-perturbed particle masses from the target density by
+    gradient descent (Adam) on L = sum (rho - rho_target)^2 through the adjoint."""
+    import torch
+    from kiwi_amd.autograd import sph_to_grid
+    say("case: 20000 log-normal particles -> 48^3, 60 gradient steps on the masses")
+    parts, dom, res = workloads.lognormal_field(n_particles=20000, n=48, seed=5)
+    parts = {k: np.ascontiguousarray(parts[k], dtype=np.float64) for k in COLS}
+    rng = np.random.default_rng(11)
+    m_true = parts["m"].copy()
+    dev = torch.device("cuda:0")
+    t = {k: torch.from_numpy(v).to(dev) for k, v in parts.items()}
+    V = float(np.prod((dom[1::2] - dom[0::2]) / np.array(res)))
+    target = sph_to_grid(t["m"], t["x"], t["y"], t["z"], t["h"], t["vx"], t["vy"], t["vz"], hip_ctx, dom, res)[0] / V
+    m = torch.from_numpy(m_true * (1 + 0.05 * rng.standard_normal(len(m_true)))).to(dev).requires_grad_(True)
+    opt = torch.optim.Adam([m], lr=0.01 * float(m_true.mean()))
+    first = None
+    for step in range(60):
+        opt.zero_grad()
+        rho = sph_to_grid(m, t["x"], t["y"], t["z"], t["h"], t["vx"], t["vy"], t["vz"], hip_ctx, dom, res)[0] / V
+        loss = torch.sum((rho - target) ** 2)
+        loss.backward()
+        opt.step()
+        first = first if first is not None else float(loss.detach())
+    final = float(loss.detach())
+    err0 = 0.05
+    err = float(torch.sqrt(torch.mean(((m.detach().cpu() - torch.from_numpy(m_true)) / torch.from_numpy(m_true)) ** 2)))
+    say(f"  loss {first:.3e} -> {final:.3e}; rms relative mass error 0.05 -> {err:.4f}")
+    # the loss falls by > 100x; masses that the density field cannot separate (particles sharing
+    # cells) stay partly unresolved, so the mass error falls by less
+    assert final < 0.01 * first
+    assert err < 0.7 * err0
