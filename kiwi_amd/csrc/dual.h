@@ -108,9 +108,11 @@ struct KMath<Dual<T, N>> {
         return chain(x, i, -i * i);
     }
     static KIWI_HD D div(const D& a, const D& b) { return a * rcp(b); }
+    // at 0 the derivative is taken as 0 rather than infinity (m4_line keeps its square roots away from
+    // 0; this only guards against NaN from 0 * inf)
     static KIWI_HD D sqrt_(const D& x) {
         const T s = M::sqrt_(x.v);
-        return chain(x, s, T(0.5) * M::rcp(s));
+        return chain(x, s, s > T(0) ? T(0.5) * M::rcp(s) : T(0));
     }
     static KIWI_HD D log_(const D& x) { return chain(x, M::log_(x.v), M::rcp(x.v)); }
     static KIWI_HD D atan_pos(const D& x) { return chain(x, M::atan_pos(x.v), M::rcp(T(1) + x.v * x.v)); }

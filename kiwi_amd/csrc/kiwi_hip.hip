@@ -242,7 +242,7 @@ struct PolicyM4 {
     // line-table slot of region k, -1: none
     static __device__ __forceinline__ int line_slot(int k) { return k - 2; }
     static __device__ __forceinline__ void line(const Ctx&, R da, R db, R* v) { m4_line<R>(da, db, v[0], v[1]); }
-    static __device__ __forceinline__ int line_kind(double l2) { return l2 <= 1.0 ? 2 : (l2 <= 4.0 ? 1 : 0); }
+    static __device__ __forceinline__ int line_kind(double l2) { return l2 < 1.0 ? 2 : (l2 < 4.0 ? 1 : 0); }
     // the lattice value without its plane and line terms, and the weight of the plane sum
     static __device__ __forceinline__ R free_part(const Ctx&, R ax, R ay, R az, R r, int k) {
         return m4_lattice_free<R>(ax, ay, az, r, k) * M4Consts<R>::inv_pi;

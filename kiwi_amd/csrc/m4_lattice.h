@@ -184,7 +184,9 @@ KIWI_HD void m4_line(R da, R db, R& D2, R& D3) {
     D3 = R(0);
     if (!(da > R(0)) || !(db > R(0))) return;
     const R l2 = da * da + db * db;
-    if (!(l2 <= R(4))) return;
+    // on the boundaries l2 = 4 and l2 = 1 the crossing vertex is the foot itself and the offset is exactly
+    // 0 (and so is its derivative, which the adjoint carries through here): take the zero branch there
+    if (!(l2 < R(4))) return;
     const M4Plane<R> pl = m4_plane(da);
     const R inv_l2 = M::rcp(l2);
     const R inv_db = M::rcp(db);
@@ -192,7 +194,7 @@ KIWI_HD void m4_line(R da, R db, R& D2, R& D3) {
     //   pi (g1 - g2) = -p/6 - 0.15 q4 + X5/30 + X3/4 - T/60 + (c1 - c2) phi      (r = 1)
     //   pi (g2 - g3) =  p/3 + 0.075 q4 - X5/120 - X3/4 + T/60 + T/4 + (c2 - c3) phi   (r = 2)
     const R da2 = da * da, db2 = db * db, dab = da * db;
-    if (l2 <= R(1)) {
+    if (l2 < R(1)) {
         const R dc = M::sqrt_(R(1) - l2);
         const R T = M::atan_pos(da * dc * inv_db);
         const R L = M::log_((R(1) + dc) * (R(1) + dc) * inv_l2);
@@ -225,9 +227,9 @@ KIWI_HD void m4_line_by_vertex(R da, R db, R& D2, R& D3) {
     D3 = R(0);
     if (!(da > R(0)) || !(db > R(0))) return;
     const R l2 = da * da + db * db;
-    if (!(l2 <= R(4))) return;
+    if (!(l2 < R(4))) return;
     const M4Plane<R> pl = m4_plane(da);
-    if (l2 <= R(1)) {
+    if (l2 < R(1)) {
         const R dc = M::sqrt_(R(1) - l2);
         const R T = M::atan_pos(M::div(da * dc, db));
         const R L = M::log_(M::div((R(1) + dc) * (R(1) + dc), l2));

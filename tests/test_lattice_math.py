@@ -305,3 +305,33 @@ def test_adjoint_lattice_gradient(lat):
             assert abs(_face_flux(a, b, c) - g[1]) < 1e-11
             assert abs(_face_flux(c, a, b) - g[3]) < 1e-11
         assert np.max(np.abs(adj(a, b, c, 1) - g)) < 5e-6
+
+
+def test_adjoint_lattice_gradient_at_region_crossings(lat):
+    """Lattice points whose edge line sits exactly on the region boundaries l^2 = 1 and l^2 = 4 (where
+    the line constants take square roots of 0): the gradient stays finite and equals central differences
+    in double; in single precision no point near the boundaries gives a non-finite gradient."""
+    lat.lat_adj_point.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, _pd]
+    lat.lat_U.restype = ctypes.c_double
+    lat.lat_U.argtypes = [ctypes.c_double] * 3
+
+    def adj(a, b, c, single=0):
+        o = (ctypes.c_double * 4)()
+        lat.lat_adj_point(a, b, c, single, o)
+        return np.array(o[:])
+
+    eps = 1e-6
+    for a, b in ((0.6, 0.8), (0.8, 0.6), (1.2, 1.6), (0.0 + 1.0, 1e-300 + 0.0 + 1.7320508075688772)):
+        for c in (0.3, 0.9, 1.7):
+            g = adj(a, b, c)
+            assert np.all(np.isfinite(g)), (a, b, c)
+            fd = [(lat.lat_U(a + eps, b, c) - lat.lat_U(a - eps, b, c)) / (2 * eps),
+                  (lat.lat_U(a, b + eps, c) - lat.lat_U(a, b - eps, c)) / (2 * eps),
+                  (lat.lat_U(a, b, c + eps) - lat.lat_U(a, b, c - eps)) / (2 * eps)]
+            assert np.max(np.abs(np.array(fd) - g[1:])) < 1e-7, (a, b, c, g, fd)
+    rng = np.random.default_rng(9)
+    for _ in range(4000):
+        th = rng.uniform(0, np.pi / 2)
+        rr = rng.choice([1.0, 2.0])
+        a, b = np.float32(rr * np.cos(th)), np.float32(rr * np.sin(th))
+        assert np.all(np.isfinite(adj(float(a), float(b), rng.uniform(0.01, 2.5), 1)))
