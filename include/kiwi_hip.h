@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KIWI_HIP_ABI_VERSION 3
+#define KIWI_HIP_ABI_VERSION 4
 
 #define KIWI_OK 0
 #define KIWI_ERR_INVALID_ARG (-1)
@@ -156,6 +156,26 @@ void* kiwi_hip_stream(kiwi_hip_ctx* ctx);
 int kiwi_hip_sph_to_grid(kiwi_hip_ctx* ctx, const kiwi_particles* particles, const kiwi_grid* grid,
                          const kiwi_kernel* kernel, int precision, const kiwi_cell_sums* out,
                          kiwi_stats* stats);
+
+/* One problem of a batch: the arguments of kiwi_hip_sph_to_grid. */
+typedef struct kiwi_batch_problem {
+    kiwi_particles particles;
+    kiwi_grid grid;
+    kiwi_kernel kernel;
+    int precision;
+    kiwi_cell_sums out;
+    kiwi_stats* stats; /* may be NULL */
+} kiwi_batch_problem;
+
+#define KIWI_MAX_BATCH 4096
+
+/* Many independent sph-to-grid problems in one call (no reference counterpart; the reference runs one
+ * cutout per process).  Problems that share a kernel path (kernel, support, arithmetic) run as one
+ * pipeline: every stage is launched once for all of them, and the cell integrals of all their
+ * (particle, cell block) items share one persistent launch.  Each problem's outputs and statistics are
+ * those kiwi_hip_sph_to_grid would return for it alone (device times cover its group's run;
+ * particle_blocks is -1 when the group holds more than one problem). */
+int kiwi_hip_sph_to_grid_batch(kiwi_hip_ctx* ctx, int n_problems, const kiwi_batch_problem* problems);
 
 /* Per-particle gradients, n each; on_device selects device or host pointers. */
 typedef struct kiwi_particle_grads {

@@ -38,6 +38,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "common.h"
 #include "gen_lattice.h"
@@ -52,23 +53,57 @@ constexpr int BLK_PTS = (BLK + 1) * (BLK + 1) * (BLK + 1);
 constexpr int MAX_AXIS = 8192;                // grid cells per axis
 constexpr unsigned ITEM_GRAB = 8;             // items a wavefront takes from the cursor at once
 
-struct DevStats {
+struct DevStats {  // one per problem
     unsigned long long skipped_outside, n_negative, n_negative_warn, pairs_bbox, pairs_sphere;
     unsigned long long n_warn;  // number of warn records claimed
     unsigned long long pts[3], lines[3], planes;
-    unsigned long long cursor;  // next unclaimed item of k_particles
     kiwi_warn_record warn[KIWI_MAX_WARN_RECORDS];
 };
+
+// One problem of a call (a single sph-to-grid call is a batch of one).  Particles of all problems are
+// concatenated, and so are their cell blocks: problem q owns particles [part_off, part_off + n_parts)
+// and blocks [block_off, block_off + nbx nby nbz).
+struct Prob {
+    GridGeom g;
+    int64_t part_off, n_parts;
+    int64_t block_off;
+    double* out[4];  // device outputs: mass, px, py, pz
+};
+
+// the problem that owns global block b (nprob is small; the search is wave-uniform)
+__device__ __forceinline__ int block_problem(const Prob* __restrict__ pr, int nprob, int64_t b) {
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pr[mid].block_off <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// add a counter of each lane to its problem's slot: one atomic per wave when the wave's lanes share a
+// problem (the common case: particles are grouped by problem), per lane otherwise
+__device__ __forceinline__ void add_by_problem(unsigned long long v, int q, DevStats* st,
+                                               unsigned long long DevStats::*field) {
+    const int q0 = __shfl(q, 0);
+    if (__all(q == q0)) {
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&(st[q0].*field), v);
+    } else if (v && q >= 0) {
+        atomicAdd(&(st[q].*field), v);
+    }
+}
 
 // ------------------------------------------------------------------------------------------------
 // 1. per-particle bounding box and work sizes
 // ------------------------------------------------------------------------------------------------
 __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                        const double* __restrict__ z, const double* __restrict__ h, double support_in_h,
-                       GridGeom g, int* __restrict__ bbox, int64_t* __restrict__ nblk, int64_t* __restrict__ nlines,
-                       DevStats* st) {
+                       const Prob* __restrict__ pr, const int* __restrict__ part_prob, int* __restrict__ bbox,
+                       int64_t* __restrict__ nblk, int64_t* __restrict__ nlines, DevStats* st) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     unsigned long long skipped = 0, pairs = 0;
+    const int q = (i < n) ? part_prob[i] : -1;
     if (i == n) {  // trailing zeros of the exclusive scans
         nblk[n] = 0;
         nlines[n] = 0;
@@ -81,12 +116,12 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
         const double hh = h[i];
         const double x0 = x[i], y0 = y[i], z0 = z[i];
         if (hh > 0.0) {
-            int q[6];
-            if (!particle_bbox(g, support_in_h, x0, y0, z0, hh, q)) {
+            int qb[6];
+            if (!particle_bbox(pr[q].g, support_in_h, x0, y0, z0, hh, qb)) {
                 skipped = 1;
             } else {
-                const int ix0 = q[0], ix1 = q[1], iy0 = q[2], iy1 = q[3], iz0 = q[4], iz1 = q[5];
-                for (int k = 0; k < 6; ++k) bb[k] = q[k];
+                const int ix0 = qb[0], ix1 = qb[1], iy0 = qb[2], iy1 = qb[3], iz0 = qb[4], iz1 = qb[5];
+                for (int k = 0; k < 6; ++k) bb[k] = qb[k];
                 const int64_t Nx = ix1 - ix0 + 1, Ny = iy1 - iy0 + 1, Nz = iz1 - iz0 + 1;
                 pairs = static_cast<unsigned long long>(Nx * Ny * Nz);
                 nb = static_cast<int64_t>(ix1 / BLK - ix0 / BLK + 1) * (iy1 / BLK - iy0 / BLK + 1) *
@@ -97,30 +132,26 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
         nblk[i] = nb;
         nlines[i] = nl;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        skipped += __shfl_down(skipped, off);
-        pairs += __shfl_down(pairs, off);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        if (skipped) atomicAdd(&st->skipped_outside, skipped);
-        if (pairs) atomicAdd(&st->pairs_bbox, pairs);
-    }
+    add_by_problem(skipped, q, st, &DevStats::skipped_outside);
+    add_by_problem(pairs, q, st, &DevStats::pairs_bbox);
 }
 
 // ------------------------------------------------------------------------------------------------
 // 3. items: (particle, cell block) keys in particle order
 // ------------------------------------------------------------------------------------------------
-__global__ void k_emit(int64_t n, const int* __restrict__ bbox, const int64_t* __restrict__ offs, GridGeom g,
-                       uint64_t* __restrict__ keys) {
+__global__ void k_emit(int64_t n, const int* __restrict__ bbox, const int64_t* __restrict__ offs,
+                       const Prob* __restrict__ pr, const int* __restrict__ part_prob, uint64_t* __restrict__ keys) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int* bb = bbox + 6 * i;
     if (bb[0] > bb[1]) return;
+    const Prob& P = pr[part_prob[i]];
+    const GridGeom& g = P.g;
     int64_t o = offs[i];
     for (int bx = bb[0] / BLK; bx <= bb[1] / BLK; ++bx)
         for (int by = bb[2] / BLK; by <= bb[3] / BLK; ++by)
             for (int bz = bb[4] / BLK; bz <= bb[5] / BLK; ++bz) {
-                const uint64_t b = (static_cast<uint64_t>(bx) * g.nby + by) * g.nbz + bz;
+                const uint64_t b = static_cast<uint64_t>(P.block_off) + (static_cast<uint64_t>(bx) * g.nby + by) * g.nbz + bz;
                 keys[o++] = (b << 32) | static_cast<uint64_t>(i);
             }
 }
@@ -151,12 +182,15 @@ __device__ __forceinline__ ItemBox item_box(const int* bb, uint32_t b, const Gri
 
 // cells of every item in block order (the sizes whose prefix sum gives the slab offsets)
 __global__ void k_item_cells(int64_t npairs, const uint64_t* __restrict__ keys, const int* __restrict__ bbox,
-                             GridGeom g, int64_t* __restrict__ cells) {
+                             const Prob* __restrict__ pr, const int* __restrict__ part_prob,
+                             int64_t* __restrict__ cells) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i == npairs) cells[npairs] = 0;
     if (i >= npairs) return;
     const uint64_t key = keys[i];
-    const ItemBox r = item_box(bbox + 6 * (key & 0xffffffffull), static_cast<uint32_t>(key >> 32), g);
+    const uint32_t p = static_cast<uint32_t>(key & 0xffffffffull);
+    const Prob& P = pr[part_prob[p]];
+    const ItemBox r = item_box(bbox + 6 * p, static_cast<uint32_t>((key >> 32) - P.block_off), P.g);
     cells[i] = static_cast<int64_t>(r.mx * r.my * r.mz);
 }
 
@@ -361,17 +395,39 @@ template <class P>
 __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                                                    const double* __restrict__ z, const double* __restrict__ h,
                                                    const int* __restrict__ bbox, const int64_t* __restrict__ loffs,
-                                                   GridGeom g, typename P::Ctx ctx,
+                                                   const Prob* __restrict__ pr, const int* __restrict__ part_prob,
+                                                   typename P::Ctx ctx,
                                                    LineVals<typename P::Real, P::NLN>* __restrict__ table,
                                                    DevStats* st) {
     using R = typename P::Real;
-    // grid-stride over particles, one wavefront per particle at a time; counters stay in registers
+    // grid-stride over particles, one wavefront per particle at a time; counters stay in registers and
+    // go to the problem's statistics when the wavefront moves on to another problem
     const int lane = threadIdx.x & 63;
     const int64_t nwaves = static_cast<int64_t>(gridDim.x) * WAVES;
     unsigned long long cnt0 = 0, cnt1 = 0, cnt2 = 0;
+    int cur_q = -1;
+    auto flush = [&]() {
+        for (int off = 32; off > 0; off >>= 1) {
+            cnt0 += __shfl_down(cnt0, off);
+            cnt1 += __shfl_down(cnt1, off);
+            cnt2 += __shfl_down(cnt2, off);
+        }
+        if (lane == 0 && cur_q >= 0) {
+            if (cnt0) atomicAdd(&st[cur_q].lines[0], cnt0);
+            if (cnt1) atomicAdd(&st[cur_q].lines[1], cnt1);
+            if (cnt2) atomicAdd(&st[cur_q].lines[2], cnt2);
+        }
+        cnt0 = cnt1 = cnt2 = 0;
+    };
     for (int64_t p = static_cast<int64_t>(blockIdx.x) * WAVES + wave_uniform(threadIdx.x >> 6); p < n; p += nwaves) {
         const int* bb = bbox + 6 * p;
         if (bb[0] > bb[1]) continue;
+        const int q = static_cast<int>(wave_uniform(static_cast<uint32_t>(part_prob[p])));
+        if (q != cur_q) {
+            flush();
+            cur_q = q;
+        }
+        const GridGeom& g = pr[q].g;
         const int L0 = bb[1] - bb[0] + 2, L1 = bb[3] - bb[2] + 2, L2 = bb[5] - bb[4] + 2;
         const int c01 = L0 * L1, c02 = L0 * L2, c12 = L1 * L2;
         const int nl = c01 + c02 + c12;
@@ -407,20 +463,11 @@ __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __re
             cnt0 += (kind == 0) ? 2 : 0;
             LineVals<R, P::NLN> e;
 #pragma unroll
-            for (int q = 0; q < P::NLN; ++q) e.v[q] = vab[q] + vba[q];
+            for (int k = 0; k < P::NLN; ++k) e.v[k] = vab[k] + vba[k];
             out[t] = e;
         }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        cnt0 += __shfl_down(cnt0, off);
-        cnt1 += __shfl_down(cnt1, off);
-        cnt2 += __shfl_down(cnt2, off);
-    }
-    if (lane == 0) {
-        if (cnt0) atomicAdd(&st->lines[0], cnt0);
-        if (cnt1) atomicAdd(&st->lines[1], cnt1);
-        if (cnt2) atomicAdd(&st->lines[2], cnt2);
-    }
+    flush();
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -475,9 +522,10 @@ template <class P>
 __global__ __launch_bounds__(THREADS) void k_particles(
     uint32_t npairs, const uint64_t* __restrict__ keys, const int64_t* __restrict__ slab,
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    const double* __restrict__ h, const int* __restrict__ bbox, const int64_t* __restrict__ loffs, GridGeom g,
-    typename P::Ctx ctx, const LineVals<typename P::Real, P::NLN>* __restrict__ lines, typename P::Real* __restrict__ ms,
-    DevStats* st, int clamp_pairs) {
+    const double* __restrict__ h, const int* __restrict__ bbox, const int64_t* __restrict__ loffs,
+    const Prob* __restrict__ pr, const int* __restrict__ part_prob, typename P::Ctx ctx,
+    const LineVals<typename P::Real, P::NLN>* __restrict__ lines, typename P::Real* __restrict__ ms, DevStats* st,
+    unsigned long long* cursor, int clamp_pairs) {
     using R = typename P::Real;
     constexpr int NLN = P::NLN;
     __shared__ WaveLds<R, P::NPL> WL[WAVES];
@@ -488,19 +536,47 @@ __global__ __launch_bounds__(THREADS) void k_particles(
     unsigned long long n_neg = 0, n_warn = 0, n_sphere = 0, c_planes = 0;
     unsigned long long c_pts[3] = {0, 0, 0};
     const unsigned long long lane_lt = (1ull << lane) - 1ull;
+    // counters go to the problem's statistics when the wavefront moves on to another problem
+    int cur_q = -1;
+    auto flush = [&]() {
+        for (int off = 32; off > 0; off >>= 1) {
+            n_neg += __shfl_down(n_neg, off);
+            n_warn += __shfl_down(n_warn, off);
+            n_sphere += __shfl_down(n_sphere, off);
+            c_planes += __shfl_down(c_planes, off);
+            for (int r = 0; r < 3; ++r) c_pts[r] += __shfl_down(c_pts[r], off);
+        }
+        if (lane == 0 && cur_q >= 0) {
+            DevStats* sq = st + cur_q;
+            if (n_neg) atomicAdd(&sq->n_negative, n_neg);
+            if (n_warn) atomicAdd(&sq->n_negative_warn, n_warn);
+            if (n_sphere) atomicAdd(&sq->pairs_sphere, n_sphere);
+            if (c_planes) atomicAdd(&sq->planes, c_planes);
+            for (int r = 0; r < 3; ++r)
+                if (c_pts[r]) atomicAdd(&sq->pts[r], c_pts[r]);
+        }
+        n_neg = n_warn = n_sphere = c_planes = 0;
+        c_pts[0] = c_pts[1] = c_pts[2] = 0;
+    };
 
     uint32_t cur = 0;
-    if (lane == 0) cur = static_cast<uint32_t>(atomicAdd(&st->cursor, static_cast<unsigned long long>(ITEM_GRAB)));
+    if (lane == 0) cur = static_cast<uint32_t>(atomicAdd(cursor, static_cast<unsigned long long>(ITEM_GRAB)));
     cur = wave_uniform(cur);
     while (cur < npairs) {
         uint32_t nxt = 0;
-        if (lane == 0) nxt = static_cast<uint32_t>(atomicAdd(&st->cursor, static_cast<unsigned long long>(ITEM_GRAB)));
+        if (lane == 0) nxt = static_cast<uint32_t>(atomicAdd(cursor, static_cast<unsigned long long>(ITEM_GRAB)));
         const uint32_t tend = min(cur + ITEM_GRAB, npairs);
         for (uint32_t t = cur; t < tend; ++t) {
             const uint64_t key = keys[t];
             const uint32_t p = static_cast<uint32_t>(key & 0xffffffffull);
+            const int q = static_cast<int>(wave_uniform(static_cast<uint32_t>(part_prob[p])));
+            if (q != cur_q) {
+                flush();
+                cur_q = q;
+            }
+            const GridGeom& g = pr[q].g;
             const int* bb = bbox + 6 * static_cast<size_t>(p);
-            const ItemBox ib = item_box(bb, static_cast<uint32_t>(key >> 32), g);
+            const ItemBox ib = item_box(bb, static_cast<uint32_t>((key >> 32) - pr[q].block_off), g);
             const int mx = ib.mx, my = ib.my, mz = ib.mz;
             const int nx = mx + 1, ny = my + 1, nz = mz + 1;  // item lattice
             const int Lx = bb[1] - bb[0] + 2, Ly = bb[3] - bb[2] + 2, Lz = bb[5] - bb[4] + 2;  // particle lattice
@@ -617,12 +693,13 @@ __global__ __launch_bounds__(THREADS) void k_particles(
                         ++n_neg;
                         if (Ms <= -0.01) {
                             ++n_warn;
-                            const unsigned long long slot = atomicAdd(&st->n_warn, 1ull);
+                            DevStats* sq = st + cur_q;
+                            const unsigned long long slot = atomicAdd(&sq->n_warn, 1ull);
                             if (slot < KIWI_MAX_WARN_RECORDS) {
-                                st->warn[slot].particle = p;
-                                st->warn[slot].cell =
+                                sq->warn[slot].particle = static_cast<int64_t>(p) - pr[cur_q].part_off;
+                                sq->warn[slot].cell =
                                     (static_cast<int64_t>(ib.x0 + i) * g.ny + (ib.y0 + j)) * g.nz + (ib.z0 + k);
-                                st->warn[slot].value = Ms;
+                                sq->warn[slot].value = Ms;
                             }
                         }
                         Ms = 0.0;
@@ -634,21 +711,7 @@ __global__ __launch_bounds__(THREADS) void k_particles(
         }
         cur = wave_uniform(nxt);
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        n_neg += __shfl_down(n_neg, off);
-        n_warn += __shfl_down(n_warn, off);
-        n_sphere += __shfl_down(n_sphere, off);
-        c_planes += __shfl_down(c_planes, off);
-        for (int r = 0; r < 3; ++r) c_pts[r] += __shfl_down(c_pts[r], off);
-    }
-    if (lane == 0) {
-        if (n_neg) atomicAdd(&st->n_negative, n_neg);
-        if (n_warn) atomicAdd(&st->n_negative_warn, n_warn);
-        if (n_sphere) atomicAdd(&st->pairs_sphere, n_sphere);
-        if (c_planes) atomicAdd(&st->planes, c_planes);
-        for (int r = 0; r < 3; ++r)
-            if (c_pts[r]) atomicAdd(&st->pts[r], c_pts[r]);
-    }
+    flush();
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -668,13 +731,16 @@ __global__ __launch_bounds__(THREADS) void k_accum(
     const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
     const double* __restrict__ m, const int* __restrict__ bbox, const uint64_t* __restrict__ keys,
     const int64_t* __restrict__ slab, const int* __restrict__ blk_begin, const int* __restrict__ blk_end,
-    const int* __restrict__ order, const R* __restrict__ ms, GridGeom g, double* __restrict__ out_mass,
-    double* __restrict__ out_px, double* __restrict__ out_py, double* __restrict__ out_pz, int clamp_pairs) {
+    const int* __restrict__ order, const R* __restrict__ ms, const Prob* __restrict__ pr, int nprob,
+    int clamp_pairs) {
     __shared__ AccLds A;
-    const int b = order[blockIdx.x];
-    const int bz = b % g.nbz;
-    const int by = (b / g.nbz) % g.nby;
-    const int bx = b / (g.nbz * g.nby);
+    const int b = order[blockIdx.x];  // global block
+    const int qb = block_problem(pr, nprob, b);
+    const GridGeom& g = pr[qb].g;
+    const int bl = static_cast<int>(b - pr[qb].block_off);  // block within its problem's grid
+    const int bz = bl % g.nbz;
+    const int by = (bl / g.nbz) % g.nby;
+    const int bx = bl / (g.nbz * g.nby);
     const int tid = threadIdx.x;
     const int wave = static_cast<int>(wave_uniform(threadIdx.x >> 6));
     // owned cells (block-local): x = 2 * wave + s, y = (tid % 64) / 8, z = tid % 8
@@ -692,7 +758,7 @@ __global__ __launch_bounds__(THREADS) void k_accum(
         if (tid < cnt) {
             const uint64_t key = keys[base + tid];
             const int p = static_cast<int>(key & 0xffffffffull);
-            const ItemBox ib = item_box(bbox + 6 * p, static_cast<uint32_t>(b), g);
+            const ItemBox ib = item_box(bbox + 6 * p, static_cast<uint32_t>(bl), g);
             const uint32_t ox = ib.x0 - bx * BLK, oy = ib.y0 - by * BLK, oz = ib.z0 - bz * BLK;
             A.box[tid] = ox | (oy << 4) | (oz << 8) | (static_cast<uint32_t>(ib.mx) << 12) |
                          (static_cast<uint32_t>(ib.my) << 16) | (static_cast<uint32_t>(ib.mz) << 20);
@@ -748,6 +814,10 @@ __global__ __launch_bounds__(THREADS) void k_accum(
     }
 
     const int iy = by * BLK + cy, iz = bz * BLK + cz;
+    double* const out_mass = pr[qb].out[0];
+    double* const out_px = pr[qb].out[1];
+    double* const out_py = pr[qb].out[2];
+    double* const out_pz = pr[qb].out[3];
 #pragma unroll
     for (int s = 0; s < ACC_CPT; ++s) {
         const int ix = bx * BLK + cx0 + s;
@@ -766,6 +836,58 @@ __global__ __launch_bounds__(THREADS) void k_accum(
     }
 }
 
+// the problem of every particle (binary search over the problems' first particles)
+__global__ void k_part_prob(int64_t n, const Prob* __restrict__ pr, int nprob, int* __restrict__ part_prob) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pr[mid].part_off <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    part_prob[i] = lo;
+}
+
+// velocity sums per problem: VSUM_BLOCKS workgroups per problem with fixed strides, then a fixed-order
+// sum of their partials (deterministic)
+__global__ __launch_bounds__(THREADS) void k_vsum_problems(const Prob* __restrict__ pr,
+                                                           const double* __restrict__ vx,
+                                                           const double* __restrict__ vy,
+                                                           const double* __restrict__ vz, double* __restrict__ part) {
+    __shared__ double sh[3][THREADS];
+    const int q = blockIdx.x / VSUM_BLOCKS, j = blockIdx.x % VSUM_BLOCKS;
+    const Prob& P = pr[q];
+    double a = 0, b = 0, c = 0;
+    for (int64_t i = static_cast<int64_t>(j) * THREADS + threadIdx.x; i < P.n_parts;
+         i += static_cast<int64_t>(VSUM_BLOCKS) * THREADS) {
+        a += vx[P.part_off + i];
+        b += vy[P.part_off + i];
+        c += vz[P.part_off + i];
+    }
+    sh[0][threadIdx.x] = a;
+    sh[1][threadIdx.x] = b;
+    sh[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int w = THREADS / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w)
+            for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) part[3 * static_cast<int64_t>(blockIdx.x) + k] = sh[k][0];
+}
+
+__global__ void k_vsum_problems_final(int nprob, const double* __restrict__ part, double* __restrict__ out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nprob) return;
+    for (int k = 0; k < 3; ++k) {
+        double t = 0;
+        for (int j = 0; j < VSUM_BLOCKS; ++j) t += part[3 * (static_cast<int64_t>(q) * VSUM_BLOCKS + j) + k];
+        out[3 * q + k] = t;
+    }
+}
+
 }  // namespace
 
 namespace {
@@ -775,25 +897,27 @@ hipError_t launch_cells(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pc
                         const double* const* dp, const int* bbox, const int64_t* loffs,
                         const uint64_t* keys_particle_order, const int64_t* slab, const uint64_t* keys_block_order,
                         const int64_t* slab_sorted, const int* blk_begin, const int* blk_end, const int* order,
-                        int nblocks, const GridGeom& g, void* lines, void* ms, double* const* om, DevStats* dst,
-                        int clamp, hipEvent_t ev_lines, hipEvent_t ev_cells, hipEvent_t ev_accum) {
+                        int nblocks, const Prob* pr, int nprob, const int* part_prob, void* lines, void* ms,
+                        DevStats* dst, unsigned long long* cursor, int clamp, hipEvent_t ev_lines, hipEvent_t ev_cells,
+                        hipEvent_t ev_accum) {
     using R = typename P::Real;
     using Line = LineVals<R, P::NLN>;
     if (n > 0 && P::HAS_LINES)
         k_lines<P><<<resident_grid(c, k_lines<P>, 1, n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], bbox, loffs,
-                                                                          g, pctx, static_cast<Line*>(lines), dst);
+                                                                          pr, part_prob, pctx,
+                                                                          static_cast<Line*>(lines), dst);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if ((e = hipEventRecord(ev_lines, s)) != hipSuccess) return e;
     if (npairs > 0)
         k_particles<P><<<resident_grid(c, k_particles<P>, ITEM_GRAB, npairs), THREADS, 0, s>>>(
-            static_cast<uint32_t>(npairs), keys_particle_order, slab, dp[0], dp[1], dp[2], dp[7], bbox, loffs, g, pctx,
-            static_cast<const Line*>(lines), static_cast<R*>(ms), dst, clamp);
+            static_cast<uint32_t>(npairs), keys_particle_order, slab, dp[0], dp[1], dp[2], dp[7], bbox, loffs, pr,
+            part_prob, pctx, static_cast<const Line*>(lines), static_cast<R*>(ms), dst, cursor, clamp);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipEventRecord(ev_cells, s)) != hipSuccess) return e;
-    k_accum<R><<<nblocks, THREADS, 0, s>>>(dp[3], dp[4], dp[5], dp[6], bbox, keys_block_order, slab_sorted, blk_begin,
-                                          blk_end, order, static_cast<const R*>(ms), g, om[0], om[1], om[2], om[3],
-                                          clamp);
+    if (nblocks > 0)
+        k_accum<R><<<nblocks, THREADS, 0, s>>>(dp[3], dp[4], dp[5], dp[6], bbox, keys_block_order, slab_sorted,
+                                              blk_begin, blk_end, order, static_cast<const R*>(ms), pr, nprob, clamp);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return hipEventRecord(ev_accum, s);
 }
@@ -814,6 +938,323 @@ hipError_t kiwi_build_m4_table(kiwi_hip_ctx* c, hipStream_t s) {
     c->m4table_ready = true;
     return hipSuccess;
 }
+
+namespace {
+
+struct HostProblem {
+    const kiwi_particles* P;
+    const kiwi_grid* G;
+    const kiwi_cell_sums* out;
+    kiwi_stats* stats;
+};
+
+// the argument rules of one problem (the reference's config and stage checks, src/config.cpp:59-206)
+int check_problem(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_grid* G, const kiwi_cell_sums* out) {
+    if (P->n < 0) return fail(c, KIWI_ERR_INVALID_ARG, "negative particle count");
+    if (P->n > 0 && (!P->x || !P->y || !P->z || !P->vx || !P->vy || !P->vz || !P->m || !P->h))
+        return fail(c, KIWI_ERR_INVALID_ARG, "null particle array");
+    if (!out->mass || !out->px || !out->py || !out->pz) return fail(c, KIWI_ERR_INVALID_ARG, "null output array");
+    if (G->nx <= 0 || G->ny <= 0 || G->nz <= 0)
+        return fail(c, KIWI_ERR_INVALID_ARG, "cartesian_grid resolution (resolved nx/ny/nz) must be > 0");
+    if (!(G->x_min < G->x_max) || !(G->y_min < G->y_max) || !(G->z_min < G->z_max))
+        return fail(c, KIWI_ERR_INVALID_ARG, "domain min must be < max on every axis");
+    if (P->n > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 particles in one call");
+    if (G->nx > MAX_AXIS || G->ny > MAX_AXIS || G->nz > MAX_AXIS)
+        return fail(c, KIWI_ERR_LIMIT, "more than 8192 cells along a grid axis");
+    return KIWI_OK;
+}
+
+// problems that can share one pipeline run: the same kernel evaluation and arithmetic
+bool same_path(const kiwi_batch_problem& a, const KernelChoice& ka, const kiwi_batch_problem& b, const KernelChoice& kb) {
+    const bool fa = ka.generic || ka.tabulated || a.precision == KIWI_FP64;
+    const bool fb = kb.generic || kb.tabulated || b.precision == KIWI_FP64;
+    return ka.generic == kb.generic && ka.tabulated == kb.tabulated && ka.support_in_h == kb.support_in_h &&
+           (!ka.generic || a.kernel.kernel_id == b.kernel.kernel_id) && fa == fb;
+}
+
+// one pipeline run over a group of problems with the same kernel path (a single call is a group of one)
+int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kernel* K, const KernelChoice& kc,
+              int precision) {
+    KIWI_HIP_CHECK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const double sup = kc.support_in_h;
+    const bool generic = kc.generic;
+
+    // problem table: grids, particle and block offsets, outputs
+    std::vector<Prob> probs(static_cast<size_t>(nprob));
+    int64_t n = 0, nblocks = 0, host_cells = 0;
+    for (int q = 0; q < nprob; ++q) {
+        const kiwi_grid* G = hp[q].G;
+        GridGeom& g = probs[q].g;
+        g.x_min = G->x_min;
+        g.y_min = G->y_min;
+        g.z_min = G->z_min;
+        g.dx = (G->x_max - G->x_min) / G->nx;
+        g.dy = (G->y_max - G->y_min) / G->ny;
+        g.dz = (G->z_max - G->z_min) / G->nz;
+        g.nx = G->nx;
+        g.ny = G->ny;
+        g.nz = G->nz;
+        g.nbx = (G->nx + BLK - 1) / BLK;
+        g.nby = (G->ny + BLK - 1) / BLK;
+        g.nbz = (G->nz + BLK - 1) / BLK;
+        probs[q].part_off = n;
+        probs[q].n_parts = hp[q].P->n;
+        probs[q].block_off = nblocks;
+        n += hp[q].P->n;
+        nblocks += static_cast<int64_t>(g.nbx) * g.nby * g.nbz;
+        if (!hp[q].out->on_device) host_cells += static_cast<int64_t>(G->nx) * G->ny * G->nz;
+    }
+    if (n > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 particles in one call");
+    if (nblocks > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 cell blocks in one call");
+
+    // particle arrays on the device: the caller's own for a single device-resident problem, else the
+    // problems' columns concatenated
+    const double* dp[8];
+    if (nprob == 1 && hp[0].P->on_device) {
+        const kiwi_particles* P = hp[0].P;
+        const double* src[8] = {P->x, P->y, P->z, P->vx, P->vy, P->vz, P->m, P->h};
+        for (int k = 0; k < 8; ++k) dp[k] = src[k];
+    } else {
+        KIWI_HIP_CHECK(c, reserve(c->parts, 8 * sizeof(double) * std::max<int64_t>(n, 1)));
+        for (int k = 0; k < 8; ++k) dp[k] = static_cast<double*>(c->parts.p) + k * n;
+        for (int q = 0; q < nprob; ++q) {
+            const kiwi_particles* P = hp[q].P;
+            const double* src[8] = {P->x, P->y, P->z, P->vx, P->vy, P->vz, P->m, P->h};
+            if (P->n == 0) continue;
+            const hipMemcpyKind kind = P->on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+            for (int k = 0; k < 8; ++k)
+                KIWI_HIP_CHECK(c, hipMemcpyAsync(const_cast<double*>(dp[k]) + probs[q].part_off, src[k],
+                                                 P->n * sizeof(double), kind, s));
+        }
+    }
+    // outputs: the caller's device arrays, or staging for host arrays
+    KIWI_HIP_CHECK(c, reserve(c->outbuf, 4 * sizeof(double) * std::max<int64_t>(host_cells, 1)));
+    {
+        int64_t o = 0;
+        for (int q = 0; q < nprob; ++q) {
+            const kiwi_cell_sums* out = hp[q].out;
+            const int64_t nc = static_cast<int64_t>(hp[q].G->nx) * hp[q].G->ny * hp[q].G->nz;
+            if (out->on_device) {
+                probs[q].out[0] = out->mass;
+                probs[q].out[1] = out->px;
+                probs[q].out[2] = out->py;
+                probs[q].out[3] = out->pz;
+            } else {
+                for (int k = 0; k < 4; ++k) probs[q].out[k] = static_cast<double*>(c->outbuf.p) + 4 * o + k * nc;
+                o += nc;
+            }
+        }
+    }
+    KIWI_HIP_CHECK(c, reserve(c->probs, sizeof(Prob) * nprob));
+    Prob* dpr = static_cast<Prob*>(c->probs.p);
+    KIWI_HIP_CHECK(c, hipMemcpyAsync(dpr, probs.data(), sizeof(Prob) * nprob, hipMemcpyHostToDevice, s));
+    KIWI_HIP_CHECK(c, reserve(c->part_prob, sizeof(int) * std::max<int64_t>(n, 1)));
+    int* part_prob = static_cast<int*>(c->part_prob.p);
+
+    // per-particle sizes (items, lines) and their exclusive scans, (n + 1) each
+    const int64_t n1 = n + 1;
+    KIWI_HIP_CHECK(c, reserve(c->stats, sizeof(DevStats) * nprob + sizeof(unsigned long long)));
+    DevStats* dst = static_cast<DevStats*>(c->stats.p);
+    unsigned long long* cursor = reinterpret_cast<unsigned long long*>(dst + nprob);
+    KIWI_HIP_CHECK(c, reserve(c->bbox, 6 * sizeof(int) * std::max<int64_t>(n, 1)));
+    KIWI_HIP_CHECK(c, reserve(c->sizes, 2 * sizeof(int64_t) * n1));
+    KIWI_HIP_CHECK(c, reserve(c->offsets, 2 * sizeof(int64_t) * n1));
+    KIWI_HIP_CHECK(c, reserve(c->ranges, 2 * sizeof(int) * std::max<int64_t>(nblocks, 1)));
+    KIWI_HIP_CHECK(c, reserve(c->work, sizeof(unsigned long long) * std::max<int64_t>(nblocks, 1)));
+    KIWI_HIP_CHECK(c, reserve(c->work_sorted, sizeof(unsigned long long) * std::max<int64_t>(nblocks, 1)));
+    KIWI_HIP_CHECK(c, reserve(c->order_in, sizeof(int) * std::max<int64_t>(nblocks, 1)));
+    KIWI_HIP_CHECK(c, reserve(c->order, sizeof(int) * std::max<int64_t>(nblocks, 1)));
+    KIWI_HIP_CHECK(c, reserve(c->vsum, sizeof(double) * 3 * nprob * (VSUM_BLOCKS + 1)));
+    int* bbox = static_cast<int*>(c->bbox.p);
+    int64_t* sz = static_cast<int64_t*>(c->sizes.p);
+    int64_t* of = static_cast<int64_t*>(c->offsets.p);
+    int64_t *nblk = sz, *nlin = sz + n1;
+    int64_t *offs = of, *loffs = of + n1;
+    int* blk_begin = static_cast<int*>(c->ranges.p);
+    int* blk_end = blk_begin + nblocks;
+    unsigned long long* work = static_cast<unsigned long long*>(c->work.p);
+    unsigned long long* work_sorted = static_cast<unsigned long long*>(c->work_sorted.p);
+    int* order_in = static_cast<int*>(c->order_in.p);
+    int* order = static_cast<int*>(c->order.p);
+    double* vsums = static_cast<double*>(c->vsum.p);
+
+    KIWI_HIP_CHECK(c, hipMemsetAsync(dst, 0, sizeof(DevStats) * nprob + sizeof(unsigned long long), s));
+    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[0], s));
+    if (n > 0) k_part_prob<<<grid1d(n), THREADS, 0, s>>>(n, dpr, nprob, part_prob);
+    k_prep<<<grid1d(n1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], sup, dpr, part_prob, bbox, nblk, nlin, dst);
+    KIWI_HIP_CHECK(c, hipGetLastError());
+    {
+        size_t tmp_bytes = 0;
+        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, nblk, offs, static_cast<int>(n1), s));
+        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, tmp_bytes));
+        for (int k = 0; k < 2; ++k) {
+            size_t tb = c->cub_tmp.cap;
+            KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, tb, sz + k * n1, of + k * n1,
+                                                               static_cast<int>(n1), s));
+        }
+    }
+    int64_t totals[2] = {0, 0};
+    std::vector<DevStats> hs(static_cast<size_t>(nprob));
+    for (int k = 0; k < 2; ++k)
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(&totals[k], of + k * n1 + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    KIWI_HIP_CHECK(c, hipMemcpyAsync(hs.data(), dst, sizeof(DevStats) * nprob, hipMemcpyDeviceToHost, s));
+    KIWI_HIP_CHECK(c, hipStreamSynchronize(s));
+    const int64_t npairs = totals[0], nlines_total = totals[1];
+    unsigned long long pairs_bbox = 0;
+    for (int q = 0; q < nprob; ++q) pairs_bbox += hs[q].pairs_bbox;
+    if (npairs > 0x7fffffffLL)
+        return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 (particle, cell-block) items in one call");
+
+    // the generic and tabulated paths always compute in double precision (mixed applies to the cubic path)
+    const bool tabulated = kc.tabulated;
+    const bool fp64 = generic || tabulated || precision == KIWI_FP64;
+    const size_t rsz = fp64 ? sizeof(double) : sizeof(float);
+    const size_t line_entry = tabulated ? 0
+                              : generic ? sizeof(LineVals<double, PolicyGen::NLN>)
+                                        : (fp64 ? sizeof(LineVals<double, 2>) : sizeof(LineVals<float, 2>));
+    const int64_t np1 = npairs + 1;
+    KIWI_HIP_CHECK(c, reserve(c->lines, line_entry * static_cast<size_t>(std::max<int64_t>(nlines_total, 1))));
+    KIWI_HIP_CHECK(c, reserve(c->ms, rsz * static_cast<size_t>(std::max<unsigned long long>(pairs_bbox, 1))));
+    KIWI_HIP_CHECK(c, reserve(c->keys_a, sizeof(uint64_t) * np1));
+    KIWI_HIP_CHECK(c, reserve(c->keys_b, sizeof(uint64_t) * np1));
+    KIWI_HIP_CHECK(c, reserve(c->pos_a, sizeof(int) * np1));
+    KIWI_HIP_CHECK(c, reserve(c->pos_b, sizeof(int) * np1));
+    KIWI_HIP_CHECK(c, reserve(c->cells, sizeof(int64_t) * np1));
+    KIWI_HIP_CHECK(c, reserve(c->slab_sorted, sizeof(int64_t) * np1));
+    KIWI_HIP_CHECK(c, reserve(c->slab, sizeof(int64_t) * np1));
+    uint64_t* keys_a = static_cast<uint64_t*>(c->keys_a.p);  // particle order
+    uint64_t* keys_b = static_cast<uint64_t*>(c->keys_b.p);  // block order
+    int* pos_a = static_cast<int*>(c->pos_a.p);
+    int* pos_b = static_cast<int*>(c->pos_b.p);  // particle-order position of each block-order item
+    int64_t* cells = static_cast<int64_t*>(c->cells.p);
+    int64_t* slab_sorted = static_cast<int64_t*>(c->slab_sorted.p);
+    int64_t* slab = static_cast<int64_t*>(c->slab.p);
+    if (n > 0) {
+        k_emit<<<grid1d(n), THREADS, 0, s>>>(n, bbox, offs, dpr, part_prob, keys_a);
+        KIWI_HIP_CHECK(c, hipGetLastError());
+    }
+    int end_bit = 33;  // at least one key bit above the particle index
+    while ((1LL << (end_bit - 32)) < nblocks) ++end_bit;
+    if (npairs > 0) {
+        k_iota<<<grid1d(npairs), THREADS, 0, s>>>(pos_a, npairs);
+        size_t sort_bytes = 0;
+        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys_a, keys_b, pos_a, pos_b,
+                                                              static_cast<int>(npairs), 32, end_bit, s));
+        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, sort_bytes));
+        sort_bytes = c->cub_tmp.cap;
+        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairs(c->cub_tmp.p, sort_bytes, keys_a, keys_b, pos_a, pos_b,
+                                                              static_cast<int>(npairs), 32, end_bit, s));
+    }
+    k_item_cells<<<grid1d(np1), THREADS, 0, s>>>(npairs, keys_b, bbox, dpr, part_prob, cells);
+    {
+        size_t tb = 0;
+        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cells, slab_sorted, static_cast<int>(np1), s));
+        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, tb));
+        tb = c->cub_tmp.cap;
+        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, tb, cells, slab_sorted, static_cast<int>(np1),
+                                                           s));
+    }
+    if (nblocks > 0) {
+        k_fill_int<<<grid1d(2 * nblocks), THREADS, 0, s>>>(blk_begin, 2 * nblocks, 0);
+        k_fill_u64<<<grid1d(nblocks), THREADS, 0, s>>>(work, nblocks);
+        k_iota<<<grid1d(nblocks), THREADS, 0, s>>>(order_in, nblocks);
+    }
+    KIWI_HIP_CHECK(c, hipGetLastError());
+    if (npairs > 0) {
+        k_scatter<<<grid1d(npairs), THREADS, 0, s>>>(npairs, pos_b, slab_sorted, slab);
+        k_ranges<<<grid1d(npairs), THREADS, 0, s>>>(npairs, keys_b, blk_begin, blk_end);
+        k_block_work<<<grid1d(npairs), THREADS, 0, s>>>(npairs, keys_b, cells, work);
+        KIWI_HIP_CHECK(c, hipGetLastError());
+    }
+    if (nblocks > 0) {
+        size_t ob = 0;
+        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, ob, work, work_sorted, order_in, order,
+                                                                        static_cast<int>(nblocks), 0, 64, s));
+        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, ob));
+        ob = c->cub_tmp.cap;
+        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairsDescending(c->cub_tmp.p, ob, work, work_sorted, order_in,
+                                                                        order, static_cast<int>(nblocks), 0, 64, s));
+    }
+    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[1], s));
+    const int clamp = fp64 ? 1 : 0;
+    const int nb = static_cast<int>(nblocks);
+    if (tabulated) {
+        KIWI_HIP_CHECK(c, kiwi_build_m4_table(c, s));
+        const PolicyTab::Ctx pc{sup, static_cast<const double*>(c->m4table.p)};
+        KIWI_HIP_CHECK(c, launch_cells<PolicyTab>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
+                                                  slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob, part_prob,
+                                                  c->lines.p, c->ms.p, dst, cursor, clamp, c->ev[2], c->ev[3],
+                                                  c->ev[4]));
+    } else if (generic) {
+        const PolicyGen::Ctx pc{sup, static_cast<const GenTab*>(c->tables.p) + K->kernel_id};
+        KIWI_HIP_CHECK(c, launch_cells<PolicyGen>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
+                                                  slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob, part_prob,
+                                                  c->lines.p, c->ms.p, dst, cursor, clamp, c->ev[2], c->ev[3],
+                                                  c->ev[4]));
+    } else if (fp64) {
+        KIWI_HIP_CHECK(c, launch_cells<PolicyM4<double>>(c, s, {sup}, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
+                                                         slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
+                                                         part_prob, c->lines.p, c->ms.p, dst, cursor, clamp, c->ev[2],
+                                                         c->ev[3], c->ev[4]));
+    } else {
+        KIWI_HIP_CHECK(c, launch_cells<PolicyM4<float>>(c, s, {sup}, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
+                                                        slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
+                                                        part_prob, c->lines.p, c->ms.p, dst, cursor, clamp, c->ev[2],
+                                                        c->ev[3], c->ev[4]));
+    }
+    k_vsum_problems<<<nprob * VSUM_BLOCKS, THREADS, 0, s>>>(dpr, dp[3], dp[4], dp[5], vsums + 3 * nprob);
+    k_vsum_problems_final<<<(nprob + 63) / 64, 64, 0, s>>>(nprob, vsums + 3 * nprob, vsums);
+    KIWI_HIP_CHECK(c, hipGetLastError());
+
+    for (int q = 0; q < nprob; ++q) {
+        const kiwi_cell_sums* out = hp[q].out;
+        if (out->on_device) continue;
+        const int64_t nc = static_cast<int64_t>(hp[q].G->nx) * hp[q].G->ny * hp[q].G->nz;
+        double* ho[4] = {out->mass, out->px, out->py, out->pz};
+        for (int k = 0; k < 4; ++k)
+            KIWI_HIP_CHECK(c, hipMemcpyAsync(ho[k], probs[q].out[k], nc * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    std::vector<double> vs(static_cast<size_t>(3 * nprob));
+    KIWI_HIP_CHECK(c, hipMemcpyAsync(hs.data(), dst, sizeof(DevStats) * nprob, hipMemcpyDeviceToHost, s));
+    KIWI_HIP_CHECK(c, hipMemcpyAsync(vs.data(), vsums, sizeof(double) * 3 * nprob, hipMemcpyDeviceToHost, s));
+    KIWI_HIP_CHECK(c, hipStreamSynchronize(s));
+    float ms_prep = 0, ms_lines = 0, ms_cells = 0, ms_accum = 0;
+    (void)hipEventElapsedTime(&ms_prep, c->ev[0], c->ev[1]);
+    (void)hipEventElapsedTime(&ms_lines, c->ev[1], c->ev[2]);
+    (void)hipEventElapsedTime(&ms_cells, c->ev[2], c->ev[3]);
+    (void)hipEventElapsedTime(&ms_accum, c->ev[3], c->ev[4]);
+    for (int q = 0; q < nprob; ++q) {
+        kiwi_stats* stats = hp[q].stats;
+        if (!stats) continue;
+        const DevStats& d = hs[q];
+        std::memset(stats, 0, sizeof(*stats));
+        stats->skipped_outside = static_cast<int64_t>(d.skipped_outside);
+        stats->n_negative = static_cast<int64_t>(d.n_negative);
+        stats->n_negative_warn = static_cast<int64_t>(d.n_negative_warn);
+        stats->pairs_bbox = static_cast<int64_t>(d.pairs_bbox);
+        stats->pairs_sphere = static_cast<int64_t>(d.pairs_sphere);
+        stats->particle_blocks = nprob == 1 ? npairs : -1;
+        const int64_t nq = hp[q].P->n;
+        for (int k = 0; k < 3; ++k) stats->v_mean[k] = nq > 0 ? vs[3 * q + k] / static_cast<double>(nq) : 0.0;
+        // device times cover the whole group's run
+        stats->device_ms_prep = ms_prep;
+        stats->device_ms_lines = ms_lines;
+        stats->device_ms_gather = ms_cells;
+        stats->device_ms_accum = ms_accum;
+        for (int r = 0; r < 3; ++r) {
+            stats->lattice_points[r] = static_cast<int64_t>(d.pts[r]);
+            stats->line_evals[r] = static_cast<int64_t>(d.lines[r]);
+        }
+        stats->plane_evals = static_cast<int64_t>(d.planes);
+        const int nw = static_cast<int>(std::min<unsigned long long>(d.n_warn, KIWI_MAX_WARN_RECORDS));
+        stats->n_warn_records = nw;
+        for (int k = 0; k < nw; ++k) stats->warn[k] = d.warn[k];
+    }
+    return KIWI_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -880,248 +1321,50 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     if (!c) return KIWI_ERR_INVALID_ARG;
     c->err.clear();
     if (!P || !G || !K || !out) return fail(c, KIWI_ERR_INVALID_ARG, "null argument");
-    if (P->n < 0) return fail(c, KIWI_ERR_INVALID_ARG, "negative particle count");
-    if (P->n > 0 && (!P->x || !P->y || !P->z || !P->vx || !P->vy || !P->vz || !P->m || !P->h))
-        return fail(c, KIWI_ERR_INVALID_ARG, "null particle array");
-    if (!out->mass || !out->px || !out->py || !out->pz) return fail(c, KIWI_ERR_INVALID_ARG, "null output array");
-    if (G->nx <= 0 || G->ny <= 0 || G->nz <= 0)
-        return fail(c, KIWI_ERR_INVALID_ARG, "cartesian_grid resolution (resolved nx/ny/nz) must be > 0");
-    if (!(G->x_min < G->x_max) || !(G->y_min < G->y_max) || !(G->z_min < G->z_max))
-        return fail(c, KIWI_ERR_INVALID_ARG, "domain min must be < max on every axis");
+    if (const int rc = check_problem(c, P, G, out)) return rc;
     if (precision != KIWI_FP64 && precision != KIWI_MIXED) return fail(c, KIWI_ERR_INVALID_ARG, "unknown precision");
-    if (K->kernel_id < 0 || K->kernel_id > KIWI_KERNEL_WC6) return fail(c, KIWI_ERR_INVALID_ARG, "unknown kernel id");
     KernelChoice kc;
     if (const int rc = resolve_kernel(c, K, &kc)) return rc;
-    const double sup = kc.support_in_h;
-    const bool generic = kc.generic;
-    if (P->n > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 particles in one call");
-    if (G->nx > MAX_AXIS || G->ny > MAX_AXIS || G->nz > MAX_AXIS)
-        return fail(c, KIWI_ERR_LIMIT, "more than 8192 cells along a grid axis");
+    const HostProblem hp{P, G, out, stats};
+    return run_group(c, &hp, 1, K, kc, precision);
+}
 
-    KIWI_HIP_CHECK(c, hipSetDevice(c->device));
-    hipStream_t s = c->stream;
-    const int64_t n = P->n;
-    GridGeom g;
-    g.x_min = G->x_min;
-    g.y_min = G->y_min;
-    g.z_min = G->z_min;
-    g.dx = (G->x_max - G->x_min) / G->nx;
-    g.dy = (G->y_max - G->y_min) / G->ny;
-    g.dz = (G->z_max - G->z_min) / G->nz;
-    g.nx = G->nx;
-    g.ny = G->ny;
-    g.nz = G->nz;
-    g.nbx = (G->nx + BLK - 1) / BLK;
-    g.nby = (G->ny + BLK - 1) / BLK;
-    g.nbz = (G->nz + BLK - 1) / BLK;
-    const int64_t nblocks = static_cast<int64_t>(g.nbx) * g.nby * g.nbz;
-    const int64_t ncells = static_cast<int64_t>(G->nx) * G->ny * G->nz;
-
-    // particle arrays on the device
-    const double* dp[8];
-    const double* hp[8] = {P->x, P->y, P->z, P->vx, P->vy, P->vz, P->m, P->h};
-    if (P->on_device) {
-        for (int k = 0; k < 8; ++k) dp[k] = hp[k];
-    } else {
-        KIWI_HIP_CHECK(c, reserve(c->parts, 8 * sizeof(double) * std::max<int64_t>(n, 1)));
-        for (int k = 0; k < 8; ++k) {
-            double* d = static_cast<double*>(c->parts.p) + k * n;
-            if (n) KIWI_HIP_CHECK(c, hipMemcpyAsync(d, hp[k], n * sizeof(double), hipMemcpyHostToDevice, s));
-            dp[k] = d;
+int kiwi_hip_sph_to_grid_batch(kiwi_hip_ctx* c, int n_problems, const kiwi_batch_problem* problems) {
+    if (!c) return KIWI_ERR_INVALID_ARG;
+    c->err.clear();
+    if (n_problems < 0 || (n_problems > 0 && !problems)) return fail(c, KIWI_ERR_INVALID_ARG, "null argument");
+    if (n_problems > KIWI_MAX_BATCH) return fail(c, KIWI_ERR_LIMIT, "more problems than KIWI_MAX_BATCH in one batch");
+    std::vector<KernelChoice> kcs(static_cast<size_t>(n_problems));
+    for (int i = 0; i < n_problems; ++i) {
+        const kiwi_batch_problem& q = problems[i];
+        if (const int rc = check_problem(c, &q.particles, &q.grid, &q.out)) {
+            c->err = "problem " + std::to_string(i) + ": " + c->err;
+            return rc;
+        }
+        if (q.precision != KIWI_FP64 && q.precision != KIWI_MIXED) {
+            return fail(c, KIWI_ERR_INVALID_ARG, "problem " + std::to_string(i) + ": unknown precision");
+        }
+        if (const int rc = resolve_kernel(c, &q.kernel, &kcs[i])) {
+            c->err = "problem " + std::to_string(i) + ": " + c->err;
+            return rc;
         }
     }
-    double* om[4];
-    if (out->on_device) {
-        om[0] = out->mass;
-        om[1] = out->px;
-        om[2] = out->py;
-        om[3] = out->pz;
-    } else {
-        KIWI_HIP_CHECK(c, reserve(c->outbuf, 4 * sizeof(double) * ncells));
-        for (int k = 0; k < 4; ++k) om[k] = static_cast<double*>(c->outbuf.p) + k * ncells;
-    }
-
-    // per-particle sizes (items, lines) and their exclusive scans, (n + 1) each
-    const int64_t n1 = n + 1;
-    KIWI_HIP_CHECK(c, reserve(c->stats, sizeof(DevStats)));
-    DevStats* dst = static_cast<DevStats*>(c->stats.p);
-    KIWI_HIP_CHECK(c, reserve(c->bbox, 6 * sizeof(int) * std::max<int64_t>(n, 1)));
-    KIWI_HIP_CHECK(c, reserve(c->sizes, 2 * sizeof(int64_t) * n1));
-    KIWI_HIP_CHECK(c, reserve(c->offsets, 2 * sizeof(int64_t) * n1));
-    KIWI_HIP_CHECK(c, reserve(c->ranges, 2 * sizeof(int) * nblocks));
-    KIWI_HIP_CHECK(c, reserve(c->work, sizeof(unsigned long long) * nblocks));
-    KIWI_HIP_CHECK(c, reserve(c->work_sorted, sizeof(unsigned long long) * nblocks));
-    KIWI_HIP_CHECK(c, reserve(c->order_in, sizeof(int) * nblocks));
-    KIWI_HIP_CHECK(c, reserve(c->order, sizeof(int) * nblocks));
-    KIWI_HIP_CHECK(c, reserve(c->vsum, sizeof(double) * (3 * VSUM_BLOCKS + 3)));
-    int* bbox = static_cast<int*>(c->bbox.p);
-    int64_t* sz = static_cast<int64_t*>(c->sizes.p);
-    int64_t* of = static_cast<int64_t*>(c->offsets.p);
-    int64_t *nblk = sz, *nlin = sz + n1;
-    int64_t *offs = of, *loffs = of + n1;
-    int* blk_begin = static_cast<int*>(c->ranges.p);
-    int* blk_end = blk_begin + nblocks;
-    unsigned long long* work = static_cast<unsigned long long*>(c->work.p);
-    unsigned long long* work_sorted = static_cast<unsigned long long*>(c->work_sorted.p);
-    int* order_in = static_cast<int*>(c->order_in.p);
-    int* order = static_cast<int*>(c->order.p);
-    double* vpart = static_cast<double*>(c->vsum.p);
-
-    KIWI_HIP_CHECK(c, hipMemsetAsync(dst, 0, sizeof(DevStats), s));
-    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[0], s));
-    k_prep<<<grid1d(n1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], sup, g, bbox, nblk, nlin, dst);
-    KIWI_HIP_CHECK(c, hipGetLastError());
-    {
-        size_t tmp_bytes = 0;
-        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, nblk, offs, static_cast<int>(n1), s));
-        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, tmp_bytes));
-        for (int k = 0; k < 2; ++k) {
-            size_t tb = c->cub_tmp.cap;
-            KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, tb, sz + k * n1, of + k * n1,
-                                                               static_cast<int>(n1), s));
+    // one pipeline run per kernel path: problems with the same kernel, support and arithmetic
+    std::vector<char> done(static_cast<size_t>(n_problems), 0);
+    for (int i = 0; i < n_problems; ++i) {
+        if (done[i]) continue;
+        std::vector<HostProblem> group;
+        for (int j = i; j < n_problems; ++j) {
+            if (done[j]) continue;
+            if (same_path(problems[i], kcs[i], problems[j], kcs[j])) {
+                group.push_back(HostProblem{&problems[j].particles, &problems[j].grid, &problems[j].out,
+                                            problems[j].stats});
+                done[j] = 1;
+            }
         }
-    }
-    int64_t totals[2] = {0, 0};
-    unsigned long long pairs_bbox = 0;
-    for (int k = 0; k < 2; ++k)
-        KIWI_HIP_CHECK(c, hipMemcpyAsync(&totals[k], of + k * n1 + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    KIWI_HIP_CHECK(c, hipMemcpyAsync(&pairs_bbox, &dst->pairs_bbox, sizeof(pairs_bbox), hipMemcpyDeviceToHost, s));
-    KIWI_HIP_CHECK(c, hipStreamSynchronize(s));
-    const int64_t npairs = totals[0], nlines_total = totals[1];
-    if (npairs > 0x7fffffffLL)
-        return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 (particle, cell-block) items in one call");
-
-    // the generic and tabulated paths always compute in double precision (mixed applies to the cubic path)
-    const bool tabulated = kc.tabulated;
-    const bool fp64 = generic || tabulated || precision == KIWI_FP64;
-    const size_t rsz = fp64 ? sizeof(double) : sizeof(float);
-    const size_t line_entry = tabulated ? 0
-                              : generic ? sizeof(LineVals<double, PolicyGen::NLN>)
-                                        : (fp64 ? sizeof(LineVals<double, 2>) : sizeof(LineVals<float, 2>));
-    const int64_t np1 = npairs + 1;
-    KIWI_HIP_CHECK(c, reserve(c->lines, line_entry * static_cast<size_t>(std::max<int64_t>(nlines_total, 1))));
-    KIWI_HIP_CHECK(c, reserve(c->ms, rsz * static_cast<size_t>(std::max<unsigned long long>(pairs_bbox, 1))));
-    KIWI_HIP_CHECK(c, reserve(c->keys_a, sizeof(uint64_t) * np1));
-    KIWI_HIP_CHECK(c, reserve(c->keys_b, sizeof(uint64_t) * np1));
-    KIWI_HIP_CHECK(c, reserve(c->pos_a, sizeof(int) * np1));
-    KIWI_HIP_CHECK(c, reserve(c->pos_b, sizeof(int) * np1));
-    KIWI_HIP_CHECK(c, reserve(c->cells, sizeof(int64_t) * np1));
-    KIWI_HIP_CHECK(c, reserve(c->slab_sorted, sizeof(int64_t) * np1));
-    KIWI_HIP_CHECK(c, reserve(c->slab, sizeof(int64_t) * np1));
-    uint64_t* keys_a = static_cast<uint64_t*>(c->keys_a.p);  // particle order
-    uint64_t* keys_b = static_cast<uint64_t*>(c->keys_b.p);  // block order
-    int* pos_a = static_cast<int*>(c->pos_a.p);
-    int* pos_b = static_cast<int*>(c->pos_b.p);  // particle-order position of each block-order item
-    int64_t* cells = static_cast<int64_t*>(c->cells.p);
-    int64_t* slab_sorted = static_cast<int64_t*>(c->slab_sorted.p);
-    int64_t* slab = static_cast<int64_t*>(c->slab.p);
-    if (n > 0) {
-        k_emit<<<grid1d(n), THREADS, 0, s>>>(n, bbox, offs, g, keys_a);
-        KIWI_HIP_CHECK(c, hipGetLastError());
-    }
-    int end_bit = 33;  // at least one key bit above the particle index
-    while ((1LL << (end_bit - 32)) < nblocks) ++end_bit;
-    if (npairs > 0) {
-        k_iota<<<grid1d(npairs), THREADS, 0, s>>>(pos_a, npairs);
-        size_t sort_bytes = 0;
-        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys_a, keys_b, pos_a, pos_b,
-                                                              static_cast<int>(npairs), 32, end_bit, s));
-        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, sort_bytes));
-        sort_bytes = c->cub_tmp.cap;
-        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairs(c->cub_tmp.p, sort_bytes, keys_a, keys_b, pos_a, pos_b,
-                                                              static_cast<int>(npairs), 32, end_bit, s));
-    }
-    k_item_cells<<<grid1d(np1), THREADS, 0, s>>>(npairs, keys_b, bbox, g, cells);
-    {
-        size_t tb = 0;
-        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cells, slab_sorted, static_cast<int>(np1), s));
-        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, tb));
-        tb = c->cub_tmp.cap;
-        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, tb, cells, slab_sorted, static_cast<int>(np1),
-                                                           s));
-    }
-    k_fill_int<<<grid1d(2 * nblocks), THREADS, 0, s>>>(blk_begin, 2 * nblocks, 0);
-    k_fill_u64<<<grid1d(nblocks), THREADS, 0, s>>>(work, nblocks);
-    k_iota<<<grid1d(nblocks), THREADS, 0, s>>>(order_in, nblocks);
-    KIWI_HIP_CHECK(c, hipGetLastError());
-    if (npairs > 0) {
-        k_scatter<<<grid1d(npairs), THREADS, 0, s>>>(npairs, pos_b, slab_sorted, slab);
-        k_ranges<<<grid1d(npairs), THREADS, 0, s>>>(npairs, keys_b, blk_begin, blk_end);
-        k_block_work<<<grid1d(npairs), THREADS, 0, s>>>(npairs, keys_b, cells, work);
-        KIWI_HIP_CHECK(c, hipGetLastError());
-    }
-    {
-        size_t ob = 0;
-        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, ob, work, work_sorted, order_in, order,
-                                                                        static_cast<int>(nblocks), 0, 64, s));
-        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, ob));
-        ob = c->cub_tmp.cap;
-        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairsDescending(c->cub_tmp.p, ob, work, work_sorted, order_in,
-                                                                        order, static_cast<int>(nblocks), 0, 64, s));
-    }
-    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[1], s));
-    const int clamp = fp64 ? 1 : 0;
-    const int nb = static_cast<int>(nblocks);
-    if (tabulated) {
-        KIWI_HIP_CHECK(c, kiwi_build_m4_table(c, s));
-        const PolicyTab::Ctx pc{sup, static_cast<const double*>(c->m4table.p)};
-        KIWI_HIP_CHECK(c, launch_cells<PolicyTab>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
-                                                  slab_sorted, blk_begin, blk_end, order, nb, g, c->lines.p, c->ms.p, om,
-                                                  dst, clamp, c->ev[2], c->ev[3], c->ev[4]));
-    } else if (generic) {
-        const PolicyGen::Ctx pc{sup, static_cast<const GenTab*>(c->tables.p) + K->kernel_id};
-        KIWI_HIP_CHECK(c, launch_cells<PolicyGen>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
-                                                  slab_sorted, blk_begin, blk_end, order, nb, g, c->lines.p, c->ms.p, om,
-                                                  dst, clamp, c->ev[2], c->ev[3], c->ev[4]));
-    } else if (fp64) {
-        KIWI_HIP_CHECK(c, launch_cells<PolicyM4<double>>(c, s, {sup}, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
-                                                         slab_sorted, blk_begin, blk_end, order, nb, g, c->lines.p,
-                                                         c->ms.p, om, dst, clamp, c->ev[2], c->ev[3], c->ev[4]));
-    } else {
-        KIWI_HIP_CHECK(c, launch_cells<PolicyM4<float>>(c, s, {sup}, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
-                                                        slab_sorted, blk_begin, blk_end, order, nb, g, c->lines.p,
-                                                        c->ms.p, om, dst, clamp, c->ev[2], c->ev[3], c->ev[4]));
-    }
-    k_vsum_partial<<<VSUM_BLOCKS, THREADS, 0, s>>>(n, dp[3], dp[4], dp[5], vpart);
-    k_vsum_final<<<1, 64, 0, s>>>(vpart, vpart + 3 * VSUM_BLOCKS);
-    KIWI_HIP_CHECK(c, hipGetLastError());
-
-    if (!out->on_device) {
-        double* ho[4] = {out->mass, out->px, out->py, out->pz};
-        for (int k = 0; k < 4; ++k)
-            KIWI_HIP_CHECK(c, hipMemcpyAsync(ho[k], om[k], ncells * sizeof(double), hipMemcpyDeviceToHost, s));
-    }
-    DevStats hs;
-    double vs[3];
-    KIWI_HIP_CHECK(c, hipMemcpyAsync(&hs, dst, sizeof(DevStats), hipMemcpyDeviceToHost, s));
-    KIWI_HIP_CHECK(c, hipMemcpyAsync(vs, vpart + 3 * VSUM_BLOCKS, sizeof(vs), hipMemcpyDeviceToHost, s));
-    KIWI_HIP_CHECK(c, hipStreamSynchronize(s));
-    if (stats) {
-        std::memset(stats, 0, sizeof(*stats));
-        stats->skipped_outside = static_cast<int64_t>(hs.skipped_outside);
-        stats->n_negative = static_cast<int64_t>(hs.n_negative);
-        stats->n_negative_warn = static_cast<int64_t>(hs.n_negative_warn);
-        stats->pairs_bbox = static_cast<int64_t>(hs.pairs_bbox);
-        stats->pairs_sphere = static_cast<int64_t>(hs.pairs_sphere);
-        stats->particle_blocks = npairs;
-        for (int k = 0; k < 3; ++k) stats->v_mean[k] = n > 0 ? vs[k] / static_cast<double>(n) : 0.0;
-        float ms_prep = 0, ms_lines = 0, ms_cells = 0, ms_accum = 0;
-        (void)hipEventElapsedTime(&ms_prep, c->ev[0], c->ev[1]);
-        (void)hipEventElapsedTime(&ms_lines, c->ev[1], c->ev[2]);
-        (void)hipEventElapsedTime(&ms_cells, c->ev[2], c->ev[3]);
-        (void)hipEventElapsedTime(&ms_accum, c->ev[3], c->ev[4]);
-        stats->device_ms_prep = ms_prep;
-        stats->device_ms_lines = ms_lines;
-        stats->device_ms_gather = ms_cells;
-        stats->device_ms_accum = ms_accum;
-        for (int r = 0; r < 3; ++r) {
-            stats->lattice_points[r] = static_cast<int64_t>(hs.pts[r]);
-            stats->line_evals[r] = static_cast<int64_t>(hs.lines[r]);
-        }
-        stats->plane_evals = static_cast<int64_t>(hs.planes);
-        const int nw = static_cast<int>(std::min<unsigned long long>(hs.n_warn, KIWI_MAX_WARN_RECORDS));
-        stats->n_warn_records = nw;
-        for (int k = 0; k < nw; ++k) stats->warn[k] = hs.warn[k];
+        if (const int rc = run_group(c, group.data(), static_cast<int>(group.size()), &problems[i].kernel, kcs[i],
+                                     problems[i].precision))
+            return rc;
     }
     return KIWI_OK;
 }

@@ -31,7 +31,8 @@ _i = ctypes.c_int
 _pd = ctypes.POINTER(ctypes.c_double)
 
 EXPORTED = ["kiwi_hip_abi_version", "kiwi_hip_create", "kiwi_hip_destroy", "kiwi_hip_last_error",
-            "kiwi_hip_stream", "kiwi_hip_sph_to_grid", "kiwi_hip_voronoi_map", "kiwi_hip_sph_to_grid_adjoint"]
+            "kiwi_hip_stream", "kiwi_hip_sph_to_grid", "kiwi_hip_voronoi_map", "kiwi_hip_sph_to_grid_adjoint",
+            "kiwi_hip_sph_to_grid_batch"]
 
 
 class KiwiError(RuntimeError):
@@ -80,6 +81,11 @@ class Stats(ctypes.Structure):
 
 
 _pll = ctypes.POINTER(ctypes.c_int64)
+
+
+class BatchProblem(ctypes.Structure):
+    _fields_ = [("particles", Particles), ("grid", Grid), ("kernel", Kernel), ("precision", _i), ("out", CellSums),
+                ("stats", ctypes.POINTER(Stats))]
 _pi = ctypes.POINTER(ctypes.c_int)
 
 
@@ -145,6 +151,8 @@ def load():
         L.kiwi_hip_sph_to_grid_adjoint.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.POINTER(Grid),
                                                    ctypes.POINTER(Kernel), _i, ctypes.POINTER(CellSums),
                                                    ctypes.POINTER(ParticleGrads), ctypes.POINTER(AdjointStats)]
+        L.kiwi_hip_sph_to_grid_batch.restype = _i
+        L.kiwi_hip_sph_to_grid_batch.argtypes = [ctypes.c_void_p, _i, ctypes.POINTER(BatchProblem)]
         L.kiwi_hip_voronoi_map.restype = _i
         L.kiwi_hip_voronoi_map.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.c_void_p,
                                            ctypes.POINTER(VoronoiCells), ctypes.POINTER(Kernel), _i,
@@ -235,6 +243,31 @@ class Context:
         self._check(self.L.kiwi_hip_sph_to_grid(self.h, ctypes.byref(P), ctypes.byref(G), ctypes.byref(kernel),
                                                 int(precision), ctypes.byref(O), ctypes.byref(st)))
         return arrs[0], arrs[1], arrs[2], arrs[3], st
+
+    def sph_to_grid_batch(self, problems):
+        """Many sph-to-grid problems in one call.  problems: list of dicts with keys parts, domain, res and
+        optionally kernel, precision, out (as for sph_to_grid).  Returns a list of (mass, px, py, pz, Stats),
+        one per problem."""
+        arr = (BatchProblem * max(len(problems), 1))()
+        keep, results = [], []
+        for i, q in enumerate(problems):
+            P = _particles(q["parts"])
+            nx, ny, nz = (int(v) for v in q["res"])
+            G = Grid(*[float(v) for v in q["domain"]], nx, ny, nz)
+            nc = nx * ny * nz
+            if q.get("out") is None:
+                outs = [np.zeros(nc) for _ in range(4)]
+                O = CellSums(*[a.ctypes.data_as(_pd) for a in outs], 0)
+            else:
+                outs = list(q["out"])
+                O = CellSums(*[_ptr(a)[0] for a in outs], 1)
+            st = Stats()
+            arr[i] = BatchProblem(P, G, q.get("kernel") or kernel_spec(), int(q.get("precision", FP64)), O,
+                                  ctypes.pointer(st))
+            keep.append((P, O, q["parts"]))
+            results.append((outs, st))
+        self._check(self.L.kiwi_hip_sph_to_grid_batch(self.h, len(problems), arr))
+        return [(o[0], o[1], o[2], o[3], st) for o, st in results]
 
     def sph_to_grid_adjoint(self, parts, domain, res, grad_cells, kernel=None, precision=FP64, out=None):
         """Gradients of a loss L with respect to the particles' m, x, y, z, h, vx, vy, vz, given grad_cells =

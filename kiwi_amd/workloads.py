@@ -10,6 +10,12 @@ lognormal_field   N particles drawn from a log-normal density field: a periodic 
                   [h_min, h_max]; velocities = the finite-difference gradient of delta at the particle's
                   cell, scaled to v_rms per axis, plus N(0, noise) per axis.  Domain [0, n]^3 with unit
                   cells.
+lognormal_box     the same for a box of nx x ny x nz unit cells: the field is generated on the enclosing
+                  cube and the particles are sampled in the box.
+cutout_batch      a batch of independent cutouts: particle counts log-uniform in [n_lo, n_hi], the
+                  largest grid axis log-uniform in [g_lo, g_hi] cells and the others 1 to 4 times shorter,
+                  kernels alternating between the cubic spline (mixed precision) and Wendland C2 (generic
+                  path, FP64).
 """
 import numpy as np
 
@@ -76,6 +82,47 @@ def lognormal_field(n_particles=1_000_000, n=256, sigma=1.5, seed=2, h_min=0.5, 
                  vy=vel[:, 1].copy(), vz=vel[:, 2].copy(), m=m, h=h)
     domain = np.array([0.0, float(n), 0.0, float(n), 0.0, float(n)])
     return parts, domain, (n, n, n)
+
+
+def lognormal_box(n_particles, dims, sigma=1.5, seed=2, h_min=0.5, h_max=50.0, v_rms=5.0, noise=2.0, hfact=1.2):
+    nx, ny, nz = (int(d) for d in dims)
+    n = max(nx, ny, nz)
+    rng = np.random.default_rng(seed + 1000)
+    delta = gaussian_field(n, sigma, seed)[:nx, :ny, :nz]
+    rho = np.exp(delta - 0.5 * sigma * sigma)
+    w = rho.reshape(-1)
+    cdf = np.cumsum(w)
+    cdf /= cdf[-1]
+    cell = np.minimum(np.searchsorted(cdf, rng.uniform(0.0, 1.0, n_particles), side="right"), w.size - 1)
+    ix, rem = np.divmod(cell, ny * nz)
+    iy, iz = np.divmod(rem, nz)
+    pos = np.stack([ix, iy, iz], 1) + rng.uniform(0.0, 1.0, size=(n_particles, 3))
+    nbar = n_particles / float(w.size)
+    h = np.clip(hfact * (1.0 / (nbar * w[cell])) ** (1.0 / 3.0), h_min, h_max)
+    vel = np.empty((n_particles, 3))
+    for a in range(3):
+        g = (0.5 * (np.roll(delta, -1, axis=a) - np.roll(delta, 1, axis=a))).reshape(-1)
+        vel[:, a] = g[cell] * (v_rms / g.std())
+    vel += rng.normal(0.0, noise, size=vel.shape)
+    m = np.full(n_particles, 1.0 / n_particles)
+    parts = dict(x=pos[:, 0].copy(), y=pos[:, 1].copy(), z=pos[:, 2].copy(), vx=vel[:, 0].copy(),
+                 vy=vel[:, 1].copy(), vz=vel[:, 2].copy(), m=m, h=h)
+    domain = np.array([0.0, float(nx), 0.0, float(ny), 0.0, float(nz)])
+    return parts, domain, (nx, ny, nz)
+
+
+def cutout_batch(n_problems=64, seed=3, n_lo=1_000, n_hi=200_000, g_lo=32, g_hi=192):
+    """A list of (parts, domain, res, kernel_name) problems (see the module docstring)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(n_problems):
+        npart = int(round(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi)))))
+        big = int(round(np.exp(rng.uniform(np.log(g_lo), np.log(g_hi)))))
+        dims = [big] + [max(8, int(round(big / rng.uniform(1.0, 4.0)))) for _ in range(2)]
+        dims = [dims[i] for i in rng.permutation(3)]
+        parts, dom, res = lognormal_box(npart, dims, seed=seed * 1000 + q)
+        out.append((parts, dom, res, "M4" if q % 2 == 0 else "WC2"))
+    return out
 
 
 def padded_domain(parts, support_in_h=2.0):
