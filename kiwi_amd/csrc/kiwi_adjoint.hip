@@ -56,35 +56,45 @@ __global__ __launch_bounds__(THREADS) void k_adj_prep(int64_t n, const double* _
                                                       uint32_t* __restrict__ work, int* __restrict__ order,
                                                       AdjDevStats* st) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
-    if (i > n) return;
-    if (i == n) {
-        nlines[n] = 0;
-        return;
-    }
-    int* bb = bbox + 6 * i;
-    bb[0] = 1;
-    bb[1] = 0;
-    int64_t nl = 0;
-    uint32_t wk = 0;
-    const double hh = h[i];
-    if (hh > 0.0) {
-        int q[6];
-        if (particle_bbox(g, 2.0, x[i], y[i], z[i], hh, q)) {
-            for (int k = 0; k < 6; ++k) bb[k] = q[k];
-            const int64_t Nx = q[1] - q[0] + 1, Ny = q[3] - q[2] + 1, Nz = q[5] - q[4] + 1;
-            nl = (Nx + 1) * (Ny + 1) + (Nx + 1) * (Nz + 1) + (Ny + 1) * (Nz + 1);
-            const int64_t blocks = static_cast<int64_t>(q[1] / BLK - q[0] / BLK + 1) * (q[3] / BLK - q[2] / BLK + 1) *
-                                   (q[5] / BLK - q[4] / BLK + 1);
-            wk = static_cast<uint32_t>(std::min<int64_t>(blocks * ADJ_PTS + Nx * Ny * Nz, 0xffffffffLL));
-            atomicAdd(&st->pairs_bbox, static_cast<unsigned long long>(Nx * Ny * Nz));
-            atomicAdd(&st->active, 1ull);
-        } else {
-            atomicAdd(&st->skipped_outside, 1ull);
+    if (i == n) nlines[n] = 0;
+    unsigned long long pairs = 0, active = 0, skipped = 0;
+    if (i < n) {  // lanes past the particles only join the counter reduction below, with zeros
+        int* bb = bbox + 6 * i;
+        bb[0] = 1;
+        bb[1] = 0;
+        int64_t nl = 0;
+        uint32_t wk = 0;
+        const double hh = h[i];
+        if (hh > 0.0) {
+            int q[6];
+            if (particle_bbox(g, 2.0, x[i], y[i], z[i], hh, q)) {
+                for (int k = 0; k < 6; ++k) bb[k] = q[k];
+                const int64_t Nx = q[1] - q[0] + 1, Ny = q[3] - q[2] + 1, Nz = q[5] - q[4] + 1;
+                nl = (Nx + 1) * (Ny + 1) + (Nx + 1) * (Nz + 1) + (Ny + 1) * (Nz + 1);
+                const int64_t blocks = static_cast<int64_t>(q[1] / BLK - q[0] / BLK + 1) *
+                                       (q[3] / BLK - q[2] / BLK + 1) * (q[5] / BLK - q[4] / BLK + 1);
+                wk = static_cast<uint32_t>(std::min<int64_t>(blocks * ADJ_PTS + Nx * Ny * Nz, 0xffffffffLL));
+                pairs = static_cast<unsigned long long>(Nx * Ny * Nz);
+                active = 1;
+            } else {
+                skipped = 1;
+            }
         }
+        nlines[i] = nl;
+        work[i] = wk;
+        order[i] = static_cast<int>(i);
     }
-    nlines[i] = nl;
-    work[i] = wk;
-    order[i] = static_cast<int>(i);
+    // one atomic per wavefront and counter
+    for (int off = 32; off > 0; off >>= 1) {
+        pairs += __shfl_down(pairs, off);
+        active += __shfl_down(active, off);
+        skipped += __shfl_down(skipped, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (pairs) atomicAdd(&st->pairs_bbox, pairs);
+        if (active) atomicAdd(&st->active, active);
+        if (skipped) atomicAdd(&st->skipped_outside, skipped);
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
