@@ -43,6 +43,10 @@ __device__ __forceinline__ R gap(R lo, R hi) {
 
 // a value that is the same in every lane, moved to a scalar register
 __device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t wave_uniform64(uint64_t v) {
+    return static_cast<uint64_t>(wave_uniform(static_cast<uint32_t>(v))) |
+           (static_cast<uint64_t>(wave_uniform(static_cast<uint32_t>(v >> 32))) << 32);
+}
 
 // LDS writes of one lane made visible to the other lanes of the same wavefront
 __device__ __forceinline__ void wave_sync() {

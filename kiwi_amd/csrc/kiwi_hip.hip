@@ -51,7 +51,15 @@ namespace {
 
 constexpr int BLK_PTS = (BLK + 1) * (BLK + 1) * (BLK + 1);
 constexpr int MAX_AXIS = 8192;                // grid cells per axis
-constexpr unsigned ITEM_GRAB = 8;             // items a wavefront takes from the cursor at once
+constexpr unsigned ITEM_GRAB = 16;            // items a wavefront takes from a cursor at once
+// The cell kernel's items are split into CURSORS contiguous slices, each with its own work cursor: one
+// cursor serialises its atomics (about 12 ns apart), which capped a single cursor at ~80M grabs/s.  A
+// workgroup starts on slice blockIdx % CURSORS (workgroups are dispatched round-robin over the 8 XCDs,
+// so a slice's items, and the line tables of their particles, stay in one XCD's L2) and then takes
+// from the other slices until all are drained.
+constexpr int CURSORS = 8;
+constexpr int CURSOR_STRIDE = 544;            // 64-bit words between cursors (4352 B: separate lines)
+constexpr size_t CURSOR_BYTES = sizeof(unsigned long long) * CURSORS * CURSOR_STRIDE;
 
 struct DevStats {  // one per problem
     unsigned long long skipped_outside, n_negative, n_negative_warn, pairs_bbox, pairs_sphere;
@@ -559,157 +567,165 @@ __global__ __launch_bounds__(THREADS) void k_particles(
         c_pts[0] = c_pts[1] = c_pts[2] = 0;
     };
 
-    uint32_t cur = 0;
-    if (lane == 0) cur = static_cast<uint32_t>(atomicAdd(cursor, static_cast<unsigned long long>(ITEM_GRAB)));
-    cur = wave_uniform(cur);
-    while (cur < npairs) {
-        uint32_t nxt = 0;
-        if (lane == 0) nxt = static_cast<uint32_t>(atomicAdd(cursor, static_cast<unsigned long long>(ITEM_GRAB)));
-        const uint32_t tend = min(cur + ITEM_GRAB, npairs);
-        for (uint32_t t = cur; t < tend; ++t) {
-            const uint64_t key = keys[t];
-            const uint32_t p = static_cast<uint32_t>(key & 0xffffffffull);
-            const int q = static_cast<int>(wave_uniform(static_cast<uint32_t>(part_prob[p])));
-            if (q != cur_q) {
-                flush();
-                cur_q = q;
-            }
-            const GridGeom& g = pr[q].g;
-            const int* bb = bbox + 6 * static_cast<size_t>(p);
-            const ItemBox ib = item_box(bb, static_cast<uint32_t>((key >> 32) - pr[q].block_off), g);
-            const int mx = ib.mx, my = ib.my, mz = ib.mz;
-            const int nx = mx + 1, ny = my + 1, nz = mz + 1;  // item lattice
-            const int Lx = bb[1] - bb[0] + 2, Ly = bb[3] - bb[2] + 2, Lz = bb[5] - bb[4] + 2;  // particle lattice
-            const int ox = ib.x0 - bb[0], oy = ib.y0 - bb[2], oz = ib.z0 - bb[4];
-            const double inv_s = 1.0 / (P::unit_in_h(ctx) * h[p]);  // lattice units
-            const double px = x[p], py = y[p], pz = z[p];
-            const R* etab = reinterpret_cast<const R*>(lines + loffs[p]);
-            R* out = ms + slab[t];
-
-            // ---- per-axis constants of the item's lattice planes ----
-            if (lane < nx + ny + nz) {
-                int a = 0, u = lane, n_a = nx;
-                if (u >= nx) {
-                    u -= nx;
-                    a = 1;
-                    n_a = ny;
-                    if (u >= ny) {
-                        u -= ny;
-                        a = 2;
-                        n_a = nz;
-                    }
+    const int home = static_cast<int>(blockIdx.x % CURSORS);
+    for (int d = 0; d < CURSORS; ++d) {
+        const int sl = (home + d) % CURSORS;
+        const uint64_t lo = static_cast<uint64_t>(npairs) * sl / CURSORS;
+        const uint64_t hi = static_cast<uint64_t>(npairs) * (sl + 1) / CURSORS;
+        unsigned long long* cs = cursor + sl * CURSOR_STRIDE;
+        uint64_t cur = 0;
+        if (lane == 0) cur = lo + atomicAdd(cs, static_cast<unsigned long long>(ITEM_GRAB));
+        cur = wave_uniform64(cur);
+        while (cur < hi) {
+            uint64_t nxt = 0;
+            if (lane == 0) nxt = lo + atomicAdd(cs, static_cast<unsigned long long>(ITEM_GRAB));
+            const uint32_t tend = static_cast<uint32_t>(min(cur + ITEM_GRAB, hi));
+            for (uint32_t t = static_cast<uint32_t>(cur); t < tend; ++t) {
+                const uint64_t key = keys[t];
+                const uint32_t p = static_cast<uint32_t>(key & 0xffffffffull);
+                const int q = static_cast<int>(wave_uniform(static_cast<uint32_t>(part_prob[p])));
+                if (q != cur_q) {
+                    flush();
+                    cur_q = q;
                 }
-                // offsets formed as the reference forms vertex coordinates: (x_min + i dx) - x_p
-                const int g0 = sel3(a, ib.x0, ib.y0, ib.z0);
-                const double lo = gmin(g, a), pa = sel3(a, px, py, pz), st_a = gstep(g, a);
-                const double d = (lo + (g0 + u) * st_a - pa) * inv_s;
-                const double dm = (lo + (g0 + max(u - 1, 0)) * st_a - pa) * inv_s;
-                const double dp = (lo + (g0 + min(u + 1, n_a - 1)) * st_a - pa) * inv_s;
-                const double dn = (lo + (g0 + u + 1) * st_a - pa) * inv_s;
-                const R dr = static_cast<R>(d);
-                R pv[P::NPL];
-                P::plane(ctx, dr < R(0) ? -dr : dr, pv);
-                const R gp = static_cast<R>(gap(dm, dp)), gc = static_cast<R>(gap(d, dn));
-                L.sg[lane] = Pair2<R>{dr, gp * gp};
-#pragma unroll
-                for (int q = 0; q < P::NPL; ++q) L.pc[q][lane] = pv[q];
-                L.cg2[lane] = gc * gc;
-                ++c_planes;
-            }
-            wave_sync();
+                const GridGeom& g = pr[q].g;
+                const int* bb = bbox + 6 * static_cast<size_t>(p);
+                const ItemBox ib = item_box(bb, static_cast<uint32_t>((key >> 32) - pr[q].block_off), g);
+                const int mx = ib.mx, my = ib.my, mz = ib.mz;
+                const int nx = mx + 1, ny = my + 1, nz = mz + 1;  // item lattice
+                // particle lattice
+                const int Lx = bb[1] - bb[0] + 2, Ly = bb[3] - bb[2] + 2, Lz = bb[5] - bb[4] + 2;
+                const int ox = ib.x0 - bb[0], oy = ib.y0 - bb[2], oz = ib.z0 - bb[4];
+                const double inv_s = 1.0 / (P::unit_in_h(ctx) * h[p]);  // lattice units
+                const double px = x[p], py = y[p], pz = z[p];
+                const R* etab = reinterpret_cast<const R*>(lines + loffs[p]);
+                R* out = ms + slab[t];
 
-            // ---- classify the lattice points: zero planes, not needed, inside the support, outside ----
-            const int nyz = ny * nz, npts = nx * nyz;
-            uint32_t nA = 0, nB = 0;
-            BoxWalk w(lane, ny, nz);
-            for (int u0 = 0; u0 < npts; u0 += 64, w.next(ny, nz)) {
-                const int u = u0 + lane;
-                bool isA = false, isB = false;
-                if (u < npts) {
-                    const Pair2<R> ex = L.sg[w.i], ey = L.sg[nx + w.j], ez = L.sg[nx + ny + w.k];
-                    if (ex.a == R(0) || ey.a == R(0) || ez.a == R(0)) {
-                        L.U[u] = R(0);  // every pyramid anchored in a plane through the particle is flat
-                    } else if (!P::SPHERE_SKIP || ex.b + ey.b + ez.b < S2) {
-                        const R r2 = ex.a * ex.a + ey.a * ey.a + ez.a * ez.a;
-                        isA = r2 < S2;
-                        isB = !isA;
-                    }
-                }
-                const unsigned long long bA = __ballot(isA), bB = __ballot(isB);
-                const uint32_t code = static_cast<uint32_t>(u) | (static_cast<uint32_t>(w.i) << 10) |
-                                      (static_cast<uint32_t>(w.j) << 14) | (static_cast<uint32_t>(w.k) << 18);
-                if (isA) L.list[nA + __popcll(bA & lane_lt)] = code;
-                if (isB) L.list[BLK_PTS - 1 - (nB + __popcll(bB & lane_lt))] = code;
-                nA += __popcll(bA);
-                nB += __popcll(bB);
-            }
-            wave_sync();
-
-            // ---- lattice values of the needed points (inside the support first, then outside) ----
-            // line-table blocks (x,y) (x,z) (y,z) of the particle's table
-            const int exz = Lx * Ly, eyz = exz + Lx * Lz;
-            const uint32_t ntot = nA + nB;
-            for (uint32_t v = lane; v < ntot; v += 64) {
-                const uint32_t code = (v < nA) ? L.list[v] : L.list[BLK_PTS - 1 - (v - nA)];
-                const int u = static_cast<int>(code & 1023u);
-                const int i = static_cast<int>((code >> 10) & 15u), j = static_cast<int>((code >> 14) & 15u),
-                          k = static_cast<int>((code >> 18) & 15u);
-                const R dx = L.sg[i].a, dy = L.sg[nx + j].a, dz = L.sg[nx + ny + k].a;
-                const R ax = dx < R(0) ? -dx : dx, ay = dy < R(0) ? -dy : dy, az = dz < R(0) ? -dz : dz;
-                const R r2 = ax * ax + ay * ay + az * az;
-                const R r = KMath<R>::sqrt_(r2);
-                const int reg = P::region(ctx, r2);
-                ++c_pts[min(reg, 3) - 1];
-                const R csum = L.pc[reg - 1][i] + L.pc[reg - 1][nx + j] + L.pc[reg - 1][nx + ny + k];
-                R dsum = R(0);
-                const int slot = P::line_slot(reg);
-                if (slot >= 0) {
-                    const int gi = ox + i, gj = oy + j, gk = oz + k;  // particle-lattice coordinates
-                    const uint32_t c = static_cast<uint32_t>(slot);
-                    dsum = (etab[NLN * static_cast<uint32_t>(gi * Ly + gj) + c] +
-                            etab[NLN * static_cast<uint32_t>(exz + gi * Lz + gk) + c]) +
-                           etab[NLN * static_cast<uint32_t>(eyz + gj * Lz + gk) + c];
-                }
-                const R W = P::free_part(ctx, ax, ay, az, r, reg) + P::plane_weight() * csum + dsum;
-                const bool neg = (dx < R(0)) != ((dy < R(0)) != (dz < R(0)));
-                L.U[u] = neg ? -W : W;
-            }
-            wave_sync();
-
-            // ---- cell sums: the 8-corner stencil, stored to the item's slab ----
-            const int ncells = mx * my * mz;
-            BoxWalk wc(lane, my, mz);
-            for (int u = lane; u < ncells; u += 64, wc.next(my, mz)) {
-                const int i = wc.i, j = wc.j, k = wc.k;
-                double Ms = 0.0;
-                if (!P::SPHERE_SKIP || L.cg2[i] + L.cg2[nx + j] + L.cg2[nx + ny + k] < S2) {
-                    ++n_sphere;
-                    const R* U0 = &L.U[(i * ny + j) * nz + k];
-                    Ms = static_cast<double>(U0[nyz]) + static_cast<double>(U0[nz]) + static_cast<double>(U0[1]) +
-                         static_cast<double>(U0[nyz + nz + 1]) - static_cast<double>(U0[0]) -
-                         static_cast<double>(U0[nyz + nz]) - static_cast<double>(U0[nyz + 1]) -
-                         static_cast<double>(U0[nz + 1]);
-                    if (clamp_pairs && Ms < 0.0) {
-                        ++n_neg;
-                        if (Ms <= -0.01) {
-                            ++n_warn;
-                            DevStats* sq = st + cur_q;
-                            const unsigned long long slot = atomicAdd(&sq->n_warn, 1ull);
-                            if (slot < KIWI_MAX_WARN_RECORDS) {
-                                sq->warn[slot].particle = static_cast<int64_t>(p) - pr[cur_q].part_off;
-                                sq->warn[slot].cell =
-                                    (static_cast<int64_t>(ib.x0 + i) * g.ny + (ib.y0 + j)) * g.nz + (ib.z0 + k);
-                                sq->warn[slot].value = Ms;
-                            }
+                // ---- per-axis constants of the item's lattice planes ----
+                if (lane < nx + ny + nz) {
+                    int a = 0, u = lane, n_a = nx;
+                    if (u >= nx) {
+                        u -= nx;
+                        a = 1;
+                        n_a = ny;
+                        if (u >= ny) {
+                            u -= ny;
+                            a = 2;
+                            n_a = nz;
                         }
-                        Ms = 0.0;
                     }
+                    // offsets formed as the reference forms vertex coordinates: (x_min + i dx) - x_p
+                    const int g0 = sel3(a, ib.x0, ib.y0, ib.z0);
+                    const double lo = gmin(g, a), pa = sel3(a, px, py, pz), st_a = gstep(g, a);
+                    const double d = (lo + (g0 + u) * st_a - pa) * inv_s;
+                    const double dm = (lo + (g0 + max(u - 1, 0)) * st_a - pa) * inv_s;
+                    const double dp = (lo + (g0 + min(u + 1, n_a - 1)) * st_a - pa) * inv_s;
+                    const double dn = (lo + (g0 + u + 1) * st_a - pa) * inv_s;
+                    const R dr = static_cast<R>(d);
+                    R pv[P::NPL];
+                    P::plane(ctx, dr < R(0) ? -dr : dr, pv);
+                    const R gp = static_cast<R>(gap(dm, dp)), gc = static_cast<R>(gap(d, dn));
+                    L.sg[lane] = Pair2<R>{dr, gp * gp};
+    #pragma unroll
+                    for (int q = 0; q < P::NPL; ++q) L.pc[q][lane] = pv[q];
+                    L.cg2[lane] = gc * gc;
+                    ++c_planes;
                 }
-                out[u] = static_cast<R>(Ms);
+                wave_sync();
+
+                // ---- classify the lattice points: zero planes, not needed, inside the support, outside ----
+                const int nyz = ny * nz, npts = nx * nyz;
+                uint32_t nA = 0, nB = 0;
+                BoxWalk w(lane, ny, nz);
+                for (int u0 = 0; u0 < npts; u0 += 64, w.next(ny, nz)) {
+                    const int u = u0 + lane;
+                    bool isA = false, isB = false;
+                    if (u < npts) {
+                        const Pair2<R> ex = L.sg[w.i], ey = L.sg[nx + w.j], ez = L.sg[nx + ny + w.k];
+                        if (ex.a == R(0) || ey.a == R(0) || ez.a == R(0)) {
+                            L.U[u] = R(0);  // every pyramid anchored in a plane through the particle is flat
+                        } else if (!P::SPHERE_SKIP || ex.b + ey.b + ez.b < S2) {
+                            const R r2 = ex.a * ex.a + ey.a * ey.a + ez.a * ez.a;
+                            isA = r2 < S2;
+                            isB = !isA;
+                        }
+                    }
+                    const unsigned long long bA = __ballot(isA), bB = __ballot(isB);
+                    const uint32_t code = static_cast<uint32_t>(u) | (static_cast<uint32_t>(w.i) << 10) |
+                                          (static_cast<uint32_t>(w.j) << 14) | (static_cast<uint32_t>(w.k) << 18);
+                    if (isA) L.list[nA + __popcll(bA & lane_lt)] = code;
+                    if (isB) L.list[BLK_PTS - 1 - (nB + __popcll(bB & lane_lt))] = code;
+                    nA += __popcll(bA);
+                    nB += __popcll(bB);
+                }
+                wave_sync();
+
+                // ---- lattice values of the needed points (inside the support first, then outside) ----
+                // line-table blocks (x,y) (x,z) (y,z) of the particle's table
+                const int exz = Lx * Ly, eyz = exz + Lx * Lz;
+                const uint32_t ntot = nA + nB;
+                for (uint32_t v = lane; v < ntot; v += 64) {
+                    const uint32_t code = (v < nA) ? L.list[v] : L.list[BLK_PTS - 1 - (v - nA)];
+                    const int u = static_cast<int>(code & 1023u);
+                    const int i = static_cast<int>((code >> 10) & 15u), j = static_cast<int>((code >> 14) & 15u),
+                              k = static_cast<int>((code >> 18) & 15u);
+                    const R dx = L.sg[i].a, dy = L.sg[nx + j].a, dz = L.sg[nx + ny + k].a;
+                    const R ax = dx < R(0) ? -dx : dx, ay = dy < R(0) ? -dy : dy, az = dz < R(0) ? -dz : dz;
+                    const R r2 = ax * ax + ay * ay + az * az;
+                    const R r = KMath<R>::sqrt_(r2);
+                    const int reg = P::region(ctx, r2);
+                    ++c_pts[min(reg, 3) - 1];
+                    const R csum = L.pc[reg - 1][i] + L.pc[reg - 1][nx + j] + L.pc[reg - 1][nx + ny + k];
+                    R dsum = R(0);
+                    const int slot = P::line_slot(reg);
+                    if (slot >= 0) {
+                        const int gi = ox + i, gj = oy + j, gk = oz + k;  // particle-lattice coordinates
+                        const uint32_t c = static_cast<uint32_t>(slot);
+                        dsum = (etab[NLN * static_cast<uint32_t>(gi * Ly + gj) + c] +
+                                etab[NLN * static_cast<uint32_t>(exz + gi * Lz + gk) + c]) +
+                               etab[NLN * static_cast<uint32_t>(eyz + gj * Lz + gk) + c];
+                    }
+                    const R W = P::free_part(ctx, ax, ay, az, r, reg) + P::plane_weight() * csum + dsum;
+                    const bool neg = (dx < R(0)) != ((dy < R(0)) != (dz < R(0)));
+                    L.U[u] = neg ? -W : W;
+                }
+                wave_sync();
+
+                // ---- cell sums: the 8-corner stencil, stored to the item's slab ----
+                const int ncells = mx * my * mz;
+                BoxWalk wc(lane, my, mz);
+                for (int u = lane; u < ncells; u += 64, wc.next(my, mz)) {
+                    const int i = wc.i, j = wc.j, k = wc.k;
+                    double Ms = 0.0;
+                    if (!P::SPHERE_SKIP || L.cg2[i] + L.cg2[nx + j] + L.cg2[nx + ny + k] < S2) {
+                        ++n_sphere;
+                        const R* U0 = &L.U[(i * ny + j) * nz + k];
+                        Ms = static_cast<double>(U0[nyz]) + static_cast<double>(U0[nz]) + static_cast<double>(U0[1]) +
+                             static_cast<double>(U0[nyz + nz + 1]) - static_cast<double>(U0[0]) -
+                             static_cast<double>(U0[nyz + nz]) - static_cast<double>(U0[nyz + 1]) -
+                             static_cast<double>(U0[nz + 1]);
+                        if (clamp_pairs && Ms < 0.0) {
+                            ++n_neg;
+                            if (Ms <= -0.01) {
+                                ++n_warn;
+                                DevStats* sq = st + cur_q;
+                                const unsigned long long slot = atomicAdd(&sq->n_warn, 1ull);
+                                if (slot < KIWI_MAX_WARN_RECORDS) {
+                                    sq->warn[slot].particle = static_cast<int64_t>(p) - pr[cur_q].part_off;
+                                    sq->warn[slot].cell =
+                                        (static_cast<int64_t>(ib.x0 + i) * g.ny + (ib.y0 + j)) * g.nz + (ib.z0 + k);
+                                    sq->warn[slot].value = Ms;
+                                }
+                            }
+                            Ms = 0.0;
+                        }
+                    }
+                    out[u] = static_cast<R>(Ms);
+                }
+                wave_sync();
             }
-            wave_sync();
+            cur = wave_uniform64(nxt);
         }
-        cur = wave_uniform(nxt);
     }
     flush();
 }
@@ -1054,9 +1070,10 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
 
     // per-particle sizes (items, lines) and their exclusive scans, (n + 1) each
     const int64_t n1 = n + 1;
-    KIWI_HIP_CHECK(c, reserve(c->stats, sizeof(DevStats) * nprob + sizeof(unsigned long long)));
+    const size_t stats_bytes = (sizeof(DevStats) * nprob + 255) / 256 * 256;
+    KIWI_HIP_CHECK(c, reserve(c->stats, stats_bytes + CURSOR_BYTES));
     DevStats* dst = static_cast<DevStats*>(c->stats.p);
-    unsigned long long* cursor = reinterpret_cast<unsigned long long*>(dst + nprob);
+    unsigned long long* cursor = reinterpret_cast<unsigned long long*>(static_cast<char*>(c->stats.p) + stats_bytes);
     KIWI_HIP_CHECK(c, reserve(c->bbox, 6 * sizeof(int) * std::max<int64_t>(n, 1)));
     KIWI_HIP_CHECK(c, reserve(c->sizes, 2 * sizeof(int64_t) * n1));
     KIWI_HIP_CHECK(c, reserve(c->offsets, 2 * sizeof(int64_t) * n1));
@@ -1079,7 +1096,7 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     int* order = static_cast<int*>(c->order.p);
     double* vsums = static_cast<double*>(c->vsum.p);
 
-    KIWI_HIP_CHECK(c, hipMemsetAsync(dst, 0, sizeof(DevStats) * nprob + sizeof(unsigned long long), s));
+    KIWI_HIP_CHECK(c, hipMemsetAsync(dst, 0, stats_bytes + CURSOR_BYTES, s));
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[0], s));
     if (n > 0) k_part_prob<<<grid1d(n), THREADS, 0, s>>>(n, dpr, nprob, part_prob);
     k_prep<<<grid1d(n1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], sup, dpr, part_prob, bbox, nblk, nlin, dst);
