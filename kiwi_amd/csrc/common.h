@@ -206,14 +206,6 @@ __global__ __attribute__((unused)) void k_vsum_final(const double* partial, doub
 
 namespace {
 
-struct GenTab {  // the fields gen_lattice.h reads, copied from a KiwiKernelTable
-    int nreg;
-    double c[KIWI_TAB_MAXREG];
-    double A3_pi;
-    double a[KIWI_TAB_MAXREG][KIWI_TAB_NI];
-    double B[KIWI_TAB_MAXREG][KIWI_TAB_MAXREG][KIWI_TAB_NB];
-};
-
 // the kernel tables on the host, in KIWI_KERNEL_M4 .. KIWI_KERNEL_WC6 order
 const KiwiKernelTable kHostTables[KIWI_TAB_NKERNELS] = {KIWI_KERNEL_TABLE_DATA};
 

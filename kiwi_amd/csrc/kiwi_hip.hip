@@ -314,19 +314,22 @@ struct PolicyGen {
     static __device__ __forceinline__ void line(const Ctx& c, double da, double db, double* v) { gen_line(*c.tab, da, db, v); }
     static __device__ __forceinline__ int line_kind(double l2) { return l2 < 1.0 ? 1 : 0; }
     static __device__ __forceinline__ double free_part(const Ctx& c, double ax, double ay, double az, double r, int k) {
-        const double d[3] = {ax, ay, az};
-        double s = 0.0;
+        const GenRegion& R = c.tab->reg[k - 1];
+        // seven reciprocals shared by the 6 pyramids: offsets, radius, squared distances to the axes
+        const double ix = 1.0 / ax, iy = 1.0 / ay, iz = 1.0 / az, ir = 1.0 / r;
+        const double ixy = 1.0 / (ax * ax + ay * ay), ixz = 1.0 / (ax * ax + az * az), iyz = 1.0 / (ay * ay + az * az);
         // T_b for edge normal b, atanh(d_c / r) for along-edge axis c; the 6 ordered (a, b) pyramids
-        const double T0 = atan(ay * az / (ax * r)), T1 = atan(ax * az / (ay * r)), T2 = atan(ax * ay / (az * r));
-        const double K0 = 0.5 * log((r + ax) * (r + ax) / (ay * ay + az * az));
-        const double K1 = 0.5 * log((r + ay) * (r + ay) / (ax * ax + az * az));
-        const double K2 = 0.5 * log((r + az) * (r + az) / (ax * ax + ay * ay));
-        s += gen_pyramid_nonphi(*c.tab, k, d[0], d[1], d[2], r, T1, K2);  // (x, y, z)
-        s += gen_pyramid_nonphi(*c.tab, k, d[0], d[2], d[1], r, T2, K1);  // (x, z, y)
-        s += gen_pyramid_nonphi(*c.tab, k, d[1], d[0], d[2], r, T0, K2);  // (y, x, z)
-        s += gen_pyramid_nonphi(*c.tab, k, d[1], d[2], d[0], r, T2, K0);  // (y, z, x)
-        s += gen_pyramid_nonphi(*c.tab, k, d[2], d[0], d[1], r, T0, K1);  // (z, x, y)
-        s += gen_pyramid_nonphi(*c.tab, k, d[2], d[1], d[0], r, T1, K0);  // (z, y, x)
+        const double T0 = atan(ay * az * ix * ir), T1 = atan(ax * az * iy * ir), T2 = atan(ax * ay * iz * ir);
+        const double K0 = 0.5 * log((r + ax) * (r + ax) * iyz);
+        const double K1 = 0.5 * log((r + ay) * (r + ay) * ixz);
+        const double K2 = 0.5 * log((r + az) * (r + az) * ixy);
+        double s = 0.0;
+        s += gen_pyramid_nonphi(R, ax, ay, az, r, T1, K2, ix, iy, ir, ixy);  // (x, y, z)
+        s += gen_pyramid_nonphi(R, ax, az, ay, r, T2, K1, ix, iz, ir, ixz);  // (x, z, y)
+        s += gen_pyramid_nonphi(R, ay, ax, az, r, T0, K2, iy, ix, ir, ixy);  // (y, x, z)
+        s += gen_pyramid_nonphi(R, ay, az, ax, r, T2, K0, iy, iz, ir, iyz);  // (y, z, x)
+        s += gen_pyramid_nonphi(R, az, ax, ay, r, T0, K1, iz, ix, ir, ixz);  // (z, x, y)
+        s += gen_pyramid_nonphi(R, az, ay, ax, r, T1, K0, iz, iy, ir, iyz);  // (z, y, x)
         return s;
     }
     static __device__ __forceinline__ double plane_weight() { return 0.5 * GEN_PI; }
@@ -1299,14 +1302,7 @@ int kiwi_hip_create(kiwi_hip_ctx** out, int device_ordinal) {
     }
     // the generic kernels' coefficient tables, resident for the context's lifetime
     GenTab tabs[KIWI_TAB_NKERNELS];
-    for (int k = 0; k < KIWI_TAB_NKERNELS; ++k) {
-        const KiwiKernelTable& T = kHostTables[k];
-        tabs[k].nreg = T.nreg;
-        std::memcpy(tabs[k].c, T.c, sizeof(T.c));
-        tabs[k].A3_pi = T.A3_pi;
-        std::memcpy(tabs[k].a, T.a, sizeof(T.a));
-        std::memcpy(tabs[k].B, T.B, sizeof(T.B));
-    }
+    for (int k = 0; k < KIWI_TAB_NKERNELS; ++k) gen_tab_init(tabs[k], kHostTables[k]);
     if (reserve(c->tables, sizeof(tabs)) != hipSuccess ||
         hipMemcpy(c->tables.p, tabs, sizeof(tabs), hipMemcpyHostToDevice) != hipSuccess) {
         kiwi_hip_destroy(c);

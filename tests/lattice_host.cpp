@@ -74,9 +74,18 @@ float lat_atan2_pos_f(float y, float x) { return KMath<float>::atan2_pos(y, x); 
 #include "../kiwi_amd/csrc/gen_lattice.h"
 
 namespace {
-const KiwiKernelTable kTabs[KIWI_TAB_NKERNELS] = {KIWI_KERNEL_TABLE_DATA};
+const KiwiKernelTable kRawTabs[KIWI_TAB_NKERNELS] = {KIWI_KERNEL_TABLE_DATA};
 
-double gen_U(const KiwiKernelTable& K, double dx, double dy, double dz) {
+struct GenTabs {
+    GenTab t[KIWI_TAB_NKERNELS];
+    GenTabs() {
+        for (int k = 0; k < KIWI_TAB_NKERNELS; ++k) gen_tab_init(t[k], kRawTabs[k]);
+    }
+};
+const GenTabs kGen;
+const GenTab* const kTabs = kGen.t;
+
+double gen_U(const GenTab& K, double dx, double dy, double dz) {
     if (dx == 0.0 || dy == 0.0 || dz == 0.0) return 0.0;
     const double W = gen_lattice_W(K, std::fabs(dx), std::fabs(dy), std::fabs(dz));
     const bool neg = (dx < 0.0) != ((dy < 0.0) != (dz < 0.0));
@@ -88,7 +97,7 @@ extern "C" {
 
 // Msum of the box [lo, hi] for kernel kid with support H: the 8-corner stencil of gen_lattice_W
 double lat_gen_box(int kid, const double* p, double H, const double* lo, const double* hi) {
-    const KiwiKernelTable& K = kTabs[kid];
+    const GenTab& K = kTabs[kid];
     double m = 0.0;
     for (int di = 0; di < 2; ++di)
         for (int dj = 0; dj < 2; ++dj)
@@ -151,7 +160,7 @@ double lat_face_walk(int kind, int kid, double sup, const double* p, double h, c
         EvM4 ev{1.0 / h};
         m = face_walk_integral(p[0], p[1], p[2], 2.0 * h, v, n, ev, &r);
     } else if (kind == 1) {
-        EvGen<KiwiKernelTable> ev{&kTabs[kid], 1.0 / (sup * h)};
+        EvGen<GenTab> ev{&kTabs[kid], 1.0 / (sup * h)};
         m = face_walk_integral(p[0], p[1], p[2], sup * h, v, n, ev, &r);
     } else {
         EvTab ev{host_table().data(), 1.0 / h};
