@@ -152,7 +152,9 @@ const char* kiwi_hip_last_error(const kiwi_hip_ctx* ctx);
 void* kiwi_hip_stream(kiwi_hip_ctx* ctx);
 
 /* Direct SPH -> Cartesian grid, replacing src/cartesian_grid.cpp:184-301.
- * precision: KIWI_FP64 or KIWI_MIXED.  out: n_cells = nx*ny*nz per array.  stats may be NULL. */
+ * precision: KIWI_FP64 or KIWI_MIXED.  KIWI_MIXED computes in FP32 for the analytic cubic spline and for
+ * the generic cubic spline and Wendland C2; the tabulated path and the other generic kernels (M5, M6,
+ * Wendland C4, C6) run in FP64 whatever is asked.  out: n_cells = nx*ny*nz per array.  stats may be NULL. */
 int kiwi_hip_sph_to_grid(kiwi_hip_ctx* ctx, const kiwi_particles* particles, const kiwi_grid* grid,
                          const kiwi_kernel* kernel, int precision, const kiwi_cell_sums* out,
                          kiwi_stats* stats);

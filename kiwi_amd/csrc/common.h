@@ -209,6 +209,13 @@ namespace {
 // the kernel tables on the host, in KIWI_KERNEL_M4 .. KIWI_KERNEL_WC6 order
 const KiwiKernelTable kHostTables[KIWI_TAB_NKERNELS] = {KIWI_KERNEL_TABLE_DATA};
 
+// Generic kernels whose mixed-precision requests run in single precision: the cubic spline and
+// Wendland C2, whose FP32 lattice values carry the cubic path's error (a few 1e-7 of a particle's mass
+// in total, growing with h in cells like the cubic path's).  The higher-order kernels (M5, M6, Wendland
+// C4 and C6) lose accuracy in their basis recurrences in single precision and run in FP64 whatever the
+// request.
+inline bool gen_mixed_ok(int kernel_id) { return kernel_id == KIWI_KERNEL_M4 || kernel_id == KIWI_KERNEL_WC2; }
+
 // Which integral a kernel request selects: make_kernel_integrator (include/kiwi/kernel_integrator.hpp:
 // 14-46, src/config.cpp:269-277) -- the cubic analytic path iff M4 with support 2h and no generic request,
 // the generic path otherwise; the tabulated integral only on the cubic path (src/config.cpp:174-178).

@@ -80,9 +80,9 @@ struct EvGen {
         const double dc = t * inv_H;  // = qR tan(phi)
         const double r = sqrt(q0 * q0 + qR * qR + dc * dc);
         const int k = gen_region_of(*K, r);
-        const double inv_q0 = 1.0 / q0, inv_qR = 1.0 / qR, inv_r = 1.0 / r, inv_l2 = 1.0 / (q0 * q0 + qR * qR);
-        const double T = atan(q0 * dc * inv_qR * inv_r);
-        const double Kh = 0.5 * log((r + dc) * (r + dc) * inv_l2);
+        const double inv_r = 1.0 / r;
+        const double T = atan(q0 * dc / qR * inv_r);
+        const double Kh = 0.5 * log((r + dc) * (r + dc) / (q0 * q0 + qR * qR));
         double Ek = 0.0, Pk = 0.0;
 #pragma unroll
         for (int q = 0; q < GEN_NREG; ++q)
@@ -90,7 +90,7 @@ struct EvGen {
                 Ek = E[q];
                 Pk = Phi[q];
             }
-        return Ek + gen_pyramid_nonphi(K->reg[k - 1], q0, qR, dc, r, T, Kh, inv_q0, inv_qR, inv_r, inv_l2) + Pk * phi;
+        return Ek + gen_pyramid_nonphi(K->reg[k - 1], q0, qR, dc, r, T, Kh, inv_r) + Pk * phi;
     }
 };
 

@@ -31,6 +31,7 @@
 #include <cmath>
 
 #include "kernel_tables.h"
+#include "m4_lattice.h"
 
 #ifdef __HIPCC__
 #define KIWI_GHD __host__ __device__ inline
@@ -49,13 +50,16 @@ constexpr int GEN_NEVEN = (GEN_NI - 1) / 2;  // even orders I_-2l, l = 1..7 (coe
 // The coefficients of one radial region in the form the lattice evaluation uses, compiled once per
 // kernel from the table (gen_tab_init): the loop bounds that the largest nonzero coefficient fixes,
 // A3 folded into every coefficient, and the binomial sums of the even orders as polynomials in q0^2.
-struct GenRegion {
-    int n_odd;                             // odd orders m = 0 .. n_odd - 1
-    int n_even;                            // even orders l = 1 .. n_even
-    double c1;                             // A3 a_0: the I_1 term
-    double odd[GEN_NODD];                  // A3 a_(2m+2)
-    double even[GEN_NEVEN + 1][GEN_NEVEN]; // [l][j]: A3 a_(2(l+j)+1) C(l+j, l); row 0 unused
+// Compiled in double; the single-precision copy is the double one rounded.
+template <typename R>
+struct GenRegionT {
+    int n_odd;                        // odd orders m = 0 .. n_odd - 1
+    int n_even;                       // even orders l = 1 .. n_even
+    R c1;                             // A3 a_0: the I_1 term
+    R odd[GEN_NODD];                  // A3 a_(2m+2)
+    R even[GEN_NEVEN + 1][GEN_NEVEN]; // [l][j]: A3 a_(2(l+j)+1) C(l+j, l); row 0 unused
 };
+using GenRegion = GenRegionT<double>;
 
 // A kernel's table as the device reads it: the KiwiKernelTable fields the evaluation needs plus the
 // compiled regions.
@@ -65,8 +69,20 @@ struct GenTab {
     double A3_pi;
     double a[KIWI_TAB_MAXREG][KIWI_TAB_NI];
     double B[KIWI_TAB_MAXREG][KIWI_TAB_MAXREG][KIWI_TAB_NB];
-    GenRegion reg[KIWI_TAB_MAXREG];
+    GenRegionT<double> reg[KIWI_TAB_MAXREG];
+    GenRegionT<float> regf[KIWI_TAB_MAXREG];
 };
+
+template <typename R>
+KIWI_GHD const GenRegionT<R>* gen_regions(const GenTab& K);
+template <>
+KIWI_GHD const GenRegionT<double>* gen_regions<double>(const GenTab& K) {
+    return K.reg;
+}
+template <>
+KIWI_GHD const GenRegionT<float>* gen_regions<float>(const GenTab& K) {
+    return K.regf;
+}
 
 inline void gen_tab_init(GenTab& G, const KiwiKernelTable& K) {
     G = GenTab{};
@@ -89,9 +105,8 @@ inline void gen_tab_init(GenTab& G, const KiwiKernelTable& K) {
             }
         GenRegion& R = G.reg[k];
         R.c1 = A3 * av[0];
-        if (imax < 2) continue;
-        R.n_odd = (imax - 2) / 2 + 1;
-        R.n_even = (imax - 1) / 2;
+        R.n_odd = imax >= 2 ? (imax - 2) / 2 + 1 : 0;
+        R.n_even = imax >= 2 ? (imax - 1) / 2 : 0;
         for (int m = 0; m < R.n_odd; ++m) R.odd[m] = A3 * av[2 * m + 2];
         for (int l = 1; l <= R.n_even; ++l) {
             double binom = 1.0;  // C(kk, l)
@@ -100,6 +115,13 @@ inline void gen_tab_init(GenTab& G, const KiwiKernelTable& K) {
                 R.even[l][kk - l] = A3 * av[2 * kk + 1] * binom;
             }
         }
+        GenRegionT<float>& F = G.regf[k];
+        F.n_odd = R.n_odd;
+        F.n_even = R.n_even;
+        F.c1 = static_cast<float>(R.c1);
+        for (int m = 0; m < GEN_NODD; ++m) F.odd[m] = static_cast<float>(R.odd[m]);
+        for (int l = 0; l <= GEN_NEVEN; ++l)
+            for (int j = 0; j < GEN_NEVEN; ++j) F.even[l][j] = static_cast<float>(R.even[l][j]);
     }
 }
 
@@ -114,22 +136,22 @@ KIWI_GHD int gen_region_of(const Tab& K, double q) {
 // Phi_k(q0) for k = 1..nreg: the coefficient of phi in G_k for a wall at q0 (zero below the wall's
 // own region j, where no vertex of this wall can lie).  generic_wall_constants,
 // src/kernel_integral_generic.cpp:96-117, reduced to the phi terms.
-template <class Tab>
-KIWI_GHD void gen_plane_phi(const Tab& K, double q0, double Phi[GEN_NREG]) {
+template <class Tab, typename R>
+KIWI_GHD void gen_plane_phi(const Tab& K, R q0, R Phi[GEN_NREG]) {
     const int j = gen_region_of(K, q0);
-    const double A3 = K.A3_pi / GEN_PI;
+    const R A3 = R(K.A3_pi / GEN_PI);
     for (int k = 1; k <= GEN_NREG; ++k) {
-        double v = 0.0;
+        R v = R(0);
         if (k <= K.nreg && k >= j) {
-            double b = 0.0;
-            for (int p = KIWI_TAB_NB - 1; p >= 0; --p) b = b * q0 + K.B[j - 1][k - 1][p];
+            R b = R(0);
+            for (int p = KIWI_TAB_NB - 1; p >= 0; --p) b = b * q0 + R(K.B[j - 1][k - 1][p]);
             v = A3 * b;
             // I_0 = phi and every even-order basis integral I_-2k (coefficient index 1 - i = 1, 3, 5, ..)
             // hold phi exactly once
-            double qp = q0;
+            R qp = q0;
             for (int idx = 1; idx < GEN_NI; idx += 2, qp *= q0 * q0) {
-                const double av = K.a[k - 1][idx];
-                if (av != 0.0) v += A3 * av * qp;
+                const R av = R(K.a[k - 1][idx]);
+                if (av != R(0)) v += A3 * av * qp;
             }
         }
         Phi[k - 1] = v;
@@ -137,106 +159,135 @@ KIWI_GHD void gen_plane_phi(const Tab& K, double q0, double Phi[GEN_NREG]) {
 }
 
 // Gnon: G_k without its phi terms, for the pyramid (wall da, edge normal db, along dc) at radius r,
-// given T = atan(da dc / (db r)), Kh = atanh(dc / r) and the reciprocals 1/da, 1/db, 1/r and
-// 1/(da^2 + db^2) (a lattice point shares them between its 6 pyramids, so no division is left here).
-// da, db, dc > 0.  The basis integrals follow basis_integrals (src/kernel_integral_generic.cpp:19-66)
-// with sin/cos of phi taken from the lattice.
-KIWI_GHD double gen_pyramid_nonphi(const GenRegion& R, double da, double db, double dc, double r, double T,
-                                   double Kh, double inv_da, double inv_db, double inv_r, double inv_l2) {
-    double sum = R.c1 * T;  // I_1
-    if (R.n_odd == 0) return sum;
-    const double a = db * inv_da, a2 = a * a, da2 = da * da;
-    // odd orders I_-(2m+1): the atanh recurrence K_m = u / (2m (1 - u^2)^m) + (2m - 1)/(2m) K_(m-1)
-    const double u = dc * inv_r;
-    const double inv_omu2 = (r * r) * inv_l2;  // 1 / (1 - u^2)
-    double Iodd = T, Km = Kh, p1a2 = 1.0, inv_om = 1.0, qp = da2;  // qp = q0^(2m+2)
+// given T = atan(da dc / (db r)), Kh = atanh(dc / r) and 1/r.  da, db, dc > 0.  The basis integrals
+// follow basis_integrals (src/kernel_integral_generic.cpp:19-66) with sin/cos of phi taken from the
+// lattice, each multiplied by its power of q0 = da and carried in scaled form, so that no intermediate
+// grows or shrinks with a = db/da (single precision overflowed there):
+//   odd orders, coefficient index 2m + 2: q0^(2m+2) I_-(2m+1) = da^2 (q0^(2m) I_-(2m-1)) + da db Kt_m,
+//     Kt_m = L^m K_m = u r^(2m) / (2m) + (2m-1)/(2m) L Kt_(m-1),  Kt_0 = Kh,  L = da^2 + db^2, u = dc/r
+//     (the atanh recurrence K_m = u / (2m (1-u^2)^m) + (2m-1)/(2m) K_(m-1) times L^m);
+//   even orders, index 2k + 1: sum_l a^2l q0^(2l+1) J_2l P_l(q0^2) = sum_l da Jt_l P_l(da^2),
+//     Jt_l = db^(2l) J_2l = db dc S^(l-1) / (2l-1) + (2l-2)/(2l-1) db^2 Jt_(l-1),  Jt_1 = db dc,
+//     S = db^2 + dc^2 (the secant recurrence J_2l = sec^(2l-2) t / (2l-1) + (2l-2)/(2l-1) J_2l-2,
+//     t = dc/db, times db^(2l)), P_l the compiled binomial sum over k >= l.
+// Every scaled term is a sum of non-negative parts.
+template <typename R>
+KIWI_GHD R gen_pyramid_nonphi(const GenRegionT<R>& G, R da, R db, R dc, R r, R T, R Kh, R inv_r) {
+    R sum = G.c1 * T;  // I_1
+    if (G.n_odd == 0) return sum;
+    const R da2 = da * da, dab = da * db, L = da2 + db * db, r2 = r * r;
+    const R u = dc * inv_r;
+    R Kt = Kh, r2m = R(1);
+    R It = da2 * T + dab * Kh;  // q0^2 I_-1
+    if (G.odd[0] != R(0)) sum += G.odd[0] * It;
 #pragma unroll
-    for (int m = 0; m < GEN_NODD; ++m) {
-        if (m >= R.n_odd) break;
-        if (m > 0) {
-            inv_om *= inv_omu2;
-            Km = u * (0.5 / m) * inv_om + ((2.0 * m - 1.0) / (2.0 * m)) * Km;
-            p1a2 *= (1.0 + a2);
-            qp *= da2;
-        }
-        Iodd += a * p1a2 * Km;
-        if (R.odd[m] != 0.0) sum += R.odd[m] * qp * Iodd;
+    for (int m = 1; m < GEN_NODD; ++m) {
+        if (m >= G.n_odd) break;
+        r2m *= r2;
+        Kt = u * R(0.5 / m) * r2m + R((2.0 * m - 1.0) / (2.0 * m)) * L * Kt;
+        It = da2 * It + dab * Kt;
+        if (G.odd[m] != R(0)) sum += G.odd[m] * It;
     }
-    // even orders I_-2k without phi: sum_l a^2l J_2l q0^(2l+1) P_l(q0^2), P_l the compiled binomial sum
-    // over k >= l, with the secant recurrence J_2l = sec^(2l-2) t / (2l-1) + (2l-2)/(2l-1) J_2l-2
-    if (R.n_even >= 1) {
-        const double t = dc * inv_db, sec2 = 1.0 + t * t;
-        double Jl = t, sp = 1.0, apl = a2, q2l = da2 * da;  // J_2l, sec^(2l-2), a^2l, q0^(2l+1)
+    if (G.n_even >= 1) {
+        const R db2 = db * db, S = db2 + dc * dc, dbc = db * dc;
+        R Jt = dbc, Sp = R(1);  // db^(2l) J_2l, S^(l-1)
 #pragma unroll
         for (int l = 1; l <= GEN_NEVEN; ++l) {
-            if (l > R.n_even) break;
+            if (l > G.n_even) break;
             if (l > 1) {
-                sp *= sec2;
-                Jl = sp * t * (1.0 / (2.0 * l - 1.0)) + ((2.0 * l - 2.0) / (2.0 * l - 1.0)) * Jl;
-                apl *= a2;
-                q2l *= da2;
+                Sp *= S;
+                Jt = dbc * Sp * R(1.0 / (2.0 * l - 1.0)) + R((2.0 * l - 2.0) / (2.0 * l - 1.0)) * db2 * Jt;
             }
-            double P = 0.0;
-            for (int j = R.n_even - l; j >= 0; --j) P = P * da2 + R.even[l][j];
-            sum += (q2l * P) * apl * Jl;
+            R P = R(0);
+            for (int j = G.n_even - l; j >= 0; --j) P = P * da2 + G.even[l][j];
+            sum += da * Jt * P;
         }
     }
     return sum;
 }
 
+// The phi-free part of the lattice value at absolute offsets (ax, ay, az) > 0, radius r (units of H)
+// in region k: the 6 ordered pyramids (wall a, edge normal b), which share T_b = atan(da dc / (db r)),
+// atanh(d_c / r) and seven reciprocals.
+template <typename R>
+KIWI_GHD R gen_lattice_free(const GenRegionT<R>& G, R ax, R ay, R az, R r) {
+    using M = KMath<R>;
+    const R ix = M::rcp(ax), iy = M::rcp(ay), iz = M::rcp(az), ir = M::rcp(r);
+    const R ixy = M::rcp(ax * ax + ay * ay), ixz = M::rcp(ax * ax + az * az), iyz = M::rcp(ay * ay + az * az);
+    const R T0 = M::atan_pos(ay * az * ix * ir), T1 = M::atan_pos(ax * az * iy * ir), T2 = M::atan_pos(ax * ay * iz * ir);
+    const R K0 = R(0.5) * M::log_((r + ax) * (r + ax) * iyz);
+    const R K1 = R(0.5) * M::log_((r + ay) * (r + ay) * ixz);
+    const R K2 = R(0.5) * M::log_((r + az) * (r + az) * ixy);
+    R s = R(0);
+    s += gen_pyramid_nonphi(G, ax, ay, az, r, T1, K2, ir);  // (x, y, z)
+    s += gen_pyramid_nonphi(G, ax, az, ay, r, T2, K1, ir);  // (x, z, y)
+    s += gen_pyramid_nonphi(G, ay, ax, az, r, T0, K2, ir);  // (y, x, z)
+    s += gen_pyramid_nonphi(G, ay, az, ax, r, T2, K0, ir);  // (y, z, x)
+    s += gen_pyramid_nonphi(G, az, ax, ay, r, T0, K1, ir);  // (z, x, y)
+    s += gen_pyramid_nonphi(G, az, ay, ax, r, T1, K0, ir);  // (z, y, x)
+    return s;
+}
+
 // G_k - G_(k+1) at the pyramid (da, db, dc), phi = atan(dc / db), Phi the wall's phi coefficients:
 // the jump of the vertex integral across the boundary between regions k and k + 1 (edge-line offsets).
-KIWI_GHD double gen_pyramid_G_jump(const GenTab& K, int k, double da, double db, double dc, const double* Phi) {
-    const double r = std::sqrt(da * da + db * db + dc * dc);
-    const double inv_da = 1.0 / da, inv_db = 1.0 / db, inv_r = 1.0 / r, inv_l2 = 1.0 / (da * da + db * db);
-    const double T = std::atan(da * dc * inv_db * inv_r);
-    const double Kh = 0.5 * std::log((r + dc) * (r + dc) * inv_l2);
-    const double g0 = gen_pyramid_nonphi(K.reg[k - 1], da, db, dc, r, T, Kh, inv_da, inv_db, inv_r, inv_l2);
-    const double g1 = gen_pyramid_nonphi(K.reg[k], da, db, dc, r, T, Kh, inv_da, inv_db, inv_r, inv_l2);
-    return (g0 - g1) + (Phi[k - 1] - Phi[k]) * std::atan2(dc, db);
+template <typename R>
+KIWI_GHD R gen_pyramid_G_jump(const GenTab& K, int k, R da, R db, R dc, const R* Phi) {
+    using M = KMath<R>;
+    const R r = M::sqrt_(da * da + db * db + dc * dc);
+    const R inv_r = M::rcp(r);
+    const R T = M::atan_pos(da * dc * M::rcp(db) * inv_r);
+    const R Kh = R(0.5) * M::log_((r + dc) * (r + dc) * M::rcp(da * da + db * db));
+    const GenRegionT<R>* G = gen_regions<R>(K);
+    const R g0 = gen_pyramid_nonphi(G[k - 1], da, db, dc, r, T, Kh, inv_r);
+    const R g1 = gen_pyramid_nonphi(G[k], da, db, dc, r, T, Kh, inv_r);
+    return (g0 - g1) + (Phi[k - 1] - Phi[k]) * M::atan2_pos(dc, db);
 }
 
 // Region offsets of the edge line with wall distance da and edge distance db (units of H), per
 // region k = 1..nreg: generic_edge_constants (src/kernel_integral_generic.cpp:119-148) with the
 // segment boundaries as virtual vertices at radius c_k.  Regions below the line's own foot region
 // get 0 (no vertex of the line lies there).
-KIWI_GHD void gen_line(const GenTab& K, double da, double db, double E[GEN_NREG]) {
-    for (int k = 0; k < GEN_NREG; ++k) E[k] = 0.0;
-    if (!(da > 0.0) || !(db > 0.0)) return;
-    const double rs2 = da * da + db * db;
-    const int k0 = gen_region_of(K, std::sqrt(rs2));
-    double Phi[GEN_NREG];
+template <typename R>
+KIWI_GHD void gen_line(const GenTab& K, R da, R db, R E[GEN_NREG]) {
+    for (int k = 0; k < GEN_NREG; ++k) E[k] = R(0);
+    if (!(da > R(0)) || !(db > R(0))) return;
+    const R rs2 = da * da + db * db;
+    const int k0 = gen_region_of(K, KMath<R>::sqrt_(rs2));
+    R Phi[GEN_NREG];
     gen_plane_phi(K, da, Phi);
-    double off = 0.0;
+    R off = R(0);
 #pragma unroll
     for (int k = 1; k < GEN_NREG; ++k) {  // fixed trip count: E stays in registers
         if (k < k0 || k >= K.nreg) continue;
         // boundary between regions k and k + 1 at radius c_k
-        const double ck = K.c[k - 1];
-        const double dc = std::sqrt(std::fmax(ck * ck - rs2, 0.0));
-        if (dc > 0.0) off += gen_pyramid_G_jump(K, k, da, db, dc, Phi);
+        const R ck = R(K.c[k - 1]);
+        const R dc = KMath<R>::sqrt_(std::fmax(ck * ck - rs2, R(0)));
+        if (dc > R(0)) off += gen_pyramid_G_jump(K, k, da, db, dc, Phi);
         E[k] = off;  // region k + 1
     }
 }
 
 // W at the lattice point with absolute offsets (ax, ay, az) > 0 (units of H), from scratch (tests).
-KIWI_GHD double gen_lattice_W(const GenTab& K, double ax, double ay, double az) {
-    const double d[3] = {ax, ay, az};
-    const double r = std::sqrt(ax * ax + ay * ay + az * az);
+template <typename R>
+KIWI_GHD R gen_lattice_W(const GenTab& K, R ax, R ay, R az) {
+    using M = KMath<R>;
+    const R d[3] = {ax, ay, az};
+    const R r = M::sqrt_(ax * ax + ay * ay + az * az);
     const int k = gen_region_of(K, r);
-    double W = 0.0;
+    const GenRegionT<R>* G = gen_regions<R>(K);
+    R W = R(0);
     for (int a = 0; a < 3; ++a) {
-        double Phi[GEN_NREG];
+        R Phi[GEN_NREG];
         gen_plane_phi(K, d[a], Phi);
-        W += 0.5 * GEN_PI * Phi[k - 1];
+        W += R(0.5 * GEN_PI) * Phi[k - 1];
         for (int b = 0; b < 3; ++b) {
             if (b == a) continue;
             const int c = 3 - a - b;
-            const double inv_l2 = 1.0 / (d[a] * d[a] + d[b] * d[b]);
-            const double T = std::atan(d[a] * d[c] / (d[b] * r));
-            const double Kh = 0.5 * std::log((r + d[c]) * (r + d[c]) * inv_l2);
-            W += gen_pyramid_nonphi(K.reg[k - 1], d[a], d[b], d[c], r, T, Kh, 1.0 / d[a], 1.0 / d[b], 1.0 / r, inv_l2);
-            double E[GEN_NREG];
+            const R inv_l2 = M::rcp(d[a] * d[a] + d[b] * d[b]);
+            const R T = M::atan_pos(d[a] * d[c] * M::rcp(d[b]) * M::rcp(r));
+            const R Kh = R(0.5) * M::log_((r + d[c]) * (r + d[c]) * inv_l2);
+            W += gen_pyramid_nonphi(G[k - 1], d[a], d[b], d[c], r, T, Kh, M::rcp(r));
+            R E[GEN_NREG];
             gen_line(K, d[a], d[b], E);
             W += E[k - 1];
         }

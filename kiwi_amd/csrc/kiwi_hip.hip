@@ -292,12 +292,14 @@ struct PolicyM4 {
     static __device__ __forceinline__ R plane_weight() { return R(0.5); }
 };
 
-// Generic tabulated-coefficient kernels (gen_lattice.h), double precision: lattice offsets in units of
-// the support H, support 1; regions of the kernel table; plane values Phi_k; the line table holds
-// the offsets of every region.
-
+// Generic tabulated-coefficient kernels (gen_lattice.h): lattice offsets in units of the support H,
+// support 1; regions of the kernel table; plane values Phi_k; the line table holds the offsets of
+// every region.  R = double is the parity mode; R = float (mixed) is used for the kernels whose
+// single-precision lattice values are as accurate as the cubic spline's (gen_mixed_ok), with the plane
+// values still formed in double and rounded.
+template <typename R>
 struct PolicyGen {
-    using Real = double;
+    using Real = R;
     static constexpr int NPL = GEN_NREG;
     static constexpr int NLN = GEN_NREG;
     static constexpr bool HAS_LINES = true;
@@ -307,32 +309,23 @@ struct PolicyGen {
         const GenTab* tab;
     };
     static __device__ __forceinline__ double unit_in_h(const Ctx& c) { return c.support_in_h; }  // lattice unit: H
-    static __device__ __forceinline__ double support2() { return 1.0; }
-    static __device__ __forceinline__ int region(const Ctx& c, double r2) { return gen_region_of(*c.tab, sqrt(r2)); }
-    static __device__ __forceinline__ void plane(const Ctx& c, double ad, double* v) { gen_plane_phi(*c.tab, ad, v); }
-    static __device__ __forceinline__ int line_slot(int k) { return k - 1; }
-    static __device__ __forceinline__ void line(const Ctx& c, double da, double db, double* v) { gen_line(*c.tab, da, db, v); }
-    static __device__ __forceinline__ int line_kind(double l2) { return l2 < 1.0 ? 1 : 0; }
-    static __device__ __forceinline__ double free_part(const Ctx& c, double ax, double ay, double az, double r, int k) {
-        const GenRegion& R = c.tab->reg[k - 1];
-        // seven reciprocals shared by the 6 pyramids: offsets, radius, squared distances to the axes
-        const double ix = 1.0 / ax, iy = 1.0 / ay, iz = 1.0 / az, ir = 1.0 / r;
-        const double ixy = 1.0 / (ax * ax + ay * ay), ixz = 1.0 / (ax * ax + az * az), iyz = 1.0 / (ay * ay + az * az);
-        // T_b for edge normal b, atanh(d_c / r) for along-edge axis c; the 6 ordered (a, b) pyramids
-        const double T0 = atan(ay * az * ix * ir), T1 = atan(ax * az * iy * ir), T2 = atan(ax * ay * iz * ir);
-        const double K0 = 0.5 * log((r + ax) * (r + ax) * iyz);
-        const double K1 = 0.5 * log((r + ay) * (r + ay) * ixz);
-        const double K2 = 0.5 * log((r + az) * (r + az) * ixy);
-        double s = 0.0;
-        s += gen_pyramid_nonphi(R, ax, ay, az, r, T1, K2, ix, iy, ir, ixy);  // (x, y, z)
-        s += gen_pyramid_nonphi(R, ax, az, ay, r, T2, K1, ix, iz, ir, ixz);  // (x, z, y)
-        s += gen_pyramid_nonphi(R, ay, ax, az, r, T0, K2, iy, ix, ir, ixy);  // (y, x, z)
-        s += gen_pyramid_nonphi(R, ay, az, ax, r, T2, K0, iy, iz, ir, iyz);  // (y, z, x)
-        s += gen_pyramid_nonphi(R, az, ax, ay, r, T0, K1, iz, ix, ir, ixz);  // (z, x, y)
-        s += gen_pyramid_nonphi(R, az, ay, ax, r, T1, K0, iz, iy, ir, iyz);  // (z, y, x)
-        return s;
+    static __device__ __forceinline__ R support2() { return R(1); }
+    static __device__ __forceinline__ int region(const Ctx& c, R r2) {
+        return gen_region_of(*c.tab, static_cast<double>(KMath<R>::sqrt_(r2)));
     }
-    static __device__ __forceinline__ double plane_weight() { return 0.5 * GEN_PI; }
+    static __device__ __forceinline__ void plane(const Ctx& c, R ad, R* v) {
+        double Phi[GEN_NREG];
+        gen_plane_phi(*c.tab, static_cast<double>(ad), Phi);
+#pragma unroll
+        for (int k = 0; k < GEN_NREG; ++k) v[k] = static_cast<R>(Phi[k]);
+    }
+    static __device__ __forceinline__ int line_slot(int k) { return k - 1; }
+    static __device__ __forceinline__ void line(const Ctx& c, R da, R db, R* v) { gen_line<R>(*c.tab, da, db, v); }
+    static __device__ __forceinline__ int line_kind(double l2) { return l2 < 1.0 ? 1 : 0; }
+    static __device__ __forceinline__ R free_part(const Ctx& c, R ax, R ay, R az, R r, int k) {
+        return gen_lattice_free<R>(gen_regions<R>(*c.tab)[k - 1], ax, ay, az, r);
+    }
+    static __device__ __forceinline__ R plane_weight() { return R(0.5 * GEN_PI); }
 };
 
 // Tabulated cubic spline (m4_table.h): the lattice value is the plain sum of the 6 tabulated vertex
@@ -985,8 +978,8 @@ int check_problem(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_grid* G, 
 
 // problems that can share one pipeline run: the same kernel evaluation and arithmetic
 bool same_path(const kiwi_batch_problem& a, const KernelChoice& ka, const kiwi_batch_problem& b, const KernelChoice& kb) {
-    const bool fa = ka.generic || ka.tabulated || a.precision == KIWI_FP64;
-    const bool fb = kb.generic || kb.tabulated || b.precision == KIWI_FP64;
+    const bool fa = ka.tabulated || a.precision == KIWI_FP64 || (ka.generic && !gen_mixed_ok(a.kernel.kernel_id));
+    const bool fb = kb.tabulated || b.precision == KIWI_FP64 || (kb.generic && !gen_mixed_ok(b.kernel.kernel_id));
     return ka.generic == kb.generic && ka.tabulated == kb.tabulated && ka.support_in_h == kb.support_in_h &&
            (!ka.generic || a.kernel.kernel_id == b.kernel.kernel_id) && fa == fb;
 }
@@ -1126,12 +1119,13 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     if (npairs > 0x7fffffffLL)
         return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 (particle, cell-block) items in one call");
 
-    // the generic and tabulated paths always compute in double precision (mixed applies to the cubic path)
+    // mixed precision applies to the cubic path and to the generic kernels gen_mixed_ok names; the
+    // tabulated path and the other generic kernels always compute in double precision
     const bool tabulated = kc.tabulated;
-    const bool fp64 = generic || tabulated || precision == KIWI_FP64;
+    const bool fp64 = tabulated || precision == KIWI_FP64 || (generic && !gen_mixed_ok(K->kernel_id));
     const size_t rsz = fp64 ? sizeof(double) : sizeof(float);
     const size_t line_entry = tabulated ? 0
-                              : generic ? sizeof(LineVals<double, PolicyGen::NLN>)
+                              : generic ? (fp64 ? sizeof(LineVals<double, GEN_NREG>) : sizeof(LineVals<float, GEN_NREG>))
                                         : (fp64 ? sizeof(LineVals<double, 2>) : sizeof(LineVals<float, 2>));
     const int64_t np1 = npairs + 1;
     KIWI_HIP_CHECK(c, reserve(c->lines, line_entry * static_cast<size_t>(std::max<int64_t>(nlines_total, 1))));
@@ -1206,12 +1200,18 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
                                                   slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob, part_prob,
                                                   c->lines.p, c->ms.p, dst, cursor, clamp, c->ev[2], c->ev[3],
                                                   c->ev[4]));
+    } else if (generic && fp64) {
+        const PolicyGen<double>::Ctx pc{sup, static_cast<const GenTab*>(c->tables.p) + K->kernel_id};
+        KIWI_HIP_CHECK(c, launch_cells<PolicyGen<double>>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
+                                                          slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
+                                                          part_prob, c->lines.p, c->ms.p, dst, cursor, clamp,
+                                                          c->ev[2], c->ev[3], c->ev[4]));
     } else if (generic) {
-        const PolicyGen::Ctx pc{sup, static_cast<const GenTab*>(c->tables.p) + K->kernel_id};
-        KIWI_HIP_CHECK(c, launch_cells<PolicyGen>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
-                                                  slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob, part_prob,
-                                                  c->lines.p, c->ms.p, dst, cursor, clamp, c->ev[2], c->ev[3],
-                                                  c->ev[4]));
+        const PolicyGen<float>::Ctx pc{sup, static_cast<const GenTab*>(c->tables.p) + K->kernel_id};
+        KIWI_HIP_CHECK(c, launch_cells<PolicyGen<float>>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
+                                                         slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
+                                                         part_prob, c->lines.p, c->ms.p, dst, cursor, clamp,
+                                                         c->ev[2], c->ev[3], c->ev[4]));
     } else if (fp64) {
         KIWI_HIP_CHECK(c, launch_cells<PolicyM4<double>>(c, s, {sup}, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
                                                          slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,

@@ -112,6 +112,10 @@ double lat_gen_box(int kid, const double* p, double H, const double* lo, const d
 }  // extern "C"
 
 extern "C" double lat_gen_W(int kid, double ax, double ay, double az) { return gen_lattice_W(kTabs[kid], ax, ay, az); }
+// the same lattice value in single precision (the mixed-precision generic path's arithmetic)
+extern "C" double lat_gen_W32(int kid, double ax, double ay, double az) {
+    return gen_lattice_W<float>(kTabs[kid], static_cast<float>(ax), static_cast<float>(ay), static_cast<float>(az));
+}
 
 // ---- tabulated cubic path (m4_table.h) ----
 #include <vector>

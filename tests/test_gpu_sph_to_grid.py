@@ -201,9 +201,32 @@ def test_generic_kernels_match_reference_cli(hip_ctx, label, name, support, gene
         assert rel_max(dens * v * V, ref_d * rv * V) <= 1e-10  # momentum
         w = ref_d / ref_d.max()
         assert np.max(np.abs(v - rv) * w) <= 1e-10 * np.max(np.abs(rv))
-    # the generic path computes in double precision whatever the request
     m2 = hip_ctx.sph_to_grid(parts, dom, res, kernel=k, precision=hip.MIXED)
-    assert np.array_equal(m2[0], mass)
+    if name in MIXED_GENERIC:  # single-precision lattice values
+        err_tot = abs(m2[0].sum() - mass.sum()) / mass.sum()
+        say(f"  mixed: total mass rel err {err_tot:.2e}, mass max err / max {rel_max(m2[0], mass):.2e}")
+        assert err_tot <= 1e-6 and rel_max(m2[0], mass) <= 1e-4
+    else:  # the higher-order generic kernels compute in double precision whatever the request
+        assert np.array_equal(m2[0], mass)
+
+
+MIXED_GENERIC = ("M4", "WC2")
+
+
+@pytest.mark.parametrize("name,generic", [("WC2", False), ("M4", True)])
+def test_generic_mixed_precision_total_mass(hip_ctx, name, generic):
+    say(f"case: generic {name} mixed vs FP64, 50,000 log-normal particles -> 64^3, total mass <= 1e-6")
+    parts, dom, res = workloads.lognormal_field(n_particles=50000, n=64, seed=5)
+    k = hip.kernel_spec(name, generic_path=generic)
+    f = hip_ctx.sph_to_grid(parts, dom, res, kernel=k, precision=hip.FP64)
+    m = hip_ctx.sph_to_grid(parts, dom, res, kernel=k, precision=hip.MIXED)
+    err = abs(m[0].sum() - f[0].sum()) / f[0].sum()
+    fields = [rel_max(x, y) for x, y in zip(m[:4], f[:4])]
+    say(f"  total mass rel err {err:.3e}; per-field max err / field max: " + ", ".join(f"{e:.2e}" for e in fields))
+    assert err <= 1e-6
+    assert max(fields) <= 1e-5
+    # the single-precision sphere test may decide a cell at the support boundary (mass ~0) differently
+    assert abs(m[4].pairs_sphere - f[4].pairs_sphere) <= 1e-6 * f[4].pairs_sphere
 
 
 @pytest.mark.parametrize("name", ["M6", "WC6"])

@@ -193,6 +193,62 @@ def test_generic_lattice_boxes_equal_the_generic_face_walk(lat, name):
     assert abs(_gen_box(lat, kid, p, H, p - 1.3 * H, p + 1.2 * H) - 1.0) < 1e-13
 
 
+@pytest.mark.parametrize("name", ["M4", "WC2"])
+def test_generic_single_precision_lattice(lat, name):
+    """The mixed-precision generic path (generic cubic spline and Wendland C2: gen_lattice.h in float)
+    against the double-precision lattice: values finite and within 2e-6 of the double ones at random
+    offsets, including offsets of 1e-6..1e-12 of the support on one or two axes (the basis recurrences
+    overflowed single precision there before they were scaled); and, for an isolated particle at
+    h = 0.8 and 2.5 cells, cell sums of the single-precision lattice (summed in double, as on the
+    device) within 1e-6 of the particle's mass in total, 1e-3 in L1, and 2e-4 of the largest cell."""
+    for f in (lat.lat_gen_W, lat.lat_gen_W32):
+        f.restype = ctypes.c_double
+        f.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    kid = KERNELS.index(name)
+    rng = np.random.default_rng(31 + kid)
+    worst = 0.0
+    for i in range(3000):
+        d = rng.uniform(0.0, 1.2, 3)
+        if i % 3 == 1:
+            d[i % 3] = 10.0 ** rng.uniform(-12, -6)
+        if i % 3 == 2:
+            d[:2] = 10.0 ** rng.uniform(-12, -6, 2)
+        if np.linalg.norm(d) < 1e-3:
+            continue
+        w64, w32 = lat.lat_gen_W(kid, *d), lat.lat_gen_W32(kid, *d)
+        assert np.isfinite(w32), d
+        worst = max(worst, abs(w32 - w64))
+    # lattice values are sums of O(1) plane, line and pyramid terms (which cancel to ~0 outside the
+    # support): the difference is a few single-precision ulps of those terms
+    assert worst < 2e-6, worst
+
+    def cells(W, p, hc):
+        H = RADKERN[name] * hc
+        n = int(np.ceil(2 * H)) + 3
+        lo = np.floor(p - H) - 1
+        g = [lo[a] + np.arange(n + 1) for a in range(3)]
+        U = np.zeros((n + 1,) * 3)
+        for i in range(n + 1):
+            for j in range(n + 1):
+                for k in range(n + 1):
+                    dx, dy, dz = (g[0][i] - p[0]) / H, (g[1][j] - p[1]) / H, (g[2][k] - p[2]) / H
+                    if dx == 0 or dy == 0 or dz == 0:
+                        continue
+                    w = W(kid, abs(dx), abs(dy), abs(dz))
+                    U[i, j, k] = -w if (dx < 0) != ((dy < 0) != (dz < 0)) else w
+        return U[1:, 1:, 1:] + U[1:, :-1, :-1] + U[:-1, 1:, :-1] + U[:-1, :-1, 1:] \
+            - U[:-1, :-1, :-1] - U[1:, 1:, :-1] - U[1:, :-1, 1:] - U[:-1, 1:, 1:]
+
+    for hc in (0.8, 2.5):
+        p = rng.uniform(0, 1, 3)
+        c64, c32 = cells(lat.lat_gen_W, p, hc), cells(lat.lat_gen_W32, p, hc)
+        M = c64.sum()
+        assert abs(M - 1.0) < 1e-12
+        assert abs(c32.sum() - M) < 1e-6
+        assert np.abs(c32 - c64).sum() < 1e-3
+        assert np.abs(c32 - c64).max() < 2e-4 * np.abs(c64).max()
+
+
 # ---------------------------------------------------------------------- tabulated cubic path (m4_table.h)
 def test_vertex_value_and_gridded_lookup_match_reference_golden(lat):
     """m4_table.h: the FP64 vertex integral that fills the lookup table, against the reference's
