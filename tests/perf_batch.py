@@ -3,13 +3,13 @@ collected by pytest.
 
 Workload (kiwi_amd.workloads.cutout_batch): 64 independent cutouts, particle counts log-uniform in
 [1e3, 2e5], the largest grid axis log-uniform in [32, 192] cells and the others 1 to 4 times shorter,
-log-normal density (sigma_ln_rho 1.5), kernels alternating between the cubic spline in mixed precision
-and Wendland C2 (generic path, FP64).  Particles and outputs resident on the device.
+log-normal density (sigma_ln_rho 1.5), kernels alternating between the cubic spline and Wendland C2
+(generic path), FP32 bulk (mixed precision) for both.  Particles and outputs resident on the device.
 
 Times the batch (one call) against the same problems as 64 single calls, checks that both give
 bit-identical sums, and prints one JSON line.
 
-    python tests/perf_batch.py [--problems 64] [--steps 3]
+    python tests/perf_batch.py [--problems 64] [--steps 3] [--fp64]
 """
 import argparse
 import json
@@ -33,6 +33,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--problems", type=int, default=64)
     ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--fp64", action="store_true", help="every problem in FP64 (default: mixed precision)")
     args = ap.parse_args()
     import torch
     torch.cuda.init()
@@ -46,7 +47,7 @@ def main():
         nc = int(np.prod(res))
         probs.append({"parts": {k: torch.from_numpy(np.ascontiguousarray(parts[k])).to(dev) for k in workloads.COLS},
                       "domain": dom, "res": res, "kernel": hip.kernel_spec(kname),
-                      "precision": hip.MIXED if kname == "M4" else hip.FP64,
+                      "precision": hip.FP64 if args.fp64 else hip.MIXED,
                       "out": tuple(torch.zeros(nc, dtype=torch.float64, device=dev) for _ in range(4))})
     n_parts = sum(len(b[0]["x"]) for b in batch)
     n_cells = sum(int(np.prod(b[2])) for b in batch)
@@ -77,7 +78,8 @@ def main():
     identical = all(bool(torch.equal(a, b)) for q, bo in zip(probs, batch_out) for a, b in zip(q["out"], bo))
     pairs = sum(r[4].pairs_sphere for r in rb)
     pairs_bbox = sum(r[4].pairs_bbox for r in rb)
-    res = {"workload": f"{len(probs)} cutouts, {n_parts} particles, {n_cells} cells, M4 mixed / WC2 FP64",
+    prec = "FP64" if args.fp64 else "mixed"
+    res = {"workload": f"{len(probs)} cutouts, {n_parts} particles, {n_cells} cells, M4 / WC2 {prec}",
            "ms_batch": ms_batch, "ms_single_calls": ms_single, "speedup": ms_single / ms_batch,
            "pairs_sphere": pairs, "pairs_bbox": pairs_bbox,
            "cell_integrals_per_s_batch": pairs / (ms_batch * 1e-3),
