@@ -207,7 +207,10 @@ int kiwi_hip_sph_to_grid_adjoint(kiwi_hip_ctx* ctx, const kiwi_particles* partic
 /* Voronoi map, replacing the particle loop of run_mapping_stage (src/mapping.cpp:400-460).
  * home_cell[n]: each particle's cell (voro++ find_voronoi_cell resolved through reverse_id_list,
  * :434-440), -1 if unresolved; host or device memory as particles->on_device.  Outputs n_cells each.
- * The face integrals run in double precision whatever `precision` asks for. */
+ * KIWI_MIXED evaluates the face integrals of the analytic cubic spline in FP32 (vertex coordinates taken
+ * relative to the particle), with the visit test and all sums in FP64, and clamps negative cell totals
+ * instead of negative (particle, cell) sums (no per-pair warnings); the other kernel paths run in FP64
+ * whatever `precision` asks for. */
 int kiwi_hip_voronoi_map(kiwi_hip_ctx* ctx, const kiwi_particles* particles, const int* home_cell,
                          const kiwi_voronoi_cells* cells, const kiwi_kernel* kernel, int precision,
                          const kiwi_cell_sums* out, kiwi_map_stats* stats);

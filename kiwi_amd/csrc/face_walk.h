@@ -16,10 +16,13 @@
 // reference's acos loses up to half the digits when R0 / r12 is within rounding of 1.
 //
 // Ev supplies the vertex integral: wall(|r0|) once per face, edge(R0) once per edge, at(phi, t) per
-// vertex.  Double precision; host (tests) and device code.
+// vertex.  Double precision (the parity mode), or single precision on vertex coordinates taken
+// relative to the particle in double and rounded (the mixed mode of the cubic spline); host (tests)
+// and device code.
 #pragma once
 
 #include <cmath>
+#include <type_traits>
 
 #include "gen_lattice.h"
 #include "m4_lattice.h"
@@ -34,30 +37,33 @@
 namespace kiwi_hip {
 
 // vertex integral of the analytic cubic spline, at h (physical units in, mass fraction out)
-struct EvM4 {
-    double inv_h;
-    double da = 0.0, db = 0.0, D2 = 0.0, D3 = 0.0;
-    M4Plane<double> pl{};
-    KIWI_FHD void wall(double ar0) {
+template <typename R>
+struct EvM4T {
+    R inv_h;
+    R da = R(0), db = R(0), D2 = R(0), D3 = R(0);
+    M4Plane<R> pl{};
+    KIWI_FHD void wall(R ar0) {
         da = ar0 * inv_h;
-        pl = m4_plane<double>(da);
+        pl = m4_plane<R>(da);
     }
-    KIWI_FHD void edge(double R0) {
+    KIWI_FHD void edge(R R0) {
         db = R0 * inv_h;
-        m4_line<double>(da, db, D2, D3);
+        m4_line<R>(da, db, D2, D3);
     }
-    KIWI_FHD double at(double phi, double t) const {
-        if (!(da > 0.0) || !(db > 0.0) || phi == 0.0) return 0.0;
-        const double dc = t * inv_h;  // = db tan(phi)
-        const double r2 = da * da + db * db + dc * dc;
-        const double r = sqrt(r2);
+    KIWI_FHD R at(R phi, R t) const {
+        using M = KMath<R>;
+        if (!(da > R(0)) || !(db > R(0)) || phi == R(0)) return R(0);
+        const R dc = t * inv_h;  // = db tan(phi)
+        const R r2 = da * da + db * db + dc * dc;
+        const R r = M::sqrt_(r2);
         const int k = m4_region(r2);
-        const double T = atan(da * dc / (db * r));
-        const double L = log((r + dc) * (r + dc) / (da * da + db * db));
-        const double D = (k == 1) ? 0.0 : (k == 2 ? D2 : D3);
-        return m4_vertex_g<double>(k, da, db, dc, r, T, L, phi, pl) + D;
+        const R T = M::atan_pos(M::div(da * dc, db * r));
+        const R L = M::log_(M::div((r + dc) * (r + dc), da * da + db * db));
+        const R D = (k == 1) ? R(0) : (k == 2 ? D2 : D3);
+        return m4_vertex_g<R>(k, da, db, dc, r, T, L, phi, pl) + D;
     }
 };
+using EvM4 = EvM4T<double>;
 
 // vertex integral of a generic kernel with support H
 template <class Tab>
@@ -108,51 +114,104 @@ struct EvTab {
     }
 };
 
-// The face integral of the polygon v[0..n) (xyz triples) for a particle at (x0, y0, z0) with kernel
-// support radius `support`; *reachable is set when the face's closest point lies within the support.
-template <class Ev>
-KIWI_FHD double face_walk_integral(double x0, double y0, double z0, double support, const double* v, int n, Ev& ev,
-                                   bool* reachable) {
+// Vertices of a face: absolute coordinates in double (the parity mode), or coordinates relative to the
+// particle, formed in double and rounded to R (the particle then sits at the origin).
+struct AbsVerts {
+    const double* v;
+    KIWI_FHD void get(int j, double& x, double& y, double& z) const {
+        x = v[3 * j];
+        y = v[3 * j + 1];
+        z = v[3 * j + 2];
+    }
+};
+template <typename R>
+struct RelVerts {
+    const double* v;
+    double x0, y0, z0;
+    KIWI_FHD void get(int j, R& x, R& y, R& z) const {
+        x = static_cast<R>(v[3 * j] - x0);
+        y = static_cast<R>(v[3 * j + 1] - y0);
+        z = static_cast<R>(v[3 * j + 2] - z0);
+    }
+};
+
+// The face integral of the polygon of n vertices (accessor vs) for a particle at (x0, y0, z0) with
+// kernel support radius `support`; *reachable is set when the face's closest point lies within the
+// support.
+//
+// The double-precision walk takes the face plane through the first three vertices, as the reference
+// does (:495-506).  Voronoi faces can have two nearly coincident vertices (edges 1e-4 of the cell), and
+// a plane from the first three is then as ill-conditioned as that edge is short: harmless in double
+// (1e-13) but 1e-4 of r0 in single precision.  The single-precision walk therefore takes Newell's normal
+// (the sum over all edges, which no short edge spoils) through the vertex centroid: the same plane for a
+// planar convex polygon in exact arithmetic.  With the normal of the first three vertices the winding
+// reference s2 equals -r0 |N| exactly, so the edge sign depends on s1 alone; the single-precision walk
+// uses that form.
+template <typename R, class V, class Ev>
+KIWI_FHD R face_walk_t(R x0, R y0, R z0, R support, const V& vs, int n, Ev& ev, bool* reachable) {
+    using M = KMath<R>;
+    constexpr bool NEWELL = !std::is_same<R, double>::value;
     *reachable = false;
-    if (n < 3) return 0.0;
-    // plane through the first three vertices (:495-506)
-    const double x1 = v[0], y1 = v[1], z1 = v[2];
-    const double x2s = v[3], y2s = v[4], z2s = v[5];
-    const double x3s = v[6], y3s = v[7], z3s = v[8];
-    const double A = (y2s - y1) * (z3s - z1) - (y3s - y1) * (z2s - z1);
-    const double B = (z2s - z1) * (x3s - x1) - (z3s - z1) * (x2s - x1);
-    const double C = (x2s - x1) * (y3s - y1) - (x3s - x1) * (y2s - y1);
-    const double D = -A * x1 - B * y1 - C * z1;
-    const double nrm = sqrt(A * A + B * B + C * C);
-    const double r0 = (A * x0 + B * y0 + C * z0 + D) / nrm;
-    const double ar0 = fabs(r0);
-    const double xp = x0 - r0 * A / nrm, yp = y0 - r0 * B / nrm, zp = z0 - r0 * C / nrm;
+    if (n < 3) return R(0);
+    R x1, y1, z1, x2s, y2s, z2s, x3s, y3s, z3s;
+    vs.get(0, x1, y1, z1);
+    vs.get(1, x2s, y2s, z2s);
+    vs.get(2, x3s, y3s, z3s);
+    R A, B, C, D;
+    if (NEWELL) {
+        R gx = R(0), gy = R(0), gz = R(0);
+        A = B = C = R(0);
+        for (int j = 0; j < n; ++j) {
+            R xa, ya, za, xb, yb, zb;
+            vs.get(j, xa, ya, za);
+            vs.get(j == n - 1 ? 0 : j + 1, xb, yb, zb);
+            A += (ya - yb) * (za + zb);
+            B += (za - zb) * (xa + xb);
+            C += (xa - xb) * (ya + yb);
+            gx += xa;
+            gy += ya;
+            gz += za;
+        }
+        const R inv_n = R(1) / static_cast<R>(n);
+        D = -(A * gx + B * gy + C * gz) * inv_n;
+    } else {
+        // plane through the first three vertices (:495-506)
+        A = (y2s - y1) * (z3s - z1) - (y3s - y1) * (z2s - z1);
+        B = (z2s - z1) * (x3s - x1) - (z3s - z1) * (x2s - x1);
+        C = (x2s - x1) * (y3s - y1) - (x3s - x1) * (y2s - y1);
+        D = -A * x1 - B * y1 - C * z1;
+    }
+    const R nrm = M::sqrt_(A * A + B * B + C * C);
+    const R r0 = M::div(A * x0 + B * y0 + C * z0 + D, nrm);
+    const R ar0 = fabs(r0);
+    const R xp = x0 - M::div(r0 * A, nrm), yp = y0 - M::div(r0 * B, nrm), zp = z0 - M::div(r0 * C, nrm);
     // winding sign reference: triple product about the particle (:525-529)
-    const double ax = x1 - x0, ay = y1 - y0, az = z1 - z0;
-    const double bx = x2s - x0, by = y2s - y0, bz = z2s - z0;
-    const double cx = x3s - x0, cy = y3s - y0, cz = z3s - z0;
-    const double s2 = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
+    const R ax = x1 - x0, ay = y1 - y0, az = z1 - z0;
+    const R bx = x2s - x0, by = y2s - y0, bz = z2s - z0;
+    const R cx = x3s - x0, cy = y3s - y0, cz = z3s - z0;
+    const R s2 = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
     ev.wall(ar0);
 
     bool inside = true, have_sign = false;
-    double sign_ref = 0.0, min_edge = INFINITY, total = 0.0;
+    R sign_ref = R(0), min_edge = R(INFINITY), total = R(0);
     for (int j = 0; j < n; ++j) {
         const int jn = (j == n - 1) ? 0 : j + 1;
-        const double x2 = v[3 * j], y2 = v[3 * j + 1], z2 = v[3 * j + 2];
-        const double x3 = v[3 * jn], y3 = v[3 * jn + 1], z3 = v[3 * jn + 2];
+        R x2, y2, z2, x3, y3, z3;
+        vs.get(j, x2, y2, z2);
+        vs.get(jn, x3, y3, z3);
         // edge geometry (:573-594): R0 is the foot's distance from the edge line, sv the signed distance
         // along the edge from vertex j to the line's own foot point, so vertex j lies |sv| and vertex
         // j+1 lies |r23 - sv| from it; a degenerate edge has R0 = 0 (no contribution)
-        const double ex = x3 - x2, ey = y3 - y2, ez = z3 - z2;
-        const double tx = xp - x2, ty = yp - y2, tz = zp - z2;
-        const double r23 = sqrt(ex * ex + ey * ey + ez * ez);
-        const double r12 = sqrt(tx * tx + ty * ty + tz * tz);
-        const double r13 = sqrt((xp - x3) * (xp - x3) + (yp - y3) * (yp - y3) + (zp - z3) * (zp - z3));
-        const double cxx = ey * tz - ez * ty, cxy = ez * tx - ex * tz, cxz = ex * ty - ey * tx;
-        double R0 = 0.0, t2 = 0.0, t3 = 0.0;
-        if (r23 > 0.0) {
-            R0 = sqrt(cxx * cxx + cxy * cxy + cxz * cxz) / r23;
-            const double sv = (ex * tx + ey * ty + ez * tz) / r23;
+        const R ex = x3 - x2, ey = y3 - y2, ez = z3 - z2;
+        const R tx = xp - x2, ty = yp - y2, tz = zp - z2;
+        const R r23 = M::sqrt_(ex * ex + ey * ey + ez * ez);
+        const R r12 = M::sqrt_(tx * tx + ty * ty + tz * tz);
+        const R r13 = M::sqrt_((xp - x3) * (xp - x3) + (yp - y3) * (yp - y3) + (zp - z3) * (zp - z3));
+        const R cxx = ey * tz - ez * ty, cxy = ez * tx - ex * tz, cxz = ex * ty - ey * tx;
+        R R0 = R(0), t2 = R(0), t3 = R(0);
+        if (r23 > R(0)) {
+            R0 = M::div(M::sqrt_(cxx * cxx + cxy * cxy + cxz * cxz), r23);
+            const R sv = M::div(ex * tx + ey * ty + ez * tz, r23);
             t2 = fabs(sv);
             t3 = fabs(r23 - sv);
         } else {
@@ -160,45 +219,52 @@ KIWI_FHD double face_walk_integral(double x0, double y0, double z0, double suppo
             t3 = r13;
         }
         // winding sign about the particle (:588-591)
-        const double px = xp - x0, py = yp - y0, pz = zp - z0;
-        const double ux = x2 - x0, uy = y2 - y0, uz = z2 - z0;
-        const double wx = x3 - x0, wy = y3 - y0, wz = z3 - z0;
-        const double s1 = px * (uy * wz - uz * wy) + py * (uz * wx - ux * wz) + pz * (ux * wy - uy * wx);
-        const double phi1 = (t2 > 0.0) ? atan2(t2, R0) : 0.0;
-        const double phi2 = (t3 > 0.0) ? atan2(t3, R0) : 0.0;
+        const R px = xp - x0, py = yp - y0, pz = zp - z0;
+        const R ux = x2 - x0, uy = y2 - y0, uz = z2 - z0;
+        const R wx = x3 - x0, wy = y3 - y0, wz = z3 - z0;
+        const R s1 = px * (uy * wz - uz * wy) + py * (uz * wx - ux * wz) + pz * (ux * wy - uy * wx);
+        const R phi1 = (t2 > R(0)) ? M::atan2_pos(t2, R0) : R(0);
+        const R phi2 = (t3 > R(0)) ? M::atan2_pos(t3, R0) : R(0);
         // is the foot inside the polygon (:600-611)
-        const double side = cxx * A + cxy * B + cxz * C;
-        const double side_eps = 1e-9 * nrm * r23 * fmax(fmax(fabs(tx), fabs(ty)), fmax(fabs(tz), 1.0));
+        const R side = cxx * A + cxy * B + cxz * C;
+        const R side_eps = R(1e-9) * nrm * r23 * fmax(fmax(fabs(tx), fabs(ty)), fmax(fabs(tz), R(1)));
         if (fabs(side) > side_eps) {
             if (!have_sign) {
                 sign_ref = side;
                 have_sign = true;
-            } else if ((side > 0.0) != (sign_ref > 0.0)) {
+            } else if ((side > R(0)) != (sign_ref > R(0))) {
                 inside = false;
             }
         }
         // the edge's perpendicular foot beyond the segment: the two vertex integrals subtract (:613-628)
         const bool beyond = (t2 >= r23) || (t3 >= r23);  // r12 sin(phi1) >= r23 || r13 sin(phi2) >= r23
         min_edge = fmin(min_edge, beyond ? fmin(r12, r13) : R0);
-        const double M = (s1 * s2 * r0 <= 0.0) ? -1.0 : 1.0;
+        const R Msg = NEWELL ? ((s1 >= R(0) || r0 == R(0)) ? R(-1) : R(1)) : ((s1 * s2 * r0 <= R(0)) ? R(-1) : R(1));
         ev.edge(R0);
         // one call site for the two vertex integrals (the evaluators inline large)
-        double I1 = 0.0, I2 = 0.0;
+        R I1 = R(0), I2 = R(0);
 #pragma unroll 1
         for (int e = 0; e < 2; ++e) {
-            const double I = ev.at(e ? phi2 : phi1, e ? t3 : t2);
+            const R I = ev.at(e ? phi2 : phi1, e ? t3 : t2);
             if (e) I2 = I;
             else I1 = I;
         }
         if (beyond)
-            total += (phi1 >= phi2) ? M * (I1 - I2) : M * (I2 - I1);
+            total += (phi1 >= phi2) ? Msg * (I1 - I2) : Msg * (I2 - I1);
         else
-            total += M * (I1 + I2);
+            total += Msg * (I1 + I2);
     }
     // closest point of the face (:642-645); a NaN distance counts as reachable, as there
-    const double fd = inside ? ar0 : sqrt(r0 * r0 + min_edge * min_edge);
+    const R fd = inside ? ar0 : M::sqrt_(r0 * r0 + min_edge * min_edge);
     *reachable = !(fd >= support);
     return total;
+}
+
+// The face integral in double precision of the polygon v[0..n) (absolute xyz triples).
+template <class Ev>
+KIWI_FHD double face_walk_integral(double x0, double y0, double z0, double support, const double* v, int n, Ev& ev,
+                                   bool* reachable) {
+    return face_walk_t<double>(x0, y0, z0, support, AbsVerts{v}, n, ev, reachable);
 }
 
 }  // namespace kiwi_hip

@@ -23,7 +23,9 @@
 //   7. k_vor_pairsum    per-pair cell integral in the cell's face order, clamped at 0
 //   8. accumulation     pairs sorted by (cell, particle); one owner per cell sums in particle order
 //
-// Everything is FP64; sums are deterministic (no floating-point atomics).
+// Everything is FP64 except, for a mixed-precision request with the analytic cubic spline, the face
+// integrals (single precision on vertex coordinates relative to the particle); the visit test, the
+// per-pair and per-cell sums stay FP64, and the sums are deterministic (no floating-point atomics).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -520,16 +522,25 @@ __global__ __launch_bounds__(THREADS) void k_vor_item_bins(int64_t nitems, int64
 // ------------------------------------------------------------------------------------------------
 // 6. face integrals
 // ------------------------------------------------------------------------------------------------
+// SINGLE: the face walk runs in single precision on vertex coordinates relative to the particle
 struct EvFactoryM4 {
+    static constexpr bool SINGLE = false;
     double sup;
     __device__ EvM4 make(double h) const { return EvM4{1.0 / h}; }
 };
+struct EvFactoryM4F {
+    static constexpr bool SINGLE = true;
+    double sup;
+    __device__ EvM4T<float> make(double h) const { return EvM4T<float>{static_cast<float>(1.0 / h)}; }
+};
 struct EvFactoryGen {
+    static constexpr bool SINGLE = false;
     double sup;
     const GenTab* tab;
     __device__ EvGen<GenTab> make(double h) const { return EvGen<GenTab>{tab, 1.0 / (sup * h)}; }
 };
 struct EvFactoryTab {
+    static constexpr bool SINGLE = false;
     double sup;
     const double* table;
     __device__ EvTab make(double h) const { return EvTab{table, 1.0 / h}; }
@@ -553,9 +564,15 @@ __global__ __launch_bounds__(THREADS) void k_vor_faces(int64_t nitems, const Fac
     const double h = ph[i];
     auto ev = fac.make(h);
     const int64_t v0 = fvp[it.face];
+    const int nv = static_cast<int>(fvp[it.face + 1] - v0);
     bool r;
-    const double m = face_walk_integral(px[i], py[i], pz[i], fac.sup * h, vxyz + 3 * v0,
-                                        static_cast<int>(fvp[it.face + 1] - v0), ev, &r);
+    double m;
+    if constexpr (F::SINGLE) {
+        const RelVerts<float> vs{vxyz + 3 * v0, px[i], py[i], pz[i]};
+        m = static_cast<double>(face_walk_t<float>(0.0f, 0.0f, 0.0f, static_cast<float>(fac.sup * h), vs, nv, ev, &r));
+    } else {
+        m = face_walk_integral(px[i], py[i], pz[i], fac.sup * h, vxyz + 3 * v0, nv, ev, &r);
+    }
     val[slot_off[it.pair] + (it.face - cfp[c])] = m;
 }
 
@@ -567,7 +584,7 @@ __global__ __launch_bounds__(THREADS) void k_vor_pairsum(int64_t nvis, const uin
                                                          const int64_t* __restrict__ src,
                                                          const double* __restrict__ val, double* __restrict__ ms,
                                                          uint64_t* __restrict__ ckeys, int* __restrict__ cidx,
-                                                         MapDevStats* st) {
+                                                         MapDevStats* st, int clamp_pairs) {
     const int64_t q = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
     if (q >= nvis) return;
     double m = 0.0;
@@ -577,7 +594,9 @@ __global__ __launch_bounds__(THREADS) void k_vor_pairsum(int64_t nvis, const uin
     }
     const uint64_t key = keys[q];
     const uint64_t i = key >> cbits, c = key & ((1ull << cbits) - 1);
-    if (m < 0.0) {  // src/mapping.cpp:198-203
+    // src/mapping.cpp:198-203; mixed precision clamps cell totals instead (k_vor_accum): a per-pair clamp
+    // of single-precision rounding noise around 0 would bias the total mass upward
+    if (clamp_pairs && m < 0.0) {
         atomicAdd(&st->n_negative, 1ull);
         if (m <= -0.01) {
             atomicAdd(&st->n_negative_warn, 1ull);
@@ -614,7 +633,8 @@ __global__ __launch_bounds__(THREADS) void k_vor_accum(int64_t nc, const uint64_
                                                        const double* __restrict__ pm, const double* __restrict__ vx,
                                                        const double* __restrict__ vy, const double* __restrict__ vz,
                                                        double* __restrict__ om, double* __restrict__ ox,
-                                                       double* __restrict__ oy, double* __restrict__ oz) {
+                                                       double* __restrict__ oy, double* __restrict__ oz,
+                                                       int clamp_totals) {
     const int64_t c = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
     if (c >= nc) return;
     const uint64_t pmask = (1ull << pbits) - 1;
@@ -627,6 +647,7 @@ __global__ __launch_bounds__(THREADS) void k_vor_accum(int64_t nc, const uint64_
         d += w * vy[i];
         e += w * vz[i];
     }
+    if (clamp_totals && a < 0.0) a = b = d = e = 0.0;  // rounding noise of an (almost) empty cell
     om[c] = a;
     ox[c] = b;
     oy[c] = d;
@@ -718,6 +739,8 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* ho
     if (precision != KIWI_FP64 && precision != KIWI_MIXED) return fail(c, KIWI_ERR_INVALID_ARG, "unknown precision");
     KernelChoice kc;
     if (const int rc = resolve_kernel(c, K, &kc)) return rc;
+    // single-precision face integrals: a mixed request with the analytic cubic spline
+    const bool single = precision == KIWI_MIXED && !kc.generic && !kc.tabulated;
     if (P->n > 0x7fffffffLL || V->n_cells > 0x7fffffffLL || V->n_faces > 0x7fffffffLL)
         return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 particles, cells or faces in one call");
 
@@ -954,6 +977,10 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* ho
             const EvFactoryGen fac{sup, static_cast<const GenTab*>(c->tables.p) + K->kernel_id};
             k_vor_faces<<<grid1d(nitems), THREADS, 0, s>>>(nitems, items, vkeys, cbits, slot_off, dp[0], dp[1], dp[2],
                                                            dp[7], fac, cfp, fvp, vxyz, val);
+        } else if (single) {
+            const EvFactoryM4F fac{2.0};
+            k_vor_faces<<<grid1d(nitems), THREADS, 0, s>>>(nitems, items, vkeys, cbits, slot_off, dp[0], dp[1], dp[2],
+                                                           dp[7], fac, cfp, fvp, vxyz, val);
         } else {
             const EvFactoryM4 fac{2.0};
             k_vor_faces<<<grid1d(nitems), THREADS, 0, s>>>(nitems, items, vkeys, cbits, slot_off, dp[0], dp[1], dp[2],
@@ -976,7 +1003,7 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* ho
     int* cend = cbeg + nc;
     if (nvis > 0) {
         k_vor_pairsum<<<grid1d(nvis), THREADS, 0, s>>>(nvis, vkeys, cbits, pbits, slot_off, src, val, ms, ckeys_in,
-                                                       cidx_in, dst);
+                                                       cidx_in, dst, single ? 0 : 1);
         KIWI_HIP_CHECK(c, hipGetLastError());
         KIWI_HIP_CHECK(c, sort_pairs(c, ckeys_in, ckeys, cidx_in, cidx, nvis, 0, cbits + pbits, s));
     }
@@ -984,7 +1011,7 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* ho
         k_fill_i32<<<grid1d(2 * nc), THREADS, 0, s>>>(cbeg, 2 * nc, 0);
         if (nvis > 0) k_vor_cell_ranges<<<grid1d(nvis), THREADS, 0, s>>>(nvis, ckeys, pbits, cbeg, cend);
         k_vor_accum<<<grid1d(nc), THREADS, 0, s>>>(nc, ckeys, pbits, cidx, cbeg, cend, ms, dp[6], dp[3], dp[4], dp[5],
-                                                   om[0], om[1], om[2], om[3]);
+                                                   om[0], om[1], om[2], om[3], single ? 1 : 0);
         KIWI_HIP_CHECK(c, hipGetLastError());
     }
     k_vsum_partial<<<VSUM_BLOCKS, THREADS, 0, s>>>(n, dp[3], dp[4], dp[5], vpart);

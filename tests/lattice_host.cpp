@@ -173,6 +173,15 @@ double lat_face_walk(int kind, int kid, double sup, const double* p, double h, c
     *reach = r ? 1 : 0;
     return m;
 }
+
+// the cubic spline's face integral in single precision on vertex coordinates relative to the particle
+// (the Voronoi map's mixed mode)
+double lat_face_walk32(const double* p, double h, const double* v, int n) {
+    bool r = false;
+    EvM4T<float> ev{static_cast<float>(1.0 / h)};
+    const RelVerts<float> vs{v, p[0], p[1], p[2]};
+    return face_walk_t<float>(0.0f, 0.0f, 0.0f, static_cast<float>(2.0 * h), vs, n, ev, &r);
+}
 }  // extern "C"
 
 // ---- the adjoint's lattice gradient (m4_adjoint.h) ----

@@ -196,6 +196,35 @@ def test_mass_conservation_inside_the_domain(hip_ctx):
     assert st.n_negative_warn == 0
 
 
+def test_mixed_precision_against_fp64(hip_ctx):
+    """Mixed precision (the cubic spline's face integrals in FP32, visit test and sums in FP64): the same
+    visited pairs and face integrals as FP64, the total mass within 1e-6, every cell within 1e-5 of the
+    largest cell's mass, and conservation inside the domain to 1e-6; the other kernel paths return the
+    FP64 result bit for bit."""
+    say("case: mixed vs FP64, 2000 sites, 1500 clustered particles; conservation subset; WC2 fallback")
+    parts, home, cells, _ = make_case(23, 2000, 1500, 1.2, clustered=True)
+    f = hip_ctx.voronoi_map(parts, home, cells)
+    m = hip_ctx.voronoi_map(parts, home, cells, precision=hip.MIXED)
+    assert m[4].pairs_visited == f[4].pairs_visited and m[4].face_evals == f[4].face_evals
+    err_tot = abs(m[0].sum() - f[0].sum()) / f[0].sum()
+    errs = [float(np.max(np.abs(a - b)) / np.max(np.abs(b))) for a, b in zip(m[:4], f[:4])]
+    say(f"  total mass rel err {err_tot:.2e}; per-field max err / max " + ", ".join(f"{e:.2e}" for e in errs))
+    assert err_tot <= 1e-6
+    assert max(errs) <= 1e-5
+    sup = 2.0 * parts["h"]
+    p = np.stack([parts[c] for c in ("x", "y", "z")], 1)
+    inside = np.all((p - sup[:, None] > -1.0) & (p + sup[:, None] < 1.0), axis=1)
+    keep = {k: np.ascontiguousarray(v[inside]) for k, v in parts.items()}
+    mk = hip_ctx.voronoi_map(keep, np.ascontiguousarray(home[inside]), cells, precision=hip.MIXED)
+    assert inside.sum() > 100
+    assert abs(mk[0].sum() - keep["m"].sum()) <= 1e-6 * keep["m"].sum()
+    k = hip.kernel_spec("WC2")
+    a = hip_ctx.voronoi_map(parts, home, cells, kernel=k)
+    b = hip_ctx.voronoi_map(parts, home, cells, kernel=k, precision=hip.MIXED)
+    for x, y in zip(a[:4], b[:4]):
+        assert np.array_equal(x, y)
+
+
 def test_device_inputs_and_determinism(hip_ctx):
     import torch
     say("case: device-resident inputs vs host inputs; two runs bit-identical")

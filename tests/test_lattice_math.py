@@ -249,6 +249,46 @@ def test_generic_single_precision_lattice(lat, name):
         assert np.abs(c32 - c64).max() < 2e-4 * np.abs(c64).max()
 
 
+def test_polygon_face_walk_single_precision(lat):
+    """The Voronoi map's mixed mode: the cubic spline's face walk in single precision, on vertex
+    coordinates relative to the particle and Newell's plane normal, against the double-precision walk on
+    the real Voronoi cells of tests/golden/voronoi_map.npz (h 0.5x to 3x the fixture's).  Faces with
+    two nearly coincident vertices (edges 1e-4 h) made the first-three-vertex plane lose 1e-4 of r0 in
+    single precision (4.7e-5 per face); with Newell's normal the worst face is ~1e-6, at walls near the
+    support edge where the plane and pyramid terms of the vertex integral (O(5)) cancel to O(0.25).
+    Bounds: every face within 3e-6, every cell (sum of its faces) within 5e-6, the median face 1e-7."""
+    lat.lat_face_walk.restype = ctypes.c_double
+    lat.lat_face_walk.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _pd, ctypes.c_double, _pd,
+                                  ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lat.lat_face_walk32.restype = ctypes.c_double
+    lat.lat_face_walk32.argtypes = [_pd, ctypes.c_double, _pd, ctypes.c_int]
+    g = np.load(os.path.join(HERE, "golden", "voronoi_map.npz"))
+    cfp, fvp, vx = g["cell_face_ptr"], g["face_vert_ptr"], g["vert_xyz"].reshape(-1, 3)
+    P = np.stack([g["p_x"], g["p_y"], g["p_z"]], 1)
+    rng = np.random.default_rng(3)
+    face_err, cell_err = [], 0.0
+    for _ in range(200):
+        i = rng.integers(len(P))
+        p = np.ascontiguousarray(P[i])
+        h = g["p_h"][i] * rng.uniform(0.5, 3.0)
+        for c in np.argsort(np.linalg.norm(g["site"] - p, axis=1))[:12]:
+            m64 = m32 = 0.0
+            for f in range(cfp[c], cfp[c + 1]):
+                v = np.ascontiguousarray(vx[fvp[f]:fvp[f + 1]]).reshape(-1)
+                r = ctypes.c_int()
+                a = lat.lat_face_walk(0, 0, 2.0, p.ctypes.data_as(_pd), h, v.ctypes.data_as(_pd), len(v) // 3,
+                                      ctypes.byref(r))
+                b = lat.lat_face_walk32(p.ctypes.data_as(_pd), h, v.ctypes.data_as(_pd), len(v) // 3)
+                assert np.isfinite(b)
+                face_err.append(abs(a - b))
+                m64 += a
+                m32 += b
+            cell_err = max(cell_err, abs(m64 - m32))
+    assert max(face_err) < 3e-6, max(face_err)
+    assert np.median(face_err) < 1e-7
+    assert cell_err < 5e-6, cell_err
+
+
 # ---------------------------------------------------------------------- tabulated cubic path (m4_table.h)
 def test_vertex_value_and_gridded_lookup_match_reference_golden(lat):
     """m4_table.h: the FP64 vertex integral that fills the lookup table, against the reference's
