@@ -71,14 +71,15 @@ struct kiwi_hip_ctx {
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     std::string err;
     DevBuf parts, bbox, sizes, offsets, keys_a, keys_b, pos_a, pos_b, cells, slab_sorted, slab, cub_tmp, ranges, work,
-        work_sorted, order_in, order, lines, ms, stats, outbuf, vsum, tables, m4table, probs, part_prob;
+        work_sorted, order_in, order, lines, ms, stats, outbuf, vsum, tables, m4table, probs, part_prob, acc_units,
+        acc_partial;
     bool m4table_ready = false;
     DevBuf vor[16];  // the Voronoi map's buffers
     template <class F>
     void for_each_buf(F f) {
         for (DevBuf* b : {&parts, &bbox, &sizes, &offsets, &keys_a, &keys_b, &pos_a, &pos_b, &cells, &slab_sorted,
                           &slab, &cub_tmp, &ranges, &work, &work_sorted, &order_in, &order, &lines, &ms, &stats,
-                          &outbuf, &vsum, &tables, &m4table, &probs, &part_prob})
+                          &outbuf, &vsum, &tables, &m4table, &probs, &part_prob, &acc_units, &acc_partial})
             f(*b);
         for (DevBuf& b : vor) f(b);
     }

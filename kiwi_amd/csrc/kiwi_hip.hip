@@ -731,6 +731,55 @@ __global__ __launch_bounds__(THREADS) void k_particles(
 // ------------------------------------------------------------------------------------------------
 constexpr int ACC_CPT = BLK * BLK * BLK / THREADS;  // 2 cells per thread, along x
 constexpr int ACC_GROUP = 8;                         // items whose sums are loaded before adding
+// A block with more than ACC_CHUNK items is added up in chunks of ACC_CHUNK items, one workgroup per
+// chunk, each writing FP64 partial sums; k_acc_combine then adds a block's partials in chunk order.  A
+// single workgroup per block left the densest blocks (5.7e4 items in a dense cutout) holding the launch.
+// The split depends only on the block's own item count, so the sums stay deterministic and a batch
+// still matches its single calls bit for bit.
+constexpr int ACC_CHUNK = 2048;
+constexpr int BLK_CELLS = BLK * BLK * BLK;
+
+// one accumulation unit: LPT position k of its block, chunk index, partial slot (-1: writes the grid)
+struct AccUnit {
+    int k, chunk, slot;
+};
+
+// units per block in LPT order, and partial slots for split blocks; entry nblocks is 0 (scan totals)
+__global__ void k_acc_count(int nblocks, const int* __restrict__ order, const int* __restrict__ beg,
+                            const int* __restrict__ end, int* __restrict__ nunits, int* __restrict__ nsplit) {
+    const int k = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k > nblocks) return;
+    int u = 0, sp = 0;
+    if (k < nblocks) {
+        const int b = order[k];
+        const int n = end[b] - beg[b];
+        u = n > ACC_CHUNK ? (n + ACC_CHUNK - 1) / ACC_CHUNK : 1;
+        sp = n > ACC_CHUNK ? u : 0;
+    }
+    nunits[k] = u;
+    nsplit[k] = sp;
+}
+
+__global__ void k_acc_fill(int nblocks, const int* __restrict__ unit_off, const int* __restrict__ split_off,
+                           AccUnit* __restrict__ units) {
+    const int k = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nblocks) return;
+    const int u0 = unit_off[k], nu = unit_off[k + 1] - u0;
+    const bool split = split_off[k + 1] > split_off[k];
+    for (int c = 0; c < nu; ++c) units[u0 + c] = AccUnit{k, c, split ? split_off[k] + c : -1};
+}
+
+// a cell's final sums into its problem's grid; mixed mode clamps the cell's total instead of every pair:
+// a negative total is rounding noise of an (almost) empty cell, which then takes the mean-velocity
+// fallback like one
+__device__ __forceinline__ void acc_store(const Prob& P, int64_t c, double mass, double p0, double p1, double p2,
+                                          int clamp_pairs) {
+    if (!clamp_pairs && mass < 0.0) mass = p0 = p1 = p2 = 0.0;
+    P.out[0][c] = mass;
+    P.out[1][c] = p0;
+    P.out[2][c] = p1;
+    P.out[3][c] = p2;
+}
 
 struct AccLds {
     long long off[THREADS];  // item slab offsets
@@ -744,9 +793,12 @@ __global__ __launch_bounds__(THREADS) void k_accum(
     const double* __restrict__ m, const int* __restrict__ bbox, const uint64_t* __restrict__ keys,
     const int64_t* __restrict__ slab, const int* __restrict__ blk_begin, const int* __restrict__ blk_end,
     const int* __restrict__ order, const R* __restrict__ ms, const Prob* __restrict__ pr, int nprob,
-    int clamp_pairs) {
+    int clamp_pairs, int nblocks, const AccUnit* __restrict__ units, const int* __restrict__ unit_off,
+    double* __restrict__ partial) {
     __shared__ AccLds A;
-    const int b = order[blockIdx.x];  // global block
+    if (static_cast<int>(blockIdx.x) >= unit_off[nblocks]) return;  // the grid is an upper bound on the units
+    const AccUnit un = units[blockIdx.x];
+    const int b = order[un.k];  // global block
     const int qb = block_problem(pr, nprob, b);
     const GridGeom& g = pr[qb].g;
     const int bl = static_cast<int>(b - pr[qb].block_off);  // block within its problem's grid
@@ -763,7 +815,7 @@ __global__ __launch_bounds__(THREADS) void k_accum(
 #pragma unroll
     for (int s = 0; s < ACC_CPT; ++s) acc[s][0] = acc[s][1] = acc[s][2] = acc[s][3] = 0.0;
 
-    const int q0 = blk_begin[b], q1 = blk_end[b];
+    const int q0 = blk_begin[b] + un.chunk * ACC_CHUNK, q1 = min(blk_end[b], q0 + ACC_CHUNK);
     for (int base = q0; base < q1; base += THREADS) {
         const int cnt = min(THREADS, q1 - base);
         __syncthreads();
@@ -825,26 +877,52 @@ __global__ __launch_bounds__(THREADS) void k_accum(
         }
     }
 
+    if (un.slot >= 0) {  // a chunk of a split block: FP64 partial sums, [slot][field][block-local cell]
+        double* const part = partial + static_cast<int64_t>(un.slot) * 4 * BLK_CELLS;
+#pragma unroll
+        for (int s = 0; s < ACC_CPT; ++s) {
+            const int local = (cx0 + s) * BLK * BLK + (tid & 63);
+#pragma unroll
+            for (int f = 0; f < 4; ++f) part[f * BLK_CELLS + local] = acc[s][f];
+        }
+        return;
+    }
     const int iy = by * BLK + cy, iz = bz * BLK + cz;
-    double* const out_mass = pr[qb].out[0];
-    double* const out_px = pr[qb].out[1];
-    double* const out_py = pr[qb].out[2];
-    double* const out_pz = pr[qb].out[3];
 #pragma unroll
     for (int s = 0; s < ACC_CPT; ++s) {
         const int ix = bx * BLK + cx0 + s;
         if (ix >= g.nx || iy >= g.ny || iz >= g.nz) continue;
         const int64_t c = (static_cast<int64_t>(ix) * g.ny + iy) * g.nz + iz;
-        double mass = acc[s][0], p0 = acc[s][1], p1 = acc[s][2], p2 = acc[s][3];
-        if (!clamp_pairs && mass < 0.0) {
-            // mixed mode clamps the cell's total instead of every pair: a negative total is rounding
-            // noise of an (almost) empty cell, which then takes the mean-velocity fallback like one
-            mass = p0 = p1 = p2 = 0.0;
+        acc_store(pr[qb], c, acc[s][0], acc[s][1], acc[s][2], acc[s][3], clamp_pairs);
+    }
+}
+
+// the cells of every split block: its chunks' partial sums added in chunk order
+__global__ __launch_bounds__(THREADS) void k_acc_combine(int nblocks, const int* __restrict__ order,
+                                                         const int* __restrict__ split_off,
+                                                         const double* __restrict__ partial,
+                                                         const Prob* __restrict__ pr, int nprob, int clamp_pairs) {
+    const int k = blockIdx.x;
+    const int s0 = split_off[k], s1 = split_off[k + 1];
+    if (s0 == s1) return;
+    const int b = order[k];
+    const int qb = block_problem(pr, nprob, b);
+    const GridGeom& g = pr[qb].g;
+    const int bl = static_cast<int>(b - pr[qb].block_off);
+    const int bz = bl % g.nbz;
+    const int by = (bl / g.nbz) % g.nby;
+    const int bx = bl / (g.nbz * g.nby);
+    for (int local = threadIdx.x; local < BLK_CELLS; local += THREADS) {
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int sl = s0; sl < s1; ++sl) {
+            const double* part = partial + static_cast<int64_t>(sl) * 4 * BLK_CELLS;
+#pragma unroll
+            for (int f = 0; f < 4; ++f) v[f] += part[f * BLK_CELLS + local];
         }
-        out_mass[c] = mass;
-        out_px[c] = p0;
-        out_py[c] = p1;
-        out_pz[c] = p2;
+        const int ix = bx * BLK + local / (BLK * BLK), iy = by * BLK + (local / BLK) % BLK, iz = bz * BLK + local % BLK;
+        if (ix >= g.nx || iy >= g.ny || iz >= g.nz) continue;
+        const int64_t c = (static_cast<int64_t>(ix) * g.ny + iy) * g.nz + iz;
+        acc_store(pr[qb], c, v[0], v[1], v[2], v[3], clamp_pairs);
     }
 }
 
@@ -910,8 +988,9 @@ hipError_t launch_cells(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pc
                         const uint64_t* keys_particle_order, const int64_t* slab, const uint64_t* keys_block_order,
                         const int64_t* slab_sorted, const int* blk_begin, const int* blk_end, const int* order,
                         int nblocks, const Prob* pr, int nprob, const int* part_prob, void* lines, void* ms,
-                        DevStats* dst, unsigned long long* cursor, int clamp, hipEvent_t ev_lines, hipEvent_t ev_cells,
-                        hipEvent_t ev_accum) {
+                        DevStats* dst, unsigned long long* cursor, int clamp, const AccUnit* units,
+                        const int* unit_off, const int* split_off, int64_t max_units, double* partial,
+                        hipEvent_t ev_lines, hipEvent_t ev_cells, hipEvent_t ev_accum) {
     using R = typename P::Real;
     using Line = LineVals<R, P::NLN>;
     if (n > 0 && P::HAS_LINES)
@@ -927,9 +1006,12 @@ hipError_t launch_cells(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pc
             part_prob, pctx, static_cast<const Line*>(lines), static_cast<R*>(ms), dst, cursor, clamp);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipEventRecord(ev_cells, s)) != hipSuccess) return e;
-    if (nblocks > 0)
-        k_accum<R><<<nblocks, THREADS, 0, s>>>(dp[3], dp[4], dp[5], dp[6], bbox, keys_block_order, slab_sorted,
-                                              blk_begin, blk_end, order, static_cast<const R*>(ms), pr, nprob, clamp);
+    if (nblocks > 0) {
+        k_accum<R><<<static_cast<unsigned>(max_units), THREADS, 0, s>>>(
+            dp[3], dp[4], dp[5], dp[6], bbox, keys_block_order, slab_sorted, blk_begin, blk_end, order,
+            static_cast<const R*>(ms), pr, nprob, clamp, nblocks, units, unit_off, partial);
+        k_acc_combine<<<nblocks, THREADS, 0, s>>>(nblocks, order, split_off, partial, pr, nprob, clamp);
+    }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return hipEventRecord(ev_accum, s);
 }
@@ -1190,6 +1272,35 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
         KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairsDescending(c->cub_tmp.p, ob, work, work_sorted, order_in,
                                                                         order, static_cast<int>(nblocks), 0, 64, s));
     }
+    // accumulation units (blocks split into chunks of ACC_CHUNK items) in LPT order; the unit count is
+    // bounded by nblocks + 2 npairs / ACC_CHUNK and the split units by 2 npairs / ACC_CHUNK, so the grid
+    // and the partial sums are sized without reading the counts back
+    const int64_t max_split = 2 * (npairs / ACC_CHUNK) + 1;
+    const int64_t max_units = nblocks + max_split;
+    if (max_units > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "too many cell blocks in one call");
+    KIWI_HIP_CHECK(c, reserve(c->acc_units, sizeof(AccUnit) * max_units + 4 * sizeof(int) * (nblocks + 1) + 256));
+    AccUnit* units = static_cast<AccUnit*>(c->acc_units.p);
+    int* nunits = reinterpret_cast<int*>(units + max_units);
+    int* nsplit = nunits + (nblocks + 1);
+    int* unit_off = nsplit + (nblocks + 1);
+    int* split_off = unit_off + (nblocks + 1);
+    KIWI_HIP_CHECK(c, reserve(c->acc_partial, sizeof(double) * 4 * BLK_CELLS * max_split));
+    if (nblocks > 0) {
+        k_acc_count<<<grid1d(nblocks + 1), THREADS, 0, s>>>(static_cast<int>(nblocks), order, blk_begin, blk_end,
+                                                             nunits, nsplit);
+        size_t tb = 0;
+        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, nunits, unit_off, static_cast<int>(nblocks + 1), s));
+        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, tb));
+        for (int k = 0; k < 2; ++k) {
+            tb = c->cub_tmp.cap;
+            KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, tb, k ? nsplit : nunits,
+                                                               k ? split_off : unit_off,
+                                                               static_cast<int>(nblocks + 1), s));
+        }
+        k_acc_fill<<<grid1d(nblocks), THREADS, 0, s>>>(static_cast<int>(nblocks), unit_off, split_off, units);
+        KIWI_HIP_CHECK(c, hipGetLastError());
+    }
+    double* partial = static_cast<double*>(c->acc_partial.p);
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[1], s));
     const int clamp = fp64 ? 1 : 0;
     const int nb = static_cast<int>(nblocks);
@@ -1198,29 +1309,34 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
         const PolicyTab::Ctx pc{sup, static_cast<const double*>(c->m4table.p)};
         KIWI_HIP_CHECK(c, launch_cells<PolicyTab>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
                                                   slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob, part_prob,
-                                                  c->lines.p, c->ms.p, dst, cursor, clamp, c->ev[2], c->ev[3],
+                                                  c->lines.p, c->ms.p, dst, cursor, clamp,
+                                                  units, unit_off, split_off, max_units, partial, c->ev[2], c->ev[3],
                                                   c->ev[4]));
     } else if (generic && fp64) {
         const PolicyGen<double>::Ctx pc{sup, static_cast<const GenTab*>(c->tables.p) + K->kernel_id};
         KIWI_HIP_CHECK(c, launch_cells<PolicyGen<double>>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
                                                           slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
                                                           part_prob, c->lines.p, c->ms.p, dst, cursor, clamp,
+                                                          units, unit_off, split_off, max_units, partial,
                                                           c->ev[2], c->ev[3], c->ev[4]));
     } else if (generic) {
         const PolicyGen<float>::Ctx pc{sup, static_cast<const GenTab*>(c->tables.p) + K->kernel_id};
         KIWI_HIP_CHECK(c, launch_cells<PolicyGen<float>>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
                                                          slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
                                                          part_prob, c->lines.p, c->ms.p, dst, cursor, clamp,
+                                                         units, unit_off, split_off, max_units, partial,
                                                          c->ev[2], c->ev[3], c->ev[4]));
     } else if (fp64) {
         KIWI_HIP_CHECK(c, launch_cells<PolicyM4<double>>(c, s, {sup}, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
                                                          slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
-                                                         part_prob, c->lines.p, c->ms.p, dst, cursor, clamp, c->ev[2],
+                                                         part_prob, c->lines.p, c->ms.p, dst, cursor, clamp,
+                                                         units, unit_off, split_off, max_units, partial, c->ev[2],
                                                          c->ev[3], c->ev[4]));
     } else {
         KIWI_HIP_CHECK(c, launch_cells<PolicyM4<float>>(c, s, {sup}, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
                                                         slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
-                                                        part_prob, c->lines.p, c->ms.p, dst, cursor, clamp, c->ev[2],
+                                                        part_prob, c->lines.p, c->ms.p, dst, cursor, clamp,
+                                                        units, unit_off, split_off, max_units, partial, c->ev[2],
                                                         c->ev[3], c->ev[4]));
     }
     k_vsum_problems<<<nprob * VSUM_BLOCKS, THREADS, 0, s>>>(dpr, dp[3], dp[4], dp[5], vsums + 3 * nprob);
