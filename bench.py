@@ -6,6 +6,9 @@ random field on 256^3, P(k) ~ k^-3, sigma_ln_rho = 1.5), equal masses, h = 1.2 (
 clipped to [0.5, 50] cells, velocities from the field's gradient (rms 5 km/s per axis) plus N(0, 2 km/s),
 M4 kernel, domain [0,256]^3 with unit cells, grid 256^3.  Inputs and outputs are resident in HBM for the
 timed region.  A step is one full kiwi_hip_sph_to_grid call (binning, sort, gather, mean velocity).
+Before each precision's warmup the library's autotuner (kiwi_hip_autotune) picks the cell kernel's
+launch shape on this workload (untimed; the shape never changes results; --no-autotune keeps the
+default), and the chosen shape is reported in config.launch_shape.
 
 The metric is cell-integrals/s: (particle, cell) pairs whose cell intersects the particle's support
 sphere, divided by the step time.  `value` is the mixed-precision rate; the FP64 rate, the speed ratio
@@ -52,8 +55,12 @@ def line_kernel_flops(st, fpu):
     return sum(st.line_evals[r] * fpu["line"][r] for r in range(3))
 
 
-def run_precision(ctx, hip, dev, dom, res, out, prec, steps, warmup):
+def run_precision(ctx, hip, dev, dom, res, out, prec, steps, warmup, autotune):
     import torch
+    shape = ctx.get_tuning()
+    if autotune:  # untimed: the launch shape (never the results) is chosen on this workload
+        shape, ms = ctx.autotune([{"parts": dev, "domain": dom, "res": res, "precision": prec, "out": out}])
+        log(f"  autotuned launch shape: item_grab {shape[0]}, cursors {shape[1]} ({ms:.2f} ms per call)")
     for _ in range(warmup):
         ctx.sph_to_grid(dev, dom, res, precision=prec, out=out)
     torch.cuda.synchronize()
@@ -69,7 +76,7 @@ def run_precision(ctx, hip, dev, dom, res, out, prec, steps, warmup):
     t1 = time.perf_counter()
     log(f"  device ms per step: prep {np.mean(pms):.2f}, lines {np.mean(lms):.2f}, cells {np.mean(gms):.2f}, "
         f"accum {np.mean(ams):.2f}")
-    return (t1 - t0) * 1e3 / steps, float(np.mean(gms)), stats
+    return (t1 - t0) * 1e3 / steps, float(np.mean(gms)), stats, shape
 
 
 def cpu_baseline(parts, dom, res, hip, ctx):
@@ -94,6 +101,7 @@ def main():
     ap.add_argument("--particles", type=int, default=1_000_000)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the mapping shards by grid slab across GPUs; that path is "
@@ -117,9 +125,11 @@ def main():
     out32 = [torch.zeros(nc, dtype=torch.float64, device="cuda:0") for _ in range(4)]
     ctx = hip.Context(0)
 
-    ms64, g64, st64 = run_precision(ctx, hip, dev, dom, res, out64, hip.FP64, args.steps, args.warmup)
+    ms64, g64, st64, shape64 = run_precision(ctx, hip, dev, dom, res, out64, hip.FP64, args.steps, args.warmup,
+                                             not args.no_autotune)
     log(f"fp64: {ms64:.2f} ms/step, cell kernel {g64:.2f} ms, pairs {st64.pairs_sphere}")
-    ms32, g32, st32 = run_precision(ctx, hip, dev, dom, res, out32, hip.MIXED, args.steps, args.warmup)
+    ms32, g32, st32, shape32 = run_precision(ctx, hip, dev, dom, res, out32, hip.MIXED, args.steps, args.warmup,
+                                             not args.no_autotune)
     log(f"mixed: {ms32:.2f} ms/step, cell kernel {g32:.2f} ms")
 
     m64 = out64[0].cpu().numpy()
@@ -152,7 +162,10 @@ def main():
         "dtype": "fp32 (FP64 accumulation)", "data": "synthetic (seeded log-normal field)",
         "config": {"workload": f"sph-to-grid, {args.particles} log-normal clustered particles -> "
                                f"{res[0]}^3, M4, h 0.5-50 cells", "particles": args.particles,
-                   "grid": list(res), "kernel": "M4", "parallelism": "single GPU"},
+                   "grid": list(res), "kernel": "M4", "parallelism": "single GPU",
+                   "launch_shape": {"fp64": {"item_grab": shape64[0], "cursors": shape64[1]},
+                                    "mixed": {"item_grab": shape32[0], "cursors": shape32[1]},
+                                    "autotuned": not args.no_autotune}},
         "fp64": {"value": pairs / (ms64 * 1e-3), "ms_per_step": ms64, "cells_ms": g64,
                  "roofline_frac": f_pair * st64.pairs_bbox / (g64 * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"],
                  "executed_frac": fl64 / (g64 * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"]},

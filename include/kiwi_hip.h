@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KIWI_HIP_ABI_VERSION 4
+#define KIWI_HIP_ABI_VERSION 5
 
 #define KIWI_OK 0
 #define KIWI_ERR_INVALID_ARG (-1)
@@ -178,6 +178,25 @@ typedef struct kiwi_batch_problem {
  * those kiwi_hip_sph_to_grid would return for it alone (device times cover its group's run;
  * particle_blocks is -1 when the group holds more than one problem). */
 int kiwi_hip_sph_to_grid_batch(kiwi_hip_ctx* ctx, int n_problems, const kiwi_batch_problem* problems);
+
+/* Launch shape of the sph-to-grid cell kernel (no reference counterpart).  Results never depend on it:
+ * every (particle, cell block) item is computed and placed the same way whatever the shape. */
+typedef struct kiwi_tuning {
+    int item_grab; /* items a wavefront takes from a work cursor at once, 1..256 (default 16) */
+    int cursors;   /* contiguous item slices with a work cursor each, 1..16 (default 8) */
+} kiwi_tuning;
+
+#define KIWI_MAX_CURSORS 16
+
+/* Set (NULL: restore the defaults) or read the context's launch shape, used by every later call. */
+int kiwi_hip_set_tuning(kiwi_hip_ctx* ctx, const kiwi_tuning* tuning);
+int kiwi_hip_get_tuning(const kiwi_hip_ctx* ctx, kiwi_tuning* tuning);
+
+/* Autotuner: runs the batch (outputs and stats as kiwi_hip_sph_to_grid_batch writes them) once per
+ * candidate shape (item_grab 8, 16, 32, 64 x cursors 1, 4, 8, 16), twice each, keeps the shape with the
+ * shortest call on the context and returns it with its time (best_ms may be NULL). */
+int kiwi_hip_autotune(kiwi_hip_ctx* ctx, int n_problems, const kiwi_batch_problem* problems, kiwi_tuning* best,
+                      double* best_ms);
 
 /* Per-particle gradients, n each; on_device selects device or host pointers. */
 typedef struct kiwi_particle_grads {

@@ -74,6 +74,7 @@ struct kiwi_hip_ctx {
         work_sorted, order_in, order, lines, ms, stats, outbuf, vsum, tables, m4table, probs, part_prob, acc_units,
         acc_partial;
     bool m4table_ready = false;
+    kiwi_tuning tune{16, 8};  // the cell kernel's launch shape (kiwi_hip_set_tuning)
     DevBuf vor[16];  // the Voronoi map's buffers
     template <class F>
     void for_each_buf(F f) {

@@ -36,6 +36,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <chrono>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -51,15 +52,14 @@ namespace {
 
 constexpr int BLK_PTS = (BLK + 1) * (BLK + 1) * (BLK + 1);
 constexpr int MAX_AXIS = 8192;                // grid cells per axis
-constexpr unsigned ITEM_GRAB = 16;            // items a wavefront takes from a cursor at once
-// The cell kernel's items are split into CURSORS contiguous slices, each with its own work cursor: one
-// cursor serialises its atomics (about 12 ns apart), which capped a single cursor at ~80M grabs/s.  A
-// workgroup starts on slice blockIdx % CURSORS (workgroups are dispatched round-robin over the 8 XCDs,
-// so a slice's items, and the line tables of their particles, stay in one XCD's L2) and then takes
-// from the other slices until all are drained.
-constexpr int CURSORS = 8;
-constexpr int CURSOR_STRIDE = 544;            // 64-bit words between cursors (4352 B: separate lines)
-constexpr size_t CURSOR_BYTES = sizeof(unsigned long long) * CURSORS * CURSOR_STRIDE;
+// The cell kernel's items are split into `cursors` contiguous slices (the context's launch shape,
+// default 8), each with its own work cursor, from which a wavefront takes `item_grab` items at a time
+// (default 16): one cursor serialises its atomics (about 12 ns apart), which capped a single cursor at
+// ~80M grabs/s.  A workgroup starts on slice blockIdx % cursors (workgroups are dispatched round-robin
+// over the 8 XCDs, so with 8 slices a slice's items, and the line tables of their particles, stay in
+// one XCD's L2) and then takes from the other slices until all are drained.
+constexpr int CURSOR_STRIDE = 544;  // 64-bit words between cursors (4352 B: separate lines)
+constexpr size_t CURSOR_BYTES = sizeof(unsigned long long) * KIWI_MAX_CURSORS * CURSOR_STRIDE;
 
 struct DevStats {  // one per problem
     unsigned long long skipped_outside, n_negative, n_negative_warn, pairs_bbox, pairs_sphere;
@@ -529,7 +529,7 @@ __global__ __launch_bounds__(THREADS) void k_particles(
     const double* __restrict__ h, const int* __restrict__ bbox, const int64_t* __restrict__ loffs,
     const Prob* __restrict__ pr, const int* __restrict__ part_prob, typename P::Ctx ctx,
     const LineVals<typename P::Real, P::NLN>* __restrict__ lines, typename P::Real* __restrict__ ms, DevStats* st,
-    unsigned long long* cursor, int clamp_pairs) {
+    unsigned long long* cursor, int clamp_pairs, int grab, int ncurs) {
     using R = typename P::Real;
     constexpr int NLN = P::NLN;
     __shared__ WaveLds<R, P::NPL> WL[WAVES];
@@ -563,19 +563,20 @@ __global__ __launch_bounds__(THREADS) void k_particles(
         c_pts[0] = c_pts[1] = c_pts[2] = 0;
     };
 
-    const int home = static_cast<int>(blockIdx.x % CURSORS);
-    for (int d = 0; d < CURSORS; ++d) {
-        const int sl = (home + d) % CURSORS;
-        const uint64_t lo = static_cast<uint64_t>(npairs) * sl / CURSORS;
-        const uint64_t hi = static_cast<uint64_t>(npairs) * (sl + 1) / CURSORS;
+    const int home = static_cast<int>(blockIdx.x % static_cast<unsigned>(ncurs));
+    const unsigned long long ugrab = static_cast<unsigned long long>(grab);
+    for (int d = 0; d < ncurs; ++d) {
+        const int sl = (home + d) % ncurs;
+        const uint64_t lo = static_cast<uint64_t>(npairs) * sl / ncurs;
+        const uint64_t hi = static_cast<uint64_t>(npairs) * (sl + 1) / ncurs;
         unsigned long long* cs = cursor + sl * CURSOR_STRIDE;
         uint64_t cur = 0;
-        if (lane == 0) cur = lo + atomicAdd(cs, static_cast<unsigned long long>(ITEM_GRAB));
+        if (lane == 0) cur = lo + atomicAdd(cs, ugrab);
         cur = wave_uniform64(cur);
         while (cur < hi) {
             uint64_t nxt = 0;
-            if (lane == 0) nxt = lo + atomicAdd(cs, static_cast<unsigned long long>(ITEM_GRAB));
-            const uint32_t tend = static_cast<uint32_t>(min(cur + ITEM_GRAB, hi));
+            if (lane == 0) nxt = lo + atomicAdd(cs, ugrab);
+            const uint32_t tend = static_cast<uint32_t>(min(cur + ugrab, hi));
             for (uint32_t t = static_cast<uint32_t>(cur); t < tend; ++t) {
                 const uint64_t key = keys[t];
                 const uint32_t p = static_cast<uint32_t>(key & 0xffffffffull);
@@ -1001,9 +1002,10 @@ hipError_t launch_cells(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pc
     if (e != hipSuccess) return e;
     if ((e = hipEventRecord(ev_lines, s)) != hipSuccess) return e;
     if (npairs > 0)
-        k_particles<P><<<resident_grid(c, k_particles<P>, ITEM_GRAB, npairs), THREADS, 0, s>>>(
+        k_particles<P><<<resident_grid(c, k_particles<P>, c->tune.item_grab, npairs), THREADS, 0, s>>>(
             static_cast<uint32_t>(npairs), keys_particle_order, slab, dp[0], dp[1], dp[2], dp[7], bbox, loffs, pr,
-            part_prob, pctx, static_cast<const Line*>(lines), static_cast<R*>(ms), dst, cursor, clamp);
+            part_prob, pctx, static_cast<const Line*>(lines), static_cast<R*>(ms), dst, cursor, clamp,
+            c->tune.item_grab, c->tune.cursors);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipEventRecord(ev_cells, s)) != hipSuccess) return e;
     if (nblocks > 0) {
@@ -1495,6 +1497,61 @@ int kiwi_hip_sph_to_grid_batch(kiwi_hip_ctx* c, int n_problems, const kiwi_batch
                                      problems[i].precision))
             return rc;
     }
+    return KIWI_OK;
+}
+
+int kiwi_hip_set_tuning(kiwi_hip_ctx* c, const kiwi_tuning* t) {
+    if (!c) return KIWI_ERR_INVALID_ARG;
+    if (!t) {
+        c->tune = kiwi_tuning{16, 8};
+        return KIWI_OK;
+    }
+    if (t->item_grab < 1 || t->item_grab > 256) return fail(c, KIWI_ERR_INVALID_ARG, "item_grab must be in 1..256");
+    if (t->cursors < 1 || t->cursors > KIWI_MAX_CURSORS)
+        return fail(c, KIWI_ERR_INVALID_ARG, "cursors must be in 1..KIWI_MAX_CURSORS");
+    c->tune = *t;
+    return KIWI_OK;
+}
+
+int kiwi_hip_get_tuning(const kiwi_hip_ctx* c, kiwi_tuning* t) {
+    if (!c || !t) return KIWI_ERR_INVALID_ARG;
+    *t = c->tune;
+    return KIWI_OK;
+}
+
+// The launch-shape autotuner: every candidate runs the caller's batch twice (the calls are synchronous,
+// so the host clock brackets the whole call); the fastest candidate stays on the context.
+int kiwi_hip_autotune(kiwi_hip_ctx* c, int n_problems, const kiwi_batch_problem* problems, kiwi_tuning* best,
+                      double* best_ms) {
+    if (!c) return KIWI_ERR_INVALID_ARG;
+    if (!best) return fail(c, KIWI_ERR_INVALID_ARG, "null tuning result");
+    const kiwi_tuning saved = c->tune;
+    // warm-up with the current shape: buffers reach their size, tables are built
+    if (const int rc = kiwi_hip_sph_to_grid_batch(c, n_problems, problems)) return rc;
+    static const int kGrab[] = {8, 16, 32, 64};
+    static const int kCursors[] = {1, 4, 8, 16};
+    kiwi_tuning bt = saved;
+    double bms = INFINITY;
+    for (const int g : kGrab)
+        for (const int k : kCursors) {
+            c->tune = kiwi_tuning{g, k};
+            double t = INFINITY;
+            for (int rep = 0; rep < 2; ++rep) {
+                const auto t0 = std::chrono::steady_clock::now();
+                if (const int rc = kiwi_hip_sph_to_grid_batch(c, n_problems, problems)) {
+                    c->tune = saved;
+                    return rc;
+                }
+                t = std::min(t, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+            }
+            if (t < bms) {
+                bms = t;
+                bt = c->tune;
+            }
+        }
+    c->tune = bt;
+    *best = bt;
+    if (best_ms) *best_ms = bms;
     return KIWI_OK;
 }
 

@@ -32,7 +32,7 @@ _pd = ctypes.POINTER(ctypes.c_double)
 
 EXPORTED = ["kiwi_hip_abi_version", "kiwi_hip_create", "kiwi_hip_destroy", "kiwi_hip_last_error",
             "kiwi_hip_stream", "kiwi_hip_sph_to_grid", "kiwi_hip_voronoi_map", "kiwi_hip_sph_to_grid_adjoint",
-            "kiwi_hip_sph_to_grid_batch"]
+            "kiwi_hip_sph_to_grid_batch", "kiwi_hip_set_tuning", "kiwi_hip_get_tuning", "kiwi_hip_autotune"]
 
 
 class KiwiError(RuntimeError):
@@ -86,6 +86,10 @@ _pll = ctypes.POINTER(ctypes.c_int64)
 class BatchProblem(ctypes.Structure):
     _fields_ = [("particles", Particles), ("grid", Grid), ("kernel", Kernel), ("precision", _i), ("out", CellSums),
                 ("stats", ctypes.POINTER(Stats))]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("item_grab", _i), ("cursors", _i)]
 _pi = ctypes.POINTER(ctypes.c_int)
 
 
@@ -153,6 +157,13 @@ def load():
                                                    ctypes.POINTER(ParticleGrads), ctypes.POINTER(AdjointStats)]
         L.kiwi_hip_sph_to_grid_batch.restype = _i
         L.kiwi_hip_sph_to_grid_batch.argtypes = [ctypes.c_void_p, _i, ctypes.POINTER(BatchProblem)]
+        L.kiwi_hip_set_tuning.restype = _i
+        L.kiwi_hip_set_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(Tuning)]
+        L.kiwi_hip_get_tuning.restype = _i
+        L.kiwi_hip_get_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(Tuning)]
+        L.kiwi_hip_autotune.restype = _i
+        L.kiwi_hip_autotune.argtypes = [ctypes.c_void_p, _i, ctypes.POINTER(BatchProblem), ctypes.POINTER(Tuning),
+                                        ctypes.POINTER(ctypes.c_double)]
         L.kiwi_hip_voronoi_map.restype = _i
         L.kiwi_hip_voronoi_map.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.c_void_p,
                                            ctypes.POINTER(VoronoiCells), ctypes.POINTER(Kernel), _i,
@@ -244,10 +255,7 @@ class Context:
                                                 int(precision), ctypes.byref(O), ctypes.byref(st)))
         return arrs[0], arrs[1], arrs[2], arrs[3], st
 
-    def sph_to_grid_batch(self, problems):
-        """Many sph-to-grid problems in one call.  problems: list of dicts with keys parts, domain, res and
-        optionally kernel, precision, out (as for sph_to_grid).  Returns a list of (mass, px, py, pz, Stats),
-        one per problem."""
+    def _batch(self, problems):
         arr = (BatchProblem * max(len(problems), 1))()
         keep, results = [], []
         for i, q in enumerate(problems):
@@ -266,8 +274,38 @@ class Context:
                                   ctypes.pointer(st))
             keep.append((P, O, q["parts"]))
             results.append((outs, st))
+        return arr, keep, results
+
+    def sph_to_grid_batch(self, problems):
+        """Many sph-to-grid problems in one call.  problems: list of dicts with keys parts, domain, res and
+        optionally kernel, precision, out (as for sph_to_grid).  Returns a list of (mass, px, py, pz, Stats),
+        one per problem."""
+        arr, keep, results = self._batch(problems)
         self._check(self.L.kiwi_hip_sph_to_grid_batch(self.h, len(problems), arr))
         return [(o[0], o[1], o[2], o[3], st) for o, st in results]
+
+    def set_tuning(self, item_grab=None, cursors=None):
+        """The cell kernel's launch shape for later calls; no arguments restores the defaults."""
+        if item_grab is None and cursors is None:
+            self._check(self.L.kiwi_hip_set_tuning(self.h, None))
+        else:
+            cur = self.get_tuning()
+            t = Tuning(int(item_grab if item_grab is not None else cur[0]),
+                       int(cursors if cursors is not None else cur[1]))
+            self._check(self.L.kiwi_hip_set_tuning(self.h, ctypes.byref(t)))
+
+    def get_tuning(self):
+        t = Tuning()
+        self._check(self.L.kiwi_hip_get_tuning(self.h, ctypes.byref(t)))
+        return t.item_grab, t.cursors
+
+    def autotune(self, problems):
+        """Time the batch (list as for sph_to_grid_batch) under every candidate launch shape, keep the
+        fastest on the context.  Returns ((item_grab, cursors), ms of the fastest call)."""
+        arr, keep, results = self._batch(problems)
+        t, ms = Tuning(), ctypes.c_double()
+        self._check(self.L.kiwi_hip_autotune(self.h, len(problems), arr, ctypes.byref(t), ctypes.byref(ms)))
+        return (t.item_grab, t.cursors), ms.value
 
     def sph_to_grid_adjoint(self, parts, domain, res, grad_cells, kernel=None, precision=FP64, out=None):
         """Gradients of a loss L with respect to the particles' m, x, y, z, h, vx, vy, vz, given grad_cells =

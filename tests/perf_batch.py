@@ -6,8 +6,9 @@ Workload (kiwi_amd.workloads.cutout_batch): 64 independent cutouts, particle cou
 log-normal density (sigma_ln_rho 1.5), kernels alternating between the cubic spline and Wendland C2
 (generic path), FP32 bulk (mixed precision) for both.  Particles and outputs resident on the device.
 
-Times the batch (one call) against the same problems as 64 single calls, checks that both give
-bit-identical sums, and prints one JSON line.
+The library's autotuner first picks the cell kernel's launch shape on the batch (untimed;
+--no-autotune keeps the default).  Then times the batch (one call) against the same problems as 64
+single calls, checks that both give bit-identical sums, and prints one JSON line.
 
     python tests/perf_batch.py [--problems 64] [--steps 3] [--fp64]
 """
@@ -34,6 +35,7 @@ def main():
     ap.add_argument("--problems", type=int, default=64)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--fp64", action="store_true", help="every problem in FP64 (default: mixed precision)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     args = ap.parse_args()
     import torch
     torch.cuda.init()
@@ -53,6 +55,10 @@ def main():
     n_cells = sum(int(np.prod(b[2])) for b in batch)
     log(f"{len(probs)} problems, {n_parts} particles, {n_cells} cells ({time.perf_counter() - t0:.1f} s)")
     ctx = hip.Context(0)
+    shape = ctx.get_tuning()
+    if not args.no_autotune:  # untimed; the launch shape never changes results
+        shape, tms = ctx.autotune(probs)
+        log(f"autotuned launch shape: item_grab {shape[0]}, cursors {shape[1]} ({tms:.1f} ms per batch)")
 
     def run_batch():
         return ctx.sph_to_grid_batch(probs)
@@ -83,7 +89,8 @@ def main():
            "ms_batch": ms_batch, "ms_single_calls": ms_single, "speedup": ms_single / ms_batch,
            "pairs_sphere": pairs, "pairs_bbox": pairs_bbox,
            "cell_integrals_per_s_batch": pairs / (ms_batch * 1e-3),
-           "cell_integrals_per_s_single": pairs / (ms_single * 1e-3), "bit_identical": identical}
+           "cell_integrals_per_s_single": pairs / (ms_single * 1e-3), "bit_identical": identical,
+           "launch_shape": {"item_grab": shape[0], "cursors": shape[1], "autotuned": not args.no_autotune}}
     log(f"batch {ms_batch:.2f} ms, single calls {ms_single:.2f} ms (x{ms_single / ms_batch:.2f}); "
         f"{pairs} sphere pairs; identical: {identical}")
     print(json.dumps(res), flush=True)

@@ -95,3 +95,32 @@ def test_batch_errors_name_the_problem(hip_ctx):
         hip_ctx.sph_to_grid_batch(probs)
     assert len(hip_ctx.sph_to_grid_batch(problems()[:2])) == 2
     assert hip_ctx.sph_to_grid_batch([]) == []
+
+
+def test_launch_shapes_do_not_change_results_and_autotune(hip_ctx):
+    say("case: the cell kernel's launch shape (item grab, cursor slices) and the autotuner")
+    probs = problems(seed=8)
+    assert hip_ctx.get_tuning() == (16, 8)
+    ref = hip_ctx.sph_to_grid_batch(probs)
+    for shape in ((1, 1), (8, 1), (64, 16), (256, 3)):
+        hip_ctx.set_tuning(*shape)
+        assert hip_ctx.get_tuning() == shape
+        got = hip_ctx.sph_to_grid_batch(probs)
+        for q, (a, b) in enumerate(zip(got, ref)):
+            for x, y in zip(a[:4], b[:4]):
+                assert np.array_equal(x, y), (shape, q)
+            assert a[4].pairs_sphere == b[4].pairs_sphere
+    for bad in ((0, 8), (257, 8), (16, 0), (16, 17)):
+        with pytest.raises(hip.KiwiError):
+            hip_ctx.set_tuning(*bad)
+    hip_ctx.set_tuning()
+    assert hip_ctx.get_tuning() == (16, 8)
+    shape, ms = hip_ctx.autotune(probs)
+    say(f"  autotuned shape {shape}, {ms:.2f} ms per call")
+    assert shape[0] in (8, 16, 32, 64) and shape[1] in (1, 4, 8, 16) and ms > 0
+    assert hip_ctx.get_tuning() == shape
+    got = hip_ctx.sph_to_grid_batch(probs)
+    for a, b in zip(got, ref):
+        for x, y in zip(a[:4], b[:4]):
+            assert np.array_equal(x, y)
+    hip_ctx.set_tuning()
