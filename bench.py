@@ -8,7 +8,7 @@ M4 kernel, domain [0,256]^3 with unit cells, grid 256^3.  Inputs and outputs are
 timed region.  A step is one full kiwi_hip_sph_to_grid call (binning, sort, gather, mean velocity).
 Before each precision's warmup the library's autotuner (kiwi_hip_autotune) picks the cell kernel's
 launch shape on this workload (untimed; the shape never changes results; --no-autotune keeps the
-default), and the chosen shape is reported in config.launch_shape.
+default, --tuning G,K sets one), and the chosen shape is reported in config.launch_shape.
 
 The metric is cell-integrals/s: (particle, cell) pairs whose cell intersects the particle's support
 sphere, divided by the step time.  `value` is the mixed-precision rate; the FP64 rate, the speed ratio
@@ -102,6 +102,7 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--tuning", default=None, help="item_grab,cursors: use this launch shape (no autotuning)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the mapping shards by grid slab across GPUs; that path is "
@@ -124,6 +125,9 @@ def main():
     out64 = [torch.zeros(nc, dtype=torch.float64, device="cuda:0") for _ in range(4)]
     out32 = [torch.zeros(nc, dtype=torch.float64, device="cuda:0") for _ in range(4)]
     ctx = hip.Context(0)
+    if args.tuning:
+        ctx.set_tuning(*(int(v) for v in args.tuning.split(",")))
+        args.no_autotune = True
 
     ms64, g64, st64, shape64 = run_precision(ctx, hip, dev, dom, res, out64, hip.FP64, args.steps, args.warmup,
                                              not args.no_autotune)
