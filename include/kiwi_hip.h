@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KIWI_HIP_ABI_VERSION 5
+#define KIWI_HIP_ABI_VERSION 6
 
 #define KIWI_OK 0
 #define KIWI_ERR_INVALID_ARG (-1)
@@ -233,6 +233,22 @@ int kiwi_hip_sph_to_grid_adjoint(kiwi_hip_ctx* ctx, const kiwi_particles* partic
 int kiwi_hip_voronoi_map(kiwi_hip_ctx* ctx, const kiwi_particles* particles, const int* home_cell,
                          const kiwi_voronoi_cells* cells, const kiwi_kernel* kernel, int precision,
                          const kiwi_cell_sums* out, kiwi_map_stats* stats);
+
+/* Per-particle rows of the inverse-mapping test. */
+typedef struct kiwi_inverse_rows {
+    double* m_recovered;       /* n: sum over visited cells with M_c > 0 of (Msum m / M_c) M_c */
+    double* coverage_fraction; /* n: m_recovered / m (0 for m <= 0) */
+    int on_device;             /* outputs and cell_mass in device memory */
+} kiwi_inverse_rows;
+
+/* The inverse-mapping test, replacing the particle loop of run_inverse_mapping_test_stage
+ * (src/mapping.cpp:540-701): the map's visits and clamped cell integrals per particle, weighted back by
+ * the forward map's cell masses.  cell_mass[n_cells]: density x volume of each cell from the forward
+ * map (mapping.dat, :553-559), in the cell order of `cells`, where rows->on_device says.  Particles with
+ * h <= 0 or no home cell get 0.  Precision as kiwi_hip_voronoi_map, with the per-pair clamp in every mode. */
+int kiwi_hip_inverse_mapping(kiwi_hip_ctx* ctx, const kiwi_particles* particles, const int* home_cell,
+                             const kiwi_voronoi_cells* cells, const kiwi_kernel* kernel, int precision,
+                             const double* cell_mass, const kiwi_inverse_rows* rows, kiwi_map_stats* stats);
 
 #ifdef __cplusplus
 }

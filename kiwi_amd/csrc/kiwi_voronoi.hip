@@ -627,6 +627,26 @@ __global__ __launch_bounds__(THREADS) void k_vor_cell_ranges(int64_t nvis, const
     if (t == nvis - 1 || (ckeys[t + 1] >> pbits) != c) end[c] = static_cast<int>(t + 1);
 }
 
+// the inverse-mapping test per particle (src/mapping.cpp:640-667): its visited pairs are contiguous and in
+// cell order; cells without forward mass drop out; m_recovered = sum (Ms m / M_c) M_c, coverage = m_rec / m
+__global__ __launch_bounds__(THREADS) void k_vor_inverse(int64_t n, const int64_t* __restrict__ vis_off,
+                                                         const uint64_t* __restrict__ keys, int cbits,
+                                                         const double* __restrict__ ms, const double* __restrict__ pm,
+                                                         const double* __restrict__ cell_mass,
+                                                         double* __restrict__ rec, double* __restrict__ cov) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t cmask = (1ull << cbits) - 1;
+    const double m = pm[i];
+    double r = 0.0;
+    for (int64_t q = vis_off[i]; q < vis_off[i + 1]; ++q) {
+        const double mc = cell_mass[keys[q] & cmask];
+        if (mc > 0.0) r += (ms[q] * m / mc) * mc;
+    }
+    rec[i] = r;
+    cov[i] = (m > 0.0) ? r / m : 0.0;
+}
+
 __global__ __launch_bounds__(THREADS) void k_vor_accum(int64_t nc, const uint64_t* __restrict__ ckeys, int pbits,
                                                        const int* __restrict__ cidx, const int* __restrict__ beg,
                                                        const int* __restrict__ end, const double* __restrict__ ms,
@@ -712,16 +732,14 @@ hipError_t read_back(T* dst, const T* src, hipStream_t s) {
     return hipStreamSynchronize(s);
 }
 
-}  // namespace
-
-extern "C" {
-
-int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* home_cell,
-                         const kiwi_voronoi_cells* V, const kiwi_kernel* K, int precision, const kiwi_cell_sums* out,
-                         kiwi_map_stats* stats) {
+// The map's pipeline; with inv set, the inverse-mapping test's per-particle sums replace the per-cell
+// accumulation (and its outputs).
+int voronoi_run(kiwi_hip_ctx* c, const kiwi_particles* P, const int* home_cell, const kiwi_voronoi_cells* V,
+                const kiwi_kernel* K, int precision, const kiwi_cell_sums* out, const double* cell_mass,
+                const kiwi_inverse_rows* inv, kiwi_map_stats* stats) {
     if (!c) return KIWI_ERR_INVALID_ARG;
     c->err.clear();
-    if (!P || !V || !K || !out) return fail(c, KIWI_ERR_INVALID_ARG, "null argument");
+    if (!P || !V || !K || (!out && !inv)) return fail(c, KIWI_ERR_INVALID_ARG, "null argument");
     if (P->n < 0) return fail(c, KIWI_ERR_INVALID_ARG, "negative particle count");
     if (P->n > 0 && (!P->x || !P->y || !P->z || !P->vx || !P->vy || !P->vz || !P->m || !P->h || !home_cell))
         return fail(c, KIWI_ERR_INVALID_ARG, "null particle array");
@@ -730,8 +748,10 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* ho
     if (!V->cell_face_ptr || (V->n_faces > 0 && (!V->face_vert_ptr || !V->face_neighbor)) ||
         (V->n_face_vertices > 0 && !V->vert_xyz) || (V->n_faces > 0 && V->n_cells == 0))
         return fail(c, KIWI_ERR_INVALID_ARG, "null tessellation array");
-    if (V->n_cells > 0 && (!out->mass || !out->px || !out->py || !out->pz))
+    if (out && V->n_cells > 0 && (!out->mass || !out->px || !out->py || !out->pz))
         return fail(c, KIWI_ERR_INVALID_ARG, "null output array");
+    if (inv && ((V->n_cells > 0 && !cell_mass) || (P->n > 0 && (!inv->m_recovered || !inv->coverage_fraction))))
+        return fail(c, KIWI_ERR_INVALID_ARG, "null cell-mass or per-particle output array");
     for (int a = 0; a < 3; ++a)
         if (!(V->domain[2 * a] < V->domain[2 * a + 1]) || !std::isfinite(V->domain[2 * a]) ||
             !std::isfinite(V->domain[2 * a + 1]))
@@ -741,6 +761,9 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* ho
     if (const int rc = resolve_kernel(c, K, &kc)) return rc;
     // single-precision face integrals: a mixed request with the analytic cubic spline
     const bool single = precision == KIWI_MIXED && !kc.generic && !kc.tabulated;
+    // per-pair clamp: the map's FP64 mode and the inverse test (the reference's compute_cell_integral);
+    // the map's mixed mode clamps cell totals instead
+    const bool clamp_pairs = !single || inv;
     if (P->n > 0x7fffffffLL || V->n_cells > 0x7fffffffLL || V->n_faces > 0x7fffffffLL)
         return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 particles, cells or faces in one call");
 
@@ -789,15 +812,27 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* ho
         vxyz = d_v;
         fnb = d_nb;
     }
-    double* om[4];
-    if (out->on_device) {
+    double* om[4] = {nullptr, nullptr, nullptr, nullptr};
+    const double* cmass = cell_mass;  // inverse test: forward cell masses on the device
+    double *rec = nullptr, *cov = nullptr;
+    if (out && out->on_device) {
         om[0] = out->mass;
         om[1] = out->px;
         om[2] = out->py;
         om[3] = out->pz;
-    } else {
+    } else if (out) {
         KIWI_HIP_CHECK(c, reserve(c->outbuf, 4 * sizeof(double) * std::max<int64_t>(nc, 1)));
         for (int k = 0; k < 4; ++k) om[k] = static_cast<double*>(c->outbuf.p) + k * nc;
+    } else if (inv->on_device) {
+        rec = inv->m_recovered;
+        cov = inv->coverage_fraction;
+    } else {
+        KIWI_HIP_CHECK(c, reserve(c->outbuf, sizeof(double) * (2 * std::max<int64_t>(n, 1) + std::max<int64_t>(nc, 1))));
+        rec = static_cast<double*>(c->outbuf.p);
+        cov = rec + std::max<int64_t>(n, 1);
+        double* dcm = cov + std::max<int64_t>(n, 1);
+        if (nc) KIWI_HIP_CHECK(c, hipMemcpyAsync(dcm, cell_mass, nc * sizeof(double), hipMemcpyHostToDevice, s));
+        cmass = dcm;
     }
 
     KIWI_HIP_CHECK(c, reserve(c->stats, sizeof(MapDevStats)));
@@ -1003,11 +1038,14 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* ho
     int* cend = cbeg + nc;
     if (nvis > 0) {
         k_vor_pairsum<<<grid1d(nvis), THREADS, 0, s>>>(nvis, vkeys, cbits, pbits, slot_off, src, val, ms, ckeys_in,
-                                                       cidx_in, dst, single ? 0 : 1);
+                                                       cidx_in, dst, clamp_pairs ? 1 : 0);
         KIWI_HIP_CHECK(c, hipGetLastError());
-        KIWI_HIP_CHECK(c, sort_pairs(c, ckeys_in, ckeys, cidx_in, cidx, nvis, 0, cbits + pbits, s));
+        if (!inv) KIWI_HIP_CHECK(c, sort_pairs(c, ckeys_in, ckeys, cidx_in, cidx, nvis, 0, cbits + pbits, s));
     }
-    if (nc > 0) {
+    if (inv) {
+        if (n > 0) k_vor_inverse<<<grid1d(n), THREADS, 0, s>>>(n, vis_off, vkeys, cbits, ms, dp[6], cmass, rec, cov);
+        KIWI_HIP_CHECK(c, hipGetLastError());
+    } else if (nc > 0) {
         k_fill_i32<<<grid1d(2 * nc), THREADS, 0, s>>>(cbeg, 2 * nc, 0);
         if (nvis > 0) k_vor_cell_ranges<<<grid1d(nvis), THREADS, 0, s>>>(nvis, ckeys, pbits, cbeg, cend);
         k_vor_accum<<<grid1d(nc), THREADS, 0, s>>>(nc, ckeys, pbits, cidx, cbeg, cend, ms, dp[6], dp[3], dp[4], dp[5],
@@ -1019,10 +1057,14 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* ho
     KIWI_HIP_CHECK(c, hipGetLastError());
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[3], s));
 
-    if (!out->on_device && nc > 0) {
+    if (out && !out->on_device && nc > 0) {
         double* ho[4] = {out->mass, out->px, out->py, out->pz};
         for (int k = 0; k < 4; ++k)
             KIWI_HIP_CHECK(c, hipMemcpyAsync(ho[k], om[k], nc * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    if (inv && !inv->on_device && n > 0) {
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(inv->m_recovered, rec, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(inv->coverage_fraction, cov, n * sizeof(double), hipMemcpyDeviceToHost, s));
     }
     MapDevStats hs;
     double vs[3];
@@ -1051,6 +1093,24 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* ho
         for (int k = 0; k < nw; ++k) stats->warn[k] = hs.warn[k];
     }
     return KIWI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kiwi_hip_voronoi_map(kiwi_hip_ctx* c, const kiwi_particles* P, const int* home_cell,
+                         const kiwi_voronoi_cells* V, const kiwi_kernel* K, int precision, const kiwi_cell_sums* out,
+                         kiwi_map_stats* stats) {
+    if (c && !out) return fail(c, KIWI_ERR_INVALID_ARG, "null argument");
+    return voronoi_run(c, P, home_cell, V, K, precision, out, nullptr, nullptr, stats);
+}
+
+int kiwi_hip_inverse_mapping(kiwi_hip_ctx* c, const kiwi_particles* P, const int* home_cell,
+                             const kiwi_voronoi_cells* V, const kiwi_kernel* K, int precision, const double* cell_mass,
+                             const kiwi_inverse_rows* rows, kiwi_map_stats* stats) {
+    if (c && !rows) return fail(c, KIWI_ERR_INVALID_ARG, "null argument");
+    return voronoi_run(c, P, home_cell, V, K, precision, nullptr, cell_mass, rows, stats);
 }
 
 }  // extern "C"

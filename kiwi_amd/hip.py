@@ -32,7 +32,8 @@ _pd = ctypes.POINTER(ctypes.c_double)
 
 EXPORTED = ["kiwi_hip_abi_version", "kiwi_hip_create", "kiwi_hip_destroy", "kiwi_hip_last_error",
             "kiwi_hip_stream", "kiwi_hip_sph_to_grid", "kiwi_hip_voronoi_map", "kiwi_hip_sph_to_grid_adjoint",
-            "kiwi_hip_sph_to_grid_batch", "kiwi_hip_set_tuning", "kiwi_hip_get_tuning", "kiwi_hip_autotune"]
+            "kiwi_hip_sph_to_grid_batch", "kiwi_hip_set_tuning", "kiwi_hip_get_tuning", "kiwi_hip_autotune",
+            "kiwi_hip_inverse_mapping"]
 
 
 class KiwiError(RuntimeError):
@@ -86,6 +87,10 @@ _pll = ctypes.POINTER(ctypes.c_int64)
 class BatchProblem(ctypes.Structure):
     _fields_ = [("particles", Particles), ("grid", Grid), ("kernel", Kernel), ("precision", _i), ("out", CellSums),
                 ("stats", ctypes.POINTER(Stats))]
+
+
+class InverseRows(ctypes.Structure):
+    _fields_ = [("m_recovered", _pd), ("coverage_fraction", _pd), ("on_device", _i)]
 
 
 class Tuning(ctypes.Structure):
@@ -157,6 +162,10 @@ def load():
                                                    ctypes.POINTER(ParticleGrads), ctypes.POINTER(AdjointStats)]
         L.kiwi_hip_sph_to_grid_batch.restype = _i
         L.kiwi_hip_sph_to_grid_batch.argtypes = [ctypes.c_void_p, _i, ctypes.POINTER(BatchProblem)]
+        L.kiwi_hip_inverse_mapping.restype = _i
+        L.kiwi_hip_inverse_mapping.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.c_void_p,
+                                               ctypes.POINTER(VoronoiCells), ctypes.POINTER(Kernel), _i, _pd,
+                                               ctypes.POINTER(InverseRows), ctypes.POINTER(MapStats)]
         L.kiwi_hip_set_tuning.restype = _i
         L.kiwi_hip_set_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(Tuning)]
         L.kiwi_hip_get_tuning.restype = _i
@@ -339,6 +348,45 @@ class Context:
         vertex), face_nbr (int32, n_faces) and domain (6 floats), all host arrays or all device tensors.
         Returns (mass, px, py, pz, MapStats) with n_cells entries each."""
         kernel = kernel or kernel_spec()
+        P, hp, V, nc = self._map_inputs(parts, home, cells)
+        if out is None:
+            arrs = [np.zeros(max(nc, 0)) for _ in range(4)]
+            O = CellSums(*[a.ctypes.data_as(_pd) for a in arrs], 0)
+        else:
+            arrs = list(out)
+            O = CellSums(*[_ptr(a)[0] for a in arrs], 1)
+        st = MapStats()
+        self._check(self.L.kiwi_hip_voronoi_map(self.h, ctypes.byref(P), ctypes.cast(hp, ctypes.c_void_p),
+                                                ctypes.byref(V), ctypes.byref(kernel), int(precision),
+                                                ctypes.byref(O), ctypes.byref(st)))
+        return arrs[0], arrs[1], arrs[2], arrs[3], st
+
+    def inverse_mapping(self, parts, home, cells, cell_mass, kernel=None, precision=FP64, out=None):
+        """The inverse-mapping test per particle (run_inverse_mapping_test_stage).  parts, home, cells: as
+        voronoi_map; cell_mass: the forward map's density x volume per cell (host array, or a device tensor
+        when out is given).  out: None (host results) or (m_recovered, coverage) device float64 tensors.
+        Returns (m_recovered, coverage_fraction, MapStats)."""
+        kernel = kernel or kernel_spec()
+        P, hp, V, nc = self._map_inputs(parts, home, cells)
+        if out is None:
+            arrs = [np.zeros(max(P.n, 0)) for _ in range(2)]
+            cm = np.ascontiguousarray(cell_mass, dtype=np.float64)
+            R = InverseRows(arrs[0].ctypes.data_as(_pd), arrs[1].ctypes.data_as(_pd), 0)
+            cmp = cm.ctypes.data_as(_pd)
+        else:
+            arrs = list(out)
+            cm = cell_mass
+            R = InverseRows(_ptr(arrs[0])[0], _ptr(arrs[1])[0], 1)
+            cmp, cdev = _ptr(cm)
+            if not cdev:
+                raise ValueError("cell_mass must be a device tensor when the outputs are")
+        st = MapStats()
+        self._check(self.L.kiwi_hip_inverse_mapping(self.h, ctypes.byref(P), ctypes.cast(hp, ctypes.c_void_p),
+                                                    ctypes.byref(V), ctypes.byref(kernel), int(precision), cmp,
+                                                    ctypes.byref(R), ctypes.byref(st)))
+        return arrs[0], arrs[1], st
+
+    def _map_inputs(self, parts, home, cells):
         P = _particles(parts)
         hp, hdev = _ptr(home, _pi, "int32")
         if P.n and hdev != P.on_device:
@@ -353,17 +401,8 @@ class Context:
         nf = len(cells["face_vert_ptr"]) - 1
         nv = int(np.prod(cells["vert_xyz"].shape)) // 3
         V = VoronoiCells(nc, nf, nv, cfp, fvp, vx, fnb, (_d * 6)(*[float(v) for v in cells["domain"]]), d0)
-        if out is None:
-            arrs = [np.zeros(max(nc, 0)) for _ in range(4)]
-            O = CellSums(*[a.ctypes.data_as(_pd) for a in arrs], 0)
-        else:
-            arrs = list(out)
-            O = CellSums(*[_ptr(a)[0] for a in arrs], 1)
-        st = MapStats()
-        self._check(self.L.kiwi_hip_voronoi_map(self.h, ctypes.byref(P), ctypes.cast(hp, ctypes.c_void_p),
-                                                ctypes.byref(V), ctypes.byref(kernel), int(precision),
-                                                ctypes.byref(O), ctypes.byref(st)))
-        return arrs[0], arrs[1], arrs[2], arrs[3], st
+        # the structures point into the caller's arrays, which outlive the call
+        return P, hp, V, nc
 
 
 def normalise_voronoi(mass, px, py, pz, volume, v_mean):

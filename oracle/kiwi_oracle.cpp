@@ -1021,6 +1021,37 @@ int oracle_voronoi_map(long long n, const double* x, const double* y, const doub
     return 0;
 }
 
+// The inverse-mapping test (run_inverse_mapping_test_stage, src/mapping.cpp:540-701): the map's BFS per
+// particle, with the particle's clamped cell integrals weighted back by the forward map's cell masses
+// M_c (density x volume): m_recovered = sum over visited cells with M_c > 0 of (Ms m / M_c) M_c
+// (:652-660), coverage = m_recovered / m for m > 0 (:666-667).  Particles with h <= 0 or no home cell
+// keep 0 (:626-644).
+int oracle_inverse_mapping(long long n, const double* x, const double* y, const double* z, const double* m,
+                           const double* h, const int* pid, const int* home, long long ncells,
+                           const long long* cell_face_ptr, const long long* face_vert_ptr, const double* face_xyz,
+                           const int* face_nbr, const OracleKernel* k, const double* cell_mass,
+                           double* m_recovered, double* coverage, long long* counts) {
+    Integrator ki;
+    if (!make_integrator(k, &ki)) return -1;
+    for (long long i = 0; i < n; ++i) m_recovered[i] = coverage[i] = 0.0;
+    const VoronoiCounts vc =
+        voronoi_bfs(ki, n, x, y, z, h, pid, home, ncells, cell_face_ptr, face_vert_ptr, face_xyz, face_nbr,
+                    [&](long long i, int cell, double Ms) {
+                        const double mi = cell_mass[cell];
+                        if (mi > 0.0) m_recovered[i] += (Ms * m[i] / mi) * mi;
+                    });
+    for (long long i = 0; i < n; ++i) coverage[i] = (m[i] > 0.0) ? m_recovered[i] / m[i] : 0.0;
+    if (counts) {
+        counts[0] = vc.unresolved;
+        counts[1] = vc.unmapped;
+        counts[2] = vc.nneg;
+        counts[3] = vc.nwarn;
+        counts[4] = vc.visited;
+        counts[5] = vc.face_evals;
+    }
+    return 0;
+}
+
 // The Voronoi map's sums for a list of cells with the face integrals in binary128: the double-precision
 // BFS above decides which (particle, cell) pairs are visited; each visited pair's cell integral is then
 // redone in binary128 over the cell's faces (vertices as given), clamped at 0 and accumulated in

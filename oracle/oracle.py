@@ -69,6 +69,9 @@ def lib():
         L.oracle_voronoi_map.restype = _i
         L.oracle_voronoi_map.argtypes = [_ll] + [_pd] * 8 + [_pi, _pi, _ll, _pll, _pll, _pd, _pi,
                                                               ctypes.POINTER(OracleKernel), _pd, _pd, _pd, _pd, _pll]
+        L.oracle_inverse_mapping.restype = _i
+        L.oracle_inverse_mapping.argtypes = [_ll] + [_pd] * 5 + [_pi, _pi, _ll, _pll, _pll, _pd, _pi,
+                                                                  ctypes.POINTER(OracleKernel), _pd, _pd, _pd, _pll]
         L.oracle_voronoi_map_cells_q.restype = _i
         L.oracle_voronoi_map_cells_q.argtypes = [_ll] + [_pd] * 8 + [_pi, _pi, _ll, _pll, _pll, _pd, _pi,
                                                                       ctypes.POINTER(OracleKernel), _ll, _pll, _pd]
@@ -202,6 +205,30 @@ def voronoi_map(parts, pid, home, cell_face_ptr, face_vert_ptr, face_xyz, face_n
     if rc != 0:
         raise ValueError(f"oracle_voronoi_map failed ({rc})")
     return out[0], out[1], out[2], out[3], counts
+
+
+def inverse_mapping(parts, pid, home, cell_face_ptr, face_vert_ptr, face_xyz, face_nbr, cell_mass, k=None):
+    """The inverse-mapping test (src/mapping.cpp:540-701) per particle: (m_recovered, coverage, counts),
+    with cell_mass the forward map's density x volume per cell."""
+    k = k or kernel()
+    n = len(parts["x"])
+    cols = [np.ascontiguousarray(parts[c], dtype=np.float64) for c in ("x", "y", "z", "m", "h")]
+    pid = np.ascontiguousarray(pid, dtype=np.int32)
+    home = np.ascontiguousarray(home, dtype=np.int32)
+    cfp = np.ascontiguousarray(cell_face_ptr, dtype=np.int64)
+    fvp = np.ascontiguousarray(face_vert_ptr, dtype=np.int64)
+    fxyz = np.ascontiguousarray(face_xyz, dtype=np.float64)
+    fnb = np.ascontiguousarray(face_nbr, dtype=np.int32)
+    cm = np.ascontiguousarray(cell_mass, dtype=np.float64)
+    rec, cov = np.zeros(n), np.zeros(n)
+    counts = np.zeros(6, dtype=np.int64)
+    rc = lib().oracle_inverse_mapping(n, *[_p(c) for c in cols], pid.ctypes.data_as(_pi), home.ctypes.data_as(_pi),
+                                      len(cfp) - 1, cfp.ctypes.data_as(_pll), fvp.ctypes.data_as(_pll), _p(fxyz),
+                                      fnb.ctypes.data_as(_pi), ctypes.byref(k), _p(cm), _p(rec), _p(cov),
+                                      counts.ctypes.data_as(_pll))
+    if rc != 0:
+        raise ValueError(f"oracle_inverse_mapping failed ({rc})")
+    return rec, cov, counts
 
 
 def voronoi_map_cells_quad(parts, pid, home, cell_face_ptr, face_vert_ptr, face_xyz, face_nbr, cells, k=None):

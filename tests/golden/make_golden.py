@@ -19,7 +19,11 @@ fixtures it writes are plain numbers (inputs and the reference's outputs), commi
                         velocity of mapping.dat, with the tessellation of the same sites by the reference's
                         vendored voro++ as CSR arrays and each particle's home cell
 
-  python tests/golden/make_golden.py [--only kernel_grids | --only voronoi]
+  inverse_mapping.npz   `kiwi inverse-mapping-test` (reference CLI, OMP_NUM_THREADS=1) chained from the same
+                        `kiwi map` run: per particle (by id) m_original, m_recovered, coverage_fraction,
+                        and the totals line of inverse-mapping-test.dat
+
+  python tests/golden/make_golden.py [--only kernel_grids | --only voronoi | --only inverse]
 """
 import ctypes
 import json
@@ -95,7 +99,7 @@ def kernel_grids():
     np.savez_compressed(os.path.join(OUT, "kernel_grids.npz"), **out)
 
 
-def run_ref_map(parts, domain, threads=1):
+def run_ref_map(parts, domain, threads=1, inverse=False):
     with tempfile.TemporaryDirectory() as td:
         snap = os.path.join(td, "snap.txt")
         workloads.write_plain_ascii(parts, snap)
@@ -109,7 +113,13 @@ def run_ref_map(parts, domain, threads=1):
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
         r = subprocess.run([O.REF_CLI, "map", cp], env=env, capture_output=True, text=True, check=True)
         rows = np.loadtxt(os.path.join(td, "work", "snap", "mapping.dat"))
-        return rows, r.stderr
+        if not inverse:
+            return rows, r.stderr
+        subprocess.run([O.REF_CLI, "inverse-mapping-test", cp], env=env, capture_output=True, text=True,
+                       check=True)
+        inv = np.loadtxt(os.path.join(td, "work", "snap", "inverse-mapping-test.dat"), skiprows=1, max_rows=1)
+        prow = np.loadtxt(os.path.join(td, "work", "snap", "inverse-mapping-test.dat"), skiprows=2, ndmin=2)
+        return rows, inv, prow
 
 
 def voronoi():
@@ -130,12 +140,25 @@ def voronoi():
                         ref_vz=ref[:, 8], ref_site=ref[:, 1:4])
 
 
+def inverse():
+    """The reference's inverse-mapping test on the voronoi() setup (same particles, sites, domain)."""
+    c1 = np.load(os.path.join(OUT, "c1_snapshot.npz"))
+    parts = {k: np.ascontiguousarray(c1[k][::8]) for k in ("x", "y", "z", "vx", "vy", "vz", "m", "h", "rho")}
+    domain = np.array(c1["domain"])
+    _, totals, prow = run_ref_map(parts, domain, inverse=True)
+    order = np.argsort(prow[:, 0])
+    prow = prow[order]
+    np.savez_compressed(os.path.join(OUT, "inverse_mapping.npz"), pid=prow[:, 0].astype(np.int32),
+                        m_original=prow[:, 1], m_recovered=prow[:, 2], coverage=prow[:, 3],
+                        total_original=totals[0], total_grid=totals[1], total_recovered=totals[2])
+
+
 def main():
     if not O.have_ref() or not os.path.exists(O.REF_CLI):
         sys.exit("oracle/_ref is not built (make -f oracle/ref.mk)")
     if "--only" in sys.argv:
         what = sys.argv[sys.argv.index("--only") + 1]
-        {"kernel_grids": kernel_grids, "voronoi": voronoi}[what]()
+        {"kernel_grids": kernel_grids, "voronoi": voronoi, "inverse": inverse}[what]()
         return
     R = O.ref()
     rng = np.random.default_rng(20251203)
@@ -224,6 +247,7 @@ def main():
     np.savez_compressed(os.path.join(OUT, "m4_gridded.npz"), h=h, phi=phi, r0=r0, R0=R0, value=val)
     kernel_grids()
     voronoi()
+    inverse()
     print("golden fixtures written to", OUT)
 
 

@@ -57,7 +57,8 @@ def test_struct_layouts_match_header(tmp_path):
               "kiwi_particle_grads": [f for f, _ in hip.ParticleGrads._fields_],
               "kiwi_adjoint_stats": [f for f, _ in hip.AdjointStats._fields_],
               "kiwi_batch_problem": [f for f, _ in hip.BatchProblem._fields_],
-              "kiwi_tuning": [f for f, _ in hip.Tuning._fields_]}
+              "kiwi_tuning": [f for f, _ in hip.Tuning._fields_],
+              "kiwi_inverse_rows": [f for f, _ in hip.InverseRows._fields_]}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for st, fs in fields.items():
         lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
@@ -73,7 +74,8 @@ def test_struct_layouts_match_header(tmp_path):
                     ("kiwi_cell_sums", hip.CellSums), ("kiwi_stats", hip.Stats),
                     ("kiwi_voronoi_cells", hip.VoronoiCells), ("kiwi_map_stats", hip.MapStats),
                     ("kiwi_particle_grads", hip.ParticleGrads), ("kiwi_adjoint_stats", hip.AdjointStats),
-                    ("kiwi_batch_problem", hip.BatchProblem), ("kiwi_tuning", hip.Tuning)):
+                    ("kiwi_batch_problem", hip.BatchProblem), ("kiwi_tuning", hip.Tuning),
+                    ("kiwi_inverse_rows", hip.InverseRows)):
         assert int(out[st]) == ctypes.sizeof(cls), st
         for f, _ in cls._fields_:
             assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, (st, f)

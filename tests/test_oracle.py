@@ -316,6 +316,33 @@ def test_voronoi_map_matches_reference_cli():
     assert abs(vol.sum() - np.prod(dom[1::2] - dom[::2])) < 1e-10 * np.prod(dom[1::2] - dom[::2])
 
 
+def test_inverse_mapping_matches_reference_cli():
+    """The oracle's inverse-mapping test (src/mapping.cpp:540-701) against the reference CLI's
+    inverse-mapping-test.dat for the setup of voronoi_map.npz (tests/golden/inverse_mapping.npz, 15
+    significant digits), with the forward masses density x volume taken from the reference's own
+    mapping.dat, as the stage does (:553-559).  Every particle's support lies inside the padded domain,
+    so every coverage is 1 to rounding; the per-particle recovered masses pin the sum itself."""
+    g = np.load(os.path.join(GOLD, "voronoi_map.npz"))
+    gi = np.load(os.path.join(GOLD, "inverse_mapping.npz"))
+    parts = {k: g["p_" + k] for k in ("x", "y", "z", "vx", "vy", "vz", "m", "h")}
+    assert np.array_equal(gi["pid"], g["pid"])
+    cell_mass = g["ref_density"] * g["ref_volume"]
+    rec, cov, counts = O.inverse_mapping(parts, g["pid"], g["home"], g["cell_face_ptr"], g["face_vert_ptr"],
+                                         g["vert_xyz"].reshape(-1), g["face_nbr"], cell_mass)
+    np.testing.assert_allclose(gi["m_original"], parts["m"], rtol=1e-14)
+    np.testing.assert_allclose(rec, gi["m_recovered"], rtol=1e-11, atol=0)
+    np.testing.assert_allclose(cov, gi["coverage"], rtol=1e-11, atol=0)
+    assert abs(rec.sum() - gi["total_recovered"]) <= 1e-12 * gi["total_recovered"]
+    assert counts[0] == 0
+    # a cell without forward mass drops out of every particle's sum (:656-660)
+    cm = cell_mass.copy()
+    c0 = int(g["home"][0])
+    cm[c0] = 0.0
+    rec0, _, _ = O.inverse_mapping(parts, g["pid"], g["home"], g["cell_face_ptr"], g["face_vert_ptr"],
+                                   g["vert_xyz"].reshape(-1), g["face_nbr"], cm)
+    assert rec0[0] < rec[0] and np.all(rec0 <= rec + 1e-15)
+
+
 def test_voronoi_binary128_arbiter():
     """The binary128 cell sums used to arbitrate Voronoi cells (oracle.voronoi_map_cells_quad): the
     reference's analytic cubic algorithm in binary128 equals the series-free generic path for the same
