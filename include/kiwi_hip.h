@@ -245,7 +245,8 @@ typedef struct kiwi_inverse_rows {
  * (src/mapping.cpp:540-701): the map's visits and clamped cell integrals per particle, weighted back by
  * the forward map's cell masses.  cell_mass[n_cells]: density x volume of each cell from the forward
  * map (mapping.dat, :553-559), in the cell order of `cells`, where rows->on_device says.  Particles with
- * h <= 0 or no home cell get 0.  Precision as kiwi_hip_voronoi_map, with the per-pair clamp in every mode. */
+ * h <= 0 or no home cell get 0.  Precision as kiwi_hip_voronoi_map (in KIWI_MIXED no per-pair clamp, so
+ * that single-precision noise around 0 does not bias the particle's sum). */
 int kiwi_hip_inverse_mapping(kiwi_hip_ctx* ctx, const kiwi_particles* particles, const int* home_cell,
                              const kiwi_voronoi_cells* cells, const kiwi_kernel* kernel, int precision,
                              const double* cell_mass, const kiwi_inverse_rows* rows, kiwi_map_stats* stats);

@@ -761,9 +761,10 @@ int voronoi_run(kiwi_hip_ctx* c, const kiwi_particles* P, const int* home_cell, 
     if (const int rc = resolve_kernel(c, K, &kc)) return rc;
     // single-precision face integrals: a mixed request with the analytic cubic spline
     const bool single = precision == KIWI_MIXED && !kc.generic && !kc.tabulated;
-    // per-pair clamp: the map's FP64 mode and the inverse test (the reference's compute_cell_integral);
-    // the map's mixed mode clamps cell totals instead
-    const bool clamp_pairs = !single || inv;
+    // per-pair clamp (the reference's compute_cell_integral) in FP64; in single precision a per-pair
+    // clamp of rounding noise around 0 would bias the sums upward, so the map clamps cell totals instead
+    // and the inverse test keeps the noise in the particle's sum
+    const bool clamp_pairs = !single;
     if (P->n > 0x7fffffffLL || V->n_cells > 0x7fffffffLL || V->n_faces > 0x7fffffffLL)
         return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 particles, cells or faces in one call");
 

@@ -8,7 +8,8 @@ sigma_ln_rho 1.5), sites = the particle positions plus the coverage-grid buffer 
 lattice of spacing 0.05 x the smallest extent, kept where no particle lies within 2 h; src/point_set.cpp:
 167-272), then one Lloyd step (each site moved to its cell's centroid; src/point_set.cpp:275-306), M4
 kernel, FP64; then the same map in mixed precision (face integrals in FP32), timed and compared with
-the FP64 result.
+the FP64 result; then the inverse-mapping test (kiwi_hip_inverse_mapping) on the same inputs with the
+FP64 map's cell masses, in both precisions.
 
     python tests/perf_voronoi_map.py [--particles 200000] [--steps 5] [--warmup 2] [--sample 3000]
 
@@ -130,6 +131,25 @@ def main():
                                     for n, a, b in zip(("mass", "px", "py", "pz"), f32, f64)}}
     log(f"mixed: {ms32:.2f} ms/call (faces {st32.device_ms_faces:.2f}); total mass {mixed['total_mass_rel_err']:.2e}, "
         f"fields {mixed['field_err_over_max']}")
+    # the inverse-mapping test on the same inputs, forward cell masses from the FP64 map
+    cm = out[0].clone()
+    drec = torch.zeros(len(parts["x"]), dtype=torch.float64, device=dev)
+    dcov = torch.zeros_like(drec)
+    inverse = {}
+    for name, prec in (("fp64", hip.FP64), ("mixed", hip.MIXED)):
+        ctx.inverse_mapping(dparts, dhome, dcells, cm, precision=prec, out=(drec, dcov))
+        torch.cuda.synchronize()
+        ti = []
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            ctx.inverse_mapping(dparts, dhome, dcells, cm, precision=prec, out=(drec, dcov))
+            torch.cuda.synchronize()
+            ti.append(time.perf_counter() - t)
+        cv = dcov.cpu().numpy()
+        inverse[name] = {"ms_per_call": 1e3 * float(np.median(ti)), "coverage_min": float(cv.min()),
+                         "coverage_max": float(cv.max()),
+                         "recovered_over_original": float(drec.sum().item() / parts["m"].sum())}
+    log(f"inverse-mapping test: {inverse}")
     res = {"workload": "voronoi map, 2e5 log-normal particles (256^3 field, sigma_ln_rho 1.5) + coverage-grid "
                        "buffer, 1 Lloyd step, M4, FP64 (and mixed)",
            "particles": args.particles, "cells": nc, "faces": len(cells["face_nbr"]),
@@ -139,7 +159,8 @@ def main():
            "cell_integrals_per_s": st.pairs_visited / (ms * 1e-3),
            "face_integrals_per_s": st.face_evals / (ms * 1e-3),
            "face_kernel_face_integrals_per_s": st.face_evals / (st.device_ms_faces * 1e-3),
-           "mass_total_over_particles": float(mass.sum() / parts["m"].sum()), "mixed": mixed}
+           "mass_total_over_particles": float(mass.sum() / parts["m"].sum()), "mixed": mixed,
+           "inverse_mapping": inverse}
     if args.sample > 0:
         k = min(args.sample, args.particles)
         sub = {c: np.ascontiguousarray(v[:k]) for c, v in parts.items()}
