@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KIWI_HIP_ABI_VERSION 6
+#define KIWI_HIP_ABI_VERSION 7
 
 #define KIWI_OK 0
 #define KIWI_ERR_INVALID_ARG (-1)
@@ -178,6 +178,39 @@ typedef struct kiwi_batch_problem {
  * those kiwi_hip_sph_to_grid would return for it alone (device times cover its group's run;
  * particle_blocks is -1 when the group holds more than one problem). */
 int kiwi_hip_sph_to_grid_batch(kiwi_hip_ctx* ctx, int n_problems, const kiwi_batch_problem* problems);
+
+/* The point set of a Voronoi map with each site's mapping.dat row (the cartesian-map input). */
+typedef struct kiwi_sites {
+    int64_t n;
+    const double *x, *y, *z;              /* site positions; sites outside the domain take no part */
+    const int* id;                        /* voro++ particle id of each site */
+    const double *density, *vx, *vy, *vz; /* the mapping.dat row of the site's cell */
+    const int* mapped;                    /* 0 where the site's cell has no mapping.dat row; NULL: all have */
+    int on_device;
+} kiwi_sites;
+
+/* cartesian_map.dat fields, res^3 each, x-major ((ix res + iy) res + iz). */
+typedef struct kiwi_cartesian_map_out {
+    double *density, *vx, *vy, *vz;
+    int* cell_id; /* the voro++ id of the Voronoi cell the centre lies in (0 when unresolved) */
+    int on_device;
+} kiwi_cartesian_map_out;
+
+typedef struct kiwi_cartesian_map_stats {
+    int64_t unresolved; /* centres without an enclosing cell (every centre when there are no sites) */
+    int64_t unmapped;   /* centres whose cell has no mapping.dat row (left at 0) */
+    double device_ms_prep, device_ms_query;
+} kiwi_cartesian_map_stats;
+
+/* The cartesian-map resample, replacing the cell loop of run_cartesian_map_stage
+ * (src/cartesian_map.cpp:148-183): every cell centre of a res^3 grid over domain[6] takes the row of the
+ * Voronoi cell it lies in (its nearest site by squared distance, as voro++ find_voronoi_cell).  Of sites
+ * at exactly equal distance the lowest index wins (voro++ keeps the first its block search meets).  Sites
+ * outside [min, max] on any axis are dropped, as voro++'s pre_container::put drops them (:67-69).  A site
+ * exactly on a max face is kept; voro++ keeps or drops it by the rounding of its block index, so pass the
+ * tessellation's own cells as the sites where such points occur. */
+int kiwi_hip_cartesian_map(kiwi_hip_ctx* ctx, const kiwi_sites* sites, const double domain[6], int res,
+                           const kiwi_cartesian_map_out* out, kiwi_cartesian_map_stats* stats);
 
 /* Launch shape of the sph-to-grid cell kernel (no reference counterpart).  Results never depend on it:
  * every (particle, cell block) item is computed and placed the same way whatever the shape. */

@@ -26,7 +26,8 @@ def _run(cmd):
 
 
 def build_lib():
-    srcs = [os.path.join(PKG_DIR, "csrc", f) for f in ("kiwi_hip.hip", "kiwi_voronoi.hip", "kiwi_adjoint.hip")]
+    srcs = [os.path.join(PKG_DIR, "csrc", f)
+            for f in ("kiwi_hip.hip", "kiwi_voronoi.hip", "kiwi_adjoint.hip", "kiwi_cartesian_map.hip")]
     tmp = LIB_PATH + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT_DIR, "include"), *srcs, "-o", tmp])
     os.replace(tmp, LIB_PATH)

@@ -33,7 +33,7 @@ _pd = ctypes.POINTER(ctypes.c_double)
 EXPORTED = ["kiwi_hip_abi_version", "kiwi_hip_create", "kiwi_hip_destroy", "kiwi_hip_last_error",
             "kiwi_hip_stream", "kiwi_hip_sph_to_grid", "kiwi_hip_voronoi_map", "kiwi_hip_sph_to_grid_adjoint",
             "kiwi_hip_sph_to_grid_batch", "kiwi_hip_set_tuning", "kiwi_hip_get_tuning", "kiwi_hip_autotune",
-            "kiwi_hip_inverse_mapping"]
+            "kiwi_hip_inverse_mapping", "kiwi_hip_cartesian_map"]
 
 
 class KiwiError(RuntimeError):
@@ -82,6 +82,7 @@ class Stats(ctypes.Structure):
 
 
 _pll = ctypes.POINTER(ctypes.c_int64)
+_pi = ctypes.POINTER(ctypes.c_int)
 
 
 class BatchProblem(ctypes.Structure):
@@ -93,9 +94,21 @@ class InverseRows(ctypes.Structure):
     _fields_ = [("m_recovered", _pd), ("coverage_fraction", _pd), ("on_device", _i)]
 
 
+class Sites(ctypes.Structure):
+    _fields_ = [("n", _i64), ("x", _pd), ("y", _pd), ("z", _pd), ("id", _pi), ("density", _pd), ("vx", _pd),
+                ("vy", _pd), ("vz", _pd), ("mapped", _pi), ("on_device", _i)]
+
+
+class CartesianMapOut(ctypes.Structure):
+    _fields_ = [("density", _pd), ("vx", _pd), ("vy", _pd), ("vz", _pd), ("cell_id", _pi), ("on_device", _i)]
+
+
+class CartesianMapStats(ctypes.Structure):
+    _fields_ = [("unresolved", _i64), ("unmapped", _i64), ("device_ms_prep", _d), ("device_ms_query", _d)]
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [("item_grab", _i), ("cursors", _i)]
-_pi = ctypes.POINTER(ctypes.c_int)
 
 
 class VoronoiCells(ctypes.Structure):
@@ -166,6 +179,9 @@ def load():
         L.kiwi_hip_inverse_mapping.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.c_void_p,
                                                ctypes.POINTER(VoronoiCells), ctypes.POINTER(Kernel), _i, _pd,
                                                ctypes.POINTER(InverseRows), ctypes.POINTER(MapStats)]
+        L.kiwi_hip_cartesian_map.restype = _i
+        L.kiwi_hip_cartesian_map.argtypes = [ctypes.c_void_p, ctypes.POINTER(Sites), ctypes.POINTER(_d), _i,
+                                             ctypes.POINTER(CartesianMapOut), ctypes.POINTER(CartesianMapStats)]
         L.kiwi_hip_set_tuning.restype = _i
         L.kiwi_hip_set_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(Tuning)]
         L.kiwi_hip_get_tuning.restype = _i
@@ -385,6 +401,39 @@ class Context:
                                                     ctypes.byref(V), ctypes.byref(kernel), int(precision), cmp,
                                                     ctypes.byref(R), ctypes.byref(st)))
         return arrs[0], arrs[1], st
+
+    def cartesian_map(self, sites, domain, res, out=None):
+        """The cartesian-map resample.  sites: dict of x, y, z (float64), id (int32), density, vx, vy, vz
+        (float64, each site's mapping.dat row) and optionally mapped (int32, 0 = no row), all host arrays or
+        all device tensors.  out: None (host results) or (density, vx, vy, vz, cell_id) device tensors of
+        res^3.  Returns (density, vx, vy, vz, cell_id, CartesianMapStats)."""
+        cols = [_ptr(sites[k]) for k in ("x", "y", "z")]
+        idp = _ptr(sites["id"], _pi, "int32")
+        vals = [_ptr(sites[k]) for k in ("density", "vx", "vy", "vz")]
+        mp = _ptr(sites["mapped"], _pi, "int32") if sites.get("mapped") is not None else (None, cols[0][1])
+        if len({q[1] for q in cols + vals + [idp, mp]}) != 1:
+            raise ValueError("site arrays must all be host arrays or all device tensors")
+        n = len(sites["x"])
+        if any(len(sites[k]) != n for k in sites if sites[k] is not None):
+            raise ValueError("site arrays must all have the same length")
+        S = Sites(n, *[q[0] for q in cols], idp[0], *[q[0] for q in vals], mp[0], cols[0][1])
+        n3 = int(res) ** 3 if 0 < int(res) <= 2048 else 0  # the library rejects other resolutions
+        if out is not None and any(len(a) != n3 for a in out):
+            raise ValueError("out tensors must hold res^3 cells each")
+        if out is None:
+            arrs = [np.zeros(n3) for _ in range(4)] + [np.zeros(n3, dtype=np.int32)]
+            O = CartesianMapOut(*[a.ctypes.data_as(_pd) for a in arrs[:4]], arrs[4].ctypes.data_as(_pi), 0)
+        else:
+            arrs = list(out)
+            op = [_ptr(a) for a in arrs[:4]] + [_ptr(arrs[4], _pi, "int32")]
+            if any(q[1] != 1 for q in op):
+                raise ValueError("out must be device tensors")
+            O = CartesianMapOut(*[q[0] for q in op], 1)
+        dom = (_d * 6)(*[float(v) for v in domain])
+        st = CartesianMapStats()
+        self._check(self.L.kiwi_hip_cartesian_map(self.h, ctypes.byref(S), dom, int(res), ctypes.byref(O),
+                                                  ctypes.byref(st)))
+        return arrs[0], arrs[1], arrs[2], arrs[3], arrs[4], st
 
     def _map_inputs(self, parts, home, cells):
         P = _particles(parts)

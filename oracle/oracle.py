@@ -207,6 +207,50 @@ def voronoi_map(parts, pid, home, cell_face_ptr, face_vert_ptr, face_xyz, face_n
     return out[0], out[1], out[2], out[3], counts
 
 
+def nearest_site(sites, q, chunk=4096):
+    """Index of the nearest of sites (n, 3) to every point of q (m, 3), brute force: (s - q)^2 summed x,
+    y, z in double with no fused multiply-add, as voro++ forms it (v_compute.cc); the first minimum
+    (lowest index) on exact ties."""
+    best = np.empty(len(q), np.int64)
+    for c0 in range(0, len(q), chunk):
+        qq = q[c0:c0 + chunk]
+        dx = sites[None, :, 0] - qq[:, None, 0]
+        dy = sites[None, :, 1] - qq[:, None, 1]
+        dz = sites[None, :, 2] - qq[:, None, 2]
+        best[c0:c0 + chunk] = np.argmin((dx * dx + dy * dy) + dz * dz, axis=1)
+    return best
+
+
+def cartesian_map(sites, ids, rows, domain, res, mapped=None, chunk=4096):
+    """The cartesian-map resample (src/cartesian_map.cpp:150-183) by brute force: every cell centre
+    x_min + (i + 0.5) * (extent / res) (:157-164), in (ix, iy, iz) order with z fastest (:164-165), takes
+    the row of its nearest site, i.e. of the Voronoi cell voro++'s find_voronoi_cell reports.  Distances
+    are (s - q)^2 summed x, y, z in double with no fused multiply-add (voro++ v_compute.cc); on an exact
+    tie the lowest site index wins.  Sites outside [min, max] on an axis are dropped, as voro++'s
+    pre_container::put drops them (:67-69).  sites: (n, 3); rows: (n, 4) density, vx, vy, vz; mapped:
+    optional (n,) bool, False = the site's id has no mapping.dat row (:175-178: id kept, row zero).
+    Returns (rows (res^3, 4), cell_id (res^3,), unresolved, unmapped)."""
+    d = np.asarray(domain, dtype=np.float64)
+    sites = np.asarray(sites, dtype=np.float64).reshape(-1, 3)
+    keep = np.all((sites >= d[0::2]) & (sites <= d[1::2]), axis=1)
+    ids, rows = np.asarray(ids)[keep], np.asarray(rows, dtype=np.float64).reshape(-1, 4)[keep]
+    mapped = None if mapped is None else np.asarray(mapped, bool)[keep]
+    sites = sites[keep]
+    ax = [d[2 * a] + (np.arange(res) + 0.5) * ((d[2 * a + 1] - d[2 * a]) / res) for a in range(3)]
+    X, Y, Z = np.meshgrid(*ax, indexing="ij")
+    q = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    nc = len(q)
+    out = np.zeros((nc, 4))
+    cid = np.zeros(nc, np.int32)
+    if len(sites) == 0:
+        return out, cid, nc, 0
+    best = nearest_site(sites, q, chunk)
+    cid[:] = ids[best]
+    ok = np.ones(nc, bool) if mapped is None else mapped[best]
+    out[ok] = rows[best[ok]]
+    return out, cid, 0, int((~ok).sum())
+
+
 def inverse_mapping(parts, pid, home, cell_face_ptr, face_vert_ptr, face_xyz, face_nbr, cell_mass, k=None):
     """The inverse-mapping test (src/mapping.cpp:540-701) per particle: (m_recovered, coverage, counts),
     with cell_mass the forward map's density x volume per cell."""

@@ -23,7 +23,10 @@ fixtures it writes are plain numbers (inputs and the reference's outputs), commi
                         `kiwi map` run: per particle (by id) m_original, m_recovered, coverage_fraction,
                         and the totals line of inverse-mapping-test.dat
 
-  python tests/golden/make_golden.py [--only kernel_grids | --only voronoi | --only inverse]
+  cartesian_map.npz     `kiwi cartesian-map` (reference CLI, resolution 20) chained from the same `kiwi map` run:
+                        the density, velocity and Voronoi cell id fields of cartesian_map.dat
+
+  python tests/golden/make_golden.py [--only kernel_grids | --only voronoi | --only inverse | --only cmap]
 """
 import ctypes
 import json
@@ -99,7 +102,7 @@ def kernel_grids():
     np.savez_compressed(os.path.join(OUT, "kernel_grids.npz"), **out)
 
 
-def run_ref_map(parts, domain, threads=1, inverse=False):
+def run_ref_map(parts, domain, threads=1, inverse=False, cmap_res=0):
     with tempfile.TemporaryDirectory() as td:
         snap = os.path.join(td, "snap.txt")
         workloads.write_plain_ascii(parts, snap)
@@ -107,12 +110,25 @@ def run_ref_map(parts, domain, threads=1, inverse=False):
                "work_folder": os.path.join(td, "work"),
                "domain": dict(zip(["x_min", "x_max", "y_min", "y_max", "z_min", "z_max"], map(float, domain))),
                "buffer": {"enabled": False}, "lloyd": {"iterations": 0}}
+        if cmap_res:
+            cfg["cartesian_map"] = {"resolution": int(cmap_res)}
         cp = os.path.join(td, "cfg.json")
         with open(cp, "w") as f:
             json.dump(cfg, f)
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
         r = subprocess.run([O.REF_CLI, "map", cp], env=env, capture_output=True, text=True, check=True)
         rows = np.loadtxt(os.path.join(td, "work", "snap", "mapping.dat"))
+        if cmap_res:
+            subprocess.run([O.REF_CLI, "cartesian-map", cp], env=env, capture_output=True, text=True, check=True)
+            with open(os.path.join(td, "work", "snap", "cartesian_map.dat"), "rb") as f:
+                raw = f.read()
+            assert raw[:8] == b"VGTCMAP1"
+            res = int(np.frombuffer(raw, np.int32, 1, 8)[0])
+            n3 = res ** 3
+            off = 12 + 48
+            fields = np.frombuffer(raw, np.float64, 4 * n3, off).reshape(4, n3)
+            ids = np.frombuffer(raw, np.int32, n3, off + 32 * n3)
+            return rows, res, fields, ids
         if not inverse:
             return rows, r.stderr
         subprocess.run([O.REF_CLI, "inverse-mapping-test", cp], env=env, capture_output=True, text=True,
@@ -153,12 +169,22 @@ def inverse():
                         total_original=totals[0], total_grid=totals[1], total_recovered=totals[2])
 
 
+def cmap():
+    """The reference's cartesian-map resample on the voronoi() setup, resolution 20."""
+    c1 = np.load(os.path.join(OUT, "c1_snapshot.npz"))
+    parts = {k: np.ascontiguousarray(c1[k][::8]) for k in ("x", "y", "z", "vx", "vy", "vz", "m", "h", "rho")}
+    domain = np.array(c1["domain"])
+    _, res, fields, ids = run_ref_map(parts, domain, cmap_res=20)
+    np.savez_compressed(os.path.join(OUT, "cartesian_map.npz"), res=res, density=fields[0], vx=fields[1],
+                        vy=fields[2], vz=fields[3], cell_id=ids.astype(np.int32))
+
+
 def main():
     if not O.have_ref() or not os.path.exists(O.REF_CLI):
         sys.exit("oracle/_ref is not built (make -f oracle/ref.mk)")
     if "--only" in sys.argv:
         what = sys.argv[sys.argv.index("--only") + 1]
-        {"kernel_grids": kernel_grids, "voronoi": voronoi, "inverse": inverse}[what]()
+        {"kernel_grids": kernel_grids, "voronoi": voronoi, "inverse": inverse, "cmap": cmap}[what]()
         return
     R = O.ref()
     rng = np.random.default_rng(20251203)
@@ -248,6 +274,7 @@ def main():
     kernel_grids()
     voronoi()
     inverse()
+    cmap()
     print("golden fixtures written to", OUT)
 
 

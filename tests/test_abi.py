@@ -58,7 +58,10 @@ def test_struct_layouts_match_header(tmp_path):
               "kiwi_adjoint_stats": [f for f, _ in hip.AdjointStats._fields_],
               "kiwi_batch_problem": [f for f, _ in hip.BatchProblem._fields_],
               "kiwi_tuning": [f for f, _ in hip.Tuning._fields_],
-              "kiwi_inverse_rows": [f for f, _ in hip.InverseRows._fields_]}
+              "kiwi_inverse_rows": [f for f, _ in hip.InverseRows._fields_],
+              "kiwi_sites": [f for f, _ in hip.Sites._fields_],
+              "kiwi_cartesian_map_out": [f for f, _ in hip.CartesianMapOut._fields_],
+              "kiwi_cartesian_map_stats": [f for f, _ in hip.CartesianMapStats._fields_]}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for st, fs in fields.items():
         lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
@@ -75,7 +78,9 @@ def test_struct_layouts_match_header(tmp_path):
                     ("kiwi_voronoi_cells", hip.VoronoiCells), ("kiwi_map_stats", hip.MapStats),
                     ("kiwi_particle_grads", hip.ParticleGrads), ("kiwi_adjoint_stats", hip.AdjointStats),
                     ("kiwi_batch_problem", hip.BatchProblem), ("kiwi_tuning", hip.Tuning),
-                    ("kiwi_inverse_rows", hip.InverseRows)):
+                    ("kiwi_inverse_rows", hip.InverseRows), ("kiwi_sites", hip.Sites),
+                    ("kiwi_cartesian_map_out", hip.CartesianMapOut),
+                    ("kiwi_cartesian_map_stats", hip.CartesianMapStats)):
         assert int(out[st]) == ctypes.sizeof(cls), st
         for f, _ in cls._fields_:
             assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, (st, f)

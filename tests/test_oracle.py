@@ -343,6 +343,64 @@ def test_inverse_mapping_matches_reference_cli():
     assert rec0[0] < rec[0] and np.all(rec0 <= rec + 1e-15)
 
 
+def test_cartesian_map_matches_reference_cli():
+    """The oracle's cartesian-map resample (nearest site of every cell centre, src/cartesian_map.cpp:
+    150-183) against the reference CLI's cartesian_map.dat at resolution 20 for the setup of
+    voronoi_map.npz (tests/golden/cartesian_map.npz): every cell id, and the rows taken from the CLI's
+    own mapping.dat, bit for bit."""
+    g = np.load(os.path.join(GOLD, "voronoi_map.npz"))
+    gc = np.load(os.path.join(GOLD, "cartesian_map.npz"))
+    rows = np.stack([g["ref_density"], g["ref_vx"], g["ref_vy"], g["ref_vz"]], 1)
+    out, cid, unres, unm = O.cartesian_map(g["site"], g["cell_id"], rows, g["domain"], int(gc["res"]))
+    assert np.array_equal(cid, gc["cell_id"])
+    for k, f in enumerate(("density", "vx", "vy", "vz")):
+        assert np.array_equal(out[:, k], gc[f]), f
+    assert unres == 0 and unm == 0
+    assert len(np.unique(cid)) > 100  # the grid samples most cells
+    # an unmapped site: its cells keep the id and take a zero row (:175-178)
+    j = int(np.flatnonzero(g["cell_id"] == cid[0])[0])
+    mapped = np.ones(len(rows), bool)
+    mapped[j] = False
+    out2, cid2, _, unm2 = O.cartesian_map(g["site"], g["cell_id"], rows, g["domain"], int(gc["res"]), mapped)
+    hit = cid == cid[0]
+    assert np.array_equal(cid2, cid) and unm2 == int(hit.sum()) > 0
+    assert np.all(out2[hit] == 0) and np.array_equal(out2[~hit], out[~hit])
+    # no sites: every centre unresolved
+    assert O.cartesian_map(np.zeros((0, 3)), np.zeros(0, np.int32), np.zeros((0, 4)), g["domain"], 4)[2] == 64
+
+
+@pytest.mark.skipif(not os.path.exists(O.VORO_SO), reason="oracle/_ref not built (needs the reference sources)")
+@pytest.mark.parametrize("clustered", [False, True])
+def test_cartesian_map_nearest_site_is_voro_cell(clustered):
+    """The nearest site in voro++'s distance order is the cell voro++'s own find_voronoi_cell reports
+    (the reference's vendored voro++, oracle/_ref/libkiwi_voro.so) for every centre of random uniform and
+    clustered site sets in a non-cubic domain.  The sites are the tessellation's cells: the clustered set
+    has points clipped onto the max faces, of which voro++ drops those whose block index rounds past the
+    last block."""
+    rng = np.random.default_rng(7 + clustered)
+    dom = np.array([-1.0, 2.0, 0.0, 1.5, -0.5, 0.5])
+    lo, hi = dom[0::2], dom[1::2]
+    n, res = 700, 18
+    if clustered:
+        c = rng.uniform(lo + 0.2 * (hi - lo), hi - 0.2 * (hi - lo), size=(3, 3))
+        sites = np.clip(c[rng.integers(3, size=n)] + rng.normal(scale=0.08, size=(n, 3)) * (hi - lo), lo, hi)
+    else:
+        sites = rng.uniform(lo, hi, size=(n, 3))
+    ids = rng.permutation(np.arange(5, 5 + n)).astype(np.int32)
+    ax = [dom[2 * a] + (np.arange(res) + 0.5) * ((dom[2 * a + 1] - dom[2 * a]) / res) for a in range(3)]
+    q = np.stack([v.ravel() for v in np.meshgrid(*ax, indexing="ij")], 1)
+    geo = O.tessellate(ids, sites, dom, q)
+    assert np.all(geo["home"] >= 0)
+    nc = len(geo["cell_id"])
+    _, cid, _, _ = O.cartesian_map(geo["site"], geo["cell_id"], np.zeros((nc, 4)), dom, res)
+    assert np.array_equal(cid, geo["cell_id"][geo["home"]])
+    # sites outside the domain take no part, as in voro++'s pre_container
+    far = np.array([[-1.5, 0.7, 0.0], [0.2, 1.6, 0.1], [0.3, 0.4, -0.75]])
+    _, cid2, _, _ = O.cartesian_map(np.vstack([far, geo["site"]]), np.concatenate([[1, 2, 3], geo["cell_id"]]),
+                                    np.zeros((nc + 3, 4)), dom, res)
+    assert np.array_equal(cid2, cid)
+
+
 def test_voronoi_binary128_arbiter():
     """The binary128 cell sums used to arbitrate Voronoi cells (oracle.voronoi_map_cells_quad): the
     reference's analytic cubic algorithm in binary128 equals the series-free generic path for the same
