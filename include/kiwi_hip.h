@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KIWI_HIP_ABI_VERSION 7
+#define KIWI_HIP_ABI_VERSION 8
 
 #define KIWI_OK 0
 #define KIWI_ERR_INVALID_ARG (-1)
@@ -211,6 +211,43 @@ typedef struct kiwi_cartesian_map_stats {
  * tessellation's own cells as the sites where such points occur. */
 int kiwi_hip_cartesian_map(kiwi_hip_ctx* ctx, const kiwi_sites* sites, const double domain[6], int res,
                            const kiwi_cartesian_map_out* out, kiwi_cartesian_map_stats* stats);
+
+/* The coverage buffer's candidate lattice (generate_cartesian_lattice, src/point_set.cpp:166-195):
+ * spacing = spacing_fraction x the domain's smallest extent, n_a = max(1, round(extent_a / spacing)) per
+ * axis, candidate (i, j, k) at min + spacing (index + 0.5), k fastest.  A host computation, no context. */
+typedef struct kiwi_lattice {
+    int nx, ny, nz;
+    double spacing;
+} kiwi_lattice;
+
+int kiwi_hip_cartesian_lattice(const double domain[6], double spacing_fraction, kiwi_lattice* lattice);
+
+/* The buffer sites, in lattice order. */
+typedef struct kiwi_buffer_sites {
+    double *x, *y, *z;
+    int* id;          /* n_particles + 1, n_particles + 2, ... (src/point_set.cpp:262-270) */
+    int64_t capacity; /* entries each array holds (the lattice size always suffices) */
+    int on_device;
+} kiwi_buffer_sites;
+
+typedef struct kiwi_coverage_stats {
+    int64_t candidates;       /* lattice size */
+    int64_t kept;             /* uncovered candidates (written when they fit the capacity) */
+    double h_max;             /* the largest h; the reference's hash bin is factor x h_max */
+    int64_t particle_columns; /* work: lattice columns tested over all particles */
+    double device_ms;
+} kiwi_coverage_stats;
+
+/* The coverage-grid buffer, replacing add_coverage_grid_buffer (src/point_set.cpp:208-273): the
+ * lattice candidates that no particle covers (within coverage_radius_factor x its h, among the particles
+ * of the 27 hash bins around the candidate as the reference searches them), returned as new sites.
+ * Uses the particles' x, y, z and h.  Errors as the reference: spacing_fraction or the factor <= 0 (the
+ * configuration's checks, src/config.cpp:109-112) and no h > 0 ("all smoothing lengths are <= 0");
+ * beyond it, non-finite particle coordinates.  KIWI_ERR_LIMIT when more sites are kept than
+ * out->capacity holds (stats->kept says how many). */
+int kiwi_hip_coverage_buffer(kiwi_hip_ctx* ctx, const kiwi_particles* particles, const double domain[6],
+                             double spacing_fraction, double coverage_radius_factor, const kiwi_buffer_sites* out,
+                             kiwi_coverage_stats* stats);
 
 /* Launch shape of the sph-to-grid cell kernel (no reference counterpart).  Results never depend on it:
  * every (particle, cell block) item is computed and placed the same way whatever the shape. */

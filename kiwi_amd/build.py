@@ -27,7 +27,8 @@ def _run(cmd):
 
 def build_lib():
     srcs = [os.path.join(PKG_DIR, "csrc", f)
-            for f in ("kiwi_hip.hip", "kiwi_voronoi.hip", "kiwi_adjoint.hip", "kiwi_cartesian_map.hip")]
+            for f in ("kiwi_hip.hip", "kiwi_voronoi.hip", "kiwi_adjoint.hip", "kiwi_cartesian_map.hip",
+                      "kiwi_coverage.hip")]
     tmp = LIB_PATH + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT_DIR, "include"), *srcs, "-o", tmp])
     os.replace(tmp, LIB_PATH)

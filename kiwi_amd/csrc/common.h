@@ -42,6 +42,23 @@ __device__ __forceinline__ R gap(R lo, R hi) {
 }
 
 // a value that is the same in every lane, moved to a scalar register
+// Double operations rounded on their own, never fused into a multiply-add: the comparisons that
+// decide which site, cell or candidate a result uses follow the reference's separately rounded
+// arithmetic.  (The compiler contracts a + b * c into one fma by default, also when written with
+// __dadd_rn / __dmul_rn, which are plain + and * here.)
+__device__ __forceinline__ double mul_sep(double a, double b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ double add_sep(double a, double b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ double sub_sep(double a, double b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
+
 __device__ __forceinline__ uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint64_t wave_uniform64(uint64_t v) {
     return static_cast<uint64_t>(wave_uniform(static_cast<uint32_t>(v))) |
@@ -158,7 +175,7 @@ __device__ __forceinline__ int cell_floor(double v, int n) {
 // (src/cartesian_grid.cpp:216-238); false if it misses the grid.  Requires h > 0.
 __device__ __forceinline__ bool particle_bbox(const GridGeom& g, double support_in_h, double x0, double y0, double z0,
                                               double hh, int* q) {
-    const double rad = support_in_h * hh;
+    const double rad = mul_sep(support_in_h, hh);  // x0 - rad as the reference rounds it, not one fma
     q[0] = max(cell_floor((x0 - rad - g.x_min) / g.dx, g.nx), 0);
     q[1] = min(cell_floor((x0 + rad - g.x_min) / g.dx, g.nx), g.nx - 1);
     q[2] = max(cell_floor((y0 - rad - g.y_min) / g.dy, g.ny), 0);

@@ -92,13 +92,16 @@ __device__ __forceinline__ void cm_scan(int2 r, double qx, double qy, double qz,
     const int t0 = __builtin_amdgcn_readfirstlane(r.x), t1 = __builtin_amdgcn_readfirstlane(r.y);
     for (int t = t0; t < t1; ++t) {
         const CmSite st = sites[t];
-        const double dx = __dadd_rn(st.x, -qx), dy = __dadd_rn(st.y, -qy), dz = __dadd_rn(st.z, -qz);
-        const double rs = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+        const double dx = sub_sep(st.x, qx), dy = sub_sep(st.y, qy), dz = sub_sep(st.z, qz);
+        const double rs = add_sep(add_sep(mul_sep(dx, dx), mul_sep(dy, dy)), mul_sep(dz, dz));
         const bool take = rs < best || (rs == best && st.idx < bi);
         best = take ? rs : best;
         bi = take ? st.idx : bi;
     }
 }
+
+// min + (i + 0.5) d, the product and the sum rounded on their own as the reference's host code does
+__device__ __forceinline__ double cm_centre(double lo, int i, double d) { return add_sep(lo, mul_sep(i + 0.5, d)); }
 
 struct CmRows {
     const int* id;
@@ -129,14 +132,14 @@ __global__ __launch_bounds__(THREADS) void k_cm_query(int res, double x_min, dou
     const bool live = iy < res && iz < res;
     // cell centre as src/cartesian_map.cpp:157-164 forms it (lanes past the grid edge take the last cell)
     const int cy = min(iy, res - 1), cz = min(iz, res - 1);
-    const double qx = x_min + (ix + 0.5) * dpx, qy = y_min + (cy + 0.5) * dpy, qz = z_min + (cz + 0.5) * dpz;
+    const double qx = cm_centre(x_min, ix, dpx), qy = cm_centre(y_min, cy, dpy), qz = cm_centre(z_min, cz, dpz);
     const double qv[3] = {qx, qy, qz};
     // the wave's bucket box: the buckets of its lowest and highest centres (the bucket map is monotone)
     const int y1 = min(y0 + CM_BRICK_Y - 1, res - 1), z1 = min(z0 + CM_BRICK_Z - 1, res - 1);
-    const int blo[3] = {cm_bucket_axis(g, 0, qx), cm_bucket_axis(g, 1, y_min + (y0 + 0.5) * dpy),
-                        cm_bucket_axis(g, 2, z_min + (z0 + 0.5) * dpz)};
-    const int bhi[3] = {blo[0], cm_bucket_axis(g, 1, y_min + (y1 + 0.5) * dpy),
-                        cm_bucket_axis(g, 2, z_min + (z1 + 0.5) * dpz)};
+    const int blo[3] = {cm_bucket_axis(g, 0, qx), cm_bucket_axis(g, 1, cm_centre(y_min, y0, dpy)),
+                        cm_bucket_axis(g, 2, cm_centre(z_min, z0, dpz))};
+    const int bhi[3] = {blo[0], cm_bucket_axis(g, 1, cm_centre(y_min, y1, dpy)),
+                        cm_bucket_axis(g, 2, cm_centre(z_min, z1, dpz))};
     double best = INFINITY;
     int bi = 0x7fffffff;
     for (int r = 0;; ++r) {

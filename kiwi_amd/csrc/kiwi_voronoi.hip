@@ -278,8 +278,8 @@ __global__ __launch_bounds__(THREADS) void k_vor_candidates(
                             const double g0 = gap(bx[0] - p0, bx[3] - p0);
                             const double g1 = gap(bx[1] - p1, bx[4] - p1);
                             const double g2 = gap(bx[2] - p2, bx[5] - p2);
-                            const double d2 = __dadd_rn(__dadd_rn(__dmul_rn(g0, g0), __dmul_rn(g1, g1)),
-                                                        __dmul_rn(g2, g2));
+                            const double d2 = add_sep(add_sep(mul_sep(g0, g0), mul_sep(g1, g1)),
+                                                        mul_sep(g2, g2));
                             is_home = (c == hc);
                             acc = (d2 < lim2) || is_home;
                         }

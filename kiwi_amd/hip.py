@@ -33,7 +33,8 @@ _pd = ctypes.POINTER(ctypes.c_double)
 EXPORTED = ["kiwi_hip_abi_version", "kiwi_hip_create", "kiwi_hip_destroy", "kiwi_hip_last_error",
             "kiwi_hip_stream", "kiwi_hip_sph_to_grid", "kiwi_hip_voronoi_map", "kiwi_hip_sph_to_grid_adjoint",
             "kiwi_hip_sph_to_grid_batch", "kiwi_hip_set_tuning", "kiwi_hip_get_tuning", "kiwi_hip_autotune",
-            "kiwi_hip_inverse_mapping", "kiwi_hip_cartesian_map"]
+            "kiwi_hip_inverse_mapping", "kiwi_hip_cartesian_map", "kiwi_hip_cartesian_lattice",
+            "kiwi_hip_coverage_buffer"]
 
 
 class KiwiError(RuntimeError):
@@ -105,6 +106,18 @@ class CartesianMapOut(ctypes.Structure):
 
 class CartesianMapStats(ctypes.Structure):
     _fields_ = [("unresolved", _i64), ("unmapped", _i64), ("device_ms_prep", _d), ("device_ms_query", _d)]
+
+
+class Lattice(ctypes.Structure):
+    _fields_ = [("nx", _i), ("ny", _i), ("nz", _i), ("spacing", _d)]
+
+
+class BufferSites(ctypes.Structure):
+    _fields_ = [("x", _pd), ("y", _pd), ("z", _pd), ("id", _pi), ("capacity", _i64), ("on_device", _i)]
+
+
+class CoverageStats(ctypes.Structure):
+    _fields_ = [("candidates", _i64), ("kept", _i64), ("h_max", _d), ("particle_columns", _i64), ("device_ms", _d)]
 
 
 class Tuning(ctypes.Structure):
@@ -182,6 +195,9 @@ def load():
         L.kiwi_hip_cartesian_map.restype = _i
         L.kiwi_hip_cartesian_map.argtypes = [ctypes.c_void_p, ctypes.POINTER(Sites), ctypes.POINTER(_d), _i,
                                              ctypes.POINTER(CartesianMapOut), ctypes.POINTER(CartesianMapStats)]
+        L.kiwi_hip_cartesian_lattice.argtypes = [ctypes.POINTER(_d), _d, ctypes.POINTER(Lattice)]
+        L.kiwi_hip_coverage_buffer.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.POINTER(_d), _d, _d,
+                                               ctypes.POINTER(BufferSites), ctypes.POINTER(CoverageStats)]
         L.kiwi_hip_set_tuning.restype = _i
         L.kiwi_hip_set_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(Tuning)]
         L.kiwi_hip_get_tuning.restype = _i
@@ -195,6 +211,15 @@ def load():
                                            ctypes.POINTER(CellSums), ctypes.POINTER(MapStats)]
         _lib = L
     return _lib
+
+
+def cartesian_lattice(domain, spacing_fraction):
+    """The coverage buffer's candidate lattice: (nx, ny, nz, spacing).  Raises ValueError on bad input."""
+    lat = Lattice()
+    dom = (_d * 6)(*[float(v) for v in domain])
+    if load().kiwi_hip_cartesian_lattice(dom, float(spacing_fraction), ctypes.byref(lat)) != 0:
+        raise ValueError("invalid domain or spacing_fraction")
+    return lat.nx, lat.ny, lat.nz, lat.spacing
 
 
 def kernel_spec(name="M4", support_in_h=0.0, generic_path=False, tabulated=False):
@@ -434,6 +459,38 @@ class Context:
         self._check(self.L.kiwi_hip_cartesian_map(self.h, ctypes.byref(S), dom, int(res), ctypes.byref(O),
                                                   ctypes.byref(st)))
         return arrs[0], arrs[1], arrs[2], arrs[3], arrs[4], st
+
+    def coverage_buffer(self, parts, domain, spacing_fraction=0.05, coverage_radius_factor=2.0, out=None):
+        """The coverage-grid buffer.  parts: particle columns (x, y, z and h are read), host arrays or
+        device tensors.  out: None (host results sized to the lattice) or (x, y, z, id) device tensors of
+        a common length (the capacity).  Returns (ids, xyz (kept, 3) or the out tensors, CoverageStats)."""
+        cols = [_ptr(parts[k]) for k in ("x", "y", "z", "h")]
+        if len({q[1] for q in cols}) != 1 or len({len(parts[k]) for k in ("x", "y", "z", "h")}) != 1:
+            raise ValueError("x, y, z, h must be host arrays or device tensors of one length")
+        P = Particles(len(parts["x"]), cols[0][0], cols[1][0], cols[2][0], None, None, None, None, cols[3][0],
+                      cols[0][1])
+        dom = (_d * 6)(*[float(v) for v in domain])
+        st = CoverageStats()
+        if out is None:
+            try:
+                nx, ny, nz, _ = cartesian_lattice(domain, spacing_fraction)
+                cap = nx * ny * nz
+            except ValueError:
+                cap = 0  # the library reports the error
+            arrs = [np.zeros(max(cap, 1)) for _ in range(3)] + [np.zeros(max(cap, 1), dtype=np.int32)]
+            B = BufferSites(*[a.ctypes.data_as(_pd) for a in arrs[:3]], arrs[3].ctypes.data_as(_pi), cap, 0)
+        else:
+            arrs = list(out)
+            op = [_ptr(a) for a in arrs[:3]] + [_ptr(arrs[3], _pi, "int32")]
+            if any(q[1] != 1 for q in op) or len({len(a) for a in arrs}) != 1:
+                raise ValueError("out must be device tensors of one length")
+            B = BufferSites(*[q[0] for q in op], len(arrs[0]), 1)
+        self._check(self.L.kiwi_hip_coverage_buffer(self.h, ctypes.byref(P), dom, float(spacing_fraction),
+                                                    float(coverage_radius_factor), ctypes.byref(B), ctypes.byref(st)))
+        if out is not None:
+            return arrs[3], tuple(arrs[:3]), st
+        k = st.kept
+        return arrs[3][:k].copy(), np.stack([a[:k] for a in arrs[:3]], 1), st
 
     def _map_inputs(self, parts, home, cells):
         P = _particles(parts)

@@ -13,6 +13,7 @@ ORACLE_SO = os.path.join(HERE, "libkiwi_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libkiwi_ref.so")
 REF_CLI = os.path.join(HERE, "_ref", "kiwi")
 VORO_SO = os.path.join(HERE, "_ref", "libkiwi_voro.so")
+PTS_SO = os.path.join(HERE, "_ref", "libkiwi_pts.so")
 
 KERNELS = ["M4", "M5", "M6", "WC2", "WC4", "WC6"]
 
@@ -205,6 +206,88 @@ def voronoi_map(parts, pid, home, cell_face_ptr, face_vert_ptr, face_xyz, face_n
     if rc != 0:
         raise ValueError(f"oracle_voronoi_map failed ({rc})")
     return out[0], out[1], out[2], out[3], counts
+
+
+def _round_half_away(v):
+    """std::round: to the nearest integer, halves away from zero (v - trunc(v) is exact)."""
+    t = float(np.trunc(v))
+    return int(t + np.sign(v)) if abs(v - t) >= 0.5 else int(t)
+
+
+def cartesian_lattice(domain, spacing_fraction):
+    """The buffer's candidate lattice (generate_cartesian_lattice, src/point_set.cpp:166-195): spacing =
+    fraction x the smallest extent; n_a = max(1, round(extent_a / spacing)) (round half away from zero);
+    candidate (i, j, k) at min + spacing (index + 0.5), k fastest.  Returns (spacing, (nx, ny, nz), xyz)."""
+    d = np.asarray(domain, dtype=np.float64)
+    ext = d[1::2] - d[0::2]
+    sp = spacing_fraction * min(ext[0], ext[1], ext[2])
+    n = [max(1, _round_half_away(e / sp)) for e in ext]
+    ax = [d[2 * a] + sp * (np.arange(n[a]) + 0.5) for a in range(3)]
+    X, Y, Z = np.meshgrid(*ax, indexing="ij")
+    return sp, tuple(n), np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+
+
+def _bin_keys(v, bin_size):
+    """Per-axis hash key of the reference's coverage hash: floor(v / bin) as int, biased by 2^19 and
+    clamped to [0, 2^20) (bin_of and bin_key, src/point_set.cpp:29-35, 217-222)."""
+    b = np.floor(np.asarray(v) / bin_size)
+    b = np.clip(b, -2.0 ** 30, 2.0 ** 30).astype(np.int64)
+    return np.clip(b + (1 << 19), 0, (1 << 20) - 1)
+
+
+def coverage_buffer(x, y, z, h, domain, spacing_fraction=0.05, factor=2.0, chunk=2048):
+    """The coverage-grid buffer (add_coverage_grid_buffer, src/point_set.cpp:208-273) restated with numpy:
+    a lattice candidate is covered when some particle whose hash bin is one of the 27 around the
+    candidate's (the reference's search set; bin = factor x h_max, h_max = the largest h, NaN ignored)
+    lies within r = factor h of it: (dx^2 + dy^2) + dz^2 <= r * r, dx = particle - candidate, all in
+    double without fused multiply-add.  The uncovered candidates are kept in lattice order with ids
+    n + 1, n + 2, ...  Returns (ids, xyz); raises ValueError as the reference throws."""
+    x, y, z, h = (np.asarray(a, dtype=np.float64) for a in (x, y, z, h))
+    hm = float(np.fmax.reduce(np.concatenate([[0.0], h])))  # std::max from 0: a NaN never takes over
+    if not hm > 0.0:
+        raise ValueError("add_coverage_grid_buffer: all smoothing lengths are <= 0")
+    bin_size = factor * hm
+    sp, n, cand = cartesian_lattice(domain, spacing_fraction)
+    pk = [_bin_keys(a, bin_size) for a in (x, y, z)]
+    r = factor * h
+    r2 = r * r
+    covered = np.zeros(len(cand), bool)
+    for c0 in range(0, len(cand), chunk):
+        c = cand[c0:c0 + chunk]
+        near = np.ones((len(c), len(x)), bool)
+        for a in range(3):
+            ck = _bin_keys(c[:, a], bin_size)
+            lo_k = _bin_keys(np.floor(c[:, a] / bin_size) - 1.0, 1.0)  # key of (bin - 1), (bin + 1)
+            hi_k = _bin_keys(np.floor(c[:, a] / bin_size) + 1.0, 1.0)
+            near &= (pk[a][None, :] == ck[:, None]) | (pk[a][None, :] == lo_k[:, None]) | (pk[a][None, :] == hi_k[:, None])
+        dx = x[None, :] - c[:, 0:1]
+        dy = y[None, :] - c[:, 1:2]
+        dz = z[None, :] - c[:, 2:3]
+        covered[c0:c0 + chunk] = np.any(near & ((dx * dx + dy * dy) + dz * dz <= r2[None, :]), axis=1)
+    keep = np.flatnonzero(~covered)
+    return (len(x) + 1 + np.arange(len(keep))).astype(np.int32), cand[keep]
+
+
+def ref_coverage_buffer(x, y, z, h, domain, spacing_fraction=0.05, factor=2.0):
+    """The reference's own add_coverage_grid_buffer through oracle/_ref/libkiwi_pts.so (particle ids
+    1..n).  Returns (ids, xyz); raises ValueError with the reference's message."""
+    L = ctypes.CDLL(PTS_SO)
+    L.ref_cartesian_lattice.argtypes = [_pd, _d, _pi, _pd]
+    L.ref_coverage_buffer.restype = _ll
+    L.ref_coverage_buffer.argtypes = [_ll, _pd, _pd, _pd, _pd, _pd, _d, _d, _pd, _pd, _pd, _pi, _ll, ctypes.c_char_p, _i]
+    cols = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, h)]
+    dom = np.ascontiguousarray(domain, dtype=np.float64)
+    n3 = np.zeros(3, np.int32)
+    sp = ctypes.c_double()
+    cap = L.ref_cartesian_lattice(_p(dom), float(spacing_fraction), n3.ctypes.data_as(_pi), ctypes.byref(sp))
+    o = [np.zeros(max(cap, 1)) for _ in range(3)]
+    oid = np.zeros(max(cap, 1), np.int32)
+    err = ctypes.create_string_buffer(512)
+    k = L.ref_coverage_buffer(len(cols[0]), *[_p(a) for a in cols], _p(dom), float(spacing_fraction), float(factor),
+                              *[_p(a) for a in o], oid.ctypes.data_as(_pi), cap, err, 512)
+    if k < 0:
+        raise ValueError(err.value.decode())
+    return oid[:k].copy(), np.stack([a[:k] for a in o], 1)
 
 
 def nearest_site(sites, q, chunk=4096):

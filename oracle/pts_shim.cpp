@@ -1,0 +1,58 @@
+// C-ABI shim over the reference's coverage-grid buffer (add_coverage_grid_buffer, src/point_set.cpp:
+// 208-273) and its candidate lattice (generate_cartesian_lattice, :166-195), linked with the reference's
+// own objects into oracle/_ref/libkiwi_pts.so.  Test infrastructure: it runs the reference function on
+// a plain-array point set so the oracle's restatement and the GPU buffer can be checked against it and
+// timed beside it; nothing here is shipped or run on a GPU.
+#include <algorithm>
+#include <cstring>
+#include <exception>
+#include <vector>
+
+#include "kiwi/config.hpp"
+#include "kiwi/point_set.hpp"
+
+extern "C" {
+
+// Lattice size of the buffer's candidates: nx, ny, nz and the spacing.
+int ref_cartesian_lattice(const double* dom, double spacing_fraction, int* n3, double* spacing) {
+    kiwi::Domain d;
+    d.x_min = dom[0], d.x_max = dom[1], d.y_min = dom[2], d.y_max = dom[3], d.z_min = dom[4], d.z_max = dom[5];
+    const kiwi::CartesianLattice lat = kiwi::generate_cartesian_lattice(d, spacing_fraction);
+    n3[0] = lat.nx, n3[1] = lat.ny, n3[2] = lat.nz;
+    *spacing = lat.spacing;
+    return static_cast<int>(lat.points.size());
+}
+
+// The buffer sites the reference appends to n particles (ids 1..n): up to cap of them into ox, oy, oz,
+// oid; returns their count, or -1 with the exception's message in err.
+long long ref_coverage_buffer(long long n, const double* x, const double* y, const double* z, const double* h,
+                              const double* dom, double spacing_fraction, double factor, double* ox, double* oy,
+                              double* oz, int* oid, long long cap, char* err, int errlen) {
+    try {
+        kiwi::PointSet p;
+        for (long long i = 0; i < n; ++i) p.append(static_cast<int>(i + 1), x[i], y[i], z[i]);
+        const std::vector<double> hv(h, h + n);
+        kiwi::Domain d;
+        d.x_min = dom[0], d.x_max = dom[1], d.y_min = dom[2], d.y_max = dom[3], d.z_min = dom[4], d.z_max = dom[5];
+        kiwi::CoverageGridBufferParams prm;
+        prm.spacing_fraction = spacing_fraction;
+        prm.coverage_radius_factor = factor;
+        const kiwi::PointSet out = kiwi::add_coverage_grid_buffer(p, hv, d, prm);
+        const long long k = static_cast<long long>(out.size()) - n;
+        for (long long j = 0; j < std::min(k, cap); ++j) {
+            ox[j] = out.x[n + j];
+            oy[j] = out.y[n + j];
+            oz[j] = out.z[n + j];
+            oid[j] = out.id[n + j];
+        }
+        return k;
+    } catch (const std::exception& e) {
+        if (errlen > 0) {
+            std::strncpy(err, e.what(), errlen - 1);
+            err[errlen - 1] = 0;
+        }
+        return -1;
+    }
+}
+
+}  // extern "C"

@@ -5,6 +5,8 @@
 #   make -f oracle/ref.mk            -> oracle/_ref/kiwi (the reference CLI)
 #                                       oracle/_ref/libkiwi_ref.so (reference kernel math + our C shim)
 #                                       oracle/_ref/libkiwi_voro.so (the vendored voro++ + our C shim)
+#                                       oracle/_ref/libkiwi_pts.so (the reference's library objects +
+#                                       our C shim over its coverage-grid buffer)
 #
 # The reference build stays on the build machine: no GPU test, smoke test or benchmark loads it.
 
@@ -25,7 +27,7 @@ VORO_OBJ := $(patsubst %.cc,$(OUT)/obj/voro_%.o,$(VORO_SRC))
 KIWI_OBJ := $(patsubst $(REF)/src/%.cpp,$(OUT)/obj/kiwi_%.o,$(KIWI_SRC))
 MATH_OBJ := $(patsubst $(REF)/src/%.cpp,$(OUT)/obj/kiwi_%.o,$(MATH_SRC))
 
-all: $(OUT)/kiwi $(OUT)/libkiwi_ref.so $(OUT)/libkiwi_voro.so
+all: $(OUT)/kiwi $(OUT)/libkiwi_ref.so $(OUT)/libkiwi_voro.so $(OUT)/libkiwi_pts.so
 
 $(OUT)/obj/voro_%.o: $(REF)/third_party/voro/src/%.cc
 	@mkdir -p $(OUT)/obj
@@ -43,6 +45,9 @@ $(OUT)/libkiwi_ref.so: oracle/ref_shim.cpp $(MATH_OBJ)
 
 $(OUT)/libkiwi_voro.so: oracle/voro_shim.cpp $(VORO_OBJ)
 	$(CXX) $(CXXFLAGS) $(INC) -shared $^ -o $@
+
+$(OUT)/libkiwi_pts.so: oracle/pts_shim.cpp $(filter-out $(OUT)/obj/kiwi_main.o,$(KIWI_OBJ)) $(VORO_OBJ)
+	$(CXX) $(CXXFLAGS) $(INC) $(DEFS) -shared $^ -o $@
 
 clean:
 	rm -rf $(OUT)

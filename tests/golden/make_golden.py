@@ -26,7 +26,8 @@ fixtures it writes are plain numbers (inputs and the reference's outputs), commi
   cartesian_map.npz     `kiwi cartesian-map` (reference CLI, resolution 20) chained from the same `kiwi map` run:
                         the density, velocity and Voronoi cell id fields of cartesian_map.dat
 
-  python tests/golden/make_golden.py [--only kernel_grids | --only voronoi | --only inverse | --only cmap]
+  python tests/golden/make_golden.py [--only kernel_grids | --only voronoi | --only inverse | --only cmap
+                                    | --only coverage]
 """
 import ctypes
 import json
@@ -102,7 +103,7 @@ def kernel_grids():
     np.savez_compressed(os.path.join(OUT, "kernel_grids.npz"), **out)
 
 
-def run_ref_map(parts, domain, threads=1, inverse=False, cmap_res=0):
+def run_ref_map(parts, domain, threads=1, inverse=False, cmap_res=0, buffer=None):
     with tempfile.TemporaryDirectory() as td:
         snap = os.path.join(td, "snap.txt")
         workloads.write_plain_ascii(parts, snap)
@@ -110,6 +111,8 @@ def run_ref_map(parts, domain, threads=1, inverse=False, cmap_res=0):
                "work_folder": os.path.join(td, "work"),
                "domain": dict(zip(["x_min", "x_max", "y_min", "y_max", "z_min", "z_max"], map(float, domain))),
                "buffer": {"enabled": False}, "lloyd": {"iterations": 0}}
+        if buffer:
+            cfg["buffer"] = {"enabled": True, "strategy": "coverage_grid", "coverage_grid": buffer}
         if cmap_res:
             cfg["cartesian_map"] = {"resolution": int(cmap_res)}
         cp = os.path.join(td, "cfg.json")
@@ -179,12 +182,44 @@ def cmap():
                         vy=fields[2], vz=fields[3], cell_id=ids.astype(np.int32))
 
 
+def coverage():
+    """The reference's coverage-grid buffer (add_coverage_grid_buffer, src/point_set.cpp:208-273): the
+    buffer sites of mapping.dat (ids past the particle count, with their positions) for two setups with
+    a mix of covered and uncovered candidates: the voronoi() particles (spacing 0.06, radius factor 1.2)
+    and 600 particles of a small log-normal field, h spread over a decade (spacing 0.04, factor 0.6)."""
+    c1 = np.load(os.path.join(OUT, "c1_snapshot.npz"))
+    cases = {}
+    parts = {k: np.ascontiguousarray(c1[k][::8]) for k in ("x", "y", "z", "vx", "vy", "vz", "m", "h", "rho")}
+    cases["c1"] = (parts, np.array(c1["domain"]), {"spacing_fraction": 0.06, "coverage_radius_factor": 1.2})
+    lp, ldom, _ = workloads.lognormal_field(n_particles=600, n=32, seed=4, h_min=0.3, h_max=6.0)
+    lp = {k: np.ascontiguousarray(lp[k], dtype=np.float64) for k in ("x", "y", "z", "vx", "vy", "vz", "m", "h")}
+    lp["rho"] = np.ones_like(lp["m"])
+    cases["lognormal"] = (lp, np.asarray(ldom, dtype=np.float64), {"spacing_fraction": 0.04,
+                                                                   "coverage_radius_factor": 0.6})
+    out = {}
+    for name, (pp, dom, prm) in cases.items():
+        with tempfile.TemporaryDirectory() as td:  # the particles as the reference parsed them
+            snap = os.path.join(td, "snap.txt")
+            workloads.write_plain_ascii(pp, snap)
+            back = np.loadtxt(snap, ndmin=2)
+        rows, _ = run_ref_map(pp, dom, buffer=prm)
+        n = len(pp["x"])
+        buf = rows[rows[:, 0] > n]
+        buf = buf[np.argsort(buf[:, 0])]
+        print(f"coverage {name}: {n} particles, {len(buf)} buffer sites", file=sys.stderr)
+        out.update({f"{name}_domain": dom, f"{name}_spacing_fraction": prm["spacing_fraction"],
+                    f"{name}_factor": prm["coverage_radius_factor"], f"{name}_snapshot": back,
+                    f"{name}_buffer_id": buf[:, 0].astype(np.int32), f"{name}_buffer_xyz": buf[:, 1:4]})
+    np.savez_compressed(os.path.join(OUT, "coverage_buffer.npz"), **out)
+
+
 def main():
     if not O.have_ref() or not os.path.exists(O.REF_CLI):
         sys.exit("oracle/_ref is not built (make -f oracle/ref.mk)")
     if "--only" in sys.argv:
         what = sys.argv[sys.argv.index("--only") + 1]
-        {"kernel_grids": kernel_grids, "voronoi": voronoi, "inverse": inverse, "cmap": cmap}[what]()
+        {"kernel_grids": kernel_grids, "voronoi": voronoi, "inverse": inverse, "cmap": cmap,
+         "coverage": coverage}[what]()
         return
     R = O.ref()
     rng = np.random.default_rng(20251203)
@@ -275,6 +310,7 @@ def main():
     voronoi()
     inverse()
     cmap()
+    coverage()
     print("golden fixtures written to", OUT)
 
 

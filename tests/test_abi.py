@@ -61,7 +61,10 @@ def test_struct_layouts_match_header(tmp_path):
               "kiwi_inverse_rows": [f for f, _ in hip.InverseRows._fields_],
               "kiwi_sites": [f for f, _ in hip.Sites._fields_],
               "kiwi_cartesian_map_out": [f for f, _ in hip.CartesianMapOut._fields_],
-              "kiwi_cartesian_map_stats": [f for f, _ in hip.CartesianMapStats._fields_]}
+              "kiwi_cartesian_map_stats": [f for f, _ in hip.CartesianMapStats._fields_],
+              "kiwi_lattice": [f for f, _ in hip.Lattice._fields_],
+              "kiwi_buffer_sites": [f for f, _ in hip.BufferSites._fields_],
+              "kiwi_coverage_stats": [f for f, _ in hip.CoverageStats._fields_]}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for st, fs in fields.items():
         lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
@@ -80,7 +83,8 @@ def test_struct_layouts_match_header(tmp_path):
                     ("kiwi_batch_problem", hip.BatchProblem), ("kiwi_tuning", hip.Tuning),
                     ("kiwi_inverse_rows", hip.InverseRows), ("kiwi_sites", hip.Sites),
                     ("kiwi_cartesian_map_out", hip.CartesianMapOut),
-                    ("kiwi_cartesian_map_stats", hip.CartesianMapStats)):
+                    ("kiwi_cartesian_map_stats", hip.CartesianMapStats), ("kiwi_lattice", hip.Lattice),
+                    ("kiwi_buffer_sites", hip.BufferSites), ("kiwi_coverage_stats", hip.CoverageStats)):
         assert int(out[st]) == ctypes.sizeof(cls), st
         for f, _ in cls._fields_:
             assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, (st, f)
@@ -96,3 +100,20 @@ def test_context_creation_fails_cleanly_without_gpu():
         pass
     with pytest.raises(hip.KiwiError):
         hip.Context(0)
+
+
+def test_cartesian_lattice_matches_reference_sizes():
+    """kiwi_hip_cartesian_lattice is a host computation (no GPU): the coverage buffer's lattice sizes and
+    spacing (generate_cartesian_lattice, src/point_set.cpp:167-181) for domains where round() sits near
+    a half, against the oracle's restatement, and its input checks."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    for dom, fr in (([0, 1, 0, 1, 0, 1], 0.05), ([-1.7, 1.7, -1.7, 1.7, -1.7, 1.7], 0.06), ([0, 2.5, 0, 1, 0, 3.5], 0.2),
+                    ([0, 1.05, 0, 1, 0, 0.95], 0.1), ([0, 10, 0, 1, 0, 1], 1.0)):
+        nx, ny, nz, sp = hip.cartesian_lattice(dom, fr)
+        osp, on, _ = O.cartesian_lattice(dom, fr)
+        assert (nx, ny, nz) == on and sp == osp, (dom, fr)
+    with pytest.raises(ValueError):
+        hip.cartesian_lattice([0, 1, 0, 1, 0, 1], 0.0)
+    with pytest.raises(ValueError):
+        hip.cartesian_lattice([0, 1, 1, 1, 0, 1], 0.05)

@@ -1,0 +1,137 @@
+"""GPU checks of kiwi_hip_coverage_buffer, the coverage-grid buffer (add_coverage_grid_buffer,
+src/point_set.cpp:208-273): against the reference CLI's buffer sites (tests/golden/coverage_buffer.npz),
+against the oracle's restatement on edge-case particle sets, and at the benchmark's size through
+properties (every kept candidate uncovered, every dropped one covered).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from kiwi_amd import hip  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def cols(p, h):
+    return {"x": np.ascontiguousarray(p[:, 0]), "y": np.ascontiguousarray(p[:, 1]),
+            "z": np.ascontiguousarray(p[:, 2]), "h": np.ascontiguousarray(h, dtype=np.float64)}
+
+
+def test_matches_reference_cli(hip_ctx):
+    g = np.load(os.path.join(GOLD, "coverage_buffer.npz"))
+    for name in ("c1", "lognormal"):
+        sn, dom = g[f"{name}_snapshot"], g[f"{name}_domain"]
+        fr, fa = float(g[f"{name}_spacing_fraction"]), float(g[f"{name}_factor"])
+        say(f"case: {name}, {len(sn)} particles, spacing {fr}, factor {fa}, vs the reference CLI")
+        ids, xyz, st = hip_ctx.coverage_buffer(cols(sn[:, :3], sn[:, 7]), dom, fr, fa)
+        assert np.array_equal(ids, g[f"{name}_buffer_id"])
+        np.testing.assert_allclose(xyz, g[f"{name}_buffer_xyz"], rtol=0, atol=1e-14 * np.abs(dom).max())
+        oid, oxyz = O.coverage_buffer(sn[:, 0], sn[:, 1], sn[:, 2], sn[:, 7], dom, fr, fa)
+        assert np.array_equal(ids, oid) and np.array_equal(xyz, oxyz)
+        assert st.kept == len(ids) and st.h_max == np.nanmax(sn[:, 7])
+
+
+def test_edge_cases_match_oracle(hip_ctx):
+    import test_oracle as T
+    rng = np.random.default_rng(17)
+    for k, (p, h, dom, fr, fa) in enumerate(T.coverage_cases(rng)):
+        say(f"case: edge set {k}: {len(p)} particles, spacing {fr}, factor {fa}")
+        ids, xyz, st = hip_ctx.coverage_buffer(cols(p, h), dom, fr, fa)
+        oid, oxyz = O.coverage_buffer(p[:, 0], p[:, 1], p[:, 2], h, dom, fr, fa)
+        assert np.array_equal(ids, oid) and np.array_equal(xyz, oxyz)
+        if os.path.exists(O.PTS_SO):
+            rid, rxyz = O.ref_coverage_buffer(p[:, 0], p[:, 1], p[:, 2], h, dom, fr, fa)
+            assert np.array_equal(ids, rid) and np.array_equal(xyz, rxyz)
+
+
+def test_infinite_h_and_tiny_sets(hip_ctx):
+    say("case: an infinite h, a single particle, h = 0 on a lattice point")
+    dom = [0.0, 1.0, 0.0, 2.0, 0.0, 1.0]
+    p = np.array([[0.2, 0.3, 0.4], [0.7, 1.5, 0.1]])
+    for h in ([np.inf, 0.1], [0.05, 0.0]):
+        ids, xyz, _ = hip_ctx.coverage_buffer(cols(p, np.array(h)), dom, 0.1, 2.0)
+        oid, oxyz = O.coverage_buffer(p[:, 0], p[:, 1], p[:, 2], np.array(h), dom, 0.1, 2.0)
+        assert np.array_equal(ids, oid) and np.array_equal(xyz, oxyz)
+    # h = 0 exactly on a candidate covers that candidate only
+    _, _, cand = O.cartesian_lattice(dom, 0.1)
+    q = cand[[7]]
+    ids, xyz, st = hip_ctx.coverage_buffer(cols(np.vstack([q, [[0.5, 0.5, 0.5]]]), np.array([0.0, 1e-9])), dom, 0.1, 2.0)
+    assert st.kept == len(cand) - 1 and not np.any(np.all(xyz == q, axis=1))
+
+
+def test_large_set_properties_and_device_io(hip_ctx):
+    """2e5 particles of the log-normal workload at spacing 0.01 (a 100^3 lattice): the oracle's
+    predicate holds for a sample of kept and of dropped candidates; device outputs equal host ones."""
+    import torch
+    from kiwi_amd import workloads
+    say("case: 2e5 log-normal particles, spacing 0.01, factor 1.0, sampled predicate + device io")
+    parts, dom, _ = workloads.lognormal_field(n_particles=200_000, n=256, seed=2)
+    p = np.stack([parts["x"], parts["y"], parts["z"]], 1).astype(np.float64)
+    h = np.asarray(parts["h"], dtype=np.float64)
+    ids, xyz, st = hip_ctx.coverage_buffer(cols(p, h), dom, 0.01, 1.0)
+    _, _, cand = O.cartesian_lattice(dom, 0.01)
+    assert st.candidates == len(cand) == 10 ** 6 and 0 < st.kept < st.candidates
+    rng = np.random.default_rng(3)
+    keep_mask = np.zeros(len(cand), bool)
+    sp, n3, _ = O.cartesian_lattice(dom, 0.01)
+    ijk = np.rint((xyz - np.asarray(dom)[0::2]) / sp - 0.5).astype(np.int64)
+    keep_mask[(ijk[:, 0] * n3[1] + ijk[:, 1]) * n3[2] + ijk[:, 2]] = True
+    assert np.array_equal(cand[keep_mask], xyz) and np.array_equal(ids, len(p) + 1 + np.arange(st.kept))
+    sub = np.concatenate([rng.choice(np.flatnonzero(keep_mask), 150), rng.choice(np.flatnonzero(~keep_mask), 150)])
+    bin_size = 1.0 * np.nanmax(h)
+    keys = [O._bin_keys(p[:, a], bin_size) for a in range(3)]
+    for c in sub:
+        q = cand[c]
+        near = np.ones(len(p), bool)
+        for a in range(3):
+            b = np.floor(q[a] / bin_size)
+            near &= np.isin(keys[a], O._bin_keys(np.array([b - 1, b, b + 1]), 1.0))
+        d = p - q
+        r = 1.0 * h
+        covered = np.any(near & ((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2] <= r * r))
+        assert covered != keep_mask[c], c
+    dev = torch.device("cuda:0")
+    dcols = {k: torch.from_numpy(v).to(dev) for k, v in cols(p, h).items()}
+    cap = st.kept + 5
+    out = tuple(torch.zeros(cap, dtype=torch.float64, device=dev) for _ in range(3)) + (
+        torch.zeros(cap, dtype=torch.int32, device=dev),)
+    did, dxyz, dst = hip_ctx.coverage_buffer(dcols, dom, 0.01, 1.0, out=out)
+    torch.cuda.synchronize()
+    assert dst.kept == st.kept and np.array_equal(did.cpu().numpy()[: st.kept], ids)
+    assert np.array_equal(torch.stack(dxyz, 1).cpu().numpy()[: st.kept], xyz)
+    say(f"  kept {st.kept} of {st.candidates}, columns {st.particle_columns}, {st.device_ms:.3f} ms")
+    small = tuple(o[:10] for o in out)
+    with pytest.raises(hip.KiwiError, match="capacity"):
+        hip_ctx.coverage_buffer(dcols, dom, 0.01, 1.0, out=small)
+
+
+def test_errors(hip_ctx):
+    say("case: bad arguments")
+    p = np.random.default_rng(1).uniform(0, 1, size=(10, 3))
+    dom = [0.0, 1.0, 0.0, 1.0, 0.0, 1.0]
+    with pytest.raises(hip.KiwiError, match="all smoothing lengths are <= 0"):
+        hip_ctx.coverage_buffer(cols(p, -np.ones(10)), dom)
+    with pytest.raises(hip.KiwiError, match="all smoothing lengths are <= 0"):
+        hip_ctx.coverage_buffer(cols(p[:0], np.ones(0)), dom)
+    with pytest.raises(hip.KiwiError, match="spacing_fraction must be > 0"):
+        hip_ctx.coverage_buffer(cols(p, np.ones(10)), dom, 0.0)
+    with pytest.raises(hip.KiwiError, match="coverage_radius_factor must be > 0"):
+        hip_ctx.coverage_buffer(cols(p, np.ones(10)), dom, 0.05, -1.0)
+    q = p.copy()
+    q[3, 1] = np.nan
+    with pytest.raises(hip.KiwiError, match="non-finite"):
+        hip_ctx.coverage_buffer(cols(q, np.ones(10)), dom)
+    ids, xyz, st = hip_ctx.coverage_buffer(cols(p, np.full(10, 1e-6)), dom, 0.25)
+    assert st.kept == 64 and np.array_equal(ids, 11 + np.arange(64))

@@ -401,6 +401,58 @@ def test_cartesian_map_nearest_site_is_voro_cell(clustered):
     assert np.array_equal(cid2, cid)
 
 
+def test_coverage_buffer_matches_reference_cli():
+    """The oracle's coverage-grid buffer (src/point_set.cpp:166-273) against the buffer sites in the
+    reference CLI's mapping.dat (tests/golden/coverage_buffer.npz: ids exact, positions written with 15
+    significant digits) for two setups with covered and uncovered candidates."""
+    g = np.load(os.path.join(GOLD, "coverage_buffer.npz"))
+    for name in ("c1", "lognormal"):
+        sn, dom = g[f"{name}_snapshot"], g[f"{name}_domain"]
+        ids, xyz = O.coverage_buffer(sn[:, 0], sn[:, 1], sn[:, 2], sn[:, 7], dom, float(g[f"{name}_spacing_fraction"]),
+                                     float(g[f"{name}_factor"]))
+        assert np.array_equal(ids, g[f"{name}_buffer_id"]), name
+        np.testing.assert_allclose(xyz, g[f"{name}_buffer_xyz"], rtol=0, atol=1e-14 * np.abs(dom).max())
+        assert 0 < len(ids) < np.prod(O.cartesian_lattice(dom, float(g[f"{name}_spacing_fraction"]))[1])
+
+
+def coverage_cases(rng):
+    """Particle sets for the coverage buffer: clustered with h over two decades in a non-cubic domain,
+    with particles outside the domain, h <= 0 and NaN h, and particles placed at exactly r from a
+    candidate along an axis."""
+    dom = np.array([-1.0, 2.0, 0.0, 1.5, -0.5, 0.5])
+    lo, hi = dom[0::2], dom[1::2]
+    n = 400
+    c = rng.uniform(lo, hi, size=(4, 3))
+    p = c[rng.integers(4, size=n)] + rng.normal(scale=0.15, size=(n, 3))
+    h = np.exp(rng.uniform(np.log(0.003), np.log(0.12), n))
+    h[:5] = [0.0, -0.05, np.nan, 0.3, 0.2]
+    sp, _, cand = O.cartesian_lattice(dom, 0.05)
+    k = rng.choice(len(cand), 6, replace=False)
+    p[-6:] = cand[k]
+    p[-6:, 0] += 2.0 * h[-6:]  # a candidate at exactly (or a rounding away from) the radius
+    return [(p, h, dom, 0.05, 2.0), (p, h, dom, 0.031, 1.0), (p[:50], h[:50] * 3, dom, 0.08, 2.5)]
+
+
+@pytest.mark.skipif(not os.path.exists(O.PTS_SO), reason="oracle/_ref not built (needs the reference sources)")
+def test_coverage_buffer_matches_reference_function():
+    """The oracle's restatement equals the reference's own add_coverage_grid_buffer (oracle/_ref/
+    libkiwi_pts.so) bit for bit on edge-case particle sets, and raises as it does with no h > 0."""
+    rng = np.random.default_rng(17)
+    for p, h, dom, fr, fa in coverage_cases(rng):
+        ids, xyz = O.coverage_buffer(p[:, 0], p[:, 1], p[:, 2], h, dom, fr, fa)
+        rid, rxyz = O.ref_coverage_buffer(p[:, 0], p[:, 1], p[:, 2], h, dom, fr, fa)
+        assert len(ids) > 0 and np.array_equal(ids, rid) and np.array_equal(xyz, rxyz)
+    # the lattice sizes of the restatement are the reference's
+    for dom, fr in (([0, 1, 0, 1, 0, 1], 0.05), ([0, 1.05, 0, 1, 0, 0.95], 0.1), ([0, 2.5, 0, 1, 0, 3.5], 0.2)):
+        _, n, _ = O.cartesian_lattice(dom, fr)
+        _, rx = O.ref_coverage_buffer([0.5], [0.5], [0.5], [1e-9], dom, fr, 1.0)
+        assert len(rx) == np.prod(n) - 1 or len(rx) == np.prod(n)
+    z = np.zeros(3)
+    for f in (O.coverage_buffer, O.ref_coverage_buffer):
+        with pytest.raises(ValueError, match="all smoothing lengths are <= 0"):
+            f(z, z, z, np.array([0.0, -1.0, np.nan]), [0, 1, 0, 1, 0, 1])
+
+
 def test_voronoi_binary128_arbiter():
     """The binary128 cell sums used to arbitrate Voronoi cells (oracle.voronoi_map_cells_quad): the
     reference's analytic cubic algorithm in binary128 equals the series-free generic path for the same
