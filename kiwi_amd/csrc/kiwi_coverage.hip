@@ -74,20 +74,38 @@ __device__ __forceinline__ void cov_range(const CovLattice& L, int a, double p, 
     hi = static_cast<int>(fmin(fmax(f1, -1.0), static_cast<double>(L.n[a] - 1)));
 }
 
-__global__ void k_cov_scan(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
-                           const double* __restrict__ z, const double* __restrict__ h,
-                           unsigned long long* __restrict__ out) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
+// a fixed grid strides over the particles; each block reduces in LDS and adds one atomic per counter
+constexpr int COV_SCAN_BLOCKS = 1024;
+
+__global__ __launch_bounds__(THREADS) void k_cov_scan(int64_t n, const double* __restrict__ x,
+                                                      const double* __restrict__ y, const double* __restrict__ z,
+                                                      const double* __restrict__ h,
+                                                      unsigned long long* __restrict__ out) {
+    __shared__ double s_h[THREADS / 64];
+    __shared__ unsigned long long s_bad[THREADS / 64];
     double hv = 0.0;
-    bool bad = false;
-    if (i < n) {
+    unsigned long long nbad = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * THREADS) {
         const double v = h[i];
-        hv = v > 0.0 ? v : 0.0;  // NaN and h <= 0 never raise the maximum from 0
-        bad = !(isfinite(x[i]) && isfinite(y[i]) && isfinite(z[i]));
+        hv = v > hv ? v : hv;  // NaN and h <= 0 never raise the maximum from 0
+        nbad += !(isfinite(x[i]) && isfinite(y[i]) && isfinite(z[i]));
     }
-    for (int o = 32; o > 0; o >>= 1) hv = fmax(hv, __shfl_xor(hv, o));
-    const unsigned long long nbad = __popcll(__ballot(bad));
+    for (int o = 32; o > 0; o >>= 1) {
+        hv = fmax(hv, __shfl_xor(hv, o));
+        nbad += __shfl_xor(nbad, o);
+    }
+    const int w = threadIdx.x / 64;
     if ((threadIdx.x & 63) == 0) {
+        s_h[w] = hv;
+        s_bad[w] = nbad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < THREADS / 64; ++k) {
+            hv = fmax(hv, s_h[k]);
+            nbad += s_bad[k];
+        }
         // for doubles >= 0 the bit patterns order as the values
         if (hv > 0.0) atomicMax(&out[0], static_cast<unsigned long long>(__double_as_longlong(hv)));
         if (nbad) atomicAdd(&out[1], nbad);
@@ -237,7 +255,8 @@ int kiwi_hip_coverage_buffer(kiwi_hip_ctx* c, const kiwi_particles* P, const dou
 
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[0], s));
     KIWI_HIP_CHECK(c, hipMemsetAsync(dst, 0, 2 * sizeof(unsigned long long), s));
-    k_cov_scan<<<grid1d(n), THREADS, 0, s>>>(n, d[0], d[1], d[2], d[3], dst);
+    k_cov_scan<<<static_cast<int>(std::min<int64_t>(grid1d(n), COV_SCAN_BLOCKS)), THREADS, 0, s>>>(n, d[0], d[1], d[2],
+                                                                                                 d[3], dst);
     KIWI_HIP_CHECK(c, hipGetLastError());
     unsigned long long hs[2] = {0, 0};
     KIWI_HIP_CHECK(c, hipMemcpyAsync(hs, dst, sizeof(hs), hipMemcpyDeviceToHost, s));
