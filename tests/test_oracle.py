@@ -260,7 +260,7 @@ def test_derived_tables_equal_reference_tables():
 def test_operation_count_follows_the_computed_path():
     """The counting instantiation behind the reference operation count (oracle/flops_per_pair.py)
     evaluates the same face integrals as the oracle, bit for bit, and counts a plausible number of
-    operations per cell (SURVEY-style static estimate: 7.9k-12.1k per pair)."""
+    operations per cell (a static estimate from the source: 7.9k-12.1k per pair)."""
     L = O.lib()
     L.oracle_face_counted.restype = ctypes.c_double
     L.oracle_face_counted.argtypes = [ctypes.c_double] * 4 + [ctypes.POINTER(ctypes.c_double), ctypes.c_int,
