@@ -1,12 +1,12 @@
-"""Builds libkiwi_hip.so (gfx950) and the host CLI in-tree with hipcc / g++.
+"""Builds libkiwi_hip.so (gfx950, hipcc) and libkiwi_io.so (host snapshot readers, g++) in-tree.
 
-    python -m kiwi_amd.build          # library + CLI
+    python -m kiwi_amd.build
 """
 import os
 import subprocess
 import sys
 
-from . import CLI_PATH, LIB_PATH, PKG_DIR, ROOT_DIR
+from . import IO_LIB_PATH, LIB_PATH, PKG_DIR, ROOT_DIR
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -35,21 +35,17 @@ def build_lib():
     return LIB_PATH
 
 
-def build_cli():
-    host = os.path.join(PKG_DIR, "host")
-    srcs = sorted(os.path.join(host, f) for f in os.listdir(host) if f.endswith(".cpp"))
-    if not srcs:
-        return None
-    tmp = CLI_PATH + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT_DIR, "include"), *srcs, "-o", tmp,
-          f"-L{PKG_DIR}", "-lkiwi_hip", f"-Wl,-rpath,{PKG_DIR}", "-Wl,-rpath,$ORIGIN"])
-    os.replace(tmp, CLI_PATH)
-    return CLI_PATH
+def build_io():
+    tmp = IO_LIB_PATH + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", "-pthread", "-I", os.path.join(ROOT_DIR, "include"),
+          os.path.join(PKG_DIR, "csrc", "kiwi_io.cpp"), "-o", tmp])
+    os.replace(tmp, IO_LIB_PATH)
+    return IO_LIB_PATH
 
 
 def build_all():
     build_lib()
-    build_cli()
+    build_io()
 
 
 if __name__ == "__main__":

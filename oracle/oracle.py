@@ -290,6 +290,37 @@ def ref_coverage_buffer(x, y, z, h, domain, spacing_fraction=0.05, factor=2.0):
     return oid[:k].copy(), np.stack([a[:k] for a in o], 1)
 
 
+def ref_load_snapshot(path, format="auto", hfact=1.2, cap=None):
+    """The reference's own load_sph_snapshot through oracle/_ref/libkiwi_pts.so.  Returns a dict of
+    arrays (id and the 9 columns); raises ValueError with the reference's message."""
+    L = ctypes.CDLL(PTS_SO)
+    L.ref_load_snapshot.restype = _ll
+    L.ref_load_snapshot.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _d, _pd, _pi, _ll, ctypes.c_char_p, _i]
+    if cap is None:
+        cap = (os.path.getsize(path) if os.path.exists(path) else 0) // 16 + 16  # a data row takes >= 16 bytes
+    cols = np.zeros((9, cap))
+    ids = np.zeros(cap, np.int32)
+    err = ctypes.create_string_buffer(4096)
+    n = L.ref_load_snapshot(os.fsencode(path), format.encode(), float(hfact), _p(cols), ids.ctypes.data_as(_pi), cap,
+                            err, 4096)
+    if n < 0:
+        raise ValueError(err.value.decode(errors="replace"))
+    assert n <= cap
+    out = {"id": ids[:n].copy()}
+    for k, f in enumerate(("x", "y", "z", "vx", "vy", "vz", "m", "h", "rho")):
+        out[f] = cols[k, :n].copy()
+    return out
+
+
+def ref_detect_format(path):
+    L = ctypes.CDLL(PTS_SO)
+    L.ref_detect_format.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _i]
+    buf = ctypes.create_string_buffer(4096)
+    if L.ref_detect_format(os.fsencode(path), buf, 4096) != 0:
+        raise ValueError(buf.value.decode(errors="replace"))
+    return buf.value.decode()
+
+
 def nearest_site(sites, q, chunk=4096):
     """Index of the nearest of sites (n, 3) to every point of q (m, 3), brute force: (s - q)^2 summed x,
     y, z in double with no fused multiply-add, as voro++ forms it (v_compute.cc); the first minimum

@@ -1,5 +1,6 @@
 // C-ABI shim over the reference's coverage-grid buffer (add_coverage_grid_buffer, src/point_set.cpp:
-// 208-273) and its candidate lattice (generate_cartesian_lattice, :166-195), linked with the reference's
+// 208-273), its candidate lattice (generate_cartesian_lattice, :166-195) and its snapshot readers
+// (load_sph_snapshot and detect_snapshot_format, src/snapshot.cpp:495-522), linked with the reference's
 // own objects into oracle/_ref/libkiwi_pts.so.  Test infrastructure: it runs the reference function on
 // a plain-array point set so the oracle's restatement and the GPU buffer can be checked against it and
 // timed beside it; nothing here is shipped or run on a GPU.
@@ -10,6 +11,7 @@
 
 #include "kiwi/config.hpp"
 #include "kiwi/point_set.hpp"
+#include "kiwi/snapshot.hpp"
 
 extern "C" {
 
@@ -51,6 +53,41 @@ long long ref_coverage_buffer(long long n, const double* x, const double* y, con
             std::strncpy(err, e.what(), errlen - 1);
             err[errlen - 1] = 0;
         }
+        return -1;
+    }
+}
+
+// The reference's snapshot reader: the row count (up to cap rows into cols, 9 columns x cap, and ids),
+// or -1 with the exception's message in err.
+long long ref_load_snapshot(const char* path, const char* format, double hfact, double* cols, int* ids, long long cap,
+                            char* err, int errlen) {
+    try {
+        const kiwi::SPHSnapshot s = kiwi::load_sph_snapshot(path, format, hfact);
+        const long long n = static_cast<long long>(s.size());
+        const std::vector<double>* c[9] = {&s.x, &s.y, &s.z, &s.vx, &s.vy, &s.vz, &s.m, &s.h, &s.rho};
+        for (long long i = 0; i < std::min(n, cap); ++i) {
+            ids[i] = s.id[i];
+            for (int k = 0; k < 9; ++k) cols[k * cap + i] = (*c[k])[i];
+        }
+        return n;
+    } catch (const std::exception& e) {
+        if (errlen > 0) {
+            std::strncpy(err, e.what(), errlen - 1);
+            err[errlen - 1] = 0;
+        }
+        return -1;
+    }
+}
+
+int ref_detect_format(const char* path, char* out, int cap) {
+    try {
+        const std::string f = kiwi::detect_snapshot_format(path);
+        std::strncpy(out, f.c_str(), cap - 1);
+        out[cap - 1] = 0;
+        return 0;
+    } catch (const std::exception& e) {
+        std::strncpy(out, e.what(), cap - 1);
+        out[cap - 1] = 0;
         return -1;
     }
 }

@@ -117,3 +117,17 @@ def test_cartesian_lattice_matches_reference_sizes():
         hip.cartesian_lattice([0, 1, 0, 1, 0, 1], 0.0)
     with pytest.raises(ValueError):
         hip.cartesian_lattice([0, 1, 1, 1, 0, 1], 0.05)
+
+
+def test_io_library_exports_every_declared_symbol():
+    """libkiwi_io.so (host snapshot readers) loads and exports every function include/kiwi_io.h declares;
+    its struct layout matches the ctypes view."""
+    from kiwi_amd import io
+    L = io.load()
+    text = open(os.path.join(ROOT, "include", "kiwi_io.h")).read()
+    names = re.findall(r"^(?:int|void|const char\*)\s+(kiwi_io_\w+)\(", text, re.M)
+    assert len(names) >= 6
+    for n in names:
+        assert hasattr(L, n), n
+    assert L.kiwi_io_abi_version() == int(re.search(r"#define KIWI_IO_ABI_VERSION (\d+)", text).group(1))
+    assert ctypes.sizeof(io.Snapshot) == 8 + 8 * 10
