@@ -7,8 +7,9 @@
 //
 //   kiwi_io_detect_format   phantom_binary (a Phantom capture record first), splash_ascii (a first
 //                           line "#...SPLASH..."), else plain_ascii (detect_snapshot_format, :495-509)
-//   kiwi_io_load_snapshot   plain_ascii (load_plain_ascii, :32-84) and splash_ascii (load_splash_ascii,
-//                           :101-491), "auto" resolved as above; phantom_binary is not read here
+//   kiwi_io_load_snapshot   plain_ascii (load_plain_ascii, :32-84), splash_ascii (load_splash_ascii,
+//                           :101-491) and phantom_binary (load_phantom_dump, src/phantom_dump.cpp:156-317),
+//                           "auto" resolved as above
 #include "kiwi_io.h"
 
 #include <algorithm>
@@ -462,54 +463,326 @@ bool load_splash(const std::string& path, double hfact, std::vector<char>& buf, 
     return true;
 }
 
-// ---------------------------------------------------------------- detection (:495-509)
-// A Phantom dump opens with the capture record (i1 r1 i2 iversion i3 = 60769, 60878.0, 60878, v, 690706)
-// framed by 4-byte lengths of 20, 24, 36 or 40 bytes in either byte order (src/phantom_dump.cpp:60-95).
-bool phantom_capture(const std::string& path) {
-    FILE* f = std::fopen(path.c_str(), "rb");
-    if (!f) return false;
-    unsigned char head[4 + 40 + 4];
-    const size_t got = std::fread(head, 1, sizeof(head), f);
-    std::fclose(f);
-    if (got < 4) return false;
-    auto load = [](const unsigned char* p, int n, bool swap, unsigned char* dst) {
-        for (int i = 0; i < n; ++i) dst[i] = swap ? p[n - 1 - i] : p[i];
-    };
-    for (bool swap : {false, true}) {
-        int32_t len;
-        load(head, 4, swap, reinterpret_cast<unsigned char*>(&len));
-        int ib, rb;
+// ---------------------------------------------------------------- phantom_binary (src/phantom_dump.cpp)
+// Fortran unformatted sequential records ([4-byte length][payload][4-byte length]) walked with a cursor
+// over the file in memory; failures carry the reference's "phantom-binary snapshot: <why> while
+// reading <what>: <path>" text (RecordReader, :50-132).
+struct Records {
+    const char* p;
+    size_t size, at = 0;
+    bool swap = false;
+    std::string path;
+    std::string err;
+
+    template <class T>
+    T get(const char* q) const {
+        unsigned char b[sizeof(T)];
+        for (size_t i = 0; i < sizeof(T); ++i) b[i] = static_cast<unsigned char>(swap ? q[sizeof(T) - 1 - i] : q[i]);
+        T v;
+        std::memcpy(&v, b, sizeof(T));
+        return v;
+    }
+    bool fail(const std::string& why, const std::string& what) {
+        err = "phantom-binary snapshot: " + why + " while reading " + what + ": " + path;
+        return false;
+    }
+    bool marker(const std::string& what, int32_t& n) {
+        if (at + 4 > size) {
+            at = size;
+            return fail("unexpected end of file", what);
+        }
+        n = get<int32_t>(p + at);
+        at += 4;
+        if (n < 0) return fail("negative record length (records over 2 GB are not supported)", what);
+        return true;
+    }
+    // the next record's payload (pointer into the file) and length
+    bool record(const std::string& what, const char*& q, size_t& len) {
+        int32_t n = 0, m = 0;
+        if (!marker(what, n)) return false;
+        if (at + static_cast<size_t>(n) > size) {
+            at = size;
+            return fail("unexpected end of file", what);
+        }
+        q = p + at;
+        len = static_cast<size_t>(n);
+        at += len;
+        if (!marker(what, m)) return false;
+        if (m != n) return fail("record markers disagree (" + std::to_string(n) + " vs " + std::to_string(m) + ")", what);
+        return true;
+    }
+    bool skip(const std::string& what) {
+        int32_t n = 0, m = 0;
+        if (!marker(what, n)) return false;
+        at = std::min(size, at + static_cast<size_t>(n));  // a seek past the end fails at the next marker
+        if (!marker(what, m)) return false;
+        if (m != n) return fail("record markers disagree while skipping", what);
+        return true;
+    }
+};
+
+struct Kind {
+    int size;
+    bool real;
+};
+
+double decode(const Records& R, const char* q, Kind k) {
+    if (k.real) return k.size == 4 ? static_cast<double>(R.get<float>(q)) : R.get<double>(q);
+    switch (k.size) {
+        case 1: return static_cast<double>(R.get<int8_t>(q));
+        case 2: return static_cast<double>(R.get<int16_t>(q));
+        case 4: return static_cast<double>(R.get<int32_t>(q));
+        default: return static_cast<double>(R.get<int64_t>(q));
+    }
+}
+
+// the capture record (i1 r1 i2 iversion i3): byte order and default int / real widths (:60-95)
+bool capture(const char* p, size_t size, bool& swap, int& ib, int& rb) {
+    if (size < 4) return false;
+    for (bool sw : {false, true}) {
+        Records R{p, size};
+        R.swap = sw;
+        const int32_t len = R.get<int32_t>(p);
         if (len == 20) ib = 4, rb = 4;
         else if (len == 24) ib = 4, rb = 8;
         else if (len == 36) ib = 8, rb = 4;
         else if (len == 40) ib = 8, rb = 8;
         else continue;
-        if (got < static_cast<size_t>(len) + 8) return false;
-        const unsigned char* p = head + 4;
-        int32_t tail;
-        load(p + len, 4, swap, reinterpret_cast<unsigned char*>(&tail));
-        if (tail != len) return false;
-        auto get_int = [&](int off) -> int64_t {
-            if (ib == 4) {
-                int32_t v;
-                load(p + off, 4, swap, reinterpret_cast<unsigned char*>(&v));
-                return v;
-            }
-            int64_t v;
-            load(p + off, 8, swap, reinterpret_cast<unsigned char*>(&v));
-            return v;
-        };
-        double r1;
-        if (rb == 4) {
-            float v;
-            load(p + ib, 4, swap, reinterpret_cast<unsigned char*>(&v));
-            r1 = v;
-        } else {
-            load(p + ib, 8, swap, reinterpret_cast<unsigned char*>(&r1));
-        }
-        return get_int(0) == 60769 && r1 == 60878.0 && get_int(ib + rb) == 60878 && get_int(3 * ib + rb) == 690706;
+        if (size < static_cast<size_t>(len) + 8) return false;
+        const char* q = p + 4;
+        if (R.get<int32_t>(q + len) != len) return false;
+        auto gi = [&](int off) { return ib == 4 ? static_cast<int64_t>(R.get<int32_t>(q + off)) : R.get<int64_t>(q + off); };
+        const double r1 = rb == 4 ? static_cast<double>(R.get<float>(q + ib)) : R.get<double>(q + ib);
+        if (gi(0) != 60769 || gi(ib + rb) != 60878 || gi(3 * ib + rb) != 690706 || r1 != 60878.0) return false;
+        swap = sw;
+        return true;
     }
     return false;
+}
+
+std::string trim_spaces(const char* q, size_t n) {
+    size_t a = 0, b = n;
+    while (a < b && q[a] == ' ') ++a;
+    while (b > a && q[b - 1] == ' ') --b;
+    return std::string(q + a, b - a);
+}
+
+bool load_phantom(const std::string& path, const std::vector<char>& buf, const kiwi_phantom_units& units,
+                  bool allow_missing_velocity, Columns& out, std::string& err) {
+    const size_t size = buf.size() - 1;
+    Records R{buf.data(), size};
+    R.path = path;
+    int ib = 0, rb = 0;
+    if (!capture(buf.data(), size, R.swap, ib, rb)) {
+        err = "phantom-binary snapshot: the file does not start with the Phantom capture pattern (is it a Phantom "
+              "dump?): " + path;
+        return false;
+    }
+    R.at = 4 + static_cast<size_t>(4 * ib + rb) + 4;
+    auto bad = [&]() {
+        err = R.err;
+        return false;
+    };
+    const char* q;
+    size_t len;
+    if (!R.record("the file identifier", q, len)) return bad();
+    if (len < 2 || (q[0] != 'F' && q[0] != 'S') || q[1] != 'T') {
+        R.fail("only tagged full dumps (file identifier 'FT:...') are supported, got '" +
+                   std::string(q, std::min<size_t>(len, 20)) + "'", "the file identifier");
+        return bad();
+    }
+    const Kind kinds[8] = {{ib, false}, {1, false}, {2, false}, {4, false}, {8, false}, {rb, true}, {4, true}, {8, true}};
+    constexpr int TAG = 16;
+    // the global header: eight typed sections of (tag, value) pairs, a tag may repeat
+    std::unordered_map<std::string, std::vector<double>> hdr;
+    for (int k = 0; k < 8; ++k) {
+        if (!R.record("a header section size", q, len)) return bad();
+        if (len != 4) {
+            R.fail("malformed header section size", "the global header");
+            return bad();
+        }
+        const int32_t nv = R.get<int32_t>(q);
+        if (nv <= 0) continue;
+        const char *tq, *vq;
+        size_t tl, vl;
+        if (!R.record("header tags", tq, tl) || !R.record("header values", vq, vl)) return bad();
+        if (tl != static_cast<size_t>(nv) * TAG || vl != static_cast<size_t>(nv) * static_cast<size_t>(kinds[k].size)) {
+            R.fail("header section " + std::to_string(k) + " has inconsistent sizes", "the global header");
+            return bad();
+        }
+        for (int i = 0; i < nv; ++i)
+            hdr[trim_spaces(tq + static_cast<size_t>(i) * TAG, TAG)].push_back(
+                decode(R, vq + static_cast<size_t>(i) * kinds[k].size, kinds[k]));
+    }
+    const std::vector<double>* need[5];
+    const char* keys[5] = {"massoftype", "hfact", "udist", "umass", "utime"};
+    for (int i = 0; i < 5; ++i) {
+        auto it = hdr.find(keys[i]);
+        if (it == hdr.end() || it->second.empty()) {
+            R.fail(std::string("required header entry '") + keys[i] + "' is missing", "the global header");
+            return bad();
+        }
+        need[i] = &it->second;
+    }
+    const std::vector<double>& massoftype = *need[0];
+    const double hfact = (*need[1])[0], udist = (*need[2])[0], umass = (*need[3])[0], utime = (*need[4])[0];
+    if (hfact <= 0.0 || udist <= 0.0 || umass <= 0.0 || utime <= 0.0) {
+        R.fail("header hfact/udist/umass/utime must be > 0", "the global header");
+        return bad();
+    }
+    auto first_of = [&](const char* key, bool& has) -> double {
+        auto it = hdr.find(key);
+        has = it != hdr.end();
+        return has && !it->second.empty() ? it->second[0] : 0.0;
+    };
+    bool has;
+    const double hb = first_of("nblocks", has);
+    const int hdr_blocks = has ? std::max(1, static_cast<int>(hb)) : 1;
+    // the array blocks
+    if (!R.record("the number of array blocks", q, len)) return bad();
+    if (len != 4) {
+        R.fail("malformed block count", "the array section");
+        return bad();
+    }
+    const int nblocks = R.get<int32_t>(q);
+    if (nblocks <= 0 || nblocks % hdr_blocks != 0) {
+        R.fail("array block count " + std::to_string(nblocks) + " is not a multiple of the header's nblocks " +
+                   std::to_string(hdr_blocks), "the array section");
+        return bad();
+    }
+    struct Block {
+        int64_t n;
+        int32_t nums[8];
+    };
+    std::vector<Block> blocks(static_cast<size_t>(nblocks));
+    for (auto& b : blocks) {
+        if (!R.record("a block header", q, len)) return bad();
+        if (len != 8 + 4 * 8) {
+            R.fail("malformed block header", "the array section");
+            return bad();
+        }
+        b.n = R.get<int64_t>(q);
+        for (int k = 0; k < 8; ++k) b.nums[k] = R.get<int32_t>(q + 8 + 4 * k);
+        if (b.n < 0) {
+            R.fail("negative particle count", "the array section");
+            return bad();
+        }
+    }
+    const double nt = first_of("ntypes", has);
+    const int ntypes = has ? static_cast<int>(nt) : static_cast<int>(massoftype.size());
+    bool several = false;
+    if (hdr.count("npartoftype")) {
+        const auto& npt = hdr["npartoftype"];
+        for (int t = 1; t < ntypes && t < static_cast<int>(npt.size()); ++t)
+            if (npt[t] > 0.0) several = true;
+    }
+    static const char* wanted[8] = {"x", "y", "z", "vx", "vy", "vz", "h", "itype"};
+    const double lf = udist / units.length_cm, mf = umass / units.mass_g, vf = (udist / utime) / units.velocity_cms;
+    const double rho_cgs = umass / (udist * udist * udist);
+    const double m_gas = massoftype[0];
+    long long n_seen = 0, n_dead = 0, n_other = 0;
+    bool zeroed = false;
+    const int nsub = nblocks / hdr_blocks;
+    for (int bi = 0; bi < nblocks; ++bi) {
+        const Block& bh = blocks[static_cast<size_t>(bi)];
+        const bool sink = bi % nsub == 1;
+        struct Arr {
+            const char* q = nullptr;
+            Kind k{0, false};
+        };
+        Arr arr[8];  // wanted order
+        for (int k = 0; k < 8; ++k)
+            for (int a = 0; a < bh.nums[k]; ++a) {
+                if (!R.record("an array tag", q, len)) return bad();
+                if (len != static_cast<size_t>(TAG)) {
+                    R.fail("malformed array tag", "the array section");
+                    return bad();
+                }
+                const std::string name = trim_spaces(q, TAG);
+                int w = -1;
+                for (int j = 0; j < 8; ++j)
+                    if (name == wanted[j]) w = j;
+                if (sink || w < 0 || arr[w].q) {
+                    if (!R.skip("array '" + name + "'")) return bad();
+                    continue;
+                }
+                if (!R.record("array '" + name + "'", q, len)) return bad();
+                if (len != static_cast<size_t>(bh.n) * static_cast<size_t>(kinds[k].size)) {
+                    R.fail("array '" + name + "' has " + std::to_string(len) + " bytes, expected " +
+                               std::to_string(bh.n * kinds[k].size), "the array section");
+                    return bad();
+                }
+                arr[w] = Arr{q, kinds[k]};
+            }
+        if (sink || bh.n == 0) continue;
+        for (int j : {0, 1, 2, 6})
+            if (!arr[j].q) {
+                R.fail(std::string("array '") + wanted[j] + "' not found in a particle block", "the array section");
+                return bad();
+            }
+        std::string missing;
+        for (int j : {3, 4, 5})
+            if (!arr[j].q) missing += std::string(missing.empty() ? "" : ", ") + wanted[j];
+        if (!missing.empty()) {
+            if (!allow_missing_velocity) {
+                R.fail("velocity array(s) " + missing + " not found in a particle block (a Phantom small dump has "
+                       "none). Set \"allow_missing_velocity\": true in the config to use the dump for density only, "
+                       "with all velocities set to zero", "the array section");
+                return bad();
+            }
+            zeroed = true;
+        }
+        const bool has_itype = arr[7].q != nullptr;
+        if (!has_itype && several) {
+            R.fail("the header lists particles of more than one type but the dump has no itype array",
+                   "the array section");
+            return bad();
+        }
+        auto val = [&](int j, int64_t i) {
+            return arr[j].q ? decode(R, arr[j].q + static_cast<size_t>(i) * arr[j].k.size, arr[j].k) : 0.0;
+        };
+        for (int64_t i = 0; i < bh.n; ++i) {
+            ++n_seen;
+            const double h = val(6, i);
+            if (!(h > 0.0)) {
+                ++n_dead;
+                continue;
+            }
+            if (has_itype && std::lround(val(7, i)) != 1) {
+                ++n_other;
+                continue;
+            }
+            const double qh = hfact / h;
+            const double row[9] = {val(0, i) * lf, val(1, i) * lf, val(2, i) * lf, val(3, i) * vf, val(4, i) * vf,
+                                   val(5, i) * vf, m_gas * mf, h * lf, m_gas * qh * qh * qh * rho_cgs};
+            for (int k = 0; k < 9; ++k) out.c[k].push_back(row[k]);
+        }
+    }
+    if (out.c[0].empty()) {
+        R.fail("no gas particles found", "the array section");
+        return bad();
+    }
+    std::cerr << "[kiwi] phantom-binary snapshot: kept " << out.c[0].size() << " of " << n_seen << " particle(s) (dropped "
+              << n_dead << " dead with h <= 0 and " << n_other << " of another type; sink particles are ignored); gas "
+              << "mass " << m_gas * mf << " each, density from the header's hfact = " << hfact << ": " << path << "\n";
+    if (zeroed)
+        std::cerr << "[kiwi] WARNING: this dump has no velocity arrays (a Phantom small dump?); all velocities are set "
+                     "to zero because allow_missing_velocity is true. Density is unaffected, but every velocity output "
+                     "(vx/vy/vz grids, POLARIS velocities, ...) is meaningless.\n";
+    return true;
+}
+
+// ---------------------------------------------------------------- detection (:495-509)
+// A Phantom dump opens with the capture record (is_phantom_dump, src/phantom_dump.cpp:145-154)
+bool phantom_capture(const std::string& path) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    char head[4 + 40 + 4];
+    const size_t got = std::fread(head, 1, sizeof(head), f);
+    std::fclose(f);
+    bool swap;
+    int ib, rb;
+    return capture(head, got, swap, ib, rb);
 }
 
 bool detect(const std::string& path, std::string& fmt, std::string& err) {
@@ -556,7 +829,12 @@ int kiwi_io_detect_format(const char* path, char* format, int capacity) {
     return KIWI_IO_OK;
 }
 
-int kiwi_io_load_snapshot(const char* path_c, const char* format_c, double hfact, kiwi_snapshot* out) {
+int kiwi_io_load_snapshot(const char* path, const char* format, double hfact, kiwi_snapshot* out) {
+    return kiwi_io_load_snapshot_ex(path, format, hfact, nullptr, 0, out);
+}
+
+int kiwi_io_load_snapshot_ex(const char* path_c, const char* format_c, double hfact, const kiwi_phantom_units* units_c,
+                             int allow_missing_velocity, kiwi_snapshot* out) {
     g_err.clear();
     if (!path_c || !out) {
         g_err = "null argument";
@@ -566,11 +844,7 @@ int kiwi_io_load_snapshot(const char* path_c, const char* format_c, double hfact
     const std::string path = path_c, format = format_c ? format_c : "auto";
     std::string fmt = format;
     if (fmt == "auto" && !detect(path, fmt, g_err)) return KIWI_IO_ERR_READ;
-    if (fmt == "phantom_binary") {
-        g_err = "phantom_binary snapshots are not read by libkiwi_io (use the reference's load_phantom_dump): " + path;
-        return KIWI_IO_ERR_UNSUPPORTED;
-    }
-    if (fmt != "plain_ascii" && fmt != "splash_ascii") {
+    if (fmt != "plain_ascii" && fmt != "splash_ascii" && fmt != "phantom_binary") {
         g_err = "unknown SPH snapshot format '" + format +
                 "' (expected auto, plain_ascii, splash_ascii, or phantom_binary)";
         return KIWI_IO_ERR_INVALID_ARG;
@@ -581,7 +855,10 @@ int kiwi_io_load_snapshot(const char* path_c, const char* format_c, double hfact
         return KIWI_IO_ERR_READ;
     }
     Columns cols;
-    const bool ok = fmt == "plain_ascii" ? load_plain(path, buf, cols, g_err) : load_splash(path, hfact, buf, cols, g_err);
+    const kiwi_phantom_units units = units_c ? *units_c : kiwi_phantom_units{3.086e18, 1.989e33, 1.0e5};
+    const bool ok = fmt == "plain_ascii"    ? load_plain(path, buf, cols, g_err)
+                    : fmt == "splash_ascii" ? load_splash(path, hfact, buf, cols, g_err)
+                                            : load_phantom(path, buf, units, allow_missing_velocity != 0, cols, g_err);
     if (!ok) return KIWI_IO_ERR_FORMAT;
     const size_t n = cols.c[0].size();
     if (n > 0x7fffffffu) {

@@ -40,6 +40,8 @@ def load():
         L.kiwi_io_last_error.restype = ctypes.c_char_p
         L.kiwi_io_detect_format.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.kiwi_io_load_snapshot.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double, ctypes.POINTER(Snapshot)]
+        L.kiwi_io_load_snapshot_ex.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double,
+                                               ctypes.POINTER(ctypes.c_double * 3), ctypes.c_int, ctypes.POINTER(Snapshot)]
         L.kiwi_io_free_snapshot.argtypes = [ctypes.POINTER(Snapshot)]
         L.kiwi_io_set_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -61,9 +63,12 @@ def detect_format(path):
     return buf.value.decode()
 
 
-def load_snapshot(path, format="auto", hfact=1.2):
+def load_snapshot(path, format="auto", hfact=1.2, units=None, allow_missing_velocity=False):
+    """units: (length_cm, mass_g, velocity_cms) of a phantom_binary read, None for the defaults."""
     s = Snapshot()
-    _check(load().kiwi_io_load_snapshot(os.fsencode(path), format.encode(), float(hfact), ctypes.byref(s)))
+    u = None if units is None else ctypes.byref((ctypes.c_double * 3)(*[float(v) for v in units]))
+    _check(load().kiwi_io_load_snapshot_ex(os.fsencode(path), format.encode(), float(hfact), u,
+                                           int(bool(allow_missing_velocity)), ctypes.byref(s)))
     try:
         n = s.n
         out = {"id": np.ctypeslib.as_array(s.id, (n,)).copy() if n else np.zeros(0, np.int32)}

@@ -290,19 +290,20 @@ def ref_coverage_buffer(x, y, z, h, domain, spacing_fraction=0.05, factor=2.0):
     return oid[:k].copy(), np.stack([a[:k] for a in o], 1)
 
 
-def ref_load_snapshot(path, format="auto", hfact=1.2, cap=None):
+def ref_load_snapshot(path, format="auto", hfact=1.2, cap=None, units=None, allow_missing_velocity=False):
     """The reference's own load_sph_snapshot through oracle/_ref/libkiwi_pts.so.  Returns a dict of
     arrays (id and the 9 columns); raises ValueError with the reference's message."""
     L = ctypes.CDLL(PTS_SO)
     L.ref_load_snapshot.restype = _ll
-    L.ref_load_snapshot.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _d, _pd, _pi, _ll, ctypes.c_char_p, _i]
+    L.ref_load_snapshot.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _d, _pd, _i, _pd, _pi, _ll, ctypes.c_char_p, _i]
     if cap is None:
         cap = (os.path.getsize(path) if os.path.exists(path) else 0) // 16 + 16  # a data row takes >= 16 bytes
     cols = np.zeros((9, cap))
     ids = np.zeros(cap, np.int32)
     err = ctypes.create_string_buffer(4096)
-    n = L.ref_load_snapshot(os.fsencode(path), format.encode(), float(hfact), _p(cols), ids.ctypes.data_as(_pi), cap,
-                            err, 4096)
+    u = None if units is None else np.ascontiguousarray(units, dtype=np.float64)
+    n = L.ref_load_snapshot(os.fsencode(path), format.encode(), float(hfact), None if u is None else _p(u),
+                            int(bool(allow_missing_velocity)), _p(cols), ids.ctypes.data_as(_pi), cap, err, 4096)
     if n < 0:
         raise ValueError(err.value.decode(errors="replace"))
     assert n <= cap

@@ -57,12 +57,15 @@ long long ref_coverage_buffer(long long n, const double* x, const double* y, con
     }
 }
 
-// The reference's snapshot reader: the row count (up to cap rows into cols, 9 columns x cap, and ids),
-// or -1 with the exception's message in err.
-long long ref_load_snapshot(const char* path, const char* format, double hfact, double* cols, int* ids, long long cap,
-                            char* err, int errlen) {
+// The reference's snapshot reader (units3: PhantomUnits length_cm, mass_g, velocity_cms, or NULL for
+// the defaults): the row count (up to cap rows into cols, 9 columns x cap, and ids), or -1 with the
+// exception's message in err.
+long long ref_load_snapshot(const char* path, const char* format, double hfact, const double* units3, int allow_missing,
+                            double* cols, int* ids, long long cap, char* err, int errlen) {
     try {
-        const kiwi::SPHSnapshot s = kiwi::load_sph_snapshot(path, format, hfact);
+        kiwi::PhantomUnits u;
+        if (units3) u.length_cm = units3[0], u.mass_g = units3[1], u.velocity_cms = units3[2];
+        const kiwi::SPHSnapshot s = kiwi::load_sph_snapshot(path, format, hfact, u, allow_missing != 0);
         const long long n = static_cast<long long>(s.size());
         const std::vector<double>* c[9] = {&s.x, &s.y, &s.z, &s.vx, &s.vy, &s.vz, &s.m, &s.h, &s.rho};
         for (long long i = 0; i < std::min(n, cap); ++i) {

@@ -1,7 +1,8 @@
 """Measurement of the host snapshot readers (libkiwi_io) against the reference's own reader
 (load_sph_snapshot through oracle/_ref/libkiwi_pts.so, one thread); not collected by pytest.
 Workload: the sph-to-grid benchmark's 1e6 log-normal particles written as plain_ascii (%.17g, as the
-reference's write_plain_ascii does) and as a splash_ascii file with the same columns plus itype.
+reference's write_plain_ascii does), as a splash_ascii file with the same columns plus itype, and as a
+tagged Phantom dump (positions and velocities real*8, h real*4, itype int*1).
 
     python tests/perf_io.py [--particles 1000000] [--threads 0] [--dir /tmp]
 
@@ -30,7 +31,7 @@ def main():
     ap.add_argument("--dir", default="/tmp")
     args = ap.parse_args()
     from oracle import oracle as O
-    from test_io import LABELS, splash_text
+    from test_io import LABELS, phantom_dump, splash_text
 
     parts, _, _ = workloads.lognormal_field(n_particles=args.particles, n=256, seed=2)
     parts["rho"] = np.asarray(parts.get("rho", np.ones_like(parts["x"])), dtype=np.float64)
@@ -41,10 +42,15 @@ def main():
                     [np.zeros_like(parts["x"])], 1)
     with open(splash, "w") as f:
         f.write(splash_text(LABELS, cols, itype=np.ones(len(cols), int)))
+    phantom = os.path.join(args.dir, f"kiwi_io_phantom_{os.getpid()}.bin")
+    arrays = [(c, 7, parts[c]) for c in ("x", "y", "z", "vx", "vy", "vz")] + [("h", 6, parts["h"]),
+                                                                             ("itype", 1, np.ones(len(cols), int))]
+    with open(phantom, "wb") as f:
+        f.write(phantom_dump([(len(cols), arrays), (0, [])], npartoftype=(len(cols), 0)))
     threads = io.set_threads(args.threads)
     rec = {"metric": "snapshot read particles/s", "particles": args.particles, "threads": threads, "cases": {}}
     try:
-        for name, path in (("plain_ascii", plain), ("splash_ascii", splash)):
+        for name, path in (("plain_ascii", plain), ("splash_ascii", splash), ("phantom_binary", phantom)):
             t = time.perf_counter()
             a = io.load_snapshot(path)
             ta = time.perf_counter() - t
@@ -60,6 +66,7 @@ def main():
     finally:
         os.remove(plain)
         os.remove(splash)
+        os.remove(phantom)
     print(json.dumps(rec), flush=True)
 
 

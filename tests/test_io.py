@@ -220,7 +220,116 @@ def test_detect_phantom_capture(tmp_path):
         n = len(body)
         p = write(tmp_path, f"cap{ib}{rb}{end}", struct.pack(f"{end}i", n) + body + struct.pack(f"{end}i", n) + b"\0" * 64)
         assert io.detect_format(p) == O.ref_detect_format(p) == "phantom_binary"
-        with pytest.raises(io.SnapshotError):
-            io.load_snapshot(p)
         q = write(tmp_path, f"nocap{ib}{rb}{end}", struct.pack(f"{end}i", n) + body[:-1] + b"\1" + struct.pack(f"{end}i", n))
         assert io.detect_format(q) == O.ref_detect_format(q) == "plain_ascii"
+
+
+# ---------------------------------------------------------------- phantom_binary
+def phantom_dump(blocks, end="<", ib=4, rb=8, ident=b"FT:Phantom test dump", header=None, hdr_nblocks=1,
+                 npartoftype=(1, 0), massoftype=(1e-6, 2e-6), cut=None, poke=None):
+    """A tagged Phantom dump: capture record, identifier, the eight typed header sections, the block
+    count and headers, then every block's arrays.  blocks: list of (n, [(tag, kind, values)]) with
+    kind 0..7 = default int, int*1, int*2, int*4, int*8, default real, real*4, real*8."""
+    ic, rc = ("i" if ib == 4 else "q"), ("f" if rb == 4 else "d")
+    fmt = {0: ic, 1: "b", 2: "h", 3: "i", 4: "q", 5: rc, 6: "f", 7: "d"}
+
+    def rec(payload):
+        return struct.pack(end + "i", len(payload)) + payload + struct.pack(end + "i", len(payload))
+
+    out = rec(struct.pack(f"{end}{ic}{rc}{ic}{ic}{ic}", 60769, 60878.0, 60878, 1, 690706))
+    out += rec(ident.ljust(100))
+    if header is None:
+        header = {0: [("nblocks", hdr_nblocks), ("ntypes", len(npartoftype))] + [("npartoftype", v) for v in npartoftype],
+                  5: [("massoftype", v) for v in massoftype] + [("hfact", 1.2), ("udist", 3.086e16),
+                                                                 ("umass", 1.989e33), ("utime", 3.15e13)]}
+    for k in range(8):
+        items = header.get(k, [])
+        out += rec(struct.pack(end + "i", len(items)))
+        if items:
+            out += rec(b"".join(t.encode().ljust(16) for t, _ in items))
+            out += rec(b"".join(struct.pack(end + fmt[k], v) for _, v in items))
+    out += rec(struct.pack(end + "i", len(blocks)))
+    for n, arrays in blocks:
+        nums = [sum(1 for a in arrays if a[1] == k) for k in range(8)]
+        out += rec(struct.pack(end + "q", n) + struct.pack(end + "8i", *nums))
+    for n, arrays in blocks:
+        for k in range(8):
+            for tag, kk, vals in arrays:
+                if kk == k:
+                    out += rec(tag.encode().ljust(16))
+                    out += rec(np.asarray(vals).astype(np.dtype(end + fmt[k])).tobytes())
+    if poke is not None:
+        out = out[:poke] + bytes([out[poke] ^ 0x40]) + out[poke + 1:]
+    return out[:cut] if cut is not None else out
+
+
+def particle_block(rng, n, vel=True, itype=True, hkind=6, extra=True):
+    a = [(c, 7, rng.normal(size=n)) for c in ("x", "y", "z")]
+    if vel:
+        a += [(c, 7, rng.normal(size=n)) for c in ("vx", "vy", "vz")]
+    h = np.abs(rng.normal(size=n)) + 0.01
+    h[::17] = 0.0
+    h[5::23] = -1.0
+    a.append(("h", hkind, h))
+    if itype:
+        a.append(("itype", 1, np.where(rng.random(n) < 0.85, 1, 2)))
+    if extra:
+        a.append(("u", 6, rng.random(n)))
+    return (n, a)
+
+
+def test_phantom_binary_variants(tmp_path, capfd):
+    rng = np.random.default_rng(6)
+    sink = (2, [("x", 7, [0.0, 1.0]), ("m", 7, [1.0, 2.0]), ("h", 7, [0.1, 0.1])])
+    cases = {
+        "le_i4_r8": dict(blocks=[particle_block(rng, 500), sink]),
+        "be_i8_r4": dict(blocks=[particle_block(rng, 300, itype=False), (0, [])], end=">", ib=8, rb=4,
+                         npartoftype=(300,), massoftype=(3e-7,)),
+        "two_tasks": dict(blocks=[particle_block(rng, 200), sink, particle_block(rng, 150), (0, [])], hdr_nblocks=2),
+        "h_real8_no_extra": dict(blocks=[particle_block(rng, 100, hkind=7, extra=False), sink]),
+    }
+    for name, kw in cases.items():
+        p = write(tmp_path, name, phantom_dump(**kw))
+        assert io.detect_format(p) == O.ref_detect_format(p) == "phantom_binary"
+        a = same(p, capfd)
+        assert a[0] == "ok" and "kept" in a[2], name
+    units = (3.0857e18, 1.9891e33, 1.0e5)
+    p = write(tmp_path, "u", phantom_dump(**cases["le_i4_r8"]))
+    a, b = io.load_snapshot(p, units=units), O.ref_load_snapshot(p, units=units)
+    assert all(np.array_equal(a[k], b[k]) for k in a)
+    capfd.readouterr()
+    # a small dump: no velocities; refused unless allowed, then zero with a warning
+    small = write(tmp_path, "small", phantom_dump(blocks=[particle_block(rng, 80, vel=False), sink],
+                                                  ident=b"ST:Phantom small"))
+    a = same(small, capfd)
+    assert a[0] == "error" and "allow_missing_velocity" in a[1]
+    x = io.load_snapshot(small, allow_missing_velocity=True)
+    ours = capfd.readouterr().err
+    y = O.ref_load_snapshot(small, allow_missing_velocity=True)
+    ref = capfd.readouterr().err
+    assert all(np.array_equal(x[k], y[k]) for k in x) and ours == ref and "WARNING" in ours
+
+
+def test_phantom_binary_errors(tmp_path, capfd):
+    rng = np.random.default_rng(7)
+    good = dict(blocks=[particle_block(rng, 60), (0, [])])
+    full = phantom_dump(**good)
+    bad = {
+        "ident": dict(good, ident=b"FX:not tagged"),
+        "no_hfact": dict(good, header={5: [("massoftype", 1e-6), ("udist", 1.0), ("umass", 1.0), ("utime", 1.0)]}),
+        "hdr_blocks": dict(good, hdr_nblocks=3),
+        "no_itype_several": dict(blocks=[particle_block(rng, 40, itype=False), (0, [])], npartoftype=(30, 10)),
+        "no_gas": dict(blocks=[(3, [("x", 7, [0, 1, 2]), ("y", 7, [0, 1, 2]), ("z", 7, [0, 1, 2]), ("vx", 7, [0] * 3),
+                                    ("vy", 7, [0] * 3), ("vz", 7, [0] * 3), ("h", 7, [0.0, -1.0, 0.0])]), (0, [])]),
+        "no_h": dict(blocks=[(2, [("x", 7, [0, 1]), ("y", 7, [0, 1]), ("z", 7, [0, 1])]), (0, [])]),
+    }
+    for name, kw in bad.items():
+        a = same(write(tmp_path, name, phantom_dump(**kw)), capfd, fmt="phantom_binary")
+        assert a[0] == "error", name
+    for cut in (3, 40, 150, 700, len(full) // 2, len(full) - 3):
+        a = same(write(tmp_path, f"cut{cut}", full[:cut]), capfd, fmt="phantom_binary")
+        assert a[0] == "error", cut
+    for poke in (len(full) - 2, 600, 1000):  # a record marker or a size field flipped
+        a = same(write(tmp_path, f"poke{poke}", phantom_dump(**good, poke=poke)), capfd, fmt="phantom_binary")
+    a = same(write(tmp_path, "plain_as_phantom", "1 2 3 4 5 6 7 8 9\n"), capfd, fmt="phantom_binary")
+    assert a[0] == "error" and "capture pattern" in a[1]
