@@ -171,3 +171,31 @@ def test_unmapped_empty_device_and_errors(hip_ctx):
     with pytest.raises(hip.KiwiError, match="non-finite"):
         hip_ctx.cartesian_map(bad, dom, 8)
     assert hip_ctx.cartesian_map(S, dom, 4)[5].unresolved == 0
+
+
+def test_exact_ties_duplicates_and_bucket_edges(hip_ctx):
+    """Sites on a lattice whose points sit exactly between cell centres (every centre equidistant from
+    several sites), duplicated sites, and sites on the bucket edges of the search grid: the chosen site
+    is the oracle's (the lowest index among exact ties) for every centre."""
+    from oracle import oracle as O
+    say("case: lattice-aligned sites with exact ties, duplicates, bucket-edge sites")
+    rng = np.random.default_rng(13)
+    dom = np.array([0.0, 1.0, 0.0, 1.0, 0.0, 1.0])
+    ax = np.arange(0, 9) / 8.0
+    lat = np.stack([v.ravel() for v in np.meshgrid(ax, ax, ax, indexing="ij")], 1)
+    for sites in (lat, lat[rng.permutation(len(lat))], np.vstack([lat[::3], lat[::3], lat[1::5]])):
+        n = len(sites)
+        S = site_rows(rng, np.ascontiguousarray(sites), np.arange(1, n + 1))
+        for res in (16, 24):
+            cid = hip_ctx.cartesian_map(S, dom, res)[4]
+            _, want, _, _ = O.cartesian_map(sites, np.arange(1, n + 1), np.zeros((n, 4)), dom, res)
+            assert np.array_equal(cid, want), (n, res)
+    # sites exactly on the search grid's bucket edges (one site a bucket: edges at k / cbrt(n))
+    n = 343
+    e = np.arange(7) / 7.0
+    edge = np.stack([v.ravel() for v in np.meshgrid(e, e, e, indexing="ij")], 1)
+    S = site_rows(rng, np.ascontiguousarray(edge), np.arange(1, n + 1))
+    for res in (20, 33):
+        cid = hip_ctx.cartesian_map(S, dom, res)[4]
+        _, want, _, _ = O.cartesian_map(edge, np.arange(1, n + 1), np.zeros((n, 4)), dom, res)
+        assert np.array_equal(cid, want), res
