@@ -101,7 +101,8 @@ typedef struct kiwi_stats {
     int64_t particle_blocks;  /* (particle, 8^3 cell block) work items */
     double v_mean[3];         /* mean velocity over all particles */
     double device_ms_prep;    /* binning, sorts, block order */
-    double device_ms_lines;   /* per-particle edge-line constants */
+    double device_ms_lines;   /* per-particle edge-line constants: the part not hidden behind the item sort
+                                 (they run on a second stream beside it) */
     double device_ms_gather;  /* cell integrals: lattice values and per-pair cell sums */
     double device_ms_accum;   /* accumulation of the per-pair sums into the grid */
     int64_t lattice_points[3]; /* lattice values evaluated, by radial region (inside h, 2h, outside) */

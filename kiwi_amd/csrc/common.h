@@ -85,6 +85,8 @@ struct kiwi_hip_ctx {
     int device = 0;
     int n_cu = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;      // second stream (default priority): line tables beside the item sort
+    hipEvent_t side_done = nullptr;  // recorded on side after its work; the main stream waits on it
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     std::string err;
     DevBuf parts, bbox, sizes, offsets, keys_a, keys_b, pos_a, pos_b, cells, slab_sorted, slab, cub_tmp, ranges, work,

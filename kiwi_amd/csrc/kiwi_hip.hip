@@ -334,8 +334,7 @@ struct PolicyGen {
 struct PolicyTab {
     using Real = double;
     static constexpr int NPL = 1;  // unused
-    static constexpr int NLN = 1;  // unused
-    static constexpr bool HAS_LINES = false;
+    static constexpr int NLN = 1;  // unused: the tabulated path has no line tables
     static constexpr bool SPHERE_SKIP = false;
     struct Ctx {
         double support_in_h;
@@ -994,12 +993,7 @@ hipError_t launch_cells(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pc
                         hipEvent_t ev_lines, hipEvent_t ev_cells, hipEvent_t ev_accum) {
     using R = typename P::Real;
     using Line = LineVals<R, P::NLN>;
-    if (n > 0 && P::HAS_LINES)
-        k_lines<P><<<resident_grid(c, k_lines<P>, 1, n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], bbox, loffs,
-                                                                          pr, part_prob, pctx,
-                                                                          static_cast<Line*>(lines), dst);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    hipError_t e;
     if ((e = hipEventRecord(ev_lines, s)) != hipSuccess) return e;
     if (npairs > 0)
         k_particles<P><<<resident_grid(c, k_particles<P>, c->tune.item_grab, npairs), THREADS, 0, s>>>(
@@ -1016,6 +1010,19 @@ hipError_t launch_cells(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pc
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return hipEventRecord(ev_accum, s);
+}
+
+// the per-particle line tables (k_lines) of a kernel path, on stream s
+template <class P>
+hipError_t launch_lines(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pctx, int64_t n, const double* const* dp,
+                        const int* bbox, const int64_t* loffs, const Prob* pr, const int* part_prob, void* lines,
+                        DevStats* dst) {
+    using Line = LineVals<typename P::Real, P::NLN>;
+    if (n > 0 && P::HAS_LINES)
+        k_lines<P><<<resident_grid(c, k_lines<P>, 1, n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], bbox, loffs,
+                                                                          pr, part_prob, pctx,
+                                                                          static_cast<Line*>(lines), dst);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -1228,6 +1235,26 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     int64_t* cells = static_cast<int64_t*>(c->cells.p);
     int64_t* slab_sorted = static_cast<int64_t*>(c->slab_sorted.p);
     int64_t* slab = static_cast<int64_t*>(c->slab.p);
+    // The line tables need only the boxes and line offsets, complete at the synchronisation above: they
+    // are computed on the second stream while the items are sorted and laid out on this one.  Buffers
+    // that grow from here on are freed with hipFree, which waits for both streams.
+    const bool side_lines = n > 0 && !tabulated;
+    if (side_lines) {
+        const GenTab* tabs = static_cast<const GenTab*>(c->tables.p) + K->kernel_id;
+        if (generic && fp64)
+            KIWI_HIP_CHECK(c, launch_lines<PolicyGen<double>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, dpr, part_prob,
+                                                              c->lines.p, dst));
+        else if (generic)
+            KIWI_HIP_CHECK(c, launch_lines<PolicyGen<float>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, dpr, part_prob,
+                                                             c->lines.p, dst));
+        else if (fp64)
+            KIWI_HIP_CHECK(c, launch_lines<PolicyM4<double>>(c, c->side, {sup}, n, dp, bbox, loffs, dpr, part_prob,
+                                                             c->lines.p, dst));
+        else
+            KIWI_HIP_CHECK(c, launch_lines<PolicyM4<float>>(c, c->side, {sup}, n, dp, bbox, loffs, dpr, part_prob,
+                                                            c->lines.p, dst));
+        KIWI_HIP_CHECK(c, hipEventRecord(c->side_done, c->side));
+    }
     if (n > 0) {
         k_emit<<<grid1d(n), THREADS, 0, s>>>(n, bbox, offs, dpr, part_prob, keys_a);
         KIWI_HIP_CHECK(c, hipGetLastError());
@@ -1304,6 +1331,7 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     }
     double* partial = static_cast<double*>(c->acc_partial.p);
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[1], s));
+    if (side_lines) KIWI_HIP_CHECK(c, hipStreamWaitEvent(s, c->side_done, 0));  // the line tables are in
     const int clamp = fp64 ? 1 : 0;
     const int nb = static_cast<int>(nblocks);
     if (tabulated) {
@@ -1412,6 +1440,11 @@ int kiwi_hip_create(kiwi_hip_ctx** out, int device_ordinal) {
         delete c;
         return KIWI_ERR_DEVICE;
     }
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->side_done, hipEventDisableTiming) != hipSuccess) {
+        kiwi_hip_destroy(c);
+        return KIWI_ERR_DEVICE;
+    }
     for (auto& e : c->ev) {
         if (hipEventCreate(&e) != hipSuccess) {
             kiwi_hip_destroy(c);
@@ -1434,11 +1467,14 @@ void kiwi_hip_destroy(kiwi_hip_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     c->for_each_buf([](DevBuf& b) {
         if (b.p) (void)hipFree(b.p);
     });
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->side_done) (void)hipEventDestroy(c->side_done);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
