@@ -14,17 +14,22 @@ The metric is cell-integrals/s: (particle, cell) pairs whose cell intersects the
 sphere, divided by the step time.  `value` is the mixed-precision rate; the FP64 rate, the speed ratio
 and the mixed-vs-FP64 relative total-mass error are reported in the same line.
 
-roofline: the cell-integral kernel (k_particles: lattice values and per-pair cell sums).  `achieved` is
-the reference algorithm's operations for the launch's work -- the pinned mean per bounding-box pair of
-kiwi_amd/flops_per_pair.json (an operation-counting run of the oracle's cubic cell loop, produced by
-oracle/flops_per_pair.py) times the bounding-box pairs of the launch -- divided by the kernel's average
-duration, measured with HIP events on the stream the kernel runs on.  The lattice restatement does far
-fewer operations than the reference algorithm, so this fraction exceeds 1; `executed` gives the same
-rate for the operations the kernel itself performs (per-unit counts of kiwi_amd/flops_per_unit.json,
-from kiwi_amd/csrc/flop_count.cpp, times the launch's work counters).  `traffic` is the HBM bytes per
-launch from rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/k_particles_m4_1M_256_traffic.json).
+roofline: the cell-integral kernel (k_particles: lattice values and per-pair cell sums), VALU-bound.
+`achieved` is the operations the kernel executes per launch -- the per-unit counts of
+kiwi_amd/flops_per_unit.json (kiwi_amd/csrc/flop_count.cpp instantiates the kernel's lattice math over
+an operation-counting scalar; +, -, x, /, sqrt and each transcendental count 1, an FMA 2) times the
+launch's work counters (lattice values by region, wall planes, cells) -- divided by the kernel's
+duration, measured with HIP events on the stream the kernel runs on; `frac` = achieved / the FP32
+vector peak (157.3 TFLOP/s).  `ref_equiv` is a different quantity, kept apart: the reference
+algorithm's operation count for the same work (the pinned mean per bounding-box pair of
+kiwi_amd/flops_per_pair.json, from oracle/flops_per_pair.py) over the same duration; the lattice
+restatement does ~1/80 of those operations, so it exceeds the peak and is no roofline fraction.
+`traffic` is the HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE passes at the launch shape
+named beside it (profiles/k_particles_m4_1M_256_traffic.json; a constant read from that file, not
+measured in this run).
 cpu_baseline: the CPU oracle (a bit-exact restatement of the reference's loop) on a bounded sample of
-the same workload (every 128th particle, same grid), on this host's cores.
+the same workload (every 128th particle, same grid), on this host's cores, with the oracle/reference
+time ratio measured on identical inputs on the build machine (oracle/cpu_calibration.json).
 """
 import argparse
 import json
@@ -49,6 +54,19 @@ def cell_kernel_flops(st, fpu):
     """Algorithmic FLOPs of one k_particles launch: lattice points by region, wall planes, cell stencils."""
     pts = sum(st.lattice_points[r] * fpu["point"][r] for r in range(3))
     return pts + st.plane_evals * fpu["plane"] + st.pairs_sphere * fpu["cell"]
+
+
+def calibration():
+    """The oracle/reference time ratio on the same sample at all cores, from oracle/cpu_calibration.json."""
+    path = os.path.join(ROOT, "oracle", "cpu_calibration.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        cal = json.load(f)
+    rows = [r for r in cal["rows"] if r["case"] == "lognormal/16"] or cal["rows"]
+    r = rows[0]
+    return {"oracle_over_reference": r["oracle_over_ref"], "case": r["case"], "threads": r["threads"],
+            "reference_pairs_sphere_per_s": r["ref_pairs_sphere_per_s"], "host": cal["host"]}
 
 
 def line_kernel_flops(st, fpu):
@@ -87,10 +105,15 @@ def cpu_baseline(parts, dom, res, hip, ctx):
     t0 = time.perf_counter()
     O.sph_to_grid(sample, dom, *res, threads=threads)
     dt = time.perf_counter() - t0
-    return {"value": gst.pairs_sphere / dt, "unit": "cell-integrals/s", "cores": threads, "kind": "port",
-            "sample": f"every {CPU_SAMPLE_EVERY}th particle ({len(sample['x'])} particles) of the same "
-                      f"workload onto the same {res[0]}^3 grid, FP64 oracle (bit-exact restatement of the "
-                      f"reference loop), {dt:.1f} s, {gst.pairs_sphere} pairs"}
+    out = {"value": gst.pairs_sphere / dt, "unit": "cell-integrals/s", "cores": threads, "kind": "port",
+           "sample": f"every {CPU_SAMPLE_EVERY}th particle ({len(sample['x'])} particles) of the same "
+                     f"workload onto the same {res[0]}^3 grid, FP64 oracle (bit-exact restatement of the "
+                     f"reference loop), {dt:.1f} s, {gst.pairs_sphere} pairs"}
+    cal = calibration()
+    if cal:
+        out["calibration"] = cal
+        out["reference_estimate"] = out["value"] * cal["oracle_over_reference"]
+    return out
 
 
 def main():
@@ -151,13 +174,15 @@ def main():
     fl32 = cell_kernel_flops(st32, fpu)
     fl64 = cell_kernel_flops(st64, fpu)
     alg = f_pair * st32.pairs_bbox  # reference-algorithm operations for the launch's work
-    ach = alg / (g32 * 1e-3) / 1e12
+    ref_equiv = alg / (g32 * 1e-3) / 1e12
     exe = fl32 / (g32 * 1e-3) / 1e12
-    traffic = None
+    traffic, traffic_shape = None, None
     tpath = os.path.join(ROOT, "profiles", "k_particles_m4_1M_256_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("mixed_bytes_per_launch")
+            tj = json.load(f)
+        traffic, traffic_shape = tj.get("mixed_bytes_per_launch"), tj.get("launch_shape")
+    lines_kernel_ms = st32.device_ms_lines_kernel
 
     result = {
         "metric": "cell-integrals/s, 1M part->256^3: mixed vs FP64 on GPU at <=1e-6 mass err",
@@ -171,21 +196,25 @@ def main():
                                     "mixed": {"item_grab": shape32[0], "cursors": shape32[1]},
                                     "autotuned": not args.no_autotune}},
         "fp64": {"value": pairs / (ms64 * 1e-3), "ms_per_step": ms64, "cells_ms": g64,
-                 "roofline_frac": f_pair * st64.pairs_bbox / (g64 * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"],
-                 "executed_frac": fl64 / (g64 * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"]},
+                 "roofline_frac": fl64 / (g64 * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"],
+                 "roofline_peak_tflops": PEAK_TFLOPS["fp64"],
+                 "ref_equiv_tflops": f_pair * st64.pairs_bbox / (g64 * 1e-3) / 1e12},
         "mixed_over_fp64": ms64 / ms32,
         "mixed_total_mass_rel_err": mass_err,
         "mixed_field_err_over_max": field_err, "mixed_mass_l1_rel_err": l1_err,
         "pairs_sphere": pairs, "pairs_bbox": st32.pairs_bbox, "particle_blocks": st32.particle_blocks,
         "cells_ms": g32, "prep_ms": st32.device_ms_prep,
-        "lines_ms": st32.device_ms_lines, "accum_ms": st32.device_ms_accum,
-        "lines_tflops": line_kernel_flops(st32, fpu) / max(st32.device_ms_lines * 1e-3, 1e-12) / 1e12,
+        "lines_exposed_ms": st32.device_ms_lines, "lines_kernel_ms": lines_kernel_ms,
+        "accum_ms": st32.device_ms_accum,
+        "lines_tflops": (line_kernel_flops(st32, fpu) / (lines_kernel_ms * 1e-3) / 1e12) if lines_kernel_ms > 0 else None,
         "work": {"lattice_points": list(st32.lattice_points), "line_evals": list(st32.line_evals),
                  "plane_evals": st32.plane_evals},
-        "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_TFLOPS["mixed"], "unit": "TFLOP/s",
-                     "frac": ach / PEAK_TFLOPS["mixed"], "traffic": traffic, "kernel": "k_particles<float>",
-                     "kernel_ms": g32, "flops_per_launch": alg, "flops_per_bbox_pair": f_pair,
-                     "executed": {"achieved": exe, "frac": exe / PEAK_TFLOPS["mixed"], "flops_per_launch": fl32}},
+        "roofline": {"bound": "valu", "achieved": exe, "peak": PEAK_TFLOPS["mixed"], "unit": "TFLOP/s",
+                     "frac": exe / PEAK_TFLOPS["mixed"], "traffic": traffic, "traffic_launch_shape": traffic_shape,
+                     "kernel": "k_particles<float>", "kernel_ms": g32, "flops_per_launch": fl32},
+        "ref_equiv": {"tflops": ref_equiv, "flops_per_launch": alg, "flops_per_bbox_pair": f_pair,
+                      "note": "reference-algorithm operations for the same work over the kernel time; not a "
+                              "roofline fraction"},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

@@ -19,8 +19,12 @@
  * Voronoi cells are in the order of the geometry the caller passes (the reference's c_loop_all order).
  *
  * Return codes: KIWI_OK (0) or a negative KIWI_ERR_*; kiwi_hip_last_error() then holds the message.
- * A context owns one HIP stream and its device scratch; it must not be used from two threads at once.
- * Calls are synchronous: outputs are complete when the call returns.
+ * A context owns two HIP streams (the main one, and a second one on which the sph-to-grid entry points
+ * compute the per-particle line tables beside the item sort; both at default priority) and its device
+ * scratch; it must not be used from two threads at once.
+ * Calls are synchronous: outputs are complete when the call returns.  Device inputs must be complete
+ * when the call is made: the context's streams do not wait for work the caller queued on other streams
+ * (a caller that produces inputs on its own stream synchronises that stream first).
  */
 #ifndef KIWI_HIP_H
 #define KIWI_HIP_H
@@ -31,7 +35,7 @@
 extern "C" {
 #endif
 
-#define KIWI_HIP_ABI_VERSION 8
+#define KIWI_HIP_ABI_VERSION 9
 
 #define KIWI_OK 0
 #define KIWI_ERR_INVALID_ARG (-1)
@@ -105,6 +109,7 @@ typedef struct kiwi_stats {
                                  (they run on a second stream beside it) */
     double device_ms_gather;  /* cell integrals: lattice values and per-pair cell sums */
     double device_ms_accum;   /* accumulation of the per-pair sums into the grid */
+    double device_ms_lines_kernel; /* the line-table kernel itself, timed on the stream it runs on */
     int64_t lattice_points[3]; /* lattice values evaluated, by radial region (inside h, 2h, outside) */
     int64_t line_evals[3];     /* edge-line constants with 0, 1 or 2 crossing evaluations */
     int64_t plane_evals;       /* wall-plane constants evaluated */

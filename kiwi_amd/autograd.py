@@ -4,7 +4,9 @@ SphToGrid.apply(m, x, y, z, h, vx, vy, vz, ctx, domain, res, precision) returns 
 (mass, px, py, pz) of kiwi_hip_sph_to_grid; its backward calls kiwi_hip_sph_to_grid_adjoint, which
 recomputes the kernel integrals (nothing per pair is stored between the two passes).  All tensors are
 float64 on the device.  PyTorch is plumbing here (device memory and the tape); the computation is the
-HIP library's.
+HIP library's.  The library's streams do not wait for torch's: kiwi_amd.hip synchronises torch's current
+stream before every call with device tensors (the zero-filled outputs, inputs an optimiser step has just
+written, the upstream gradients), and the calls are synchronous.
 """
 import torch
 

@@ -25,12 +25,26 @@ def _run(cmd):
         sys.stderr.write(err + "\n")
 
 
+SOURCES = ("kiwi_hip.hip", "kiwi_voronoi.hip", "kiwi_adjoint.hip", "kiwi_cartesian_map.hip", "kiwi_coverage.hip")
+OBJ_DIR = os.path.join(PKG_DIR, "_build")
+
+
+def _compile(src):
+    obj = os.path.join(OBJ_DIR, os.path.splitext(os.path.basename(src))[0] + ".o")
+    _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT_DIR, "include"), "-c", src, "-o", obj + ".tmp"])
+    os.replace(obj + ".tmp", obj)
+    return obj
+
+
 def build_lib():
-    srcs = [os.path.join(PKG_DIR, "csrc", f)
-            for f in ("kiwi_hip.hip", "kiwi_voronoi.hip", "kiwi_adjoint.hip", "kiwi_cartesian_map.hip",
-                      "kiwi_coverage.hip")]
+    """Every translation unit compiled on its own (in parallel), then linked into one shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    srcs = [os.path.join(PKG_DIR, "csrc", f) for f in SOURCES]
+    with ThreadPoolExecutor(max_workers=min(len(srcs), os.cpu_count() or 1)) as ex:
+        objs = list(ex.map(_compile, srcs))
     tmp = LIB_PATH + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT_DIR, "include"), *srcs, "-o", tmp])
+    _run([HIPCC, *HIP_FLAGS, "-shared", *objs, "-o", tmp])
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

@@ -88,6 +88,7 @@ struct kiwi_hip_ctx {
     hipStream_t side = nullptr;      // second stream (default priority): line tables beside the item sort
     hipEvent_t side_done = nullptr;  // recorded on side after its work; the main stream waits on it
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev_side[2] = {nullptr, nullptr};  // bracket the line-table kernel on the second stream
     std::string err;
     DevBuf parts, bbox, sizes, offsets, keys_a, keys_b, pos_a, pos_b, cells, slab_sorted, slab, cub_tmp, ranges, work,
         work_sorted, order_in, order, lines, ms, stats, outbuf, vsum, tables, m4table, probs, part_prob, acc_units,

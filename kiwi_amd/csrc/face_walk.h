@@ -135,6 +135,30 @@ struct RelVerts {
     }
 };
 
+// small 3-vectors for the walk's geometry
+template <typename R>
+struct Vec3 {
+    R x, y, z;
+};
+template <typename R>
+KIWI_FHD Vec3<R> vsub(const Vec3<R>& a, const Vec3<R>& b) {
+    return Vec3<R>{a.x - b.x, a.y - b.y, a.z - b.z};
+}
+template <typename R>
+KIWI_FHD Vec3<R> vcross(const Vec3<R>& a, const Vec3<R>& b) {
+    return Vec3<R>{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <typename R>
+KIWI_FHD R vdot(const Vec3<R>& a, const Vec3<R>& b) {
+    return (a.x * b.x + a.y * b.y) + a.z * b.z;
+}
+template <typename R, class V>
+KIWI_FHD Vec3<R> vertex(const V& vs, int j) {
+    Vec3<R> v;
+    vs.get(j, v.x, v.y, v.z);
+    return v;
+}
+
 // The face integral of the polygon of n vertices (accessor vs) for a particle at (x0, y0, z0) with
 // kernel support radius `support`; *reachable is set when the face's closest point lies within the
 // support.
@@ -153,10 +177,8 @@ KIWI_FHD R face_walk_t(R x0, R y0, R z0, R support, const V& vs, int n, Ev& ev, 
     constexpr bool NEWELL = !std::is_same<R, double>::value;
     *reachable = false;
     if (n < 3) return R(0);
-    R x1, y1, z1, x2s, y2s, z2s, x3s, y3s, z3s;
-    vs.get(0, x1, y1, z1);
-    vs.get(1, x2s, y2s, z2s);
-    vs.get(2, x3s, y3s, z3s);
+    const Vec3<R> P = {x0, y0, z0};
+    const Vec3<R> V0 = vertex<R>(vs, 0), V1 = vertex<R>(vs, 1), V2 = vertex<R>(vs, 2);
     R A, B, C, D;
     if (NEWELL) {
         R gx = R(0), gy = R(0), gz = R(0);
@@ -175,25 +197,26 @@ KIWI_FHD R face_walk_t(R x0, R y0, R z0, R support, const V& vs, int n, Ev& ev, 
         const R inv_n = R(1) / static_cast<R>(n);
         D = -(A * gx + B * gy + C * gz) * inv_n;
     } else {
-        // plane through the first three vertices (:495-506)
-        A = (y2s - y1) * (z3s - z1) - (y3s - y1) * (z2s - z1);
-        B = (z2s - z1) * (x3s - x1) - (z3s - z1) * (x2s - x1);
-        C = (x2s - x1) * (y3s - y1) - (x3s - x1) * (y2s - y1);
-        D = -A * x1 - B * y1 - C * z1;
+        // the plane of the first three vertices (:495-506): normal (V1 - V0) x (V2 - V0) through V0
+        const Vec3<R> N = vcross(vsub(V1, V0), vsub(V2, V0));
+        A = N.x;
+        B = N.y;
+        C = N.z;
+        D = -((N.x * V0.x + N.y * V0.y) + N.z * V0.z);
     }
     const R nrm = M::sqrt_(A * A + B * B + C * C);
     const R r0 = M::div(A * x0 + B * y0 + C * z0 + D, nrm);
     const R ar0 = fabs(r0);
     const R xp = x0 - M::div(r0 * A, nrm), yp = y0 - M::div(r0 * B, nrm), zp = z0 - M::div(r0 * C, nrm);
-    // winding sign reference: triple product about the particle (:525-529)
-    const R ax = x1 - x0, ay = y1 - y0, az = z1 - z0;
-    const R bx = x2s - x0, by = y2s - y0, bz = z2s - z0;
-    const R cx = x3s - x0, cy = y3s - y0, cz = z3s - z0;
-    const R s2 = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
+    // orientation of the first three vertices as seen from the particle: the triple product
+    // (V0 - P) . ((V1 - P) x (V2 - P)) (:525-529), against which every edge's winding is signed
+    const R s2 = vdot(vsub(V0, P), vcross(vsub(V1, P), vsub(V2, P)));
     ev.wall(ar0);
 
-    bool inside = true, have_sign = false;
-    R sign_ref = R(0), min_edge = R(INFINITY), total = R(0);
+    // the foot lies inside the polygon while every edge that is not (numerically) through it sees it on
+    // the same side as the first such edge
+    bool inside = true, side_known = false, first_positive = false;
+    R min_edge = R(INFINITY), total = R(0);
     for (int j = 0; j < n; ++j) {
         const int jn = (j == n - 1) ? 0 : j + 1;
         R x2, y2, z2, x3, y3, z3;
@@ -225,14 +248,17 @@ KIWI_FHD R face_walk_t(R x0, R y0, R z0, R support, const V& vs, int n, Ev& ev, 
         const R s1 = px * (uy * wz - uz * wy) + py * (uz * wx - ux * wz) + pz * (ux * wy - uy * wx);
         const R phi1 = (t2 > R(0)) ? M::atan2_pos(t2, R0) : R(0);
         const R phi2 = (t3 > R(0)) ? M::atan2_pos(t3, R0) : R(0);
-        // is the foot inside the polygon (:600-611)
-        const R side = cxx * A + cxy * B + cxz * C;
-        const R side_eps = R(1e-9) * nrm * r23 * fmax(fmax(fabs(tx), fabs(ty)), fmax(fabs(tz), R(1)));
-        if (fabs(side) > side_eps) {
-            if (!have_sign) {
-                sign_ref = side;
-                have_sign = true;
-            } else if ((side > R(0)) != (sign_ref > R(0))) {
+        // which side of the edge the foot is on: the edge-to-foot cross product against the face normal
+        // (:600-611), decided only beyond a relative tolerance of 1e-9 of |N| |edge| max(1, |foot - V_j|_inf)
+        const R orient = cxx * A + cxy * B + cxz * C;
+        const R tmax = fmax(fmax(fabs(tx), fabs(ty)), fmax(fabs(tz), R(1)));
+        const R tol = R(1e-9) * nrm * r23 * tmax;
+        if (fabs(orient) > tol) {
+            const bool pos = orient > R(0);
+            if (!side_known) {
+                first_positive = pos;
+                side_known = true;
+            } else if (pos != first_positive) {
                 inside = false;
             }
         }

@@ -1080,6 +1080,9 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
               int precision) {
     KIWI_HIP_CHECK(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
+    // a line-table pass of an earlier call that returned early (an error after its launch) reads buffers
+    // this call rewrites: the main stream waits for it first
+    KIWI_HIP_CHECK(c, hipStreamWaitEvent(s, c->side_done, 0));
     const double sup = kc.support_in_h;
     const bool generic = kc.generic;
 
@@ -1240,6 +1243,7 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     // that grow from here on are freed with hipFree, which waits for both streams.
     const bool side_lines = n > 0 && !tabulated;
     if (side_lines) {
+        KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[0], c->side));
         const GenTab* tabs = static_cast<const GenTab*>(c->tables.p) + K->kernel_id;
         if (generic && fp64)
             KIWI_HIP_CHECK(c, launch_lines<PolicyGen<double>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, dpr, part_prob,
@@ -1253,6 +1257,7 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
         else
             KIWI_HIP_CHECK(c, launch_lines<PolicyM4<float>>(c, c->side, {sup}, n, dp, bbox, loffs, dpr, part_prob,
                                                             c->lines.p, dst));
+        KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[1], c->side));
         KIWI_HIP_CHECK(c, hipEventRecord(c->side_done, c->side));
     }
     if (n > 0) {
@@ -1390,6 +1395,8 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     (void)hipEventElapsedTime(&ms_lines, c->ev[1], c->ev[2]);
     (void)hipEventElapsedTime(&ms_cells, c->ev[2], c->ev[3]);
     (void)hipEventElapsedTime(&ms_accum, c->ev[3], c->ev[4]);
+    float ms_lines_kernel = 0;
+    if (side_lines) (void)hipEventElapsedTime(&ms_lines_kernel, c->ev_side[0], c->ev_side[1]);
     for (int q = 0; q < nprob; ++q) {
         kiwi_stats* stats = hp[q].stats;
         if (!stats) continue;
@@ -1408,6 +1415,7 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
         stats->device_ms_lines = ms_lines;
         stats->device_ms_gather = ms_cells;
         stats->device_ms_accum = ms_accum;
+        stats->device_ms_lines_kernel = ms_lines_kernel;
         for (int r = 0; r < 3; ++r) {
             stats->lattice_points[r] = static_cast<int64_t>(d.pts[r]);
             stats->line_evals[r] = static_cast<int64_t>(d.lines[r]);
@@ -1445,6 +1453,12 @@ int kiwi_hip_create(kiwi_hip_ctx** out, int device_ordinal) {
         kiwi_hip_destroy(c);
         return KIWI_ERR_DEVICE;
     }
+    for (auto& e : c->ev_side) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            kiwi_hip_destroy(c);
+            return KIWI_ERR_DEVICE;
+        }
+    }
     for (auto& e : c->ev) {
         if (hipEventCreate(&e) != hipSuccess) {
             kiwi_hip_destroy(c);
@@ -1472,6 +1486,8 @@ void kiwi_hip_destroy(kiwi_hip_ctx* c) {
         if (b.p) (void)hipFree(b.p);
     });
     for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->ev_side)
         if (e) (void)hipEventDestroy(e);
     if (c->side_done) (void)hipEventDestroy(c->side_done);
     if (c->side) (void)hipStreamDestroy(c->side);

@@ -69,6 +69,7 @@ class Stats(ctypes.Structure):
     _fields_ = [("skipped_outside", _i64), ("n_negative", _i64), ("n_negative_warn", _i64), ("pairs_bbox", _i64),
                 ("pairs_sphere", _i64), ("particle_blocks", _i64), ("v_mean", _d * 3),
                 ("device_ms_prep", _d), ("device_ms_lines", _d), ("device_ms_gather", _d), ("device_ms_accum", _d),
+                ("device_ms_lines_kernel", _d),
                 ("lattice_points", _i64 * 3), ("line_evals", _i64 * 3),
                 ("plane_evals", _i64), ("n_warn_records", _i), ("warn", WarnRecord * 16)]
 
@@ -239,6 +240,23 @@ def _ptr(a, ctype=_pd, dtype="float64"):
     return ctypes.cast(ctypes.c_void_p(a.data_ptr()), ctype), int(a.is_cuda)
 
 
+def _device_inputs_ready(*objs):
+    """Waits for torch's current stream when any of objs (tensors, or dicts / sequences of them) is a
+    device tensor.  The library's streams do not order themselves after work torch has queued (a zero
+    fill of an output, an optimiser step that wrote an input), so that work must be complete first."""
+    stack = list(objs)
+    while stack:
+        o = stack.pop()
+        if isinstance(o, dict):
+            stack.extend(o.values())
+        elif isinstance(o, (list, tuple)):
+            stack.extend(o)
+        elif getattr(o, "is_cuda", False):
+            import torch
+            torch.cuda.current_stream(o.device).synchronize()
+            return
+
+
 def _particles(parts):
     cols = [parts[c] for c in ("x", "y", "z", "vx", "vy", "vz", "m", "h")]
     ptrs = [_ptr(c) for c in cols]
@@ -290,6 +308,7 @@ class Context:
         out: None (host numpy results) or a tuple of 4 device float64 tensors of nx*ny*nz.
         Returns (mass, px, py, pz, Stats)."""
         kernel = kernel or kernel_spec()
+        _device_inputs_ready(parts, out)
         P = _particles(parts)
         nx, ny, nz = (int(v) for v in res)
         G = Grid(*[float(v) for v in domain], nx, ny, nz)
@@ -306,6 +325,7 @@ class Context:
         return arrs[0], arrs[1], arrs[2], arrs[3], st
 
     def _batch(self, problems):
+        _device_inputs_ready([(q["parts"], q.get("out")) for q in problems])
         arr = (BatchProblem * max(len(problems), 1))()
         keep, results = [], []
         for i, q in enumerate(problems):
@@ -363,6 +383,7 @@ class Context:
         out: None (host numpy results) or a dict of 8 device float64 tensors keyed by GRAD_FIELDS.
         Returns (dict of gradients, AdjointStats)."""
         kernel = kernel or kernel_spec()
+        _device_inputs_ready(parts, grad_cells, out)
         P = _particles(parts)
         nx, ny, nz = (int(v) for v in res)
         G = Grid(*[float(v) for v in domain], nx, ny, nz)
@@ -389,6 +410,7 @@ class Context:
         vertex), face_nbr (int32, n_faces) and domain (6 floats), all host arrays or all device tensors.
         Returns (mass, px, py, pz, MapStats) with n_cells entries each."""
         kernel = kernel or kernel_spec()
+        _device_inputs_ready(parts, home, cells, out)
         P, hp, V, nc = self._map_inputs(parts, home, cells)
         if out is None:
             arrs = [np.zeros(max(nc, 0)) for _ in range(4)]
@@ -408,6 +430,7 @@ class Context:
         when out is given).  out: None (host results) or (m_recovered, coverage) device float64 tensors.
         Returns (m_recovered, coverage_fraction, MapStats)."""
         kernel = kernel or kernel_spec()
+        _device_inputs_ready(parts, home, cells, cell_mass, out)
         P, hp, V, nc = self._map_inputs(parts, home, cells)
         if out is None:
             arrs = [np.zeros(max(P.n, 0)) for _ in range(2)]
@@ -432,6 +455,7 @@ class Context:
         (float64, each site's mapping.dat row) and optionally mapped (int32, 0 = no row), all host arrays or
         all device tensors.  out: None (host results) or (density, vx, vy, vz, cell_id) device tensors of
         res^3.  Returns (density, vx, vy, vz, cell_id, CartesianMapStats)."""
+        _device_inputs_ready(sites, out)
         cols = [_ptr(sites[k]) for k in ("x", "y", "z")]
         idp = _ptr(sites["id"], _pi, "int32")
         vals = [_ptr(sites[k]) for k in ("density", "vx", "vy", "vz")]
@@ -464,6 +488,7 @@ class Context:
         """The coverage-grid buffer.  parts: particle columns (x, y, z and h are read), host arrays or
         device tensors.  out: None (host results sized to the lattice) or (x, y, z, id) device tensors of
         a common length (the capacity).  Returns (ids, xyz (kept, 3) or the out tensors, CoverageStats)."""
+        _device_inputs_ready(parts, out)
         cols = [_ptr(parts[k]) for k in ("x", "y", "z", "h")]
         if len({q[1] for q in cols}) != 1 or len({len(parts[k]) for k in ("x", "y", "z", "h")}) != 1:
             raise ValueError("x, y, z, h must be host arrays or device tensors of one length")

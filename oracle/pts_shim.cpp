@@ -3,7 +3,9 @@
 // (load_sph_snapshot and detect_snapshot_format, src/snapshot.cpp:495-522), linked with the reference's
 // own objects into oracle/_ref/libkiwi_pts.so.  Test infrastructure: it runs the reference function on
 // a plain-array point set so the oracle's restatement and the GPU buffer can be checked against it and
-// timed beside it; nothing here is shipped or run on a GPU.
+// timed beside it.  The library is git-ignored but travels with the working tree to the GPU machine,
+// where the coverage-buffer GPU tests load it on the host as their checker; libkiwi_hip never links or
+// calls it.
 #include <algorithm>
 #include <cstring>
 #include <exception>

@@ -8,7 +8,9 @@
 #                                       oracle/_ref/libkiwi_pts.so (the reference's library objects +
 #                                       our C shim over its coverage-grid buffer)
 #
-# The reference build stays on the build machine: no GPU test, smoke test or benchmark loads it.
+# The outputs are git-ignored but travel with the working tree to the GPU machine.  There the GPU tests
+# load libkiwi_voro.so (tessellation of their inputs) and libkiwi_pts.so (the reference's coverage
+# buffer as the checker) on the host; libkiwi_hip, smoke() and bench.py never load anything from here.
 
 REF      ?= /root/reference
 OUT      ?= oracle/_ref

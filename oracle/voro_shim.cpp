@@ -5,7 +5,9 @@
 // compute_cell_integral forms them, src/mapping.cpp:168-175) and neighbour cell indices, and finds each
 // particle's home cell with find_voronoi_cell (src/mapping.cpp:434-440); each cell's centroid gives
 // the Lloyd step (lloyd_step, src/point_set.cpp:275-306).  The Voronoi golden fixture and the Voronoi
-// test and measurement inputs are written from it; nothing here is shipped or run on a GPU.
+// test and measurement inputs are written from it.  Test infrastructure: the library is git-ignored but
+// travels with the working tree to the GPU machine, where the Voronoi, inverse-mapping and cartesian-map
+// GPU tests load it on the host to tessellate their inputs; libkiwi_hip never links or calls it.
 #include <algorithm>
 #include <cstring>
 #include <vector>

@@ -25,7 +25,8 @@ def say(msg):
 def oracle():
     from oracle import oracle as O
     if not os.path.exists(O.VORO_SO):
-        pytest.skip("voro++ shim not built (make -f oracle/ref.mk)")
+        pytest.fail(f"voro++ shim not built at {O.VORO_SO}: these tests tessellate their inputs with it "
+                    f"(make -f oracle/ref.mk where the reference sources exist)")
     return O
 
 

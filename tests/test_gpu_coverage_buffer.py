@@ -51,9 +51,9 @@ def test_edge_cases_match_oracle(hip_ctx):
         ids, xyz, st = hip_ctx.coverage_buffer(cols(p, h), dom, fr, fa)
         oid, oxyz = O.coverage_buffer(p[:, 0], p[:, 1], p[:, 2], h, dom, fr, fa)
         assert np.array_equal(ids, oid) and np.array_equal(xyz, oxyz)
-        if os.path.exists(O.PTS_SO):
-            rid, rxyz = O.ref_coverage_buffer(p[:, 0], p[:, 1], p[:, 2], h, dom, fr, fa)
-            assert np.array_equal(ids, rid) and np.array_equal(xyz, rxyz)
+        assert os.path.exists(O.PTS_SO), f"the reference's coverage buffer is not built at {O.PTS_SO}"
+        rid, rxyz = O.ref_coverage_buffer(p[:, 0], p[:, 1], p[:, 2], h, dom, fr, fa)
+        assert np.array_equal(ids, rid) and np.array_equal(xyz, rxyz)
 
 
 def test_infinite_h_and_tiny_sets(hip_ctx):

@@ -44,7 +44,8 @@ def say(msg):
 def oracle():
     from oracle import oracle as O
     if not os.path.exists(O.VORO_SO):
-        pytest.skip(f"voro++ shim not built at {O.VORO_SO} (python -c 'import __graft_entry__ as g; g.build()')")
+        pytest.fail(f"voro++ shim not built at {O.VORO_SO}: these tests tessellate their inputs with it "
+                    f"(python -c 'import __graft_entry__ as g; g.build()' where the reference sources exist)")
     return O
 
 
