@@ -310,6 +310,14 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* ctx, const kiwi_particles* particles, con
                          const kiwi_voronoi_cells* cells, const kiwi_kernel* kernel, int precision,
                          const kiwi_cell_sums* out, kiwi_map_stats* stats);
 
+/* The Lloyd step (lloyd_step, src/point_set.cpp:275-302, which takes voronoicell_base::centroid of every
+ * cell): each cell's centroid, the next position of its site, and its volume, from the flattened
+ * tessellation.  site_xyz: 3 * n_cells doubles (x y z of cell c at 3c), the sites the tessellation was
+ * built from; centroid_xyz: 3 * n_cells doubles out; volume: n_cells doubles out, or NULL.  All arrays live
+ * where cells->on_device says.  A cell of zero volume keeps its site.  Malformed offsets: KIWI_ERR_INVALID_ARG. */
+int kiwi_hip_voronoi_centroids(kiwi_hip_ctx* ctx, const kiwi_voronoi_cells* cells, const double* site_xyz,
+                               double* centroid_xyz, double* volume);
+
 /* Per-particle rows of the inverse-mapping test. */
 typedef struct kiwi_inverse_rows {
     double* m_recovered;       /* n: sum over visited cells with M_c > 0 of (Msum m / M_c) M_c */

@@ -25,7 +25,8 @@ def _run(cmd):
         sys.stderr.write(err + "\n")
 
 
-SOURCES = ("kiwi_hip.hip", "kiwi_voronoi.hip", "kiwi_adjoint.hip", "kiwi_cartesian_map.hip", "kiwi_coverage.hip")
+SOURCES = ("kiwi_hip.hip", "kiwi_voronoi.hip", "kiwi_adjoint.hip", "kiwi_cartesian_map.hip", "kiwi_coverage.hip",
+           "kiwi_lloyd.hip")
 OBJ_DIR = os.path.join(PKG_DIR, "_build")
 
 
@@ -52,7 +53,7 @@ def build_lib():
 def build_io():
     tmp = IO_LIB_PATH + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared", "-pthread", "-I", os.path.join(ROOT_DIR, "include"),
-          os.path.join(PKG_DIR, "csrc", "kiwi_io.cpp"), "-o", tmp])
+          os.path.join(PKG_DIR, "csrc", "kiwi_io.cpp"), os.path.join(PKG_DIR, "csrc", "kiwi_config.cpp"), "-o", tmp])
     os.replace(tmp, IO_LIB_PATH)
     return IO_LIB_PATH
 

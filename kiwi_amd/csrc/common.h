@@ -76,9 +76,29 @@ inline int grid1d(int64_t n) { return static_cast<int>((n + THREADS - 1) / THREA
 
 }  // namespace
 
+// A fixed device area that buffers are carved from in place of hipMalloc (the allocation-free entry
+// point): regions are handed out in order and never freed; a call starts again from offset 0.
+struct ScratchArena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+};
+
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    ScratchArena* arena = nullptr;  // carve from this area instead of allocating
+};
+
+// The device buffers of one sph-to-grid pipeline run: the context's own, or shadows carved from the
+// allocation-free scratch area.
+#define KIWI_PIPE_BUFS(X)                                                                                      \
+    X(parts) X(bbox) X(sizes) X(offsets) X(keys_a) X(keys_b) X(pos_a) X(pos_b) X(cells) X(slab_sorted) X(slab) \
+    X(cub_tmp) X(ranges) X(work) X(work_sorted) X(order_in) X(order) X(lines) X(ms) X(stats) X(outbuf) X(vsum)  \
+    X(probs) X(part_prob) X(acc_units) X(acc_partial)
+struct PipeBufs {
+#define KIWI_PB_MEMBER(n) DevBuf* n;
+    KIWI_PIPE_BUFS(KIWI_PB_MEMBER)
+#undef KIWI_PB_MEMBER
 };
 
 struct kiwi_hip_ctx {
@@ -96,6 +116,16 @@ struct kiwi_hip_ctx {
     bool m4table_ready = false;
     kiwi_tuning tune{16, 8};  // the cell kernel's launch shape (kiwi_hip_set_tuning)
     DevBuf vor[16];  // the Voronoi map's buffers
+    // the allocation-free entry point: its arena over the device's scratch area and shadow buffers
+    ScratchArena noalloc_arena;
+    DevBuf noalloc_bufs[32];
+    PipeBufs own_bufs() {
+        PipeBufs b;
+#define KIWI_PB_OWN(n) b.n = &n;
+        KIWI_PIPE_BUFS(KIWI_PB_OWN)
+#undef KIWI_PB_OWN
+        return b;
+    }
     template <class F>
     void for_each_buf(F f) {
         for (DevBuf* b : {&parts, &bbox, &sizes, &offsets, &keys_a, &keys_b, &pos_a, &pos_b, &cells, &slab_sorted,
@@ -126,6 +156,15 @@ int hip_fail(kiwi_hip_ctx* c, hipError_t e, const char* what) {
 
 hipError_t reserve(DevBuf& b, size_t bytes) {
     if (bytes <= b.cap) return hipSuccess;
+    if (b.arena) {  // a new region of the fixed area (the old one is not reused within the call)
+        ScratchArena& a = *b.arena;
+        const size_t off = (a.used + 255) & ~static_cast<size_t>(255);
+        if (off > a.cap || bytes > a.cap - off) return hipErrorOutOfMemory;
+        b.p = a.base + off;
+        b.cap = bytes;
+        a.used = off + bytes;
+        return hipSuccess;
+    }
     if (b.p) {
         hipError_t e = hipFree(b.p);
         if (e != hipSuccess) return e;

@@ -77,7 +77,9 @@ int main() {
             int k = m4_region(r2);
             CF csum = CF(0.1) + CF(0.2) + CF(0.3);
             CF dsum = (k == 1) ? CF(0) : CF(0.1) + CF(0.1) + CF(0.1) + CF(0.1) + CF(0.1) + CF(0.1);
-            CF W = m4_lattice_free<CF>(ax, ay, az, rr, k) * M4Consts<CF>::inv_pi + CF(0.5) * csum + dsum;
+            // the kernel's free part (PolicyM4::free_part): a constant outside the support
+            CF f = (k == 3) ? CF(-0.25) : m4_lattice_free<CF>(ax, ay, az, rr, k) * M4Consts<CF>::inv_pi;
+            CF W = f + CF(0.5) * csum + dsum;
             (void)W;
         });
     }

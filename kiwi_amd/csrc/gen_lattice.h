@@ -208,13 +208,15 @@ KIWI_GHD R gen_pyramid_nonphi(const GenRegionT<R>& G, R da, R db, R dc, R r, R T
 
 // The phi-free part of the lattice value at absolute offsets (ax, ay, az) > 0, radius r (units of H)
 // in region k: the 6 ordered pyramids (wall a, edge normal b), which share T_b = atan(da dc / (db r)),
-// atanh(d_c / r) and seven reciprocals.
+// atanh(d_c / r) and six reciprocals.
 template <typename R>
 KIWI_GHD R gen_lattice_free(const GenRegionT<R>& G, R ax, R ay, R az, R r) {
     using M = KMath<R>;
-    const R ix = M::rcp(ax), iy = M::rcp(ay), iz = M::rcp(az), ir = M::rcp(r);
+    const R ix = M::rcp(ax), iy = M::rcp(ay), ir = M::rcp(r);
     const R ixy = M::rcp(ax * ax + ay * ay), ixz = M::rcp(ax * ax + az * az), iyz = M::rcp(ay * ay + az * az);
-    const R T0 = M::atan_pos(ay * az * ix * ir), T1 = M::atan_pos(ax * az * iy * ir), T2 = M::atan_pos(ax * ay * iz * ir);
+    // T0 + T1 + T2 = pi/2 for positive offsets (m4_lattice.h): one arctangent fewer per point
+    const R T0 = M::atan_pos(ay * az * ix * ir), T1 = M::atan_pos(ax * az * iy * ir);
+    const R T2 = M4Consts<R>::half_pi - T0 - T1;
     const R K0 = R(0.5) * M::log_((r + ax) * (r + ax) * iyz);
     const R K1 = R(0.5) * M::log_((r + ay) * (r + ay) * ixz);
     const R K2 = R(0.5) * M::log_((r + az) * (r + az) * ixy);

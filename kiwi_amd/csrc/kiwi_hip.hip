@@ -287,6 +287,7 @@ struct PolicyM4 {
     static __device__ __forceinline__ int line_kind(double l2) { return l2 < 1.0 ? 2 : (l2 < 4.0 ? 1 : 0); }
     // the lattice value without its plane and line terms, and the weight of the plane sum
     static __device__ __forceinline__ R free_part(const Ctx&, R ax, R ay, R az, R r, int k) {
+        if (k == 3) return R(-0.25);  // -(pi/4) / pi: outside the support the free part is constant
         return m4_lattice_free<R>(ax, ay, az, r, k) * M4Consts<R>::inv_pi;
     }
     static __device__ __forceinline__ R plane_weight() { return R(0.5); }
@@ -536,7 +537,10 @@ __global__ __launch_bounds__(THREADS) void k_particles(
     const int lane = threadIdx.x & 63;
     WaveLds<R, P::NPL>& L = WL[wave];
     const R S2 = P::support2();
-    unsigned long long n_neg = 0, n_warn = 0, n_sphere = 0, c_planes = 0;
+    // per-lane counters (clamped negatives, rare) and wavefront totals (counted from ballots; lane 0 takes
+    // part in every pass of the loops that count them, so lane 0's copies are the wave's totals)
+    unsigned long long n_neg = 0, n_warn = 0;
+    unsigned long long n_sphere = 0, c_planes = 0;
     unsigned long long c_pts[3] = {0, 0, 0};
     const unsigned long long lane_lt = (1ull << lane) - 1ull;
     // counters go to the problem's statistics when the wavefront moves on to another problem
@@ -545,9 +549,6 @@ __global__ __launch_bounds__(THREADS) void k_particles(
         for (int off = 32; off > 0; off >>= 1) {
             n_neg += __shfl_down(n_neg, off);
             n_warn += __shfl_down(n_warn, off);
-            n_sphere += __shfl_down(n_sphere, off);
-            c_planes += __shfl_down(c_planes, off);
-            for (int r = 0; r < 3; ++r) c_pts[r] += __shfl_down(c_pts[r], off);
         }
         if (lane == 0 && cur_q >= 0) {
             DevStats* sq = st + cur_q;
@@ -625,8 +626,8 @@ __global__ __launch_bounds__(THREADS) void k_particles(
     #pragma unroll
                     for (int q = 0; q < P::NPL; ++q) L.pc[q][lane] = pv[q];
                     L.cg2[lane] = gc * gc;
-                    ++c_planes;
                 }
+                c_planes += static_cast<unsigned>(nx + ny + nz);
                 wave_sync();
 
                 // ---- classify the lattice points: zero planes, not needed, inside the support, outside ----
@@ -670,7 +671,10 @@ __global__ __launch_bounds__(THREADS) void k_particles(
                     const R r2 = ax * ax + ay * ay + az * az;
                     const R r = KMath<R>::sqrt_(r2);
                     const int reg = P::region(ctx, r2);
-                    ++c_pts[min(reg, 3) - 1];
+                    // lane 0 takes part in every pass of this loop, so its counters hold the wave's totals
+                    c_pts[0] += __popcll(__ballot(reg == 1));
+                    c_pts[1] += __popcll(__ballot(reg == 2));
+                    c_pts[2] += __popcll(__ballot(reg >= 3));
                     const R csum = L.pc[reg - 1][i] + L.pc[reg - 1][nx + j] + L.pc[reg - 1][nx + ny + k];
                     R dsum = R(0);
                     const int slot = P::line_slot(reg);
@@ -693,8 +697,9 @@ __global__ __launch_bounds__(THREADS) void k_particles(
                 for (int u = lane; u < ncells; u += 64, wc.next(my, mz)) {
                     const int i = wc.i, j = wc.j, k = wc.k;
                     double Ms = 0.0;
-                    if (!P::SPHERE_SKIP || L.cg2[i] + L.cg2[nx + j] + L.cg2[nx + ny + k] < S2) {
-                        ++n_sphere;
+                    const bool in_sphere = !P::SPHERE_SKIP || L.cg2[i] + L.cg2[nx + j] + L.cg2[nx + ny + k] < S2;
+                    n_sphere += __popcll(__ballot(in_sphere));  // wave total, held by lane 0 as above
+                    if (in_sphere) {
                         const R* U0 = &L.U[(i * ny + j) * nz + k];
                         Ms = static_cast<double>(U0[nyz]) + static_cast<double>(U0[nz]) + static_cast<double>(U0[1]) +
                              static_cast<double>(U0[nyz + nz + 1]) - static_cast<double>(U0[0]) -
@@ -785,6 +790,7 @@ struct AccLds {
     long long off[THREADS];  // item slab offsets
     uint32_t box[THREADS];   // block-local origin and extents, 4 bits each: ox oy oz mx my mz
     double m[THREADS], v[3][THREADS];
+    uint16_t list[WAVES][THREADS];  // per wavefront: the staged items whose x-range meets its slices
 };
 
 template <typename R>
@@ -816,6 +822,9 @@ __global__ __launch_bounds__(THREADS) void k_accum(
     for (int s = 0; s < ACC_CPT; ++s) acc[s][0] = acc[s][1] = acc[s][2] = acc[s][3] = 0.0;
 
     const int q0 = blk_begin[b] + un.chunk * ACC_CHUNK, q1 = min(blk_end[b], q0 + ACC_CHUNK);
+    const int lane = tid & 63;
+    const unsigned long long lane_lt = (1ull << lane) - 1ull;
+    uint16_t* const wlist = A.list[wave];
     for (int base = q0; base < q1; base += THREADS) {
         const int cnt = min(THREADS, q1 - base);
         __syncthreads();
@@ -833,22 +842,38 @@ __global__ __launch_bounds__(THREADS) void k_accum(
             A.v[2][tid] = vz[p];
         }
         __syncthreads();
-        for (int g0 = 0; g0 < cnt; g0 += ACC_GROUP) {
+        // the wavefront's own list: staged items whose x-range meets its slices [cx0, cx0 + ACC_CPT), in
+        // list order (the others add nothing to its cells)
+        int nw = 0;
+        for (int c0 = 0; c0 < cnt; c0 += 64) {
+            const int t = c0 + lane;
+            bool ov = false;
+            if (t < cnt) {
+                const uint32_t bq = A.box[t];
+                const int ox = bq & 15, mx = (bq >> 12) & 15;
+                ov = ox < cx0 + ACC_CPT && ox + mx > cx0;
+            }
+            const unsigned long long bal = __ballot(ov);
+            if (ov) wlist[nw + __popcll(bal & lane_lt)] = static_cast<uint16_t>(t);
+            nw += __popcll(bal);
+        }
+        wave_sync();
+        for (int g0 = 0; g0 < nw; g0 += ACC_GROUP) {
             // issue the loads of a group of items first (independent loads in flight) ...
             // Every load is issued unconditionally from an in-bounds address (coordinates outside the
             // item are clamped into it) and masked afterwards: no branch separates a load from its use,
-            // so the group's loads stay in flight together.
+            // so the group's loads stay in flight together.  An item's slab is (x my + y) mz + z.
             R raw[ACC_GROUP][ACC_CPT];
             bool ok[ACC_GROUP][ACC_CPT];
 #pragma unroll
             for (int u = 0; u < ACC_GROUP; ++u) {
-                const int q = min(g0 + u, cnt - 1);
+                const int q = wave_uniform(wlist[min(g0 + u, nw - 1)]);
                 // item geometry is the same for every lane: scalar registers
                 const uint32_t bq = wave_uniform(A.box[q]);
                 const int ox = bq & 15, oy = (bq >> 4) & 15, oz = (bq >> 8) & 15;
                 const int mx = (bq >> 12) & 15, my = (bq >> 16) & 15, mz = (bq >> 20) & 15;
                 const int ly = cy - oy, lz = cz - oz;
-                const bool in_yz = (g0 + u < cnt) && ly >= 0 && ly < my && lz >= 0 && lz < mz;
+                const bool in_yz = (g0 + u < nw) && ly >= 0 && ly < my && lz >= 0 && lz < mz;
                 const R* row = ms + A.off[q] + (min(max(ly, 0), my - 1) * mz + min(max(lz, 0), mz - 1));
 #pragma unroll
                 for (int s = 0; s < ACC_CPT; ++s) {
@@ -860,8 +885,8 @@ __global__ __launch_bounds__(THREADS) void k_accum(
             // ... then add them in list order
 #pragma unroll
             for (int u = 0; u < ACC_GROUP; ++u) {
-                const int q = g0 + u;
-                if (q >= cnt) break;
+                if (g0 + u >= nw) break;
+                const int q = wave_uniform(wlist[g0 + u]);
                 const double mm = A.m[q], v0 = A.v[0][q], v1 = A.v[1][q], v2 = A.v[2][q];
 #pragma unroll
                 for (int s = 0; s < ACC_CPT; ++s) {
@@ -1077,7 +1102,7 @@ bool same_path(const kiwi_batch_problem& a, const KernelChoice& ka, const kiwi_b
 
 // one pipeline run over a group of problems with the same kernel path (a single call is a group of one)
 int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kernel* K, const KernelChoice& kc,
-              int precision) {
+              int precision, const PipeBufs& B) {
     KIWI_HIP_CHECK(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
     // a line-table pass of an earlier call that returned early (an error after its launch) reads buffers
@@ -1122,8 +1147,8 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
         const double* src[8] = {P->x, P->y, P->z, P->vx, P->vy, P->vz, P->m, P->h};
         for (int k = 0; k < 8; ++k) dp[k] = src[k];
     } else {
-        KIWI_HIP_CHECK(c, reserve(c->parts, 8 * sizeof(double) * std::max<int64_t>(n, 1)));
-        for (int k = 0; k < 8; ++k) dp[k] = static_cast<double*>(c->parts.p) + k * n;
+        KIWI_HIP_CHECK(c, reserve((*B.parts), 8 * sizeof(double) * std::max<int64_t>(n, 1)));
+        for (int k = 0; k < 8; ++k) dp[k] = static_cast<double*>((*B.parts).p) + k * n;
         for (int q = 0; q < nprob; ++q) {
             const kiwi_particles* P = hp[q].P;
             const double* src[8] = {P->x, P->y, P->z, P->vx, P->vy, P->vz, P->m, P->h};
@@ -1135,7 +1160,7 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
         }
     }
     // outputs: the caller's device arrays, or staging for host arrays
-    KIWI_HIP_CHECK(c, reserve(c->outbuf, 4 * sizeof(double) * std::max<int64_t>(host_cells, 1)));
+    KIWI_HIP_CHECK(c, reserve((*B.outbuf), 4 * sizeof(double) * std::max<int64_t>(host_cells, 1)));
     {
         int64_t o = 0;
         for (int q = 0; q < nprob; ++q) {
@@ -1147,46 +1172,50 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
                 probs[q].out[2] = out->py;
                 probs[q].out[3] = out->pz;
             } else {
-                for (int k = 0; k < 4; ++k) probs[q].out[k] = static_cast<double*>(c->outbuf.p) + 4 * o + k * nc;
+                for (int k = 0; k < 4; ++k) probs[q].out[k] = static_cast<double*>((*B.outbuf).p) + 4 * o + k * nc;
                 o += nc;
             }
         }
     }
-    KIWI_HIP_CHECK(c, reserve(c->probs, sizeof(Prob) * nprob));
-    Prob* dpr = static_cast<Prob*>(c->probs.p);
+    KIWI_HIP_CHECK(c, reserve((*B.probs), sizeof(Prob) * nprob));
+    Prob* dpr = static_cast<Prob*>((*B.probs).p);
     KIWI_HIP_CHECK(c, hipMemcpyAsync(dpr, probs.data(), sizeof(Prob) * nprob, hipMemcpyHostToDevice, s));
-    KIWI_HIP_CHECK(c, reserve(c->part_prob, sizeof(int) * std::max<int64_t>(n, 1)));
-    int* part_prob = static_cast<int*>(c->part_prob.p);
+    KIWI_HIP_CHECK(c, reserve((*B.part_prob), sizeof(int) * std::max<int64_t>(n, 1)));
+    int* part_prob = static_cast<int*>((*B.part_prob).p);
 
     // per-particle sizes (items, lines) and their exclusive scans, (n + 1) each
     const int64_t n1 = n + 1;
     const size_t stats_bytes = (sizeof(DevStats) * nprob + 255) / 256 * 256;
-    KIWI_HIP_CHECK(c, reserve(c->stats, stats_bytes + CURSOR_BYTES));
-    DevStats* dst = static_cast<DevStats*>(c->stats.p);
-    unsigned long long* cursor = reinterpret_cast<unsigned long long*>(static_cast<char*>(c->stats.p) + stats_bytes);
-    KIWI_HIP_CHECK(c, reserve(c->bbox, 6 * sizeof(int) * std::max<int64_t>(n, 1)));
-    KIWI_HIP_CHECK(c, reserve(c->sizes, 2 * sizeof(int64_t) * n1));
-    KIWI_HIP_CHECK(c, reserve(c->offsets, 2 * sizeof(int64_t) * n1));
-    KIWI_HIP_CHECK(c, reserve(c->ranges, 2 * sizeof(int) * std::max<int64_t>(nblocks, 1)));
-    KIWI_HIP_CHECK(c, reserve(c->work, sizeof(unsigned long long) * std::max<int64_t>(nblocks, 1)));
-    KIWI_HIP_CHECK(c, reserve(c->work_sorted, sizeof(unsigned long long) * std::max<int64_t>(nblocks, 1)));
-    KIWI_HIP_CHECK(c, reserve(c->order_in, sizeof(int) * std::max<int64_t>(nblocks, 1)));
-    KIWI_HIP_CHECK(c, reserve(c->order, sizeof(int) * std::max<int64_t>(nblocks, 1)));
-    KIWI_HIP_CHECK(c, reserve(c->vsum, sizeof(double) * 3 * nprob * (VSUM_BLOCKS + 1)));
-    int* bbox = static_cast<int*>(c->bbox.p);
-    int64_t* sz = static_cast<int64_t*>(c->sizes.p);
-    int64_t* of = static_cast<int64_t*>(c->offsets.p);
+    KIWI_HIP_CHECK(c, reserve((*B.stats), stats_bytes + CURSOR_BYTES));
+    DevStats* dst = static_cast<DevStats*>((*B.stats).p);
+    unsigned long long* cursor = reinterpret_cast<unsigned long long*>(static_cast<char*>((*B.stats).p) + stats_bytes);
+    KIWI_HIP_CHECK(c, reserve((*B.bbox), 6 * sizeof(int) * std::max<int64_t>(n, 1)));
+    KIWI_HIP_CHECK(c, reserve((*B.sizes), 2 * sizeof(int64_t) * n1));
+    KIWI_HIP_CHECK(c, reserve((*B.offsets), 2 * sizeof(int64_t) * n1));
+    KIWI_HIP_CHECK(c, reserve((*B.ranges), 2 * sizeof(int) * std::max<int64_t>(nblocks, 1)));
+    KIWI_HIP_CHECK(c, reserve((*B.work), sizeof(unsigned long long) * std::max<int64_t>(nblocks, 1)));
+    KIWI_HIP_CHECK(c, reserve((*B.work_sorted), sizeof(unsigned long long) * std::max<int64_t>(nblocks, 1)));
+    KIWI_HIP_CHECK(c, reserve((*B.order_in), sizeof(int) * std::max<int64_t>(nblocks, 1)));
+    KIWI_HIP_CHECK(c, reserve((*B.order), sizeof(int) * std::max<int64_t>(nblocks, 1)));
+    KIWI_HIP_CHECK(c, reserve((*B.vsum), sizeof(double) * 3 * nprob * (VSUM_BLOCKS + 1)));
+    int* bbox = static_cast<int*>((*B.bbox).p);
+    int64_t* sz = static_cast<int64_t*>((*B.sizes).p);
+    int64_t* of = static_cast<int64_t*>((*B.offsets).p);
     int64_t *nblk = sz, *nlin = sz + n1;
     int64_t *offs = of, *loffs = of + n1;
-    int* blk_begin = static_cast<int*>(c->ranges.p);
+    int* blk_begin = static_cast<int*>((*B.ranges).p);
     int* blk_end = blk_begin + nblocks;
-    unsigned long long* work = static_cast<unsigned long long*>(c->work.p);
-    unsigned long long* work_sorted = static_cast<unsigned long long*>(c->work_sorted.p);
-    int* order_in = static_cast<int*>(c->order_in.p);
-    int* order = static_cast<int*>(c->order.p);
-    double* vsums = static_cast<double*>(c->vsum.p);
+    unsigned long long* work = static_cast<unsigned long long*>((*B.work).p);
+    unsigned long long* work_sorted = static_cast<unsigned long long*>((*B.work_sorted).p);
+    int* order_in = static_cast<int*>((*B.order_in).p);
+    int* order = static_cast<int*>((*B.order).p);
+    double* vsums = static_cast<double*>((*B.vsum).p);
 
-    KIWI_HIP_CHECK(c, hipMemsetAsync(dst, 0, stats_bytes + CURSOR_BYTES, s));
+    {  // counters and work cursors start at zero (a fill kernel: no memset node in a captured stream)
+        const int64_t words = static_cast<int64_t>((stats_bytes + CURSOR_BYTES) / sizeof(unsigned long long));
+        k_fill_u64<<<grid1d(words), THREADS, 0, s>>>(reinterpret_cast<unsigned long long*>(dst), words);
+        KIWI_HIP_CHECK(c, hipGetLastError());
+    }
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[0], s));
     if (n > 0) k_part_prob<<<grid1d(n), THREADS, 0, s>>>(n, dpr, nprob, part_prob);
     k_prep<<<grid1d(n1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], sup, dpr, part_prob, bbox, nblk, nlin, dst);
@@ -1194,10 +1223,10 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     {
         size_t tmp_bytes = 0;
         KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, nblk, offs, static_cast<int>(n1), s));
-        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, tmp_bytes));
+        KIWI_HIP_CHECK(c, reserve((*B.cub_tmp), tmp_bytes));
         for (int k = 0; k < 2; ++k) {
-            size_t tb = c->cub_tmp.cap;
-            KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, tb, sz + k * n1, of + k * n1,
+            size_t tb = (*B.cub_tmp).cap;
+            KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum((*B.cub_tmp).p, tb, sz + k * n1, of + k * n1,
                                                                static_cast<int>(n1), s));
         }
     }
@@ -1222,22 +1251,22 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
                               : generic ? (fp64 ? sizeof(LineVals<double, GEN_NREG>) : sizeof(LineVals<float, GEN_NREG>))
                                         : (fp64 ? sizeof(LineVals<double, 2>) : sizeof(LineVals<float, 2>));
     const int64_t np1 = npairs + 1;
-    KIWI_HIP_CHECK(c, reserve(c->lines, line_entry * static_cast<size_t>(std::max<int64_t>(nlines_total, 1))));
-    KIWI_HIP_CHECK(c, reserve(c->ms, rsz * static_cast<size_t>(std::max<unsigned long long>(pairs_bbox, 1))));
-    KIWI_HIP_CHECK(c, reserve(c->keys_a, sizeof(uint64_t) * np1));
-    KIWI_HIP_CHECK(c, reserve(c->keys_b, sizeof(uint64_t) * np1));
-    KIWI_HIP_CHECK(c, reserve(c->pos_a, sizeof(int) * np1));
-    KIWI_HIP_CHECK(c, reserve(c->pos_b, sizeof(int) * np1));
-    KIWI_HIP_CHECK(c, reserve(c->cells, sizeof(int64_t) * np1));
-    KIWI_HIP_CHECK(c, reserve(c->slab_sorted, sizeof(int64_t) * np1));
-    KIWI_HIP_CHECK(c, reserve(c->slab, sizeof(int64_t) * np1));
-    uint64_t* keys_a = static_cast<uint64_t*>(c->keys_a.p);  // particle order
-    uint64_t* keys_b = static_cast<uint64_t*>(c->keys_b.p);  // block order
-    int* pos_a = static_cast<int*>(c->pos_a.p);
-    int* pos_b = static_cast<int*>(c->pos_b.p);  // particle-order position of each block-order item
-    int64_t* cells = static_cast<int64_t*>(c->cells.p);
-    int64_t* slab_sorted = static_cast<int64_t*>(c->slab_sorted.p);
-    int64_t* slab = static_cast<int64_t*>(c->slab.p);
+    KIWI_HIP_CHECK(c, reserve((*B.lines), line_entry * static_cast<size_t>(std::max<int64_t>(nlines_total, 1))));
+    KIWI_HIP_CHECK(c, reserve((*B.ms), rsz * static_cast<size_t>(std::max<unsigned long long>(pairs_bbox, 1))));
+    KIWI_HIP_CHECK(c, reserve((*B.keys_a), sizeof(uint64_t) * np1));
+    KIWI_HIP_CHECK(c, reserve((*B.keys_b), sizeof(uint64_t) * np1));
+    KIWI_HIP_CHECK(c, reserve((*B.pos_a), sizeof(int) * np1));
+    KIWI_HIP_CHECK(c, reserve((*B.pos_b), sizeof(int) * np1));
+    KIWI_HIP_CHECK(c, reserve((*B.cells), sizeof(int64_t) * np1));
+    KIWI_HIP_CHECK(c, reserve((*B.slab_sorted), sizeof(int64_t) * np1));
+    KIWI_HIP_CHECK(c, reserve((*B.slab), sizeof(int64_t) * np1));
+    uint64_t* keys_a = static_cast<uint64_t*>((*B.keys_a).p);  // particle order
+    uint64_t* keys_b = static_cast<uint64_t*>((*B.keys_b).p);  // block order
+    int* pos_a = static_cast<int*>((*B.pos_a).p);
+    int* pos_b = static_cast<int*>((*B.pos_b).p);  // particle-order position of each block-order item
+    int64_t* cells = static_cast<int64_t*>((*B.cells).p);
+    int64_t* slab_sorted = static_cast<int64_t*>((*B.slab_sorted).p);
+    int64_t* slab = static_cast<int64_t*>((*B.slab).p);
     // The line tables need only the boxes and line offsets, complete at the synchronisation above: they
     // are computed on the second stream while the items are sorted and laid out on this one.  Buffers
     // that grow from here on are freed with hipFree, which waits for both streams.
@@ -1247,16 +1276,16 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
         const GenTab* tabs = static_cast<const GenTab*>(c->tables.p) + K->kernel_id;
         if (generic && fp64)
             KIWI_HIP_CHECK(c, launch_lines<PolicyGen<double>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, dpr, part_prob,
-                                                              c->lines.p, dst));
+                                                              (*B.lines).p, dst));
         else if (generic)
             KIWI_HIP_CHECK(c, launch_lines<PolicyGen<float>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, dpr, part_prob,
-                                                             c->lines.p, dst));
+                                                             (*B.lines).p, dst));
         else if (fp64)
             KIWI_HIP_CHECK(c, launch_lines<PolicyM4<double>>(c, c->side, {sup}, n, dp, bbox, loffs, dpr, part_prob,
-                                                             c->lines.p, dst));
+                                                             (*B.lines).p, dst));
         else
             KIWI_HIP_CHECK(c, launch_lines<PolicyM4<float>>(c, c->side, {sup}, n, dp, bbox, loffs, dpr, part_prob,
-                                                            c->lines.p, dst));
+                                                            (*B.lines).p, dst));
         KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[1], c->side));
         KIWI_HIP_CHECK(c, hipEventRecord(c->side_done, c->side));
     }
@@ -1271,18 +1300,18 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
         size_t sort_bytes = 0;
         KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys_a, keys_b, pos_a, pos_b,
                                                               static_cast<int>(npairs), 32, end_bit, s));
-        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, sort_bytes));
-        sort_bytes = c->cub_tmp.cap;
-        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairs(c->cub_tmp.p, sort_bytes, keys_a, keys_b, pos_a, pos_b,
+        KIWI_HIP_CHECK(c, reserve((*B.cub_tmp), sort_bytes));
+        sort_bytes = (*B.cub_tmp).cap;
+        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairs((*B.cub_tmp).p, sort_bytes, keys_a, keys_b, pos_a, pos_b,
                                                               static_cast<int>(npairs), 32, end_bit, s));
     }
     k_item_cells<<<grid1d(np1), THREADS, 0, s>>>(npairs, keys_b, bbox, dpr, part_prob, cells);
     {
         size_t tb = 0;
         KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cells, slab_sorted, static_cast<int>(np1), s));
-        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, tb));
-        tb = c->cub_tmp.cap;
-        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, tb, cells, slab_sorted, static_cast<int>(np1),
+        KIWI_HIP_CHECK(c, reserve((*B.cub_tmp), tb));
+        tb = (*B.cub_tmp).cap;
+        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum((*B.cub_tmp).p, tb, cells, slab_sorted, static_cast<int>(np1),
                                                            s));
     }
     if (nblocks > 0) {
@@ -1301,9 +1330,9 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
         size_t ob = 0;
         KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, ob, work, work_sorted, order_in, order,
                                                                         static_cast<int>(nblocks), 0, 64, s));
-        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, ob));
-        ob = c->cub_tmp.cap;
-        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairsDescending(c->cub_tmp.p, ob, work, work_sorted, order_in,
+        KIWI_HIP_CHECK(c, reserve((*B.cub_tmp), ob));
+        ob = (*B.cub_tmp).cap;
+        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairsDescending((*B.cub_tmp).p, ob, work, work_sorted, order_in,
                                                                         order, static_cast<int>(nblocks), 0, 64, s));
     }
     // accumulation units (blocks split into chunks of ACC_CHUNK items) in LPT order; the unit count is
@@ -1312,29 +1341,29 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     const int64_t max_split = 2 * (npairs / ACC_CHUNK) + 1;
     const int64_t max_units = nblocks + max_split;
     if (max_units > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "too many cell blocks in one call");
-    KIWI_HIP_CHECK(c, reserve(c->acc_units, sizeof(AccUnit) * max_units + 4 * sizeof(int) * (nblocks + 1) + 256));
-    AccUnit* units = static_cast<AccUnit*>(c->acc_units.p);
+    KIWI_HIP_CHECK(c, reserve((*B.acc_units), sizeof(AccUnit) * max_units + 4 * sizeof(int) * (nblocks + 1) + 256));
+    AccUnit* units = static_cast<AccUnit*>((*B.acc_units).p);
     int* nunits = reinterpret_cast<int*>(units + max_units);
     int* nsplit = nunits + (nblocks + 1);
     int* unit_off = nsplit + (nblocks + 1);
     int* split_off = unit_off + (nblocks + 1);
-    KIWI_HIP_CHECK(c, reserve(c->acc_partial, sizeof(double) * 4 * BLK_CELLS * max_split));
+    KIWI_HIP_CHECK(c, reserve((*B.acc_partial), sizeof(double) * 4 * BLK_CELLS * max_split));
     if (nblocks > 0) {
         k_acc_count<<<grid1d(nblocks + 1), THREADS, 0, s>>>(static_cast<int>(nblocks), order, blk_begin, blk_end,
                                                              nunits, nsplit);
         size_t tb = 0;
         KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, nunits, unit_off, static_cast<int>(nblocks + 1), s));
-        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, tb));
+        KIWI_HIP_CHECK(c, reserve((*B.cub_tmp), tb));
         for (int k = 0; k < 2; ++k) {
-            tb = c->cub_tmp.cap;
-            KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, tb, k ? nsplit : nunits,
+            tb = (*B.cub_tmp).cap;
+            KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum((*B.cub_tmp).p, tb, k ? nsplit : nunits,
                                                                k ? split_off : unit_off,
                                                                static_cast<int>(nblocks + 1), s));
         }
         k_acc_fill<<<grid1d(nblocks), THREADS, 0, s>>>(static_cast<int>(nblocks), unit_off, split_off, units);
         KIWI_HIP_CHECK(c, hipGetLastError());
     }
-    double* partial = static_cast<double*>(c->acc_partial.p);
+    double* partial = static_cast<double*>((*B.acc_partial).p);
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[1], s));
     if (side_lines) KIWI_HIP_CHECK(c, hipStreamWaitEvent(s, c->side_done, 0));  // the line tables are in
     const int clamp = fp64 ? 1 : 0;
@@ -1344,33 +1373,33 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
         const PolicyTab::Ctx pc{sup, static_cast<const double*>(c->m4table.p)};
         KIWI_HIP_CHECK(c, launch_cells<PolicyTab>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
                                                   slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob, part_prob,
-                                                  c->lines.p, c->ms.p, dst, cursor, clamp,
+                                                  (*B.lines).p, (*B.ms).p, dst, cursor, clamp,
                                                   units, unit_off, split_off, max_units, partial, c->ev[2], c->ev[3],
                                                   c->ev[4]));
     } else if (generic && fp64) {
         const PolicyGen<double>::Ctx pc{sup, static_cast<const GenTab*>(c->tables.p) + K->kernel_id};
         KIWI_HIP_CHECK(c, launch_cells<PolicyGen<double>>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
                                                           slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
-                                                          part_prob, c->lines.p, c->ms.p, dst, cursor, clamp,
+                                                          part_prob, (*B.lines).p, (*B.ms).p, dst, cursor, clamp,
                                                           units, unit_off, split_off, max_units, partial,
                                                           c->ev[2], c->ev[3], c->ev[4]));
     } else if (generic) {
         const PolicyGen<float>::Ctx pc{sup, static_cast<const GenTab*>(c->tables.p) + K->kernel_id};
         KIWI_HIP_CHECK(c, launch_cells<PolicyGen<float>>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
                                                          slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
-                                                         part_prob, c->lines.p, c->ms.p, dst, cursor, clamp,
+                                                         part_prob, (*B.lines).p, (*B.ms).p, dst, cursor, clamp,
                                                          units, unit_off, split_off, max_units, partial,
                                                          c->ev[2], c->ev[3], c->ev[4]));
     } else if (fp64) {
         KIWI_HIP_CHECK(c, launch_cells<PolicyM4<double>>(c, s, {sup}, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
                                                          slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
-                                                         part_prob, c->lines.p, c->ms.p, dst, cursor, clamp,
+                                                         part_prob, (*B.lines).p, (*B.ms).p, dst, cursor, clamp,
                                                          units, unit_off, split_off, max_units, partial, c->ev[2],
                                                          c->ev[3], c->ev[4]));
     } else {
         KIWI_HIP_CHECK(c, launch_cells<PolicyM4<float>>(c, s, {sup}, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
                                                         slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
-                                                        part_prob, c->lines.p, c->ms.p, dst, cursor, clamp,
+                                                        part_prob, (*B.lines).p, (*B.ms).p, dst, cursor, clamp,
                                                         units, unit_off, split_off, max_units, partial, c->ev[2],
                                                         c->ev[3], c->ev[4]));
     }
@@ -1509,7 +1538,7 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     KernelChoice kc;
     if (const int rc = resolve_kernel(c, K, &kc)) return rc;
     const HostProblem hp{P, G, out, stats};
-    return run_group(c, &hp, 1, K, kc, precision);
+    return run_group(c, &hp, 1, K, kc, precision, c->own_bufs());
 }
 
 int kiwi_hip_sph_to_grid_batch(kiwi_hip_ctx* c, int n_problems, const kiwi_batch_problem* problems) {
@@ -1546,7 +1575,7 @@ int kiwi_hip_sph_to_grid_batch(kiwi_hip_ctx* c, int n_problems, const kiwi_batch
             }
         }
         if (const int rc = run_group(c, group.data(), static_cast<int>(group.size()), &problems[i].kernel, kcs[i],
-                                     problems[i].precision))
+                                     problems[i].precision, c->own_bufs()))
             return rc;
     }
     return KIWI_OK;

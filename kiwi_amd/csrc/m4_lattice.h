@@ -254,24 +254,29 @@ KIWI_HD int m4_region(R r2) {
 // ax, ay, az > 0, radius r, in region k.  The full W is
 //     W = free / pi + 0.5 * sum_a c_k(d_a) + sum_{a != b} D_k(d_a, d_b)
 // because the phi terms of the pyramids (a,b,c) and (a,c,b) add up to c_k(d_a) (pi/2) / pi.
+//
+// The three angles add up to pi/2 for positive offsets: atan(yz/(xr)) + atan(xz/(yr)) + atan(xy/(zr)) =
+// pi/2 (the solid angle of the octant box's three far faces seen from the particle is that of the
+// whole octant, 4 pi / 8, less nothing: tan of the sum of two is the cotangent of the third).  So T_2 is
+// pi/2 - T_0 - T_1, and outside the support, where only -(T_0 + T_1 + T_2)/2 is left, the free part is
+// the constant -pi/4: such a point takes no transcendental at all.
 template <typename R>
 KIWI_HD R m4_lattice_free(R ax, R ay, R az, R r, int k) {
     using M = KMath<R>;
+    if (k == 3) return R(-0.25) * M4Consts<R>::pi;
     const R ayz = ay * az, axz = ax * az, axy = ax * ay;
     // T_b: b is the edge-normal axis, {a, c} the other two (symmetric in a <-> c)
-    R T0, T1, T2;
+    R T0, T1;
     if (std::is_same<R, double>::value) {
-        const R inv = M::rcp(ax * ayz * r);  // one division for the three arguments
+        const R inv = M::rcp(ax * ayz * r);  // one division for the two arguments
         T0 = M::atan_pos(ayz * ayz * inv);
         T1 = M::atan_pos(axz * axz * inv);
-        T2 = M::atan_pos(axy * axy * inv);
     } else {
         // single precision: one reciprocal per angle (atan2 of the ratio's two sides)
         T0 = M::atan2_pos(ayz, ax * r);
         T1 = M::atan2_pos(axz, ay * r);
-        T2 = M::atan2_pos(axy, az * r);
     }
-    if (k == 3) return R(-0.5) * (T0 + T1 + T2);
+    const R T2 = M4Consts<R>::half_pi - T0 - T1;
     // L_c: c is the along-edge axis, log((r + d_c)^2 / (r^2 - d_c^2)) with r^2 - d_c^2 formed exactly
     const R sx = ay * ay + az * az, sy = ax * ax + az * az, sz = ax * ax + ay * ay;
     // one reciprocal for the three denominators.  In single precision the product is kept above
@@ -315,7 +320,7 @@ KIWI_HD R m4_lattice_free(R ax, R ay, R az, R r, int k) {
     const R SX3 = sx3 + R(3) * pr + lx3;
     const R SX5 = sx5 + R(4) * pr * r2 + lx5;
     if (k == 1) return p * (R(1) - R(0.5) * r2) + SX5 * R(1.0 / 40.0);
-    return p * (R(2) + R(0.5) * r2) - R(0.25) * SX3 - SX5 * R(1.0 / 120.0) + (T0 + T1 + T2) * R(1.0 / 30.0);
+    return p * (R(2) + R(0.5) * r2) - R(0.25) * SX3 - SX5 * R(1.0 / 120.0) + M4Consts<R>::half_pi * R(1.0 / 30.0);
 }
 
 // The same sum taken pyramid by pyramid (m4_pyramid_free); kept as the readable form of the closed

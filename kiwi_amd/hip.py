@@ -34,7 +34,7 @@ EXPORTED = ["kiwi_hip_abi_version", "kiwi_hip_create", "kiwi_hip_destroy", "kiwi
             "kiwi_hip_stream", "kiwi_hip_sph_to_grid", "kiwi_hip_voronoi_map", "kiwi_hip_sph_to_grid_adjoint",
             "kiwi_hip_sph_to_grid_batch", "kiwi_hip_set_tuning", "kiwi_hip_get_tuning", "kiwi_hip_autotune",
             "kiwi_hip_inverse_mapping", "kiwi_hip_cartesian_map", "kiwi_hip_cartesian_lattice",
-            "kiwi_hip_coverage_buffer"]
+            "kiwi_hip_coverage_buffer", "kiwi_hip_voronoi_centroids"]
 
 
 class KiwiError(RuntimeError):
@@ -206,6 +206,8 @@ def load():
         L.kiwi_hip_autotune.restype = _i
         L.kiwi_hip_autotune.argtypes = [ctypes.c_void_p, _i, ctypes.POINTER(BatchProblem), ctypes.POINTER(Tuning),
                                         ctypes.POINTER(ctypes.c_double)]
+        L.kiwi_hip_voronoi_centroids.restype = _i
+        L.kiwi_hip_voronoi_centroids.argtypes = [ctypes.c_void_p, ctypes.POINTER(VoronoiCells), _pd, _pd, _pd]
         L.kiwi_hip_voronoi_map.restype = _i
         L.kiwi_hip_voronoi_map.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.c_void_p,
                                            ctypes.POINTER(VoronoiCells), ctypes.POINTER(Kernel), _i,
@@ -516,6 +518,33 @@ class Context:
             return arrs[3], tuple(arrs[:3]), st
         k = st.kept
         return arrs[3][:k].copy(), np.stack([a[:k] for a in arrs[:3]], 1), st
+
+    def voronoi_centroids(self, cells, sites, out=None):
+        """The Lloyd step: each cell's centroid (its site's next position) and volume.  cells: as
+        voronoi_map; sites: (n_cells, 3) float64, host array or device tensor like the cells.  out: None
+        (host results) or (centroids (n_cells, 3), volumes (n_cells,)) device tensors.
+        Returns (centroids, volumes)."""
+        _device_inputs_ready(cells, sites, out)
+        cfp, d0 = _ptr(cells["cell_face_ptr"], _pll, "int64")
+        fvp, d1 = _ptr(cells["face_vert_ptr"], _pll, "int64")
+        vx, d2 = _ptr(cells["vert_xyz"])
+        sp, d3 = _ptr(sites)
+        if len({d0, d1, d2, d3}) != 1:
+            raise ValueError("tessellation arrays and sites must all be host arrays or all device tensors")
+        nc = len(cells["cell_face_ptr"]) - 1
+        if int(np.prod(sites.shape)) != 3 * nc:
+            raise ValueError("sites must hold 3 coordinates per cell")
+        nf = len(cells["face_vert_ptr"]) - 1
+        nv = int(np.prod(cells["vert_xyz"].shape)) // 3
+        V = VoronoiCells(nc, nf, nv, cfp, fvp, vx, None, (_d * 6)(*[float(v) for v in cells["domain"]]), d0)
+        if out is None:
+            cen, vol = np.zeros((max(nc, 0), 3)), np.zeros(max(nc, 0))
+            cp, vp = cen.ctypes.data_as(_pd), vol.ctypes.data_as(_pd)
+        else:
+            cen, vol = out
+            cp, vp = _ptr(cen)[0], _ptr(vol)[0]
+        self._check(self.L.kiwi_hip_voronoi_centroids(self.h, ctypes.byref(V), sp, cp, vp))
+        return cen, vol
 
     def _map_inputs(self, parts, home, cells):
         P = _particles(parts)

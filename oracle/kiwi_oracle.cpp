@@ -22,6 +22,7 @@
 #include <cstring>
 #include <limits>
 #include <unordered_map>
+#include <unordered_set>
 #include <utility>
 #include <vector>
 
@@ -1056,11 +1057,22 @@ int oracle_inverse_mapping(long long n, const double* x, const double* y, const 
 // BFS above decides which (particle, cell) pairs are visited; each visited pair's cell integral is then
 // redone in binary128 over the cell's faces (vertices as given), clamped at 0 and accumulated in
 // binary128.  Used to arbitrate cells where two double-precision evaluations disagree.
+//
+// share_lower = 0: every cell is integrated over its own copy of each face.  share_lower = 1: a face
+// whose neighbour has the lower index, is visited by the same particle and holds exactly one face back
+// (the cell holding exactly one face towards it) takes the negated integral of the neighbour's copy --
+// the rule by which the GPU map integrates a shared face once.  share_lower = 2: the reference's face
+// cache (src/mapping.cpp:87-159) -- a face towards a neighbour the particle's BFS processed earlier takes
+// the negated integral of the neighbour's first face towards this cell, and a second face towards the
+// same neighbour repeats the first one's value.  voro++ computes each cell's vertices on its own, within
+// its plane tolerance (which scales with the domain's extent), so the two copies of a face can differ by
+// more than rounding; the map's result depends on which copy a cell is closed with.
 int oracle_voronoi_map_cells_q(long long n, const double* x, const double* y, const double* z, const double* vx,
                                const double* vy, const double* vz, const double* m, const double* h, const int* pid,
                                const int* home, long long ncells, const long long* cell_face_ptr,
                                const long long* face_vert_ptr, const double* face_xyz, const int* face_nbr,
-                               const OracleKernel* k, long long nlist, const long long* cells, double* out4) {
+                               const OracleKernel* k, long long nlist, const long long* cells, int share_lower,
+                               double* out4) {
     Integrator ki;
     if (!make_integrator(k, &ki) || ki.tabulated) return -1;
     std::vector<long long> slot(static_cast<size_t>(ncells), -1);
@@ -1069,11 +1081,37 @@ int oracle_voronoi_map_cells_q(long long n, const double* x, const double* y, co
         slot[static_cast<size_t>(cells[q])] = q;
     }
     std::vector<std::vector<long long>> visitors(static_cast<size_t>(nlist));  // particles per listed cell
+    // (particle, cell) pairs the BFS visits, with their processing rank within the particle's BFS
+    std::unordered_map<unsigned long long, long long> visited;
+    long long last_i = -1, rank = 0;
+    auto pair_key = [&](long long i, long long c) {
+        return static_cast<unsigned long long>(i) * static_cast<unsigned long long>(ncells) +
+               static_cast<unsigned long long>(c);
+    };
     voronoi_bfs(ki, n, x, y, z, h, pid, home, ncells, cell_face_ptr, face_vert_ptr, face_xyz, face_nbr,
                 [&](long long i, int cell, double) {
                     const long long sl = slot[static_cast<size_t>(cell)];
                     if (sl >= 0) visitors[static_cast<size_t>(sl)].push_back(i);
+                    if (i != last_i) {
+                        last_i = i;
+                        rank = 0;
+                    }
+                    if (share_lower) visited.emplace(pair_key(i, cell), rank++);
                 });
+    // the face of the lower-index neighbour each listed cell's face is closed with (-1: its own copy)
+    auto twin_of = [&](long long c, long long f) -> long long {
+        const int nb = face_nbr[f];
+        if (share_lower != 1 || nb < 0 || nb >= c) return -1;
+        int own = 0, back = 0;
+        long long t = -1;
+        for (long long g = cell_face_ptr[c]; g < cell_face_ptr[c + 1]; ++g) own += face_nbr[g] == nb;
+        for (long long g = cell_face_ptr[nb]; g < cell_face_ptr[nb + 1]; ++g)
+            if (face_nbr[g] == c) {
+                ++back;
+                t = g;
+            }
+        return (own == 1 && back == 1) ? t : -1;
+    };
     std::vector<__float128> acc(static_cast<size_t>(4 * nlist), 0);
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1)
@@ -1084,18 +1122,43 @@ int oracle_voronoi_map_cells_q(long long n, const double* x, const double* y, co
         for (const long long i : visitors[static_cast<size_t>(sl)]) {  // particle order, as the BFS
             const __float128 hh = h[i];
             __float128 Ms = 0;
+            std::vector<std::pair<int, __float128>> seen;  // share_lower 2: neighbours met in this cell
+            const auto rc = visited.find(pair_key(i, c));
             for (long long f = cell_face_ptr[c]; f < cell_face_ptr[c + 1]; ++f) {
-                const long long v0 = face_vert_ptr[f];
-                const int nv = static_cast<int>(face_vert_ptr[f + 1] - v0);
+                long long g = twin_of(c, f);
+                if (g >= 0 && !visited.count(pair_key(i, face_nbr[f]))) g = -1;
+                const int nb = face_nbr[f];
+                if (share_lower == 2 && nb >= 0) {
+                    bool repeat = false;
+                    for (const auto& s : seen)
+                        if (s.first == nb) {
+                            Ms += s.second;
+                            repeat = true;
+                            break;
+                        }
+                    if (repeat) continue;
+                    const auto rn = visited.find(pair_key(i, nb));
+                    if (rn != visited.end() && rc != visited.end() && rn->second < rc->second)
+                        for (long long q = cell_face_ptr[nb]; q < cell_face_ptr[nb + 1]; ++q)
+                            if (face_nbr[q] == c) {
+                                g = q;
+                                break;
+                            }
+                }
+                const long long fe = g >= 0 ? g : f;
+                const long long v0 = face_vert_ptr[fe];
+                const int nv = static_cast<int>(face_vert_ptr[fe + 1] - v0);
+                __float128 val;
                 if (ki.gen) {
                     const __float128 H = static_cast<__float128>(ki.support_in_h) * hh;
                     EvalGen<__float128> ev{ki.gen, H, {}, {}};
-                    Ms += face_walk<EvalGen<__float128>>(x[i], y[i], z[i], H, face_xyz + 3 * v0, nv, nullptr, ev);
+                    val = face_walk<EvalGen<__float128>>(x[i], y[i], z[i], H, face_xyz + 3 * v0, nv, nullptr, ev);
                 } else {
                     EvalM4<__float128> ev{hh, nullptr, {}, {}};
-                    Ms += face_walk<EvalM4<__float128>>(x[i], y[i], z[i], 2 * hh, face_xyz + 3 * v0, nv, nullptr,
-                                                        ev);
+                    val = face_walk<EvalM4<__float128>>(x[i], y[i], z[i], 2 * hh, face_xyz + 3 * v0, nv, nullptr, ev);
                 }
+                Ms += g >= 0 ? -val : val;
+                if (share_lower == 2 && nb >= 0) seen.emplace_back(nb, g >= 0 ? -val : val);
             }
             if (Ms < 0) Ms = 0;
             const __float128 t = Ms * m[i];

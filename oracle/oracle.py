@@ -75,7 +75,7 @@ def lib():
                                                                   ctypes.POINTER(OracleKernel), _pd, _pd, _pd, _pll]
         L.oracle_voronoi_map_cells_q.restype = _i
         L.oracle_voronoi_map_cells_q.argtypes = [_ll] + [_pd] * 8 + [_pi, _pi, _ll, _pll, _pll, _pd, _pi,
-                                                                      ctypes.POINTER(OracleKernel), _ll, _pll, _pd]
+                                                                      ctypes.POINTER(OracleKernel), _ll, _pll, _i, _pd]
         _lib = L
     return _lib
 
@@ -390,9 +390,13 @@ def inverse_mapping(parts, pid, home, cell_face_ptr, face_vert_ptr, face_xyz, fa
     return rec, cov, counts
 
 
-def voronoi_map_cells_quad(parts, pid, home, cell_face_ptr, face_vert_ptr, face_xyz, face_nbr, cells, k=None):
+def voronoi_map_cells_quad(parts, pid, home, cell_face_ptr, face_vert_ptr, face_xyz, face_nbr, cells, k=None,
+                           share_lower=True):
     """The Voronoi map's (mass, px, py, pz) for the listed cells with the face integrals in binary128 (the
-    visits decided by the double-precision BFS).  Returns an array of shape (len(cells), 4)."""
+    visits decided by the double-precision BFS).  share_lower: True (1) -- a face shared with a visited
+    lower-index neighbour takes the negated integral of the neighbour's copy (the GPU map's rule); False
+    (0) -- every cell uses its own copies; 2 -- the reference's face cache (a neighbour the particle's BFS
+    processed earlier lends its copy).  Returns an array of shape (len(cells), 4)."""
     k = k or kernel()
     n = len(parts["x"])
     cols = [np.ascontiguousarray(parts[c], dtype=np.float64) for c in ("x", "y", "z", "vx", "vy", "vz", "m", "h")]
@@ -407,7 +411,8 @@ def voronoi_map_cells_quad(parts, pid, home, cell_face_ptr, face_vert_ptr, face_
     rc = lib().oracle_voronoi_map_cells_q(n, *[_p(c) for c in cols], pid.ctypes.data_as(_pi),
                                           home.ctypes.data_as(_pi), len(cfp) - 1, cfp.ctypes.data_as(_pll),
                                           fvp.ctypes.data_as(_pll), _p(fxyz), fnb.ctypes.data_as(_pi),
-                                          ctypes.byref(k), len(cells), cells.ctypes.data_as(_pll), _p(out))
+                                          ctypes.byref(k), len(cells), cells.ctypes.data_as(_pll),
+                                          int(share_lower), _p(out))
     if rc != 0:
         raise ValueError("oracle_voronoi_map_cells_q failed")
     return out

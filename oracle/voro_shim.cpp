@@ -9,6 +9,7 @@
 // travels with the working tree to the GPU machine, where the Voronoi, inverse-mapping and cartesian-map
 // GPU tests load it on the host to tessellate their inputs; libkiwi_hip never links or calls it.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -33,8 +34,14 @@ void* voro_build(int npts, const int* id, const double* x, const double* y, cons
     using namespace voro;
     pre_container pcon(dom[0], dom[1], dom[2], dom[3], dom[4], dom[5], false, false, false);
     for (int i = 0; i < npts; ++i) pcon.put(id[i], x[i], y[i], z[i]);
+    // the reference's block counts: voro++'s guess scaled by the default grid_blocks.block_scale_factor
+    // of 3 (guess_scaled_blocks, include/kiwi/voro_grid.hpp:18-26); the blocks set the cells' plane
+    // tolerance, so near-degenerate vertices resolve as in the reference's own tessellation
     int nx, ny, nz;
     pcon.guess_optimal(nx, ny, nz);
+    nx = std::max(1, static_cast<int>(std::round(nx * 3.0)));
+    ny = std::max(1, static_cast<int>(std::round(ny * 3.0)));
+    nz = std::max(1, static_cast<int>(std::round(nz * 3.0)));
     container con(dom[0], dom[1], dom[2], dom[3], dom[4], dom[5], nx, ny, nz, false, false, false, 8);
     pcon.setup(con);
 

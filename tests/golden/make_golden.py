@@ -26,8 +26,20 @@ fixtures it writes are plain numbers (inputs and the reference's outputs), commi
   cartesian_map.npz     `kiwi cartesian-map` (reference CLI, resolution 20) chained from the same `kiwi map` run:
                         the density, velocity and Voronoi cell id fields of cartesian_map.dat
 
+  realdata.npz          the reference's only real-data fixture, tests/data/brick_03927_cutout.ascii (a
+                        SPLASH export of a Phantom run: 7,300 gas particles as the reference's own reader
+                        keeps them), with the domain rule of the reference's tests/real_data_test.py
+                        (bounding box padded by 3 x 1.02 x max h, whole 1 pc cells): `kiwi sph-to-grid`
+                        (OMP_NUM_THREADS=1) for the cubic spline, the cubic spline through the generic
+                        integral, the tabulated cubic, the quintic spline and Wendland C6, plus the cubic
+                        spline on 0.2 pc cells over the same padded box; and `kiwi map` with the
+                        coverage_grid buffer and one Lloyd step (the test's Voronoi checks): the point set
+                        after the buffer (points-buff.dat), after the Lloyd step (points-final.dat) and
+                        mapping.dat; and `kiwi map` with neither (the sites are the particles, so the
+                        tessellation here is the reference's own): mapping.dat
+
   python tests/golden/make_golden.py [--only kernel_grids | --only voronoi | --only inverse | --only cmap
-                                    | --only coverage]
+                                    | --only coverage | --only realdata]
 """
 import ctypes
 import json
@@ -213,13 +225,95 @@ def coverage():
     np.savez_compressed(os.path.join(OUT, "coverage_buffer.npz"), **out)
 
 
+REAL_SNAPSHOT = "/root/reference/tests/data/brick_03927_cutout.ascii"
+REAL_CASES = [("cubic", None, False), ("cubic_generic", {"generic_path": True}, False), ("cubic_tab", None, True),
+              ("quintic_spline", {"name": "quintic_spline"}, False), ("wendland_c6", {"name": "wendland_c6"}, False)]
+
+
+def real_domain(parts, cell):
+    """The reference test's padded domain (tests/real_data_test.py:112-119): bounding box padded by
+    3 x 1.02 x max h, whole cells of `cell` per axis."""
+    pad = 3.0 * float(parts["h"].max()) * 1.02
+    dom, res = [], []
+    for c in ("x", "y", "z"):
+        a, b = float(parts[c].min()) - pad, float(parts[c].max()) + pad
+        n = max(1, int(np.ceil((b - a) / cell)))
+        dom += [a, a + n * cell]
+        res.append(n)
+    return np.array(dom), tuple(res)
+
+
+def realdata():
+    """The reference CLI on its own real-data cutout (module docstring)."""
+    parts = O.ref_load_snapshot(REAL_SNAPSHOT)
+    cols = ("x", "y", "z", "vx", "vy", "vz", "m", "h")
+    out = {f"p_{k}": parts[k] for k in cols}
+    out["p_id"] = parts["id"]
+    dom, res = real_domain(parts, 1.0)
+    out["domain"], out["res"] = dom, np.array(res)
+    for label, kcfg, tab in REAL_CASES:
+        shape, gdom, body = run_ref_file(REAL_SNAPSHOT, dom, res, kernel=kcfg, tabulated=tab)
+        assert tuple(shape) == res and np.array_equal(gdom, dom)
+        for i, f in enumerate(("density", "vx", "vy", "vz")):
+            out[f"{label}_{f}"] = body[i]
+        print(f"realdata {label}: {res}", file=sys.stderr)
+    fdom, fres = real_domain(parts, 0.2)
+    shape, _, body = run_ref_file(REAL_SNAPSHOT, fdom, fres)
+    out["fine_domain"], out["fine_res"] = fdom, np.array(fres)
+    for i, f in enumerate(("density", "vx", "vy", "vz")):
+        out[f"fine_{f}"] = body[i]
+    print(f"realdata fine cubic: {fres}", file=sys.stderr)
+    buff, final, rows = run_ref_file_map(REAL_SNAPSHOT, dom)
+    out["buff_id"], out["buff_xyz"] = buff[:, 0].astype(np.int32), buff[:, 1:4]
+    out["final_id"], out["final_xyz"] = final[:, 0].astype(np.int32), final[:, 1:4]
+    out["map_rows"] = rows
+    _, _, plain = run_ref_file_map(REAL_SNAPSHOT, dom, buffer=False)
+    out["map_plain_rows"] = plain
+    print(f"realdata map: {len(buff)} sites, {len(rows)} rows", file=sys.stderr)
+    np.savez_compressed(os.path.join(OUT, "realdata.npz"), **out)
+
+
+def run_ref_file(snapshot, domain, res, kernel=None, tabulated=False):
+    with tempfile.TemporaryDirectory() as td:
+        cfg = {"run_name": "real", "input_snapshot": snapshot, "work_folder": os.path.join(td, "work"),
+               "domain": dict(zip(["x_min", "x_max", "y_min", "y_max", "z_min", "z_max"], map(float, domain))),
+               "cartesian_grid": {"resolution_x": int(res[0]), "resolution_y": int(res[1]),
+                                  "resolution_z": int(res[2]), "use_tabulated_integral": bool(tabulated)}}
+        if kernel:
+            cfg["kernel"] = kernel
+        cp = os.path.join(td, "cfg.json")
+        with open(cp, "w") as f:
+            json.dump(cfg, f)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([O.REF_CLI, "sph-to-grid", cp], env=env, capture_output=True, text=True, check=True)
+        base = os.path.splitext(os.path.basename(snapshot))[0]
+        return read_vgtcgrd2(os.path.join(td, "work", base, "cartesian_grid.dat"))
+
+
+def run_ref_file_map(snapshot, domain, buffer=True):
+    with tempfile.TemporaryDirectory() as td:
+        cfg = {"run_name": "real", "input_snapshot": snapshot, "work_folder": os.path.join(td, "work"),
+               "domain": dict(zip(["x_min", "x_max", "y_min", "y_max", "z_min", "z_max"], map(float, domain))),
+               "buffer": {"enabled": bool(buffer), "strategy": "coverage_grid"},
+               "lloyd": {"iterations": 1 if buffer else 0, "save_each_iteration": False}}
+        cp = os.path.join(td, "cfg.json")
+        with open(cp, "w") as f:
+            json.dump(cfg, f)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([O.REF_CLI, "map", cp], env=env, capture_output=True, text=True, check=True)
+        d = os.path.join(td, "work", os.path.splitext(os.path.basename(snapshot))[0])
+        return (np.loadtxt(os.path.join(d, "points-buff.dat"), ndmin=2) if buffer else None,
+                np.loadtxt(os.path.join(d, "points-final.dat"), ndmin=2),
+                np.loadtxt(os.path.join(d, "mapping.dat"), ndmin=2))
+
+
 def main():
     if not O.have_ref() or not os.path.exists(O.REF_CLI):
         sys.exit("oracle/_ref is not built (make -f oracle/ref.mk)")
     if "--only" in sys.argv:
         what = sys.argv[sys.argv.index("--only") + 1]
         {"kernel_grids": kernel_grids, "voronoi": voronoi, "inverse": inverse, "cmap": cmap,
-         "coverage": coverage}[what]()
+         "coverage": coverage, "realdata": realdata}[what]()
         return
     R = O.ref()
     rng = np.random.default_rng(20251203)
@@ -311,6 +405,7 @@ def main():
     inverse()
     cmap()
     coverage()
+    realdata()
     print("golden fixtures written to", OUT)
 
 

@@ -38,7 +38,8 @@ def assert_voronoi_cells_match(gpu4, ref4, parts, pid, home, cells, tol=1e-12, q
                                k=None):
     """The rule for the Voronoi map: every field within `tol` of the field maximum of the reference's
     double-precision result, or else, for that cell, within `quad_tol` of the cell recomputed with the
-    face integrals in binary128 (oracle.voronoi_map_cells_quad; each cell's own faces).
+    face integrals in binary128 (oracle.voronoi_map_cells_quad, closing each cell with the same copy of a
+    shared face as the GPU does).
 
     Two effects separate the double results from binary128.  (1) The reference's analytic cubic vertex
     integral sums an I-series (src/kernel_integral.cpp) whose double value is off by up to ~1e-10 of a
@@ -47,8 +48,10 @@ def assert_voronoi_cells_match(gpu4, ref4, parts, pid, home, cells, tol=1e-12, q
     the other cell negated (the reference's face cache, src/mapping.cpp:87-159; the GPU does the same
     with the lower cell index as owner).  voro++ computes each cell's vertices on its own, so the two
     copies of a face differ by rounding and a cell closed with its neighbour's copy is off by ~5e-16 of
-    a particle's mass per pair.  Both the reference and the GPU carry (2), each on the faces its owner
-    rule assigns, so the HIP value is not required to be closer to binary128 than the reference's."""
+    a particle's mass per pair -- more where voro++'s plane tolerance, which scales with the domain's
+    extent, snaps a vertex differently in the two cells.  Both the reference and the GPU carry (2), each
+    on the faces its owner rule assigns, so the HIP value is not required to be closer to binary128 than
+    the reference's; the binary128 evaluation closes cells with the GPU's copies."""
     gpu4 = [np.asarray(a) for a in gpu4]
     ref4 = [np.asarray(a) for a in ref4]
     scale = [max(np.max(np.abs(r)), 1e-300) for r in ref4]
