@@ -29,6 +29,7 @@
 #ifndef KIWI_HIP_H
 #define KIWI_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -164,6 +165,21 @@ void* kiwi_hip_stream(kiwi_hip_ctx* ctx);
 int kiwi_hip_sph_to_grid(kiwi_hip_ctx* ctx, const kiwi_particles* particles, const kiwi_grid* grid,
                          const kiwi_kernel* kernel, int precision, const kiwi_cell_sums* out,
                          kiwi_stats* stats);
+
+/* kiwi_hip_sph_to_grid without device allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory during the call (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Every buffer of the pipeline run is carved from a fixed scratch area
+ * of KIWI_NOALLOC_SCRATCH_BYTES that the library holds on each device from its load until the process
+ * ends; the call makes no hipMalloc or hipFree.  Particles (n > 0) and outputs must be device pointers;
+ * the tabulated integral is not available (KIWI_ERR_UNSUPPORTED).  A problem whose buffers do not fit
+ * the area is rejected with KIWI_ERR_LIMIT (pairs of bounding-box cells dominate: about 4 bytes each in
+ * KIWI_MIXED, 8 in KIWI_FP64).  Results equal kiwi_hip_sph_to_grid's bit for bit.  All calls on one
+ * device share its area, which keeps the last call's contents: calls that overlap on one device, from
+ * any context, race and must be serialised by the caller. */
+#define KIWI_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int kiwi_hip_sph_to_grid_noalloc(kiwi_hip_ctx* ctx, const kiwi_particles* particles, const kiwi_grid* grid,
+                                 const kiwi_kernel* kernel, int precision, const kiwi_cell_sums* out,
+                                 kiwi_stats* stats);
 
 /* One problem of a batch: the arguments of kiwi_hip_sph_to_grid. */
 typedef struct kiwi_batch_problem {

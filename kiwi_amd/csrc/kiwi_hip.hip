@@ -38,6 +38,7 @@
 #include <cstdlib>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -48,7 +49,31 @@
 
 using namespace kiwi_hip;
 
+// The allocation-free entry point's scratch (kiwi_hip_sph_to_grid_noalloc): a fixed device area that
+// every buffer of its pipeline run is carved from.  Each device holds its own copy from the library's
+// load until the process ends; every call on a device shares it.
+static __device__ __align__(256) unsigned char scratch[KIWI_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+// scratch's device address on each device (host code never passes the array's name itself)
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+bool g_scratch_done[MAX_DEVICES] = {};
+
+// called with the device current (the context's creation); false if the address cannot be had
+bool scratch_address(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return false;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_done[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return false;
+        g_scratch_ptr[device] = p;
+        g_scratch_done[device] = true;
+    }
+    return g_scratch_ptr[device] != nullptr;
+}
 
 constexpr int BLK_PTS = (BLK + 1) * (BLK + 1) * (BLK + 1);
 constexpr int MAX_AXIS = 8192;                // grid cells per axis
@@ -1494,6 +1519,8 @@ int kiwi_hip_create(kiwi_hip_ctx** out, int device_ordinal) {
             return KIWI_ERR_DEVICE;
         }
     }
+    // the allocation-free scratch's address on this device (the device is current here)
+    (void)scratch_address(device_ordinal);
     // the generic kernels' coefficient tables, resident for the context's lifetime
     GenTab tabs[KIWI_TAB_NKERNELS];
     for (int k = 0; k < KIWI_TAB_NKERNELS; ++k) gen_tab_init(tabs[k], kHostTables[k]);
@@ -1539,6 +1566,41 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     if (const int rc = resolve_kernel(c, K, &kc)) return rc;
     const HostProblem hp{P, G, out, stats};
     return run_group(c, &hp, 1, K, kc, precision, c->own_bufs());
+}
+
+int kiwi_hip_sph_to_grid_noalloc(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_grid* G, const kiwi_kernel* K,
+                                 int precision, const kiwi_cell_sums* out, kiwi_stats* stats) {
+    if (!c) return KIWI_ERR_INVALID_ARG;
+    c->err.clear();
+    if (!P || !G || !K || !out) return fail(c, KIWI_ERR_INVALID_ARG, "null argument");
+    if (const int rc = check_problem(c, P, G, out)) return rc;
+    if (precision != KIWI_FP64 && precision != KIWI_MIXED) return fail(c, KIWI_ERR_INVALID_ARG, "unknown precision");
+    if ((P->n > 0 && !P->on_device) || !out->on_device)
+        return fail(c, KIWI_ERR_INVALID_ARG, "the allocation-free entry point takes device particles and outputs");
+    KernelChoice kc;
+    if (const int rc = resolve_kernel(c, K, &kc)) return rc;
+    if (kc.tabulated)
+        return fail(c, KIWI_ERR_UNSUPPORTED, "the allocation-free entry point has no tabulated integral (its table "
+                                             "is allocated on first use)");
+    if (!scratch_address(c->device) || !g_scratch_ptr[c->device])
+        return fail(c, KIWI_ERR_DEVICE, "the allocation-free scratch has no device address");
+    // every buffer of the run is carved afresh from the start of this device's scratch
+    c->noalloc_arena = ScratchArena{static_cast<char*>(g_scratch_ptr[c->device]), KIWI_NOALLOC_SCRATCH_BYTES, 0};
+    PipeBufs B;
+    int slot = 0;
+#define KIWI_PB_SHADOW(n)                                       \
+    c->noalloc_bufs[slot] = DevBuf{nullptr, 0, &c->noalloc_arena}; \
+    B.n = &c->noalloc_bufs[slot++];
+    KIWI_PIPE_BUFS(KIWI_PB_SHADOW)
+#undef KIWI_PB_SHADOW
+    const HostProblem hp{P, G, out, stats};
+    const int rc = run_group(c, &hp, 1, K, kc, precision, B);
+    if (rc == KIWI_ERR_OUT_OF_MEMORY) {
+        return fail(c, KIWI_ERR_LIMIT, "the problem needs more device scratch than the allocation-free entry point "
+                                       "holds (KIWI_NOALLOC_SCRATCH_BYTES = " +
+                                           std::to_string(KIWI_NOALLOC_SCRATCH_BYTES >> 20) + " MiB): " + c->err);
+    }
+    return rc;
 }
 
 int kiwi_hip_sph_to_grid_batch(kiwi_hip_ctx* c, int n_problems, const kiwi_batch_problem* problems) {

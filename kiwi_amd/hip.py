@@ -34,7 +34,7 @@ EXPORTED = ["kiwi_hip_abi_version", "kiwi_hip_create", "kiwi_hip_destroy", "kiwi
             "kiwi_hip_stream", "kiwi_hip_sph_to_grid", "kiwi_hip_voronoi_map", "kiwi_hip_sph_to_grid_adjoint",
             "kiwi_hip_sph_to_grid_batch", "kiwi_hip_set_tuning", "kiwi_hip_get_tuning", "kiwi_hip_autotune",
             "kiwi_hip_inverse_mapping", "kiwi_hip_cartesian_map", "kiwi_hip_cartesian_lattice",
-            "kiwi_hip_coverage_buffer", "kiwi_hip_voronoi_centroids"]
+            "kiwi_hip_coverage_buffer", "kiwi_hip_voronoi_centroids", "kiwi_hip_sph_to_grid_noalloc"]
 
 
 class KiwiError(RuntimeError):
@@ -183,6 +183,8 @@ def load():
         L.kiwi_hip_sph_to_grid.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.POINTER(Grid),
                                            ctypes.POINTER(Kernel), _i, ctypes.POINTER(CellSums),
                                            ctypes.POINTER(Stats)]
+        L.kiwi_hip_sph_to_grid_noalloc.restype = _i
+        L.kiwi_hip_sph_to_grid_noalloc.argtypes = L.kiwi_hip_sph_to_grid.argtypes
         L.kiwi_hip_sph_to_grid_adjoint.restype = _i
         L.kiwi_hip_sph_to_grid_adjoint.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.POINTER(Grid),
                                                    ctypes.POINTER(Kernel), _i, ctypes.POINTER(CellSums),
@@ -324,6 +326,23 @@ class Context:
         st = Stats()
         self._check(self.L.kiwi_hip_sph_to_grid(self.h, ctypes.byref(P), ctypes.byref(G), ctypes.byref(kernel),
                                                 int(precision), ctypes.byref(O), ctypes.byref(st)))
+        return arrs[0], arrs[1], arrs[2], arrs[3], st
+
+    def sph_to_grid_noalloc(self, parts, domain, res, out, kernel=None, precision=FP64):
+        """kiwi_hip_sph_to_grid_noalloc: as sph_to_grid with device particle columns and a tuple of 4 device
+        float64 output tensors, computed without device allocation (in the library's fixed scratch area).
+        Returns (mass, px, py, pz, Stats)."""
+        kernel = kernel or kernel_spec()
+        _device_inputs_ready(parts, out)
+        P = _particles(parts)
+        nx, ny, nz = (int(v) for v in res)
+        G = Grid(*[float(v) for v in domain], nx, ny, nz)
+        arrs = list(out)
+        ptrs = [_ptr(a) for a in arrs]
+        O = CellSums(*[q[0] for q in ptrs], int(all(q[1] for q in ptrs)))
+        st = Stats()
+        self._check(self.L.kiwi_hip_sph_to_grid_noalloc(self.h, ctypes.byref(P), ctypes.byref(G), ctypes.byref(kernel),
+                                                        int(precision), ctypes.byref(O), ctypes.byref(st)))
         return arrs[0], arrs[1], arrs[2], arrs[3], st
 
     def _batch(self, problems):
