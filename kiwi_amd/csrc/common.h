@@ -94,7 +94,10 @@ struct DevBuf {
 #define KIWI_PIPE_BUFS(X)                                                                                      \
     X(parts) X(bbox) X(sizes) X(offsets) X(keys_a) X(keys_b) X(pos_a) X(pos_b) X(cells) X(slab_sorted) X(slab) \
     X(cub_tmp) X(ranges) X(work) X(work_sorted) X(order_in) X(order) X(lines) X(ms) X(stats) X(outbuf) X(vsum)  \
-    X(probs) X(part_prob) X(acc_units) X(acc_partial)
+    X(probs) X(part_prob) X(acc_units) X(acc_partial) X(acc_parts)
+#define KIWI_PB_COUNT(n) +1
+constexpr int KIWI_N_PIPE_BUFS = 0 KIWI_PIPE_BUFS(KIWI_PB_COUNT);
+#undef KIWI_PB_COUNT
 struct PipeBufs {
 #define KIWI_PB_MEMBER(n) DevBuf* n;
     KIWI_PIPE_BUFS(KIWI_PB_MEMBER)
@@ -112,13 +115,13 @@ struct kiwi_hip_ctx {
     std::string err;
     DevBuf parts, bbox, sizes, offsets, keys_a, keys_b, pos_a, pos_b, cells, slab_sorted, slab, cub_tmp, ranges, work,
         work_sorted, order_in, order, lines, ms, stats, outbuf, vsum, tables, m4table, probs, part_prob, acc_units,
-        acc_partial;
+        acc_partial, acc_parts;
     bool m4table_ready = false;
     kiwi_tuning tune{16, 8};  // the cell kernel's launch shape (kiwi_hip_set_tuning)
     DevBuf vor[16];  // the Voronoi map's buffers
     // the allocation-free entry point: its arena over the device's scratch area and shadow buffers
     ScratchArena noalloc_arena;
-    DevBuf noalloc_bufs[32];
+    DevBuf noalloc_bufs[KIWI_N_PIPE_BUFS];
     PipeBufs own_bufs() {
         PipeBufs b;
 #define KIWI_PB_OWN(n) b.n = &n;
@@ -130,7 +133,8 @@ struct kiwi_hip_ctx {
     void for_each_buf(F f) {
         for (DevBuf* b : {&parts, &bbox, &sizes, &offsets, &keys_a, &keys_b, &pos_a, &pos_b, &cells, &slab_sorted,
                           &slab, &cub_tmp, &ranges, &work, &work_sorted, &order_in, &order, &lines, &ms, &stats,
-                          &outbuf, &vsum, &tables, &m4table, &probs, &part_prob, &acc_units, &acc_partial})
+                          &outbuf, &vsum, &tables, &m4table, &probs, &part_prob, &acc_units, &acc_partial,
+                          &acc_parts})
             f(*b);
         for (DevBuf& b : vor) f(b);
     }

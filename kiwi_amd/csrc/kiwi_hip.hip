@@ -130,9 +130,60 @@ __device__ __forceinline__ void add_by_problem(unsigned long long v, int q, DevS
 // ------------------------------------------------------------------------------------------------
 // 1. per-particle bounding box and work sizes
 // ------------------------------------------------------------------------------------------------
+// The accumulation's view of a particle: mass, velocity and cell box in one 64-byte record.  k_accum
+// reads one per item in block order, that is at scattered particle positions: one cache line instead
+// of one for each of five separate arrays.
+struct alignas(64) AccPart {
+    double m, v[3];
+    int bb[6];
+    int pad[2];
+};
+
+// Squared distance (units of h) between the particle's coordinate p and the cell range [c0, c1] along
+// axis a, with the range's faces formed as the cell kernel forms its lattice coordinates, lo + c step.
+// Every operation is rounded on its own, so that k_prep and k_emit take identical decisions.
+__device__ __forceinline__ double range_gap2(const GridGeom& g, int a, int c0, int c1, double p, double inv_h) {
+    const double lo = mul_sep(sub_sep(add_sep(gmin(g, a), mul_sep(static_cast<double>(c0), gstep(g, a))), p), inv_h);
+    const double hi =
+        mul_sep(sub_sep(add_sep(gmin(g, a), mul_sep(static_cast<double>(c1 + 1), gstep(g, a))), p), inv_h);
+    const double d = gap(lo, hi);
+    return mul_sep(d, d);
+}
+
+// Calls f(bx, by, bz) for every 8^3 cell block the particle's box [bb] overlaps that holds a cell its
+// support sphere (radius sup h) reaches; with skip off, for every block of the box.  A block is left out
+// only when the nearest point of its part of the box lies beyond the support by a relative margin of
+// 1e-5 (squared), far above the rounding of the cell kernel's own in-sphere test: none of its cells
+// passes that test, so every one of them would hold an exact zero and no counter would count it.
+template <class F>
+__device__ __forceinline__ void for_each_block(const int* bb, const GridGeom& g, double px, double py, double pz,
+                                               double hh, double sup, bool skip, F f) {
+    const double inv_h = 1.0 / hh;
+    const double lim = mul_sep(mul_sep(sup, sup), 1.00001);
+    for (int bx = bb[0] / BLK; bx <= bb[1] / BLK; ++bx) {
+        const double gx = skip ? range_gap2(g, 0, max(bb[0], bx * BLK), min(bb[1], bx * BLK + BLK - 1), px, inv_h) : 0.0;
+        if (gx >= lim) continue;
+        for (int by = bb[2] / BLK; by <= bb[3] / BLK; ++by) {
+            const double gxy =
+                skip ? add_sep(gx, range_gap2(g, 1, max(bb[2], by * BLK), min(bb[3], by * BLK + BLK - 1), py, inv_h))
+                     : 0.0;
+            if (gxy >= lim) continue;
+            for (int bz = bb[4] / BLK; bz <= bb[5] / BLK; ++bz) {
+                if (skip &&
+                    add_sep(gxy, range_gap2(g, 2, max(bb[4], bz * BLK), min(bb[5], bz * BLK + BLK - 1), pz, inv_h)) >=
+                        lim)
+                    continue;
+                f(bx, by, bz);
+            }
+        }
+    }
+}
+
 __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
-                       const double* __restrict__ z, const double* __restrict__ h, double support_in_h,
-                       const Prob* __restrict__ pr, const int* __restrict__ part_prob, int* __restrict__ bbox,
+                       const double* __restrict__ z, const double* __restrict__ h, const double* __restrict__ m,
+                       const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
+                       double support_in_h, int skip_outside, const Prob* __restrict__ pr,
+                       const int* __restrict__ part_prob, int* __restrict__ bbox, AccPart* __restrict__ ap,
                        int64_t* __restrict__ nblk, int64_t* __restrict__ nlines, DevStats* st) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     unsigned long long skipped = 0, pairs = 0;
@@ -143,25 +194,31 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
     }
     if (i < n) {
         int64_t nb = 0, nl = 0;
-        int* bb = bbox + 6 * i;
-        bb[0] = 1;
-        bb[1] = 0;  // empty unless set below
+        int qb[6] = {1, 0, 0, 0, 0, 0};  // empty unless set below
         const double hh = h[i];
         const double x0 = x[i], y0 = y[i], z0 = z[i];
         if (hh > 0.0) {
-            int qb[6];
-            if (!particle_bbox(pr[q].g, support_in_h, x0, y0, z0, hh, qb)) {
+            int tb[6];
+            if (!particle_bbox(pr[q].g, support_in_h, x0, y0, z0, hh, tb)) {
                 skipped = 1;
             } else {
-                const int ix0 = qb[0], ix1 = qb[1], iy0 = qb[2], iy1 = qb[3], iz0 = qb[4], iz1 = qb[5];
-                for (int k = 0; k < 6; ++k) bb[k] = qb[k];
-                const int64_t Nx = ix1 - ix0 + 1, Ny = iy1 - iy0 + 1, Nz = iz1 - iz0 + 1;
+                for (int k = 0; k < 6; ++k) qb[k] = tb[k];
+                const int64_t Nx = qb[1] - qb[0] + 1, Ny = qb[3] - qb[2] + 1, Nz = qb[5] - qb[4] + 1;
                 pairs = static_cast<unsigned long long>(Nx * Ny * Nz);
-                nb = static_cast<int64_t>(ix1 / BLK - ix0 / BLK + 1) * (iy1 / BLK - iy0 / BLK + 1) *
-                     (iz1 / BLK - iz0 / BLK + 1);
+                for_each_block(qb, pr[q].g, x0, y0, z0, hh, support_in_h, skip_outside != 0,
+                               [&](int, int, int) { ++nb; });
                 nl = (Nx + 1) * (Ny + 1) + (Nx + 1) * (Nz + 1) + (Ny + 1) * (Nz + 1);
             }
         }
+        int* bb = bbox + 6 * i;
+        AccPart a;
+        a.m = m[i];
+        a.v[0] = vx[i];
+        a.v[1] = vy[i];
+        a.v[2] = vz[i];
+        for (int k = 0; k < 6; ++k) bb[k] = a.bb[k] = qb[k];
+        a.pad[0] = a.pad[1] = 0;
+        ap[i] = a;
         nblk[i] = nb;
         nlines[i] = nl;
     }
@@ -172,7 +229,10 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
 // ------------------------------------------------------------------------------------------------
 // 3. items: (particle, cell block) keys in particle order
 // ------------------------------------------------------------------------------------------------
-__global__ void k_emit(int64_t n, const int* __restrict__ bbox, const int64_t* __restrict__ offs,
+// the blocks k_prep counted, in the same order of decisions
+__global__ void k_emit(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                       const double* __restrict__ z, const double* __restrict__ h, double support_in_h,
+                       int skip_outside, const int* __restrict__ bbox, const int64_t* __restrict__ offs,
                        const Prob* __restrict__ pr, const int* __restrict__ part_prob, uint64_t* __restrict__ keys) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -180,13 +240,16 @@ __global__ void k_emit(int64_t n, const int* __restrict__ bbox, const int64_t* _
     if (bb[0] > bb[1]) return;
     const Prob& P = pr[part_prob[i]];
     const GridGeom& g = P.g;
+    int qb[6];
+    for (int k = 0; k < 6; ++k) qb[k] = bb[k];
     int64_t o = offs[i];
-    for (int bx = bb[0] / BLK; bx <= bb[1] / BLK; ++bx)
-        for (int by = bb[2] / BLK; by <= bb[3] / BLK; ++by)
-            for (int bz = bb[4] / BLK; bz <= bb[5] / BLK; ++bz) {
-                const uint64_t b = static_cast<uint64_t>(P.block_off) + (static_cast<uint64_t>(bx) * g.nby + by) * g.nbz + bz;
-                keys[o++] = (b << 32) | static_cast<uint64_t>(i);
-            }
+    const int64_t o_end = offs[i + 1];
+    for_each_block(qb, g, x[i], y[i], z[i], h[i], support_in_h, skip_outside != 0, [&](int bx, int by, int bz) {
+        if (o < o_end) {
+            const uint64_t b = static_cast<uint64_t>(P.block_off) + (static_cast<uint64_t>(bx) * g.nby + by) * g.nbz + bz;
+            keys[o++] = (b << 32) | static_cast<uint64_t>(i);
+        }
+    });
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -820,8 +883,7 @@ struct AccLds {
 
 template <typename R>
 __global__ __launch_bounds__(THREADS) void k_accum(
-    const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
-    const double* __restrict__ m, const int* __restrict__ bbox, const uint64_t* __restrict__ keys,
+    const AccPart* __restrict__ ap, const uint64_t* __restrict__ keys,
     const int64_t* __restrict__ slab, const int* __restrict__ blk_begin, const int* __restrict__ blk_end,
     const int* __restrict__ order, const R* __restrict__ ms, const Prob* __restrict__ pr, int nprob,
     int clamp_pairs, int nblocks, const AccUnit* __restrict__ units, const int* __restrict__ unit_off,
@@ -855,16 +917,16 @@ __global__ __launch_bounds__(THREADS) void k_accum(
         __syncthreads();
         if (tid < cnt) {
             const uint64_t key = keys[base + tid];
-            const int p = static_cast<int>(key & 0xffffffffull);
-            const ItemBox ib = item_box(bbox + 6 * p, static_cast<uint32_t>(bl), g);
+            const AccPart& a = ap[key & 0xffffffffull];
+            const ItemBox ib = item_box(a.bb, static_cast<uint32_t>(bl), g);
             const uint32_t ox = ib.x0 - bx * BLK, oy = ib.y0 - by * BLK, oz = ib.z0 - bz * BLK;
             A.box[tid] = ox | (oy << 4) | (oz << 8) | (static_cast<uint32_t>(ib.mx) << 12) |
                          (static_cast<uint32_t>(ib.my) << 16) | (static_cast<uint32_t>(ib.mz) << 20);
             A.off[tid] = slab[base + tid];
-            A.m[tid] = m[p];
-            A.v[0][tid] = vx[p];
-            A.v[1][tid] = vy[p];
-            A.v[2][tid] = vz[p];
+            A.m[tid] = a.m;
+            A.v[0][tid] = a.v[0];
+            A.v[1][tid] = a.v[1];
+            A.v[2][tid] = a.v[2];
         }
         __syncthreads();
         // the wavefront's own list: staged items whose x-range meets its slices [cx0, cx0 + ACC_CPT), in
@@ -1034,7 +1096,7 @@ namespace {
 
 template <class P>
 hipError_t launch_cells(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pctx, int64_t n, int64_t npairs,
-                        const double* const* dp, const int* bbox, const int64_t* loffs,
+                        const double* const* dp, const int* bbox, const AccPart* ap, const int64_t* loffs,
                         const uint64_t* keys_particle_order, const int64_t* slab, const uint64_t* keys_block_order,
                         const int64_t* slab_sorted, const int* blk_begin, const int* blk_end, const int* order,
                         int nblocks, const Prob* pr, int nprob, const int* part_prob, void* lines, void* ms,
@@ -1054,7 +1116,7 @@ hipError_t launch_cells(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pc
     if ((e = hipEventRecord(ev_cells, s)) != hipSuccess) return e;
     if (nblocks > 0) {
         k_accum<R><<<static_cast<unsigned>(max_units), THREADS, 0, s>>>(
-            dp[3], dp[4], dp[5], dp[6], bbox, keys_block_order, slab_sorted, blk_begin, blk_end, order,
+            ap, keys_block_order, slab_sorted, blk_begin, blk_end, order,
             static_cast<const R*>(ms), pr, nprob, clamp, nblocks, units, unit_off, partial);
         k_acc_combine<<<nblocks, THREADS, 0, s>>>(nblocks, order, split_off, partial, pr, nprob, clamp);
     }
@@ -1215,6 +1277,7 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     DevStats* dst = static_cast<DevStats*>((*B.stats).p);
     unsigned long long* cursor = reinterpret_cast<unsigned long long*>(static_cast<char*>((*B.stats).p) + stats_bytes);
     KIWI_HIP_CHECK(c, reserve((*B.bbox), 6 * sizeof(int) * std::max<int64_t>(n, 1)));
+    KIWI_HIP_CHECK(c, reserve((*B.acc_parts), sizeof(AccPart) * std::max<int64_t>(n, 1)));
     KIWI_HIP_CHECK(c, reserve((*B.sizes), 2 * sizeof(int64_t) * n1));
     KIWI_HIP_CHECK(c, reserve((*B.offsets), 2 * sizeof(int64_t) * n1));
     KIWI_HIP_CHECK(c, reserve((*B.ranges), 2 * sizeof(int) * std::max<int64_t>(nblocks, 1)));
@@ -1224,6 +1287,7 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     KIWI_HIP_CHECK(c, reserve((*B.order), sizeof(int) * std::max<int64_t>(nblocks, 1)));
     KIWI_HIP_CHECK(c, reserve((*B.vsum), sizeof(double) * 3 * nprob * (VSUM_BLOCKS + 1)));
     int* bbox = static_cast<int*>((*B.bbox).p);
+    AccPart* ap = static_cast<AccPart*>((*B.acc_parts).p);
     int64_t* sz = static_cast<int64_t*>((*B.sizes).p);
     int64_t* of = static_cast<int64_t*>((*B.offsets).p);
     int64_t *nblk = sz, *nlin = sz + n1;
@@ -1243,7 +1307,11 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     }
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[0], s));
     if (n > 0) k_part_prob<<<grid1d(n), THREADS, 0, s>>>(n, dpr, nprob, part_prob);
-    k_prep<<<grid1d(n1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], sup, dpr, part_prob, bbox, nblk, nlin, dst);
+    // blocks wholly outside the support sphere hold only exact zeros and are left out, except on the
+    // tabulated path, whose table values do not vanish there
+    const int skip_outside = kc.tabulated ? 0 : 1;
+    k_prep<<<grid1d(n1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], dp[6], dp[3], dp[4], dp[5], sup, skip_outside,
+                                          dpr, part_prob, bbox, ap, nblk, nlin, dst);
     KIWI_HIP_CHECK(c, hipGetLastError());
     {
         size_t tmp_bytes = 0;
@@ -1315,7 +1383,8 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
         KIWI_HIP_CHECK(c, hipEventRecord(c->side_done, c->side));
     }
     if (n > 0) {
-        k_emit<<<grid1d(n), THREADS, 0, s>>>(n, bbox, offs, dpr, part_prob, keys_a);
+        k_emit<<<grid1d(n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], sup, skip_outside, bbox, offs, dpr,
+                                             part_prob, keys_a);
         KIWI_HIP_CHECK(c, hipGetLastError());
     }
     int end_bit = 33;  // at least one key bit above the particle index
@@ -1396,33 +1465,33 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     if (tabulated) {
         KIWI_HIP_CHECK(c, kiwi_build_m4_table(c, s));
         const PolicyTab::Ctx pc{sup, static_cast<const double*>(c->m4table.p)};
-        KIWI_HIP_CHECK(c, launch_cells<PolicyTab>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
+        KIWI_HIP_CHECK(c, launch_cells<PolicyTab>(c, s, pc, n, npairs, dp, bbox, ap, loffs, keys_a, slab, keys_b,
                                                   slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob, part_prob,
                                                   (*B.lines).p, (*B.ms).p, dst, cursor, clamp,
                                                   units, unit_off, split_off, max_units, partial, c->ev[2], c->ev[3],
                                                   c->ev[4]));
     } else if (generic && fp64) {
         const PolicyGen<double>::Ctx pc{sup, static_cast<const GenTab*>(c->tables.p) + K->kernel_id};
-        KIWI_HIP_CHECK(c, launch_cells<PolicyGen<double>>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
+        KIWI_HIP_CHECK(c, launch_cells<PolicyGen<double>>(c, s, pc, n, npairs, dp, bbox, ap, loffs, keys_a, slab, keys_b,
                                                           slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
                                                           part_prob, (*B.lines).p, (*B.ms).p, dst, cursor, clamp,
                                                           units, unit_off, split_off, max_units, partial,
                                                           c->ev[2], c->ev[3], c->ev[4]));
     } else if (generic) {
         const PolicyGen<float>::Ctx pc{sup, static_cast<const GenTab*>(c->tables.p) + K->kernel_id};
-        KIWI_HIP_CHECK(c, launch_cells<PolicyGen<float>>(c, s, pc, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
+        KIWI_HIP_CHECK(c, launch_cells<PolicyGen<float>>(c, s, pc, n, npairs, dp, bbox, ap, loffs, keys_a, slab, keys_b,
                                                          slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
                                                          part_prob, (*B.lines).p, (*B.ms).p, dst, cursor, clamp,
                                                          units, unit_off, split_off, max_units, partial,
                                                          c->ev[2], c->ev[3], c->ev[4]));
     } else if (fp64) {
-        KIWI_HIP_CHECK(c, launch_cells<PolicyM4<double>>(c, s, {sup}, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
+        KIWI_HIP_CHECK(c, launch_cells<PolicyM4<double>>(c, s, {sup}, n, npairs, dp, bbox, ap, loffs, keys_a, slab, keys_b,
                                                          slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
                                                          part_prob, (*B.lines).p, (*B.ms).p, dst, cursor, clamp,
                                                          units, unit_off, split_off, max_units, partial, c->ev[2],
                                                          c->ev[3], c->ev[4]));
     } else {
-        KIWI_HIP_CHECK(c, launch_cells<PolicyM4<float>>(c, s, {sup}, n, npairs, dp, bbox, loffs, keys_a, slab, keys_b,
+        KIWI_HIP_CHECK(c, launch_cells<PolicyM4<float>>(c, s, {sup}, n, npairs, dp, bbox, ap, loffs, keys_a, slab, keys_b,
                                                         slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
                                                         part_prob, (*B.lines).p, (*B.ms).p, dst, cursor, clamp,
                                                         units, unit_off, split_off, max_units, partial, c->ev[2],
