@@ -76,6 +76,8 @@ bool scratch_address(int device) {
 }
 
 constexpr int BLK_PTS = (BLK + 1) * (BLK + 1) * (BLK + 1);
+constexpr int AX_STRIDE = BLK + 1;  // LDS entries per axis in the cell kernel
+constexpr int LINE_LDS_AX = 128;    // coordinates per axis the line kernel keeps in LDS
 constexpr int MAX_AXIS = 8192;                // grid cells per axis
 // The cell kernel's items are split into `cursors` contiguous slices (the context's launch shape,
 // default 8), each with its own work cursor, from which a wavefront takes `item_grab` items at a time
@@ -182,7 +184,7 @@ __device__ __forceinline__ void for_each_block(const int* bb, const GridGeom& g,
 __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                        const double* __restrict__ z, const double* __restrict__ h, const double* __restrict__ m,
                        const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
-                       double support_in_h, int skip_outside, const Prob* __restrict__ pr,
+                       double support_in_h, int skip_outside, int axis_units, const Prob* __restrict__ pr,
                        const int* __restrict__ part_prob, int* __restrict__ bbox, AccPart* __restrict__ ap,
                        int64_t* __restrict__ nblk, int64_t* __restrict__ nlines, DevStats* st) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -208,6 +210,7 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
                 for_each_block(qb, pr[q].g, x0, y0, z0, hh, support_in_h, skip_outside != 0,
                                [&](int, int, int) { ++nb; });
                 nl = (Nx + 1) * (Ny + 1) + (Nx + 1) * (Nz + 1) + (Ny + 1) * (Nz + 1);
+                if (axis_units > 0) nl += axis_units - 1 + static_cast<int64_t>(axis_units) * (Nx + Ny + Nz + 3);
             }
         }
         int* bb = bbox + 6 * i;
@@ -353,6 +356,7 @@ __global__ void k_block_work(int64_t npairs, const uint64_t* __restrict__ keys, 
 template <typename R>
 struct PolicyM4 {
     using Real = R;
+    static constexpr int WAVES_PER_EU = sizeof(R) == 4 ? 8 : 1;  // the cell kernel's occupancy request
     static constexpr int NPL = 3;  // plane values per lattice coordinate
     static constexpr int NLN = 2;  // line-table values per axis pair
     static constexpr bool HAS_LINES = true;
@@ -372,6 +376,12 @@ struct PolicyM4 {
     // line-table slot of region k, -1: none
     static __device__ __forceinline__ int line_slot(int k) { return k - 2; }
     static __device__ __forceinline__ void line(const Ctx&, R da, R db, R* v) { m4_line<R>(da, db, v[0], v[1]); }
+    // the table entry of a pair of coordinates from their |d| and plane values: the two lines' offsets
+    // summed, shared terms formed once (m4_line_pair)
+    static constexpr bool FUSED_PAIRS = true;
+    static __device__ __forceinline__ void line_pair(const Ctx&, R da, const R* pa, R db, const R* pb, R* v) {
+        m4_line_pair<R>(da, db, pa[0] - pa[1], pa[1] - pa[2], pb[0] - pb[1], pb[1] - pb[2], v[0], v[1]);
+    }
     static __device__ __forceinline__ int line_kind(double l2) { return l2 < 1.0 ? 2 : (l2 < 4.0 ? 1 : 0); }
     // the lattice value without its plane and line terms, and the weight of the plane sum
     static __device__ __forceinline__ R free_part(const Ctx&, R ax, R ay, R az, R r, int k) {
@@ -389,6 +399,7 @@ struct PolicyM4 {
 template <typename R>
 struct PolicyGen {
     using Real = R;
+    static constexpr int WAVES_PER_EU = 1;
     static constexpr int NPL = GEN_NREG;
     static constexpr int NLN = GEN_NREG;
     static constexpr bool HAS_LINES = true;
@@ -410,6 +421,8 @@ struct PolicyGen {
     }
     static __device__ __forceinline__ int line_slot(int k) { return k - 1; }
     static __device__ __forceinline__ void line(const Ctx& c, R da, R db, R* v) { gen_line<R>(*c.tab, da, db, v); }
+    static constexpr bool FUSED_PAIRS = false;
+    static __device__ __forceinline__ void line_pair(const Ctx&, R, const R*, R, const R*, R*) {}
     static __device__ __forceinline__ int line_kind(double l2) { return l2 < 1.0 ? 1 : 0; }
     static __device__ __forceinline__ R free_part(const Ctx& c, R ax, R ay, R az, R r, int k) {
         return gen_lattice_free<R>(gen_regions<R>(*c.tab)[k - 1], ax, ay, az, r);
@@ -422,8 +435,10 @@ struct PolicyGen {
 // cancel outside the support, so, as in the reference, every bounding-box cell is evaluated.
 struct PolicyTab {
     using Real = double;
+    static constexpr int WAVES_PER_EU = 1;
     static constexpr int NPL = 1;  // unused
     static constexpr int NLN = 1;  // unused: the tabulated path has no line tables
+    static constexpr bool HAS_LINES = false;
     static constexpr bool SPHERE_SKIP = false;
     struct Ctx {
         double support_in_h;
@@ -475,6 +490,33 @@ struct LineVals {
     R v[N];
 };
 
+// Per-axis values of a particle's lattice coordinates: per axis a the L_a = box cells + 1 coordinates,
+// axes x, y, z one after the other.  k_lines writes them after the particle's line-pair table (at the
+// next multiple of the entry size); the cell kernel copies each item's window into its LDS.
+template <typename R, int NPL>
+struct alignas(8 * sizeof(R)) AxisVals {
+    R d;        // signed offset (lattice - particle) / scale
+    R cg2;      // squared gap between the particle and the cell [u, u + 1] (unused for the last coordinate)
+    R pv[NPL];  // plane values of |d| (M4: c1..c3; generic: Phi_1..Phi_4)
+};
+
+// line-table entries per axis entry (0: the path has no line tables)
+template <class P>
+constexpr int axis_units() {
+    if constexpr (P::HAS_LINES)
+        return static_cast<int>(sizeof(AxisVals<typename P::Real, P::NPL>) /
+                                sizeof(LineVals<typename P::Real, P::NLN>));
+    else
+        return 0;
+}
+
+// A particle's region of the line table: its line pairs (La Lb entries for each axis pair), then up to
+// units - 1 entries of padding, then its axis table.  Returns the axis table's first entry (in line-
+// table entries from the start of the table).
+__device__ __forceinline__ int64_t axis_table_start(int64_t loff, int npairs_p, int units) {
+    return (loff + npairs_p + units - 1) / units * units;
+}
+
 // ------------------------------------------------------------------------------------------------
 // 6. edge-line constants, once per particle
 // ------------------------------------------------------------------------------------------------
@@ -496,7 +538,15 @@ __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __re
     // go to the problem's statistics when the wavefront moves on to another problem
     const int lane = threadIdx.x & 63;
     const int64_t nwaves = static_cast<int64_t>(gridDim.x) * WAVES;
+    constexpr int UNITS = axis_units<P>();
+    struct LAx {
+        R ad;          // |d|
+        R pv[P::NPL];  // plane values of |d|
+    };
+    __shared__ LAx LDS_AX[WAVES][P::FUSED_PAIRS ? 3 * LINE_LDS_AX : 1];
+    LAx* const LA = LDS_AX[wave_uniform(threadIdx.x >> 6)];
     unsigned long long cnt0 = 0, cnt1 = 0, cnt2 = 0;
+    unsigned long long c_planes = 0;  // wave total (uniform)
     int cur_q = -1;
     auto flush = [&]() {
         for (int off = 32; off > 0; off >>= 1) {
@@ -508,8 +558,9 @@ __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __re
             if (cnt0) atomicAdd(&st[cur_q].lines[0], cnt0);
             if (cnt1) atomicAdd(&st[cur_q].lines[1], cnt1);
             if (cnt2) atomicAdd(&st[cur_q].lines[2], cnt2);
+            if (c_planes) atomicAdd(&st[cur_q].planes, c_planes);
         }
-        cnt0 = cnt1 = cnt2 = 0;
+        cnt0 = cnt1 = cnt2 = c_planes = 0;
     };
     for (int64_t p = static_cast<int64_t>(blockIdx.x) * WAVES + wave_uniform(threadIdx.x >> 6); p < n; p += nwaves) {
         const int* bb = bbox + 6 * p;
@@ -527,6 +578,44 @@ __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __re
         const double inv_s = 1.0 / (P::unit_in_h(ctx) * h[p]);  // lattice units
         const double p0 = x[p], p1 = y[p], p2 = z[p];
         LineVals<R, P::NLN>* out = table + loffs[p];
+        // fused pairs take |d| and the plane values of both coordinates from the wave's LDS copy of the
+        // axis table (particles with at most LINE_LDS_AX coordinates per axis)
+        const bool use_lds = P::FUSED_PAIRS && L0 <= LINE_LDS_AX && L1 <= LINE_LDS_AX && L2 <= LINE_LDS_AX;
+
+        // the axis table: offsets formed as the reference forms vertex coordinates, (x_min + i dx) - x_p
+        AxisVals<R, P::NPL>* at = reinterpret_cast<AxisVals<R, P::NPL>*>(table + axis_table_start(loffs[p], nl, UNITS));
+        const int na = L0 + L1 + L2;
+        for (int t = lane; t < na; t += 64) {
+            int a = 0, u = t;
+            if (u >= L0) {
+                u -= L0;
+                a = 1;
+                if (u >= L1) {
+                    u -= L1;
+                    a = 2;
+                }
+            }
+            const int g0 = sel3(a, bb[0], bb[2], bb[4]);
+            const double lo = gmin(g, a), pa = sel3(a, p0, p1, p2), st_a = gstep(g, a);
+            const double d = (lo + (g0 + u) * st_a - pa) * inv_s;
+            const double dn = (lo + (g0 + u + 1) * st_a - pa) * inv_s;
+            AxisVals<R, P::NPL> e;
+            e.d = static_cast<R>(d);
+            const R gc = static_cast<R>(gap(d, dn));
+            e.cg2 = gc * gc;
+            const R ad = e.d < R(0) ? -e.d : e.d;
+            P::plane(ctx, ad, e.pv);
+            at[t] = e;
+            if (use_lds) {
+                LAx& la = LA[a * LINE_LDS_AX + u];
+                la.ad = ad;
+#pragma unroll
+                for (int k = 0; k < P::NPL; ++k) la.pv[k] = e.pv[k];
+            }
+        }
+        c_planes += static_cast<unsigned>(na);
+        if (use_lds) wave_sync();
+
         for (int t = lane; t < nl; t += 64) {
             // pair blocks (x,y) (x,z) (y,z) of sizes c01 c02 c12
             int u = t, a = 0, b = 1, Lb = L1;
@@ -542,22 +631,31 @@ __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __re
                 }
             }
             const int ia = small_div(u, Lb, inv_b), ib = u - ia * Lb;
-            const double da = fabs((gmin(g, a) + (sel3(a, bb[0], bb[2], bb[4]) + ia) * gstep(g, a) -
-                                    sel3(a, p0, p1, p2)) * inv_s);
-            const double db = fabs((gmin(g, b) + (sel3(b, bb[0], bb[2], bb[4]) + ib) * gstep(g, b) -
-                                    sel3(b, p0, p1, p2)) * inv_s);
-            R vab[P::NLN], vba[P::NLN];
-            P::line(ctx, static_cast<R>(da), static_cast<R>(db), vab);
-            P::line(ctx, static_cast<R>(db), static_cast<R>(da), vba);
-            const int kind = P::line_kind(da * da + db * db);
+            LineVals<R, P::NLN> e;
+            int kind;
+            if (use_lds) {
+                const LAx& A = LA[a * LINE_LDS_AX + ia];
+                const LAx& B = LA[b * LINE_LDS_AX + ib];
+                P::line_pair(ctx, A.ad, A.pv, B.ad, B.pv, e.v);
+                kind = P::line_kind(static_cast<double>(A.ad) * A.ad + static_cast<double>(B.ad) * B.ad);
+            } else {
+                const double da = fabs((gmin(g, a) + (sel3(a, bb[0], bb[2], bb[4]) + ia) * gstep(g, a) -
+                                        sel3(a, p0, p1, p2)) * inv_s);
+                const double db = fabs((gmin(g, b) + (sel3(b, bb[0], bb[2], bb[4]) + ib) * gstep(g, b) -
+                                        sel3(b, p0, p1, p2)) * inv_s);
+                R vab[P::NLN], vba[P::NLN];
+                P::line(ctx, static_cast<R>(da), static_cast<R>(db), vab);
+                P::line(ctx, static_cast<R>(db), static_cast<R>(da), vba);
+#pragma unroll
+                for (int k = 0; k < P::NLN; ++k) e.v[k] = vab[k] + vba[k];
+                kind = P::line_kind(da * da + db * db);
+            }
             cnt2 += (kind == 2) ? 2 : 0;
             cnt1 += (kind == 1) ? 2 : 0;
             cnt0 += (kind == 0) ? 2 : 0;
-            LineVals<R, P::NLN> e;
-#pragma unroll
-            for (int k = 0; k < P::NLN; ++k) e.v[k] = vab[k] + vba[k];
             out[t] = e;
         }
+        if (use_lds) wave_sync();  // the LDS copy is rewritten for the next particle
     }
     flush();
 }
@@ -569,30 +667,45 @@ __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __re
 // <= 27 entries, n = cells + 1 per axis).
 template <typename R, int NPL>
 struct WaveLds {
-    Pair2<R> sg[3 * (BLK + 1)];  // a: signed offset (lattice - particle) / scale; b: squared gap to the
-                                 // cells on either side of the lattice plane (point test)
-    R pc[NPL][3 * (BLK + 1)];    // plane values of |offset| (M4: c1..c3; generic: Phi_1..Phi_4)
-    R cg2[3 * (BLK + 1)];        // squared gap to the cell slab [u, u+1] (cell test; last entry unused)
-    uint32_t list[BLK_PTS];      // needed points: u | i << 10 | j << 14 | k << 18; inside the support
-                                 // from the front, outside from the back
+    // Axis arrays at a fixed stride: x coordinates at [0, 9), y at [9, 18), z at [18, 27) (an item has at
+    // most 9 lattice coordinates per axis), so that every read of them takes an immediate offset.
+    Pair2<R> sg[3 * AX_STRIDE];  // a: signed offset (lattice - particle) / scale; b: squared gap to the
+                                 // cells on either side of the lattice plane within the item (point test)
+    R pc[NPL][3 * AX_STRIDE];    // plane values of |offset| (M4: c1..c3; generic: Phi_1..Phi_4)
+    R cg2[3 * AX_STRIDE];        // squared gap to the cell slab [u, u+1] (cell test; last entry unused)
+    uint16_t list[BLK_PTS];      // needed points: i | j << 4 | k << 8; inside the support from the
+                                 // front, outside from the back
     R U[BLK_PTS];                // signed lattice values s * W, (i, j, k) k fastest
 };
 
+// ceil(2^16 / d) for d = 1..81.  With m = ceil(2^16 / d) = (2^16 + e) / d, 0 <= e < d:
+// (u m) >> 16 = floor(u / d + u e / (d 2^16)), which is floor(u / d) when (u mod d) + u e / 2^16 < d;
+// for 0 <= u <= 64, u e < 64 * 81 < 2^16 makes that hold.  tests/test_cell_kernel_tables.py checks the
+// table and the identity for every (u, d) the walks use.
+__constant__ uint32_t kMagic16[82] = {
+    0,     65536, 32768, 21846, 16384, 13108, 10923, 9363, 8192, 7282, 6554, 5958, 5462, 5042,
+    4682,  4370,  4096,  3856,  3641,  3450,  3277,  3121, 2979, 2850, 2731, 2622, 2521, 2428,
+    2341,  2260,  2185,  2115,  2048,  1986,  1928,  1873, 1821, 1772, 1725, 1681, 1639, 1599,
+    1561,  1525,  1490,  1457,  1425,  1395,  1366,  1338, 1311, 1286, 1261, 1237, 1214, 1192,
+    1171,  1150,  1130,  1111,  1093,  1075,  1058,  1041, 1024, 1009, 993,  979,  964,  950,
+    937,   924,   911,   898,   886,   874,   863,   852,  841,  830,  820,  810};
+
 // Lane-strided walk over an n0 x n1 x n2 box (index u = (i n1 + j) n2 + k): the lane's first point
 // and the (i, j, k) step of 64 points, so that each iteration advances with adds and compares.
+// n1 n2 <= 81; the divisions are multiplications by kMagic16.
 struct BoxWalk {
     int i, j, k;
     int di, dj, dk;
     __device__ __forceinline__ BoxWalk(int lane, int n1, int n2) {
         const int n12 = n1 * n2;
-        const float r12 = __builtin_amdgcn_rcpf(static_cast<float>(n12)), r2 = __builtin_amdgcn_rcpf(static_cast<float>(n2));
-        i = small_div(lane, n12, r12);
+        const uint32_t m12 = kMagic16[n12], m2 = kMagic16[n2];
+        i = static_cast<int>((static_cast<uint32_t>(lane) * m12) >> 16);
         const int r = lane - i * n12;
-        j = small_div(r, n2, r2);
+        j = static_cast<int>((static_cast<uint32_t>(r) * m2) >> 16);
         k = r - j * n2;
-        di = small_div(64, n12, r12);
+        di = static_cast<int>((64u * m12) >> 16);
         const int s = 64 - di * n12;
-        dj = small_div(s, n2, r2);
+        dj = static_cast<int>((static_cast<uint32_t>(s) * m2) >> 16);
         dk = s - dj * n2;
     }
     __device__ __forceinline__ void next(int n1, int n2) {
@@ -610,8 +723,26 @@ struct BoxWalk {
     }
 };
 
+// the value of lane - 1 within each row of 16 lanes (DPP row_shr:1); the first lane of a row keeps its own
+template <typename R>
+__device__ __forceinline__ R row_prev(R v) {
+    if constexpr (sizeof(R) == 4) {
+        const int b = __builtin_bit_cast(int, v);
+        return __builtin_bit_cast(R, __builtin_amdgcn_update_dpp(b, b, 0x111, 0xf, 0xf, false));
+    } else {
+        const long long b = __builtin_bit_cast(long long, v);
+        const int lo = static_cast<int>(b & 0xffffffffll), hi = static_cast<int>(b >> 32);
+        const int plo = __builtin_amdgcn_update_dpp(lo, lo, 0x111, 0xf, 0xf, false);
+        const int phi = __builtin_amdgcn_update_dpp(hi, hi, 0x111, 0xf, 0xf, false);
+        return __builtin_bit_cast(R, (static_cast<long long>(phi) << 32) | static_cast<uint32_t>(plo));
+    }
+}
+
+// occupancy: the mixed cubic path is fitted to 8 wavefronts per SIMD (64 VGPRs, 20 KB of LDS per
+// workgroup); the cell kernel waits on memory for about a third of its wavefront cycles, and more
+// resident wavefronts hide it
 template <class P>
-__global__ __launch_bounds__(THREADS) void k_particles(
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(P::WAVES_PER_EU))) void k_particles(
     uint32_t npairs, const uint64_t* __restrict__ keys, const int64_t* __restrict__ slab,
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     const double* __restrict__ h, const int* __restrict__ bbox, const int64_t* __restrict__ loffs,
@@ -620,6 +751,7 @@ __global__ __launch_bounds__(THREADS) void k_particles(
     unsigned long long* cursor, int clamp_pairs, int grab, int ncurs) {
     using R = typename P::Real;
     constexpr int NLN = P::NLN;
+    constexpr int UNITS = axis_units<P>();
     __shared__ WaveLds<R, P::NPL> WL[WAVES];
     const int wave = static_cast<int>(wave_uniform(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
@@ -650,6 +782,16 @@ __global__ __launch_bounds__(THREADS) void k_particles(
         n_neg = n_warn = n_sphere = c_planes = 0;
         c_pts[0] = c_pts[1] = c_pts[2] = 0;
     };
+    // the lane's axis entry in the set-up of an item: axis lane / 16, coordinate lane % 16
+    const int ax = lane >> 4, au = lane & 15;
+
+    // the current particle's values (a wavefront's items come in particle order: most share them)
+    uint32_t cur_p = 0xffffffffu;
+    int bb[6] = {0, 0, 0, 0, 0, 0};
+    int Lx = 0, Ly = 0, Lz = 0;
+    double px = 0.0, py = 0.0, pz = 0.0, inv_s = 0.0;
+    const R* etab = nullptr;
+    const AxisVals<R, P::NPL>* atab = nullptr;
 
     const int home = static_cast<int>(blockIdx.x % static_cast<unsigned>(ncurs));
     const unsigned long long ugrab = static_cast<unsigned long long>(grab);
@@ -674,48 +816,67 @@ __global__ __launch_bounds__(THREADS) void k_particles(
                     cur_q = q;
                 }
                 const GridGeom& g = pr[q].g;
-                const int* bb = bbox + 6 * static_cast<size_t>(p);
+                if (p != cur_p) {
+                    cur_p = p;
+                    const int* pb = bbox + 6 * static_cast<size_t>(p);
+                    for (int k = 0; k < 6; ++k) bb[k] = pb[k];
+                    Lx = bb[1] - bb[0] + 2;  // particle lattice
+                    Ly = bb[3] - bb[2] + 2;
+                    Lz = bb[5] - bb[4] + 2;
+                    px = x[p];
+                    py = y[p];
+                    pz = z[p];
+                    inv_s = 1.0 / (P::unit_in_h(ctx) * h[p]);  // lattice units
+                    if constexpr (P::HAS_LINES) {
+                        const int npairs_p = Lx * Ly + Lx * Lz + Ly * Lz;
+                        const int64_t lo_p = loffs[p];
+                        etab = reinterpret_cast<const R*>(lines + lo_p);
+                        atab = reinterpret_cast<const AxisVals<R, P::NPL>*>(lines + axis_table_start(lo_p, npairs_p, UNITS));
+                    }
+                }
                 const ItemBox ib = item_box(bb, static_cast<uint32_t>((key >> 32) - pr[q].block_off), g);
                 const int mx = ib.mx, my = ib.my, mz = ib.mz;
                 const int nx = mx + 1, ny = my + 1, nz = mz + 1;  // item lattice
-                // particle lattice
-                const int Lx = bb[1] - bb[0] + 2, Ly = bb[3] - bb[2] + 2, Lz = bb[5] - bb[4] + 2;
                 const int ox = ib.x0 - bb[0], oy = ib.y0 - bb[2], oz = ib.z0 - bb[4];
-                const double inv_s = 1.0 / (P::unit_in_h(ctx) * h[p]);  // lattice units
-                const double px = x[p], py = y[p], pz = z[p];
-                const R* etab = reinterpret_cast<const R*>(lines + loffs[p]);
                 R* out = ms + slab[t];
 
                 // ---- per-axis constants of the item's lattice planes ----
-                if (lane < nx + ny + nz) {
-                    int a = 0, u = lane, n_a = nx;
-                    if (u >= nx) {
-                        u -= nx;
-                        a = 1;
-                        n_a = ny;
-                        if (u >= ny) {
-                            u -= ny;
-                            a = 2;
-                            n_a = nz;
+                {
+                    const int n_a = (ax == 0) ? nx : ((ax == 1) ? ny : nz);
+                    if (ax < 3 && au < n_a) {
+                        R dr, cg2v, pv[P::NPL];
+                        if constexpr (P::HAS_LINES) {
+                            // the item's window of the particle's axis table
+                            const int o_a = (ax == 0) ? ox : ((ax == 1) ? oy : oz);
+                            const int base = (ax == 0) ? 0 : ((ax == 1) ? Lx : Lx + Ly);
+                            const AxisVals<R, P::NPL> e = atab[base + o_a + au];
+                            dr = e.d;
+                            cg2v = e.cg2;
+#pragma unroll
+                            for (int k = 0; k < P::NPL; ++k) pv[k] = e.pv[k];
+                        } else {
+                            // offsets formed as the reference forms vertex coordinates: (x_min + i dx) - x_p
+                            const int g0 = sel3(ax, ib.x0, ib.y0, ib.z0);
+                            const double lo_a = gmin(g, ax), pa = sel3(ax, px, py, pz), st_a = gstep(g, ax);
+                            const double dd = (lo_a + (g0 + au) * st_a - pa) * inv_s;
+                            const double dn = (lo_a + (g0 + au + 1) * st_a - pa) * inv_s;
+                            dr = static_cast<R>(dd);
+                            const R gc = static_cast<R>(gap(dd, dn));
+                            cg2v = gc * gc;
+                            P::plane(ctx, dr < R(0) ? -dr : dr, pv);
                         }
+                        // point test: the gap of the cells on either side of the plane within the item,
+                        // min(gap of cell u - 1, gap of cell u), clamped at the item's first and last plane
+                        const R prev = row_prev(cg2v);  // cell u - 1 (the first plane keeps its own)
+                        const R gp2 = (au == n_a - 1) ? prev : (prev < cg2v ? prev : cg2v);
+                        const int slot = ax * AX_STRIDE + au;
+                        L.sg[slot] = Pair2<R>{dr, gp2};
+#pragma unroll
+                        for (int k = 0; k < P::NPL; ++k) L.pc[k][slot] = pv[k];
+                        L.cg2[slot] = cg2v;
                     }
-                    // offsets formed as the reference forms vertex coordinates: (x_min + i dx) - x_p
-                    const int g0 = sel3(a, ib.x0, ib.y0, ib.z0);
-                    const double lo = gmin(g, a), pa = sel3(a, px, py, pz), st_a = gstep(g, a);
-                    const double d = (lo + (g0 + u) * st_a - pa) * inv_s;
-                    const double dm = (lo + (g0 + max(u - 1, 0)) * st_a - pa) * inv_s;
-                    const double dp = (lo + (g0 + min(u + 1, n_a - 1)) * st_a - pa) * inv_s;
-                    const double dn = (lo + (g0 + u + 1) * st_a - pa) * inv_s;
-                    const R dr = static_cast<R>(d);
-                    R pv[P::NPL];
-                    P::plane(ctx, dr < R(0) ? -dr : dr, pv);
-                    const R gp = static_cast<R>(gap(dm, dp)), gc = static_cast<R>(gap(d, dn));
-                    L.sg[lane] = Pair2<R>{dr, gp * gp};
-    #pragma unroll
-                    for (int q = 0; q < P::NPL; ++q) L.pc[q][lane] = pv[q];
-                    L.cg2[lane] = gc * gc;
                 }
-                c_planes += static_cast<unsigned>(nx + ny + nz);
+                if constexpr (!P::HAS_LINES) c_planes += static_cast<unsigned>(nx + ny + nz);
                 wave_sync();
 
                 // ---- classify the lattice points: zero planes, not needed, inside the support, outside ----
@@ -726,7 +887,7 @@ __global__ __launch_bounds__(THREADS) void k_particles(
                     const int u = u0 + lane;
                     bool isA = false, isB = false;
                     if (u < npts) {
-                        const Pair2<R> ex = L.sg[w.i], ey = L.sg[nx + w.j], ez = L.sg[nx + ny + w.k];
+                        const Pair2<R> ex = L.sg[w.i], ey = L.sg[AX_STRIDE + w.j], ez = L.sg[2 * AX_STRIDE + w.k];
                         if (ex.a == R(0) || ey.a == R(0) || ez.a == R(0)) {
                             L.U[u] = R(0);  // every pyramid anchored in a plane through the particle is flat
                         } else if (!P::SPHERE_SKIP || ex.b + ey.b + ez.b < S2) {
@@ -736,8 +897,7 @@ __global__ __launch_bounds__(THREADS) void k_particles(
                         }
                     }
                     const unsigned long long bA = __ballot(isA), bB = __ballot(isB);
-                    const uint32_t code = static_cast<uint32_t>(u) | (static_cast<uint32_t>(w.i) << 10) |
-                                          (static_cast<uint32_t>(w.j) << 14) | (static_cast<uint32_t>(w.k) << 18);
+                    const uint16_t code = static_cast<uint16_t>(w.i | (w.j << 4) | (w.k << 8));
                     if (isA) L.list[nA + __popcll(bA & lane_lt)] = code;
                     if (isB) L.list[BLK_PTS - 1 - (nB + __popcll(bB & lane_lt))] = code;
                     nA += __popcll(bA);
@@ -751,19 +911,19 @@ __global__ __launch_bounds__(THREADS) void k_particles(
                 const uint32_t ntot = nA + nB;
                 for (uint32_t v = lane; v < ntot; v += 64) {
                     const uint32_t code = (v < nA) ? L.list[v] : L.list[BLK_PTS - 1 - (v - nA)];
-                    const int u = static_cast<int>(code & 1023u);
-                    const int i = static_cast<int>((code >> 10) & 15u), j = static_cast<int>((code >> 14) & 15u),
-                              k = static_cast<int>((code >> 18) & 15u);
-                    const R dx = L.sg[i].a, dy = L.sg[nx + j].a, dz = L.sg[nx + ny + k].a;
-                    const R ax = dx < R(0) ? -dx : dx, ay = dy < R(0) ? -dy : dy, az = dz < R(0) ? -dz : dz;
-                    const R r2 = ax * ax + ay * ay + az * az;
+                    const int i = static_cast<int>(code & 15u), j = static_cast<int>((code >> 4) & 15u),
+                              k = static_cast<int>(code >> 8);
+                    const int u = (i * ny + j) * nz + k;
+                    const R dx = L.sg[i].a, dy = L.sg[AX_STRIDE + j].a, dz = L.sg[2 * AX_STRIDE + k].a;
+                    const R ax_ = dx < R(0) ? -dx : dx, ay = dy < R(0) ? -dy : dy, az = dz < R(0) ? -dz : dz;
+                    const R r2 = ax_ * ax_ + ay * ay + az * az;
                     const R r = KMath<R>::sqrt_(r2);
                     const int reg = P::region(ctx, r2);
                     // lane 0 takes part in every pass of this loop, so its counters hold the wave's totals
                     c_pts[0] += __popcll(__ballot(reg == 1));
                     c_pts[1] += __popcll(__ballot(reg == 2));
                     c_pts[2] += __popcll(__ballot(reg >= 3));
-                    const R csum = L.pc[reg - 1][i] + L.pc[reg - 1][nx + j] + L.pc[reg - 1][nx + ny + k];
+                    const R csum = L.pc[reg - 1][i] + L.pc[reg - 1][AX_STRIDE + j] + L.pc[reg - 1][2 * AX_STRIDE + k];
                     R dsum = R(0);
                     const int slot = P::line_slot(reg);
                     if (slot >= 0) {
@@ -773,7 +933,7 @@ __global__ __launch_bounds__(THREADS) void k_particles(
                                 etab[NLN * static_cast<uint32_t>(exz + gi * Lz + gk) + c]) +
                                etab[NLN * static_cast<uint32_t>(eyz + gj * Lz + gk) + c];
                     }
-                    const R W = P::free_part(ctx, ax, ay, az, r, reg) + P::plane_weight() * csum + dsum;
+                    const R W = P::free_part(ctx, ax_, ay, az, r, reg) + P::plane_weight() * csum + dsum;
                     const bool neg = (dx < R(0)) != ((dy < R(0)) != (dz < R(0)));
                     L.U[u] = neg ? -W : W;
                 }
@@ -785,7 +945,8 @@ __global__ __launch_bounds__(THREADS) void k_particles(
                 for (int u = lane; u < ncells; u += 64, wc.next(my, mz)) {
                     const int i = wc.i, j = wc.j, k = wc.k;
                     double Ms = 0.0;
-                    const bool in_sphere = !P::SPHERE_SKIP || L.cg2[i] + L.cg2[nx + j] + L.cg2[nx + ny + k] < S2;
+                    const bool in_sphere =
+                        !P::SPHERE_SKIP || L.cg2[i] + L.cg2[AX_STRIDE + j] + L.cg2[2 * AX_STRIDE + k] < S2;
                     n_sphere += __popcll(__ballot(in_sphere));  // wave total, held by lane 0 as above
                     if (in_sphere) {
                         const R* U0 = &L.U[(i * ny + j) * nz + k];
@@ -798,12 +959,12 @@ __global__ __launch_bounds__(THREADS) void k_particles(
                             if (Ms <= -0.01) {
                                 ++n_warn;
                                 DevStats* sq = st + cur_q;
-                                const unsigned long long slot = atomicAdd(&sq->n_warn, 1ull);
-                                if (slot < KIWI_MAX_WARN_RECORDS) {
-                                    sq->warn[slot].particle = static_cast<int64_t>(p) - pr[cur_q].part_off;
-                                    sq->warn[slot].cell =
+                                const unsigned long long wslot = atomicAdd(&sq->n_warn, 1ull);
+                                if (wslot < KIWI_MAX_WARN_RECORDS) {
+                                    sq->warn[wslot].particle = static_cast<int64_t>(p) - pr[cur_q].part_off;
+                                    sq->warn[wslot].cell =
                                         (static_cast<int64_t>(ib.x0 + i) * g.ny + (ib.y0 + j)) * g.nz + (ib.z0 + k);
-                                    sq->warn[slot].value = Ms;
+                                    sq->warn[wslot].value = Ms;
                                 }
                             }
                             Ms = 0.0;
@@ -1310,8 +1471,15 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     // blocks wholly outside the support sphere hold only exact zeros and are left out, except on the
     // tabulated path, whose table values do not vanish there
     const int skip_outside = kc.tabulated ? 0 : 1;
+    // the arithmetic of the path: mixed precision applies to the cubic path and to the generic kernels
+    // gen_mixed_ok names; the tabulated path and the other generic kernels always compute in double
+    const bool tabulated = kc.tabulated;
+    const bool fp64 = tabulated || precision == KIWI_FP64 || (generic && !gen_mixed_ok(K->kernel_id));
+    const int ax_units = tabulated ? 0
+                         : generic ? (fp64 ? axis_units<PolicyGen<double>>() : axis_units<PolicyGen<float>>())
+                                   : (fp64 ? axis_units<PolicyM4<double>>() : axis_units<PolicyM4<float>>());
     k_prep<<<grid1d(n1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], dp[6], dp[3], dp[4], dp[5], sup, skip_outside,
-                                          dpr, part_prob, bbox, ap, nblk, nlin, dst);
+                                          ax_units, dpr, part_prob, bbox, ap, nblk, nlin, dst);
     KIWI_HIP_CHECK(c, hipGetLastError());
     {
         size_t tmp_bytes = 0;
@@ -1335,10 +1503,6 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     if (npairs > 0x7fffffffLL)
         return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 (particle, cell-block) items in one call");
 
-    // mixed precision applies to the cubic path and to the generic kernels gen_mixed_ok names; the
-    // tabulated path and the other generic kernels always compute in double precision
-    const bool tabulated = kc.tabulated;
-    const bool fp64 = tabulated || precision == KIWI_FP64 || (generic && !gen_mixed_ok(K->kernel_id));
     const size_t rsz = fp64 ? sizeof(double) : sizeof(float);
     const size_t line_entry = tabulated ? 0
                               : generic ? (fp64 ? sizeof(LineVals<double, GEN_NREG>) : sizeof(LineVals<float, GEN_NREG>))

@@ -218,6 +218,52 @@ KIWI_HD void m4_line(R da, R db, R& D2, R& D3) {
           (pl.c2 - pl.c3) * phi) * M4Consts<R>::inv_pi + D2;
 }
 
+// The line-table entry of one pair of lattice coordinates: m4_line(da, db) + m4_line(db, da), the two
+// offsets summed, with the terms the two share formed once (l2, the crossing distance dc, the logarithm
+// L, p = da db dc).  c12a, c23a are c1 - c2 and c2 - c3 of m4_plane(da), c12b, c23b those of db.
+// Summing the two pyramids' polynomials: X3_ab + X3_ba = da^4 T_ab + db^4 T_ba + da db (r dc + l2 L)
+// (the (l2 + 2 da^2) and (l2 + 2 db^2) weights of L add up to 4 l2), q4_ab + q4_ba = p (3 l2 + 2 dc^2 / 3),
+// and X5 keeps its per-pyramid da^2 X3_ab + db^2 X3_ba with the shared part counted twice.
+template <typename R>
+KIWI_HD void m4_line_pair(R da, R db, R c12a, R c23a, R c12b, R c23b, R& S2, R& S3) {
+    using M = KMath<R>;
+    S2 = R(0);
+    S3 = R(0);
+    if (!(da > R(0)) || !(db > R(0))) return;
+    const R l2 = da * da + db * db;
+    if (!(l2 < R(4))) return;
+    const R inv_l2 = M::rcp(l2);
+    const R inv_da = M::rcp(da), inv_db = M::rcp(db);
+    const R da2 = da * da, db2 = db * db, dab = da * db;
+    const R da4 = da2 * da2, db4 = db2 * db2;
+    if (l2 < R(1)) {
+        const R dc = M::sqrt_(R(1) - l2);
+        const R Tab = M::atan_pos(da * dc * inv_db), Tba = M::atan_pos(db * dc * inv_da);
+        const R L = M::log_((R(1) + dc) * (R(1) + dc) * inv_l2);
+        const R phab = M::atan2_pos(dc, db), phba = M::atan2_pos(dc, da);
+        const R p = dab * dc;
+        const R q4 = p * (R(3) * l2 + R(2.0 / 3.0) * dc * dc);
+        const R sh = R(0.25) * dab * R(2) * dc;  // the L-free shared part of one X3
+        const R X3ab = da4 * Tab + sh + R(0.25) * dab * (l2 + R(2) * da2) * L;
+        const R X3ba = db4 * Tba + sh + R(0.25) * dab * (l2 + R(2) * db2) * L;
+        const R X5 = da2 * X3ab + db2 * X3ba + R(0.125) * dab * (R(2) * dc * (R(5) - R(3) * dc * dc) + R(3) * l2 * l2 * L);
+        S2 = (-p * R(1.0 / 3.0) - R(0.15) * q4 + X5 * R(1.0 / 30.0) + R(0.25) * (X3ab + X3ba) -
+              (Tab + Tba) * R(1.0 / 60.0) + c12a * phab + c12b * phba) * M4Consts<R>::inv_pi;
+    }
+    const R dc = M::sqrt_(R(4) - l2);
+    const R Tab = M::atan_pos(da * dc * inv_db * R(0.5)), Tba = M::atan_pos(db * dc * inv_da * R(0.5));
+    const R L = M::log_((R(2) + dc) * (R(2) + dc) * inv_l2);
+    const R phab = M::atan2_pos(dc, db), phba = M::atan2_pos(dc, da);
+    const R p = dab * dc;
+    const R q4 = p * (R(3) * l2 + R(2.0 / 3.0) * dc * dc);
+    const R sh = R(0.25) * dab * R(4) * dc;
+    const R X3ab = da4 * Tab + sh + R(0.25) * dab * (l2 + R(2) * da2) * L;
+    const R X3ba = db4 * Tba + sh + R(0.25) * dab * (l2 + R(2) * db2) * L;
+    const R X5 = da2 * X3ab + db2 * X3ba + R(0.125) * dab * (R(4) * dc * (R(20) - R(3) * dc * dc) + R(3) * l2 * l2 * L);
+    S3 = (p * R(2.0 / 3.0) + R(0.075) * q4 - X5 * R(1.0 / 120.0) - R(0.25) * (X3ab + X3ba) +
+          (Tab + Tba) * R(1.0 / 60.0 + 0.25) + c23a * phab + c23b * phba) * M4Consts<R>::inv_pi + S2;
+}
+
 // m4_line written as the difference of two full vertex integrals (the literal reading of
 // compute_edge_constants); the checked reference for the shared-polynomial form above.
 template <typename R>

@@ -58,6 +58,28 @@ void lat_line_both(double da, double db, double* out) {
     m4_line_by_vertex<double>(da, db, out[2], out[3]);
 }
 
+}  // extern "C"
+
+// the fused pair entry and the two separate offsets it sums, in double (out[0..3]) and float (out[4..7])
+template <typename R>
+static void line_pair(R da, R db, double* out) {
+    const M4Plane<R> pa = m4_plane<R>(da), pb = m4_plane<R>(db);
+    R s2, s3, ab2, ab3, ba2, ba3;
+    m4_line_pair<R>(da, db, pa.c1 - pa.c2, pa.c2 - pa.c3, pb.c1 - pb.c2, pb.c2 - pb.c3, s2, s3);
+    m4_line<R>(da, db, ab2, ab3);
+    m4_line<R>(db, da, ba2, ba3);
+    out[0] = s2;
+    out[1] = s3;
+    out[2] = static_cast<double>(ab2) + static_cast<double>(ba2);
+    out[3] = static_cast<double>(ab3) + static_cast<double>(ba3);
+}
+extern "C" {
+
+void lat_line_pair(double da, double db, double* out) {
+    line_pair<double>(da, db, out);
+    line_pair<float>(static_cast<float>(da), static_cast<float>(db), out + 4);
+}
+
 double lat_box(const double* p, double h, const double* lo, const double* hi, int single) {
     return single ? lattice_box<float>(p, h, lo, hi) : lattice_box<double>(p, h, lo, hi);
 }
