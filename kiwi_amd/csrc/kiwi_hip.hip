@@ -776,6 +776,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(P::WAVE
             if (n_warn) atomicAdd(&sq->n_negative_warn, n_warn);
             if (n_sphere) atomicAdd(&sq->pairs_sphere, n_sphere);
             if (c_planes) atomicAdd(&sq->planes, c_planes);
+            c_pts[1] -= c_pts[0] + c_pts[2];  // c_pts[1] counted every evaluated point
             for (int r = 0; r < 3; ++r)
                 if (c_pts[r]) atomicAdd(&sq->pts[r], c_pts[r]);
         }
@@ -879,23 +880,17 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(P::WAVE
                 if constexpr (!P::HAS_LINES) c_planes += static_cast<unsigned>(nx + ny + nz);
                 wave_sync();
 
-                // ---- classify the lattice points: zero planes, not needed, inside the support, outside ----
+                // ---- classify the lattice points: not needed, inside the support, outside ----
+                // (points in a plane through the particle are listed like the others; their value is 0)
                 const int nyz = ny * nz, npts = nx * nyz;
                 uint32_t nA = 0, nB = 0;
                 BoxWalk w(lane, ny, nz);
                 for (int u0 = 0; u0 < npts; u0 += 64, w.next(ny, nz)) {
                     const int u = u0 + lane;
-                    bool isA = false, isB = false;
-                    if (u < npts) {
-                        const Pair2<R> ex = L.sg[w.i], ey = L.sg[AX_STRIDE + w.j], ez = L.sg[2 * AX_STRIDE + w.k];
-                        if (ex.a == R(0) || ey.a == R(0) || ez.a == R(0)) {
-                            L.U[u] = R(0);  // every pyramid anchored in a plane through the particle is flat
-                        } else if (!P::SPHERE_SKIP || ex.b + ey.b + ez.b < S2) {
-                            const R r2 = ex.a * ex.a + ey.a * ey.a + ez.a * ez.a;
-                            isA = r2 < S2;
-                            isB = !isA;
-                        }
-                    }
+                    const Pair2<R> ex = L.sg[w.i], ey = L.sg[AX_STRIDE + w.j], ez = L.sg[2 * AX_STRIDE + w.k];
+                    const bool needed = u < npts && (!P::SPHERE_SKIP || ex.b + ey.b + ez.b < S2);
+                    const R r2 = ex.a * ex.a + ey.a * ey.a + ez.a * ez.a;
+                    const bool isA = needed && r2 < S2, isB = needed && !(r2 < S2);
                     const unsigned long long bA = __ballot(isA), bB = __ballot(isB);
                     const uint16_t code = static_cast<uint16_t>(w.i | (w.j << 4) | (w.k << 8));
                     if (isA) L.list[nA + __popcll(bA & lane_lt)] = code;
@@ -920,8 +915,8 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(P::WAVE
                     const R r = KMath<R>::sqrt_(r2);
                     const int reg = P::region(ctx, r2);
                     // lane 0 takes part in every pass of this loop, so its counters hold the wave's totals
+                    // (region 2 is the rest: c_pts[1] gets the item's total below)
                     c_pts[0] += __popcll(__ballot(reg == 1));
-                    c_pts[1] += __popcll(__ballot(reg == 2));
                     c_pts[2] += __popcll(__ballot(reg >= 3));
                     const R csum = L.pc[reg - 1][i] + L.pc[reg - 1][AX_STRIDE + j] + L.pc[reg - 1][2 * AX_STRIDE + k];
                     R dsum = R(0);
@@ -935,8 +930,11 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(P::WAVE
                     }
                     const R W = P::free_part(ctx, ax_, ay, az, r, reg) + P::plane_weight() * csum + dsum;
                     const bool neg = (dx < R(0)) != ((dy < R(0)) != (dz < R(0)));
-                    L.U[u] = neg ? -W : W;
+                    // every pyramid anchored in a plane through the particle is flat
+                    const bool flat = dx == R(0) || dy == R(0) || dz == R(0);
+                    L.U[u] = flat ? R(0) : (neg ? -W : W);
                 }
+                c_pts[1] += ntot;  // less the region-1 and region-3 points counted above (flush)
                 wave_sync();
 
                 // ---- cell sums: the 8-corner stencil, stored to the item's slab ----
