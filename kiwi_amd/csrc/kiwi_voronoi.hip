@@ -44,6 +44,9 @@ namespace {
 constexpr double BUCKET_CELLS = 2.0;  // mean cells per bucket of the candidate grid
 constexpr int MAX_BUCKET_AXIS = 1024;
 constexpr double SUPPORT_MARGIN = 1.0 + 1e-7;  // candidate test slack over the exact face test
+// Lanes per (particle, cell) pair in the per-pair kernels (a quarter wavefront): the pair's faces are
+// taken 16 at a time, so that the face arrays and the pair's slots are read and written contiguously.
+constexpr int PAIR_LANES = 16;
 
 struct MapDevStats {
     unsigned long long unresolved, unmapped, n_negative, n_negative_warn;
@@ -155,6 +158,51 @@ __global__ __launch_bounds__(THREADS) void k_vor_twins(int64_t nf, const int64_t
     twin[f] = t;
 }
 
+// Per-face bounds for the visit test, one cache line per face: the unit normal and offset of the plane
+// through the first three vertices (n . x + off is the signed plane distance; all zero for a face without
+// a plane), and a bounding sphere of the vertices (their centroid and largest distance from it).
+struct alignas(64) FaceGeo {
+    double nx, ny, nz, off;
+    double cx, cy, cz, rad;
+};
+
+__global__ __launch_bounds__(THREADS) void k_vor_face_geo(int64_t nf, const int64_t* __restrict__ fvp,
+                                                          const double* __restrict__ vxyz, FaceGeo* __restrict__ fg) {
+    const int64_t f = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
+    if (f >= nf) return;
+    const int64_t v0 = fvp[f], n = fvp[f + 1] - v0;
+    const double* v = vxyz + 3 * v0;
+    FaceGeo g = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, INFINITY};
+    if (n >= 3) {
+        const double ax = v[3] - v[0], ay = v[4] - v[1], az = v[5] - v[2];
+        const double bx = v[6] - v[0], by = v[7] - v[1], bz = v[8] - v[2];
+        const double A = ay * bz - az * by, B = az * bx - ax * bz, C = ax * by - ay * bx;
+        const double nrm = sqrt(A * A + B * B + C * C);
+        if (nrm > 0.0) {
+            g.nx = A / nrm;
+            g.ny = B / nrm;
+            g.nz = C / nrm;
+            g.off = -(g.nx * v[0] + g.ny * v[1] + g.nz * v[2]);
+        }
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int64_t k = 0; k < n; ++k) {
+            sx += v[3 * k];
+            sy += v[3 * k + 1];
+            sz += v[3 * k + 2];
+        }
+        g.cx = sx / static_cast<double>(n);
+        g.cy = sy / static_cast<double>(n);
+        g.cz = sz / static_cast<double>(n);
+        double r2 = 0.0;
+        for (int64_t k = 0; k < n; ++k) {
+            const double dx = v[3 * k] - g.cx, dy = v[3 * k + 1] - g.cy, dz = v[3 * k + 2] - g.cz;
+            r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+        }
+        g.rad = sqrt(r2);
+    }
+    fg[f] = g;
+}
+
 // ------------------------------------------------------------------------------------------------
 // 2. bucket grid over the cell boxes
 // ------------------------------------------------------------------------------------------------
@@ -172,9 +220,15 @@ __device__ __forceinline__ bool cell_bucket_range(const BucketGrid& B, const dou
 }
 
 // entries per cell, and the cell's lowest bucket per axis packed 10 bits each (x | y << 10 | z << 20)
+// What the candidate test reads of a cell, in one cache line: its box and lowest bucket per axis.
+struct alignas(64) CellRec {
+    double lo[3], hi[3];
+    int blo;  // x | y << 10 | z << 20
+};
+
 __global__ __launch_bounds__(THREADS) void k_vor_bucket_count(int64_t nc, BucketGrid B,
                                                               const double* __restrict__ box,
-                                                              int64_t* __restrict__ cnt, int* __restrict__ cell_blo) {
+                                                              int64_t* __restrict__ cnt, CellRec* __restrict__ crec) {
     const int64_t c = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
     if (c > nc) return;
     int64_t k = 0;
@@ -182,7 +236,13 @@ __global__ __launch_bounds__(THREADS) void k_vor_bucket_count(int64_t nc, Bucket
     if (c < nc) {
         if (cell_bucket_range(B, box, c, blo, bhi))
             k = static_cast<int64_t>(bhi[0] - blo[0] + 1) * (bhi[1] - blo[1] + 1) * (bhi[2] - blo[2] + 1);
-        cell_blo[c] = blo[0] | (blo[1] << 10) | (blo[2] << 20);
+        CellRec r;
+        for (int a = 0; a < 3; ++a) {
+            r.lo[a] = box[6 * c + a];
+            r.hi[a] = box[6 * c + 3 + a];
+        }
+        r.blo = blo[0] | (blo[1] << 10) | (blo[2] << 20);
+        crec[c] = r;
     }
     cnt[c] = k;  // cnt[nc] = 0 closes the scan
 }
@@ -232,7 +292,7 @@ template <bool FILL>
 __global__ __launch_bounds__(THREADS) void k_vor_candidates(
     int64_t n, const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     const double* __restrict__ ph, const int* __restrict__ home, int64_t nc, double sup, BucketGrid B,
-    const double* __restrict__ box, const int* __restrict__ cell_blo, const uint64_t* __restrict__ bkeys,
+    const CellRec* __restrict__ crec, const uint64_t* __restrict__ bkeys,
     const int* __restrict__ boff, int64_t* __restrict__ cnt, const int64_t* __restrict__ off,
     int* __restrict__ cand_cell, int* __restrict__ cand_part, MapDevStats* st) {
     const int lane = static_cast<int>(threadIdx.x & 63);
@@ -270,14 +330,14 @@ __global__ __launch_bounds__(THREADS) void k_vor_candidates(
                         const uint64_t key = bkeys[e];
                         c = static_cast<int>(key & 0xffffffffu);
                         const int z = static_cast<int>(key >> 32) - row;
-                        const int cb = cell_blo[c];
+                        const CellRec& cr = crec[c];
+                        const int cb = cr.blo;
                         // lowest shared bucket per axis
                         if (max(cb & 1023, q0) == x && max((cb >> 10) & 1023, q1) == y && max(cb >> 20, q2) == z) {
-                            const double* bx = box + 6 * static_cast<int64_t>(c);
                             // rounded operations: the count and fill passes take identical decisions
-                            const double g0 = gap(bx[0] - p0, bx[3] - p0);
-                            const double g1 = gap(bx[1] - p1, bx[4] - p1);
-                            const double g2 = gap(bx[2] - p2, bx[5] - p2);
+                            const double g0 = gap(cr.lo[0] - p0, cr.hi[0] - p0);
+                            const double g1 = gap(cr.lo[1] - p1, cr.hi[1] - p1);
+                            const double g2 = gap(cr.lo[2] - p2, cr.hi[2] - p2);
                             const double d2 = add_sep(add_sep(mul_sep(g0, g0), mul_sep(g1, g1)),
                                                         mul_sep(g2, g2));
                             is_home = (c == hc);
@@ -344,6 +404,27 @@ __device__ __forceinline__ bool face_reachable(double x0, double y0, double z0, 
     return r;
 }
 
+// A face whose plane or bounding sphere lies beyond the support is skipped without its vertices: the
+// bound is widened by 1e-9 H plus 1e-12 of the coordinates' magnitude, far above the rounding of either
+// distance, so a skipped face is one the exact test rejects too.  Otherwise one of its first three
+// vertices inside the support by a relative margin of 1e-6 on the squared distance makes it reachable (its
+// distance is at most that vertex's), and the rest take the exact test.
+__device__ __forceinline__ bool face_reachable_bounded(double x0, double y0, double z0, double H, double slack,
+                                                       const FaceGeo& g, const double* v, int nvf) {
+    if (fabs(g.nx * x0 + g.ny * y0 + g.nz * z0 + g.off) >= H + slack + 1e-12 * fabs(g.off)) return false;
+    const double dx = x0 - g.cx, dy = y0 - g.cy, dz = z0 - g.cz;
+    if (sqrt(dx * dx + dy * dy + dz * dz) - g.rad >= H + slack + 1e-12 * g.rad) return false;
+    if (nvf >= 3) {
+        const double in2 = H * H * (1.0 - 1e-6);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double ex = v[3 * k] - x0, ey = v[3 * k + 1] - y0, ez = v[3 * k + 2] - z0;
+            if (ex * ex + ey * ey + ez * ez < in2) return true;
+        }
+    }
+    return face_reachable(x0, y0, z0, H, v, nvf);
+}
+
 __global__ __launch_bounds__(THREADS) void k_vor_reach(int64_t ncand, const int* __restrict__ cand_cell,
                                                        const int* __restrict__ cand_part,
                                                        const double* __restrict__ px,
@@ -352,7 +433,8 @@ __global__ __launch_bounds__(THREADS) void k_vor_reach(int64_t ncand, const int*
                                                        const double* __restrict__ ph, const int* __restrict__ home,
                                                        double sup, const int64_t* __restrict__ cfp,
                                                        const int64_t* __restrict__ fvp,
-                                                       const double* __restrict__ vxyz, int64_t* __restrict__ flag) {
+                                                       const double* __restrict__ vxyz,
+                                                       const FaceGeo* __restrict__ fg, int64_t* __restrict__ flag) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
     if (t > ncand) return;
     if (t == ncand) {
@@ -363,9 +445,10 @@ __global__ __launch_bounds__(THREADS) void k_vor_reach(int64_t ncand, const int*
     bool vis = (home[i] == c);
     if (!vis) {
         const double x0 = px[i], y0 = py[i], z0 = pz[i], H = sup * ph[i];
+        const double slack = 1e-9 * H + 1e-12 * (fabs(x0) + fabs(y0) + fabs(z0));
         for (int64_t f = cfp[c]; f < cfp[c + 1] && !vis; ++f) {
             const int64_t v0 = fvp[f];
-            vis = face_reachable(x0, y0, z0, H, vxyz + 3 * v0, static_cast<int>(fvp[f + 1] - v0));
+            vis = face_reachable_bounded(x0, y0, z0, H, slack, fg[f], vxyz + 3 * v0, static_cast<int>(fvp[f + 1] - v0));
         }
     }
     flag[t] = vis ? 1 : 0;
@@ -420,6 +503,28 @@ __device__ __forceinline__ int64_t find_key(const uint64_t* __restrict__ keys, i
     return (lo < e && keys[lo] == key) ? lo : -1;
 }
 
+// The pairs a block's searches can reach, [first pair of the block's first particle, end of the block),
+// are staged in LDS when they fit OWN_LDS keys; the binary searches then read LDS.
+constexpr int OWN_LDS = 2048;
+
+__device__ __forceinline__ int64_t find_key_lds(const uint64_t* __restrict__ sk, int64_t base, int64_t lo, int64_t e,
+                                                uint64_t key) {
+    int64_t l = lo - base, h = e - base;
+    const int64_t end = h;
+    while (l < h) {
+        const int64_t mid = (l + h) >> 1;
+        if (sk[mid] < key) l = mid + 1;
+        else h = mid;
+    }
+    return (l < end && sk[l] == key) ? l + base : -1;
+}
+
+// sum over the 16 lanes of a quarter wavefront
+__device__ __forceinline__ int64_t row_sum(int64_t v) {
+    for (int m = 1; m < PAIR_LANES; m <<= 1) v += __shfl_xor(v, m, PAIR_LANES);
+    return v;
+}
+
 // src[slot]: the slot itself for a face this pair integrates, ~(owner's slot) for a face shared with a
 // lower-index visited cell of the same particle (integrated there, entered here negated)
 __global__ __launch_bounds__(THREADS) void k_vor_owned(int64_t nvis, const uint64_t* __restrict__ keys, int cbits,
@@ -428,10 +533,25 @@ __global__ __launch_bounds__(THREADS) void k_vor_owned(int64_t nvis, const uint6
                                                        const int64_t* __restrict__ cfp, const int* __restrict__ fnb,
                                                        const int* __restrict__ twin, int64_t* __restrict__ src,
                                                        int64_t* __restrict__ nown) {
-    const int64_t q = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
+    constexpr int PAIRS = THREADS / PAIR_LANES;
+    __shared__ uint64_t sk[OWN_LDS];
+    const int64_t q0 = static_cast<int64_t>(blockIdx.x) * PAIRS;
+    const int64_t q = q0 + threadIdx.x / PAIR_LANES;
+    const int r = static_cast<int>(threadIdx.x % PAIR_LANES);
+    // the block's search range (uniform): every search of pair q lies in [vis_off[i(q)], q)
+    const int64_t qe = min(q0 + PAIRS, nvis);
+    int64_t base = 0;
+    bool staged = false;
+    if (q0 < nvis) {
+        base = vis_off[static_cast<int64_t>(keys[q0] >> cbits)];
+        staged = qe - base <= OWN_LDS;
+        if (staged)
+            for (int64_t k = base + threadIdx.x; k < qe; k += THREADS) sk[k - base] = keys[k];
+    }
+    __syncthreads();
     if (q > nvis) return;
     if (q == nvis) {
-        nown[q] = 0;
+        if (r == 0) nown[q] = 0;
         return;
     }
     const uint64_t cmask = (1ull << cbits) - 1;
@@ -441,19 +561,23 @@ __global__ __launch_bounds__(THREADS) void k_vor_owned(int64_t nvis, const uint6
     const int64_t i = static_cast<int64_t>(key >> cbits);
     const int64_t lo = vis_off[i];
     const int64_t f0 = cfp[c], f1 = cfp[c + 1];
-    int64_t slot = slot_off[q], owned = 0;
-    for (int64_t f = f0; f < f1; ++f, ++slot) {
+    const int64_t slot0 = slot_off[q];
+    int64_t owned = 0;
+    for (int64_t f = f0 + r; f < f1; f += PAIR_LANES) {
+        const int64_t slot = slot0 + (f - f0);
         const int nb = fnb[f], tw = twin[f];
         int64_t s = slot;
         if (nb >= 0 && nb < c && tw >= 0) {
             // the neighbour's pair sorts before this one
-            const int64_t q2 = find_key(keys, lo, q, pbase | static_cast<uint32_t>(nb));
+            const uint64_t k2 = pbase | static_cast<uint32_t>(nb);
+            const int64_t q2 = staged ? find_key_lds(sk, base, lo, q, k2) : find_key(keys, lo, q, k2);
             if (q2 >= 0) s = ~(slot_off[q2] + (tw - cfp[nb]));
         }
         src[slot] = s;
         owned += (s >= 0) ? 1 : 0;
     }
-    nown[q] = owned;
+    owned = row_sum(owned);
+    if (r == 0) nown[q] = owned;
 }
 
 struct FaceItem {
@@ -461,18 +585,37 @@ struct FaceItem {
     int face;  // global face index
 };
 
+// the pair's owned faces in face order, 16 at a time: each lane's rank among the owned faces of its round
+// comes from the quarter wavefront's ballot bits
 __global__ __launch_bounds__(THREADS) void k_vor_items(int64_t nvis, const uint64_t* __restrict__ keys,
                                                        uint64_t cmask, const int64_t* __restrict__ slot_off,
                                                        const int64_t* __restrict__ own_off,
                                                        const int64_t* __restrict__ cfp,
                                                        const int64_t* __restrict__ src, FaceItem* __restrict__ items) {
-    const int64_t q = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
-    if (q >= nvis) return;
-    const int64_t c = static_cast<int64_t>(keys[q] & cmask);
-    const int64_t f0 = cfp[c], f1 = cfp[c + 1];
-    int64_t slot = slot_off[q], o = own_off[q];
-    for (int64_t f = f0; f < f1; ++f, ++slot)
-        if (src[slot] >= 0) items[o++] = FaceItem{static_cast<int>(q), static_cast<int>(f)};
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
+    const int64_t q = gid / PAIR_LANES;
+    const int r = static_cast<int>(gid % PAIR_LANES);
+    const int shift = static_cast<int>(threadIdx.x & 63) & ~(PAIR_LANES - 1);  // the row's first lane
+    const bool live = q < nvis;
+    int64_t f0 = 0, f1 = 0, slot0 = 0, o = 0;
+    if (live) {
+        const int64_t c = static_cast<int64_t>(keys[q] & cmask);
+        f0 = cfp[c];
+        f1 = cfp[c + 1];
+        slot0 = slot_off[q];
+        o = own_off[q];
+    }
+    // every lane of the wavefront takes part in every round (the ballots), live or not
+    int64_t nf = f1 - f0;
+    for (int off = 1; off < 64; off <<= 1) nf = max(nf, static_cast<int64_t>(__shfl_xor(nf, off)));
+    for (int64_t k = 0; k < nf; k += PAIR_LANES) {
+        const int64_t f = f0 + k + r;
+        const bool own = live && f < f1 && src[slot0 + k + r] >= 0;
+        const uint64_t b = __ballot(own);
+        const uint32_t rowbits = static_cast<uint32_t>((b >> shift) & 0xffffull);
+        if (own) items[o + __popc(rowbits & ((1u << r) - 1u))] = FaceItem{static_cast<int>(q), static_cast<int>(f)};
+        o += __popc(rowbits);
+    }
 }
 
 // Face items grouped by vertex count (a stable counting sort), so that the lanes of a wavefront walk
@@ -585,13 +728,19 @@ __global__ __launch_bounds__(THREADS) void k_vor_pairsum(int64_t nvis, const uin
                                                          const double* __restrict__ val, double* __restrict__ ms,
                                                          uint64_t* __restrict__ ckeys, int* __restrict__ cidx,
                                                          MapDevStats* st, int clamp_pairs) {
-    const int64_t q = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
-    if (q >= nvis) return;
+    // 16 lanes per pair: lane r adds slots r, r + 16, ... of the pair, then the quarter wavefront adds its
+    // lanes in a fixed butterfly order (deterministic)
+    const int64_t gid = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
+    const int64_t q = gid / PAIR_LANES;
+    const int r = static_cast<int>(gid % PAIR_LANES);
+    if (q >= nvis) return;  // whole quarter wavefronts leave together
     double m = 0.0;
-    for (int64_t s = slot_off[q]; s < slot_off[q + 1]; ++s) {
+    for (int64_t s = slot_off[q] + r; s < slot_off[q + 1]; s += PAIR_LANES) {
         const int64_t k = src[s];
         m += (k >= 0) ? val[k] : -val[~k];
     }
+    for (int msk = 1; msk < PAIR_LANES; msk <<= 1) m += __shfl_xor(m, msk, PAIR_LANES);
+    if (r != 0) return;
     const uint64_t key = keys[q];
     const uint64_t i = key >> cbits, c = key & ((1ull << cbits) - 1);
     // src/mapping.cpp:198-203; mixed precision clamps cell totals instead (k_vor_accum): a per-pair clamp
@@ -844,11 +993,13 @@ int voronoi_run(kiwi_hip_ctx* c, const kiwi_particles* P, const int* home_cell, 
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[0], s));
 
     // 1. geometry
-    KIWI_HIP_CHECK(c, reserve(c->vor[2], bytes_of({6 * sizeof(double) * nc, sizeof(int) * nf, sizeof(int) * nf})));
+    KIWI_HIP_CHECK(c, reserve(c->vor[2], bytes_of({6 * sizeof(double) * nc, sizeof(int) * nf, sizeof(int) * nf,
+                                                   sizeof(FaceGeo) * nf})));
     size_t o2 = 0;
     double* box = at_offset<double>(c->vor[2].p, o2, 6 * nc);
     int* face_cell = at_offset<int>(c->vor[2].p, o2, nf);
     int* twin = at_offset<int>(c->vor[2].p, o2, nf);
+    FaceGeo* fgeo = at_offset<FaceGeo>(c->vor[2].p, o2, nf);
     if (nc > 0) k_vor_cells<<<grid1d(nc), THREADS, 0, s>>>(nc, nf, nv, cfp, fvp, vxyz, fnb, box, face_cell, dst);
     KIWI_HIP_CHECK(c, hipGetLastError());
     {
@@ -856,7 +1007,10 @@ int voronoi_run(kiwi_hip_ctx* c, const kiwi_particles* P, const int* home_cell, 
         KIWI_HIP_CHECK(c, read_back(&bad, &dst->bad_geometry, s));
         if (bad) return fail(c, KIWI_ERR_INVALID_ARG, "malformed tessellation (face or vertex ranges, neighbours)");
     }
-    if (nf > 0) k_vor_twins<<<grid1d(nf), THREADS, 0, s>>>(nf, cfp, fnb, face_cell, twin);
+    if (nf > 0) {
+        k_vor_twins<<<grid1d(nf), THREADS, 0, s>>>(nf, cfp, fnb, face_cell, twin);
+        k_vor_face_geo<<<grid1d(nf), THREADS, 0, s>>>(nf, fvp, vxyz, fgeo);
+    }
     KIWI_HIP_CHECK(c, hipGetLastError());
 
     // 2. bucket grid over the domain
@@ -874,13 +1028,13 @@ int voronoi_run(kiwi_hip_ctx* c, const kiwi_particles* P, const int* home_cell, 
     const int bbits = bits_for(nbuckets);
     if (nbuckets > 0x7ffffffeLL) return fail(c, KIWI_ERR_LIMIT, "bucket grid too large");
     KIWI_HIP_CHECK(c, reserve(c->vor[3], bytes_of({sizeof(int64_t) * (nc + 1), sizeof(int64_t) * (nc + 1),
-                                                   sizeof(int) * (nbuckets + 1), sizeof(int) * nc})));
+                                                   sizeof(int) * (nbuckets + 1), sizeof(CellRec) * nc})));
     size_t o3 = 0;
     int64_t* bcnt = at_offset<int64_t>(c->vor[3].p, o3, nc + 1);
     int64_t* boff = at_offset<int64_t>(c->vor[3].p, o3, nc + 1);
     int* bucket_off = at_offset<int>(c->vor[3].p, o3, nbuckets + 1);
-    int* cell_blo = at_offset<int>(c->vor[3].p, o3, nc);
-    k_vor_bucket_count<<<grid1d(nc + 1), THREADS, 0, s>>>(nc, B, box, bcnt, cell_blo);
+    CellRec* crec = at_offset<CellRec>(c->vor[3].p, o3, nc);
+    k_vor_bucket_count<<<grid1d(nc + 1), THREADS, 0, s>>>(nc, B, box, bcnt, crec);
     KIWI_HIP_CHECK(c, hipGetLastError());
     KIWI_HIP_CHECK(c, scan_i64(c, bcnt, boff, nc + 1, s));
     int64_t nentries = 0;
@@ -907,7 +1061,7 @@ int voronoi_run(kiwi_hip_ctx* c, const kiwi_particles* P, const int* home_cell, 
     int64_t* coff = at_offset<int64_t>(c->vor[5].p, o5, n + 1);
     int64_t* vis_off = at_offset<int64_t>(c->vor[5].p, o5, n + 1);
     k_vor_candidates<false><<<grid1d(64 * (n + 1)), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], home, nc, sup, B,
-                                                                      box, cell_blo, bkeys, bucket_off, ccnt, nullptr,
+                                                                      crec, bkeys, bucket_off, ccnt, nullptr,
                                                                       nullptr, nullptr, dst);
     KIWI_HIP_CHECK(c, hipGetLastError());
     KIWI_HIP_CHECK(c, scan_i64(c, ccnt, coff, n + 1, s));
@@ -926,8 +1080,8 @@ int voronoi_run(kiwi_hip_ctx* c, const kiwi_particles* P, const int* home_cell, 
     int64_t* flag = at_offset<int64_t>(c->vor[6].p, o6, ncand + 1);
     int64_t* flag_off = at_offset<int64_t>(c->vor[6].p, o6, ncand + 1);
     if (n > 0)
-        k_vor_candidates<true><<<grid1d(64 * n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], home, nc, sup, B, box,
-                                                                  cell_blo, bkeys, bucket_off, nullptr, coff, cand_cell,
+        k_vor_candidates<true><<<grid1d(64 * n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], home, nc, sup, B, crec,
+                                                                  bkeys, bucket_off, nullptr, coff, cand_cell,
                                                                   cand_part, dst);
     KIWI_HIP_CHECK(c, hipGetLastError());
     {
@@ -938,7 +1092,8 @@ int voronoi_run(kiwi_hip_ctx* c, const kiwi_particles* P, const int* home_cell, 
 
     // 4. visits
     k_vor_reach<<<grid1d(ncand + 1), THREADS, 0, s>>>(ncand, cand_cell, cand_part, dp[0], dp[1], dp[2], dp[7], home,
-                                                      sup, cfp, fvp, vxyz, flag);
+                                                      sup, cfp, fvp, vxyz, fgeo, flag);
+
     KIWI_HIP_CHECK(c, hipGetLastError());
     KIWI_HIP_CHECK(c, scan_i64(c, flag, flag_off, ncand + 1, s));
     k_vor_particle_visits<<<grid1d(n + 1), THREADS, 0, s>>>(n, coff, flag_off, vis_off, dst);
@@ -972,7 +1127,8 @@ int voronoi_run(kiwi_hip_ctx* c, const kiwi_particles* P, const int* home_cell, 
     size_t o8 = 0;
     int64_t* src = at_offset<int64_t>(c->vor[8].p, o8, nslots);
     double* val = at_offset<double>(c->vor[8].p, o8, nslots);
-    k_vor_owned<<<grid1d(nvis + 1), THREADS, 0, s>>>(nvis, vkeys, cbits, vis_off, slot_off, cfp, fnb, twin, src, nown);
+    k_vor_owned<<<grid1d(PAIR_LANES * (nvis + 1)), THREADS, 0, s>>>(nvis, vkeys, cbits, vis_off, slot_off, cfp, fnb,
+                                                                     twin, src, nown);
     KIWI_HIP_CHECK(c, hipGetLastError());
     KIWI_HIP_CHECK(c, scan_i64(c, nown, own_off, nvis + 1, s));
     int64_t nitems = 0;
@@ -988,7 +1144,8 @@ int voronoi_run(kiwi_hip_ctx* c, const kiwi_particles* P, const int* home_cell, 
     int64_t* bin_cnt = at_offset<int64_t>(c->vor[9].p, o9, NV_BINS * nchunks + 1);
     int64_t* bin_off = at_offset<int64_t>(c->vor[9].p, o9, NV_BINS * nchunks + 1);
     if (nvis > 0)
-        k_vor_items<<<grid1d(nvis), THREADS, 0, s>>>(nvis, vkeys, cmask, slot_off, own_off, cfp, src, items_pair);
+        k_vor_items<<<grid1d(PAIR_LANES * nvis), THREADS, 0, s>>>(nvis, vkeys, cmask, slot_off, own_off, cfp, src,
+                                                                  items_pair);
     KIWI_HIP_CHECK(c, hipGetLastError());
     if (nitems > 0) {
         k_fill_i64<<<1, 64, 0, s>>>(bin_cnt + NV_BINS * nchunks, 1, 0);
@@ -1038,7 +1195,7 @@ int voronoi_run(kiwi_hip_ctx* c, const kiwi_particles* P, const int* home_cell, 
     int* cbeg = at_offset<int>(c->vor[10].p, o10, 2 * nc);
     int* cend = cbeg + nc;
     if (nvis > 0) {
-        k_vor_pairsum<<<grid1d(nvis), THREADS, 0, s>>>(nvis, vkeys, cbits, pbits, slot_off, src, val, ms, ckeys_in,
+        k_vor_pairsum<<<grid1d(PAIR_LANES * nvis), THREADS, 0, s>>>(nvis, vkeys, cbits, pbits, slot_off, src, val, ms, ckeys_in,
                                                        cidx_in, dst, clamp_pairs ? 1 : 0);
         KIWI_HIP_CHECK(c, hipGetLastError());
         if (!inv) KIWI_HIP_CHECK(c, sort_pairs(c, ckeys_in, ckeys, cidx_in, cidx, nvis, 0, cbits + pbits, s));
