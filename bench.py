@@ -14,13 +14,17 @@ The metric is cell-integrals/s: (particle, cell) pairs whose cell intersects the
 sphere, divided by the step time.  `value` is the mixed-precision rate; the FP64 rate, the speed ratio
 and the mixed-vs-FP64 relative total-mass error are reported in the same line.
 
-roofline: the cell-integral kernel (k_particles: lattice values and per-pair cell sums), VALU-bound.
-`achieved` is the operations the kernel executes per launch -- the per-unit counts of
+Mixed precision computes particles whose support radius spans at least the FP64-fallback threshold in
+cells (kiwi_hip_set_fp64_fallback; default 16) in FP64, in a second cell kernel beside the FP32 one; the
+line also reports the same mixed step without the fallback.
+
+roofline: the FP32 cell-integral kernel (k_particles<float>: lattice values and per-pair cell sums),
+VALU-bound.  `achieved` is the operations the kernel executes per launch -- the per-unit counts of
 kiwi_amd/flops_per_unit.json (kiwi_amd/csrc/flop_count.cpp instantiates the kernel's lattice math over
 an operation-counting scalar; +, -, x, /, sqrt and each transcendental count 1, an FMA 2) times the
-launch's work counters (lattice values by region, wall planes, cells) -- divided by the kernel's
-duration, measured with HIP events on the stream the kernel runs on; `frac` = achieved / the FP32
-vector peak (157.3 TFLOP/s).  `ref_equiv` is a different quantity, kept apart: the reference
+launch's work counters (lattice values by region and cells, less the FP64 fallback's) -- divided by the
+kernel's duration, measured with HIP events on the stream the kernel runs on; `frac` = achieved / the
+FP32 vector peak (157.3 TFLOP/s).  `ref_equiv` is a different quantity, kept apart: the reference
 algorithm's operation count for the same work (the pinned mean per bounding-box pair of
 kiwi_amd/flops_per_pair.json, from oracle/flops_per_pair.py) over the same duration; the lattice
 restatement does ~1/80 of those operations, so it exceeds the peak and is no roofline fraction.
@@ -50,10 +54,17 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cell_kernel_flops(st, fpu):
-    """Algorithmic FLOPs of one k_particles launch: lattice points by region, wall planes, cell stencils."""
-    pts = sum(st.lattice_points[r] * fpu["point"][r] for r in range(3))
-    return pts + st.plane_evals * fpu["plane"] + st.pairs_sphere * fpu["cell"]
+def cell_kernel_flops(st, fpu, part="main"):
+    """Algorithmic FLOPs of a cell-kernel launch: lattice values by region and cell stencils (the wall-plane
+    constants come from the line kernel's axis tables).  part "main": the kernel on the main stream (in
+    mixed precision the FP32 one, i.e. the totals less the FP64 fallback's); "fallback": the fallback's."""
+    fb_pts, fb_pairs = list(st.fp64_fallback_points), st.fp64_fallback_pairs
+    if part == "fallback":
+        pts, pairs = fb_pts, fb_pairs
+    else:
+        pts = [st.lattice_points[r] - fb_pts[r] for r in range(3)]
+        pairs = st.pairs_sphere - fb_pairs
+    return sum(pts[r] * fpu["point"][r] for r in range(3)) + pairs * fpu["cell"]
 
 
 def calibration():
@@ -74,6 +85,7 @@ def line_kernel_flops(st, fpu):
 
 
 def run_precision(ctx, hip, dev, dom, res, out, prec, steps, warmup, autotune):
+    """Times `steps` calls; returns (host ms per step, mean stats of the device times, last stats, shape)."""
     import torch
     shape = ctx.get_tuning()
     if autotune:  # untimed: the launch shape (never the results) is chosen on this workload
@@ -82,19 +94,22 @@ def run_precision(ctx, hip, dev, dom, res, out, prec, steps, warmup, autotune):
     for _ in range(warmup):
         ctx.sph_to_grid(dev, dom, res, precision=prec, out=out)
     torch.cuda.synchronize()
-    gms, lms, pms, ams, stats = [], [], [], [], None
+    gms, kms, fms, lms, pms, ams, stats = [], [], [], [], [], [], None
     t0 = time.perf_counter()
     for _ in range(steps):
         *_, stats = ctx.sph_to_grid(dev, dom, res, precision=prec, out=out)
         gms.append(stats.device_ms_gather)
+        kms.append(stats.device_ms_gather_kernel)
+        fms.append(stats.device_ms_gather_fallback)
         lms.append(stats.device_ms_lines)
         pms.append(stats.device_ms_prep)
         ams.append(stats.device_ms_accum)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    log(f"  device ms per step: prep {np.mean(pms):.2f}, lines {np.mean(lms):.2f}, cells {np.mean(gms):.2f}, "
-        f"accum {np.mean(ams):.2f}")
-    return (t1 - t0) * 1e3 / steps, float(np.mean(gms)), stats, shape
+    log(f"  device ms per step: prep {np.mean(pms):.2f}, lines {np.mean(lms):.2f}, cells {np.mean(gms):.2f} "
+        f"(main kernel {np.mean(kms):.2f}, FP64 fallback kernel {np.mean(fms):.2f}), accum {np.mean(ams):.2f}")
+    times = {"cells": float(np.mean(gms)), "kernel": float(np.mean(kms)), "fallback": float(np.mean(fms))}
+    return (t1 - t0) * 1e3 / steps, times, stats, shape
 
 
 def cpu_baseline(parts, dom, res, hip, ctx):
@@ -126,6 +141,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--tuning", default=None, help="item_grab,cursors: use this launch shape (no autotuning)")
+    ap.add_argument("--fp64-fallback", type=float, default=None,
+                    help="mixed precision's FP64 fallback threshold, support radius in cells (<= 0: off; "
+                         "default: the library's)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the mapping shards by grid slab across GPUs; that path is "
@@ -148,16 +166,19 @@ def main():
     out64 = [torch.zeros(nc, dtype=torch.float64, device="cuda:0") for _ in range(4)]
     out32 = [torch.zeros(nc, dtype=torch.float64, device="cuda:0") for _ in range(4)]
     ctx = hip.Context(0)
+    if args.fp64_fallback is not None:
+        ctx.set_fp64_fallback(args.fp64_fallback)
+    fb_cells = ctx.get_fp64_fallback()
     if args.tuning:
         ctx.set_tuning(*(int(v) for v in args.tuning.split(",")))
         args.no_autotune = True
 
     ms64, g64, st64, shape64 = run_precision(ctx, hip, dev, dom, res, out64, hip.FP64, args.steps, args.warmup,
                                              not args.no_autotune)
-    log(f"fp64: {ms64:.2f} ms/step, cell kernel {g64:.2f} ms, pairs {st64.pairs_sphere}")
+    log(f"fp64: {ms64:.2f} ms/step, cell kernel {g64['kernel']:.2f} ms, pairs {st64.pairs_sphere}")
     ms32, g32, st32, shape32 = run_precision(ctx, hip, dev, dom, res, out32, hip.MIXED, args.steps, args.warmup,
                                              not args.no_autotune)
-    log(f"mixed: {ms32:.2f} ms/step, cell kernel {g32:.2f} ms")
+    log(f"mixed: {ms32:.2f} ms/step, cell kernels {g32['cells']:.2f} ms")
 
     m64 = out64[0].cpu().numpy()
     m32 = out32[0].cpu().numpy()
@@ -169,13 +190,27 @@ def main():
     log(f"mixed vs fp64: total-mass rel err {mass_err:.3e}, mass L1 rel err {l1_err:.3e}, "
         f"per-field max err/max {field_err}")
 
+    # the same mixed step without the FP64 fallback, for comparison (untimed for `value`)
+    no_fb = None
+    if fb_cells > 0:
+        ctx.set_fp64_fallback(0.0)
+        ms_nf, g_nf, st_nf, _ = run_precision(ctx, hip, dev, dom, res, out32, hip.MIXED, max(3, args.steps // 2), 1,
+                                              False)
+        m_nf = out32[0].cpu().numpy()
+        no_fb = {"ms_per_step": ms_nf, "cells_ms": g_nf["cells"],
+                 "total_mass_rel_err": abs(float(np.sum(m_nf)) - tot64) / tot64,
+                 "mass_l1_rel_err": float(np.sum(np.abs(m_nf - m64))) / float(np.sum(np.abs(m64)))}
+        ctx.set_fp64_fallback(fb_cells)
+        log(f"mixed without the FP64 fallback: {ms_nf:.2f} ms/step, mass L1 rel err {no_fb['mass_l1_rel_err']:.3e}")
+
     pairs = st32.pairs_sphere
     value = pairs / (ms32 * 1e-3)
-    fl32 = cell_kernel_flops(st32, fpu)
+    fl32 = cell_kernel_flops(st32, fpu)  # the FP32 cell kernel's share of the work
+    fl_fb = cell_kernel_flops(st32, fpu, "fallback")
     fl64 = cell_kernel_flops(st64, fpu)
-    alg = f_pair * st32.pairs_bbox  # reference-algorithm operations for the launch's work
-    ref_equiv = alg / (g32 * 1e-3) / 1e12
-    exe = fl32 / (g32 * 1e-3) / 1e12
+    alg = f_pair * st32.pairs_bbox  # reference-algorithm operations for the step's cell work
+    ref_equiv = alg / (g32["cells"] * 1e-3) / 1e12
+    exe = fl32 / (g32["kernel"] * 1e-3) / 1e12
     traffic, traffic_shape = None, None
     tpath = os.path.join(ROOT, "profiles", "k_particles_m4_1M_256_traffic.json")
     if os.path.exists(tpath):
@@ -195,15 +230,25 @@ def main():
                    "launch_shape": {"fp64": {"item_grab": shape64[0], "cursors": shape64[1]},
                                     "mixed": {"item_grab": shape32[0], "cursors": shape32[1]},
                                     "autotuned": not args.no_autotune}},
-        "fp64": {"value": pairs / (ms64 * 1e-3), "ms_per_step": ms64, "cells_ms": g64,
-                 "roofline_frac": fl64 / (g64 * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"],
+        "fp64": {"value": pairs / (ms64 * 1e-3), "ms_per_step": ms64, "cells_ms": g64["kernel"],
+                 "roofline_frac": fl64 / (g64["kernel"] * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"],
                  "roofline_peak_tflops": PEAK_TFLOPS["fp64"],
-                 "ref_equiv_tflops": f_pair * st64.pairs_bbox / (g64 * 1e-3) / 1e12},
+                 "ref_equiv_tflops": f_pair * st64.pairs_bbox / (g64["kernel"] * 1e-3) / 1e12},
+        "fp64_fallback": {
+            "support_cells": fb_cells,
+            "note": "mixed precision computes particles whose support radius spans >= support_cells cells in "
+                    "FP64 (their FP32 lattice differences lose up to ~4e-2 of the particle at h = 30 cells); "
+                    "that cell kernel runs on the second stream beside the FP32 one",
+            "items_share": st32.fp64_fallback_blocks / max(st32.particle_blocks, 1),
+            "pairs_share": st32.fp64_fallback_pairs / max(st32.pairs_sphere, 1),
+            "kernel_ms": g32["fallback"], "flops_per_launch": fl_fb,
+            "tflops": fl_fb / (g32["fallback"] * 1e-3) / 1e12 if g32["fallback"] > 0 else None,
+            "without_fallback": no_fb},
         "mixed_over_fp64": ms64 / ms32,
         "mixed_total_mass_rel_err": mass_err,
         "mixed_field_err_over_max": field_err, "mixed_mass_l1_rel_err": l1_err,
         "pairs_sphere": pairs, "pairs_bbox": st32.pairs_bbox, "particle_blocks": st32.particle_blocks,
-        "cells_ms": g32, "prep_ms": st32.device_ms_prep,
+        "cells_ms": g32["cells"], "prep_ms": st32.device_ms_prep,
         "lines_exposed_ms": st32.device_ms_lines, "lines_kernel_ms": lines_kernel_ms,
         "accum_ms": st32.device_ms_accum,
         "lines_tflops": (line_kernel_flops(st32, fpu) / (lines_kernel_ms * 1e-3) / 1e12) if lines_kernel_ms > 0 else None,
@@ -211,7 +256,7 @@ def main():
                  "plane_evals": st32.plane_evals},
         "roofline": {"bound": "valu", "achieved": exe, "peak": PEAK_TFLOPS["mixed"], "unit": "TFLOP/s",
                      "frac": exe / PEAK_TFLOPS["mixed"], "traffic": traffic, "traffic_launch_shape": traffic_shape,
-                     "kernel": "k_particles<float>", "kernel_ms": g32, "flops_per_launch": fl32},
+                     "kernel": "k_particles<float>", "kernel_ms": g32["kernel"], "flops_per_launch": fl32},
         "ref_equiv": {"tflops": ref_equiv, "flops_per_launch": alg, "flops_per_bbox_pair": f_pair,
                       "note": "reference-algorithm operations for the same work over the kernel time; not a "
                               "roofline fraction"},

@@ -20,8 +20,9 @@
  *
  * Return codes: KIWI_OK (0) or a negative KIWI_ERR_*; kiwi_hip_last_error() then holds the message.
  * A context owns two HIP streams (the main one, and a second one on which the sph-to-grid entry points
- * compute the per-particle line tables beside the item sort; both at default priority) and its device
- * scratch; it must not be used from two threads at once.
+ * compute the per-particle line tables beside the item sort and mixed precision's FP64-fallback cell
+ * kernel beside the FP32 one; both at default priority) and its device scratch; it must not be used from
+ * two threads at once.
  * Calls are synchronous: outputs are complete when the call returns.  Device inputs must be complete
  * when the call is made: the context's streams do not wait for work the caller queued on other streams
  * (a caller that produces inputs on its own stream synchronises that stream first).
@@ -36,7 +37,7 @@
 extern "C" {
 #endif
 
-#define KIWI_HIP_ABI_VERSION 9
+#define KIWI_HIP_ABI_VERSION 10
 
 #define KIWI_OK 0
 #define KIWI_ERR_INVALID_ARG (-1)
@@ -114,6 +115,14 @@ typedef struct kiwi_stats {
     int64_t lattice_points[3]; /* lattice values evaluated, by radial region (inside h, 2h, outside) */
     int64_t line_evals[3];     /* edge-line constants with 0, 1 or 2 crossing evaluations */
     int64_t plane_evals;       /* wall-plane constants evaluated */
+    /* mixed precision's large-support FP64 fallback (kiwi_hip_set_fp64_fallback), a part of the totals
+       above: its (particle, cell block) items, sphere pairs, lattice values by region, and the time of its
+       cell kernel (device_ms_gather includes it) */
+    int64_t fp64_fallback_blocks;
+    int64_t fp64_fallback_pairs;
+    int64_t fp64_fallback_points[3];
+    double device_ms_gather_fallback; /* the fallback's cell kernel, on the second stream beside the other */
+    double device_ms_gather_kernel;   /* the main cell kernel alone (the FP32 one in mixed precision) */
     int n_warn_records;
     kiwi_warn_record warn[KIWI_MAX_WARN_RECORDS];
 } kiwi_stats;
@@ -279,6 +288,20 @@ typedef struct kiwi_tuning {
 } kiwi_tuning;
 
 #define KIWI_MAX_CURSORS 16
+
+/* Mixed precision's FP64 fallback (no reference counterpart; the reference computes everything in FP64).
+ * A cell's integral is a difference of lattice values of order one, while the cell holds a share of the
+ * particle of order (cell / support)^3: in FP32 the relative error of a cell's sum grows with the cube
+ * of the support radius in cells.  In KIWI_MIXED, a particle whose support radius (support_radius_in_h x h)
+ * spans at least support_cells times the smallest cell edge of its grid has its lattice values, line and
+ * plane constants and cell sums computed in FP64 (stored, like the FP32 ones, as float per-pair sums and
+ * accumulated in FP64).  support_cells <= 0 switches the fallback off; the default is
+ * KIWI_FP64_FALLBACK_CELLS.  KIWI_FP64 and the tabulated path are unaffected.  Within a cell, the
+ * fallback's particles are added after the others (each group in particle order), so mixed results stay
+ * bit-reproducible, and a batch still equals its single calls bit for bit. */
+#define KIWI_FP64_FALLBACK_CELLS 16.0
+int kiwi_hip_set_fp64_fallback(kiwi_hip_ctx* ctx, double support_cells);
+int kiwi_hip_get_fp64_fallback(const kiwi_hip_ctx* ctx, double* support_cells);
 
 /* Set (NULL: restore the defaults) or read the context's launch shape, used by every later call. */
 int kiwi_hip_set_tuning(kiwi_hip_ctx* ctx, const kiwi_tuning* tuning);

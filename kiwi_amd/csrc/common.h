@@ -110,14 +110,18 @@ struct kiwi_hip_ctx {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;      // second stream (default priority): line tables beside the item sort
     hipEvent_t side_done = nullptr;  // recorded on side after its work; the main stream waits on it
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    hipEvent_t ev_side[2] = {nullptr, nullptr};  // bracket the line-table kernel on the second stream
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // on the second stream: [0, 1] bracket the line-table kernel, [2, 3] the FP64 fallback's cell kernel
+    hipEvent_t ev_side[4] = {nullptr, nullptr, nullptr, nullptr};
     std::string err;
     DevBuf parts, bbox, sizes, offsets, keys_a, keys_b, pos_a, pos_b, cells, slab_sorted, slab, cub_tmp, ranges, work,
         work_sorted, order_in, order, lines, ms, stats, outbuf, vsum, tables, m4table, probs, part_prob, acc_units,
         acc_partial, acc_parts;
     bool m4table_ready = false;
     kiwi_tuning tune{16, 8};  // the cell kernel's launch shape (kiwi_hip_set_tuning)
+    // mixed precision's FP64 fallback: support radius in cells from which a particle computes in FP64
+    // (kiwi_hip_set_fp64_fallback; <= 0: off)
+    double fb_cells = KIWI_FP64_FALLBACK_CELLS;
     DevBuf vor[16];  // the Voronoi map's buffers
     // the allocation-free entry point: its arena over the device's scratch area and shadow buffers
     ScratchArena noalloc_arena;

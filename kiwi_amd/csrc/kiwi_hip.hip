@@ -86,7 +86,8 @@ constexpr int MAX_AXIS = 8192;                // grid cells per axis
 // over the 8 XCDs, so with 8 slices a slice's items, and the line tables of their particles, stay in
 // one XCD's L2) and then takes from the other slices until all are drained.
 constexpr int CURSOR_STRIDE = 544;  // 64-bit words between cursors (4352 B: separate lines)
-constexpr size_t CURSOR_BYTES = sizeof(unsigned long long) * KIWI_MAX_CURSORS * CURSOR_STRIDE;
+// two sets: the FP32 items and the FP64 fallback's items of mixed precision
+constexpr size_t CURSOR_BYTES = 2 * sizeof(unsigned long long) * KIWI_MAX_CURSORS * CURSOR_STRIDE;
 
 struct DevStats {  // one per problem
     unsigned long long skipped_outside, n_negative, n_negative_warn, pairs_bbox, pairs_sphere;
@@ -181,21 +182,32 @@ __device__ __forceinline__ void for_each_block(const int* bb, const GridGeom& g,
     }
 }
 
+// Mixed precision's FP64 fallback: a particle whose support radius spans at least fb_cells of the
+// smallest cell edge of its grid has its lattice values computed in FP64.  A cell's sum is a difference
+// of O(1) lattice values while the cell holds O((cell / support)^3) of the particle, so the FP32 error of
+// a cell sum, relative to the cell's share, grows with the cube of the support in cells.
+__device__ __forceinline__ bool fp64_fallback(const GridGeom& g, double support_in_h, double h, double fb_cells) {
+    return support_in_h * h >= fb_cells * fmin(g.dx, fmin(g.dy, g.dz));
+}
+
 __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                        const double* __restrict__ z, const double* __restrict__ h, const double* __restrict__ m,
                        const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
-                       double support_in_h, int skip_outside, int axis_units, const Prob* __restrict__ pr,
-                       const int* __restrict__ part_prob, int* __restrict__ bbox, AccPart* __restrict__ ap,
-                       int64_t* __restrict__ nblk, int64_t* __restrict__ nlines, DevStats* st) {
+                       double support_in_h, int skip_outside, int axis_units, double fb_cells,
+                       const Prob* __restrict__ pr, const int* __restrict__ part_prob, int* __restrict__ bbox,
+                       AccPart* __restrict__ ap, int64_t* __restrict__ nblk, int64_t* __restrict__ nlines,
+                       int64_t* __restrict__ nblk_fb, DevStats* st) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     unsigned long long skipped = 0, pairs = 0;
     const int q = (i < n) ? part_prob[i] : -1;
     if (i == n) {  // trailing zeros of the exclusive scans
         nblk[n] = 0;
         nlines[n] = 0;
+        if (fb_cells > 0.0) nblk_fb[n] = 0;
     }
     if (i < n) {
         int64_t nb = 0, nl = 0;
+        bool fb = false;
         int qb[6] = {1, 0, 0, 0, 0, 0};  // empty unless set below
         const double hh = h[i];
         const double x0 = x[i], y0 = y[i], z0 = z[i];
@@ -211,6 +223,13 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
                                [&](int, int, int) { ++nb; });
                 nl = (Nx + 1) * (Ny + 1) + (Nx + 1) * (Nz + 1) + (Ny + 1) * (Nz + 1);
                 if (axis_units > 0) nl += axis_units - 1 + static_cast<int64_t>(axis_units) * (Nx + Ny + Nz + 3);
+                if (fb_cells > 0.0) {
+                    // with the FP64 fallback on, sizes count FP32 entries: a fallback particle's FP64 entries
+                    // take two each, and every size is even, so that its region starts at an FP64 entry
+                    fb = fp64_fallback(pr[q].g, support_in_h, hh, fb_cells);
+                    if (fb) nl *= 2;
+                    nl = (nl + 1) & ~static_cast<int64_t>(1);
+                }
             }
         }
         int* bb = bbox + 6 * i;
@@ -222,7 +241,8 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
         for (int k = 0; k < 6; ++k) bb[k] = a.bb[k] = qb[k];
         a.pad[0] = a.pad[1] = 0;
         ap[i] = a;
-        nblk[i] = nb;
+        nblk[i] = fb ? 0 : nb;
+        if (fb_cells > 0.0) nblk_fb[i] = fb ? nb : 0;
         nlines[i] = nl;
     }
     add_by_problem(skipped, q, st, &DevStats::skipped_outside);
@@ -236,7 +256,8 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
 __global__ void k_emit(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                        const double* __restrict__ z, const double* __restrict__ h, double support_in_h,
                        int skip_outside, const int* __restrict__ bbox, const int64_t* __restrict__ offs,
-                       const Prob* __restrict__ pr, const int* __restrict__ part_prob, uint64_t* __restrict__ keys) {
+                       const int64_t* __restrict__ offs_fb, const Prob* __restrict__ pr,
+                       const int* __restrict__ part_prob, uint64_t* __restrict__ keys) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int* bb = bbox + 6 * i;
@@ -245,8 +266,12 @@ __global__ void k_emit(int64_t n, const double* __restrict__ x, const double* __
     const GridGeom& g = P.g;
     int qb[6];
     for (int k = 0; k < 6; ++k) qb[k] = bb[k];
-    int64_t o = offs[i];
-    const int64_t o_end = offs[i + 1];
+    // the FP64 fallback's items follow all the others (offs[n] of them), in particle order
+    int64_t o = offs[i], o_end = offs[i + 1];
+    if (offs_fb && offs_fb[i + 1] > offs_fb[i]) {
+        o = offs[n] + offs_fb[i];
+        o_end = offs[n] + offs_fb[i + 1];
+    }
     for_each_block(qb, g, x[i], y[i], z[i], h[i], support_in_h, skip_outside != 0, [&](int bx, int by, int bz) {
         if (o < o_end) {
             const uint64_t b = static_cast<uint64_t>(P.block_off) + (static_cast<uint64_t>(bx) * g.nby + by) * g.nbz + bz;
@@ -529,6 +554,7 @@ template <class P>
 __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
                                                    const double* __restrict__ z, const double* __restrict__ h,
                                                    const int* __restrict__ bbox, const int64_t* __restrict__ loffs,
+                                                   const int64_t* __restrict__ sel, int loff_shift,
                                                    const Prob* __restrict__ pr, const int* __restrict__ part_prob,
                                                    typename P::Ctx ctx,
                                                    LineVals<typename P::Real, P::NLN>* __restrict__ table,
@@ -565,25 +591,29 @@ __global__ __launch_bounds__(THREADS) void k_lines(int64_t n, const double* __re
     for (int64_t p = static_cast<int64_t>(blockIdx.x) * WAVES + wave_uniform(threadIdx.x >> 6); p < n; p += nwaves) {
         const int* bb = bbox + 6 * p;
         if (bb[0] > bb[1]) continue;
+        // with the FP64 fallback on, each precision's launch takes only the particles with items of its own
+        // (sel: their item counts)
+        if (sel && sel[p] == 0) continue;
         const int q = static_cast<int>(wave_uniform(static_cast<uint32_t>(part_prob[p])));
         if (q != cur_q) {
             flush();
             cur_q = q;
         }
         const GridGeom& g = pr[q].g;
+        const int64_t loff = loffs[p] >> loff_shift;  // in this precision's entries
         const int L0 = bb[1] - bb[0] + 2, L1 = bb[3] - bb[2] + 2, L2 = bb[5] - bb[4] + 2;
         const int c01 = L0 * L1, c02 = L0 * L2, c12 = L1 * L2;
         const int nl = c01 + c02 + c12;
         const float inv1 = __builtin_amdgcn_rcpf(static_cast<float>(L1)), inv2 = __builtin_amdgcn_rcpf(static_cast<float>(L2));
         const double inv_s = 1.0 / (P::unit_in_h(ctx) * h[p]);  // lattice units
         const double p0 = x[p], p1 = y[p], p2 = z[p];
-        LineVals<R, P::NLN>* out = table + loffs[p];
+        LineVals<R, P::NLN>* out = table + loff;
         // fused pairs take |d| and the plane values of both coordinates from the wave's LDS copy of the
         // axis table (particles with at most LINE_LDS_AX coordinates per axis)
         const bool use_lds = P::FUSED_PAIRS && L0 <= LINE_LDS_AX && L1 <= LINE_LDS_AX && L2 <= LINE_LDS_AX;
 
         // the axis table: offsets formed as the reference forms vertex coordinates, (x_min + i dx) - x_p
-        AxisVals<R, P::NPL>* at = reinterpret_cast<AxisVals<R, P::NPL>*>(table + axis_table_start(loffs[p], nl, UNITS));
+        AxisVals<R, P::NPL>* at = reinterpret_cast<AxisVals<R, P::NPL>*>(table + axis_table_start(loff, nl, UNITS));
         const int na = L0 + L1 + L2;
         for (int t = lane; t < na; t += 64) {
             int a = 0, u = t;
@@ -741,13 +771,15 @@ __device__ __forceinline__ R row_prev(R v) {
 // occupancy: the mixed cubic path is fitted to 8 wavefronts per SIMD (64 VGPRs, 20 KB of LDS per
 // workgroup); the cell kernel waits on memory for about a third of its wavefront cycles, and more
 // resident wavefronts hide it
-template <class P>
+// Items [first, first + npairs) of the particle-order item list; cell sums are stored as OutR (the
+// FP64 fallback of mixed precision computes in double and stores float, like the FP32 items).
+template <class P, typename OutR>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(P::WAVES_PER_EU))) void k_particles(
-    uint32_t npairs, const uint64_t* __restrict__ keys, const int64_t* __restrict__ slab,
+    uint32_t first, uint32_t npairs, const uint64_t* __restrict__ keys, const int64_t* __restrict__ slab,
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    const double* __restrict__ h, const int* __restrict__ bbox, const int64_t* __restrict__ loffs,
+    const double* __restrict__ h, const int* __restrict__ bbox, const int64_t* __restrict__ loffs, int loff_shift,
     const Prob* __restrict__ pr, const int* __restrict__ part_prob, typename P::Ctx ctx,
-    const LineVals<typename P::Real, P::NLN>* __restrict__ lines, typename P::Real* __restrict__ ms, DevStats* st,
+    const LineVals<typename P::Real, P::NLN>* __restrict__ lines, OutR* __restrict__ ms, DevStats* st,
     unsigned long long* cursor, int clamp_pairs, int grab, int ncurs) {
     using R = typename P::Real;
     constexpr int NLN = P::NLN;
@@ -798,8 +830,8 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(P::WAVE
     const unsigned long long ugrab = static_cast<unsigned long long>(grab);
     for (int d = 0; d < ncurs; ++d) {
         const int sl = (home + d) % ncurs;
-        const uint64_t lo = static_cast<uint64_t>(npairs) * sl / ncurs;
-        const uint64_t hi = static_cast<uint64_t>(npairs) * (sl + 1) / ncurs;
+        const uint64_t lo = first + static_cast<uint64_t>(npairs) * sl / ncurs;
+        const uint64_t hi = first + static_cast<uint64_t>(npairs) * (sl + 1) / ncurs;
         unsigned long long* cs = cursor + sl * CURSOR_STRIDE;
         uint64_t cur = 0;
         if (lane == 0) cur = lo + atomicAdd(cs, ugrab);
@@ -830,7 +862,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(P::WAVE
                     inv_s = 1.0 / (P::unit_in_h(ctx) * h[p]);  // lattice units
                     if constexpr (P::HAS_LINES) {
                         const int npairs_p = Lx * Ly + Lx * Lz + Ly * Lz;
-                        const int64_t lo_p = loffs[p];
+                        const int64_t lo_p = loffs[p] >> loff_shift;  // in this precision's entries
                         etab = reinterpret_cast<const R*>(lines + lo_p);
                         atab = reinterpret_cast<const AxisVals<R, P::NPL>*>(lines + axis_table_start(lo_p, npairs_p, UNITS));
                     }
@@ -839,7 +871,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(P::WAVE
                 const int mx = ib.mx, my = ib.my, mz = ib.mz;
                 const int nx = mx + 1, ny = my + 1, nz = mz + 1;  // item lattice
                 const int ox = ib.x0 - bb[0], oy = ib.y0 - bb[2], oz = ib.z0 - bb[4];
-                R* out = ms + slab[t];
+                OutR* out = ms + slab[t];
 
                 // ---- per-axis constants of the item's lattice planes ----
                 {
@@ -968,7 +1000,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(P::WAVE
                             Ms = 0.0;
                         }
                     }
-                    out[u] = static_cast<R>(Ms);
+                    out[u] = static_cast<OutR>(Ms);
                 }
                 wave_sync();
             }
@@ -1253,48 +1285,61 @@ __global__ void k_vsum_problems_final(int nprob, const double* __restrict__ part
 
 namespace {
 
-template <class P>
-hipError_t launch_cells(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pctx, int64_t n, int64_t npairs,
-                        const double* const* dp, const int* bbox, const AccPart* ap, const int64_t* loffs,
-                        const uint64_t* keys_particle_order, const int64_t* slab, const uint64_t* keys_block_order,
+// the cell kernel of a kernel path over items [first, first + count), cell sums stored as OutR
+template <class P, typename OutR>
+hipError_t launch_particles(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pctx, int64_t first, int64_t count,
+                            const double* const* dp, const int* bbox, const int64_t* loffs, int loff_shift,
+                            const uint64_t* keys_particle_order, const int64_t* slab, const Prob* pr,
+                            const int* part_prob, void* lines, void* ms, DevStats* dst, unsigned long long* cursor,
+                            int clamp) {
+    using Line = LineVals<typename P::Real, P::NLN>;
+    if (count > 0)
+        k_particles<P, OutR><<<resident_grid(c, k_particles<P, OutR>, c->tune.item_grab, count), THREADS, 0, s>>>(
+            static_cast<uint32_t>(first), static_cast<uint32_t>(count), keys_particle_order, slab, dp[0], dp[1], dp[2],
+            dp[7], bbox, loffs, loff_shift, pr, part_prob, pctx, static_cast<const Line*>(lines),
+            static_cast<OutR*>(ms), dst, cursor, clamp, c->tune.item_grab, c->tune.cursors);
+    return hipGetLastError();
+}
+
+// the accumulation of the per-pair cell sums (type R) into the grids
+template <typename R>
+hipError_t launch_accum(hipStream_t s, const AccPart* ap, const uint64_t* keys_block_order,
                         const int64_t* slab_sorted, const int* blk_begin, const int* blk_end, const int* order,
-                        int nblocks, const Prob* pr, int nprob, const int* part_prob, void* lines, void* ms,
-                        DevStats* dst, unsigned long long* cursor, int clamp, const AccUnit* units,
-                        const int* unit_off, const int* split_off, int64_t max_units, double* partial,
-                        hipEvent_t ev_lines, hipEvent_t ev_cells, hipEvent_t ev_accum) {
-    using R = typename P::Real;
-    using Line = LineVals<R, P::NLN>;
-    hipError_t e;
-    if ((e = hipEventRecord(ev_lines, s)) != hipSuccess) return e;
-    if (npairs > 0)
-        k_particles<P><<<resident_grid(c, k_particles<P>, c->tune.item_grab, npairs), THREADS, 0, s>>>(
-            static_cast<uint32_t>(npairs), keys_particle_order, slab, dp[0], dp[1], dp[2], dp[7], bbox, loffs, pr,
-            part_prob, pctx, static_cast<const Line*>(lines), static_cast<R*>(ms), dst, cursor, clamp,
-            c->tune.item_grab, c->tune.cursors);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = hipEventRecord(ev_cells, s)) != hipSuccess) return e;
+                        int nblocks, const Prob* pr, int nprob, const void* ms, int clamp, const AccUnit* units,
+                        const int* unit_off, const int* split_off, int64_t max_units, double* partial) {
     if (nblocks > 0) {
         k_accum<R><<<static_cast<unsigned>(max_units), THREADS, 0, s>>>(
-            ap, keys_block_order, slab_sorted, blk_begin, blk_end, order,
-            static_cast<const R*>(ms), pr, nprob, clamp, nblocks, units, unit_off, partial);
+            ap, keys_block_order, slab_sorted, blk_begin, blk_end, order, static_cast<const R*>(ms), pr, nprob, clamp,
+            nblocks, units, unit_off, partial);
         k_acc_combine<<<nblocks, THREADS, 0, s>>>(nblocks, order, split_off, partial, pr, nprob, clamp);
     }
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    return hipEventRecord(ev_accum, s);
+    return hipGetLastError();
 }
 
 // the per-particle line tables (k_lines) of a kernel path, on stream s
+// (sel: with the FP64 fallback on, the item counts of this precision's particles; loff_shift: 1 for the
+// fallback's FP64 tables, whose offsets count FP32 entries)
 template <class P>
 hipError_t launch_lines(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pctx, int64_t n, const double* const* dp,
-                        const int* bbox, const int64_t* loffs, const Prob* pr, const int* part_prob, void* lines,
-                        DevStats* dst) {
+                        const int* bbox, const int64_t* loffs, const int64_t* sel, int loff_shift, const Prob* pr,
+                        const int* part_prob, void* lines, DevStats* dst) {
     using Line = LineVals<typename P::Real, P::NLN>;
     if (n > 0 && P::HAS_LINES)
         k_lines<P><<<resident_grid(c, k_lines<P>, 1, n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], bbox, loffs,
-                                                                          pr, part_prob, pctx,
+                                                                          sel, loff_shift, pr, part_prob, pctx,
                                                                           static_cast<Line*>(lines), dst);
     return hipGetLastError();
 }
+
+// The FP64 fallback's tables sit in the FP32 table at even FP32-entry offsets: an FP64 entry is two FP32
+// entries and an axis-table entry spans the same number of line entries in both precisions.
+template <class P32, class P64>
+constexpr bool fallback_layout_ok() {
+    return sizeof(LineVals<typename P64::Real, P64::NLN>) == 2 * sizeof(LineVals<typename P32::Real, P32::NLN>) &&
+           axis_units<P32>() == axis_units<P64>() && P32::NLN == P64::NLN && P32::NPL == P64::NPL;
+}
+static_assert(fallback_layout_ok<PolicyM4<float>, PolicyM4<double>>(), "M4 fallback table layout");
+static_assert(fallback_layout_ok<PolicyGen<float>, PolicyGen<double>>(), "generic fallback table layout");
 
 }  // namespace
 
@@ -1431,14 +1476,16 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
 
     // per-particle sizes (items, lines) and their exclusive scans, (n + 1) each
     const int64_t n1 = n + 1;
-    const size_t stats_bytes = (sizeof(DevStats) * nprob + 255) / 256 * 256;
+    // two statistics sets: everything, and the FP64 fallback's cell kernel (merged into the first at the end)
+    const size_t stats_bytes = (sizeof(DevStats) * 2 * nprob + 255) / 256 * 256;
     KIWI_HIP_CHECK(c, reserve((*B.stats), stats_bytes + CURSOR_BYTES));
     DevStats* dst = static_cast<DevStats*>((*B.stats).p);
+    DevStats* dst_fb = dst + nprob;
     unsigned long long* cursor = reinterpret_cast<unsigned long long*>(static_cast<char*>((*B.stats).p) + stats_bytes);
     KIWI_HIP_CHECK(c, reserve((*B.bbox), 6 * sizeof(int) * std::max<int64_t>(n, 1)));
     KIWI_HIP_CHECK(c, reserve((*B.acc_parts), sizeof(AccPart) * std::max<int64_t>(n, 1)));
-    KIWI_HIP_CHECK(c, reserve((*B.sizes), 2 * sizeof(int64_t) * n1));
-    KIWI_HIP_CHECK(c, reserve((*B.offsets), 2 * sizeof(int64_t) * n1));
+    KIWI_HIP_CHECK(c, reserve((*B.sizes), 3 * sizeof(int64_t) * n1));
+    KIWI_HIP_CHECK(c, reserve((*B.offsets), 3 * sizeof(int64_t) * n1));
     KIWI_HIP_CHECK(c, reserve((*B.ranges), 2 * sizeof(int) * std::max<int64_t>(nblocks, 1)));
     KIWI_HIP_CHECK(c, reserve((*B.work), sizeof(unsigned long long) * std::max<int64_t>(nblocks, 1)));
     KIWI_HIP_CHECK(c, reserve((*B.work_sorted), sizeof(unsigned long long) * std::max<int64_t>(nblocks, 1)));
@@ -1449,8 +1496,8 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     AccPart* ap = static_cast<AccPart*>((*B.acc_parts).p);
     int64_t* sz = static_cast<int64_t*>((*B.sizes).p);
     int64_t* of = static_cast<int64_t*>((*B.offsets).p);
-    int64_t *nblk = sz, *nlin = sz + n1;
-    int64_t *offs = of, *loffs = of + n1;
+    int64_t *nblk = sz, *nlin = sz + n1, *nblk_fb = sz + 2 * n1;
+    int64_t *offs = of, *loffs = of + n1, *offs_fb = of + 2 * n1;
     int* blk_begin = static_cast<int*>((*B.ranges).p);
     int* blk_end = blk_begin + nblocks;
     unsigned long long* work = static_cast<unsigned long long*>((*B.work).p);
@@ -1476,26 +1523,32 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     const int ax_units = tabulated ? 0
                          : generic ? (fp64 ? axis_units<PolicyGen<double>>() : axis_units<PolicyGen<float>>())
                                    : (fp64 ? axis_units<PolicyM4<double>>() : axis_units<PolicyM4<float>>());
+    // mixed precision's FP64 fallback for particles whose support spans many cells (fp64_fallback): their
+    // items follow the FP32 items and run through the FP64 cell kernel, which stores float cell sums
+    const double fb_cells = fp64 ? 0.0 : c->fb_cells;
+    const bool split = fb_cells > 0.0;
+    const int nscan = split ? 3 : 2;
     k_prep<<<grid1d(n1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], dp[6], dp[3], dp[4], dp[5], sup, skip_outside,
-                                          ax_units, dpr, part_prob, bbox, ap, nblk, nlin, dst);
+                                          ax_units, fb_cells, dpr, part_prob, bbox, ap, nblk, nlin, nblk_fb, dst);
     KIWI_HIP_CHECK(c, hipGetLastError());
     {
         size_t tmp_bytes = 0;
         KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, nblk, offs, static_cast<int>(n1), s));
         KIWI_HIP_CHECK(c, reserve((*B.cub_tmp), tmp_bytes));
-        for (int k = 0; k < 2; ++k) {
+        for (int k = 0; k < nscan; ++k) {
             size_t tb = (*B.cub_tmp).cap;
             KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum((*B.cub_tmp).p, tb, sz + k * n1, of + k * n1,
                                                                static_cast<int>(n1), s));
         }
     }
-    int64_t totals[2] = {0, 0};
+    int64_t totals[3] = {0, 0, 0};
     std::vector<DevStats> hs(static_cast<size_t>(nprob));
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < nscan; ++k)
         KIWI_HIP_CHECK(c, hipMemcpyAsync(&totals[k], of + k * n1 + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     KIWI_HIP_CHECK(c, hipMemcpyAsync(hs.data(), dst, sizeof(DevStats) * nprob, hipMemcpyDeviceToHost, s));
     KIWI_HIP_CHECK(c, hipStreamSynchronize(s));
-    const int64_t npairs = totals[0], nlines_total = totals[1];
+    // items: the FP32 ones [0, n32), then the FP64 fallback's [n32, npairs)
+    const int64_t n32 = totals[0], npairs = totals[0] + totals[2], nlines_total = totals[1];
     unsigned long long pairs_bbox = 0;
     for (int q = 0; q < nprob; ++q) pairs_bbox += hs[q].pairs_bbox;
     if (npairs > 0x7fffffffLL)
@@ -1529,24 +1582,33 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     if (side_lines) {
         KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[0], c->side));
         const GenTab* tabs = static_cast<const GenTab*>(c->tables.p) + K->kernel_id;
-        if (generic && fp64)
-            KIWI_HIP_CHECK(c, launch_lines<PolicyGen<double>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, dpr, part_prob,
-                                                              (*B.lines).p, dst));
-        else if (generic)
-            KIWI_HIP_CHECK(c, launch_lines<PolicyGen<float>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, dpr, part_prob,
-                                                             (*B.lines).p, dst));
-        else if (fp64)
-            KIWI_HIP_CHECK(c, launch_lines<PolicyM4<double>>(c, c->side, {sup}, n, dp, bbox, loffs, dpr, part_prob,
-                                                             (*B.lines).p, dst));
-        else
-            KIWI_HIP_CHECK(c, launch_lines<PolicyM4<float>>(c, c->side, {sup}, n, dp, bbox, loffs, dpr, part_prob,
-                                                            (*B.lines).p, dst));
+        void* lt = (*B.lines).p;
+        const int64_t* sel32 = split ? nblk : nullptr;
+        if (generic && fp64) {
+            KIWI_HIP_CHECK(c, launch_lines<PolicyGen<double>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, nullptr, 0,
+                                                              dpr, part_prob, lt, dst));
+        } else if (generic) {
+            KIWI_HIP_CHECK(c, launch_lines<PolicyGen<float>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, sel32, 0, dpr,
+                                                             part_prob, lt, dst));
+            if (split)
+                KIWI_HIP_CHECK(c, launch_lines<PolicyGen<double>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, nblk_fb,
+                                                                  1, dpr, part_prob, lt, dst));
+        } else if (fp64) {
+            KIWI_HIP_CHECK(c, launch_lines<PolicyM4<double>>(c, c->side, {sup}, n, dp, bbox, loffs, nullptr, 0, dpr,
+                                                             part_prob, lt, dst));
+        } else {
+            KIWI_HIP_CHECK(c, launch_lines<PolicyM4<float>>(c, c->side, {sup}, n, dp, bbox, loffs, sel32, 0, dpr,
+                                                            part_prob, lt, dst));
+            if (split)
+                KIWI_HIP_CHECK(c, launch_lines<PolicyM4<double>>(c, c->side, {sup}, n, dp, bbox, loffs, nblk_fb, 1, dpr,
+                                                                 part_prob, lt, dst));
+        }
         KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[1], c->side));
         KIWI_HIP_CHECK(c, hipEventRecord(c->side_done, c->side));
     }
     if (n > 0) {
-        k_emit<<<grid1d(n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], sup, skip_outside, bbox, offs, dpr,
-                                             part_prob, keys_a);
+        k_emit<<<grid1d(n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], sup, skip_outside, bbox, offs,
+                                             split ? offs_fb : nullptr, dpr, part_prob, keys_a);
         KIWI_HIP_CHECK(c, hipGetLastError());
     }
     int end_bit = 33;  // at least one key bit above the particle index
@@ -1624,41 +1686,68 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     if (side_lines) KIWI_HIP_CHECK(c, hipStreamWaitEvent(s, c->side_done, 0));  // the line tables are in
     const int clamp = fp64 ? 1 : 0;
     const int nb = static_cast<int>(nblocks);
+    unsigned long long* cursor_fb = cursor + KIWI_MAX_CURSORS * CURSOR_STRIDE;  // the fallback's work cursors
+    void* lt = (*B.lines).p;
+    void* msb = (*B.ms).p;
+    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[2], s));
+    // the FP64 fallback's cell kernel runs on the second stream beside the FP32 one (disjoint items,
+    // slab ranges, counters and work cursors); the accumulation waits for both
+    const bool fb_on_side = split && npairs > n32 && !tabulated;
+    if (fb_on_side) {
+        KIWI_HIP_CHECK(c, hipStreamWaitEvent(c->side, c->ev[2], 0));
+        KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[2], c->side));
+    }
     if (tabulated) {
         KIWI_HIP_CHECK(c, kiwi_build_m4_table(c, s));
         const PolicyTab::Ctx pc{sup, static_cast<const double*>(c->m4table.p)};
-        KIWI_HIP_CHECK(c, launch_cells<PolicyTab>(c, s, pc, n, npairs, dp, bbox, ap, loffs, keys_a, slab, keys_b,
-                                                  slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob, part_prob,
-                                                  (*B.lines).p, (*B.ms).p, dst, cursor, clamp,
-                                                  units, unit_off, split_off, max_units, partial, c->ev[2], c->ev[3],
-                                                  c->ev[4]));
-    } else if (generic && fp64) {
-        const PolicyGen<double>::Ctx pc{sup, static_cast<const GenTab*>(c->tables.p) + K->kernel_id};
-        KIWI_HIP_CHECK(c, launch_cells<PolicyGen<double>>(c, s, pc, n, npairs, dp, bbox, ap, loffs, keys_a, slab, keys_b,
-                                                          slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
-                                                          part_prob, (*B.lines).p, (*B.ms).p, dst, cursor, clamp,
-                                                          units, unit_off, split_off, max_units, partial,
-                                                          c->ev[2], c->ev[3], c->ev[4]));
+        KIWI_HIP_CHECK(c, hipEventRecord(c->ev[2], s));  // the table build is not the cell kernel's time
+        KIWI_HIP_CHECK(c, (launch_particles<PolicyTab, double>(c, s, pc, 0, npairs, dp, bbox, loffs, 0, keys_a, slab,
+                                                               dpr, part_prob, lt, msb, dst, cursor, clamp)));
     } else if (generic) {
-        const PolicyGen<float>::Ctx pc{sup, static_cast<const GenTab*>(c->tables.p) + K->kernel_id};
-        KIWI_HIP_CHECK(c, launch_cells<PolicyGen<float>>(c, s, pc, n, npairs, dp, bbox, ap, loffs, keys_a, slab, keys_b,
-                                                         slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
-                                                         part_prob, (*B.lines).p, (*B.ms).p, dst, cursor, clamp,
-                                                         units, unit_off, split_off, max_units, partial,
-                                                         c->ev[2], c->ev[3], c->ev[4]));
+        const GenTab* tabs = static_cast<const GenTab*>(c->tables.p) + K->kernel_id;
+        if (fp64) {
+            KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<double>, double>(c, s, {sup, tabs}, 0, npairs, dp, bbox, loffs,
+                                                                           0, keys_a, slab, dpr, part_prob, lt, msb,
+                                                                           dst, cursor, clamp)));
+        } else {
+            KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<float>, float>(c, s, {sup, tabs}, 0, n32, dp, bbox, loffs, 0,
+                                                                         keys_a, slab, dpr, part_prob, lt, msb, dst,
+                                                                         cursor, clamp)));
+                if (fb_on_side) {
+                KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<double>, float>(c, c->side, {sup, tabs}, n32,
+                                                                              npairs - n32, dp, bbox, loffs, 1, keys_a,
+                                                                              slab, dpr, part_prob, lt, msb, dst_fb,
+                                                                              cursor_fb, clamp)));
+            }
+        }
     } else if (fp64) {
-        KIWI_HIP_CHECK(c, launch_cells<PolicyM4<double>>(c, s, {sup}, n, npairs, dp, bbox, ap, loffs, keys_a, slab, keys_b,
-                                                         slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
-                                                         part_prob, (*B.lines).p, (*B.ms).p, dst, cursor, clamp,
-                                                         units, unit_off, split_off, max_units, partial, c->ev[2],
-                                                         c->ev[3], c->ev[4]));
+        KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<double>, double>(c, s, {sup}, 0, npairs, dp, bbox, loffs, 0, keys_a,
+                                                                      slab, dpr, part_prob, lt, msb, dst, cursor,
+                                                                      clamp)));
     } else {
-        KIWI_HIP_CHECK(c, launch_cells<PolicyM4<float>>(c, s, {sup}, n, npairs, dp, bbox, ap, loffs, keys_a, slab, keys_b,
-                                                        slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
-                                                        part_prob, (*B.lines).p, (*B.ms).p, dst, cursor, clamp,
-                                                        units, unit_off, split_off, max_units, partial, c->ev[2],
-                                                        c->ev[3], c->ev[4]));
+        KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<float>, float>(c, s, {sup}, 0, n32, dp, bbox, loffs, 0, keys_a,
+                                                                    slab, dpr, part_prob, lt, msb, dst, cursor,
+                                                                    clamp)));
+        if (fb_on_side) {
+            KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<double>, float>(c, c->side, {sup}, n32, npairs - n32, dp, bbox,
+                                                                         loffs, 1, keys_a, slab, dpr, part_prob, lt,
+                                                                         msb, dst_fb, cursor_fb, clamp)));
+        }
     }
+    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[5], s));  // the main cell kernel's end
+    if (fb_on_side) {
+        KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[3], c->side));
+        KIWI_HIP_CHECK(c, hipEventRecord(c->side_done, c->side));
+        KIWI_HIP_CHECK(c, hipStreamWaitEvent(s, c->side_done, 0));
+    }
+    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[3], s));
+    if (fp64)
+        KIWI_HIP_CHECK(c, launch_accum<double>(s, ap, keys_b, slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
+                                               msb, clamp, units, unit_off, split_off, max_units, partial));
+    else
+        KIWI_HIP_CHECK(c, launch_accum<float>(s, ap, keys_b, slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
+                                              msb, clamp, units, unit_off, split_off, max_units, partial));
+    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[4], s));
     k_vsum_problems<<<nprob * VSUM_BLOCKS, THREADS, 0, s>>>(dpr, dp[3], dp[4], dp[5], vsums + 3 * nprob);
     k_vsum_problems_final<<<(nprob + 63) / 64, 64, 0, s>>>(nprob, vsums + 3 * nprob, vsums);
     KIWI_HIP_CHECK(c, hipGetLastError());
@@ -1672,7 +1761,9 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
             KIWI_HIP_CHECK(c, hipMemcpyAsync(ho[k], probs[q].out[k], nc * sizeof(double), hipMemcpyDeviceToHost, s));
     }
     std::vector<double> vs(static_cast<size_t>(3 * nprob));
+    std::vector<DevStats> hs_fb(static_cast<size_t>(nprob));
     KIWI_HIP_CHECK(c, hipMemcpyAsync(hs.data(), dst, sizeof(DevStats) * nprob, hipMemcpyDeviceToHost, s));
+    KIWI_HIP_CHECK(c, hipMemcpyAsync(hs_fb.data(), dst_fb, sizeof(DevStats) * nprob, hipMemcpyDeviceToHost, s));
     KIWI_HIP_CHECK(c, hipMemcpyAsync(vs.data(), vsums, sizeof(double) * 3 * nprob, hipMemcpyDeviceToHost, s));
     KIWI_HIP_CHECK(c, hipStreamSynchronize(s));
     float ms_prep = 0, ms_lines = 0, ms_cells = 0, ms_accum = 0;
@@ -1680,12 +1771,23 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     (void)hipEventElapsedTime(&ms_lines, c->ev[1], c->ev[2]);
     (void)hipEventElapsedTime(&ms_cells, c->ev[2], c->ev[3]);
     (void)hipEventElapsedTime(&ms_accum, c->ev[3], c->ev[4]);
+    float ms_fb = 0, ms_cells_kernel = 0;
+    (void)hipEventElapsedTime(&ms_cells_kernel, c->ev[2], c->ev[5]);
+    if (fb_on_side) (void)hipEventElapsedTime(&ms_fb, c->ev_side[2], c->ev_side[3]);
     float ms_lines_kernel = 0;
     if (side_lines) (void)hipEventElapsedTime(&ms_lines_kernel, c->ev_side[0], c->ev_side[1]);
     for (int q = 0; q < nprob; ++q) {
         kiwi_stats* stats = hp[q].stats;
         if (!stats) continue;
-        const DevStats& d = hs[q];
+        // the fallback's cell-kernel counters join the totals (its warning records stay apart: mixed
+        // precision does not clamp pairs, so it records none)
+        DevStats d = hs[q];
+        const DevStats& f = hs_fb[q];
+        d.pairs_sphere += f.pairs_sphere;
+        d.n_negative += f.n_negative;
+        d.n_negative_warn += f.n_negative_warn;
+        d.planes += f.planes;
+        for (int r = 0; r < 3; ++r) d.pts[r] += f.pts[r];
         std::memset(stats, 0, sizeof(*stats));
         stats->skipped_outside = static_cast<int64_t>(d.skipped_outside);
         stats->n_negative = static_cast<int64_t>(d.n_negative);
@@ -1706,6 +1808,11 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
             stats->line_evals[r] = static_cast<int64_t>(d.lines[r]);
         }
         stats->plane_evals = static_cast<int64_t>(d.planes);
+        stats->fp64_fallback_blocks = nprob == 1 ? npairs - n32 : -1;
+        stats->fp64_fallback_pairs = static_cast<int64_t>(f.pairs_sphere);
+        for (int r = 0; r < 3; ++r) stats->fp64_fallback_points[r] = static_cast<int64_t>(f.pts[r]);
+        stats->device_ms_gather_fallback = ms_fb;
+        stats->device_ms_gather_kernel = ms_cells_kernel;
         const int nw = static_cast<int>(std::min<unsigned long long>(d.n_warn, KIWI_MAX_WARN_RECORDS));
         stats->n_warn_records = nw;
         for (int k = 0; k < nw; ++k) stats->warn[k] = d.warn[k];
@@ -1884,6 +1991,19 @@ int kiwi_hip_set_tuning(kiwi_hip_ctx* c, const kiwi_tuning* t) {
     if (t->cursors < 1 || t->cursors > KIWI_MAX_CURSORS)
         return fail(c, KIWI_ERR_INVALID_ARG, "cursors must be in 1..KIWI_MAX_CURSORS");
     c->tune = *t;
+    return KIWI_OK;
+}
+
+int kiwi_hip_set_fp64_fallback(kiwi_hip_ctx* c, double support_cells) {
+    if (!c) return KIWI_ERR_INVALID_ARG;
+    if (std::isnan(support_cells)) return fail(c, KIWI_ERR_INVALID_ARG, "fp64 fallback threshold is NaN");
+    c->fb_cells = support_cells;
+    return KIWI_OK;
+}
+
+int kiwi_hip_get_fp64_fallback(const kiwi_hip_ctx* c, double* support_cells) {
+    if (!c || !support_cells) return KIWI_ERR_INVALID_ARG;
+    *support_cells = c->fb_cells;
     return KIWI_OK;
 }
 

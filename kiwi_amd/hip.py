@@ -19,6 +19,7 @@ import numpy as np
 from . import LIB_PATH
 
 FP64, MIXED = 0, 1
+FP64_FALLBACK_CELLS = 16.0  # KIWI_FP64_FALLBACK_CELLS
 KERNELS = ["M4", "M5", "M6", "WC2", "WC4", "WC6"]
 ALIASES = {"cubic": "M4", "cubic_spline": "M4", "m4": "M4", "quartic": "M5", "quartic_spline": "M5", "m5": "M5",
            "quintic": "M6", "quintic_spline": "M6", "m6": "M6", "wendland_c2": "WC2", "wendlandc2": "WC2",
@@ -34,7 +35,8 @@ EXPORTED = ["kiwi_hip_abi_version", "kiwi_hip_create", "kiwi_hip_destroy", "kiwi
             "kiwi_hip_stream", "kiwi_hip_sph_to_grid", "kiwi_hip_voronoi_map", "kiwi_hip_sph_to_grid_adjoint",
             "kiwi_hip_sph_to_grid_batch", "kiwi_hip_set_tuning", "kiwi_hip_get_tuning", "kiwi_hip_autotune",
             "kiwi_hip_inverse_mapping", "kiwi_hip_cartesian_map", "kiwi_hip_cartesian_lattice",
-            "kiwi_hip_coverage_buffer", "kiwi_hip_voronoi_centroids", "kiwi_hip_sph_to_grid_noalloc"]
+            "kiwi_hip_coverage_buffer", "kiwi_hip_voronoi_centroids", "kiwi_hip_sph_to_grid_noalloc",
+            "kiwi_hip_set_fp64_fallback", "kiwi_hip_get_fp64_fallback"]
 
 
 class KiwiError(RuntimeError):
@@ -71,11 +73,15 @@ class Stats(ctypes.Structure):
                 ("device_ms_prep", _d), ("device_ms_lines", _d), ("device_ms_gather", _d), ("device_ms_accum", _d),
                 ("device_ms_lines_kernel", _d),
                 ("lattice_points", _i64 * 3), ("line_evals", _i64 * 3),
-                ("plane_evals", _i64), ("n_warn_records", _i), ("warn", WarnRecord * 16)]
+                ("plane_evals", _i64), ("fp64_fallback_blocks", _i64), ("fp64_fallback_pairs", _i64),
+                ("fp64_fallback_points", _i64 * 3), ("device_ms_gather_fallback", _d), ("device_ms_gather_kernel", _d),
+                ("n_warn_records", _i),
+                ("warn", WarnRecord * 16)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_
-             if f not in ("v_mean", "warn", "lattice_points", "line_evals")}
+             if f not in ("v_mean", "warn", "lattice_points", "line_evals", "fp64_fallback_points")}
+        d["fp64_fallback_points"] = list(self.fp64_fallback_points)
         d["v_mean"] = list(self.v_mean)
         d["lattice_points"] = list(self.lattice_points)
         d["line_evals"] = list(self.line_evals)
@@ -202,6 +208,10 @@ def load():
         L.kiwi_hip_coverage_buffer.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.POINTER(_d), _d, _d,
                                                ctypes.POINTER(BufferSites), ctypes.POINTER(CoverageStats)]
         L.kiwi_hip_set_tuning.restype = _i
+        L.kiwi_hip_set_fp64_fallback.argtypes = [ctypes.c_void_p, _d]
+        L.kiwi_hip_set_fp64_fallback.restype = _i
+        L.kiwi_hip_get_fp64_fallback.argtypes = [ctypes.c_void_p, ctypes.POINTER(_d)]
+        L.kiwi_hip_get_fp64_fallback.restype = _i
         L.kiwi_hip_set_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(Tuning)]
         L.kiwi_hip_get_tuning.restype = _i
         L.kiwi_hip_get_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(Tuning)]
@@ -384,6 +394,16 @@ class Context:
             t = Tuning(int(item_grab if item_grab is not None else cur[0]),
                        int(cursors if cursors is not None else cur[1]))
             self._check(self.L.kiwi_hip_set_tuning(self.h, ctypes.byref(t)))
+
+    def set_fp64_fallback(self, support_cells):
+        """Mixed precision's FP64 fallback: particles whose support radius spans at least support_cells grid
+        cells compute in FP64 (<= 0: off; default FP64_FALLBACK_CELLS)."""
+        self._check(self.L.kiwi_hip_set_fp64_fallback(self.h, float(support_cells)))
+
+    def get_fp64_fallback(self):
+        v = _d()
+        self._check(self.L.kiwi_hip_get_fp64_fallback(self.h, ctypes.byref(v)))
+        return v.value
 
     def get_tuning(self):
         t = Tuning()
