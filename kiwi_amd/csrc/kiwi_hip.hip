@@ -780,8 +780,11 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(P::WAVE
     const double* __restrict__ h, const int* __restrict__ bbox, const int64_t* __restrict__ loffs, int loff_shift,
     const Prob* __restrict__ pr, const int* __restrict__ part_prob, typename P::Ctx ctx,
     const LineVals<typename P::Real, P::NLN>* __restrict__ lines, OutR* __restrict__ ms, DevStats* st,
-    unsigned long long* cursor, int clamp_pairs, int grab, int ncurs) {
+    unsigned long long* cursor, int grab, int ncurs) {
     using R = typename P::Real;
+    // the reference clamps every (particle, cell) sum at zero (src/cartesian_grid.cpp:264-269): FP64 mode
+    // does the same; mixed precision (float per-pair sums) clamps each cell's total instead (k_accum)
+    constexpr bool CLAMP = std::is_same<OutR, double>::value;
     constexpr int NLN = P::NLN;
     constexpr int UNITS = axis_units<P>();
     __shared__ WaveLds<R, P::NPL> WL[WAVES];
@@ -970,37 +973,72 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(P::WAVE
                 wave_sync();
 
                 // ---- cell sums: the 8-corner stencil, stored to the item's slab ----
-                const int ncells = mx * my * mz;
-                BoxWalk wc(lane, my, mz);
-                for (int u = lane; u < ncells; u += 64, wc.next(my, mz)) {
-                    const int i = wc.i, j = wc.j, k = wc.k;
-                    double Ms = 0.0;
-                    const bool in_sphere =
-                        !P::SPHERE_SKIP || L.cg2[i] + L.cg2[AX_STRIDE + j] + L.cg2[2 * AX_STRIDE + k] < S2;
-                    n_sphere += __popcll(__ballot(in_sphere));  // wave total, held by lane 0 as above
-                    if (in_sphere) {
-                        const R* U0 = &L.U[(i * ny + j) * nz + k];
-                        Ms = static_cast<double>(U0[nyz]) + static_cast<double>(U0[nz]) + static_cast<double>(U0[1]) +
-                             static_cast<double>(U0[nyz + nz + 1]) - static_cast<double>(U0[0]) -
-                             static_cast<double>(U0[nyz + nz]) - static_cast<double>(U0[nyz + 1]) -
-                             static_cast<double>(U0[nz + 1]);
-                        if (clamp_pairs && Ms < 0.0) {
-                            ++n_neg;
-                            if (Ms <= -0.01) {
-                                ++n_warn;
-                                DevStats* sq = st + cur_q;
-                                const unsigned long long wslot = atomicAdd(&sq->n_warn, 1ull);
-                                if (wslot < KIWI_MAX_WARN_RECORDS) {
-                                    sq->warn[wslot].particle = static_cast<int64_t>(p) - pr[cur_q].part_off;
-                                    sq->warn[wslot].cell =
-                                        (static_cast<int64_t>(ib.x0 + i) * g.ny + (ib.y0 + j)) * g.nz + (ib.z0 + k);
-                                    sq->warn[wslot].value = Ms;
-                                }
+                // the reference's per-pair clamp at zero, with its warning records (FP64 mode)
+                auto clamp_cell = [&](double& Ms, int i, int j, int k) {
+                    if (CLAMP && Ms < 0.0) {
+                        ++n_neg;
+                        if (Ms <= -0.01) {
+                            ++n_warn;
+                            DevStats* sq = st + cur_q;
+                            const unsigned long long wslot = atomicAdd(&sq->n_warn, 1ull);
+                            if (wslot < KIWI_MAX_WARN_RECORDS) {
+                                sq->warn[wslot].particle = static_cast<int64_t>(p) - pr[cur_q].part_off;
+                                sq->warn[wslot].cell =
+                                    (static_cast<int64_t>(ib.x0 + i) * g.ny + (ib.y0 + j)) * g.nz + (ib.z0 + k);
+                                sq->warn[wslot].value = Ms;
                             }
-                            Ms = 0.0;
                         }
+                        Ms = 0.0;
                     }
-                    out[u] = static_cast<OutR>(Ms);
+                };
+                const int cyz = my * mz;
+                if (cyz > 32) {
+                    // x-walk: lane (j, k) of the y-z face walks its row of cells along x.  A cell's stencil is
+                    // the difference of the face sums E(i + 1) - E(i), E(i) = U[i][j+1][k+1] - U[i][j+1][k]
+                    // - U[i][j][k+1] + U[i][j][k] (sums of four floats, exact in double), so every face sum
+                    // is formed once instead of twice.  Stores stay contiguous: cells (i, j, k) k fastest.
+                    const bool act = lane < cyz;
+                    const int j = act ? static_cast<int>((static_cast<uint32_t>(lane) * kMagic16[mz]) >> 16) : 0;
+                    const int k = act ? lane - j * mz : 0;
+                    const R cgy = L.cg2[AX_STRIDE + j], cgz = L.cg2[2 * AX_STRIDE + k];
+                    const R* Uc = &L.U[j * nz + k];
+                    auto face = [&](int i) {
+                        const R* q = Uc + i * nyz;
+                        return (static_cast<double>(q[nz + 1]) + static_cast<double>(q[0])) -
+                               static_cast<double>(q[nz]) - static_cast<double>(q[1]);
+                    };
+                    double Ep = face(0);
+                    for (int i = 0; i < mx; ++i) {
+                        const double En = face(i + 1);
+                        const bool in_sphere = act && (!P::SPHERE_SKIP || L.cg2[i] + cgy + cgz < S2);
+                        n_sphere += __popcll(__ballot(in_sphere));  // wave total, held by lane 0 as above
+                        double Ms = 0.0;
+                        if (in_sphere) {
+                            Ms = En - Ep;
+                            clamp_cell(Ms, i, j, k);
+                        }
+                        if (act) out[i * cyz + lane] = static_cast<OutR>(Ms);
+                        Ep = En;
+                    }
+                } else {
+                    const int ncells = mx * cyz;
+                    BoxWalk wc(lane, my, mz);
+                    for (int u = lane; u < ncells; u += 64, wc.next(my, mz)) {
+                        const int i = wc.i, j = wc.j, k = wc.k;
+                        double Ms = 0.0;
+                        const bool in_sphere =
+                            !P::SPHERE_SKIP || L.cg2[i] + L.cg2[AX_STRIDE + j] + L.cg2[2 * AX_STRIDE + k] < S2;
+                        n_sphere += __popcll(__ballot(in_sphere));  // wave total, held by lane 0 as above
+                        if (in_sphere) {
+                            const R* U0 = &L.U[(i * ny + j) * nz + k];
+                            Ms = static_cast<double>(U0[nyz]) + static_cast<double>(U0[nz]) +
+                                 static_cast<double>(U0[1]) + static_cast<double>(U0[nyz + nz + 1]) -
+                                 static_cast<double>(U0[0]) - static_cast<double>(U0[nyz + nz]) -
+                                 static_cast<double>(U0[nyz + 1]) - static_cast<double>(U0[nz + 1]);
+                            clamp_cell(Ms, i, j, k);
+                        }
+                        out[u] = static_cast<OutR>(Ms);
+                    }
                 }
                 wave_sync();
             }
@@ -1293,11 +1331,12 @@ hipError_t launch_particles(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ct
                             const int* part_prob, void* lines, void* ms, DevStats* dst, unsigned long long* cursor,
                             int clamp) {
     using Line = LineVals<typename P::Real, P::NLN>;
+    if ((clamp != 0) != std::is_same<OutR, double>::value) return hipErrorInvalidValue;  // the kernel's CLAMP
     if (count > 0)
         k_particles<P, OutR><<<resident_grid(c, k_particles<P, OutR>, c->tune.item_grab, count), THREADS, 0, s>>>(
             static_cast<uint32_t>(first), static_cast<uint32_t>(count), keys_particle_order, slab, dp[0], dp[1], dp[2],
             dp[7], bbox, loffs, loff_shift, pr, part_prob, pctx, static_cast<const Line*>(lines),
-            static_cast<OutR*>(ms), dst, cursor, clamp, c->tune.item_grab, c->tune.cursors);
+            static_cast<OutR*>(ms), dst, cursor, c->tune.item_grab, c->tune.cursors);
     return hipGetLastError();
 }
 
