@@ -1733,8 +1733,20 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     // slab ranges, counters and work cursors); the accumulation waits for both
     const bool fb_on_side = split && npairs > n32 && !tabulated;
     if (fb_on_side) {
+        // (both kernels are VALU-issue bound: sharing the CUs does not shorten their sum, and a smaller
+        // fallback grid only lengthens its tail; measured 16.3 / 17.0 / 19.7 ms for 1, 1/2, 1/4 of its grid)
         KIWI_HIP_CHECK(c, hipStreamWaitEvent(c->side, c->ev[2], 0));
         KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[2], c->side));
+        if (generic) {
+            const GenTab* tabs = static_cast<const GenTab*>(c->tables.p) + K->kernel_id;
+            KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<double>, float>(c, c->side, {sup, tabs}, n32, npairs - n32, dp,
+                                                                          bbox, loffs, 1, keys_a, slab, dpr, part_prob,
+                                                                          lt, msb, dst_fb, cursor_fb, clamp)));
+        } else {
+            KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<double>, float>(c, c->side, {sup}, n32, npairs - n32, dp, bbox,
+                                                                         loffs, 1, keys_a, slab, dpr, part_prob, lt,
+                                                                         msb, dst_fb, cursor_fb, clamp)));
+        }
     }
     if (tabulated) {
         KIWI_HIP_CHECK(c, kiwi_build_m4_table(c, s));
@@ -1752,13 +1764,7 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
             KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<float>, float>(c, s, {sup, tabs}, 0, n32, dp, bbox, loffs, 0,
                                                                          keys_a, slab, dpr, part_prob, lt, msb, dst,
                                                                          cursor, clamp)));
-                if (fb_on_side) {
-                KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<double>, float>(c, c->side, {sup, tabs}, n32,
-                                                                              npairs - n32, dp, bbox, loffs, 1, keys_a,
-                                                                              slab, dpr, part_prob, lt, msb, dst_fb,
-                                                                              cursor_fb, clamp)));
             }
-        }
     } else if (fp64) {
         KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<double>, double>(c, s, {sup}, 0, npairs, dp, bbox, loffs, 0, keys_a,
                                                                       slab, dpr, part_prob, lt, msb, dst, cursor,
@@ -1767,11 +1773,6 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
         KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<float>, float>(c, s, {sup}, 0, n32, dp, bbox, loffs, 0, keys_a,
                                                                     slab, dpr, part_prob, lt, msb, dst, cursor,
                                                                     clamp)));
-        if (fb_on_side) {
-            KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<double>, float>(c, c->side, {sup}, n32, npairs - n32, dp, bbox,
-                                                                         loffs, 1, keys_a, slab, dpr, part_prob, lt,
-                                                                         msb, dst_fb, cursor_fb, clamp)));
-        }
     }
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[5], s));  // the main cell kernel's end
     if (fb_on_side) {
