@@ -18,7 +18,8 @@
 //   3. candidates       per particle, the cells whose boxes meet the support ball (count, scan, fill)
 //   4. k_vor_reach      visited flag per candidate pair; compaction; (particle, cell) sort
 //   5. owned faces      per visited pair, which of its faces it integrates and where mirrored values
-//                       come from; compaction to a flat face list
+//                       come from; compaction to a flat list of (pair, face) items, grouped by vertex
+//                       count (FP64 face walks) or sorted by face (FP32 face walks)
 //   6. k_vor_faces      the face integrals, one thread per (particle, face) -- the hot kernel
 //   7. k_vor_pairsum    per-pair cell integral in the cell's face order, clamped at 0
 //   8. accumulation     pairs sorted by (cell, particle); one owner per cell sums in particle order
@@ -31,6 +32,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdint>
 #include <cstring>
 
@@ -618,9 +620,19 @@ __global__ __launch_bounds__(THREADS) void k_vor_items(int64_t nvis, const uint6
     }
 }
 
-// Face items grouped by vertex count (a stable counting sort), so that the lanes of a wavefront walk
-// faces with the same number of edges.  One wavefront per chunk of BIN_CHUNK items; hist[bin][chunk]
-// counts, and after an exclusive scan in (bin, chunk) order gives each chunk's first slot per bin.
+// Face items in the order the face kernel reads them best.  Every item writes its own slot, so the order
+// changes no result.
+//  - FP32 face walks are close to memory-bound: the items are sorted by face (a stable radix sort of the
+//    items read as 64-bit keys, face in the high word), so the lanes of a wavefront mostly walk the same
+//    face for different particles and its vertices are read once per wavefront, not once per lane
+//    (2e5-site map: the FP32 face kernel 29.3 -> 13.3 ms).
+//  - FP64 face walks are VALU-bound and branch on the particle's distances: they keep the particle order
+//    and group the items by vertex count (a stable counting sort), so that the lanes of a wavefront walk
+//    faces with the same number of edges for one particle (sorted by face they ran 52.3 -> 58.7 ms).
+static_assert(sizeof(FaceItem) == 8 && offsetof(FaceItem, face) == 4, "FaceItem read as a 64-bit sort key");
+//
+// Vertex-count groups: one wavefront per chunk of BIN_CHUNK items; hist[bin][chunk] counts, and after an
+// exclusive scan in (bin, chunk) order gives each chunk's first slot per bin.
 constexpr int NV_BINS = 14;  // 3 .. 15 vertices, and 16 or more
 constexpr int BIN_CHUNK = 512;
 
@@ -1147,7 +1159,10 @@ int voronoi_run(kiwi_hip_ctx* c, const kiwi_particles* P, const int* home_cell, 
         k_vor_items<<<grid1d(PAIR_LANES * nvis), THREADS, 0, s>>>(nvis, vkeys, cmask, slot_off, own_off, cfp, src,
                                                                   items_pair);
     KIWI_HIP_CHECK(c, hipGetLastError());
-    if (nitems > 0) {
+    if (nitems > 0 && single) {
+        KIWI_HIP_CHECK(c, sort_keys(c, reinterpret_cast<const uint64_t*>(items_pair), reinterpret_cast<uint64_t*>(items),
+                                    nitems, 32, 32 + bits_for(nf), s));
+    } else if (nitems > 0) {
         k_fill_i64<<<1, 64, 0, s>>>(bin_cnt + NV_BINS * nchunks, 1, 0);
         k_vor_item_bins<false><<<grid1d(64 * nchunks), THREADS, 0, s>>>(nitems, nchunks, items_pair, fvp, bin_cnt,
                                                                        nullptr);
