@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define KIWI_HIP_ABI_VERSION 10
+#define KIWI_HIP_ABI_VERSION 11
 
 #define KIWI_OK 0
 #define KIWI_ERR_INVALID_ARG (-1)
@@ -302,6 +302,21 @@ typedef struct kiwi_tuning {
 #define KIWI_FP64_FALLBACK_CELLS 16.0
 int kiwi_hip_set_fp64_fallback(kiwi_hip_ctx* ctx, double support_cells);
 int kiwi_hip_get_fp64_fallback(const kiwi_hip_ctx* ctx, double* support_cells);
+
+/* Graph mode (no reference counterpart).  With on = 1, sph-to-grid calls (single and batched) whose outputs
+ * are on the device run their pipeline after the sizing pass (the item layout, sorts, line tables, cell
+ * kernels, accumulation and statistics) as a captured HIP graph: captured on the first call with a given
+ * signature (problem sizes, kernel path, launch shape and the addresses of every buffer involved), then
+ * replayed, one graph launch per kernel path, while the signature holds; a call with another signature
+ * captures anew (the context keeps the 8 most recently used).  The replay does all the work of an
+ * eager call and gives the same results bit for bit; the library's own stable radix sort replaces
+ * hipcub's in the captured pipeline (which would record memset nodes), and a capture that holds a memset
+ * node is refused with KIWI_ERR_DEVICE.  Calls with host outputs, the tabulated path and the
+ * allocation-free entry point always run eagerly.  Statistics: device_ms_prep is the sizing pass and
+ * device_ms_gather the whole replayed pipeline; the other device times are 0 for a replayed call.
+ * on = 0 (the default) releases the captured pipelines. */
+int kiwi_hip_set_graphs(kiwi_hip_ctx* ctx, int on);
+int kiwi_hip_get_graphs(const kiwi_hip_ctx* ctx, int* on);
 
 /* Set (NULL: restore the defaults) or read the context's launch shape, used by every later call. */
 int kiwi_hip_set_tuning(kiwi_hip_ctx* ctx, const kiwi_tuning* tuning);

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/kiwi_hip.h"
 #include "kernel_tables.h"
@@ -89,6 +90,15 @@ struct DevBuf {
     ScratchArena* arena = nullptr;  // carve from this area instead of allocating
 };
 
+// One captured sph-to-grid pipeline (kiwi_hip_set_graphs) and what its launches hold: sizes, launch
+// shape, and the address of every buffer they touch (a buffer that has been reallocated since the
+// capture changes the signature, so a replay only ever uses live buffers).
+struct GraphEntry {
+    std::vector<uint64_t> sig;
+    hipGraphExec_t exec = nullptr;
+};
+constexpr size_t KIWI_GRAPH_CACHE = 8;  // captured pipelines kept per context (least recently used out)
+
 // The device buffers of one sph-to-grid pipeline run: the context's own, or shadows carved from the
 // allocation-free scratch area.
 #define KIWI_PIPE_BUFS(X)                                                                                      \
@@ -123,6 +133,15 @@ struct kiwi_hip_ctx {
     // (kiwi_hip_set_fp64_fallback; <= 0: off)
     double fb_cells = KIWI_FP64_FALLBACK_CELLS;
     DevBuf vor[16];  // the Voronoi map's buffers
+    // sph-to-grid HIP graphs (kiwi_hip_set_graphs): the captured pipelines, most recently used last; the
+    // dependency-only events that fork the second stream in a capture; the graph-mode radix sort's
+    // buffers; pinned host memory that the replayed pipeline copies its statistics to
+    int graphs = 0;
+    std::vector<GraphEntry> graph_cache;
+    hipEvent_t fork = nullptr, fork2 = nullptr;
+    DevBuf g_ktmp, g_vtmp, g_hist, g_hoff;
+    void* g_host = nullptr;
+    size_t g_host_cap = 0;
     // the allocation-free entry point: its arena over the device's scratch area and shadow buffers
     ScratchArena noalloc_arena;
     DevBuf noalloc_bufs[KIWI_N_PIPE_BUFS];
@@ -138,7 +157,7 @@ struct kiwi_hip_ctx {
         for (DevBuf* b : {&parts, &bbox, &sizes, &offsets, &keys_a, &keys_b, &pos_a, &pos_b, &cells, &slab_sorted,
                           &slab, &cub_tmp, &ranges, &work, &work_sorted, &order_in, &order, &lines, &ms, &stats,
                           &outbuf, &vsum, &tables, &m4table, &probs, &part_prob, &acc_units, &acc_partial,
-                          &acc_parts})
+                          &acc_parts, &g_ktmp, &g_vtmp, &g_hist, &g_hoff})
             f(*b);
         for (DevBuf& b : vor) f(b);
     }

@@ -46,6 +46,7 @@
 #include "gen_lattice.h"
 #include "m4_lattice.h"
 #include "m4_table.h"
+#include "radix_sort.h"
 
 using namespace kiwi_hip;
 
@@ -1432,7 +1433,7 @@ bool same_path(const kiwi_batch_problem& a, const KernelChoice& ka, const kiwi_b
 
 // one pipeline run over a group of problems with the same kernel path (a single call is a group of one)
 int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kernel* K, const KernelChoice& kc,
-              int precision, const PipeBufs& B) {
+              int precision, const PipeBufs& B, bool allow_graph) {
     KIWI_HIP_CHECK(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
     // a line-table pass of an earlier call that returned early (an error after its launch) reads buffers
@@ -1614,84 +1615,9 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     int64_t* cells = static_cast<int64_t*>((*B.cells).p);
     int64_t* slab_sorted = static_cast<int64_t*>((*B.slab_sorted).p);
     int64_t* slab = static_cast<int64_t*>((*B.slab).p);
-    // The line tables need only the boxes and line offsets, complete at the synchronisation above: they
-    // are computed on the second stream while the items are sorted and laid out on this one.  Buffers
-    // that grow from here on are freed with hipFree, which waits for both streams.
     const bool side_lines = n > 0 && !tabulated;
-    if (side_lines) {
-        KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[0], c->side));
-        const GenTab* tabs = static_cast<const GenTab*>(c->tables.p) + K->kernel_id;
-        void* lt = (*B.lines).p;
-        const int64_t* sel32 = split ? nblk : nullptr;
-        if (generic && fp64) {
-            KIWI_HIP_CHECK(c, launch_lines<PolicyGen<double>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, nullptr, 0,
-                                                              dpr, part_prob, lt, dst));
-        } else if (generic) {
-            KIWI_HIP_CHECK(c, launch_lines<PolicyGen<float>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, sel32, 0, dpr,
-                                                             part_prob, lt, dst));
-            if (split)
-                KIWI_HIP_CHECK(c, launch_lines<PolicyGen<double>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, nblk_fb,
-                                                                  1, dpr, part_prob, lt, dst));
-        } else if (fp64) {
-            KIWI_HIP_CHECK(c, launch_lines<PolicyM4<double>>(c, c->side, {sup}, n, dp, bbox, loffs, nullptr, 0, dpr,
-                                                             part_prob, lt, dst));
-        } else {
-            KIWI_HIP_CHECK(c, launch_lines<PolicyM4<float>>(c, c->side, {sup}, n, dp, bbox, loffs, sel32, 0, dpr,
-                                                            part_prob, lt, dst));
-            if (split)
-                KIWI_HIP_CHECK(c, launch_lines<PolicyM4<double>>(c, c->side, {sup}, n, dp, bbox, loffs, nblk_fb, 1, dpr,
-                                                                 part_prob, lt, dst));
-        }
-        KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[1], c->side));
-        KIWI_HIP_CHECK(c, hipEventRecord(c->side_done, c->side));
-    }
-    if (n > 0) {
-        k_emit<<<grid1d(n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], sup, skip_outside, bbox, offs,
-                                             split ? offs_fb : nullptr, dpr, part_prob, keys_a);
-        KIWI_HIP_CHECK(c, hipGetLastError());
-    }
     int end_bit = 33;  // at least one key bit above the particle index
     while ((1LL << (end_bit - 32)) < nblocks) ++end_bit;
-    if (npairs > 0) {
-        k_iota<<<grid1d(npairs), THREADS, 0, s>>>(pos_a, npairs);
-        size_t sort_bytes = 0;
-        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys_a, keys_b, pos_a, pos_b,
-                                                              static_cast<int>(npairs), 32, end_bit, s));
-        KIWI_HIP_CHECK(c, reserve((*B.cub_tmp), sort_bytes));
-        sort_bytes = (*B.cub_tmp).cap;
-        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairs((*B.cub_tmp).p, sort_bytes, keys_a, keys_b, pos_a, pos_b,
-                                                              static_cast<int>(npairs), 32, end_bit, s));
-    }
-    k_item_cells<<<grid1d(np1), THREADS, 0, s>>>(npairs, keys_b, bbox, dpr, part_prob, cells);
-    {
-        size_t tb = 0;
-        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cells, slab_sorted, static_cast<int>(np1), s));
-        KIWI_HIP_CHECK(c, reserve((*B.cub_tmp), tb));
-        tb = (*B.cub_tmp).cap;
-        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum((*B.cub_tmp).p, tb, cells, slab_sorted, static_cast<int>(np1),
-                                                           s));
-    }
-    if (nblocks > 0) {
-        k_fill_int<<<grid1d(2 * nblocks), THREADS, 0, s>>>(blk_begin, 2 * nblocks, 0);
-        k_fill_u64<<<grid1d(nblocks), THREADS, 0, s>>>(work, nblocks);
-        k_iota<<<grid1d(nblocks), THREADS, 0, s>>>(order_in, nblocks);
-    }
-    KIWI_HIP_CHECK(c, hipGetLastError());
-    if (npairs > 0) {
-        k_scatter<<<grid1d(npairs), THREADS, 0, s>>>(npairs, pos_b, slab_sorted, slab);
-        k_ranges<<<grid1d(npairs), THREADS, 0, s>>>(npairs, keys_b, blk_begin, blk_end);
-        k_block_work<<<grid1d(npairs), THREADS, 0, s>>>(npairs, keys_b, cells, work);
-        KIWI_HIP_CHECK(c, hipGetLastError());
-    }
-    if (nblocks > 0) {
-        size_t ob = 0;
-        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, ob, work, work_sorted, order_in, order,
-                                                                        static_cast<int>(nblocks), 0, 64, s));
-        KIWI_HIP_CHECK(c, reserve((*B.cub_tmp), ob));
-        ob = (*B.cub_tmp).cap;
-        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairsDescending((*B.cub_tmp).p, ob, work, work_sorted, order_in,
-                                                                        order, static_cast<int>(nblocks), 0, 64, s));
-    }
     // accumulation units (blocks split into chunks of ACC_CHUNK items) in LPT order; the unit count is
     // bounded by nblocks + 2 npairs / ACC_CHUNK and the split units by 2 npairs / ACC_CHUNK, so the grid
     // and the partial sums are sized without reading the counts back
@@ -1699,123 +1625,341 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     const int64_t max_units = nblocks + max_split;
     if (max_units > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "too many cell blocks in one call");
     KIWI_HIP_CHECK(c, reserve((*B.acc_units), sizeof(AccUnit) * max_units + 4 * sizeof(int) * (nblocks + 1) + 256));
+    KIWI_HIP_CHECK(c, reserve((*B.acc_partial), sizeof(double) * 4 * BLK_CELLS * max_split));
     AccUnit* units = static_cast<AccUnit*>((*B.acc_units).p);
     int* nunits = reinterpret_cast<int*>(units + max_units);
     int* nsplit = nunits + (nblocks + 1);
     int* unit_off = nsplit + (nblocks + 1);
     int* split_off = unit_off + (nblocks + 1);
-    KIWI_HIP_CHECK(c, reserve((*B.acc_partial), sizeof(double) * 4 * BLK_CELLS * max_split));
-    if (nblocks > 0) {
-        k_acc_count<<<grid1d(nblocks + 1), THREADS, 0, s>>>(static_cast<int>(nblocks), order, blk_begin, blk_end,
-                                                             nunits, nsplit);
-        size_t tb = 0;
-        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, nunits, unit_off, static_cast<int>(nblocks + 1), s));
-        KIWI_HIP_CHECK(c, reserve((*B.cub_tmp), tb));
-        for (int k = 0; k < 2; ++k) {
-            tb = (*B.cub_tmp).cap;
-            KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum((*B.cub_tmp).p, tb, k ? nsplit : nunits,
-                                                               k ? split_off : unit_off,
-                                                               static_cast<int>(nblocks + 1), s));
-        }
-        k_acc_fill<<<grid1d(nblocks), THREADS, 0, s>>>(static_cast<int>(nblocks), unit_off, split_off, units);
-        KIWI_HIP_CHECK(c, hipGetLastError());
-    }
     double* partial = static_cast<double*>((*B.acc_partial).p);
-    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[1], s));
-    if (side_lines) KIWI_HIP_CHECK(c, hipStreamWaitEvent(s, c->side_done, 0));  // the line tables are in
+    if (tabulated) KIWI_HIP_CHECK(c, kiwi_build_m4_table(c, s));  // once per context, outside any capture
+
+    // Graph mode (kiwi_hip_set_graphs): everything from here to the statistics' copies is captured into a
+    // HIP graph once per signature (sizes, launch shape, every buffer address) and replayed afterwards.
+    // It needs device outputs (host copies would bake the caller's host addresses into the graph) and
+    // sorts with the library's own radix sort, which records no memset nodes.
+    bool dev_out = true;
+    for (int q = 0; q < nprob; ++q) dev_out = dev_out && hp[q].out->on_device;
+    const bool graph = allow_graph && c->graphs && dev_out && !tabulated && npairs > 0;
+    // scan and sort storage, reserved before any launch of the rest (a capture must not allocate)
+    int wbits = 1;  // a block's work (its items' cells) is at most pairs_bbox
+    while (wbits < 64 && (pairs_bbox >> wbits) != 0) ++wbits;
+    {
+        size_t need = 0, tb = 0;
+        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cells, slab_sorted, static_cast<int>(np1), s));
+        need = std::max(need, tb);
+        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, nunits, unit_off, static_cast<int>(nblocks + 1),
+                                                           s));
+        need = std::max(need, tb);
+        if (graph) {
+            const int64_t m = std::max<int64_t>(npairs, nblocks);
+            KIWI_HIP_CHECK(c, rs_scan_bytes(m, &tb));
+            need = std::max(need, tb);
+            const size_t nh = sizeof(int64_t) * static_cast<size_t>(RS_BINS * rs_chunks(m) + 1);
+            KIWI_HIP_CHECK(c, reserve(c->g_ktmp, sizeof(uint64_t) * static_cast<size_t>(m + 1)));
+            KIWI_HIP_CHECK(c, reserve(c->g_vtmp, sizeof(int) * static_cast<size_t>(m + 1)));
+            KIWI_HIP_CHECK(c, reserve(c->g_hist, nh));
+            KIWI_HIP_CHECK(c, reserve(c->g_hoff, nh));
+            const size_t hb = sizeof(DevStats) * 2 * nprob + sizeof(double) * 3 * nprob;
+            if (hb > c->g_host_cap) {
+                if (c->g_host) KIWI_HIP_CHECK(c, hipHostFree(c->g_host));
+                c->g_host = nullptr;
+                c->g_host_cap = 0;
+                KIWI_HIP_CHECK(c, hipHostMalloc(&c->g_host, hb, hipHostMallocDefault));
+                c->g_host_cap = hb;
+            }
+        } else {
+            if (npairs > 0) {
+                KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys_a, keys_b, pos_a, pos_b,
+                                                                      static_cast<int>(npairs), 32, end_bit, s));
+                need = std::max(need, tb);
+            }
+            if (nblocks > 0) {
+                KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tb, work, work_sorted, order_in,
+                                                                                order, static_cast<int>(nblocks), 0,
+                                                                                64, s));
+                need = std::max(need, tb);
+            }
+        }
+        KIWI_HIP_CHECK(c, reserve((*B.cub_tmp), need));
+    }
+    void* cub = (*B.cub_tmp).p;
+    const size_t cub_cap = (*B.cub_tmp).cap;
     const int clamp = fp64 ? 1 : 0;
     const int nb = static_cast<int>(nblocks);
     unsigned long long* cursor_fb = cursor + KIWI_MAX_CURSORS * CURSOR_STRIDE;  // the fallback's work cursors
     void* lt = (*B.lines).p;
     void* msb = (*B.ms).p;
-    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[2], s));
+    const GenTab* tabs = static_cast<const GenTab*>(c->tables.p) + K->kernel_id;
     // the FP64 fallback's cell kernel runs on the second stream beside the FP32 one (disjoint items,
     // slab ranges, counters and work cursors); the accumulation waits for both
     const bool fb_on_side = split && npairs > n32 && !tabulated;
-    if (fb_on_side) {
-        // (both kernels are VALU-issue bound: sharing the CUs does not shorten their sum, and a smaller
-        // fallback grid only lengthens its tail; measured 16.3 / 17.0 / 19.7 ms for 1, 1/2, 1/4 of its grid)
-        KIWI_HIP_CHECK(c, hipStreamWaitEvent(c->side, c->ev[2], 0));
-        KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[2], c->side));
-        if (generic) {
-            const GenTab* tabs = static_cast<const GenTab*>(c->tables.p) + K->kernel_id;
-            KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<double>, float>(c, c->side, {sup, tabs}, n32, npairs - n32, dp,
-                                                                          bbox, loffs, 1, keys_a, slab, dpr, part_prob,
-                                                                          lt, msb, dst_fb, cursor_fb, clamp)));
-        } else {
-            KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<double>, float>(c, c->side, {sup}, n32, npairs - n32, dp, bbox,
-                                                                         loffs, 1, keys_a, slab, dpr, part_prob, lt,
-                                                                         msb, dst_fb, cursor_fb, clamp)));
-        }
-    }
-    if (tabulated) {
-        KIWI_HIP_CHECK(c, kiwi_build_m4_table(c, s));
-        const PolicyTab::Ctx pc{sup, static_cast<const double*>(c->m4table.p)};
-        KIWI_HIP_CHECK(c, hipEventRecord(c->ev[2], s));  // the table build is not the cell kernel's time
-        KIWI_HIP_CHECK(c, (launch_particles<PolicyTab, double>(c, s, pc, 0, npairs, dp, bbox, loffs, 0, keys_a, slab,
-                                                               dpr, part_prob, lt, msb, dst, cursor, clamp)));
-    } else if (generic) {
-        const GenTab* tabs = static_cast<const GenTab*>(c->tables.p) + K->kernel_id;
-        if (fp64) {
-            KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<double>, double>(c, s, {sup, tabs}, 0, npairs, dp, bbox, loffs,
-                                                                           0, keys_a, slab, dpr, part_prob, lt, msb,
-                                                                           dst, cursor, clamp)));
-        } else {
-            KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<float>, float>(c, s, {sup, tabs}, 0, n32, dp, bbox, loffs, 0,
-                                                                         keys_a, slab, dpr, part_prob, lt, msb, dst,
-                                                                         cursor, clamp)));
-            }
-    } else if (fp64) {
-        KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<double>, double>(c, s, {sup}, 0, npairs, dp, bbox, loffs, 0, keys_a,
-                                                                      slab, dpr, part_prob, lt, msb, dst, cursor,
-                                                                      clamp)));
-    } else {
-        KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<float>, float>(c, s, {sup}, 0, n32, dp, bbox, loffs, 0, keys_a,
-                                                                    slab, dpr, part_prob, lt, msb, dst, cursor,
-                                                                    clamp)));
-    }
-    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[5], s));  // the main cell kernel's end
-    if (fb_on_side) {
-        KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[3], c->side));
-        KIWI_HIP_CHECK(c, hipEventRecord(c->side_done, c->side));
-        KIWI_HIP_CHECK(c, hipStreamWaitEvent(s, c->side_done, 0));
-    }
-    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[3], s));
-    if (fp64)
-        KIWI_HIP_CHECK(c, launch_accum<double>(s, ap, keys_b, slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
-                                               msb, clamp, units, unit_off, split_off, max_units, partial));
-    else
-        KIWI_HIP_CHECK(c, launch_accum<float>(s, ap, keys_b, slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
-                                              msb, clamp, units, unit_off, split_off, max_units, partial));
-    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[4], s));
-    k_vsum_problems<<<nprob * VSUM_BLOCKS, THREADS, 0, s>>>(dpr, dp[3], dp[4], dp[5], vsums + 3 * nprob);
-    k_vsum_problems_final<<<(nprob + 63) / 64, 64, 0, s>>>(nprob, vsums + 3 * nprob, vsums);
-    KIWI_HIP_CHECK(c, hipGetLastError());
-
-    for (int q = 0; q < nprob; ++q) {
-        const kiwi_cell_sums* out = hp[q].out;
-        if (out->on_device) continue;
-        const int64_t nc = static_cast<int64_t>(hp[q].G->nx) * hp[q].G->ny * hp[q].G->nz;
-        double* ho[4] = {out->mass, out->px, out->py, out->pz};
-        for (int k = 0; k < 4; ++k)
-            KIWI_HIP_CHECK(c, hipMemcpyAsync(ho[k], probs[q].out[k], nc * sizeof(double), hipMemcpyDeviceToHost, s));
-    }
     std::vector<double> vs(static_cast<size_t>(3 * nprob));
     std::vector<DevStats> hs_fb(static_cast<size_t>(nprob));
-    KIWI_HIP_CHECK(c, hipMemcpyAsync(hs.data(), dst, sizeof(DevStats) * nprob, hipMemcpyDeviceToHost, s));
-    KIWI_HIP_CHECK(c, hipMemcpyAsync(hs_fb.data(), dst_fb, sizeof(DevStats) * nprob, hipMemcpyDeviceToHost, s));
-    KIWI_HIP_CHECK(c, hipMemcpyAsync(vs.data(), vsums, sizeof(double) * 3 * nprob, hipMemcpyDeviceToHost, s));
+    DevStats* g_stats = static_cast<DevStats*>(c->g_host);
+    double* g_vs = reinterpret_cast<double*>(g_stats + (graph ? 2 * nprob : 0));
+
+    // The rest of the pipeline.  Captured (graph mode): no timing events, the library's radix sort, and
+    // the statistics copied to pinned host memory.
+    auto enqueue = [&](bool capturing) -> int {
+        // The line tables need only the boxes and line offsets: they are computed on the second stream
+        // while the items are sorted and laid out on this one.
+        if (side_lines) {
+            KIWI_HIP_CHECK(c, hipEventRecord(c->fork, s));
+            KIWI_HIP_CHECK(c, hipStreamWaitEvent(c->side, c->fork, 0));
+            if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[0], c->side));
+            const int64_t* sel32 = split ? nblk : nullptr;
+            if (generic && fp64) {
+                KIWI_HIP_CHECK(c, launch_lines<PolicyGen<double>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, nullptr,
+                                                                  0, dpr, part_prob, lt, dst));
+            } else if (generic) {
+                KIWI_HIP_CHECK(c, launch_lines<PolicyGen<float>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, sel32, 0,
+                                                                 dpr, part_prob, lt, dst));
+                if (split)
+                    KIWI_HIP_CHECK(c, launch_lines<PolicyGen<double>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs,
+                                                                      nblk_fb, 1, dpr, part_prob, lt, dst));
+            } else if (fp64) {
+                KIWI_HIP_CHECK(c, launch_lines<PolicyM4<double>>(c, c->side, {sup}, n, dp, bbox, loffs, nullptr, 0, dpr,
+                                                                 part_prob, lt, dst));
+            } else {
+                KIWI_HIP_CHECK(c, launch_lines<PolicyM4<float>>(c, c->side, {sup}, n, dp, bbox, loffs, sel32, 0, dpr,
+                                                                part_prob, lt, dst));
+                if (split)
+                    KIWI_HIP_CHECK(c, launch_lines<PolicyM4<double>>(c, c->side, {sup}, n, dp, bbox, loffs, nblk_fb, 1,
+                                                                     dpr, part_prob, lt, dst));
+            }
+            if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[1], c->side));
+            KIWI_HIP_CHECK(c, hipEventRecord(c->side_done, c->side));
+        }
+        if (n > 0) {
+            k_emit<<<grid1d(n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], sup, skip_outside, bbox, offs,
+                                                 split ? offs_fb : nullptr, dpr, part_prob, keys_a);
+            KIWI_HIP_CHECK(c, hipGetLastError());
+        }
+        if (npairs > 0) {
+            k_iota<<<grid1d(npairs), THREADS, 0, s>>>(pos_a, npairs);
+            if (capturing) {
+                KIWI_HIP_CHECK(c, rs_sort_pairs(keys_a, pos_a, keys_b, pos_b, static_cast<uint64_t*>(c->g_ktmp.p),
+                                                static_cast<int*>(c->g_vtmp.p), npairs, 32, end_bit, false,
+                                                static_cast<int64_t*>(c->g_hist.p), static_cast<int64_t*>(c->g_hoff.p),
+                                                cub, cub_cap, s));
+            } else {
+                size_t tb = cub_cap;
+                KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairs(cub, tb, keys_a, keys_b, pos_a, pos_b,
+                                                                      static_cast<int>(npairs), 32, end_bit, s));
+            }
+        }
+        k_item_cells<<<grid1d(np1), THREADS, 0, s>>>(npairs, keys_b, bbox, dpr, part_prob, cells);
+        {
+            size_t tb = cub_cap;
+            KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(cub, tb, cells, slab_sorted, static_cast<int>(np1), s));
+        }
+        if (nblocks > 0) {
+            k_fill_int<<<grid1d(2 * nblocks), THREADS, 0, s>>>(blk_begin, 2 * nblocks, 0);
+            k_fill_u64<<<grid1d(nblocks), THREADS, 0, s>>>(work, nblocks);
+            k_iota<<<grid1d(nblocks), THREADS, 0, s>>>(order_in, nblocks);
+        }
+        KIWI_HIP_CHECK(c, hipGetLastError());
+        if (npairs > 0) {
+            k_scatter<<<grid1d(npairs), THREADS, 0, s>>>(npairs, pos_b, slab_sorted, slab);
+            k_ranges<<<grid1d(npairs), THREADS, 0, s>>>(npairs, keys_b, blk_begin, blk_end);
+            k_block_work<<<grid1d(npairs), THREADS, 0, s>>>(npairs, keys_b, cells, work);
+            KIWI_HIP_CHECK(c, hipGetLastError());
+        }
+        if (nblocks > 0) {
+            if (capturing) {
+                KIWI_HIP_CHECK(c, rs_sort_pairs(reinterpret_cast<const uint64_t*>(work), order_in,
+                                                reinterpret_cast<uint64_t*>(work_sorted), order,
+                                                static_cast<uint64_t*>(c->g_ktmp.p),
+                                                static_cast<int*>(c->g_vtmp.p), nblocks, 0, wbits, true,
+                                                static_cast<int64_t*>(c->g_hist.p), static_cast<int64_t*>(c->g_hoff.p),
+                                                cub, cub_cap, s));
+            } else {
+                size_t tb = cub_cap;
+                KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortPairsDescending(cub, tb, work, work_sorted, order_in,
+                                                                                order, static_cast<int>(nblocks), 0,
+                                                                                64, s));
+            }
+            k_acc_count<<<grid1d(nblocks + 1), THREADS, 0, s>>>(static_cast<int>(nblocks), order, blk_begin, blk_end,
+                                                                 nunits, nsplit);
+            for (int k = 0; k < 2; ++k) {
+                size_t tb = cub_cap;
+                KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(cub, tb, k ? nsplit : nunits, k ? split_off : unit_off,
+                                                                   static_cast<int>(nblocks + 1), s));
+            }
+            k_acc_fill<<<grid1d(nblocks), THREADS, 0, s>>>(static_cast<int>(nblocks), unit_off, split_off, units);
+            KIWI_HIP_CHECK(c, hipGetLastError());
+        }
+        if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev[1], s));
+        if (side_lines) KIWI_HIP_CHECK(c, hipStreamWaitEvent(s, c->side_done, 0));  // the line tables are in
+        if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev[2], s));
+        if (fb_on_side) {
+            // (both kernels are VALU-issue bound: sharing the CUs does not shorten their sum, and a smaller
+            // fallback grid only lengthens its tail; measured 16.3 / 17.0 / 19.7 ms for 1, 1/2, 1/4 of its grid)
+            KIWI_HIP_CHECK(c, hipEventRecord(c->fork2, s));
+            KIWI_HIP_CHECK(c, hipStreamWaitEvent(c->side, c->fork2, 0));
+            if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[2], c->side));
+            if (generic) {
+                KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<double>, float>(c, c->side, {sup, tabs}, n32, npairs - n32,
+                                                                              dp, bbox, loffs, 1, keys_a, slab, dpr,
+                                                                              part_prob, lt, msb, dst_fb, cursor_fb,
+                                                                              clamp)));
+            } else {
+                KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<double>, float>(c, c->side, {sup}, n32, npairs - n32, dp,
+                                                                             bbox, loffs, 1, keys_a, slab, dpr,
+                                                                             part_prob, lt, msb, dst_fb, cursor_fb,
+                                                                             clamp)));
+            }
+        }
+        if (tabulated) {
+            const PolicyTab::Ctx pc{sup, static_cast<const double*>(c->m4table.p)};
+            KIWI_HIP_CHECK(c, (launch_particles<PolicyTab, double>(c, s, pc, 0, npairs, dp, bbox, loffs, 0, keys_a, slab,
+                                                                   dpr, part_prob, lt, msb, dst, cursor, clamp)));
+        } else if (generic) {
+            if (fp64) {
+                KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<double>, double>(c, s, {sup, tabs}, 0, npairs, dp, bbox,
+                                                                               loffs, 0, keys_a, slab, dpr, part_prob,
+                                                                               lt, msb, dst, cursor, clamp)));
+            } else {
+                KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<float>, float>(c, s, {sup, tabs}, 0, n32, dp, bbox, loffs,
+                                                                             0, keys_a, slab, dpr, part_prob, lt, msb,
+                                                                             dst, cursor, clamp)));
+            }
+        } else if (fp64) {
+            KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<double>, double>(c, s, {sup}, 0, npairs, dp, bbox, loffs, 0,
+                                                                          keys_a, slab, dpr, part_prob, lt, msb, dst,
+                                                                          cursor, clamp)));
+        } else {
+            KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<float>, float>(c, s, {sup}, 0, n32, dp, bbox, loffs, 0, keys_a,
+                                                                        slab, dpr, part_prob, lt, msb, dst, cursor,
+                                                                        clamp)));
+        }
+        if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev[5], s));  // the main cell kernel's end
+        if (fb_on_side) {
+            if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[3], c->side));
+            KIWI_HIP_CHECK(c, hipEventRecord(c->side_done, c->side));
+            KIWI_HIP_CHECK(c, hipStreamWaitEvent(s, c->side_done, 0));
+        }
+        if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev[3], s));
+        if (fp64)
+            KIWI_HIP_CHECK(c, launch_accum<double>(s, ap, keys_b, slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
+                                                   msb, clamp, units, unit_off, split_off, max_units, partial));
+        else
+            KIWI_HIP_CHECK(c, launch_accum<float>(s, ap, keys_b, slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
+                                                  msb, clamp, units, unit_off, split_off, max_units, partial));
+        if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev[4], s));
+        k_vsum_problems<<<nprob * VSUM_BLOCKS, THREADS, 0, s>>>(dpr, dp[3], dp[4], dp[5], vsums + 3 * nprob);
+        k_vsum_problems_final<<<(nprob + 63) / 64, 64, 0, s>>>(nprob, vsums + 3 * nprob, vsums);
+        KIWI_HIP_CHECK(c, hipGetLastError());
+        if (capturing) {
+            KIWI_HIP_CHECK(c, hipMemcpyAsync(g_stats, dst, sizeof(DevStats) * 2 * nprob, hipMemcpyDeviceToHost, s));
+            KIWI_HIP_CHECK(c, hipMemcpyAsync(g_vs, vsums, sizeof(double) * 3 * nprob, hipMemcpyDeviceToHost, s));
+            return KIWI_OK;
+        }
+        for (int q = 0; q < nprob; ++q) {
+            const kiwi_cell_sums* out = hp[q].out;
+            if (out->on_device) continue;
+            const int64_t nc = static_cast<int64_t>(hp[q].G->nx) * hp[q].G->ny * hp[q].G->nz;
+            double* ho[4] = {out->mass, out->px, out->py, out->pz};
+            for (int k = 0; k < 4; ++k)
+                KIWI_HIP_CHECK(c, hipMemcpyAsync(ho[k], probs[q].out[k], nc * sizeof(double), hipMemcpyDeviceToHost, s));
+        }
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(hs.data(), dst, sizeof(DevStats) * nprob, hipMemcpyDeviceToHost, s));
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(hs_fb.data(), dst_fb, sizeof(DevStats) * nprob, hipMemcpyDeviceToHost, s));
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(vs.data(), vsums, sizeof(double) * 3 * nprob, hipMemcpyDeviceToHost, s));
+        return KIWI_OK;
+    };
+
+    if (!graph) {
+        if (const int rc = enqueue(false)) return rc;
+    } else {
+        // what the captured launches hold: every size and switch they were enqueued with, the launch shape,
+        // and the address of every buffer (a reallocated buffer changes the signature)
+        std::vector<uint64_t> sig = {static_cast<uint64_t>(nprob), static_cast<uint64_t>(n),
+                                     static_cast<uint64_t>(nblocks), static_cast<uint64_t>(npairs),
+                                     static_cast<uint64_t>(n32), static_cast<uint64_t>(nlines_total),
+                                     static_cast<uint64_t>(pairs_bbox), static_cast<uint64_t>(end_bit),
+                                     static_cast<uint64_t>(wbits), static_cast<uint64_t>(generic),
+                                     static_cast<uint64_t>(fp64), static_cast<uint64_t>(split),
+                                     static_cast<uint64_t>(skip_outside), static_cast<uint64_t>(K->kernel_id),
+                                     __builtin_bit_cast(uint64_t, sup), static_cast<uint64_t>(c->tune.item_grab),
+                                     static_cast<uint64_t>(c->tune.cursors), static_cast<uint64_t>(cub_cap),
+                                     reinterpret_cast<uint64_t>(cub), reinterpret_cast<uint64_t>(c->g_host)};
+        for (int k = 0; k < 8; ++k) sig.push_back(reinterpret_cast<uint64_t>(dp[k]));
+#define KIWI_PB_SIG(nm) sig.push_back(reinterpret_cast<uint64_t>((*B.nm).p));
+        KIWI_PIPE_BUFS(KIWI_PB_SIG)
+#undef KIWI_PB_SIG
+        for (const DevBuf* b : {&c->g_ktmp, &c->g_vtmp, &c->g_hist, &c->g_hoff, &c->tables})
+            sig.push_back(reinterpret_cast<uint64_t>(b->p));
+        auto& cache = c->graph_cache;
+        size_t hit = cache.size();
+        for (size_t i = 0; i < cache.size(); ++i)
+            if (cache[i].sig == sig) hit = i;
+        KIWI_HIP_CHECK(c, hipEventRecord(c->ev[1], s));  // the sizing pass ends here
+        if (hit == cache.size()) {
+            KIWI_HIP_CHECK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue(true);
+            hipGraph_t gr = nullptr;
+            const hipError_t ec = hipStreamEndCapture(s, &gr);
+            if (rc != KIWI_OK) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return rc;
+            }
+            if (ec != hipSuccess) return hip_fail(c, ec, "hipStreamEndCapture");
+            // a memset node would come from a library call: such graphs have given wrong results on replays
+            // after the first, so the capture is refused
+            size_t nnodes = 0;
+            (void)hipGraphGetNodes(gr, nullptr, &nnodes);
+            std::vector<hipGraphNode_t> nodes(nnodes);
+            bool has_memset = false;
+            if (nnodes > 0 && hipGraphGetNodes(gr, nodes.data(), &nnodes) == hipSuccess) {
+                for (hipGraphNode_t nd : nodes) {
+                    hipGraphNodeType ty;
+                    if (hipGraphNodeGetType(nd, &ty) == hipSuccess && ty == hipGraphNodeTypeMemset) has_memset = true;
+                }
+            }
+            if (has_memset) {
+                (void)hipGraphDestroy(gr);
+                return fail(c, KIWI_ERR_DEVICE, "the captured pipeline holds a memset node; graph mode refused");
+            }
+            hipGraphExec_t ex = nullptr;
+            const hipError_t ei = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (ei != hipSuccess) return hip_fail(c, ei, "hipGraphInstantiate");
+            if (cache.size() >= KIWI_GRAPH_CACHE) {
+                (void)hipGraphExecDestroy(cache.front().exec);
+                cache.erase(cache.begin());
+            }
+            cache.push_back(GraphEntry{std::move(sig), ex});
+        } else if (hit + 1 != cache.size()) {  // most recently used last
+            GraphEntry e = std::move(cache[hit]);
+            cache.erase(cache.begin() + static_cast<std::ptrdiff_t>(hit));
+            cache.push_back(std::move(e));
+        }
+        KIWI_HIP_CHECK(c, hipGraphLaunch(cache.back().exec, s));
+        KIWI_HIP_CHECK(c, hipEventRecord(c->ev[4], s));
+    }
     KIWI_HIP_CHECK(c, hipStreamSynchronize(s));
+    if (graph) {
+        std::memcpy(hs.data(), g_stats, sizeof(DevStats) * nprob);
+        std::memcpy(hs_fb.data(), g_stats + nprob, sizeof(DevStats) * nprob);
+        std::memcpy(vs.data(), g_vs, sizeof(double) * 3 * nprob);
+    }
+    // device times; a replayed graph is timed as a whole (device_ms_gather), after the sizing pass
     float ms_prep = 0, ms_lines = 0, ms_cells = 0, ms_accum = 0;
-    (void)hipEventElapsedTime(&ms_prep, c->ev[0], c->ev[1]);
-    (void)hipEventElapsedTime(&ms_lines, c->ev[1], c->ev[2]);
-    (void)hipEventElapsedTime(&ms_cells, c->ev[2], c->ev[3]);
-    (void)hipEventElapsedTime(&ms_accum, c->ev[3], c->ev[4]);
-    float ms_fb = 0, ms_cells_kernel = 0;
-    (void)hipEventElapsedTime(&ms_cells_kernel, c->ev[2], c->ev[5]);
-    if (fb_on_side) (void)hipEventElapsedTime(&ms_fb, c->ev_side[2], c->ev_side[3]);
-    float ms_lines_kernel = 0;
-    if (side_lines) (void)hipEventElapsedTime(&ms_lines_kernel, c->ev_side[0], c->ev_side[1]);
+    float ms_fb = 0, ms_cells_kernel = 0, ms_lines_kernel = 0;
+    if (graph) {
+        (void)hipEventElapsedTime(&ms_prep, c->ev[0], c->ev[1]);
+        (void)hipEventElapsedTime(&ms_cells, c->ev[1], c->ev[4]);
+    } else {
+        (void)hipEventElapsedTime(&ms_prep, c->ev[0], c->ev[1]);
+        (void)hipEventElapsedTime(&ms_lines, c->ev[1], c->ev[2]);
+        (void)hipEventElapsedTime(&ms_cells, c->ev[2], c->ev[3]);
+        (void)hipEventElapsedTime(&ms_accum, c->ev[3], c->ev[4]);
+        (void)hipEventElapsedTime(&ms_cells_kernel, c->ev[2], c->ev[5]);
+        if (fb_on_side) (void)hipEventElapsedTime(&ms_fb, c->ev_side[2], c->ev_side[3]);
+        if (side_lines) (void)hipEventElapsedTime(&ms_lines_kernel, c->ev_side[0], c->ev_side[1]);
+    }
     for (int q = 0; q < nprob; ++q) {
         kiwi_stats* stats = hp[q].stats;
         if (!stats) continue;
@@ -1881,7 +2025,9 @@ int kiwi_hip_create(kiwi_hip_ctx** out, int device_ordinal) {
         return KIWI_ERR_DEVICE;
     }
     if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->side_done, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->side_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork2, hipEventDisableTiming) != hipSuccess) {
         kiwi_hip_destroy(c);
         return KIWI_ERR_DEVICE;
     }
@@ -1916,6 +2062,10 @@ void kiwi_hip_destroy(kiwi_hip_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->side) (void)hipStreamSynchronize(c->side);
+    for (GraphEntry& g : c->graph_cache)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    c->graph_cache.clear();
+    if (c->g_host) (void)hipHostFree(c->g_host);
     c->for_each_buf([](DevBuf& b) {
         if (b.p) (void)hipFree(b.p);
     });
@@ -1924,6 +2074,8 @@ void kiwi_hip_destroy(kiwi_hip_ctx* c) {
     for (auto& e : c->ev_side)
         if (e) (void)hipEventDestroy(e);
     if (c->side_done) (void)hipEventDestroy(c->side_done);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->fork2) (void)hipEventDestroy(c->fork2);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1943,7 +2095,7 @@ int kiwi_hip_sph_to_grid(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_gr
     KernelChoice kc;
     if (const int rc = resolve_kernel(c, K, &kc)) return rc;
     const HostProblem hp{P, G, out, stats};
-    return run_group(c, &hp, 1, K, kc, precision, c->own_bufs());
+    return run_group(c, &hp, 1, K, kc, precision, c->own_bufs(), true);
 }
 
 int kiwi_hip_sph_to_grid_noalloc(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_grid* G, const kiwi_kernel* K,
@@ -1972,7 +2124,7 @@ int kiwi_hip_sph_to_grid_noalloc(kiwi_hip_ctx* c, const kiwi_particles* P, const
     KIWI_PIPE_BUFS(KIWI_PB_SHADOW)
 #undef KIWI_PB_SHADOW
     const HostProblem hp{P, G, out, stats};
-    const int rc = run_group(c, &hp, 1, K, kc, precision, B);
+    const int rc = run_group(c, &hp, 1, K, kc, precision, B, false);
     if (rc == KIWI_ERR_OUT_OF_MEMORY) {
         return fail(c, KIWI_ERR_LIMIT, "the problem needs more device scratch than the allocation-free entry point "
                                        "holds (KIWI_NOALLOC_SCRATCH_BYTES = " +
@@ -2015,7 +2167,7 @@ int kiwi_hip_sph_to_grid_batch(kiwi_hip_ctx* c, int n_problems, const kiwi_batch
             }
         }
         if (const int rc = run_group(c, group.data(), static_cast<int>(group.size()), &problems[i].kernel, kcs[i],
-                                     problems[i].precision, c->own_bufs()))
+                                     problems[i].precision, c->own_bufs(), true))
             return rc;
     }
     return KIWI_OK;
@@ -2031,6 +2183,26 @@ int kiwi_hip_set_tuning(kiwi_hip_ctx* c, const kiwi_tuning* t) {
     if (t->cursors < 1 || t->cursors > KIWI_MAX_CURSORS)
         return fail(c, KIWI_ERR_INVALID_ARG, "cursors must be in 1..KIWI_MAX_CURSORS");
     c->tune = *t;
+    return KIWI_OK;
+}
+
+int kiwi_hip_set_graphs(kiwi_hip_ctx* c, int on) {
+    if (!c) return KIWI_ERR_INVALID_ARG;
+    if (on != 0 && on != 1) return fail(c, KIWI_ERR_INVALID_ARG, "graph mode must be 0 or 1");
+    c->graphs = on;
+    if (!on) {  // the captured pipelines are released with the mode
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        for (GraphEntry& g : c->graph_cache)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        c->graph_cache.clear();
+    }
+    return KIWI_OK;
+}
+
+int kiwi_hip_get_graphs(const kiwi_hip_ctx* c, int* on) {
+    if (!c || !on) return KIWI_ERR_INVALID_ARG;
+    *on = c->graphs;
     return KIWI_OK;
 }
 

@@ -36,7 +36,8 @@ EXPORTED = ["kiwi_hip_abi_version", "kiwi_hip_create", "kiwi_hip_destroy", "kiwi
             "kiwi_hip_sph_to_grid_batch", "kiwi_hip_set_tuning", "kiwi_hip_get_tuning", "kiwi_hip_autotune",
             "kiwi_hip_inverse_mapping", "kiwi_hip_cartesian_map", "kiwi_hip_cartesian_lattice",
             "kiwi_hip_coverage_buffer", "kiwi_hip_voronoi_centroids", "kiwi_hip_sph_to_grid_noalloc",
-            "kiwi_hip_set_fp64_fallback", "kiwi_hip_get_fp64_fallback"]
+            "kiwi_hip_set_fp64_fallback", "kiwi_hip_get_fp64_fallback", "kiwi_hip_set_graphs",
+            "kiwi_hip_get_graphs"]
 
 
 class KiwiError(RuntimeError):
@@ -208,6 +209,10 @@ def load():
         L.kiwi_hip_coverage_buffer.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.POINTER(_d), _d, _d,
                                                ctypes.POINTER(BufferSites), ctypes.POINTER(CoverageStats)]
         L.kiwi_hip_set_tuning.restype = _i
+        L.kiwi_hip_set_graphs.argtypes = [ctypes.c_void_p, _i]
+        L.kiwi_hip_set_graphs.restype = _i
+        L.kiwi_hip_get_graphs.argtypes = [ctypes.c_void_p, ctypes.POINTER(_i)]
+        L.kiwi_hip_get_graphs.restype = _i
         L.kiwi_hip_set_fp64_fallback.argtypes = [ctypes.c_void_p, _d]
         L.kiwi_hip_set_fp64_fallback.restype = _i
         L.kiwi_hip_get_fp64_fallback.argtypes = [ctypes.c_void_p, ctypes.POINTER(_d)]
@@ -394,6 +399,15 @@ class Context:
             t = Tuning(int(item_grab if item_grab is not None else cur[0]),
                        int(cursors if cursors is not None else cur[1]))
             self._check(self.L.kiwi_hip_set_tuning(self.h, ctypes.byref(t)))
+
+    def set_graphs(self, on):
+        """Graph mode: sph-to-grid calls with device outputs replay a captured HIP graph of their pipeline."""
+        self._check(self.L.kiwi_hip_set_graphs(self.h, 1 if on else 0))
+
+    def get_graphs(self):
+        v = _i()
+        self._check(self.L.kiwi_hip_get_graphs(self.h, ctypes.byref(v)))
+        return bool(v.value)
 
     def set_fp64_fallback(self, support_cells):
         """Mixed precision's FP64 fallback: particles whose support radius spans at least support_cells grid

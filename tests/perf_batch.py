@@ -8,7 +8,9 @@ log-normal density (sigma_ln_rho 1.5), kernels alternating between the cubic spl
 
 The library's autotuner first picks the cell kernel's launch shape on the batch (untimed;
 --no-autotune keeps the default).  Then times the batch (one call) against the same problems as 64
-single calls, checks that both give bit-identical sums, and prints one JSON line.
+single calls, checks that both give bit-identical sums, times the batch again in graph mode
+(kiwi_hip_set_graphs: one captured HIP graph replayed per kernel path after the sizing pass) and checks
+it against the eager batch bit for bit, and prints one JSON line.
 
     python tests/perf_batch.py [--problems 64] [--steps 3] [--fp64]
 """
@@ -82,17 +84,22 @@ def main():
     batch_out = [tuple(o.clone() for o in q["out"]) for q in probs]
     ms_single, rs = timed(run_single)
     identical = all(bool(torch.equal(a, b)) for q, bo in zip(probs, batch_out) for a, b in zip(q["out"], bo))
+    ctx.set_graphs(True)
+    ms_graph, _ = timed(run_batch)
+    ctx.set_graphs(False)
+    graph_identical = all(bool(torch.equal(a, b)) for q, bo in zip(probs, batch_out) for a, b in zip(q["out"], bo))
     pairs = sum(r[4].pairs_sphere for r in rb)
     pairs_bbox = sum(r[4].pairs_bbox for r in rb)
     prec = "FP64" if args.fp64 else "mixed"
     res = {"workload": f"{len(probs)} cutouts, {n_parts} particles, {n_cells} cells, M4 / WC2 {prec}",
            "ms_batch": ms_batch, "ms_single_calls": ms_single, "speedup": ms_single / ms_batch,
+           "ms_batch_graph": ms_graph, "graph_bit_identical": graph_identical,
            "pairs_sphere": pairs, "pairs_bbox": pairs_bbox,
            "cell_integrals_per_s_batch": pairs / (ms_batch * 1e-3),
            "cell_integrals_per_s_single": pairs / (ms_single * 1e-3), "bit_identical": identical,
            "launch_shape": {"item_grab": shape[0], "cursors": shape[1], "autotuned": not args.no_autotune}}
-    log(f"batch {ms_batch:.2f} ms, single calls {ms_single:.2f} ms (x{ms_single / ms_batch:.2f}); "
-        f"{pairs} sphere pairs; identical: {identical}")
+    log(f"batch {ms_batch:.2f} ms, single calls {ms_single:.2f} ms (x{ms_single / ms_batch:.2f}), graph mode "
+        f"{ms_graph:.2f} ms; {pairs} sphere pairs; identical: {identical}, graph identical: {graph_identical}")
     print(json.dumps(res), flush=True)
 
 
