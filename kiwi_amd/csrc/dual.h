@@ -84,6 +84,15 @@ KIWI_HD bool operator<=(const Dual<T, N>& a, const Dual<T, N>& b) { return a.v <
 template <typename T, int N>
 KIWI_HD bool operator>=(const Dual<T, N>& a, const Dual<T, N>& b) { return a.v >= b.v; }
 
+template <typename T, int N>
+KIWI_HD T kval(const Dual<T, N>& x) {
+    return x.v;
+}
+template <typename T, int N>
+KIWI_HD bool operator!=(const Dual<T, N>& a, const Dual<T, N>& b) { return a.v != b.v; }
+template <typename T, int N>
+KIWI_HD bool operator==(const Dual<T, N>& a, const Dual<T, N>& b) { return a.v == b.v; }
+
 // unary function with value f and derivative df at the argument
 template <typename T, int N>
 KIWI_HD Dual<T, N> chain(const Dual<T, N>& x, T f, T df) {

@@ -73,6 +73,15 @@ struct GenTab {
     GenRegionT<float> regf[KIWI_TAB_MAXREG];
 };
 
+// The coefficient type of a value type: the value type itself, and for dual numbers (dual.h) the type of
+// their components, so that coefficients stay plain constants.
+template <typename R>
+struct GenCoef {
+    using type = R;
+};
+template <typename R>
+using gen_coef_t = typename GenCoef<R>::type;
+
 template <typename R>
 KIWI_GHD const GenRegionT<R>* gen_regions(const GenTab& K);
 template <>
@@ -138,7 +147,7 @@ KIWI_GHD int gen_region_of(const Tab& K, double q) {
 // src/kernel_integral_generic.cpp:96-117, reduced to the phi terms.
 template <class Tab, typename R>
 KIWI_GHD void gen_plane_phi(const Tab& K, R q0, R Phi[GEN_NREG]) {
-    const int j = gen_region_of(K, q0);
+    const int j = gen_region_of(K, static_cast<double>(kval(q0)));
     const R A3 = R(K.A3_pi / GEN_PI);
     for (int k = 1; k <= GEN_NREG; ++k) {
         R v = R(0);
@@ -171,22 +180,22 @@ KIWI_GHD void gen_plane_phi(const Tab& K, R q0, R Phi[GEN_NREG]) {
 //     S = db^2 + dc^2 (the secant recurrence J_2l = sec^(2l-2) t / (2l-1) + (2l-2)/(2l-1) J_2l-2,
 //     t = dc/db, times db^(2l)), P_l the compiled binomial sum over k >= l.
 // Every scaled term is a sum of non-negative parts.
-template <typename R>
-KIWI_GHD R gen_pyramid_nonphi(const GenRegionT<R>& G, R da, R db, R dc, R r, R T, R Kh, R inv_r) {
-    R sum = G.c1 * T;  // I_1
+template <typename R, typename C>
+KIWI_GHD R gen_pyramid_nonphi(const GenRegionT<C>& G, R da, R db, R dc, R r, R T, R Kh, R inv_r) {
+    R sum = R(G.c1) * T;  // I_1
     if (G.n_odd == 0) return sum;
     const R da2 = da * da, dab = da * db, L = da2 + db * db, r2 = r * r;
     const R u = dc * inv_r;
     R Kt = Kh, r2m = R(1);
     R It = da2 * T + dab * Kh;  // q0^2 I_-1
-    if (G.odd[0] != R(0)) sum += G.odd[0] * It;
+    if (G.odd[0] != C(0)) sum += R(G.odd[0]) * It;
 #pragma unroll
     for (int m = 1; m < GEN_NODD; ++m) {
         if (m >= G.n_odd) break;
         r2m *= r2;
         Kt = u * R(0.5 / m) * r2m + R((2.0 * m - 1.0) / (2.0 * m)) * L * Kt;
         It = da2 * It + dab * Kt;
-        if (G.odd[m] != R(0)) sum += G.odd[m] * It;
+        if (G.odd[m] != C(0)) sum += R(G.odd[m]) * It;
     }
     if (G.n_even >= 1) {
         const R db2 = db * db, S = db2 + dc * dc, dbc = db * dc;
@@ -199,7 +208,7 @@ KIWI_GHD R gen_pyramid_nonphi(const GenRegionT<R>& G, R da, R db, R dc, R r, R T
                 Jt = dbc * Sp * R(1.0 / (2.0 * l - 1.0)) + R((2.0 * l - 2.0) / (2.0 * l - 1.0)) * db2 * Jt;
             }
             R P = R(0);
-            for (int j = G.n_even - l; j >= 0; --j) P = P * da2 + G.even[l][j];
+            for (int j = G.n_even - l; j >= 0; --j) P = P * da2 + R(G.even[l][j]);
             sum += da * Jt * P;
         }
     }
@@ -209,8 +218,8 @@ KIWI_GHD R gen_pyramid_nonphi(const GenRegionT<R>& G, R da, R db, R dc, R r, R T
 // The phi-free part of the lattice value at absolute offsets (ax, ay, az) > 0, radius r (units of H)
 // in region k: the 6 ordered pyramids (wall a, edge normal b), which share T_b = atan(da dc / (db r)),
 // atanh(d_c / r) and six reciprocals.
-template <typename R>
-KIWI_GHD R gen_lattice_free(const GenRegionT<R>& G, R ax, R ay, R az, R r) {
+template <typename R, typename C>
+KIWI_GHD R gen_lattice_free(const GenRegionT<C>& G, R ax, R ay, R az, R r) {
     using M = KMath<R>;
     const R ix = M::rcp(ax), iy = M::rcp(ay), ir = M::rcp(r);
     const R ixy = M::rcp(ax * ax + ay * ay), ixz = M::rcp(ax * ax + az * az), iyz = M::rcp(ay * ay + az * az);
@@ -239,7 +248,7 @@ KIWI_GHD R gen_pyramid_G_jump(const GenTab& K, int k, R da, R db, R dc, const R*
     const R inv_r = M::rcp(r);
     const R T = M::atan_pos(da * dc * M::rcp(db) * inv_r);
     const R Kh = R(0.5) * M::log_((r + dc) * (r + dc) * M::rcp(da * da + db * db));
-    const GenRegionT<R>* G = gen_regions<R>(K);
+    const GenRegionT<gen_coef_t<R>>* G = gen_regions<gen_coef_t<R>>(K);
     const R g0 = gen_pyramid_nonphi(G[k - 1], da, db, dc, r, T, Kh, inv_r);
     const R g1 = gen_pyramid_nonphi(G[k], da, db, dc, r, T, Kh, inv_r);
     return (g0 - g1) + (Phi[k - 1] - Phi[k]) * M::atan2_pos(dc, db);
@@ -254,7 +263,7 @@ KIWI_GHD void gen_line(const GenTab& K, R da, R db, R E[GEN_NREG]) {
     for (int k = 0; k < GEN_NREG; ++k) E[k] = R(0);
     if (!(da > R(0)) || !(db > R(0))) return;
     const R rs2 = da * da + db * db;
-    const int k0 = gen_region_of(K, KMath<R>::sqrt_(rs2));
+    const int k0 = gen_region_of(K, static_cast<double>(kval(KMath<R>::sqrt_(rs2))));
     R Phi[GEN_NREG];
     gen_plane_phi(K, da, Phi);
     R off = R(0);
@@ -263,7 +272,8 @@ KIWI_GHD void gen_line(const GenTab& K, R da, R db, R E[GEN_NREG]) {
         if (k < k0 || k >= K.nreg) continue;
         // boundary between regions k and k + 1 at radius c_k
         const R ck = R(K.c[k - 1]);
-        const R dc = KMath<R>::sqrt_(std::fmax(ck * ck - rs2, R(0)));
+        const R dd = ck * ck - rs2;
+        const R dc = KMath<R>::sqrt_(dd > R(0) ? dd : R(0));
         if (dc > R(0)) off += gen_pyramid_G_jump(K, k, da, db, dc, Phi);
         E[k] = off;  // region k + 1
     }

@@ -1,5 +1,5 @@
-// libkiwi_hip -- the adjoint of the Cartesian sph-to-grid for the analytic cubic spline: given
-// dL/d(cell mass, px, py, pz), the gradients dL/d(m, x, y, z, h, vx, vy, vz) per particle.  The reference
+// libkiwi_hip -- the adjoint of the Cartesian sph-to-grid for the analytic cubic spline and the generic
+// kernels: given dL/d(cell mass, px, py, pz), the gradients dL/d(m, x, y, z, h, vx, vy, vz) per particle.  The reference
 // has no adjoint; this differentiates the forward of kiwi_hip.hip (the lattice restatement of
 // src/cartesian_grid.cpp:239-279), recomputing the kernel integrals instead of storing per-pair terms.
 //
@@ -9,8 +9,9 @@
 //   dL/dv_i   = m_i sum_c Ms_i(c) dL/dp(c)
 //   dL/dx_i   = -(m_i / h_i) sum_c w_i(c) sum_corners e dU/da
 //   dL/dh_i   = -(m_i / h_i) sum_c w_i(c) sum_corners e (a dU/da + b dU/db + c dU/dc)
-// where (a, b, c) are a corner's offsets in units of h.  dU/da comes from m4_adjoint.h (dual numbers
-// through the forward's closed form).  Cells beyond the support are exact zeros in the forward and
+// where (a, b, c) are a corner's offsets in units of h (for a generic kernel, offsets are in units of the
+// support H = s h and both chain factors carry 1/H instead of 1/h).  dU/da comes from m4_adjoint.h or
+// gen_adjoint.h (dual numbers through the forward's closed form).  Cells beyond the support are exact zeros in the forward and
 // are skipped here; in FP64 a pair the forward clamps at 0 (Ms < 0) has zero derivative.  Mixed mode
 // clamps only negative cell totals, rounding noise of empty cells, and the adjoint ignores that clamp.
 //
@@ -29,6 +30,7 @@
 #include <cstring>
 
 #include "common.h"
+#include "gen_adjoint.h"
 #include "m4_adjoint.h"
 
 using namespace kiwi_hip;
@@ -41,6 +43,44 @@ struct AdjDevStats {
     unsigned long long skipped_outside, pairs_bbox, points, cells, active;
 };
 
+// The kernel path of the adjoint: the analytic cubic spline (offsets in units of h, support 2) or a generic
+// kernel (offsets in units of the support H, support 1); the line-table entry; a lattice value with
+// its gradient.
+template <typename T>
+struct AdjM4 {
+    using Real = T;
+    using Line = AdjLine<T>;
+    struct Ctx {
+        double support_in_h;
+    };
+    static __device__ __forceinline__ double unit_in_h(const Ctx&) { return 1.0; }
+    static __device__ __forceinline__ T support2() { return T(4); }
+    static __device__ __forceinline__ Line line(const Ctx&, T da, T db) { return m4_adj_line<T>(da, db); }
+    static __device__ __forceinline__ void point(const Ctx&, T ax, T ay, T az, const Line& lxy, const Line& lxz,
+                                                 const Line& lyz, T& W, T& Wx, T& Wy, T& Wz) {
+        m4_adj_point<T>(ax, ay, az, lxy, lxz, lyz, W, Wx, Wy, Wz);
+    }
+};
+
+template <typename T>
+struct AdjGen {
+    using Real = T;
+    using Line = AdjLineGen<T>;
+    struct Ctx {
+        double support_in_h;
+        const GenTab* tab;
+    };
+    static __device__ __forceinline__ double unit_in_h(const Ctx& c) { return c.support_in_h; }
+    static __device__ __forceinline__ T support2() { return T(1); }
+    static __device__ __forceinline__ Line line(const Ctx& c, T da, T db) { return gen_adj_line<T>(*c.tab, da, db); }
+    static __device__ __forceinline__ void point(const Ctx& c, T ax, T ay, T az, const Line& lxy, const Line& lxz,
+                                                 const Line& lyz, T& W, T& Wx, T& Wy, T& Wz) {
+        // the forward's region decision (PolicyGen::region): the radius in T, compared in double
+        const int k = gen_region_of(*c.tab, static_cast<double>(KMath<T>::sqrt_(ax * ax + ay * ay + az * az)));
+        gen_adj_point<T>(*c.tab, k, ax, ay, az, lxy, lxz, lyz, W, Wx, Wy, Wz);
+    }
+};
+
 __global__ void k_adj_fill(double* p, int64_t n) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
     if (i < n) p[i] = 0.0;
@@ -51,7 +91,7 @@ __global__ void k_adj_fill(double* p, int64_t n) {
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(THREADS) void k_adj_prep(int64_t n, const double* __restrict__ x,
                                                       const double* __restrict__ y, const double* __restrict__ z,
-                                                      const double* __restrict__ h, GridGeom g,
+                                                      const double* __restrict__ h, GridGeom g, double sup,
                                                       int* __restrict__ bbox, int64_t* __restrict__ nlines,
                                                       uint32_t* __restrict__ work, int* __restrict__ order,
                                                       AdjDevStats* st) {
@@ -67,7 +107,7 @@ __global__ __launch_bounds__(THREADS) void k_adj_prep(int64_t n, const double* _
         const double hh = h[i];
         if (hh > 0.0) {
             int q[6];
-            if (particle_bbox(g, 2.0, x[i], y[i], z[i], hh, q)) {
+            if (particle_bbox(g, sup, x[i], y[i], z[i], hh, q)) {
                 for (int k = 0; k < 6; ++k) bb[k] = q[k];
                 const int64_t Nx = q[1] - q[0] + 1, Ny = q[3] - q[2] + 1, Nz = q[5] - q[4] + 1;
                 nl = (Nx + 1) * (Ny + 1) + (Nx + 1) * (Nz + 1) + (Ny + 1) * (Nz + 1);
@@ -101,12 +141,13 @@ __global__ __launch_bounds__(THREADS) void k_adj_prep(int64_t n, const double* _
 // 2. edge-line constants with derivatives; table layout as the forward's k_lines: three unordered
 //    axis pairs (x,y) (x,z) (y,z), each La x Lb, a-index major
 // ------------------------------------------------------------------------------------------------
-template <typename T>
+template <class P>
 __global__ __launch_bounds__(THREADS) void k_adj_lines(int64_t n, const double* __restrict__ x,
                                                        const double* __restrict__ y, const double* __restrict__ z,
                                                        const double* __restrict__ h, const int* __restrict__ bbox,
                                                        const int64_t* __restrict__ loffs, GridGeom g,
-                                                       AdjLine<T>* __restrict__ table) {
+                                                       typename P::Ctx ctx, typename P::Line* __restrict__ table) {
+    using T = typename P::Real;
     const int lane = threadIdx.x & 63;
     const int64_t nwaves = static_cast<int64_t>(gridDim.x) * WAVES;
     for (int64_t p = static_cast<int64_t>(blockIdx.x) * WAVES + wave_uniform(threadIdx.x >> 6); p < n; p += nwaves) {
@@ -115,9 +156,9 @@ __global__ __launch_bounds__(THREADS) void k_adj_lines(int64_t n, const double* 
         const int L0 = bb[1] - bb[0] + 2, L1 = bb[3] - bb[2] + 2, L2 = bb[5] - bb[4] + 2;
         const int c01 = L0 * L1, c02 = L0 * L2, c12 = L1 * L2;
         const int nl = c01 + c02 + c12;
-        const double inv_h = 1.0 / h[p];
+        const double inv_h = 1.0 / (P::unit_in_h(ctx) * h[p]);  // lattice units
         const double p0 = x[p], p1 = y[p], p2 = z[p];
-        AdjLine<T>* out = table + loffs[p];
+        typename P::Line* out = table + loffs[p];
         for (int t = lane; t < nl; t += 64) {
             int u = t, a = 0, b = 1, Lb = L1;
             if (u >= c01) {
@@ -134,7 +175,7 @@ __global__ __launch_bounds__(THREADS) void k_adj_lines(int64_t n, const double* 
                                     sel3(a, p0, p1, p2)) * inv_h);
             const double db = fabs((gmin(g, b) + (sel3(b, bb[0], bb[2], bb[4]) + ib) * gstep(g, b) -
                                     sel3(b, p0, p1, p2)) * inv_h);
-            out[t] = m4_adj_line<T>(static_cast<T>(da), static_cast<T>(db));
+            out[t] = P::line(ctx, static_cast<T>(da), static_cast<T>(db));
         }
     }
 }
@@ -150,26 +191,29 @@ struct AdjLds {
     T cg[3 * BLK];        // squared gap of each cell layer
 };
 
-template <typename T>
+template <class P>
 __global__ __launch_bounds__(64) void k_adj_particles(
     const int* __restrict__ order, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, const double* __restrict__ vx, const double* __restrict__ vy,
     const double* __restrict__ vz, const double* __restrict__ m, const double* __restrict__ h,
-    const int* __restrict__ bbox, const int64_t* __restrict__ loffs, const AdjLine<T>* __restrict__ table, GridGeom g,
+    const int* __restrict__ bbox, const int64_t* __restrict__ loffs, const typename P::Line* __restrict__ table,
+    typename P::Ctx ctx, GridGeom g,
     const double* __restrict__ gm, const double* __restrict__ gx, const double* __restrict__ gy,
     const double* __restrict__ gz, int clamp_pairs, double* __restrict__ o_m, double* __restrict__ o_x,
     double* __restrict__ o_y, double* __restrict__ o_z, double* __restrict__ o_h, double* __restrict__ o_vx,
     double* __restrict__ o_vy, double* __restrict__ o_vz, AdjDevStats* st) {
+    using T = typename P::Real;
+    using Line = typename P::Line;
     __shared__ AdjLds<T> L;
     const int lane = threadIdx.x;
     const int p = order[blockIdx.x];
     const int* bb = bbox + 6 * static_cast<int64_t>(p);
     const double x0 = x[p], y0 = y[p], z0 = z[p], hh = h[p], mm = m[p];
     const double v0 = vx[p], v1 = vy[p], v2 = vz[p];
-    const double inv_h = 1.0 / hh;
-    const T S2 = T(4);  // support^2 in units of h
+    const double inv_h = 1.0 / (P::unit_in_h(ctx) * hh);  // lattice units (h, or the support H)
+    const T S2 = P::support2();                            // support^2 in lattice units
     const int Lx = bb[1] - bb[0] + 2, Ly = bb[3] - bb[2] + 2, Lz = bb[5] - bb[4] + 2;
-    const AdjLine<T>* lt = table + loffs[p];
+    const Line* lt = table + loffs[p];
     const int exz = Lx * Ly, eyz = exz + Lx * Lz;
     double a_m = 0, a_vx = 0, a_vy = 0, a_vz = 0, a_x = 0, a_y = 0, a_z = 0, a_h = 0;
     unsigned long long n_pts = 0, n_cells = 0;
@@ -220,11 +264,10 @@ __global__ __launch_bounds__(64) void k_adj_particles(
                     if (dx != T(0) && dy != T(0) && dz != T(0) && L.gp[i] + L.gp[nx + j] + L.gp[nx + ny + k] < S2) {
                         ++n_pts;
                         const int gi = ox + i, gj = oy + j, gk = oz + k;
-                        const AdjLine<T> lxy = lt[gi * Ly + gj], lxz = lt[exz + gi * Lz + gk],
-                                         lyz = lt[eyz + gj * Lz + gk];
+                        const Line lxy = lt[gi * Ly + gj], lxz = lt[exz + gi * Lz + gk], lyz = lt[eyz + gj * Lz + gk];
                         T W, Wx, Wy, Wz;
                         const T ax = dx < T(0) ? -dx : dx, ay = dy < T(0) ? -dy : dy, az = dz < T(0) ? -dz : dz;
-                        m4_adj_point<T>(ax, ay, az, lxy, lxz, lyz, W, Wx, Wy, Wz);
+                        P::point(ctx, ax, ay, az, lxy, lxz, lyz, W, Wx, Wy, Wz);
                         const T sx = dx < T(0) ? T(-1) : T(1), sy = dy < T(0) ? T(-1) : T(1),
                                 sz = dz < T(0) ? T(-1) : T(1);
                         U = sx * sy * sz * W;
@@ -286,7 +329,8 @@ __global__ __launch_bounds__(64) void k_adj_particles(
         n_cells += __shfl_down(n_cells, off);
     }
     if (lane == 0) {
-        const double s = -mm * inv_h;
+        // an offset is (corner - x) / (u h) in lattice units u h: d/dx = -1 / (u h), d/dh = -offset / h
+        const double s = -mm * inv_h, sh = -mm / hh;
         o_m[p] = a_m;
         o_vx[p] = mm * a_vx;
         o_vy[p] = mm * a_vy;
@@ -294,7 +338,7 @@ __global__ __launch_bounds__(64) void k_adj_particles(
         o_x[p] = s * a_x;
         o_y[p] = s * a_y;
         o_z[p] = s * a_z;
-        o_h[p] = s * a_h;
+        o_h[p] = sh * a_h;
         atomicAdd(&st->points, n_pts);
         atomicAdd(&st->cells, n_cells);
     }
@@ -314,22 +358,23 @@ size_t carve_bytes(std::initializer_list<size_t> parts) {
     return t + 256;
 }
 
-template <typename T>
-hipError_t run_adjoint(kiwi_hip_ctx* c, hipStream_t s, int64_t n, int64_t n_active, const double* const* dp,
-                       const int* bbox, const int64_t* loffs, int64_t nlines, const int* order, const GridGeom& g,
-                       const double* const* gin, int clamp, double* const* go, AdjDevStats* dst) {
-    hipError_t e = reserve(c->lines, sizeof(AdjLine<T>) * static_cast<size_t>(std::max<int64_t>(nlines, 1)));
+template <class P>
+hipError_t run_adjoint(kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx ctx, int64_t n, int64_t n_active,
+                       const double* const* dp, const int* bbox, const int64_t* loffs, int64_t nlines, const int* order,
+                       const GridGeom& g, const double* const* gin, int clamp, double* const* go, AdjDevStats* dst) {
+    using Line = typename P::Line;
+    hipError_t e = reserve(c->lines, sizeof(Line) * static_cast<size_t>(std::max<int64_t>(nlines, 1)));
     if (e != hipSuccess) return e;
-    AdjLine<T>* table = static_cast<AdjLine<T>*>(c->lines.p);
+    Line* table = static_cast<Line*>(c->lines.p);
     if (n > 0) {
-        k_adj_lines<T><<<resident_grid(c, k_adj_lines<T>, 1, n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], bbox,
-                                                                                 loffs, g, table);
+        k_adj_lines<P><<<resident_grid(c, k_adj_lines<P>, 1, n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], bbox,
+                                                                                 loffs, g, ctx, table);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (n_active > 0) {
-        k_adj_particles<T><<<static_cast<unsigned>(n_active), 64, 0, s>>>(
-            order, dp[0], dp[1], dp[2], dp[3], dp[4], dp[5], dp[6], dp[7], bbox, loffs, table, g, gin[0], gin[1], gin[2],
-            gin[3], clamp, go[0], go[1], go[2], go[3], go[4], go[5], go[6], go[7], dst);
+        k_adj_particles<P><<<static_cast<unsigned>(n_active), 64, 0, s>>>(
+            order, dp[0], dp[1], dp[2], dp[3], dp[4], dp[5], dp[6], dp[7], bbox, loffs, table, ctx, g, gin[0], gin[1],
+            gin[2], gin[3], clamp, go[0], go[1], go[2], go[3], go[4], go[5], go[6], go[7], dst);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;
@@ -360,8 +405,10 @@ int kiwi_hip_sph_to_grid_adjoint(kiwi_hip_ctx* c, const kiwi_particles* P, const
     if (precision != KIWI_FP64 && precision != KIWI_MIXED) return fail(c, KIWI_ERR_INVALID_ARG, "unknown precision");
     KernelChoice kc;
     if (const int rc = resolve_kernel(c, K, &kc)) return rc;
-    if (kc.generic || kc.tabulated)
-        return fail(c, KIWI_ERR_UNSUPPORTED, "the adjoint is available for the analytic cubic spline (M4, support 2h)");
+    if (kc.tabulated)
+        return fail(c, KIWI_ERR_UNSUPPORTED,
+                    "the adjoint is available for the analytic and generic integrals (the tabulated integral's "
+                    "trilinear lookup is not differentiated)");
     if (P->n > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 particles in one call");
 
     KIWI_HIP_CHECK(c, hipSetDevice(c->device));
@@ -431,7 +478,8 @@ int kiwi_hip_sph_to_grid_adjoint(kiwi_hip_ctx* c, const kiwi_particles* P, const
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[0], s));
     for (int k = 0; k < 8; ++k)
         if (n) k_adj_fill<<<grid1d(n), THREADS, 0, s>>>(go[k], n);
-    k_adj_prep<<<grid1d(n1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], g, bbox, nlin, work, order_in, dst);
+    const double sup = kc.support_in_h;
+    k_adj_prep<<<grid1d(n1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], g, sup, bbox, nlin, work, order_in, dst);
     KIWI_HIP_CHECK(c, hipGetLastError());
     {
         size_t tb = 0;
@@ -458,10 +506,22 @@ int kiwi_hip_sph_to_grid_adjoint(kiwi_hip_ctx* c, const kiwi_particles* P, const
     const int64_t n_active = static_cast<int64_t>(hs0.active);
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[1], s));
     const int clamp = precision == KIWI_FP64 ? 1 : 0;
-    if (precision == KIWI_FP64)
-        KIWI_HIP_CHECK(c, run_adjoint<double>(c, s, n, n_active, dp, bbox, loffs, nlines, order, g, gin, clamp, go, dst));
+    // the forward's arithmetic: mixed precision in FP32 for the cubic spline and the generic kernels that
+    // allow it (gen_mixed_ok), FP64 otherwise
+    const GenTab* tabs = static_cast<const GenTab*>(c->tables.p) + K->kernel_id;
+    const bool single = precision == KIWI_MIXED && (!kc.generic || gen_mixed_ok(K->kernel_id));
+    if (kc.generic && single)
+        KIWI_HIP_CHECK(c, run_adjoint<AdjGen<float>>(c, s, {sup, tabs}, n, n_active, dp, bbox, loffs, nlines, order, g,
+                                                     gin, clamp, go, dst));
+    else if (kc.generic)
+        KIWI_HIP_CHECK(c, run_adjoint<AdjGen<double>>(c, s, {sup, tabs}, n, n_active, dp, bbox, loffs, nlines, order, g,
+                                                      gin, 1, go, dst));  // FP64: per-pair clamp
+    else if (single)
+        KIWI_HIP_CHECK(c, run_adjoint<AdjM4<float>>(c, s, {sup}, n, n_active, dp, bbox, loffs, nlines, order, g, gin,
+                                                    clamp, go, dst));
     else
-        KIWI_HIP_CHECK(c, run_adjoint<float>(c, s, n, n_active, dp, bbox, loffs, nlines, order, g, gin, clamp, go, dst));
+        KIWI_HIP_CHECK(c, run_adjoint<AdjM4<double>>(c, s, {sup}, n, n_active, dp, bbox, loffs, nlines, order, g, gin,
+                                                     clamp, go, dst));
     KIWI_HIP_CHECK(c, hipEventRecord(c->ev[2], s));
     if (!grad_out->on_device && n > 0) {
         double* ho[8] = {grad_out->m, grad_out->x, grad_out->y, grad_out->z,

@@ -74,6 +74,10 @@ KIWI_HD R atan_poly01(R t) {
 template <typename R>
 struct KMath;
 
+// the plain value of a scalar (dual numbers, dual.h, add their own overload)
+KIWI_HD double kval(double x) { return x; }
+KIWI_HD float kval(float x) { return x; }
+
 template <>
 struct KMath<double> {
     static KIWI_HD double rcp(double x) { return 1.0 / x; }

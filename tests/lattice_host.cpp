@@ -228,3 +228,30 @@ void lat_adj_point(double ax, double ay, double az, int single, double* out) {
     m4_adj_point<double>(ax, ay, az, lxy, lxz, lyz, out[0], out[1], out[2], out[3]);
 }
 }  // extern "C"
+
+// ---- the generic kernels' lattice gradient (gen_adjoint.h) ----
+#include "../kiwi_amd/csrc/gen_adjoint.h"
+
+extern "C" {
+// W, dW/dax, dW/day, dW/daz at absolute offsets (units of the support H) for kernel kid, in double
+// (single = 0) or float, with the region and line entries as the adjoint kernel forms them
+void lat_gen_adj_point(int kid, double ax, double ay, double az, int single, double* out) {
+    const GenTab& K = kTabs[kid];
+    if (single) {
+        const float fx = static_cast<float>(ax), fy = static_cast<float>(ay), fz = static_cast<float>(az);
+        const int k = gen_region_of(K, static_cast<double>(std::sqrt(fx * fx + fy * fy + fz * fz)));
+        const AdjLineGen<float> lxy = gen_adj_line<float>(K, fx, fy), lxz = gen_adj_line<float>(K, fx, fz),
+                                lyz = gen_adj_line<float>(K, fy, fz);
+        float W, Wx, Wy, Wz;
+        gen_adj_point<float>(K, k, fx, fy, fz, lxy, lxz, lyz, W, Wx, Wy, Wz);
+        out[0] = W;
+        out[1] = Wx;
+        out[2] = Wy;
+        out[3] = Wz;
+        return;
+    }
+    const int k = gen_region_of(K, std::sqrt(ax * ax + ay * ay + az * az));
+    const AdjLineGen<double> lxy = gen_adj_line(K, ax, ay), lxz = gen_adj_line(K, ax, az), lyz = gen_adj_line(K, ay, az);
+    gen_adj_point<double>(K, k, ax, ay, az, lxy, lxz, lyz, out[0], out[1], out[2], out[3]);
+}
+}  // extern "C"

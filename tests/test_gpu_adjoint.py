@@ -9,7 +9,11 @@ Pinned by:
     x-momentum dL/dvx = m and dL/dm = vx;
   - the single-precision lattice (KIWI_MIXED) against FP64: 2e-4 of the largest value for the mass and
     velocity gradients, 3e-2 for the position and h gradients (FP32 rounding, see the test);
-  - tests/test_lattice_math.py checks the lattice gradient itself against quadrature of the face flux.
+  - tests/test_lattice_math.py checks the lattice gradient itself against quadrature of the face flux,
+    and the generic kernels' lattice gradient against central differences of their lattice value;
+  - the generic path (M4 forced generic, M5, M6, WC2, WC4, WC6): finite differences of its own FP64 forward
+    on random particles (1e-5), the total-mass invariants on a padded domain, and mixed against FP64 for
+    the kernels the forward runs in FP32 (WC2).
 """
 import os
 import sys
@@ -40,13 +44,13 @@ class Loss:
     """L = sum_c w_c (mass_c / V - t_c)^2 + sum_c u_c . p_c  and its gradient with respect to the sums.  The
     momentum weights are scaled so that both terms give gradients of comparable size."""
 
-    def __init__(self, ctx, parts, dom, res, seed=3):
+    def __init__(self, ctx, parts, dom, res, seed=3, kernel=None):
         rng = np.random.default_rng(seed)
-        self.ctx, self.dom, self.res = ctx, dom, res
+        self.ctx, self.dom, self.res, self.kernel = ctx, dom, res, kernel
         self.V = np.prod((dom[1::2] - dom[0::2]) / np.array(res))
         pert = dict(parts)
         pert["m"] = parts["m"] * (1 + 0.05 * rng.standard_normal(len(parts["m"])))
-        self.t = ctx.sph_to_grid(pert, dom, res)[0] / self.V
+        self.t = ctx.sph_to_grid(pert, dom, res, kernel=kernel)[0] / self.V
         nc = int(np.prod(res))
         self.w = rng.uniform(0.5, 1.5, nc)
         g_mass = 2 * np.abs(self.t).max() * 0.05 / self.V
@@ -56,12 +60,12 @@ class Loss:
     def terms(self, parts):
         """per-cell terms of L (a finite difference of L sums the per-cell differences, so cells the
         perturbed particle does not reach cancel exactly)"""
-        mass, px, py, pz, _ = self.ctx.sph_to_grid(parts, self.dom, self.res)
+        mass, px, py, pz, _ = self.ctx.sph_to_grid(parts, self.dom, self.res, kernel=self.kernel)
         r = mass / self.V - self.t
         return self.w * r * r + self.u[0] * px + self.u[1] * py + self.u[2] * pz
 
     def cell_grads(self, parts):
-        mass = self.ctx.sph_to_grid(parts, self.dom, self.res)[0]
+        mass = self.ctx.sph_to_grid(parts, self.dom, self.res, kernel=self.kernel)[0]
         return (2 * self.w * (mass / self.V - self.t) / self.V, self.u[0], self.u[1], self.u[2])
 
 
@@ -182,9 +186,8 @@ def test_autograd_function_and_edge_cases(hip_ctx):
     for f in hip.GRAD_FIELDS:
         np.testing.assert_allclose(t[f].grad.cpu().numpy(), g[f], rtol=1e-12, atol=1e-14 * np.abs(g[f]).max())
         assert np.all(g[f][:6] == 0.0)
-    for kern in (hip.kernel_spec("WC2"), hip.kernel_spec("M4", tabulated=True)):
-        with pytest.raises(hip.KiwiError):
-            hip_ctx.sph_to_grid_adjoint(parts, dom, res, L.cell_grads(parts), kernel=kern)
+    with pytest.raises(hip.KiwiError):  # the tabulated integral's table lookup is not differentiated
+        hip_ctx.sph_to_grid_adjoint(parts, dom, res, L.cell_grads(parts), kernel=hip.kernel_spec("M4", tabulated=True))
 
 
 def test_mass_recovery_by_gradient_descent(hip_ctx):
@@ -219,3 +222,82 @@ def test_mass_recovery_by_gradient_descent(hip_ctx):
     # cells) stay partly unresolved, so the mass error falls by less
     assert final < 0.01 * first
     assert err < 0.7 * err0
+
+
+GENERIC = [("WC2", False), ("M5", False), ("M6", False), ("WC4", False), ("WC6", False), ("M4", True)]
+
+
+def small_cloud(seed=21):
+    """600 particles of varying h in a padded box, 24^3 cells"""
+    rng = np.random.default_rng(seed)
+    n = 600
+    parts = {"x": rng.uniform(-1, 1, n), "y": rng.uniform(-1, 1, n), "z": rng.uniform(-1, 1, n),
+             "vx": rng.standard_normal(n), "vy": rng.standard_normal(n), "vz": rng.standard_normal(n),
+             "m": rng.uniform(0.5, 1.5, n) * 1e-3, "h": rng.uniform(0.08, 0.3, n)}
+    parts = {k: np.ascontiguousarray(v) for k, v in parts.items()}
+    dom = workloads.padded_domain(parts, support_in_h=3.0)  # the widest default support (M6: 3h)
+    return parts, dom, (24, 24, 24)
+
+
+@pytest.mark.parametrize("name,generic", GENERIC)
+def test_generic_adjoint_matches_finite_differences(hip_ctx, name, generic):
+    say(f"case: generic {name}{' (forced generic)' if generic else ''}, 600 particles -> 24^3, FP64 adjoint vs "
+        f"central differences on 12 particles")
+    parts, dom, res = small_cloud()
+    kern = hip.kernel_spec(name, generic_path=generic)
+    L = Loss(hip_ctx, parts, dom, res, kernel=kern)
+    g, st = hip_ctx.sph_to_grid_adjoint(parts, dom, res, L.cell_grads(parts), kernel=kern)
+    assert st.skipped_outside == 0 and st.cells > 0
+    rng = np.random.default_rng(8)
+    idx = rng.choice(len(parts["x"]), 12, replace=False)
+    for f in ("m", "x", "y", "z", "h", "vy"):
+        scale = np.sqrt(np.mean(g[f] ** 2))
+        worst = 0.0
+        for i in idx:
+            base = parts[f][i]
+
+            def diff(step):
+                lp, lm = dict(parts), dict(parts)
+                lp[f] = parts[f].copy()
+                lm[f] = parts[f].copy()
+                lp[f][i] = base + step
+                lm[f][i] = base - step
+                return np.sum(L.terms(lp) - L.terms(lm))
+
+            if f in ("m", "vy"):
+                eps = 1e-6 * abs(base) if f == "m" else 1e-4
+                fd = diff(eps) / (2 * eps)
+            else:
+                eps = 1e-3 * parts["h"][i]
+                fd = (8 * diff(eps) - diff(2 * eps)) / (12 * eps)
+            worst = max(worst, abs(g[f][i] - fd) / max(abs(fd), 1e-2 * scale))
+        say(f"  d/d{f}: worst relative error {worst:.2e}")
+        assert worst < 1e-5, (name, f, worst)
+    # the total-mass invariants on the padded domain: dM/dm = 1, dM/dx = dM/dh = 0
+    nc = int(np.prod(res))
+    one, zero = np.ones(nc), np.zeros(nc)
+    g, _ = hip_ctx.sph_to_grid_adjoint(parts, dom, res, (one, zero, zero, zero), kernel=kern)
+    np.testing.assert_allclose(g["m"], 1.0, rtol=0, atol=1e-11)
+    m_over_h = parts["m"] / parts["h"]
+    for f in ("x", "y", "z", "h"):
+        assert np.max(np.abs(g[f]) / m_over_h) < 1e-10, (name, f)
+
+
+def test_generic_adjoint_mixed_close_to_fp64(hip_ctx):
+    say("case: WC2 (FP32 lattice in mixed precision), mixed adjoint vs FP64 adjoint")
+    parts, dom, res = small_cloud(seed=22)
+    kern = hip.kernel_spec("WC2")
+    L = Loss(hip_ctx, parts, dom, res, kernel=kern)
+    gc = L.cell_grads(parts)
+    g64, _ = hip_ctx.sph_to_grid_adjoint(parts, dom, res, gc, kernel=kern)
+    g32, _ = hip_ctx.sph_to_grid_adjoint(parts, dom, res, gc, kernel=kern, precision=hip.MIXED)
+    for f in hip.GRAD_FIELDS:
+        e = np.max(np.abs(g32[f] - g64[f])) / np.abs(g64[f]).max()
+        say(f"  d/d{f}: max |mixed - fp64| / max|fp64| = {e:.2e}")
+        assert e < (2e-4 if f in ("m", "vx", "vy", "vz") else 3e-2), (f, e)
+    # M5 has no FP32 lattice: its mixed request runs the FP64 adjoint
+    k5 = hip.kernel_spec("M5")
+    a, _ = hip_ctx.sph_to_grid_adjoint(parts, dom, res, gc, kernel=k5)
+    b, _ = hip_ctx.sph_to_grid_adjoint(parts, dom, res, gc, kernel=k5, precision=hip.MIXED)
+    for f in hip.GRAD_FIELDS:
+        assert np.array_equal(a[f], b[f])

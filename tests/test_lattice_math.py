@@ -454,3 +454,39 @@ def test_adjoint_lattice_gradient_at_region_crossings(lat):
         rr = rng.choice([1.0, 2.0])
         a, b = np.float32(rr * np.cos(th)), np.float32(rr * np.sin(th))
         assert np.all(np.isfinite(adj(float(a), float(b), rng.uniform(0.01, 2.5), 1)))
+
+
+@pytest.mark.parametrize("kid", range(6))
+def test_generic_adjoint_lattice_gradient(lat, kid):
+    """The generic kernels' lattice gradient (gen_adjoint.h, dual numbers through gen_lattice.h's closed
+    form) for M4, M5, M6, WC2, WC4, WC6: the value equals the forward's W (gen_lattice_W) and the gradient
+    equals central differences of W, at random offsets inside and beyond the support (units of H),
+    including points next to the region boundaries; the single-precision gradient of the kernels the
+    mixed mode runs in FP32 (M4, WC2) stays within 1e-4 of the double one."""
+    lat.lat_gen_adj_point.argtypes = [ctypes.c_int] + [ctypes.c_double] * 3 + [ctypes.c_int, _pd]
+    lat.lat_gen_W.restype = ctypes.c_double
+    lat.lat_gen_W.argtypes = [ctypes.c_int] + [ctypes.c_double] * 3
+
+    def adj(a, b, c, single=0):
+        o = (ctypes.c_double * 4)()
+        lat.lat_gen_adj_point(kid, a, b, c, single, o)
+        return np.array(o[:])
+
+    def W(a, b, c):
+        return lat.lat_gen_W(kid, a, b, c)
+
+    rng = np.random.default_rng(40 + kid)
+    pts = list(rng.uniform(0.01, 1.2, size=(150, 3))) + [(0.3, 0.3, 0.3), (1e-3, 0.45, 0.5), (0.5, 0.6, 0.55)]
+    eps = 1e-5
+    worst = 0.0
+    for a, b, c in pts:
+        g = adj(a, b, c)
+        assert abs(g[0] - W(a, b, c)) < 1e-13, (a, b, c)  # summation order (WC6 cancels near the edge)
+        fd = np.array([(W(a + eps, b, c) - W(a - eps, b, c)) / (2 * eps),
+                       (W(a, b + eps, c) - W(a, b - eps, c)) / (2 * eps),
+                       (W(a, b, c + eps) - W(a, b, c - eps)) / (2 * eps)])
+        err = np.max(np.abs(fd - g[1:]))
+        worst = max(worst, err)
+        assert err < 1e-7, (kid, a, b, c, g, fd)
+        if kid in (0, 3):
+            assert np.max(np.abs(adj(a, b, c, 1) - g)) < 1e-4, (kid, a, b, c)
