@@ -19,10 +19,10 @@
  * Voronoi cells are in the order of the geometry the caller passes (the reference's c_loop_all order).
  *
  * Return codes: KIWI_OK (0) or a negative KIWI_ERR_*; kiwi_hip_last_error() then holds the message.
- * A context owns two HIP streams (the main one, and a second one on which the sph-to-grid entry points
- * compute the per-particle line tables beside the item sort and mixed precision's FP64-fallback cell
- * kernel beside the FP32 one; both at default priority) and its device scratch; it must not be used from
- * two threads at once.
+ * A context owns three HIP streams, all at default priority: the main one; a second one on which the
+ * sph-to-grid entry points compute the per-particle line tables beside the item sort; a third one for
+ * mixed precision's FP64 fallback, whose line tables and cell kernel run beside the item sort and the
+ * FP32 cell kernel.  It owns its device scratch too, and must not be used from two threads at once.
  * Calls are synchronous: outputs are complete when the call returns.  Device inputs must be complete
  * when the call is made: the context's streams do not wait for work the caller queued on other streams
  * (a caller that produces inputs on its own stream synchronises that stream first).
@@ -111,17 +111,20 @@ typedef struct kiwi_stats {
                                  (they run on a second stream beside it) */
     double device_ms_gather;  /* cell integrals: lattice values and per-pair cell sums */
     double device_ms_accum;   /* accumulation of the per-pair sums into the grid */
-    double device_ms_lines_kernel; /* the line-table kernel itself, timed on the stream it runs on */
+    double device_ms_lines_kernel; /* the line-table kernels themselves, timed on the streams they run on
+                                      (from their start to the end of the last of them) */
     int64_t lattice_points[3]; /* lattice values evaluated, by radial region (inside h, 2h, outside) */
     int64_t line_evals[3];     /* edge-line constants with 0, 1 or 2 crossing evaluations */
     int64_t plane_evals;       /* wall-plane constants evaluated */
     /* mixed precision's large-support FP64 fallback (kiwi_hip_set_fp64_fallback), a part of the totals
        above: its (particle, cell block) items, sphere pairs, lattice values by region, and the time of its
-       cell kernel (device_ms_gather includes it) */
+       cell kernel */
     int64_t fp64_fallback_blocks;
     int64_t fp64_fallback_pairs;
     int64_t fp64_fallback_points[3];
-    double device_ms_gather_fallback; /* the fallback's cell kernel, on the second stream beside the other */
+    double device_ms_gather_fallback; /* the fallback's cell kernel, on the third stream: it starts when the
+                                         items are laid out, before the FP32 one (part of it falls within
+                                         device_ms_prep and device_ms_lines) */
     double device_ms_gather_kernel;   /* the main cell kernel alone (the FP32 one in mixed precision) */
     int n_warn_records;
     kiwi_warn_record warn[KIWI_MAX_WARN_RECORDS];

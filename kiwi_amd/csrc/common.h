@@ -120,9 +120,13 @@ struct kiwi_hip_ctx {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;      // second stream (default priority): line tables beside the item sort
     hipEvent_t side_done = nullptr;  // recorded on side after its work; the main stream waits on it
+    // third stream (default priority): mixed precision's FP64 fallback, its line tables and cell kernel
+    hipStream_t side2 = nullptr;
+    hipEvent_t side2_done = nullptr;  // recorded on side2 after its work; the main stream waits on it
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    // on the second stream: [0, 1] bracket the line-table kernel, [2, 3] the FP64 fallback's cell kernel
-    hipEvent_t ev_side[4] = {nullptr, nullptr, nullptr, nullptr};
+    // [0, 1] bracket the line-table kernel on the second stream; on the third: [2, 3] the FP64 fallback's
+    // cell kernel, [4, 5] its line tables
+    hipEvent_t ev_side[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     std::string err;
     DevBuf parts, bbox, sizes, offsets, keys_a, keys_b, pos_a, pos_b, cells, slab_sorted, slab, cub_tmp, ranges, work,
         work_sorted, order_in, order, lines, ms, stats, outbuf, vsum, tables, m4table, probs, part_prob, acc_units,

@@ -183,6 +183,73 @@ __device__ __forceinline__ void for_each_block(const int* bb, const GridGeom& g,
     }
 }
 
+// Boxes of up to LANE_BLOCKS cell blocks are walked by their own lane; larger ones (a support of tens of
+// cells spans thousands of blocks) by the whole wavefront, so that one large particle does not hold its
+// wavefront for thousands of serial steps.
+constexpr int LANE_BLOCKS = 8;
+
+__device__ __forceinline__ int box_blocks(const int* bb) {
+    return (bb[1] / BLK - bb[0] / BLK + 1) * (bb[3] / BLK - bb[2] / BLK + 1) * (bb[5] / BLK - bb[4] / BLK + 1);
+}
+
+// for_each_block for one particle by the whole wavefront (every argument wave-uniform, every lane
+// calling): the lanes test 64 consecutive blocks of the box at a time, in for_each_block's order (x, then
+// y, then z) and with its decisions; f(bx, by, bz, rank) runs for every block taken, rank = the number of
+// blocks taken before it.  Returns the number of blocks taken.
+template <class F>
+__device__ __forceinline__ int for_each_block_wave(const int* bb, const GridGeom& g, double px, double py,
+                                                   double pz, double hh, double sup, bool skip, F f) {
+    const double inv_h = 1.0 / hh;
+    const double lim = mul_sep(mul_sep(sup, sup), 1.00001);
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const unsigned long long lane_lt = (1ull << lane) - 1ull;
+    const int bx0 = bb[0] / BLK, by0 = bb[2] / BLK, bz0 = bb[4] / BLK;
+    const int nby = bb[3] / BLK - by0 + 1, nbz = bb[5] / BLK - bz0 + 1;
+    const int nyz = nby * nbz;
+    const int total = (bb[1] / BLK - bx0 + 1) * nyz;  // at most (8192 / 8)^3 blocks
+    int taken = 0;
+    for (int t0 = 0; t0 < total; t0 += 64) {
+        const int t = t0 + lane;
+        bool take = false;
+        int bx = 0, by = 0, bz = 0;
+        if (t < total) {
+            const int tx = t / nyz, r = t - tx * nyz;
+            const int ty = r / nbz;
+            bx = bx0 + tx;
+            by = by0 + ty;
+            bz = bz0 + (r - ty * nbz);
+            take = true;
+            if (skip) {
+                const double gx = range_gap2(g, 0, max(bb[0], bx * BLK), min(bb[1], bx * BLK + BLK - 1), px, inv_h);
+                const double gxy =
+                    add_sep(gx, range_gap2(g, 1, max(bb[2], by * BLK), min(bb[3], by * BLK + BLK - 1), py, inv_h));
+                const double gxyz =
+                    add_sep(gxy, range_gap2(g, 2, max(bb[4], bz * BLK), min(bb[5], bz * BLK + BLK - 1), pz, inv_h));
+                take = !(gx >= lim) && !(gxy >= lim) && !(gxyz >= lim);
+            }
+        }
+        const unsigned long long bal = __ballot(take);
+        if (take) f(bx, by, bz, taken + __popcll(bal & lane_lt));
+        taken += __popcll(bal);
+    }
+    return taken;
+}
+
+// the wavefront's lanes in `todo` one after the other: bb, the position and h of the lane's particle
+// broadcast to every lane, then g(src, bb, px, py, pz, hh)
+template <class G>
+__device__ __forceinline__ void each_wave_particle(unsigned long long todo, const int* bb, double px, double py,
+                                                   double pz, double hh, G g) {
+    while (todo) {
+        const int src = __ffsll(static_cast<long long>(todo)) - 1;
+        todo &= todo - 1ull;
+        int wb[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) wb[k] = __shfl(bb[k], src);
+        g(src, wb, __shfl(px, src), __shfl(py, src), __shfl(pz, src), __shfl(hh, src));
+    }
+}
+
 // Mixed precision's FP64 fallback: a particle whose support radius spans at least fb_cells of the
 // smallest cell edge of its grid has its lattice values computed in FP64.  A cell's sum is a difference
 // of O(1) lattice values while the cell holds O((cell / support)^3) of the particle, so the FP32 error of
@@ -206,12 +273,15 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
         nlines[n] = 0;
         if (fb_cells > 0.0) nblk_fb[n] = 0;
     }
+    int64_t nb = 0, nl = 0;
+    bool fb = false, wide = false;
+    int qb[6] = {1, 0, 0, 0, 0, 0};  // empty unless set below
+    double hh = 0.0, x0 = 0.0, y0 = 0.0, z0 = 0.0;
     if (i < n) {
-        int64_t nb = 0, nl = 0;
-        bool fb = false;
-        int qb[6] = {1, 0, 0, 0, 0, 0};  // empty unless set below
-        const double hh = h[i];
-        const double x0 = x[i], y0 = y[i], z0 = z[i];
+        hh = h[i];
+        x0 = x[i];
+        y0 = y[i];
+        z0 = z[i];
         if (hh > 0.0) {
             int tb[6];
             if (!particle_bbox(pr[q].g, support_in_h, x0, y0, z0, hh, tb)) {
@@ -220,8 +290,10 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
                 for (int k = 0; k < 6; ++k) qb[k] = tb[k];
                 const int64_t Nx = qb[1] - qb[0] + 1, Ny = qb[3] - qb[2] + 1, Nz = qb[5] - qb[4] + 1;
                 pairs = static_cast<unsigned long long>(Nx * Ny * Nz);
-                for_each_block(qb, pr[q].g, x0, y0, z0, hh, support_in_h, skip_outside != 0,
-                               [&](int, int, int) { ++nb; });
+                wide = box_blocks(qb) > LANE_BLOCKS;
+                if (!wide)
+                    for_each_block(qb, pr[q].g, x0, y0, z0, hh, support_in_h, skip_outside != 0,
+                                   [&](int, int, int) { ++nb; });
                 nl = (Nx + 1) * (Ny + 1) + (Nx + 1) * (Nz + 1) + (Ny + 1) * (Nz + 1);
                 if (axis_units > 0) nl += axis_units - 1 + static_cast<int64_t>(axis_units) * (Nx + Ny + Nz + 3);
                 if (fb_cells > 0.0) {
@@ -233,6 +305,17 @@ __global__ void k_prep(int64_t n, const double* __restrict__ x, const double* __
                 }
             }
         }
+    }
+    // the wide boxes' blocks, counted by the whole wavefront (every lane takes part)
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    each_wave_particle(__ballot(wide), qb, x0, y0, z0, hh,
+                       [&](int src, const int* wb, double wx, double wy, double wz, double wh) {
+                           const int wq = __shfl(q, src);
+                           const int cnt = for_each_block_wave(wb, pr[wq].g, wx, wy, wz, wh, support_in_h,
+                                                               skip_outside != 0, [](int, int, int, int) {});
+                           if (lane == src) nb = cnt;
+                       });
+    if (i < n) {
         int* bb = bbox + 6 * i;
         AccPart a;
         a.m = m[i];
@@ -260,25 +343,58 @@ __global__ void k_emit(int64_t n, const double* __restrict__ x, const double* __
                        const int64_t* __restrict__ offs_fb, const Prob* __restrict__ pr,
                        const int* __restrict__ part_prob, uint64_t* __restrict__ keys) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int* bb = bbox + 6 * i;
-    if (bb[0] > bb[1]) return;
-    const Prob& P = pr[part_prob[i]];
-    const GridGeom& g = P.g;
-    int qb[6];
-    for (int k = 0; k < 6; ++k) qb[k] = bb[k];
-    // the FP64 fallback's items follow all the others (offs[n] of them), in particle order
-    int64_t o = offs[i], o_end = offs[i + 1];
-    if (offs_fb && offs_fb[i + 1] > offs_fb[i]) {
-        o = offs[n] + offs_fb[i];
-        o_end = offs[n] + offs_fb[i + 1];
-    }
-    for_each_block(qb, g, x[i], y[i], z[i], h[i], support_in_h, skip_outside != 0, [&](int bx, int by, int bz) {
-        if (o < o_end) {
-            const uint64_t b = static_cast<uint64_t>(P.block_off) + (static_cast<uint64_t>(bx) * g.nby + by) * g.nbz + bz;
-            keys[o++] = (b << 32) | static_cast<uint64_t>(i);
+    int qb[6] = {1, 0, 0, 0, 0, 0};
+    int q = 0;
+    int64_t o = 0, o_end = 0;
+    double px = 0.0, py = 0.0, pz = 0.0, hh = 0.0;
+    bool wide = false;
+    if (i < n) {
+        const int* bb = bbox + 6 * i;
+        for (int k = 0; k < 6; ++k) qb[k] = bb[k];
+        if (qb[0] <= qb[1]) {
+            q = part_prob[i];
+            px = x[i];
+            py = y[i];
+            pz = z[i];
+            hh = h[i];
+            // the FP64 fallback's items follow all the others (offs[n] of them), in particle order
+            o = offs[i];
+            o_end = offs[i + 1];
+            if (offs_fb && offs_fb[i + 1] > offs_fb[i]) {
+                o = offs[n] + offs_fb[i];
+                o_end = offs[n] + offs_fb[i + 1];
+            }
+            wide = box_blocks(qb) > LANE_BLOCKS;
+            if (!wide) {
+                const Prob& P = pr[q];
+                const GridGeom& g = P.g;
+                for_each_block(qb, g, px, py, pz, hh, support_in_h, skip_outside != 0, [&](int bx, int by, int bz) {
+                    if (o < o_end) {
+                        const uint64_t b =
+                            static_cast<uint64_t>(P.block_off) + (static_cast<uint64_t>(bx) * g.nby + by) * g.nbz + bz;
+                        keys[o++] = (b << 32) | static_cast<uint64_t>(i);
+                    }
+                });
+            }
         }
-    });
+    }
+    // the wide boxes by the whole wavefront: consecutive lanes write consecutive keys
+    const int64_t i0 = i - static_cast<int64_t>(threadIdx.x & 63);  // the wavefront's first particle
+    each_wave_particle(__ballot(wide), qb, px, py, pz, hh,
+                       [&](int src, const int* wb, double wx, double wy, double wz, double wh) {
+                           const Prob& P = pr[__shfl(q, src)];
+                           const GridGeom& g = P.g;
+                           const int64_t wo = __shfl(o, src), wend = __shfl(o_end, src);
+                           const uint64_t wi = static_cast<uint64_t>(i0 + src);
+                           for_each_block_wave(wb, g, wx, wy, wz, wh, support_in_h, skip_outside != 0,
+                                               [&](int bx, int by, int bz, int rank) {
+                                                   if (wo + rank < wend) {
+                                                       const uint64_t b = static_cast<uint64_t>(P.block_off) +
+                                                                          (static_cast<uint64_t>(bx) * g.nby + by) * g.nbz + bz;
+                                                       keys[wo + rank] = (b << 32) | wi;
+                                                   }
+                                               });
+                       });
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1436,9 +1552,10 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
               int precision, const PipeBufs& B, bool allow_graph) {
     KIWI_HIP_CHECK(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
-    // a line-table pass of an earlier call that returned early (an error after its launch) reads buffers
-    // this call rewrites: the main stream waits for it first
+    // a line-table pass or fallback kernel of an earlier call that returned early (an error after its
+    // launch) reads buffers this call rewrites: the main stream waits for them first
     KIWI_HIP_CHECK(c, hipStreamWaitEvent(s, c->side_done, 0));
+    KIWI_HIP_CHECK(c, hipStreamWaitEvent(s, c->side2_done, 0));
     const double sup = kc.support_in_h;
     const bool generic = kc.generic;
 
@@ -1703,10 +1820,23 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     // the statistics copied to pinned host memory.
     auto enqueue = [&](bool capturing) -> int {
         // The line tables need only the boxes and line offsets: they are computed on the second stream
-        // while the items are sorted and laid out on this one.
+        // while the items are sorted and laid out on this one.  Mixed precision's FP64 fallback computes its
+        // own line tables on the third stream, where its cell kernel follows them as soon as the items are
+        // laid out: it is the longest of the cell kernels, and starts before the FP32 one.
         if (side_lines) {
             KIWI_HIP_CHECK(c, hipEventRecord(c->fork, s));
             KIWI_HIP_CHECK(c, hipStreamWaitEvent(c->side, c->fork, 0));
+            if (fb_on_side) {
+                KIWI_HIP_CHECK(c, hipStreamWaitEvent(c->side2, c->fork, 0));
+                if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[4], c->side2));
+                if (generic)
+                    KIWI_HIP_CHECK(c, launch_lines<PolicyGen<double>>(c, c->side2, {sup, tabs}, n, dp, bbox, loffs,
+                                                                      nblk_fb, 1, dpr, part_prob, lt, dst));
+                else
+                    KIWI_HIP_CHECK(c, launch_lines<PolicyM4<double>>(c, c->side2, {sup}, n, dp, bbox, loffs, nblk_fb, 1,
+                                                                     dpr, part_prob, lt, dst));
+                if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[5], c->side2));
+            }
             if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[0], c->side));
             const int64_t* sel32 = split ? nblk : nullptr;
             if (generic && fp64) {
@@ -1715,18 +1845,12 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
             } else if (generic) {
                 KIWI_HIP_CHECK(c, launch_lines<PolicyGen<float>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, sel32, 0,
                                                                  dpr, part_prob, lt, dst));
-                if (split)
-                    KIWI_HIP_CHECK(c, launch_lines<PolicyGen<double>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs,
-                                                                      nblk_fb, 1, dpr, part_prob, lt, dst));
             } else if (fp64) {
                 KIWI_HIP_CHECK(c, launch_lines<PolicyM4<double>>(c, c->side, {sup}, n, dp, bbox, loffs, nullptr, 0, dpr,
                                                                  part_prob, lt, dst));
             } else {
                 KIWI_HIP_CHECK(c, launch_lines<PolicyM4<float>>(c, c->side, {sup}, n, dp, bbox, loffs, sel32, 0, dpr,
                                                                 part_prob, lt, dst));
-                if (split)
-                    KIWI_HIP_CHECK(c, launch_lines<PolicyM4<double>>(c, c->side, {sup}, n, dp, bbox, loffs, nblk_fb, 1,
-                                                                     dpr, part_prob, lt, dst));
             }
             if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[1], c->side));
             KIWI_HIP_CHECK(c, hipEventRecord(c->side_done, c->side));
@@ -1791,26 +1915,30 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
             KIWI_HIP_CHECK(c, hipGetLastError());
         }
         if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev[1], s));
-        if (side_lines) KIWI_HIP_CHECK(c, hipStreamWaitEvent(s, c->side_done, 0));  // the line tables are in
-        if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev[2], s));
         if (fb_on_side) {
-            // (both kernels are VALU-issue bound: sharing the CUs does not shorten their sum, and a smaller
-            // fallback grid only lengthens its tail; measured 16.3 / 17.0 / 19.7 ms for 1, 1/2, 1/4 of its grid)
+            // the fallback's cell kernel on the third stream, after its line tables there and the item layout
+            // here (both cell kernels are VALU-issue bound: sharing the CUs does not shorten their sum, and a
+            // smaller fallback grid only lengthens its tail; measured 16.3 / 17.0 / 19.7 ms for 1, 1/2, 1/4 of
+            // its grid)
             KIWI_HIP_CHECK(c, hipEventRecord(c->fork2, s));
-            KIWI_HIP_CHECK(c, hipStreamWaitEvent(c->side, c->fork2, 0));
-            if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[2], c->side));
+            KIWI_HIP_CHECK(c, hipStreamWaitEvent(c->side2, c->fork2, 0));
+            if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[2], c->side2));
             if (generic) {
-                KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<double>, float>(c, c->side, {sup, tabs}, n32, npairs - n32,
-                                                                              dp, bbox, loffs, 1, keys_a, slab, dpr,
-                                                                              part_prob, lt, msb, dst_fb, cursor_fb,
-                                                                              clamp)));
+                KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<double>, float>(c, c->side2, {sup, tabs}, n32,
+                                                                              npairs - n32, dp, bbox, loffs, 1, keys_a,
+                                                                              slab, dpr, part_prob, lt, msb, dst_fb,
+                                                                              cursor_fb, clamp)));
             } else {
-                KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<double>, float>(c, c->side, {sup}, n32, npairs - n32, dp,
+                KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<double>, float>(c, c->side2, {sup}, n32, npairs - n32, dp,
                                                                              bbox, loffs, 1, keys_a, slab, dpr,
                                                                              part_prob, lt, msb, dst_fb, cursor_fb,
                                                                              clamp)));
             }
+            if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[3], c->side2));
+            KIWI_HIP_CHECK(c, hipEventRecord(c->side2_done, c->side2));
         }
+        if (side_lines) KIWI_HIP_CHECK(c, hipStreamWaitEvent(s, c->side_done, 0));  // the line tables are in
+        if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev[2], s));
         if (tabulated) {
             const PolicyTab::Ctx pc{sup, static_cast<const double*>(c->m4table.p)};
             KIWI_HIP_CHECK(c, (launch_particles<PolicyTab, double>(c, s, pc, 0, npairs, dp, bbox, loffs, 0, keys_a, slab,
@@ -1835,11 +1963,7 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
                                                                         clamp)));
         }
         if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev[5], s));  // the main cell kernel's end
-        if (fb_on_side) {
-            if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[3], c->side));
-            KIWI_HIP_CHECK(c, hipEventRecord(c->side_done, c->side));
-            KIWI_HIP_CHECK(c, hipStreamWaitEvent(s, c->side_done, 0));
-        }
+        if (fb_on_side) KIWI_HIP_CHECK(c, hipStreamWaitEvent(s, c->side2_done, 0));
         if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev[3], s));
         if (fp64)
             KIWI_HIP_CHECK(c, launch_accum<double>(s, ap, keys_b, slab_sorted, blk_begin, blk_end, order, nb, dpr, nprob,
@@ -1958,7 +2082,13 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
         (void)hipEventElapsedTime(&ms_accum, c->ev[3], c->ev[4]);
         (void)hipEventElapsedTime(&ms_cells_kernel, c->ev[2], c->ev[5]);
         if (fb_on_side) (void)hipEventElapsedTime(&ms_fb, c->ev_side[2], c->ev_side[3]);
-        if (side_lines) (void)hipEventElapsedTime(&ms_lines_kernel, c->ev_side[0], c->ev_side[1]);
+        if (side_lines) {
+            // from the line tables' start to the end of the last of them (the fallback's on the third stream)
+            (void)hipEventElapsedTime(&ms_lines_kernel, c->ev_side[0], c->ev_side[1]);
+            float ms_fb_lines = 0;
+            if (fb_on_side && hipEventElapsedTime(&ms_fb_lines, c->ev_side[0], c->ev_side[5]) == hipSuccess)
+                ms_lines_kernel = std::max(ms_lines_kernel, ms_fb_lines);
+        }
     }
     for (int q = 0; q < nprob; ++q) {
         kiwi_stats* stats = hp[q].stats;
@@ -2026,6 +2156,8 @@ int kiwi_hip_create(kiwi_hip_ctx** out, int device_ordinal) {
     }
     if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->side_done, hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->side2_done, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->fork2, hipEventDisableTiming) != hipSuccess) {
         kiwi_hip_destroy(c);
@@ -2062,6 +2194,7 @@ void kiwi_hip_destroy(kiwi_hip_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->side2) (void)hipStreamSynchronize(c->side2);
     for (GraphEntry& g : c->graph_cache)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     c->graph_cache.clear();
@@ -2074,9 +2207,11 @@ void kiwi_hip_destroy(kiwi_hip_ctx* c) {
     for (auto& e : c->ev_side)
         if (e) (void)hipEventDestroy(e);
     if (c->side_done) (void)hipEventDestroy(c->side_done);
+    if (c->side2_done) (void)hipEventDestroy(c->side2_done);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->fork2) (void)hipEventDestroy(c->fork2);
     if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->side2) (void)hipStreamDestroy(c->side2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
