@@ -34,7 +34,7 @@
 #include "m4_lattice.h"
 
 #ifdef __HIPCC__
-#define KIWI_GHD __host__ __device__ inline
+#define KIWI_GHD __host__ __device__ __forceinline__
 #else
 #define KIWI_GHD inline
 #endif
@@ -93,7 +93,7 @@ KIWI_GHD const GenRegionT<float>* gen_regions<float>(const GenTab& K) {
     return K.regf;
 }
 
-inline void gen_tab_init(GenTab& G, const KiwiKernelTable& K) {
+constexpr void gen_tab_init(GenTab& G, const KiwiKernelTable& K) {
     G = GenTab{};
     G.nreg = K.nreg;
     for (int k = 0; k < KIWI_TAB_MAXREG; ++k) G.c[k] = K.c[k];
@@ -132,6 +132,24 @@ inline void gen_tab_init(GenTab& G, const KiwiKernelTable& K) {
         for (int l = 0; l <= GEN_NEVEN; ++l)
             for (int j = 0; j < GEN_NEVEN; ++j) F.even[l][j] = static_cast<float>(R.even[l][j]);
     }
+}
+
+// The six kernels' tables as constants of the code (order M4 M5 M6 WC2 WC4 WC6).
+constexpr KiwiKernelTable kGenTableData[KIWI_TAB_NKERNELS] = {KIWI_KERNEL_TABLE_DATA};
+
+constexpr GenTab gen_tab_make(const KiwiKernelTable& K) {
+    GenTab G{};
+    gen_tab_init(G, K);
+    return G;
+}
+
+// Kernel KID's GenTab compiled into the code: evaluated through it (every function here inlines), the
+// lattice functions fold the kernel's coefficients, loop bounds and zero terms into their instructions
+// instead of reading and testing them at run time.
+template <int KID>
+KIWI_GHD const GenTab& gen_fixed_tab() {
+    static constexpr GenTab T = gen_tab_make(kGenTableData[KID]);
+    return T;
 }
 
 // region of a radius q (units of H), 1-based: region_of, src/kernel_integral_generic.cpp:68-72
@@ -217,10 +235,12 @@ KIWI_GHD R gen_pyramid_nonphi(const GenRegionT<C>& G, R da, R db, R dc, R r, R T
 
 // The phi-free part of the lattice value at absolute offsets (ax, ay, az) > 0, radius r (units of H)
 // in region k: the 6 ordered pyramids (wall a, edge normal b), which share T_b = atan(da dc / (db r)),
-// atanh(d_c / r) and six reciprocals.
+// atanh(d_c / r) and six reciprocals.  A region holding only the I_1 term (beyond the support) gives
+// c1 times the sum of the six T, twice T0 + T1 + T2 = pi: a constant, returned as such.
 template <typename R, typename C>
 KIWI_GHD R gen_lattice_free(const GenRegionT<C>& G, R ax, R ay, R az, R r) {
     using M = KMath<R>;
+    if (G.n_odd == 0 && G.n_even == 0) return R(static_cast<double>(G.c1) * GEN_PI);
     const R ix = M::rcp(ax), iy = M::rcp(ay), ir = M::rcp(r);
     const R ixy = M::rcp(ax * ax + ay * ay), ixz = M::rcp(ax * ax + az * az), iyz = M::rcp(ay * ay + az * az);
     // T0 + T1 + T2 = pi/2 for positive offsets (m4_lattice.h): one arctangent fewer per point
@@ -237,6 +257,44 @@ KIWI_GHD R gen_lattice_free(const GenRegionT<C>& G, R ax, R ay, R az, R r) {
     s += gen_pyramid_nonphi(G, az, ax, ay, r, T0, K1, ir);  // (z, x, y)
     s += gen_pyramid_nonphi(G, az, ay, ax, r, T1, K0, ir);  // (z, y, x)
     return s;
+}
+
+// gen_lattice_free for kernel KID (gen_fixed_tab) in region k (1-based).  The outermost region holds only
+// the I_1 term (every kernel's table), a constant; the inner regions share one evaluation whose loop
+// bounds are the largest of theirs and whose coefficients are selected per region among constants.
+template <typename R, int KID>
+KIWI_GHD R gen_lattice_free_fixed(R ax, R ay, R az, R r, int k) {
+    using C = gen_coef_t<R>;
+    constexpr int NR = kGenTableData[KID].nreg;
+    const GenRegionT<C>* G = gen_regions<C>(gen_fixed_tab<KID>());
+    if (k >= NR) return R(static_cast<double>(G[NR - 1].c1) * GEN_PI);
+    if constexpr (NR == 2) {
+        return gen_lattice_free<R>(G[0], ax, ay, az, r);
+    } else {
+        auto pick = [&](auto field) {  // field(G[k - 1]) among the inner regions
+            auto v = field(G[NR - 2]);
+#pragma unroll
+            for (int q = NR - 3; q >= 0; --q) v = (k == q + 1) ? field(G[q]) : v;
+            return v;
+        };
+        GenRegionT<C> S{};
+        S.n_odd = 0;
+        S.n_even = 0;
+#pragma unroll
+        for (int q = 0; q < NR - 1; ++q) {
+            S.n_odd = G[q].n_odd > S.n_odd ? G[q].n_odd : S.n_odd;
+            S.n_even = G[q].n_even > S.n_even ? G[q].n_even : S.n_even;
+        }
+        S.c1 = pick([](const GenRegionT<C>& g) { return g.c1; });
+#pragma unroll
+        for (int m = 0; m < GEN_NODD; ++m) S.odd[m] = pick([m](const GenRegionT<C>& g) { return g.odd[m]; });
+#pragma unroll
+        for (int l = 1; l <= GEN_NEVEN; ++l)
+#pragma unroll
+            for (int j = 0; j < GEN_NEVEN; ++j)
+                S.even[l][j] = pick([l, j](const GenRegionT<C>& g) { return g.even[l][j]; });
+        return gen_lattice_free<R>(S, ax, ay, az, r);
+    }
 }
 
 // G_k - G_(k+1) at the pyramid (da, db, dc), phi = atan(dc / db), Phi the wall's phi coefficients:

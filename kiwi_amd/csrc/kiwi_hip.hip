@@ -537,8 +537,9 @@ struct PolicyM4 {
 // support 1; regions of the kernel table; plane values Phi_k; the line table holds the offsets of
 // every region.  R = double is the parity mode; R = float (mixed) is used for the kernels whose
 // single-precision lattice values are as accurate as the cubic spline's (gen_mixed_ok), with the plane
-// values still formed in double and rounded.
-template <typename R>
+// values still formed in double and rounded.  KID >= 0: kernel KID's table compiled into the code
+// (gen_fixed_tab); KID = -1: the context's table, read at run time.
+template <typename R, int KID = -1>
 struct PolicyGen {
     using Real = R;
     static constexpr int WAVES_PER_EU = 1;
@@ -550,27 +551,54 @@ struct PolicyGen {
         double support_in_h;  // H / h
         const GenTab* tab;
     };
+    static __device__ __forceinline__ const GenTab& tab(const Ctx& c) {
+        if constexpr (KID >= 0)
+            return gen_fixed_tab<KID>();
+        else
+            return *c.tab;
+    }
     static __device__ __forceinline__ double unit_in_h(const Ctx& c) { return c.support_in_h; }  // lattice unit: H
     static __device__ __forceinline__ R support2() { return R(1); }
     static __device__ __forceinline__ int region(const Ctx& c, R r2) {
-        return gen_region_of(*c.tab, static_cast<double>(KMath<R>::sqrt_(r2)));
+        return gen_region_of(tab(c), static_cast<double>(KMath<R>::sqrt_(r2)));
     }
     static __device__ __forceinline__ void plane(const Ctx& c, R ad, R* v) {
         double Phi[GEN_NREG];
-        gen_plane_phi(*c.tab, static_cast<double>(ad), Phi);
+        gen_plane_phi(tab(c), static_cast<double>(ad), Phi);
 #pragma unroll
         for (int k = 0; k < GEN_NREG; ++k) v[k] = static_cast<R>(Phi[k]);
     }
     static __device__ __forceinline__ int line_slot(int k) { return k - 1; }
-    static __device__ __forceinline__ void line(const Ctx& c, R da, R db, R* v) { gen_line<R>(*c.tab, da, db, v); }
+    static __device__ __forceinline__ void line(const Ctx& c, R da, R db, R* v) { gen_line<R>(tab(c), da, db, v); }
     static constexpr bool FUSED_PAIRS = false;
     static __device__ __forceinline__ void line_pair(const Ctx&, R, const R*, R, const R*, R*) {}
     static __device__ __forceinline__ int line_kind(double l2) { return l2 < 1.0 ? 1 : 0; }
     static __device__ __forceinline__ R free_part(const Ctx& c, R ax, R ay, R az, R r, int k) {
-        return gen_lattice_free<R>(gen_regions<R>(*c.tab)[k - 1], ax, ay, az, r);
+        if constexpr (KID >= 0)
+            return gen_lattice_free_fixed<R, KID>(ax, ay, az, r, k);
+        else
+            return gen_lattice_free<R>(gen_regions<R>(*c.tab)[k - 1], ax, ay, az, r);
     }
     static __device__ __forceinline__ R plane_weight() { return R(0.5 * GEN_PI); }
 };
+
+// Calls f(P{}) with the generic policy P for kernel kid in precision R: the kernels that run in single
+// precision (the cubic spline on the generic path, Wendland C2) with their tables compiled in; in
+// double, the others through the context's table.
+template <typename R, class F>
+hipError_t with_gen_policy(int kid, F&& f) {
+    switch (kid) {
+        case KIWI_KERNEL_M4:
+            return f(PolicyGen<R, KIWI_KERNEL_M4>{});
+        case KIWI_KERNEL_WC2:
+            return f(PolicyGen<R, KIWI_KERNEL_WC2>{});
+        default:
+            if constexpr (std::is_same<R, float>::value)
+                return hipErrorInvalidValue;  // single precision only for the two above (gen_mixed_ok)
+            else
+                return f(PolicyGen<R>{});
+    }
+}
 
 // Tabulated cubic spline (m4_table.h): the lattice value is the plain sum of the 6 tabulated vertex
 // integrals anchored at the point (no plane or line terms).  The table's interpolation error does not
@@ -1808,6 +1836,7 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     void* lt = (*B.lines).p;
     void* msb = (*B.ms).p;
     const GenTab* tabs = static_cast<const GenTab*>(c->tables.p) + K->kernel_id;
+    const int kid = K->kernel_id;
     // the FP64 fallback's cell kernel runs on the second stream beside the FP32 one (disjoint items,
     // slab ranges, counters and work cursors); the accumulation waits for both
     const bool fb_on_side = split && npairs > n32 && !tabulated;
@@ -1830,8 +1859,11 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
                 KIWI_HIP_CHECK(c, hipStreamWaitEvent(c->side2, c->fork, 0));
                 if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[4], c->side2));
                 if (generic)
-                    KIWI_HIP_CHECK(c, launch_lines<PolicyGen<double>>(c, c->side2, {sup, tabs}, n, dp, bbox, loffs,
-                                                                      nblk_fb, 1, dpr, part_prob, lt, dst));
+                    KIWI_HIP_CHECK(c, with_gen_policy<double>(kid, [&](auto pol) {
+                                       using P = decltype(pol);
+                                       return launch_lines<P>(c, c->side2, typename P::Ctx{sup, tabs}, n, dp, bbox,
+                                                              loffs, nblk_fb, 1, dpr, part_prob, lt, dst);
+                                   }));
                 else
                     KIWI_HIP_CHECK(c, launch_lines<PolicyM4<double>>(c, c->side2, {sup}, n, dp, bbox, loffs, nblk_fb, 1,
                                                                      dpr, part_prob, lt, dst));
@@ -1840,11 +1872,17 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
             if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[0], c->side));
             const int64_t* sel32 = split ? nblk : nullptr;
             if (generic && fp64) {
-                KIWI_HIP_CHECK(c, launch_lines<PolicyGen<double>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, nullptr,
-                                                                  0, dpr, part_prob, lt, dst));
+                KIWI_HIP_CHECK(c, with_gen_policy<double>(kid, [&](auto pol) {
+                                   using P = decltype(pol);
+                                   return launch_lines<P>(c, c->side, typename P::Ctx{sup, tabs}, n, dp, bbox, loffs,
+                                                          nullptr, 0, dpr, part_prob, lt, dst);
+                               }));
             } else if (generic) {
-                KIWI_HIP_CHECK(c, launch_lines<PolicyGen<float>>(c, c->side, {sup, tabs}, n, dp, bbox, loffs, sel32, 0,
-                                                                 dpr, part_prob, lt, dst));
+                KIWI_HIP_CHECK(c, with_gen_policy<float>(kid, [&](auto pol) {
+                                   using P = decltype(pol);
+                                   return launch_lines<P>(c, c->side, typename P::Ctx{sup, tabs}, n, dp, bbox, loffs,
+                                                          sel32, 0, dpr, part_prob, lt, dst);
+                               }));
             } else if (fp64) {
                 KIWI_HIP_CHECK(c, launch_lines<PolicyM4<double>>(c, c->side, {sup}, n, dp, bbox, loffs, nullptr, 0, dpr,
                                                                  part_prob, lt, dst));
@@ -1924,10 +1962,12 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
             KIWI_HIP_CHECK(c, hipStreamWaitEvent(c->side2, c->fork2, 0));
             if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[2], c->side2));
             if (generic) {
-                KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<double>, float>(c, c->side2, {sup, tabs}, n32,
-                                                                              npairs - n32, dp, bbox, loffs, 1, keys_a,
-                                                                              slab, dpr, part_prob, lt, msb, dst_fb,
-                                                                              cursor_fb, clamp)));
+                KIWI_HIP_CHECK(c, with_gen_policy<double>(kid, [&](auto pol) {
+                                   using P = decltype(pol);
+                                   return launch_particles<P, float>(c, c->side2, typename P::Ctx{sup, tabs}, n32,
+                                                                     npairs - n32, dp, bbox, loffs, 1, keys_a, slab,
+                                                                     dpr, part_prob, lt, msb, dst_fb, cursor_fb, clamp);
+                               }));
             } else {
                 KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<double>, float>(c, c->side2, {sup}, n32, npairs - n32, dp,
                                                                              bbox, loffs, 1, keys_a, slab, dpr,
@@ -1945,13 +1985,19 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
                                                                    dpr, part_prob, lt, msb, dst, cursor, clamp)));
         } else if (generic) {
             if (fp64) {
-                KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<double>, double>(c, s, {sup, tabs}, 0, npairs, dp, bbox,
-                                                                               loffs, 0, keys_a, slab, dpr, part_prob,
-                                                                               lt, msb, dst, cursor, clamp)));
+                KIWI_HIP_CHECK(c, with_gen_policy<double>(kid, [&](auto pol) {
+                                   using P = decltype(pol);
+                                   return launch_particles<P, double>(c, s, typename P::Ctx{sup, tabs}, 0, npairs, dp,
+                                                                      bbox, loffs, 0, keys_a, slab, dpr, part_prob, lt,
+                                                                      msb, dst, cursor, clamp);
+                               }));
             } else {
-                KIWI_HIP_CHECK(c, (launch_particles<PolicyGen<float>, float>(c, s, {sup, tabs}, 0, n32, dp, bbox, loffs,
-                                                                             0, keys_a, slab, dpr, part_prob, lt, msb,
-                                                                             dst, cursor, clamp)));
+                KIWI_HIP_CHECK(c, with_gen_policy<float>(kid, [&](auto pol) {
+                                   using P = decltype(pol);
+                                   return launch_particles<P, float>(c, s, typename P::Ctx{sup, tabs}, 0, n32, dp, bbox,
+                                                                     loffs, 0, keys_a, slab, dpr, part_prob, lt, msb,
+                                                                     dst, cursor, clamp);
+                               }));
             }
         } else if (fp64) {
             KIWI_HIP_CHECK(c, (launch_particles<PolicyM4<double>, double>(c, s, {sup}, 0, npairs, dp, bbox, loffs, 0,
