@@ -582,10 +582,10 @@ struct PolicyGen {
     static __device__ __forceinline__ R plane_weight() { return R(0.5 * GEN_PI); }
 };
 
-// Calls f(P{}) with the generic policy P for kernel kid in precision R: the kernels that run in single
-// precision (the cubic spline on the generic path, Wendland C2) with their tables compiled in; in
-// double, the others through the context's table.
-template <typename R, class F>
+// Calls f(P{}) with the generic policy P of kernel kid in precision R, its table compiled in.  MIXED: only
+// the kernels that mixed precision computes in single precision (gen_mixed_ok), whose FP64 fallback is
+// the double-precision policy; single precision implies it.
+template <typename R, bool MIXED = std::is_same<R, float>::value, class F>
 hipError_t with_gen_policy(int kid, F&& f) {
     switch (kid) {
         case KIWI_KERNEL_M4:
@@ -593,11 +593,29 @@ hipError_t with_gen_policy(int kid, F&& f) {
         case KIWI_KERNEL_WC2:
             return f(PolicyGen<R, KIWI_KERNEL_WC2>{});
         default:
-            if constexpr (std::is_same<R, float>::value)
-                return hipErrorInvalidValue;  // single precision only for the two above (gen_mixed_ok)
-            else
-                return f(PolicyGen<R>{});
+            break;
     }
+    if constexpr (!MIXED) {
+        switch (kid) {
+            case KIWI_KERNEL_M5:
+                return f(PolicyGen<R, KIWI_KERNEL_M5>{});
+            case KIWI_KERNEL_M6:
+                return f(PolicyGen<R, KIWI_KERNEL_M6>{});
+            case KIWI_KERNEL_WC4:
+                return f(PolicyGen<R, KIWI_KERNEL_WC4>{});
+            case KIWI_KERNEL_WC6:
+                return f(PolicyGen<R, KIWI_KERNEL_WC6>{});
+            default:
+                break;
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+// the FP64 fallback's policy: with_gen_policy for the kernels mixed precision computes in single precision
+template <class F>
+hipError_t with_fallback_gen_policy(int kid, F&& f) {
+    return with_gen_policy<double, true>(kid, std::forward<F>(f));
 }
 
 // Tabulated cubic spline (m4_table.h): the lattice value is the plain sum of the 6 tabulated vertex
@@ -1098,7 +1116,11 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(P::WAVE
                     // (region 2 is the rest: c_pts[1] gets the item's total below)
                     c_pts[0] += __popcll(__ballot(reg == 1));
                     c_pts[2] += __popcll(__ballot(reg >= 3));
-                    const R csum = L.pc[reg - 1][i] + L.pc[reg - 1][AX_STRIDE + j] + L.pc[reg - 1][2 * AX_STRIDE + k];
+                    // plane and line terms: the analytic paths only (the tabulated path keeps one unused plane
+                    // row, so its regions must not index the rows)
+                    R csum = R(0);
+                    if constexpr (P::HAS_LINES)
+                        csum = L.pc[reg - 1][i] + L.pc[reg - 1][AX_STRIDE + j] + L.pc[reg - 1][2 * AX_STRIDE + k];
                     R dsum = R(0);
                     const int slot = P::line_slot(reg);
                     if (slot >= 0) {
@@ -1859,7 +1881,7 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
                 KIWI_HIP_CHECK(c, hipStreamWaitEvent(c->side2, c->fork, 0));
                 if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[4], c->side2));
                 if (generic)
-                    KIWI_HIP_CHECK(c, with_gen_policy<double>(kid, [&](auto pol) {
+                    KIWI_HIP_CHECK(c, with_fallback_gen_policy(kid, [&](auto pol) {
                                        using P = decltype(pol);
                                        return launch_lines<P>(c, c->side2, typename P::Ctx{sup, tabs}, n, dp, bbox,
                                                               loffs, nblk_fb, 1, dpr, part_prob, lt, dst);
@@ -1962,7 +1984,7 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
             KIWI_HIP_CHECK(c, hipStreamWaitEvent(c->side2, c->fork2, 0));
             if (!capturing) KIWI_HIP_CHECK(c, hipEventRecord(c->ev_side[2], c->side2));
             if (generic) {
-                KIWI_HIP_CHECK(c, with_gen_policy<double>(kid, [&](auto pol) {
+                KIWI_HIP_CHECK(c, with_fallback_gen_policy(kid, [&](auto pol) {
                                    using P = decltype(pol);
                                    return launch_particles<P, float>(c, c->side2, typename P::Ctx{sup, tabs}, n32,
                                                                      npairs - n32, dp, bbox, loffs, 1, keys_a, slab,

@@ -61,6 +61,7 @@ def test_batch_equals_single_calls(hip_ctx):
         single = hip_ctx.sph_to_grid(prob["parts"], prob["domain"], prob["res"], kernel=prob["kernel"],
                                      precision=prob["precision"])
         for a, b in zip(res[:4], single[:4]):
+            assert np.all(np.isfinite(a)) and np.all(np.isfinite(b)), q
             assert np.array_equal(a, b), q
         for f in COUNTERS:
             assert getattr(res[4], f) == getattr(single[4], f), (q, f)
