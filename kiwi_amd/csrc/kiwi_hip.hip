@@ -104,8 +104,22 @@ struct Prob {
     GridGeom g;
     int64_t part_off, n_parts;
     int64_t block_off;
-    double* out[4];  // device outputs: mass, px, py, pz
+    double* out[4];         // device outputs: mass, px, py, pz
+    const double* src[8];   // the caller's device columns x y z vx vy vz m h (null: host columns)
 };
+
+// A batch's device-resident particle columns concatenated into the pipeline's arrays (8 columns of n
+// each) by one launch instead of 8 copies per problem: blockIdx.y is the problem.
+__global__ __launch_bounds__(THREADS) void k_gather_parts(const Prob* __restrict__ pr, int64_t n,
+                                                          double* __restrict__ dst) {
+    const Prob& P = pr[blockIdx.y];
+    if (!P.src[0]) return;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x; i < P.n_parts;
+         i += static_cast<int64_t>(gridDim.x) * THREADS) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) dst[k * n + P.part_off + i] = P.src[k][i];
+    }
+}
 
 // the problem that owns global block b (nprob is small; the search is wave-uniform)
 __device__ __forceinline__ int block_problem(const Prob* __restrict__ pr, int nprob, int64_t b) {
@@ -1530,10 +1544,18 @@ hipError_t launch_lines(const kiwi_hip_ctx* c, hipStream_t s, typename P::Ctx pc
                         const int* bbox, const int64_t* loffs, const int64_t* sel, int loff_shift, const Prob* pr,
                         const int* part_prob, void* lines, DevStats* dst) {
     using Line = LineVals<typename P::Real, P::NLN>;
+    // The line kernel runs beside the item layout, whose short kernels need free slots while it runs (a
+    // grid of the full resident size holds every slot until it ends): twice the resident size, so that its
+    // workgroups retire and the layout's kernels are dispatched between them.  Measured on 1M log-normal
+    // particles -> 256^3 against the resident grid: FP64 53.1 -> 51.9 ms, mixed 24.0 -> 23.9 ms, Wendland
+    // C2 mixed 31.1 -> 31.3 ms (4x and 8x: FP64 51.6, mixed 24.1-24.2; half the resident grid: mixed 23.5,
+    // FP64 58.0 ms).
+    const int64_t resident = resident_grid(c, k_lines<P>, 1, n);
+    const int64_t grid = std::min<int64_t>((n + WAVES - 1) / WAVES, 2 * resident);
     if (n > 0 && P::HAS_LINES)
-        k_lines<P><<<resident_grid(c, k_lines<P>, 1, n), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], bbox, loffs,
-                                                                          sel, loff_shift, pr, part_prob, pctx,
-                                                                          static_cast<Line*>(lines), dst);
+        k_lines<P><<<static_cast<unsigned>(grid), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], bbox, loffs, sel,
+                                                                   loff_shift, pr, part_prob, pctx,
+                                                                   static_cast<Line*>(lines), dst);
     return hipGetLastError();
 }
 
@@ -1630,6 +1652,11 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
         probs[q].part_off = n;
         probs[q].n_parts = hp[q].P->n;
         probs[q].block_off = nblocks;
+        {
+            const kiwi_particles* P = hp[q].P;
+            const double* src[8] = {P->x, P->y, P->z, P->vx, P->vy, P->vz, P->m, P->h};
+            for (int k = 0; k < 8; ++k) probs[q].src[k] = P->on_device ? src[k] : nullptr;
+        }
         n += hp[q].P->n;
         nblocks += static_cast<int64_t>(g.nbx) * g.nby * g.nbz;
         if (!hp[q].out->on_device) host_cells += static_cast<int64_t>(G->nx) * G->ny * G->nz;
@@ -1637,26 +1664,6 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     if (n > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 particles in one call");
     if (nblocks > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 cell blocks in one call");
 
-    // particle arrays on the device: the caller's own for a single device-resident problem, else the
-    // problems' columns concatenated
-    const double* dp[8];
-    if (nprob == 1 && hp[0].P->on_device) {
-        const kiwi_particles* P = hp[0].P;
-        const double* src[8] = {P->x, P->y, P->z, P->vx, P->vy, P->vz, P->m, P->h};
-        for (int k = 0; k < 8; ++k) dp[k] = src[k];
-    } else {
-        KIWI_HIP_CHECK(c, reserve((*B.parts), 8 * sizeof(double) * std::max<int64_t>(n, 1)));
-        for (int k = 0; k < 8; ++k) dp[k] = static_cast<double*>((*B.parts).p) + k * n;
-        for (int q = 0; q < nprob; ++q) {
-            const kiwi_particles* P = hp[q].P;
-            const double* src[8] = {P->x, P->y, P->z, P->vx, P->vy, P->vz, P->m, P->h};
-            if (P->n == 0) continue;
-            const hipMemcpyKind kind = P->on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-            for (int k = 0; k < 8; ++k)
-                KIWI_HIP_CHECK(c, hipMemcpyAsync(const_cast<double*>(dp[k]) + probs[q].part_off, src[k],
-                                                 P->n * sizeof(double), kind, s));
-        }
-    }
     // outputs: the caller's device arrays, or staging for host arrays
     KIWI_HIP_CHECK(c, reserve((*B.outbuf), 4 * sizeof(double) * std::max<int64_t>(host_cells, 1)));
     {
@@ -1678,6 +1685,34 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     KIWI_HIP_CHECK(c, reserve((*B.probs), sizeof(Prob) * nprob));
     Prob* dpr = static_cast<Prob*>((*B.probs).p);
     KIWI_HIP_CHECK(c, hipMemcpyAsync(dpr, probs.data(), sizeof(Prob) * nprob, hipMemcpyHostToDevice, s));
+    // particle arrays on the device: the caller's own for a single device-resident problem, else the
+    // problems' columns concatenated (device columns by k_gather_parts, host columns copied)
+    const double* dp[8];
+    if (nprob == 1 && hp[0].P->on_device) {
+        for (int k = 0; k < 8; ++k) dp[k] = probs[0].src[k];
+    } else {
+        KIWI_HIP_CHECK(c, reserve((*B.parts), 8 * sizeof(double) * std::max<int64_t>(n, 1)));
+        for (int k = 0; k < 8; ++k) dp[k] = static_cast<double*>((*B.parts).p) + k * n;
+        int64_t dev_max = 0;
+        for (int q = 0; q < nprob; ++q) {
+            const kiwi_particles* P = hp[q].P;
+            if (P->n == 0) continue;
+            if (P->on_device) {
+                dev_max = std::max<int64_t>(dev_max, P->n);
+                continue;
+            }
+            const double* src[8] = {P->x, P->y, P->z, P->vx, P->vy, P->vz, P->m, P->h};
+            for (int k = 0; k < 8; ++k)
+                KIWI_HIP_CHECK(c, hipMemcpyAsync(const_cast<double*>(dp[k]) + probs[q].part_off, src[k],
+                                                 P->n * sizeof(double), hipMemcpyHostToDevice, s));
+        }
+        if (dev_max > 0) {
+            const unsigned gx = static_cast<unsigned>(std::min<int64_t>((dev_max + THREADS - 1) / THREADS, 1024));
+            k_gather_parts<<<dim3(gx, static_cast<unsigned>(nprob)), THREADS, 0, s>>>(
+                dpr, n, static_cast<double*>((*B.parts).p));
+            KIWI_HIP_CHECK(c, hipGetLastError());
+        }
+    }
     KIWI_HIP_CHECK(c, reserve((*B.part_prob), sizeof(int) * std::max<int64_t>(n, 1)));
     int* part_prob = static_cast<int*>((*B.part_prob).p);
 
