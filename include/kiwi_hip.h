@@ -171,9 +171,12 @@ const char* kiwi_hip_last_error(const kiwi_hip_ctx* ctx);
 void* kiwi_hip_stream(kiwi_hip_ctx* ctx);
 
 /* Direct SPH -> Cartesian grid, replacing src/cartesian_grid.cpp:184-301.
- * precision: KIWI_FP64 or KIWI_MIXED.  KIWI_MIXED computes in FP32 for the analytic cubic spline and for
- * the generic cubic spline and Wendland C2; the tabulated path and the other generic kernels (M5, M6,
- * Wendland C4, C6) run in FP64 whatever is asked.  out: n_cells = nx*ny*nz per array.  stats may be NULL. */
+ * precision: KIWI_FP64 or KIWI_MIXED.  KIWI_MIXED computes the lattice values in FP32 (sums in FP64) for
+ * the analytic cubic spline, the generic cubic spline and Wendland C2, and for M5 while the FP64 fallback
+ * for large supports is on (kiwi_hip_set_fp64_fallback).  M6, Wendland C4 and C6 (whose FP32 lattice
+ * values lose more than 1e-6 of the mass of particles cut by the domain boundary) and the tabulated path
+ * run in FP64 whatever is asked.  out: n_cells = nx*ny*nz per array.  stats
+ * may be NULL. */
 int kiwi_hip_sph_to_grid(kiwi_hip_ctx* ctx, const kiwi_particles* particles, const kiwi_grid* grid,
                          const kiwi_kernel* kernel, int precision, const kiwi_cell_sums* out,
                          kiwi_stats* stats);
@@ -299,7 +302,8 @@ typedef struct kiwi_tuning {
  * spans at least support_cells times the smallest cell edge of its grid has its lattice values, line and
  * plane constants and cell sums computed in FP64 (stored, like the FP32 ones, as float per-pair sums and
  * accumulated in FP64).  support_cells <= 0 switches the fallback off; the default is
- * KIWI_FP64_FALLBACK_CELLS.  KIWI_FP64 and the tabulated path are unaffected.  Within a cell, the
+ * KIWI_FP64_FALLBACK_CELLS; M5, whose FP32 lattice values cancel more, falls back from 0.75 of it.
+ * KIWI_FP64 and the tabulated path are unaffected.  Within a cell, the
  * fallback's particles are added after the others (each group in particle order), so mixed results stay
  * bit-reproducible, and a batch still equals its single calls bit for bit. */
 #define KIWI_FP64_FALLBACK_CELLS 16.0

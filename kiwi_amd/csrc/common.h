@@ -308,6 +308,20 @@ const KiwiKernelTable kHostTables[KIWI_TAB_NKERNELS] = {KIWI_KERNEL_TABLE_DATA};
 // request.
 inline bool gen_mixed_ok(int kernel_id) { return kernel_id == KIWI_KERNEL_M4 || kernel_id == KIWI_KERNEL_WC2; }
 
+// The sph-to-grid forward in mixed precision also computes M5 in FP32 while the FP64 fallback is on.  Its
+// monomial expansion cancels more in FP32 than the cubic spline's or Wendland C2's (lattice terms up to
+// 6.5 against |W| <= 1/8, against 2-3.4), so its fallback starts at 0.75 of the threshold, where an
+// isolated particle's FP32 cell error matches theirs at the full threshold (host measurement, mass-
+// weighted L1 of the FP32 cell sums against FP64 at supports of 4 / 6 / 8 / 12 cells: WC2 1.1e-4 /
+// 2.9e-4 / 5.6e-4 / 1.6e-3, M5 1.8e-4 / 5.2e-4 / 1.1e-3 / 3.3e-3).  M6 and Wendland C4 and C6 stay in
+// FP64: their terms reach 13-150, and a particle cut by the domain boundary (whose mass inside is a
+// difference of boundary lattice values, whatever its size) keeps a total-mass error of 1.6e-6 to 1.9e-5
+// in FP32, above the 1e-6 that mixed precision keeps (500 particles -> 20^3 of the generic fixtures).
+inline bool gen_forward_fp32(int kernel_id, double fb_cells) {
+    return gen_mixed_ok(kernel_id) || (kernel_id == KIWI_KERNEL_M5 && fb_cells > 0.0);
+}
+inline double gen_fallback_scale(int kernel_id) { return kernel_id == KIWI_KERNEL_M5 ? 0.75 : 1.0; }
+
 // Which integral a kernel request selects: make_kernel_integrator (include/kiwi/kernel_integrator.hpp:
 // 14-46, src/config.cpp:269-277) -- the cubic analytic path iff M4 with support 2h and no generic request,
 // the generic path otherwise; the tabulated integral only on the cubic path (src/config.cpp:174-178).

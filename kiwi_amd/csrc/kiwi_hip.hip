@@ -550,7 +550,8 @@ struct PolicyM4 {
 // Generic tabulated-coefficient kernels (gen_lattice.h): lattice offsets in units of the support H,
 // support 1; regions of the kernel table; plane values Phi_k; the line table holds the offsets of
 // every region.  R = double is the parity mode; R = float (mixed) is used for the kernels whose
-// single-precision lattice values are as accurate as the cubic spline's (gen_mixed_ok), with the plane
+// single-precision lattice values are as accurate as the cubic spline's (gen_mixed_ok) and, with the FP64
+// fallback on, for M5 below its scaled fallback threshold (gen_forward_fp32), with the plane
 // values still formed in double and rounded.  KID >= 0: kernel KID's table compiled into the code
 // (gen_fixed_tab); KID = -1: the context's table, read at run time.
 template <typename R, int KID = -1>
@@ -596,14 +597,16 @@ struct PolicyGen {
     static __device__ __forceinline__ R plane_weight() { return R(0.5 * GEN_PI); }
 };
 
-// Calls f(P{}) with the generic policy P of kernel kid in precision R, its table compiled in.  MIXED: only
-// the kernels that mixed precision computes in single precision (gen_mixed_ok), whose FP64 fallback is
-// the double-precision policy; single precision implies it.
+// Calls f(P{}) with the generic policy P of kernel kid in precision R, its table compiled in.  MIXED:
+// only the kernels that mixed precision computes in FP32 (gen_forward_fp32), whose FP64 fallback is the
+// double-precision policy; single precision implies it.
 template <typename R, bool MIXED = std::is_same<R, float>::value, class F>
 hipError_t with_gen_policy(int kid, F&& f) {
     switch (kid) {
         case KIWI_KERNEL_M4:
             return f(PolicyGen<R, KIWI_KERNEL_M4>{});
+        case KIWI_KERNEL_M5:
+            return f(PolicyGen<R, KIWI_KERNEL_M5>{});
         case KIWI_KERNEL_WC2:
             return f(PolicyGen<R, KIWI_KERNEL_WC2>{});
         default:
@@ -611,8 +614,6 @@ hipError_t with_gen_policy(int kid, F&& f) {
     }
     if constexpr (!MIXED) {
         switch (kid) {
-            case KIWI_KERNEL_M5:
-                return f(PolicyGen<R, KIWI_KERNEL_M5>{});
             case KIWI_KERNEL_M6:
                 return f(PolicyGen<R, KIWI_KERNEL_M6>{});
             case KIWI_KERNEL_WC4:
@@ -626,7 +627,7 @@ hipError_t with_gen_policy(int kid, F&& f) {
     return hipErrorInvalidValue;
 }
 
-// the FP64 fallback's policy: with_gen_policy for the kernels mixed precision computes in single precision
+// the FP64 fallback's policy: with_gen_policy for the kernels mixed precision computes in FP32
 template <class F>
 hipError_t with_fallback_gen_policy(int kid, F&& f) {
     return with_gen_policy<double, true>(kid, std::forward<F>(f));
@@ -1612,9 +1613,12 @@ int check_problem(kiwi_hip_ctx* c, const kiwi_particles* P, const kiwi_grid* G, 
 }
 
 // problems that can share one pipeline run: the same kernel evaluation and arithmetic
-bool same_path(const kiwi_batch_problem& a, const KernelChoice& ka, const kiwi_batch_problem& b, const KernelChoice& kb) {
-    const bool fa = ka.tabulated || a.precision == KIWI_FP64 || (ka.generic && !gen_mixed_ok(a.kernel.kernel_id));
-    const bool fb = kb.tabulated || b.precision == KIWI_FP64 || (kb.generic && !gen_mixed_ok(b.kernel.kernel_id));
+bool same_path(const kiwi_batch_problem& a, const KernelChoice& ka, const kiwi_batch_problem& b, const KernelChoice& kb,
+               double fb_cells) {
+    const bool fa = ka.tabulated || a.precision == KIWI_FP64 ||
+                    (ka.generic && !gen_forward_fp32(a.kernel.kernel_id, fb_cells));
+    const bool fb = kb.tabulated || b.precision == KIWI_FP64 ||
+                    (kb.generic && !gen_forward_fp32(b.kernel.kernel_id, fb_cells));
     return ka.generic == kb.generic && ka.tabulated == kb.tabulated && ka.support_in_h == kb.support_in_h &&
            (!ka.generic || a.kernel.kernel_id == b.kernel.kernel_id) && fa == fb;
 }
@@ -1759,15 +1763,17 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     // tabulated path, whose table values do not vanish there
     const int skip_outside = kc.tabulated ? 0 : 1;
     // the arithmetic of the path: mixed precision applies to the cubic path and to the generic kernels
-    // gen_mixed_ok names; the tabulated path and the other generic kernels always compute in double
+    // (those beyond gen_mixed_ok only with the FP64 fallback on: gen_forward_fp32); the tabulated path
+    // always computes in double
     const bool tabulated = kc.tabulated;
-    const bool fp64 = tabulated || precision == KIWI_FP64 || (generic && !gen_mixed_ok(K->kernel_id));
+    const bool fp64 = tabulated || precision == KIWI_FP64 || (generic && !gen_forward_fp32(K->kernel_id, c->fb_cells));
     const int ax_units = tabulated ? 0
                          : generic ? (fp64 ? axis_units<PolicyGen<double>>() : axis_units<PolicyGen<float>>())
                                    : (fp64 ? axis_units<PolicyM4<double>>() : axis_units<PolicyM4<float>>());
     // mixed precision's FP64 fallback for particles whose support spans many cells (fp64_fallback): their
-    // items follow the FP32 items and run through the FP64 cell kernel, which stores float cell sums
-    const double fb_cells = fp64 ? 0.0 : c->fb_cells;
+    // items follow the FP32 items and run through the FP64 cell kernel, which stores float cell sums; the
+    // higher-order generic kernels' threshold is scaled down (gen_fallback_scale)
+    const double fb_cells = fp64 ? 0.0 : c->fb_cells * (generic ? gen_fallback_scale(K->kernel_id) : 1.0);
     const bool split = fb_cells > 0.0;
     const int nscan = split ? 3 : 2;
     k_prep<<<grid1d(n1), THREADS, 0, s>>>(n, dp[0], dp[1], dp[2], dp[7], dp[6], dp[3], dp[4], dp[5], sup, skip_outside,
@@ -2398,7 +2404,7 @@ int kiwi_hip_sph_to_grid_batch(kiwi_hip_ctx* c, int n_problems, const kiwi_batch
         std::vector<HostProblem> group;
         for (int j = i; j < n_problems; ++j) {
             if (done[j]) continue;
-            if (same_path(problems[i], kcs[i], problems[j], kcs[j])) {
+            if (same_path(problems[i], kcs[i], problems[j], kcs[j], c->fb_cells)) {
                 group.push_back(HostProblem{&problems[j].particles, &problems[j].grid, &problems[j].out,
                                             problems[j].stats});
                 done[j] = 1;
