@@ -238,7 +238,8 @@ def main():
             "support_cells": fb_cells,
             "note": "mixed precision computes particles whose support radius spans >= support_cells cells in "
                     "FP64 (their FP32 lattice differences lose up to ~4e-2 of the particle at h = 30 cells); "
-                    "that cell kernel runs on the second stream beside the FP32 one",
+                    "that cell kernel runs on the context's third stream beside the FP32 one, from the end of the "
+                    "item layout",
             "items_share": st32.fp64_fallback_blocks / max(st32.particle_blocks, 1),
             "pairs_share": st32.fp64_fallback_pairs / max(st32.pairs_sphere, 1),
             "kernel_ms": g32["fallback"], "flops_per_launch": fl_fb,

@@ -108,7 +108,7 @@ typedef struct kiwi_stats {
     double v_mean[3];         /* mean velocity over all particles */
     double device_ms_prep;    /* binning, sorts, block order */
     double device_ms_lines;   /* per-particle edge-line constants: the part not hidden behind the item sort
-                                 (they run on a second stream beside it) */
+                                 (they run on other streams beside it) */
     double device_ms_gather;  /* cell integrals: lattice values and per-pair cell sums */
     double device_ms_accum;   /* accumulation of the per-pair sums into the grid */
     double device_ms_lines_kernel; /* the line-table kernels themselves, timed on the streams they run on

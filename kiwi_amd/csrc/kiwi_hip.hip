@@ -1900,7 +1900,7 @@ int run_group(kiwi_hip_ctx* c, const HostProblem* hp, int nprob, const kiwi_kern
     void* msb = (*B.ms).p;
     const GenTab* tabs = static_cast<const GenTab*>(c->tables.p) + K->kernel_id;
     const int kid = K->kernel_id;
-    // the FP64 fallback's cell kernel runs on the second stream beside the FP32 one (disjoint items,
+    // the FP64 fallback's cell kernel runs on the third stream beside the FP32 one (disjoint items,
     // slab ranges, counters and work cursors); the accumulation waits for both
     const bool fb_on_side = split && npairs > n32 && !tabulated;
     std::vector<double> vs(static_cast<size_t>(3 * nprob));
