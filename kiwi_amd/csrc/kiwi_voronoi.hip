@@ -406,27 +406,35 @@ __device__ __forceinline__ bool face_reachable(double x0, double y0, double z0, 
     return r;
 }
 
-// A face whose plane or bounding sphere lies beyond the support is skipped without its vertices: the
-// bound is widened by 1e-9 H plus 1e-12 of the coordinates' magnitude, far above the rounding of either
-// distance, so a skipped face is one the exact test rejects too.  Otherwise one of its first three
-// vertices inside the support by a relative margin of 1e-6 on the squared distance makes it reachable (its
-// distance is at most that vertex's), and the rest take the exact test.
-__device__ __forceinline__ bool face_reachable_bounded(double x0, double y0, double z0, double H, double slack,
-                                                       const FaceGeo& g, const double* v, int nvf) {
-    if (fabs(g.nx * x0 + g.ny * y0 + g.nz * z0 + g.off) >= H + slack + 1e-12 * fabs(g.off)) return false;
+// The cheap part of a face's test: a face whose plane or bounding sphere lies beyond the support is
+// rejected without its vertices (the bound is widened by 1e-9 H plus 1e-12 of the coordinates' magnitude,
+// far above the rounding of either distance, so a rejected face is one the exact test rejects too); one of
+// its first three vertices inside the support by a relative margin of 1e-6 on the squared distance makes
+// it reachable (its distance is at most that vertex's); a face of fewer than three vertices is decided by
+// face_reachable at once.  FACE_EXACT: the rest, which take the exact test.
+constexpr int FACE_OUT = 0, FACE_IN = 1, FACE_EXACT = 2;
+
+__device__ __forceinline__ int face_quick(double x0, double y0, double z0, double H, double slack, const FaceGeo& g,
+                                          const double* v, int nvf) {
+    if (fabs(g.nx * x0 + g.ny * y0 + g.nz * z0 + g.off) >= H + slack + 1e-12 * fabs(g.off)) return FACE_OUT;
     const double dx = x0 - g.cx, dy = y0 - g.cy, dz = z0 - g.cz;
-    if (sqrt(dx * dx + dy * dy + dz * dz) - g.rad >= H + slack + 1e-12 * g.rad) return false;
-    if (nvf >= 3) {
-        const double in2 = H * H * (1.0 - 1e-6);
+    if (sqrt(dx * dx + dy * dy + dz * dz) - g.rad >= H + slack + 1e-12 * g.rad) return FACE_OUT;
+    if (nvf < 3) return nvf != 0 ? FACE_IN : FACE_OUT;
+    const double in2 = H * H * (1.0 - 1e-6);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const double ex = v[3 * k] - x0, ey = v[3 * k + 1] - y0, ez = v[3 * k + 2] - z0;
-            if (ex * ex + ey * ey + ez * ez < in2) return true;
-        }
+    for (int k = 0; k < 3; ++k) {
+        const double ex = v[3 * k] - x0, ey = v[3 * k + 1] - y0, ez = v[3 * k + 2] - z0;
+        if (ex * ex + ey * ey + ez * ez < in2) return FACE_IN;
     }
-    return face_reachable(x0, y0, z0, H, v, nvf);
+    return FACE_EXACT;
 }
 
+// The visit test of a candidate pair is the OR over its cell's faces of face_quick, with face_reachable
+// for the faces face_quick leaves open.  Those exact tests are few and long; tested where they arise, a
+// wavefront's other lanes would wait on them, so they run in a pass of their own, one lane per (pair,
+// face).  First pass: the home cell and the faces' cheap tests; the pair's visit flag, or the number of
+// its faces left open (nexact) and which (exmask, bit j for the cell's face j < 64).  Entry ncand of
+// flag and nexact is 0 (the scans' totals).
 __global__ __launch_bounds__(THREADS) void k_vor_reach(int64_t ncand, const int* __restrict__ cand_cell,
                                                        const int* __restrict__ cand_part,
                                                        const double* __restrict__ px,
@@ -436,24 +444,95 @@ __global__ __launch_bounds__(THREADS) void k_vor_reach(int64_t ncand, const int*
                                                        double sup, const int64_t* __restrict__ cfp,
                                                        const int64_t* __restrict__ fvp,
                                                        const double* __restrict__ vxyz,
-                                                       const FaceGeo* __restrict__ fg, int64_t* __restrict__ flag) {
+                                                       const FaceGeo* __restrict__ fg, int64_t* __restrict__ flag,
+                                                       int64_t* __restrict__ nexact, uint64_t* __restrict__ exmask) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
     if (t > ncand) return;
     if (t == ncand) {
         flag[t] = 0;
+        nexact[t] = 0;
         return;
     }
     const int c = cand_cell[t], i = cand_part[t];
     bool vis = (home[i] == c);
+    int64_t nex = 0;
+    uint64_t mask = 0;
     if (!vis) {
         const double x0 = px[i], y0 = py[i], z0 = pz[i], H = sup * ph[i];
         const double slack = 1e-9 * H + 1e-12 * (fabs(x0) + fabs(y0) + fabs(z0));
-        for (int64_t f = cfp[c]; f < cfp[c + 1] && !vis; ++f) {
+        const int64_t f0 = cfp[c];
+        for (int64_t f = f0; f < cfp[c + 1] && !vis; ++f) {
             const int64_t v0 = fvp[f];
-            vis = face_reachable_bounded(x0, y0, z0, H, slack, fg[f], vxyz + 3 * v0, static_cast<int>(fvp[f + 1] - v0));
+            const int q = face_quick(x0, y0, z0, H, slack, fg[f], vxyz + 3 * v0, static_cast<int>(fvp[f + 1] - v0));
+            vis = q == FACE_IN;
+            if (q == FACE_EXACT) {
+                ++nex;
+                if (f - f0 < 64) mask |= 1ull << (f - f0);
+            }
         }
     }
     flag[t] = vis ? 1 : 0;
+    nexact[t] = vis ? 0 : nex;
+    exmask[t] = mask;
+}
+
+// Second pass: the (pair, face) entries of the faces left open, pair t's at ex_off[t], in face order:
+// read from exmask, or, for a cell of more than 64 faces, decided again by the same cheap tests.
+__global__ __launch_bounds__(THREADS) void k_vor_reach_list(int64_t ncand, const int* __restrict__ cand_cell,
+                                                            const int* __restrict__ cand_part,
+                                                            const double* __restrict__ px,
+                                                            const double* __restrict__ py,
+                                                            const double* __restrict__ pz,
+                                                            const double* __restrict__ ph, double sup,
+                                                            const int64_t* __restrict__ cfp,
+                                                            const int64_t* __restrict__ fvp,
+                                                            const double* __restrict__ vxyz,
+                                                            const FaceGeo* __restrict__ fg,
+                                                            const int64_t* __restrict__ nexact,
+                                                            const uint64_t* __restrict__ exmask,
+                                                            const int64_t* __restrict__ ex_off,
+                                                            uint64_t* __restrict__ ex) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
+    if (t >= ncand || nexact[t] == 0) return;
+    const int c = cand_cell[t], i = cand_part[t];
+    if (cfp[c + 1] - cfp[c] <= 64) {
+        int64_t o = ex_off[t];
+        for (uint64_t m = exmask[t]; m; m &= m - 1ull)
+            ex[o++] = (static_cast<uint64_t>(t) << 32) | static_cast<uint64_t>(__ffsll(static_cast<long long>(m)) - 1);
+        return;
+    }
+    const double x0 = px[i], y0 = py[i], z0 = pz[i], H = sup * ph[i];
+    const double slack = 1e-9 * H + 1e-12 * (fabs(x0) + fabs(y0) + fabs(z0));
+    int64_t o = ex_off[t];
+    const int64_t o_end = o + nexact[t];
+    for (int64_t f = cfp[c]; f < cfp[c + 1] && o < o_end; ++f) {
+        const int64_t v0 = fvp[f];
+        if (face_quick(x0, y0, z0, H, slack, fg[f], vxyz + 3 * v0, static_cast<int>(fvp[f + 1] - v0)) == FACE_EXACT)
+            ex[o++] = (static_cast<uint64_t>(t) << 32) | static_cast<uint64_t>(f - cfp[c]);
+    }
+}
+
+// Third pass: one exact test per listed (pair, face); a reachable face sets its pair's flag (every
+// writer writes 1, so the order of the writes does not matter).
+__global__ __launch_bounds__(THREADS) void k_vor_reach_exact(int64_t nex, const uint64_t* __restrict__ ex,
+                                                             const int* __restrict__ cand_cell,
+                                                             const int* __restrict__ cand_part,
+                                                             const double* __restrict__ px,
+                                                             const double* __restrict__ py,
+                                                             const double* __restrict__ pz,
+                                                             const double* __restrict__ ph, double sup,
+                                                             const int64_t* __restrict__ cfp,
+                                                             const int64_t* __restrict__ fvp,
+                                                             const double* __restrict__ vxyz,
+                                                             int64_t* __restrict__ flag) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
+    if (q >= nex) return;
+    const uint64_t e = ex[q];
+    const int64_t t = static_cast<int64_t>(e >> 32);
+    const int c = cand_cell[t], i = cand_part[t];
+    const int64_t f = cfp[c] + static_cast<int64_t>(e & 0xffffffffull);
+    const int64_t v0 = fvp[f];
+    if (face_reachable(px[i], py[i], pz[i], sup * ph[i], vxyz + 3 * v0, static_cast<int>(fvp[f + 1] - v0))) flag[t] = 1;
 }
 
 // visited pairs of particle i: [vis_off[i], vis_off[i+1]); particles that reached only their home cell
@@ -1102,11 +1181,28 @@ int voronoi_run(kiwi_hip_ctx* c, const kiwi_particles* P, const int* home_cell, 
         if (mism) return fail(c, KIWI_ERR_DEVICE, "internal error: candidate count and fill passes disagree");
     }
 
-    // 4. visits
+    // 4. visits: cheap tests per pair, then the exact tests they leave open, one lane each
+    KIWI_HIP_CHECK(c, reserve(c->vor[11], bytes_of({sizeof(int64_t) * (ncand + 1), sizeof(int64_t) * (ncand + 1),
+                                                    sizeof(uint64_t) * (ncand + 1)})));
+    size_t o11 = 0;
+    int64_t* nexact = at_offset<int64_t>(c->vor[11].p, o11, ncand + 1);
+    int64_t* ex_off = at_offset<int64_t>(c->vor[11].p, o11, ncand + 1);
+    uint64_t* exmask = at_offset<uint64_t>(c->vor[11].p, o11, ncand + 1);
     k_vor_reach<<<grid1d(ncand + 1), THREADS, 0, s>>>(ncand, cand_cell, cand_part, dp[0], dp[1], dp[2], dp[7], home,
-                                                      sup, cfp, fvp, vxyz, fgeo, flag);
-
+                                                      sup, cfp, fvp, vxyz, fgeo, flag, nexact, exmask);
     KIWI_HIP_CHECK(c, hipGetLastError());
+    KIWI_HIP_CHECK(c, scan_i64(c, nexact, ex_off, ncand + 1, s));
+    int64_t nex = 0;
+    KIWI_HIP_CHECK(c, read_back(&nex, ex_off + ncand, s));
+    if (nex > 0) {
+        KIWI_HIP_CHECK(c, reserve(c->vor[12], sizeof(uint64_t) * nex));
+        uint64_t* ex = static_cast<uint64_t*>(c->vor[12].p);
+        k_vor_reach_list<<<grid1d(ncand), THREADS, 0, s>>>(ncand, cand_cell, cand_part, dp[0], dp[1], dp[2], dp[7],
+                                                           sup, cfp, fvp, vxyz, fgeo, nexact, exmask, ex_off, ex);
+        k_vor_reach_exact<<<grid1d(nex), THREADS, 0, s>>>(nex, ex, cand_cell, cand_part, dp[0], dp[1], dp[2], dp[7],
+                                                          sup, cfp, fvp, vxyz, flag);
+        KIWI_HIP_CHECK(c, hipGetLastError());
+    }
     KIWI_HIP_CHECK(c, scan_i64(c, flag, flag_off, ncand + 1, s));
     k_vor_particle_visits<<<grid1d(n + 1), THREADS, 0, s>>>(n, coff, flag_off, vis_off, dst);
     KIWI_HIP_CHECK(c, hipGetLastError());
