@@ -162,6 +162,41 @@ def test_random_tessellations_match_oracle(hip_ctx, seed, n_sites, n_parts, h_sc
     assert st.n_negative_warn == counts[3] == 0
 
 
+def test_many_faced_cell_matches_oracle(hip_ctx):
+    """A cell of more than 64 faces: the site at the centre of 150 sites on a sphere.  The visit test
+    lists the faces its cheap tests leave open from a 64-bit mask per pair, and decides them again for a
+    cell with more faces; particles around the sphere reach the central cell's faces partly."""
+    say("case: a central cell of ~150 faces (150 sites on a sphere around it), 300 particles vs oracle")
+    O = oracle()
+    rng = np.random.default_rng(21)
+    dom = np.array([-1.0, 1.0, -1.0, 1.0, -1.0, 1.0])
+    k = np.arange(150) + 0.5
+    th, ph = np.arccos(1 - 2 * k / 150), np.pi * (1 + 5 ** 0.5) * k
+    shell = 0.3 * np.stack([np.cos(ph) * np.sin(th), np.sin(ph) * np.sin(th), np.cos(th)], 1)
+    sites = np.vstack([[[0.0, 0.0, 0.0]], shell, rng.uniform(-0.95, 0.95, size=(200, 3))])
+    sites = sites[np.r_[0:151, 151 + np.flatnonzero(np.linalg.norm(sites[151:], axis=1) > 0.5)]]
+    n = 300
+    r = rng.uniform(0.0, 0.45, n)
+    d = rng.normal(size=(n, 3))
+    pts = d / np.linalg.norm(d, axis=1)[:, None] * r[:, None]
+    parts = {"x": pts[:, 0], "y": pts[:, 1], "z": pts[:, 2], "vx": rng.normal(size=n), "vy": rng.normal(size=n),
+             "vz": rng.normal(size=n), "m": rng.uniform(0.5, 1.5, n), "h": rng.uniform(0.02, 0.12, n)}
+    parts = {c: np.ascontiguousarray(v, dtype=np.float64) for c, v in parts.items()}
+    geo = O.tessellate(np.arange(1, len(sites) + 1), sites, dom, pts)
+    cells = {c: np.ascontiguousarray(geo[c]) for c in ("cell_face_ptr", "face_vert_ptr", "face_nbr")}
+    cells["vert_xyz"] = np.ascontiguousarray(geo["vert_xyz"].reshape(-1))
+    cells["domain"] = dom
+    home = geo["home"].astype(np.int32)
+    nfaces = np.diff(cells["cell_face_ptr"])
+    say(f"  {len(sites)} cells, most faces {nfaces.max()}")
+    assert nfaces.max() > 64
+    got = hip_ctx.voronoi_map(parts, home, cells)
+    ref = run_oracle(parts, home, cells)
+    assert_matches_oracle(got[:4], ref[:4], parts, home, cells)
+    st, counts = got[4], ref[4]
+    assert st.pairs_visited == counts[4] and st.face_evals == counts[5] and st.unmapped == counts[1]
+
+
 @pytest.mark.parametrize("name,generic,tabulated", [("M4", False, True), ("WC2", True, False),
                                                     ("M6", True, False), ("WC6", True, False)])
 def test_kernels_match_oracle(hip_ctx, name, generic, tabulated):
