@@ -356,8 +356,9 @@ typedef struct kiwi_adjoint_stats {
  * The analytic cubic spline and the generic kernels (M4 forced generic or with another support, M5, M6,
  * Wendland C2/C4/C6); the tabulated integral returns KIWI_ERR_UNSUPPORTED.  FP64 differentiates the FP64
  * forward (pairs it clamps at 0 have zero derivative); KIWI_MIXED evaluates the lattice values and
- * gradients in FP32 with FP64 accumulation for the kernels whose forward runs in FP32 (the cubic spline,
- * generic M4 and Wendland C2) and in FP64 for the others, as the forward does. */
+ * gradients in FP32 with FP64 accumulation for the cubic spline, generic M4 and Wendland C2, and in FP64
+ * for the others (M5 included: its forward computes in FP32 only beside the FP64 fallback for large
+ * supports, which the adjoint does not have). */
 int kiwi_hip_sph_to_grid_adjoint(kiwi_hip_ctx* ctx, const kiwi_particles* particles, const kiwi_grid* grid,
                                  const kiwi_kernel* kernel, int precision, const kiwi_cell_sums* grad_cells,
                                  const kiwi_particle_grads* grad_particles, kiwi_adjoint_stats* stats);

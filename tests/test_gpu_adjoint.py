@@ -295,7 +295,7 @@ def test_generic_adjoint_mixed_close_to_fp64(hip_ctx):
         e = np.max(np.abs(g32[f] - g64[f])) / np.abs(g64[f]).max()
         say(f"  d/d{f}: max |mixed - fp64| / max|fp64| = {e:.2e}")
         assert e < (2e-4 if f in ("m", "vx", "vy", "vz") else 3e-2), (f, e)
-    # M5 has no FP32 lattice: its mixed request runs the FP64 adjoint
+    # the adjoint has no FP64 fallback, so M5 (FP32 in the forward only beside it) runs in FP64
     k5 = hip.kernel_spec("M5")
     a, _ = hip_ctx.sph_to_grid_adjoint(parts, dom, res, gc, kernel=k5)
     b, _ = hip_ctx.sph_to_grid_adjoint(parts, dom, res, gc, kernel=k5, precision=hip.MIXED)
