@@ -53,14 +53,19 @@ KIWI_GHD void gen_adj_plane(const GenTab& K, T d, int k, T& phi, T& dphi) {
 }
 
 // W at absolute offsets (ax, ay, az) > 0 (units of H) in region k and dW/dax, dW/day, dW/daz; lxy, lxz,
-// lyz are the line entries of the three axis pairs at this point
-template <typename T>
+// lyz are the line entries of the three axis pairs at this point.  KID >= 0: K is gen_fixed_tab<KID>(),
+// whose free part is evaluated by gen_lattice_free_fixed (coefficients folded into the code).
+template <typename T, int KID = -1>
 KIWI_GHD void gen_adj_point(const GenTab& K, int k, T ax, T ay, T az, const AdjLineGen<T>& lxy,
                             const AdjLineGen<T>& lxz, const AdjLineGen<T>& lyz, T& W, T& Wx, T& Wy, T& Wz) {
     using D = Dual<T, 3>;
     const D X = D::var(ax, 0), Y = D::var(ay, 1), Z = D::var(az, 2);
     const D R = KMath<D>::sqrt_(X * X + Y * Y + Z * Z);
-    const D f = gen_lattice_free<D>(gen_regions<T>(K)[k - 1], X, Y, Z, R);
+    D f;
+    if constexpr (KID >= 0)
+        f = gen_lattice_free_fixed<D, KID>(X, Y, Z, R, k);
+    else
+        f = gen_lattice_free<D>(gen_regions<T>(K)[k - 1], X, Y, Z, R);
     W = f.v;
     Wx = f.d[0];
     Wy = f.d[1];

@@ -62,7 +62,8 @@ struct AdjM4 {
     }
 };
 
-template <typename T>
+// KID >= 0: kernel KID's table compiled into the code (gen_fixed_tab), as in the forward's PolicyGen
+template <typename T, int KID = -1>
 struct AdjGen {
     using Real = T;
     using Line = AdjLineGen<T>;
@@ -70,16 +71,51 @@ struct AdjGen {
         double support_in_h;
         const GenTab* tab;
     };
+    static __device__ __forceinline__ const GenTab& tab(const Ctx& c) {
+        if constexpr (KID >= 0)
+            return gen_fixed_tab<KID>();
+        else
+            return *c.tab;
+    }
     static __device__ __forceinline__ double unit_in_h(const Ctx& c) { return c.support_in_h; }
     static __device__ __forceinline__ T support2() { return T(1); }
-    static __device__ __forceinline__ Line line(const Ctx& c, T da, T db) { return gen_adj_line<T>(*c.tab, da, db); }
+    static __device__ __forceinline__ Line line(const Ctx& c, T da, T db) { return gen_adj_line<T>(tab(c), da, db); }
     static __device__ __forceinline__ void point(const Ctx& c, T ax, T ay, T az, const Line& lxy, const Line& lxz,
                                                  const Line& lyz, T& W, T& Wx, T& Wy, T& Wz) {
         // the forward's region decision (PolicyGen::region): the radius in T, compared in double
-        const int k = gen_region_of(*c.tab, static_cast<double>(KMath<T>::sqrt_(ax * ax + ay * ay + az * az)));
-        gen_adj_point<T>(*c.tab, k, ax, ay, az, lxy, lxz, lyz, W, Wx, Wy, Wz);
+        const int k = gen_region_of(tab(c), static_cast<double>(KMath<T>::sqrt_(ax * ax + ay * ay + az * az)));
+        gen_adj_point<T, KID>(tab(c), k, ax, ay, az, lxy, lxz, lyz, W, Wx, Wy, Wz);
     }
 };
+
+// Calls f(P{}) with the adjoint's generic policy of kernel kid in precision T (single precision: the
+// kernels gen_mixed_ok names).
+template <typename T, class F>
+hipError_t with_adj_gen(int kid, F&& f) {
+    switch (kid) {
+        case KIWI_KERNEL_M4:
+            return f(AdjGen<T, KIWI_KERNEL_M4>{});
+        case KIWI_KERNEL_WC2:
+            return f(AdjGen<T, KIWI_KERNEL_WC2>{});
+        default:
+            break;
+    }
+    if constexpr (std::is_same<T, double>::value) {
+        switch (kid) {
+            case KIWI_KERNEL_M5:
+                return f(AdjGen<T, KIWI_KERNEL_M5>{});
+            case KIWI_KERNEL_M6:
+                return f(AdjGen<T, KIWI_KERNEL_M6>{});
+            case KIWI_KERNEL_WC4:
+                return f(AdjGen<T, KIWI_KERNEL_WC4>{});
+            case KIWI_KERNEL_WC6:
+                return f(AdjGen<T, KIWI_KERNEL_WC6>{});
+            default:
+                break;
+        }
+    }
+    return hipErrorInvalidValue;
+}
 
 __global__ void k_adj_fill(double* p, int64_t n) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
@@ -511,11 +547,17 @@ int kiwi_hip_sph_to_grid_adjoint(kiwi_hip_ctx* c, const kiwi_particles* P, const
     const GenTab* tabs = static_cast<const GenTab*>(c->tables.p) + K->kernel_id;
     const bool single = precision == KIWI_MIXED && (!kc.generic || gen_mixed_ok(K->kernel_id));
     if (kc.generic && single)
-        KIWI_HIP_CHECK(c, run_adjoint<AdjGen<float>>(c, s, {sup, tabs}, n, n_active, dp, bbox, loffs, nlines, order, g,
-                                                     gin, clamp, go, dst));
+        KIWI_HIP_CHECK(c, with_adj_gen<float>(K->kernel_id, [&](auto pol) {
+                           using P = decltype(pol);
+                           return run_adjoint<P>(c, s, typename P::Ctx{sup, tabs}, n, n_active, dp, bbox, loffs, nlines,
+                                                 order, g, gin, clamp, go, dst);
+                       }));
     else if (kc.generic)
-        KIWI_HIP_CHECK(c, run_adjoint<AdjGen<double>>(c, s, {sup, tabs}, n, n_active, dp, bbox, loffs, nlines, order, g,
-                                                      gin, 1, go, dst));  // FP64: per-pair clamp
+        KIWI_HIP_CHECK(c, with_adj_gen<double>(K->kernel_id, [&](auto pol) {
+                           using P = decltype(pol);
+                           return run_adjoint<P>(c, s, typename P::Ctx{sup, tabs}, n, n_active, dp, bbox, loffs, nlines,
+                                                 order, g, gin, 1, go, dst);  // FP64: per-pair clamp
+                       }));
     else if (single)
         KIWI_HIP_CHECK(c, run_adjoint<AdjM4<float>>(c, s, {sup}, n, n_active, dp, bbox, loffs, nlines, order, g, gin,
                                                     clamp, go, dst));

@@ -9,9 +9,11 @@ copy with 5% Gaussian mass perturbations.
      differences of the FP64 forward (fourth order for x, y, z, h), evaluated by replacing the
      particle's own contribution (a one-particle forward at the shifted value) in the full grid;
   3. the mixed adjoint against the FP64 adjoint (per gradient, relative to its largest value);
-  4. 200 steps of mass recovery (Adam on the masses through torch.autograd).
+  4. 200 steps of mass recovery (Adam on the masses through torch.autograd; cubic spline only).
+--kernel K times and checks the adjoint of another kernel (M5, M6, WC2, WC4, WC6 or M4g, the cubic
+spline on the generic path) on the same workload.
 
-    python tests/perf_adjoint_fit.py [--particles 200000] [--grid 128] [--fd 1000] [--steps 200]
+    python tests/perf_adjoint_fit.py [--particles 200000] [--grid 128] [--fd 1000] [--steps 200] [--kernel M4]
 
 Prints progress to stderr and one JSON line to stdout.
 """
@@ -39,6 +41,7 @@ def main():
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--fd", type=int, default=1000)
     ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--kernel", default="M4", help="M4 (analytic), M4g (generic path), M5, M6, WC2, WC4, WC6")
     args = ap.parse_args()
 
     import torch
@@ -46,6 +49,9 @@ def main():
     from kiwi_amd import hip
     from kiwi_amd.autograd import sph_to_grid
 
+    kern = hip.kernel_spec("M4", generic_path=True) if args.kernel == "M4g" else hip.kernel_spec(args.kernel)
+    if args.kernel != "M4" and args.steps:
+        sys.exit("the mass-recovery fit runs the cubic spline only: pass --steps 0 with --kernel")
     parts, dom, res = workloads.lognormal_field(n_particles=args.particles, n=args.grid, seed=4)
     parts = {k: np.ascontiguousarray(parts[k], dtype=np.float64) for k in workloads.COLS}
     n = len(parts["x"])
@@ -63,11 +69,12 @@ def main():
 
     def forward(tt, precision=hip.FP64):
         out = tuple(torch.zeros(nc, dtype=torch.float64, device=dev) for _ in range(4))
-        ctx.sph_to_grid(tt, dom, res, precision=precision, out=out)
+        ctx.sph_to_grid(tt, dom, res, kernel=kern, precision=precision, out=out)
         return out
 
     target = forward(tp)[0] / V
-    res_json = {"workload": f"adjoint: {n} log-normal particles onto {res[0]}^3, M4, L = sum (rho - rho_target)^2"}
+    res_json = {"workload": f"adjoint: {n} log-normal particles onto {res[0]}^3, {args.kernel}, "
+                            f"L = sum (rho - rho_target)^2"}
 
     # 1. times
     def timed(fn, reps=5):
@@ -88,11 +95,11 @@ def main():
     outs = {f: torch.zeros(n, dtype=torch.float64, device=dev) for f in hip.GRAD_FIELDS}
     outs32 = {f: torch.zeros(n, dtype=torch.float64, device=dev) for f in hip.GRAD_FIELDS}
     times = {"forward_fp64_ms": timed(lambda: forward(t)), "forward_mixed_ms": timed(lambda: forward(t, hip.MIXED)),
-             "adjoint_fp64_ms": timed(lambda: ctx.sph_to_grid_adjoint(t, dom, res, gcells, out=outs)),
-             "adjoint_mixed_ms": timed(lambda: ctx.sph_to_grid_adjoint(t, dom, res, gcells, precision=hip.MIXED,
-                                                                       out=outs32))}
-    st = ctx.sph_to_grid_adjoint(t, dom, res, gcells, out=outs)[1]
-    ctx.sph_to_grid_adjoint(t, dom, res, gcells, precision=hip.MIXED, out=outs32)
+             "adjoint_fp64_ms": timed(lambda: ctx.sph_to_grid_adjoint(t, dom, res, gcells, kernel=kern, out=outs)),
+             "adjoint_mixed_ms": timed(lambda: ctx.sph_to_grid_adjoint(t, dom, res, gcells, kernel=kern,
+                                                                       precision=hip.MIXED, out=outs32))}
+    st = ctx.sph_to_grid_adjoint(t, dom, res, gcells, kernel=kern, out=outs)[1]
+    ctx.sph_to_grid_adjoint(t, dom, res, gcells, kernel=kern, precision=hip.MIXED, out=outs32)
     torch.cuda.synchronize()
     res_json.update(times)
     res_json["adjoint_cells"] = st.cells
@@ -111,7 +118,7 @@ def main():
             one[k].copy_(t[k][i:i + 1])
         if f is not None:
             one[f].fill_(value)
-        ctx.sph_to_grid(one, dom, res, out=single_out)
+        ctx.sph_to_grid(one, dom, res, kernel=kern, out=single_out)
         return single_out[0].clone()
 
     fd_err = {}
