@@ -10,6 +10,10 @@
  *   kiwi_hip_voronoi_map   <- the particle loop of run_mapping_stage (src/mapping.cpp:400-460): the BFS
  *                             over the tessellation with compute_cell_integral (:105-206) per cell.
  *
+ * and, beside them, the inverse-mapping test, the cartesian-map resample, the coverage-grid buffer, the
+ * Lloyd step's centroids and the Voronoi tessellation the map consumes (kiwi_hip_voronoi_tessellate, in
+ * place of the voro++ container of src/mapping.cpp:222-269 and src/tessellation.cpp:23-61).
+ *
  * The host keeps what the reference keeps outside those loops: configuration, snapshot reading,
  * normalisation (src/cartesian_grid.cpp:294-318, src/mapping.cpp:473-502), file output and caching.
  *
@@ -37,7 +41,7 @@
 extern "C" {
 #endif
 
-#define KIWI_HIP_ABI_VERSION 11
+#define KIWI_HIP_ABI_VERSION 12
 
 #define KIWI_OK 0
 #define KIWI_ERR_INVALID_ARG (-1)
@@ -381,6 +385,44 @@ int kiwi_hip_voronoi_map(kiwi_hip_ctx* ctx, const kiwi_particles* particles, con
  * where cells->on_device says.  A cell of zero volume keeps its site.  Malformed offsets: KIWI_ERR_INVALID_ARG. */
 int kiwi_hip_voronoi_centroids(kiwi_hip_ctx* ctx, const kiwi_voronoi_cells* cells, const double* site_xyz,
                                double* centroid_xyz, double* volume);
+
+/* A point set: n positions, host or device memory. */
+typedef struct kiwi_point_set {
+    int64_t n;
+    const double *x, *y, *z;
+    int on_device;
+} kiwi_point_set;
+
+/* The last tessellation of a context: device arrays owned by the context, valid until its next
+ * kiwi_hip_voronoi_tessellate call or kiwi_hip_destroy. */
+typedef struct kiwi_tessellation {
+    kiwi_voronoi_cells cells; /* on_device = 1; domain = the tessellated box */
+    const int* cell_site;     /* n_cells: the input index of each cell's site */
+    int64_t n_dropped;        /* sites the container drops (outside the box, or on a max face by voro++'s
+                                 rounding of its block index) */
+    double device_ms_prep, device_ms_count, device_ms_fill; /* cell order and buckets; count pass; fill pass */
+} kiwi_tessellation;
+
+/* The Voronoi tessellation of the sites inside domain[6] (non-periodic), replacing the voro++ container
+ * the reference fills and walks in build_cell_geometry (src/mapping.cpp:222-269) and
+ * run_tessellation_stage (src/tessellation.cpp:23-61): each cell cut from the domain box by its
+ * neighbours' bisector planes with voro++'s plane tolerance (voronoicell_base, default cell size), in
+ * FP64.  Cells come in voro++'s c_loop_all order over its block grid (pre_container::guess_optimal scaled
+ * by the default block_scale_factor 3, include/kiwi/voro_grid.hpp:18-26): the order of the mapping.dat
+ * rows.  Sites are kept as voro++'s container keeps them.  Faces and neighbours as kiwi_voronoi_cells,
+ * each face's vertex cycle in voro++'s orientation (it may start at another vertex).  Non-finite
+ * coordinates: KIWI_ERR_INVALID_ARG; a cell beyond 384 vertices, 192 faces or 48 vertices on a face:
+ * KIWI_ERR_LIMIT. */
+int kiwi_hip_voronoi_tessellate(kiwi_hip_ctx* ctx, const kiwi_point_set* sites, const double domain[6],
+                                kiwi_tessellation* out);
+
+/* Copies the context's last tessellation into the caller's arrays (sizes from its kiwi_tessellation):
+ * cell_face_ptr[n_cells + 1], face_vert_ptr[n_faces + 1], vert_xyz[3 n_face_vertices],
+ * face_neighbor[n_faces], cell_site[n_cells]; device memory if on_device, else host memory.  The voro++
+ * counterpart is the per-cell face_vertices / vertices / neighbors read-out of build_cell_geometry
+ * (src/mapping.cpp:249-266). */
+int kiwi_hip_voronoi_tessellation_fetch(kiwi_hip_ctx* ctx, int64_t* cell_face_ptr, int64_t* face_vert_ptr,
+                                        double* vert_xyz, int* face_neighbor, int* cell_site, int on_device);
 
 /* Per-particle rows of the inverse-mapping test. */
 typedef struct kiwi_inverse_rows {

@@ -26,7 +26,7 @@ def _run(cmd):
 
 
 SOURCES = ("kiwi_hip.hip", "kiwi_voronoi.hip", "kiwi_adjoint.hip", "kiwi_cartesian_map.hip", "kiwi_coverage.hip",
-           "kiwi_lloyd.hip")
+           "kiwi_lloyd.hip", "kiwi_tessellate.hip")
 OBJ_DIR = os.path.join(PKG_DIR, "_build")
 
 

@@ -137,6 +137,9 @@ struct kiwi_hip_ctx {
     // (kiwi_hip_set_fp64_fallback; <= 0: off)
     double fb_cells = KIWI_FP64_FALLBACK_CELLS;
     DevBuf vor[16];  // the Voronoi map's buffers
+    // the Voronoi tessellation's buffers; [8] .. [11] hold the last tessellation (kiwi_hip_voronoi_tessellate)
+    DevBuf tess[12];
+    int64_t tess_sizes[3] = {-1, 0, 0};  // the last tessellation's cells, faces, face-vertex entries (-1: none)
     // sph-to-grid HIP graphs (kiwi_hip_set_graphs): the captured pipelines, most recently used last; the
     // dependency-only events that fork the second stream in a capture; the graph-mode radix sort's
     // buffers; pinned host memory that the replayed pipeline copies its statistics to
@@ -164,6 +167,7 @@ struct kiwi_hip_ctx {
                           &acc_parts, &g_ktmp, &g_vtmp, &g_hist, &g_hoff})
             f(*b);
         for (DevBuf& b : vor) f(b);
+        for (DevBuf& b : tess) f(b);
     }
 };
 
@@ -184,6 +188,18 @@ int hip_fail(kiwi_hip_ctx* c, hipError_t e, const char* what) {
         hipError_t e_ = (expr);                                  \
         if (e_ != hipSuccess) return hip_fail((ctx), e_, #expr); \
     } while (0)
+
+// Whether a kernel may dereference p: device, managed or pinned host memory (not pageable host memory,
+// which a kernel faults on).  A failed query leaves no error behind for later checks.
+inline bool gpu_accessible(const void* p) {
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type != hipMemoryTypeUnregistered;
+}
 
 hipError_t reserve(DevBuf& b, size_t bytes) {
     if (bytes <= b.cap) return hipSuccess;

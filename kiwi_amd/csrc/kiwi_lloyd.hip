@@ -92,6 +92,14 @@ int kiwi_hip_voronoi_centroids(kiwi_hip_ctx* c, const kiwi_voronoi_cells* V, con
     if (nc > 0x7fffffffLL || nf > 0x7fffffffLL)
         return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 cells or faces in one call");
     KIWI_HIP_CHECK(c, hipSetDevice(c->device));
+    if (V->on_device) {  // every array a kernel reads or writes: device memory, as on_device says
+        const void* need[6] = {V->cell_face_ptr, nf > 0 ? static_cast<const void*>(V->face_vert_ptr) : site_xyz,
+                               nv > 0 ? V->vert_xyz : site_xyz, site_xyz, centroid_xyz,
+                               volume ? volume : centroid_xyz};
+        for (const void* p : need)
+            if (!gpu_accessible(p))
+                return fail(c, KIWI_ERR_INVALID_ARG, "cells->on_device is set but an array is host memory");
+    }
     hipStream_t s = c->stream;
     const int64_t* cfp = V->cell_face_ptr;
     const int64_t* fvp = V->face_vert_ptr;

@@ -1,0 +1,392 @@
+// libkiwi_hip: the Voronoi tessellation (kiwi_hip_voronoi_tessellate) of a point set inside the domain
+// box -- the geometry the reference builds with voro++ for its mapping stage and tessellation summary
+// (build_cell_geometry, src/mapping.cpp:222-269; run_tessellation_stage, src/tessellation.cpp:23-61) --
+// flattened to the CSR arrays the Voronoi map, the inverse-mapping test and the Lloyd step take.
+//
+//   1. k_tv_count      non-finite sites, and the sites voro++'s pre_container keeps (inside [min, max])
+//   2. k_tv_block_keys voro++'s block of every site (guess_optimal x 3 blocks, container put_remap):
+//                      sites the container drops sort past the last block
+//   3. sort, k_tv_order  the cell order: blocks x fastest, sites in input order within a block (voro++
+//                      c_loop_all), cell <-> site maps
+//   4. k_tv_bucket_keys, sort, k_tv_gather   the kept sites in a bucket grid of about 2 a bucket, for the
+//                      neighbour search
+//   5. k_tv_cells<false> one thread per cell (a strided workspace cell per thread in flight): the domain
+//                      box cut by the neighbours' bisector planes, shell by shell of buckets until the
+//                      farthest vertex rules out the rest; writes the cell's face and face-vertex counts
+//   6. scans           face and face-vertex offsets per cell
+//   7. k_tv_cells<true>  the same cells written as face offsets, absolute vertex coordinates and
+//                      neighbour cell indices: read back from the workspace where every cell had a
+//                      slot of its own, computed again otherwise
+//
+// The cell computation (kiwi_amd/csrc/voro_cell.h) is voro++'s plane cut with its tolerance, in FP64; the
+// cell order and the sites kept follow voro++'s block grid exactly; each face's vertex cycle has
+// voro++'s orientation and may start at another vertex.  Recomputing in step 7 gives the same cells as
+// step 5 (the same code on the same inputs).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+
+#include "common.h"
+#include "voro_cell.h"
+
+namespace {
+
+using kiwi_hip::VCell;
+using kiwi_hip::VcGrid;
+using kiwi_hip::VcSite;
+
+// cells computed at once: each thread in flight holds one workspace cell (VC_BYTES, 34 KB), at most 8.9 GB
+// in all.  Up to this many cells the fill pass reads the count pass's cells instead of recomputing them.
+constexpr int64_t TV_MAX_INFLIGHT = 262144;
+
+struct TvBox {
+    double d[6];
+    int nbk[3];
+};
+
+// [0] non-finite coordinates, [1] sites inside [min, max] on every axis
+__global__ void k_tv_count(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                           const double* __restrict__ z, TvBox b, unsigned long long* counts) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
+    if (i >= n) return;
+    const double p[3] = {x[i], y[i], z[i]};
+    bool fin = true, in = true;
+    for (int a = 0; a < 3; ++a) {
+        fin = fin && isfinite(p[a]);
+        in = in && p[a] >= b.d[2 * a] && p[a] <= b.d[2 * a + 1];
+    }
+    if (!fin) atomicAdd(&counts[0], 1ull);
+    if (in) atomicAdd(&counts[1], 1ull);
+}
+
+__global__ void k_tv_block_keys(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                                const double* __restrict__ z, TvBox b, uint64_t nblocks, uint64_t* __restrict__ keys) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
+    if (i >= n) return;
+    const int64_t ijk = kiwi_hip::vc_block_index(b.d, b.nbk, x[i], y[i], z[i]);
+    const uint64_t k = ijk < 0 ? nblocks : static_cast<uint64_t>(ijk);
+    keys[i] = (k << 32) | static_cast<uint64_t>(i);
+}
+
+// sorted block keys -> cell c = position among the kept sites; n_cells = the kept count
+__global__ void k_tv_order(int64_t n, uint64_t nblocks, const uint64_t* __restrict__ keys, int* __restrict__ cell_site,
+                           int* __restrict__ site_cell, unsigned long long* n_cells) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
+    if (t >= n) return;
+    const uint64_t k = keys[t];
+    if ((k >> 32) >= nblocks) return;
+    const int i = static_cast<int>(k & 0xffffffffull);
+    cell_site[t] = i;
+    site_cell[i] = static_cast<int>(t);
+    if (t == n - 1 || (keys[t + 1] >> 32) >= nblocks) *n_cells = static_cast<unsigned long long>(t + 1);
+}
+
+__global__ void k_tv_bucket_keys(int nc, const int* __restrict__ cell_site, const double* __restrict__ x,
+                                 const double* __restrict__ y, const double* __restrict__ z, VcGrid g,
+                                 uint64_t* __restrict__ keys) {
+    const int c = blockIdx.x * THREADS + threadIdx.x;
+    if (c >= nc) return;
+    const int i = cell_site[c];
+    const uint64_t b = (static_cast<uint64_t>(kiwi_hip::vc_bucket_axis(g, 0, x[i])) * g.nb[1] +
+                        kiwi_hip::vc_bucket_axis(g, 1, y[i])) * g.nb[2] + kiwi_hip::vc_bucket_axis(g, 2, z[i]);
+    keys[c] = (b << 32) | static_cast<uint64_t>(i);
+}
+
+// the kept sites in bucket order, [beg, end) of every bucket at range[2 b], range[2 b + 1]
+__global__ void k_tv_gather(int nc, const uint64_t* __restrict__ keys, const double* __restrict__ x,
+                            const double* __restrict__ y, const double* __restrict__ z, VcSite* __restrict__ sites,
+                            int* __restrict__ range) {
+    const int t = blockIdx.x * THREADS + threadIdx.x;
+    if (t >= nc) return;
+    const uint64_t k = keys[t];
+    const uint64_t b = k >> 32;
+    const int i = static_cast<int>(k & 0xffffffffull);
+    sites[t] = VcSite{x[i], y[i], z[i], i, 0};
+    if (t == 0 || (keys[t - 1] >> 32) != b) range[2 * b] = t;
+    if (t == nc - 1 || (keys[t + 1] >> 32) != b) range[2 * b + 1] = t + 1;
+}
+
+struct TvOut {
+    int64_t* nface;        // count pass: faces per cell
+    int64_t* nfv;          // count pass: face-vertex entries per cell
+    const int64_t* cfp;    // fill pass: first face of each cell
+    const int64_t* cvo;    // fill pass: first face-vertex entry of each cell
+    int64_t* fvp;          // fill pass: face -> first face-vertex entry
+    double* vxyz;          // fill pass: 3 per face-vertex entry
+    int* fnbr;             // fill pass: neighbour cell per face
+    unsigned long long* bad;  // cells whose computation failed ([0] empty, [1] past the per-cell limits)
+};
+
+template <bool FILL>
+__global__ __launch_bounds__(THREADS) void k_tv_cells(int nc, int64_t inflight, const int* __restrict__ cell_site,
+                                                      const int* __restrict__ site_cell, const double* __restrict__ x,
+                                                      const double* __restrict__ y, const double* __restrict__ z,
+                                                      TvBox b, VcGrid g, const int* __restrict__ range,
+                                                      const VcSite* __restrict__ sites, double tol, void* ws, TvOut o) {
+    const int64_t slot = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
+    if (slot >= inflight) return;
+    VCell cell;
+    kiwi_hip::vc_bind(cell, ws, inflight, slot);
+    const bool kept = inflight >= nc;  // every cell in its own slot: the count pass's cells are still there
+    for (int64_t c = slot; c < nc; c += inflight) {
+        const int i = cell_site[c];
+        const double sx = x[i], sy = y[i], sz = z[i];
+        int rc = kiwi_hip::VC_CUT;
+        if (FILL && kept)
+            kiwi_hip::vc_load(cell);
+        else
+            rc = kiwi_hip::vc_compute(cell, b.d, g, range, sites, site_cell, i, sx, sy, sz, tol);
+        if (rc != kiwi_hip::VC_CUT) {
+            if (!FILL) {
+                atomicAdd(&o.bad[rc == kiwi_hip::VC_EMPTY ? 0 : 1], 1ull);
+                o.nface[c] = 0;
+                o.nfv[c] = 0;
+            }
+            continue;
+        }
+        if (!FILL) {
+            o.nface[c] = cell.nf;
+            o.nfv[c] = kiwi_hip::vc_face_vertices(cell);
+            kiwi_hip::vc_save(cell);
+        } else {
+            kiwi_hip::vc_write(cell, sx, sy, sz, site_cell, o.cfp[c], o.cvo[c], o.fvp, o.vxyz, o.fnbr);
+        }
+    }
+}
+
+int bits_for(uint64_t v) {  // bits that hold 0..v
+    int b = 1;
+    while ((v >> b) != 0) ++b;
+    return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kiwi_hip_voronoi_tessellate(kiwi_hip_ctx* c, const kiwi_point_set* pts, const double domain[6],
+                                kiwi_tessellation* out) {
+    if (!c) return KIWI_ERR_INVALID_ARG;
+    c->err.clear();
+    if (!pts || !domain || !out) return fail(c, KIWI_ERR_INVALID_ARG, "null argument");
+    c->tess_sizes[0] = -1;
+    if (pts->n < 0) return fail(c, KIWI_ERR_INVALID_ARG, "negative site count");
+    if (pts->n > 0x7fffffffLL) return fail(c, KIWI_ERR_LIMIT, "more than 2^31-1 sites in one call");
+    if (pts->n > 0 && (!pts->x || !pts->y || !pts->z)) return fail(c, KIWI_ERR_INVALID_ARG, "null site array");
+    for (int a = 0; a < 3; ++a)
+        if (!(domain[2 * a] < domain[2 * a + 1]) || !std::isfinite(domain[2 * a]) || !std::isfinite(domain[2 * a + 1]))
+            return fail(c, KIWI_ERR_INVALID_ARG, "domain min must be < max on every axis");
+    KIWI_HIP_CHECK(c, hipSetDevice(c->device));
+    if (pts->on_device && pts->n > 0 && !(gpu_accessible(pts->x) && gpu_accessible(pts->y) && gpu_accessible(pts->z)))
+        return fail(c, KIWI_ERR_INVALID_ARG, "sites->on_device is set but a coordinate array is host memory");
+    hipStream_t s = c->stream;
+    const int64_t n = pts->n;
+    const int64_t n1 = std::max<int64_t>(n, 1);
+    DevBuf* B = c->tess;
+
+    // sites on the device
+    const double* dx = pts->x;
+    const double* dy = pts->y;
+    const double* dz = pts->z;
+    if (!pts->on_device) {
+        KIWI_HIP_CHECK(c, reserve(B[0], 3 * sizeof(double) * n1));
+        double* d = static_cast<double*>(B[0].p);
+        if (n) {
+            KIWI_HIP_CHECK(c, hipMemcpyAsync(d, pts->x, n * sizeof(double), hipMemcpyHostToDevice, s));
+            KIWI_HIP_CHECK(c, hipMemcpyAsync(d + n, pts->y, n * sizeof(double), hipMemcpyHostToDevice, s));
+            KIWI_HIP_CHECK(c, hipMemcpyAsync(d + 2 * n, pts->z, n * sizeof(double), hipMemcpyHostToDevice, s));
+        }
+        dx = d;
+        dy = d + n;
+        dz = d + 2 * n;
+    }
+    TvBox box;
+    for (int a = 0; a < 6; ++a) box.d[a] = domain[a];
+    // [0] non-finite sites, [1] sites inside [min, max], [2] cells, [3] cells cut away, [4] cells past the limits
+    KIWI_HIP_CHECK(c, reserve(B[1], 5 * sizeof(unsigned long long)));
+    unsigned long long* dcnt = static_cast<unsigned long long*>(B[1].p);
+    KIWI_HIP_CHECK(c, hipMemsetAsync(dcnt, 0, 5 * sizeof(unsigned long long), s));
+    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[0], s));
+    unsigned long long cnt[2] = {0, 0};
+    if (n) {
+        k_tv_count<<<grid1d(n), THREADS, 0, s>>>(n, dx, dy, dz, box, dcnt);
+        KIWI_HIP_CHECK(c, hipGetLastError());
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(cnt, dcnt, sizeof(cnt), hipMemcpyDeviceToHost, s));
+        KIWI_HIP_CHECK(c, hipStreamSynchronize(s));
+        if (cnt[0]) return fail(c, KIWI_ERR_INVALID_ARG, "non-finite site coordinate");
+    }
+    kiwi_hip::vc_block_dims(box.d, static_cast<int64_t>(cnt[1]), box.nbk);
+    const uint64_t nblocks = static_cast<uint64_t>(box.nbk[0]) * box.nbk[1] * box.nbk[2];
+    if (nblocks >= (1ull << 31)) return fail(c, KIWI_ERR_LIMIT, "too many voro++ blocks");
+
+    // cell order
+    KIWI_HIP_CHECK(c, reserve(B[2], 2 * sizeof(uint64_t) * n1));
+    uint64_t* keys_in = static_cast<uint64_t*>(B[2].p);
+    uint64_t* keys = keys_in + n1;
+    KIWI_HIP_CHECK(c, reserve(B[3], 2 * sizeof(int) * n1));
+    int* site_cell = static_cast<int*>(B[3].p);
+    int* cell_site = site_cell + n1;
+    KIWI_HIP_CHECK(c, hipMemsetAsync(site_cell, 0xff, sizeof(int) * n1, s));
+    unsigned long long hnc = 0;
+    if (n) {
+        k_tv_block_keys<<<grid1d(n), THREADS, 0, s>>>(n, dx, dy, dz, box, nblocks, keys_in);
+        KIWI_HIP_CHECK(c, hipGetLastError());
+        const int hi_bit = 32 + bits_for(nblocks);
+        size_t tb = 0;
+        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortKeys(nullptr, tb, keys_in, keys, static_cast<int>(n), 32, hi_bit, s));
+        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, tb));
+        tb = c->cub_tmp.cap;
+        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortKeys(c->cub_tmp.p, tb, keys_in, keys, static_cast<int>(n), 32,
+                                                            hi_bit, s));
+        k_tv_order<<<grid1d(n), THREADS, 0, s>>>(n, nblocks, keys, cell_site, site_cell, dcnt + 2);
+        KIWI_HIP_CHECK(c, hipGetLastError());
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(&hnc, dcnt + 2, sizeof(hnc), hipMemcpyDeviceToHost, s));
+        KIWI_HIP_CHECK(c, hipStreamSynchronize(s));
+    }
+    const int nc = static_cast<int>(hnc);
+
+    // bucket grid of the kept sites
+    const VcGrid g = kiwi_hip::vc_grid_make(box.d, nc);
+    const int64_t nb = static_cast<int64_t>(g.nb[0]) * g.nb[1] * g.nb[2];
+    KIWI_HIP_CHECK(c, reserve(B[4], sizeof(VcSite) * std::max(nc, 1)));
+    VcSite* sites = static_cast<VcSite*>(B[4].p);
+    KIWI_HIP_CHECK(c, reserve(B[5], 2 * sizeof(int) * nb));
+    int* range = static_cast<int*>(B[5].p);
+    KIWI_HIP_CHECK(c, hipMemsetAsync(range, 0, 2 * sizeof(int) * nb, s));
+    if (nc) {
+        k_tv_bucket_keys<<<grid1d(nc), THREADS, 0, s>>>(nc, cell_site, dx, dy, dz, g, keys_in);
+        KIWI_HIP_CHECK(c, hipGetLastError());
+        const int hi_bit = 32 + bits_for(static_cast<uint64_t>(nb));
+        size_t tb = 0;
+        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortKeys(nullptr, tb, keys_in, keys, nc, 0, hi_bit, s));
+        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, tb));
+        tb = c->cub_tmp.cap;
+        KIWI_HIP_CHECK(c, hipcub::DeviceRadixSort::SortKeys(c->cub_tmp.p, tb, keys_in, keys, nc, 0, hi_bit, s));
+        k_tv_gather<<<grid1d(nc), THREADS, 0, s>>>(nc, keys, dx, dy, dz, sites, range);
+        KIWI_HIP_CHECK(c, hipGetLastError());
+    }
+    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[1], s));
+
+    // count pass
+    const int64_t inflight = std::min<int64_t>(std::max(nc, 1), TV_MAX_INFLIGHT);
+    KIWI_HIP_CHECK(c, reserve(B[6], static_cast<size_t>(kiwi_hip::VC_BYTES) * inflight));
+    KIWI_HIP_CHECK(c, reserve(B[7], 3 * sizeof(int64_t) * (static_cast<int64_t>(nc) + 1)));
+    int64_t* nface = static_cast<int64_t*>(B[7].p);
+    int64_t* nfv = nface + nc + 1;
+    int64_t* cvo = nfv + nc + 1;
+    // outputs owned by the context until the next call: cell_face_ptr, cell_site, then the sized ones
+    KIWI_HIP_CHECK(c, reserve(B[8], sizeof(int64_t) * (static_cast<int64_t>(nc) + 1)));
+    int64_t* cfp = static_cast<int64_t*>(B[8].p);
+    KIWI_HIP_CHECK(c, reserve(B[9], sizeof(int) * std::max(nc, 1)));
+    int* out_site = static_cast<int*>(B[9].p);
+    TvOut o{nface, nfv, cfp, cvo, nullptr, nullptr, nullptr, dcnt + 3};
+    const double tol = kiwi_hip::VC_TOL_DEFAULT;
+    const int grid = static_cast<int>((inflight + THREADS - 1) / THREADS);
+    int64_t tot[2] = {0, 0};  // faces, face-vertex entries
+    if (nc) {
+        k_tv_cells<false><<<grid, THREADS, 0, s>>>(nc, inflight, cell_site, site_cell, dx, dy, dz, box, g, range, sites,
+                                                   tol, B[6].p, o);
+        KIWI_HIP_CHECK(c, hipGetLastError());
+        size_t tb = 0;
+        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, nface, cfp, nc + 1, s));
+        KIWI_HIP_CHECK(c, reserve(c->cub_tmp, tb));
+        tb = c->cub_tmp.cap;
+        // nface[nc] and nfv[nc] are 0, so the scans' last entries are the totals
+        KIWI_HIP_CHECK(c, hipMemsetAsync(nface + nc, 0, sizeof(int64_t), s));
+        KIWI_HIP_CHECK(c, hipMemsetAsync(nfv + nc, 0, sizeof(int64_t), s));
+        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, tb, nface, cfp, nc + 1, s));
+        KIWI_HIP_CHECK(c, hipcub::DeviceScan::ExclusiveSum(c->cub_tmp.p, tb, nfv, cvo, nc + 1, s));
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(&tot[0], cfp + nc, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(&tot[1], cvo + nc, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        unsigned long long bad[2] = {0, 0};
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(bad, dcnt + 3, sizeof(bad), hipMemcpyDeviceToHost, s));
+        KIWI_HIP_CHECK(c, hipStreamSynchronize(s));
+        if (bad[1])
+            return fail(c, KIWI_ERR_LIMIT, std::to_string(bad[1]) + " Voronoi cell(s) past the per-cell limits (" +
+                                               std::to_string(kiwi_hip::VC_MAXV) + " vertices, " +
+                                               std::to_string(kiwi_hip::VC_MAXF) + " faces, " +
+                                               std::to_string(kiwi_hip::VC_MAXFV) + " vertices a face)");
+        if (bad[0]) return fail(c, KIWI_ERR_DEVICE, "a Voronoi cell was cut away entirely");
+    } else {
+        KIWI_HIP_CHECK(c, hipMemsetAsync(cfp, 0, sizeof(int64_t), s));
+    }
+    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[2], s));
+
+    // fill pass
+    KIWI_HIP_CHECK(c, reserve(B[10], sizeof(int64_t) * (tot[0] + 1)));
+    KIWI_HIP_CHECK(c, reserve(B[11], (3 * sizeof(double) + sizeof(int)) * std::max<int64_t>(tot[1] + tot[0], 1)));
+    int64_t* fvp = static_cast<int64_t*>(B[10].p);
+    double* vxyz = static_cast<double*>(B[11].p);
+    int* fnbr = reinterpret_cast<int*>(vxyz + 3 * tot[1]);
+    if (nc) {
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(out_site, cell_site, sizeof(int) * nc, hipMemcpyDeviceToDevice, s));
+        o.fvp = fvp;
+        o.vxyz = vxyz;
+        o.fnbr = fnbr;
+        k_tv_cells<true><<<grid, THREADS, 0, s>>>(nc, inflight, cell_site, site_cell, dx, dy, dz, box, g, range, sites,
+                                                  tol, B[6].p, o);
+        KIWI_HIP_CHECK(c, hipGetLastError());
+    }
+    KIWI_HIP_CHECK(c, hipMemcpyAsync(fvp + tot[0], &tot[1], sizeof(int64_t), hipMemcpyHostToDevice, s));
+    KIWI_HIP_CHECK(c, hipEventRecord(c->ev[3], s));
+    KIWI_HIP_CHECK(c, hipStreamSynchronize(s));
+
+    c->tess_sizes[0] = nc;
+    c->tess_sizes[1] = tot[0];
+    c->tess_sizes[2] = tot[1];
+    kiwi_voronoi_cells& V = out->cells;
+    V.n_cells = nc;
+    V.n_faces = tot[0];
+    V.n_face_vertices = tot[1];
+    V.cell_face_ptr = cfp;
+    V.face_vert_ptr = fvp;
+    V.vert_xyz = vxyz;
+    V.face_neighbor = fnbr;
+    for (int a = 0; a < 6; ++a) V.domain[a] = domain[a];
+    V.on_device = 1;
+    out->cell_site = out_site;
+    out->n_dropped = n - nc;
+    float ms[3] = {0, 0, 0};
+    (void)hipEventElapsedTime(&ms[0], c->ev[0], c->ev[1]);
+    (void)hipEventElapsedTime(&ms[1], c->ev[1], c->ev[2]);
+    (void)hipEventElapsedTime(&ms[2], c->ev[2], c->ev[3]);
+    out->device_ms_prep = ms[0];
+    out->device_ms_count = ms[1];
+    out->device_ms_fill = ms[2];
+    return KIWI_OK;
+}
+
+int kiwi_hip_voronoi_tessellation_fetch(kiwi_hip_ctx* c, int64_t* cell_face_ptr, int64_t* face_vert_ptr,
+                                        double* vert_xyz, int* face_neighbor, int* cell_site, int on_device) {
+    if (!c) return KIWI_ERR_INVALID_ARG;
+    c->err.clear();
+    if (c->tess_sizes[0] < 0) return fail(c, KIWI_ERR_INVALID_ARG, "no tessellation on this context");
+    const int64_t nc = c->tess_sizes[0], nf = c->tess_sizes[1], nv = c->tess_sizes[2];
+    if (!cell_face_ptr || !face_vert_ptr || (nv && !vert_xyz) || (nf && !face_neighbor) || (nc && !cell_site))
+        return fail(c, KIWI_ERR_INVALID_ARG, "null output array");
+    KIWI_HIP_CHECK(c, hipSetDevice(c->device));
+    if (on_device) {
+        const void* cf = cell_face_ptr;
+        const void* outs[5] = {cf, face_vert_ptr, nv ? static_cast<const void*>(vert_xyz) : cf,
+                               nf ? static_cast<const void*>(face_neighbor) : cf,
+                               nc ? static_cast<const void*>(cell_site) : cf};
+        for (const void* p : outs)
+            if (!gpu_accessible(p)) return fail(c, KIWI_ERR_INVALID_ARG, "on_device is set but an output is host memory");
+    }
+    hipStream_t s = c->stream;
+    const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    const double* vxyz = static_cast<const double*>(c->tess[11].p);
+    KIWI_HIP_CHECK(c, hipMemcpyAsync(cell_face_ptr, c->tess[8].p, sizeof(int64_t) * (nc + 1), k, s));
+    KIWI_HIP_CHECK(c, hipMemcpyAsync(face_vert_ptr, c->tess[10].p, sizeof(int64_t) * (nf + 1), k, s));
+    if (nv) KIWI_HIP_CHECK(c, hipMemcpyAsync(vert_xyz, vxyz, 3 * sizeof(double) * nv, k, s));
+    if (nf)
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(face_neighbor, reinterpret_cast<const int*>(vxyz + 3 * nv), sizeof(int) * nf, k, s));
+    if (nc) KIWI_HIP_CHECK(c, hipMemcpyAsync(cell_site, c->tess[9].p, sizeof(int) * nc, k, s));
+    KIWI_HIP_CHECK(c, hipStreamSynchronize(s));
+    return KIWI_OK;
+}
+
+}  // extern "C"

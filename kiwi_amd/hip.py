@@ -37,7 +37,7 @@ EXPORTED = ["kiwi_hip_abi_version", "kiwi_hip_create", "kiwi_hip_destroy", "kiwi
             "kiwi_hip_inverse_mapping", "kiwi_hip_cartesian_map", "kiwi_hip_cartesian_lattice",
             "kiwi_hip_coverage_buffer", "kiwi_hip_voronoi_centroids", "kiwi_hip_sph_to_grid_noalloc",
             "kiwi_hip_set_fp64_fallback", "kiwi_hip_get_fp64_fallback", "kiwi_hip_set_graphs",
-            "kiwi_hip_get_graphs"]
+            "kiwi_hip_get_graphs", "kiwi_hip_voronoi_tessellate", "kiwi_hip_voronoi_tessellation_fetch"]
 
 
 class KiwiError(RuntimeError):
@@ -138,6 +138,15 @@ class VoronoiCells(ctypes.Structure):
                 ("on_device", _i)]
 
 
+class PointSet(ctypes.Structure):
+    _fields_ = [("n", _i64), ("x", _pd), ("y", _pd), ("z", _pd), ("on_device", _i)]
+
+
+class Tessellation(ctypes.Structure):
+    _fields_ = [("cells", VoronoiCells), ("cell_site", _pi), ("n_dropped", _i64), ("device_ms_prep", _d),
+                ("device_ms_count", _d), ("device_ms_fill", _d)]
+
+
 class MapStats(ctypes.Structure):
     _fields_ = [("unresolved", _i64), ("unmapped", _i64), ("n_negative", _i64), ("n_negative_warn", _i64),
                 ("pairs_candidate", _i64), ("pairs_visited", _i64), ("face_evals", _i64), ("v_mean", _d * 3),
@@ -225,6 +234,11 @@ def load():
                                         ctypes.POINTER(ctypes.c_double)]
         L.kiwi_hip_voronoi_centroids.restype = _i
         L.kiwi_hip_voronoi_centroids.argtypes = [ctypes.c_void_p, ctypes.POINTER(VoronoiCells), _pd, _pd, _pd]
+        L.kiwi_hip_voronoi_tessellate.restype = _i
+        L.kiwi_hip_voronoi_tessellate.argtypes = [ctypes.c_void_p, ctypes.POINTER(PointSet), ctypes.POINTER(_d),
+                                                  ctypes.POINTER(Tessellation)]
+        L.kiwi_hip_voronoi_tessellation_fetch.restype = _i
+        L.kiwi_hip_voronoi_tessellation_fetch.argtypes = [ctypes.c_void_p, _pll, _pll, _pd, _pi, _pi, _i]
         L.kiwi_hip_voronoi_map.restype = _i
         L.kiwi_hip_voronoi_map.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.c_void_p,
                                            ctypes.POINTER(VoronoiCells), ctypes.POINTER(Kernel), _i,
@@ -295,6 +309,7 @@ class Context:
         if rc != 0:
             raise KiwiError(rc, f"kiwi_hip_create(device={device}) failed")
         self.h = h
+        self.device = int(device)
 
     def close(self):
         if self.h:
@@ -574,9 +589,9 @@ class Context:
 
     def voronoi_centroids(self, cells, sites, out=None):
         """The Lloyd step: each cell's centroid (its site's next position) and volume.  cells: as
-        voronoi_map; sites: (n_cells, 3) float64, host array or device tensor like the cells.  out: None
-        (host results) or (centroids (n_cells, 3), volumes (n_cells,)) device tensors.
-        Returns (centroids, volumes)."""
+        voronoi_map; sites: (n_cells, 3) float64, host array or device tensor like the cells.  out:
+        (centroids (n_cells, 3), volumes (n_cells,)) arrays where the cells live, or None: new ones there
+        (numpy arrays for host cells, device tensors for device cells).  Returns (centroids, volumes)."""
         _device_inputs_ready(cells, sites, out)
         cfp, d0 = _ptr(cells["cell_face_ptr"], _pll, "int64")
         fvp, d1 = _ptr(cells["face_vert_ptr"], _pll, "int64")
@@ -590,14 +605,64 @@ class Context:
         nf = len(cells["face_vert_ptr"]) - 1
         nv = int(np.prod(cells["vert_xyz"].shape)) // 3
         V = VoronoiCells(nc, nf, nv, cfp, fvp, vx, None, (_d * 6)(*[float(v) for v in cells["domain"]]), d0)
+        if out is None and d0:  # the library writes the outputs where the cells live
+            import torch
+            dev = cells["cell_face_ptr"].device
+            out = (torch.empty((max(nc, 0), 3), dtype=torch.float64, device=dev),
+                   torch.empty(max(nc, 0), dtype=torch.float64, device=dev))
         if out is None:
             cen, vol = np.zeros((max(nc, 0), 3)), np.zeros(max(nc, 0))
             cp, vp = cen.ctypes.data_as(_pd), vol.ctypes.data_as(_pd)
         else:
             cen, vol = out
-            cp, vp = _ptr(cen)[0], _ptr(vol)[0]
+            (cp, e0), (vp, e1) = _ptr(cen), _ptr(vol)
+            if e0 != d0 or e1 != d0:
+                raise ValueError("centroid and volume outputs must live where the cells do")
         self._check(self.L.kiwi_hip_voronoi_centroids(self.h, ctypes.byref(V), sp, cp, vp))
         return cen, vol
+
+    def voronoi_tessellate(self, sites, domain):
+        """The Voronoi tessellation of the sites inside the domain box (kiwi_hip_voronoi_tessellate).
+        sites: (n, 3) float64, host array or device tensor.  Returns (cells, info): cells as voronoi_map
+        takes them, device tensors (cell_face_ptr, face_vert_ptr, vert_xyz (n_vertices, 3), face_nbr) plus
+        cell_site (int32: the input index of each cell's site, voro++'s cell order) and domain; info: dict
+        with n_dropped and the device times.  The tensors are copies (kiwi_hip_voronoi_tessellation_fetch):
+        the library's own arrays are reused by its next tessellation."""
+        import torch
+        if isinstance(sites, torch.Tensor):
+            if sites.dtype != torch.float64 or sites.dim() != 2 or sites.shape[1] != 3:
+                raise ValueError("sites must be an (n, 3) float64 tensor")
+            cols = [sites[:, k].contiguous() for k in range(3)]
+            _device_inputs_ready(cols)  # the column copies are queued on torch's stream
+            on_dev = int(sites.is_cuda)
+            ptrs = [ctypes.cast(ctypes.c_void_p(c.data_ptr()), _pd) for c in cols]
+        else:
+            a = np.asarray(sites, dtype=np.float64)
+            if a.ndim != 2 or a.shape[1] != 3:
+                raise ValueError("sites must be an (n, 3) array")
+            cols = [np.ascontiguousarray(a[:, k]) for k in range(3)]
+            on_dev = 0
+            ptrs = [c.ctypes.data_as(_pd) for c in cols]
+        dom = (_d * 6)(*[float(v) for v in domain])
+        S = PointSet(len(cols[0]), *ptrs, on_dev)
+        T = Tessellation()
+        self._check(self.L.kiwi_hip_voronoi_tessellate(self.h, ctypes.byref(S), dom, ctypes.byref(T)))
+        V = T.cells
+        dev = torch.device("cuda", self.device)
+        nc, nf, nv = int(V.n_cells), int(V.n_faces), int(V.n_face_vertices)
+        cells = {"cell_face_ptr": torch.empty(nc + 1, dtype=torch.int64, device=dev),
+                 "face_vert_ptr": torch.empty(nf + 1, dtype=torch.int64, device=dev),
+                 "vert_xyz": torch.empty((nv, 3), dtype=torch.float64, device=dev),
+                 "face_nbr": torch.empty(nf, dtype=torch.int32, device=dev),
+                 "cell_site": torch.empty(nc, dtype=torch.int32, device=dev)}
+        ptr = {k: ctypes.c_void_p(t.data_ptr()) for k, t in cells.items()}
+        self._check(self.L.kiwi_hip_voronoi_tessellation_fetch(
+            self.h, ctypes.cast(ptr["cell_face_ptr"], _pll), ctypes.cast(ptr["face_vert_ptr"], _pll),
+            ctypes.cast(ptr["vert_xyz"], _pd), ctypes.cast(ptr["face_nbr"], _pi), ctypes.cast(ptr["cell_site"], _pi), 1))
+        cells["domain"] = [float(v) for v in domain]
+        info = {"n_dropped": int(T.n_dropped), "device_ms_prep": T.device_ms_prep,
+                "device_ms_count": T.device_ms_count, "device_ms_fill": T.device_ms_fill}
+        return cells, info
 
     def _map_inputs(self, parts, home, cells):
         P = _particles(parts)

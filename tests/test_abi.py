@@ -64,7 +64,9 @@ def test_struct_layouts_match_header(tmp_path):
               "kiwi_cartesian_map_stats": [f for f, _ in hip.CartesianMapStats._fields_],
               "kiwi_lattice": [f for f, _ in hip.Lattice._fields_],
               "kiwi_buffer_sites": [f for f, _ in hip.BufferSites._fields_],
-              "kiwi_coverage_stats": [f for f, _ in hip.CoverageStats._fields_]}
+              "kiwi_coverage_stats": [f for f, _ in hip.CoverageStats._fields_],
+              "kiwi_point_set": [f for f, _ in hip.PointSet._fields_],
+              "kiwi_tessellation": [f for f, _ in hip.Tessellation._fields_]}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for st, fs in fields.items():
         lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
