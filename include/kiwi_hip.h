@@ -416,6 +416,15 @@ typedef struct kiwi_tessellation {
 int kiwi_hip_voronoi_tessellate(kiwi_hip_ctx* ctx, const kiwi_point_set* sites, const double domain[6],
                                 kiwi_tessellation* out);
 
+/* Each point's home cell in the context's last tessellation, replacing voro++'s find_voronoi_cell and
+ * the reverse_id_list lookup of the map stage (src/mapping.cpp:434-440): the cell of its nearest site by
+ * squared distance (formed as voro++ forms it, each product and sum rounded on its own; of sites at
+ * exactly equal distance the lowest input index wins, where voro++ keeps the first its block search
+ * meets), or -1 for a point the container does not hold (outside the box, or on a max face by the
+ * rounding of its block index) or when there are no cells.  home_cell[n] where points->on_device says;
+ * unresolved (may be NULL): the count of -1s. */
+int kiwi_hip_voronoi_home_cells(kiwi_hip_ctx* ctx, const kiwi_point_set* points, int* home_cell, int64_t* unresolved);
+
 /* Copies the context's last tessellation into the caller's arrays (sizes from its kiwi_tessellation):
  * cell_face_ptr[n_cells + 1], face_vert_ptr[n_faces + 1], vert_xyz[3 n_face_vertices],
  * face_neighbor[n_faces], cell_site[n_cells]; device memory if on_device, else host memory.  The voro++

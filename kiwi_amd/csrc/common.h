@@ -114,6 +114,13 @@ struct PipeBufs {
 #undef KIWI_PB_MEMBER
 };
 
+// The last tessellation's search structures (kiwi_hip_voronoi_tessellate), for the home-cell lookup:
+// voro++'s block grid over the domain and the bucket grid of the kept sites.
+struct TessGrid {
+    double dom[6], lo[3], edge[3], tolb;
+    int nbk[3], nb[3];
+};
+
 struct kiwi_hip_ctx {
     int device = 0;
     int n_cu = 0;
@@ -140,6 +147,7 @@ struct kiwi_hip_ctx {
     // the Voronoi tessellation's buffers; [8] .. [11] hold the last tessellation (kiwi_hip_voronoi_tessellate)
     DevBuf tess[12];
     int64_t tess_sizes[3] = {-1, 0, 0};  // the last tessellation's cells, faces, face-vertex entries (-1: none)
+    TessGrid tess_grid{};
     // sph-to-grid HIP graphs (kiwi_hip_set_graphs): the captured pipelines, most recently used last; the
     // dependency-only events that fork the second stream in a capture; the graph-mode radix sort's
     // buffers; pinned host memory that the replayed pipeline copies its statistics to

@@ -14,6 +14,8 @@
 //                      box cut by the neighbours' bisector planes, shell by shell of buckets until the
 //                      farthest vertex rules out the rest; writes the cell's face and face-vertex counts
 //   6. scans           face and face-vertex offsets per cell
+//   (k_tv_home, kiwi_hip_voronoi_home_cells: points -> the cell of their nearest site, over the same
+//   bucket grid, after a tessellation)
 //   7. k_tv_cells<true>  the same cells written as face offsets, absolute vertex coordinates and
 //                      neighbour cell indices: read back from the workspace where every cell had a
 //                      slot of its own, computed again otherwise
@@ -157,6 +159,65 @@ __global__ __launch_bounds__(THREADS, 2) void k_tv_cells(int nc, int64_t infligh
             kiwi_hip::vc_write(cell, sx, sy, sz, site_cell, o.cfp[c], o.cvo[c], o.fvp, o.vxyz, o.fnbr);
         }
     }
+}
+
+// Each point's home cell: the cell of its nearest kept site (squared distance formed as voro++'s
+// find_voronoi_cell forms it, each product and sum rounded on its own; the lowest site index among exact
+// ties), over shells of buckets until the unscanned ones lie farther than the best distance.  A point the
+// container would not hold (outside the box, or on a max face by the rounding of its block index) has no
+// home cell (-1), as find_voronoi_cell returns false for it (container.cc:237-263).
+__global__ void k_tv_home(int64_t m, const double* __restrict__ px, const double* __restrict__ py,
+                          const double* __restrict__ pz, TvBox b, VcGrid g, const int* __restrict__ range,
+                          const VcSite* __restrict__ sites, const int* __restrict__ site_cell, int nc,
+                          int* __restrict__ home, unsigned long long* unresolved) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * THREADS + threadIdx.x;
+    if (i >= m) return;
+    const double q[3] = {px[i], py[i], pz[i]};
+    if (nc == 0 || kiwi_hip::vc_block_index(b.d, b.nbk, q[0], q[1], q[2]) < 0) {
+        home[i] = -1;
+        atomicAdd(unresolved, 1ull);
+        return;
+    }
+    const int bq[3] = {kiwi_hip::vc_bucket_axis(g, 0, q[0]), kiwi_hip::vc_bucket_axis(g, 1, q[1]),
+                       kiwi_hip::vc_bucket_axis(g, 2, q[2])};
+    double best = INFINITY;
+    int bi = 0x7fffffff;
+    for (int r = 0;; ++r) {
+        if (r > 0) {  // every unscanned site lies beyond one face of the box of radius r - 1
+            double dmin = INFINITY;
+            for (int a = 0; a < 3; ++a) {
+                if (bq[a] - r >= 0) dmin = fmin(dmin, q[a] - (g.lo[a] + (bq[a] - r + 1) * g.edge[a]));
+                if (bq[a] + r < g.nb[a]) dmin = fmin(dmin, g.lo[a] + (bq[a] + r) * g.edge[a] - q[a]);
+            }
+            if (dmin == INFINITY) break;
+            dmin -= g.tolb;
+            if (dmin > 0.0 && dmin * dmin > best) break;
+        }
+        for (int dk = -r; dk <= r; ++dk) {
+            const int k = bq[2] + dk;
+            if (k < 0 || k >= g.nb[2]) continue;
+            for (int dj = -r; dj <= r; ++dj) {
+                const int j = bq[1] + dj;
+                if (j < 0 || j >= g.nb[1]) continue;
+                const bool full = dk == -r || dk == r || dj == -r || dj == r;
+                for (int di = -r; di <= r; di += full ? 1 : 2 * r) {
+                    const int ii = bq[0] + di;
+                    if (ii < 0 || ii >= g.nb[0]) continue;
+                    const int64_t bk = (static_cast<int64_t>(ii) * g.nb[1] + j) * g.nb[2] + k;
+                    const int t1 = range[2 * bk + 1];
+                    for (int t = range[2 * bk]; t < t1; ++t) {
+                        const VcSite s = sites[t];
+                        const double dx = sub_sep(s.x, q[0]), dy = sub_sep(s.y, q[1]), dz = sub_sep(s.z, q[2]);
+                        const double rs = add_sep(add_sep(mul_sep(dx, dx), mul_sep(dy, dy)), mul_sep(dz, dz));
+                        const bool take = rs < best || (rs == best && s.idx < bi);
+                        best = take ? rs : best;
+                        bi = take ? s.idx : bi;
+                    }
+                }
+            }
+        }
+    }
+    home[i] = site_cell[bi];
 }
 
 int bits_for(uint64_t v) {  // bits that hold 0..v
@@ -339,6 +400,15 @@ int kiwi_hip_voronoi_tessellate(kiwi_hip_ctx* c, const kiwi_point_set* pts, cons
     c->tess_sizes[0] = nc;
     c->tess_sizes[1] = tot[0];
     c->tess_sizes[2] = tot[1];
+    TessGrid& tg = c->tess_grid;
+    for (int a = 0; a < 6; ++a) tg.dom[a] = domain[a];
+    for (int a = 0; a < 3; ++a) {
+        tg.lo[a] = g.lo[a];
+        tg.edge[a] = g.edge[a];
+        tg.nb[a] = g.nb[a];
+        tg.nbk[a] = box.nbk[a];
+    }
+    tg.tolb = g.tolb;
     kiwi_voronoi_cells& V = out->cells;
     V.n_cells = nc;
     V.n_faces = tot[0];
@@ -358,6 +428,64 @@ int kiwi_hip_voronoi_tessellate(kiwi_hip_ctx* c, const kiwi_point_set* pts, cons
     out->device_ms_prep = ms[0];
     out->device_ms_count = ms[1];
     out->device_ms_fill = ms[2];
+    return KIWI_OK;
+}
+
+int kiwi_hip_voronoi_home_cells(kiwi_hip_ctx* c, const kiwi_point_set* pts, int* home_cell, int64_t* unresolved) {
+    if (!c) return KIWI_ERR_INVALID_ARG;
+    c->err.clear();
+    if (!pts || (!home_cell && pts->n > 0)) return fail(c, KIWI_ERR_INVALID_ARG, "null argument");
+    if (c->tess_sizes[0] < 0) return fail(c, KIWI_ERR_INVALID_ARG, "no tessellation on this context");
+    if (pts->n < 0) return fail(c, KIWI_ERR_INVALID_ARG, "negative point count");
+    const int64_t m = pts->n;
+    if (m > 0 && (!pts->x || !pts->y || !pts->z)) return fail(c, KIWI_ERR_INVALID_ARG, "null point array");
+    KIWI_HIP_CHECK(c, hipSetDevice(c->device));
+    if (pts->on_device && m > 0 &&
+        !(gpu_accessible(pts->x) && gpu_accessible(pts->y) && gpu_accessible(pts->z) && gpu_accessible(home_cell)))
+        return fail(c, KIWI_ERR_INVALID_ARG, "points->on_device is set but an array is host memory");
+    hipStream_t s = c->stream;
+    DevBuf* B = c->tess;
+    const double* dx = pts->x;
+    const double* dy = pts->y;
+    const double* dz = pts->z;
+    int* dh = home_cell;
+    if (!pts->on_device && m > 0) {
+        KIWI_HIP_CHECK(c, reserve(B[0], (3 * sizeof(double) + sizeof(int)) * m));
+        double* d = static_cast<double*>(B[0].p);
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(d, pts->x, m * sizeof(double), hipMemcpyHostToDevice, s));
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(d + m, pts->y, m * sizeof(double), hipMemcpyHostToDevice, s));
+        KIWI_HIP_CHECK(c, hipMemcpyAsync(d + 2 * m, pts->z, m * sizeof(double), hipMemcpyHostToDevice, s));
+        dx = d;
+        dy = d + m;
+        dz = d + 2 * m;
+        dh = reinterpret_cast<int*>(d + 3 * m);
+    }
+    const TessGrid& tg = c->tess_grid;
+    TvBox box;
+    VcGrid g;
+    for (int a = 0; a < 6; ++a) box.d[a] = tg.dom[a];
+    for (int a = 0; a < 3; ++a) {
+        box.nbk[a] = tg.nbk[a];
+        g.lo[a] = tg.lo[a];
+        g.edge[a] = tg.edge[a];
+        g.nb[a] = tg.nb[a];
+    }
+    g.tolb = tg.tolb;
+    KIWI_HIP_CHECK(c, reserve(B[1], 6 * sizeof(unsigned long long)));
+    unsigned long long* dun = static_cast<unsigned long long*>(B[1].p) + 5;
+    KIWI_HIP_CHECK(c, hipMemsetAsync(dun, 0, sizeof(unsigned long long), s));
+    if (m > 0) {
+        k_tv_home<<<grid1d(m), THREADS, 0, s>>>(m, dx, dy, dz, box, g, static_cast<const int*>(B[5].p),
+                                                 static_cast<const VcSite*>(B[4].p), static_cast<const int*>(B[3].p),
+                                                 static_cast<int>(c->tess_sizes[0]), dh, dun);
+        KIWI_HIP_CHECK(c, hipGetLastError());
+        if (!pts->on_device)
+            KIWI_HIP_CHECK(c, hipMemcpyAsync(home_cell, dh, m * sizeof(int), hipMemcpyDeviceToHost, s));
+    }
+    unsigned long long hun = 0;
+    KIWI_HIP_CHECK(c, hipMemcpyAsync(&hun, dun, sizeof(hun), hipMemcpyDeviceToHost, s));
+    KIWI_HIP_CHECK(c, hipStreamSynchronize(s));
+    if (unresolved) *unresolved = static_cast<int64_t>(hun);
     return KIWI_OK;
 }
 

@@ -37,7 +37,8 @@ EXPORTED = ["kiwi_hip_abi_version", "kiwi_hip_create", "kiwi_hip_destroy", "kiwi
             "kiwi_hip_inverse_mapping", "kiwi_hip_cartesian_map", "kiwi_hip_cartesian_lattice",
             "kiwi_hip_coverage_buffer", "kiwi_hip_voronoi_centroids", "kiwi_hip_sph_to_grid_noalloc",
             "kiwi_hip_set_fp64_fallback", "kiwi_hip_get_fp64_fallback", "kiwi_hip_set_graphs",
-            "kiwi_hip_get_graphs", "kiwi_hip_voronoi_tessellate", "kiwi_hip_voronoi_tessellation_fetch"]
+            "kiwi_hip_get_graphs", "kiwi_hip_voronoi_tessellate", "kiwi_hip_voronoi_tessellation_fetch",
+            "kiwi_hip_voronoi_home_cells"]
 
 
 class KiwiError(RuntimeError):
@@ -237,6 +238,9 @@ def load():
         L.kiwi_hip_voronoi_tessellate.restype = _i
         L.kiwi_hip_voronoi_tessellate.argtypes = [ctypes.c_void_p, ctypes.POINTER(PointSet), ctypes.POINTER(_d),
                                                   ctypes.POINTER(Tessellation)]
+        L.kiwi_hip_voronoi_home_cells.restype = _i
+        L.kiwi_hip_voronoi_home_cells.argtypes = [ctypes.c_void_p, ctypes.POINTER(PointSet), _pi,
+                                                  ctypes.POINTER(_i64)]
         L.kiwi_hip_voronoi_tessellation_fetch.restype = _i
         L.kiwi_hip_voronoi_tessellation_fetch.argtypes = [ctypes.c_void_p, _pll, _pll, _pd, _pi, _pi, _i]
         L.kiwi_hip_voronoi_map.restype = _i
@@ -663,6 +667,34 @@ class Context:
         info = {"n_dropped": int(T.n_dropped), "device_ms_prep": T.device_ms_prep,
                 "device_ms_count": T.device_ms_count, "device_ms_fill": T.device_ms_fill}
         return cells, info
+
+    def voronoi_home_cells(self, points):
+        """Each point's home cell in the context's last tessellation (kiwi_hip_voronoi_home_cells): int32,
+        -1 where the container would not hold the point.  points: (m, 3) float64 host array or device
+        tensor; the result lives there too.  Returns (home, unresolved)."""
+        import torch
+        if isinstance(points, torch.Tensor):
+            if points.dtype != torch.float64 or points.dim() != 2 or points.shape[1] != 3:
+                raise ValueError("points must be an (m, 3) float64 tensor")
+            cols = [points[:, k].contiguous() for k in range(3)]
+            home = torch.empty(points.shape[0], dtype=torch.int32, device=points.device)
+            _device_inputs_ready(cols, home)
+            ptrs = [ctypes.cast(ctypes.c_void_p(c.data_ptr()), _pd) for c in cols]
+            hp = ctypes.cast(ctypes.c_void_p(home.data_ptr()), _pi)
+            on_dev = int(points.is_cuda)
+        else:
+            a = np.asarray(points, dtype=np.float64)
+            if a.ndim != 2 or a.shape[1] != 3:
+                raise ValueError("points must be an (m, 3) array")
+            cols = [np.ascontiguousarray(a[:, k]) for k in range(3)]
+            home = np.zeros(len(a), dtype=np.int32)
+            ptrs = [c.ctypes.data_as(_pd) for c in cols]
+            hp = home.ctypes.data_as(_pi)
+            on_dev = 0
+        un = ctypes.c_int64(0)
+        self._check(self.L.kiwi_hip_voronoi_home_cells(self.h, ctypes.byref(PointSet(len(cols[0]), *ptrs, on_dev)),
+                                                       hp, ctypes.byref(un)))
+        return home, int(un.value)
 
     def _map_inputs(self, parts, home, cells):
         P = _particles(parts)

@@ -90,13 +90,24 @@ def main():
         torch.cuda.synchronize()
         step.append(time.perf_counter() - a)
     ms1, ms_step = 1e3 * float(np.median(one)), 1e3 * float(np.median(step))
+    # home cells of the sites' own positions in the final tessellation (the map's particles, for `map`)
+    ctx.voronoi_home_cells(dsites)
+    torch.cuda.synchronize()
+    th = []
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        home, unres = ctx.voronoi_home_cells(dsites)
+        torch.cuda.synchronize()
+        th.append(time.perf_counter() - a)
+    ms_home = 1e3 * float(np.median(th))
     nc, nf = len(c0["cell_site"]), len(c0["face_nbr"])
     log(f"GPU: tessellation {ms1:.1f} ms (device: order/buckets {i0['device_ms_prep']:.2f}, count "
         f"{i0['device_ms_count']:.2f}, fill {i0['device_ms_fill']:.2f}); {nc} cells, {nf} faces, "
-        f"{len(c0['vert_xyz'])} face vertices; Lloyd step + final tessellation {ms_step:.1f} ms")
+        f"{len(c0['vert_xyz'])} face vertices; Lloyd step + final tessellation {ms_step:.1f} ms; home cells of "
+        f"{len(sites)} points {ms_home:.2f} ms")
     res = {"workload": label + ", then one Lloyd step and the final tessellation",
            "cells": nc, "faces": nf, "face_vertices": int(len(c0["vert_xyz"])),
-           "gpu_ms_tessellation": ms1, "gpu_ms_lloyd_step_and_final": ms_step,
+           "gpu_ms_tessellation": ms1, "gpu_ms_lloyd_step_and_final": ms_step, "gpu_ms_home_cells": ms_home,
            "device_ms": {"prep": i0["device_ms_prep"], "count": i0["device_ms_count"], "fill": i0["device_ms_fill"]},
            "cells_per_s": nc / (ms1 * 1e-3)}
     if not args.no_cpu:

@@ -8,8 +8,9 @@ reference's vendored copy (oracle/_ref/libkiwi_voro.so), on the container the re
   - volumes and centroids (kiwi_hip_voronoi_centroids over the GPU geometry) within 1e-12 of voro++'s
     (extent^3 and extent), and the volumes summing to the box;
   - the lattice's exact cubes (tools/test_lattice_grid.cpp:95-198);
-  - the Voronoi map over the GPU geometry equal to the map over voro++'s (1e-12 of the field maximum,
-    the same visits).
+  - the Voronoi map over the GPU geometry equal to the map over voro++'s (1e-12 / 1e-11 of the field
+    maximum, the same visits), also with the home cells from kiwi_hip_voronoi_home_cells, which equal
+    voro++'s find_voronoi_cell exactly.
 """
 import os
 import sys
@@ -214,3 +215,63 @@ def test_centroids_of_device_cells_without_outputs(ctx):
     ctx.voronoi_centroids(cells, site, out=out)
     assert torch.equal(cen, out[0]) and torch.equal(vol, out[1])
     assert abs(float(vol.sum()) - 1.0) <= 1e-12
+
+
+def test_home_cells_match_voro(ctx):
+    """kiwi_hip_voronoi_home_cells against voro++'s find_voronoi_cell resolved to cell indices (the map
+    stage's home cells, src/mapping.cpp:434-440): points inside, outside, on the box's faces and on
+    sites; host and device inputs give the same."""
+    say("case: home cells, 1200 clustered sites, 3000 points")
+    import torch
+    O = oracle()
+    rng = np.random.default_rng(13)
+    dom = np.array([-1.0, 2.0, 0.0, 1.0, 0.0, 1.5])
+    lo, hi = dom[0::2], dom[1::2]
+    sites = np.clip(rng.normal(loc=[0.5, 0.5, 0.7], scale=0.3, size=(1200, 3)), lo, hi)
+    pts = rng.uniform(lo, hi, size=(3000, 3))
+    pts[:100] = hi + rng.uniform(0.01, 0.5, size=(100, 3))  # outside
+    pts[100:150, 0] = hi[0]  # on the x max face: held or not by the rounding of voro++'s block index
+    pts[150:200, 1] = lo[1]  # on the y min face
+    pts[200:250] = sites[rng.integers(1200, size=50)]  # on sites
+    r = O.tessellate(np.arange(len(sites)), sites, dom, pts)
+    ctx.voronoi_tessellate(torch.from_numpy(sites).cuda(), dom)
+    hd, und = ctx.voronoi_home_cells(torch.from_numpy(pts).cuda())
+    hh, unh = ctx.voronoi_home_cells(pts)
+    np.testing.assert_array_equal(hd.cpu().numpy(), r["home"])
+    np.testing.assert_array_equal(hh, r["home"])
+    assert und == unh == int(np.sum(r["home"] < 0)) >= 100
+
+
+def test_map_on_gpu_tessellation_and_home_cells(ctx):
+    """The map stage with no voro++: tessellation, home cells and map on the GPU, against the map over
+    voro++'s geometry and home cells: the same visits, sums within 1e-11 of each field's maximum (the
+    vertices agree to rounding and a face's cycle may start at another vertex, which rounds the face
+    integrals differently; measured 1.1e-12 on the momentum, whose signed sums cancel)."""
+    say("case: GPU tessellation + home cells + map vs voro++ inputs, 2000 sites, 1500 particles")
+    import torch
+    O = oracle()
+    rng = np.random.default_rng(14)
+    dom = np.array([0.0, 2.0, 0.0, 2.0, 0.0, 2.0])
+    sites = rng.uniform(0, 2, size=(2000, 3))
+    pts = rng.uniform(0, 2, size=(1500, 3))
+    parts = {"x": pts[:, 0].copy(), "y": pts[:, 1].copy(), "z": pts[:, 2].copy(), "vx": rng.normal(size=1500),
+             "vy": rng.normal(size=1500), "vz": rng.normal(size=1500), "m": rng.uniform(0.5, 1.5, 1500),
+             "h": 0.2 * rng.uniform(0.5, 2.0, 1500)}
+    parts = {k: np.ascontiguousarray(v) for k, v in parts.items()}
+    geo = O.tessellate(np.arange(len(sites)), sites, dom, pts)
+    vcells = {k: np.ascontiguousarray(geo[k]) for k in ("cell_face_ptr", "face_vert_ptr", "face_nbr")}
+    vcells["vert_xyz"] = np.ascontiguousarray(geo["vert_xyz"].reshape(-1))
+    vcells["domain"] = dom
+    ref = ctx.voronoi_map(parts, geo["home"].astype(np.int32), vcells)
+    dev = torch.device("cuda:0")
+    gcells, _ = ctx.voronoi_tessellate(torch.from_numpy(sites).to(dev), dom)
+    dparts = {k: torch.from_numpy(v).to(dev) for k, v in parts.items()}
+    home, un = ctx.voronoi_home_cells(torch.from_numpy(pts).to(dev))
+    assert un == 0
+    nc = len(gcells["cell_site"])
+    out = tuple(torch.zeros(nc, dtype=torch.float64, device=dev) for _ in range(4))
+    got = ctx.voronoi_map(dparts, home, gcells, out=out)
+    for a, b, name in zip(got[:4], ref[:4], ("mass", "px", "py", "pz")):
+        a = a.cpu().numpy()
+        assert np.max(np.abs(a - b)) <= 1e-11 * np.max(np.abs(b)), name
+    assert (got[4].pairs_visited, got[4].face_evals) == (ref[4].pairs_visited, ref[4].face_evals)
