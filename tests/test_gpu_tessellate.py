@@ -275,3 +275,25 @@ def test_map_on_gpu_tessellation_and_home_cells(ctx):
         a = a.cpu().numpy()
         assert np.max(np.abs(a - b)) <= 1e-11 * np.max(np.abs(b)), name
     assert (got[4].pairs_visited, got[4].face_evals) == (ref[4].pairs_visited, ref[4].face_evals)
+
+
+def test_map_sites_2e5(ctx):
+    """The Voronoi map benchmark's sites at full size: 2e5 particles of the log-normal field (256^3 box,
+    sigma_ln_rho 1.5; kiwi_amd.workloads.lognormal_field, seed 2), tessellated: the same cells, order,
+    faces and neighbours as voro++ (faces below 1e-12 of the extent squared aside), volumes and
+    centroids within voro++'s tolerance bounds, the volumes summing to the box, and the particles' home
+    cells equal to voro++'s."""
+    say("case: 2e5 log-normal map sites, 256^3 box")
+    import torch
+    from kiwi_amd import workloads
+    O = oracle()
+    parts, dom, _ = workloads.lognormal_field(n_particles=200_000, n=256, seed=2)
+    pts = np.ascontiguousarray(np.stack([parts["x"], parts["y"], parts["z"]], 1).astype(np.float64))
+    dom = [float(v) for v in dom]
+    _, g, r = check_case(ctx, pts, dom, cycles=False, min_area=1e-12, tol_bounds=True)
+    assert len(g["cell_site"]) == 200_000
+    rng = np.random.default_rng(15)
+    q = rng.uniform(np.asarray(dom[0::2]), np.asarray(dom[1::2]), size=(20000, 3))
+    rh = O.tessellate(np.arange(len(pts)), pts, np.asarray(dom), q)["home"]
+    home, _ = ctx.voronoi_home_cells(torch.from_numpy(q).cuda())
+    np.testing.assert_array_equal(home.cpu().numpy(), rh)
