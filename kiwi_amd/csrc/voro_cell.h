@@ -171,20 +171,22 @@ KIWI_VHD int vc_edge_vertex(VCell& c, int& nx, int a, int b) {
     return v;
 }
 
+// Whether the bisector plane of the neighbour at relative position (px, py, pz) leaves the cell as it is
+// (no vertex outside it): vc_cut's first test, without storing anything.
+KIWI_VHD bool vc_misses(const VCell& c, double px, double py, double pz, double tol) {
+    const double rsq = px * px + py * py + pz * pz;
+    for (int v = 0; v < c.nv; ++v)
+        if (2.0 * (c.X(v) * px + c.Y(v) * py + c.Z(v) * pz) - rsq > tol) return false;
+    return true;
+}
+
 // Cuts the cell by the bisector plane of the neighbour at relative position (px, py, pz), index nb.
 KIWI_VHD int vc_cut(VCell& c, double px, double py, double pz, int nb, double tol) {
     const double rsq = px * px + py * py + pz * pz;
     const int nv0 = c.nv;
-    // classify first without storing anything: most planes tested miss the cell
+    // (callers test vc_misses first: most planes tested miss the cell, and that test stores nothing)
     int nout = 0, nin = 0;
     for (int v = 0; v < nv0; ++v) {
-        const double u = 2.0 * (c.X(v) * px + c.Y(v) * py + c.Z(v) * pz) - rsq;
-        nout += u > tol;
-        nin += u < -tol;
-    }
-    if (nout == 0) return VC_UNCHANGED;
-    nout = nin = 0;
-    for (int v = 0; v < nv0; ++v) {  // again, kept for the cut (its counts decide, whatever the compiler did)
         const double u = 2.0 * (c.X(v) * px + c.Y(v) * py + c.Z(v) * pz) - rsq;
         c.S(v) = u;
         const int8_t sd = u > tol ? 1 : (u < -tol ? -1 : 0);
@@ -384,54 +386,92 @@ KIWI_VHD int64_t vc_block_index(const double dom[6], const int nbk[3], double x,
     return ijk;
 }
 
+// A lane's place in its walk over the bucket shells: shell r, bucket offset (dk, dj, di) within it
+// (dk slowest; inside the shell's interior slabs only di = -r and r), and the sites [t, t1) of the
+// current bucket.
+struct VcWalk {
+    int r, dk, dj, di, t, t1;
+    bool fresh;  // (dk, dj, di) is the next bucket to visit, not the last one visited
+};
+
+// Moves the walk to its next bucket inside the grid and loads its site range; false when the walk
+// ends: every bucket scanned, or every unscanned bucket farther than twice the farthest vertex (every
+// site of a bucket at Chebyshev radius >= r lies beyond one face of the box of radius r - 1 about the
+// site's own bucket b).
+KIWI_VHD bool vc_next_bucket(VcWalk& w, const VcGrid& g, const int b[3], const double p[3], double mr2,
+                             const int* range) {
+    for (;;) {
+        if (w.fresh) {
+            w.fresh = false;
+        } else {
+            const bool full = w.dk == -w.r || w.dk == w.r || w.dj == -w.r || w.dj == w.r;
+            w.di += full ? 1 : 2 * w.r;
+            if (w.di > w.r) {
+                w.di = -w.r;
+                if (++w.dj > w.r) {
+                    w.dj = -w.r;
+                    if (++w.dk > w.r) {  // the shell is done: the next one, if it can matter
+                        const int r = ++w.r;
+                        double dmin = INFINITY;
+                        for (int a = 0; a < 3; ++a) {
+                            if (b[a] - r >= 0) dmin = fmin(dmin, p[a] - (g.lo[a] + (b[a] - r + 1) * g.edge[a]));
+                            if (b[a] + r < g.nb[a]) dmin = fmin(dmin, g.lo[a] + (b[a] + r) * g.edge[a] - p[a]);
+                        }
+                        if (dmin == INFINITY) return false;  // every bucket scanned
+                        dmin -= g.tolb;
+                        if (dmin > 0.0 && dmin * dmin > 4.0 * mr2) return false;
+                        w.dk = w.dj = w.di = -r;
+                    }
+                }
+            }
+        }
+        const int i = b[0] + w.di, j = b[1] + w.dj, k = b[2] + w.dk;
+        if (i < 0 || i >= g.nb[0] || j < 0 || j >= g.nb[1] || k < 0 || k >= g.nb[2]) continue;
+        const int64_t bk = (static_cast<int64_t>(i) * g.nb[1] + j) * g.nb[2] + k;
+        w.t = range[2 * bk];
+        w.t1 = range[2 * bk + 1];
+        return true;
+    }
+}
+
 // Computes the cell of site `self` at (sx, sy, sz): the domain box cut by every kept site (site_cell >= 0)
 // closer than twice the cell's farthest vertex, scanned in shells of buckets of growing Chebyshev radius
 // until the unscanned buckets are all farther.  range[2 b], range[2 b + 1]: bucket b's sites in `sites`.
-// Returns VC_CUT, or VC_EMPTY / VC_OVERFLOW.
+// Each round first advances to the lane's next site whose plane cuts the cell, then cuts: on a wavefront,
+// whose lanes walk different buckets, the cuts of all lanes then run together instead of one lane's at
+// a time.  Returns VC_CUT, or VC_EMPTY / VC_OVERFLOW.
 KIWI_VHD int vc_compute(VCell& c, const double dom[6], const VcGrid& g, const int* range, const VcSite* sites,
                         const int* site_cell, int self, double sx, double sy, double sz, double tol) {
     vc_init_box(c, dom, sx, sy, sz);
     double mr2 = vc_max_r2(c);
     const double p[3] = {sx, sy, sz};
     const int b[3] = {vc_bucket_axis(g, 0, sx), vc_bucket_axis(g, 1, sy), vc_bucket_axis(g, 2, sz)};
-    for (int r = 0;; ++r) {
-        if (r > 0) {
-            // every site of a bucket at Chebyshev radius >= r lies beyond one face of the box of radius r - 1
-            double dmin = INFINITY;
-            for (int a = 0; a < 3; ++a) {
-                if (b[a] - r >= 0) dmin = fmin(dmin, p[a] - (g.lo[a] + (b[a] - r + 1) * g.edge[a]));
-                if (b[a] + r < g.nb[a]) dmin = fmin(dmin, g.lo[a] + (b[a] + r) * g.edge[a] - p[a]);
+    VcWalk w{0, 0, 0, 0, 0, 0, true};
+    for (;;) {
+        bool found = false;
+        double px = 0.0, py = 0.0, pz = 0.0;
+        int nb = 0;
+        for (;;) {
+            if (w.t < w.t1) {
+                const VcSite q = sites[w.t++];
+                if (q.idx == self || site_cell[q.idx] < 0) continue;
+                px = q.x - sx;
+                py = q.y - sy;
+                pz = q.z - sz;
+                if (px * px + py * py + pz * pz > 4.0 * mr2) continue;  // its plane misses the cell
+                if (vc_misses(c, px, py, pz, tol)) continue;
+                nb = q.idx;
+                found = true;
+                break;
             }
-            if (dmin == INFINITY) break;  // every bucket scanned
-            dmin -= g.tolb;
-            if (dmin > 0.0 && dmin * dmin > 4.0 * mr2) break;
+            if (!vc_next_bucket(w, g, b, p, mr2, range)) break;
         }
-        for (int dk = -r; dk <= r; ++dk) {
-            const int k = b[2] + dk;
-            if (k < 0 || k >= g.nb[2]) continue;
-            for (int dj = -r; dj <= r; ++dj) {
-                const int j = b[1] + dj;
-                if (j < 0 || j >= g.nb[1]) continue;
-                const bool full = dk == -r || dk == r || dj == -r || dj == r;
-                for (int di = -r; di <= r; di += full ? 1 : 2 * r) {
-                    const int i = b[0] + di;
-                    if (i < 0 || i >= g.nb[0]) continue;
-                    const int64_t bk = (static_cast<int64_t>(i) * g.nb[1] + j) * g.nb[2] + k;
-                    const int t1 = range[2 * bk + 1];
-                    for (int t = range[2 * bk]; t < t1; ++t) {
-                        const VcSite q = sites[t];
-                        if (q.idx == self || site_cell[q.idx] < 0) continue;
-                        const double px = q.x - sx, py = q.y - sy, pz = q.z - sz;
-                        if (px * px + py * py + pz * pz > 4.0 * mr2) continue;  // its plane misses the cell
-                        const int rc = vc_cut(c, px, py, pz, q.idx, tol);
-                        if (rc == VC_CUT)
-                            mr2 = vc_max_r2(c);
-                        else if (rc != VC_UNCHANGED)
-                            return rc;
-                    }
-                }
-            }
-        }
+        if (!found) break;
+        const int rc = vc_cut(c, px, py, pz, nb, tol);
+        if (rc == VC_CUT)
+            mr2 = vc_max_r2(c);
+        else if (rc != VC_UNCHANGED)
+            return rc;
     }
     return VC_CUT;
 }
