@@ -1,6 +1,8 @@
 // Host build of the GPU tessellation's per-cell code (kiwi_amd/csrc/voro_cell.h) for the CPU tests: the
-// same cell, bucket-grid and block-order functions the device kernels call, driven by a plain loop, so
-// that the geometry can be checked against voro++ without a GPU.  Test infrastructure, built by
+// same cell, bucket-grid and block-order functions the device kernels call, with the device's workspace
+// layout (every cell in its own slot of one strided workspace: computed and saved, then reloaded and
+// written out), driven by a plain loop, so that the geometry can be checked against voro++ without a
+// GPU.  Test infrastructure, built by
 // tests/test_voro_cell_host.py with g++ into a temporary directory.
 #include <algorithm>
 #include <cstdint>
@@ -67,15 +69,22 @@ int vc_host_tessellate(int n, const double* x, const double* y, const double* z,
         if (t == 0 || bkey[bo[t - 1]] != b) range[2 * b] = t;
         if (t == nc - 1 || bkey[bo[t + 1]] != b) range[2 * b + 1] = t + 1;
     }
-    std::vector<char> store(VC_BYTES + 64);
+    // the device layout: every cell in its own slot of one strided workspace, computed, saved, reloaded
+    std::vector<char> store(static_cast<size_t>(VC_BYTES) * (nc > 0 ? nc : 1) + 64);
     VCell cell;
-    vc_bind(cell, store.data(), 1, 0);
     int status = 0;
     for (int c = 0; c < nc; ++c) {
         const int i = G->cell_site[c];
+        vc_bind(cell, store.data(), nc, c);
         const int rc = vc_compute(cell, dom, g, range.data(), sites.data(), site_cell.data(), i, x[i], y[i], z[i],
                                   VC_TOL_DEFAULT);
         if (rc != VC_CUT && !status) status = rc;
+        vc_save(cell);
+    }
+    for (int c = 0; c < nc; ++c) {
+        const int i = G->cell_site[c];
+        vc_bind(cell, store.data(), nc, c);
+        vc_load(cell);
         const int64_t f0 = G->cfp.back(), v0 = G->fvp.back();
         const int nfv = vc_face_vertices(cell);
         G->fvp.resize(f0 + cell.nf + 1);
