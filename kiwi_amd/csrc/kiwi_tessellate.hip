@@ -122,10 +122,10 @@ struct TvOut {
     unsigned long long* bad;  // cells whose computation failed ([0] empty, [1] past the per-cell limits)
 };
 
-// two waves a SIMD (256 registers a lane, a few spilled): 1.4x the throughput of one (2e5 log-normal sites:
-// count pass 580 -> 419 ms)
+// three waves a SIMD (168 registers a lane, 420 bytes spilled): 2e5 log-normal sites 184 -> 136 ms against
+// two waves, the cells bit-identical (one wave: 580 ms before the lockstep walk, two: 419)
 template <bool FILL>
-__global__ __launch_bounds__(THREADS, 2) void k_tv_cells(int nc, int64_t inflight, const int* __restrict__ cell_site,
+__global__ __launch_bounds__(THREADS, 3) void k_tv_cells(int nc, int64_t inflight, const int* __restrict__ cell_site,
                                                       const int* __restrict__ site_cell, const double* __restrict__ x,
                                                       const double* __restrict__ y, const double* __restrict__ z,
                                                       TvBox b, VcGrid g, const int* __restrict__ range,
