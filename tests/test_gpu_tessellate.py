@@ -297,3 +297,13 @@ def test_map_sites_2e5(ctx):
     rh = O.tessellate(np.arange(len(pts)), pts, np.asarray(dom), q)["home"]
     home, _ = ctx.voronoi_home_cells(torch.from_numpy(q).cuda())
     np.testing.assert_array_equal(home.cpu().numpy(), rh)
+
+
+def test_more_cells_than_workspace_slots(ctx):
+    """3e5 sites: more cells than the 262,144 workspace slots, so the fill pass computes the cells again
+    instead of reading the count pass's; the same cells, faces and neighbours as voro++ and volumes
+    summing to the box."""
+    say("case: 3e5 uniform sites (fill pass recomputes)")
+    pts = np.random.default_rng(16).random((300_000, 3)) * [3.0, 2.0, 1.0]
+    _, g, _ = check_case(ctx, pts, [0, 3, 0, 2, 0, 1], cycles=False, min_area=1e-12, tol_bounds=True)
+    assert len(g["cell_site"]) == 300_000
