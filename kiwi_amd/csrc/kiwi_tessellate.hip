@@ -142,7 +142,7 @@ __global__ __launch_bounds__(THREADS, 3) void k_tv_cells(int nc, int64_t infligh
         if (FILL && kept)
             kiwi_hip::vc_load(cell);
         else
-            rc = kiwi_hip::vc_compute(cell, b.d, g, range, sites, site_cell, i, sx, sy, sz, tol);
+            rc = kiwi_hip::vc_compute(cell, b.d, g, range, sites, i, sx, sy, sz, tol);
         if (rc != kiwi_hip::VC_CUT) {
             if (!FILL) {
                 atomicAdd(&o.bad[rc == kiwi_hip::VC_EMPTY ? 0 : 1], 1ull);

@@ -386,6 +386,10 @@ KIWI_VHD int64_t vc_block_index(const double dom[6], const int nbk[3], double x,
     return ijk;
 }
 
+struct alignas(8) int2v {
+    int x, y;
+};
+
 // A lane's place in its walk over the bucket shells: shell r, bucket offset (dk, dj, di) within it
 // (dk slowest; inside the shell's interior slabs only di = -r and r), and the sites [t, t1) of the
 // current bucket.
@@ -428,20 +432,22 @@ KIWI_VHD bool vc_next_bucket(VcWalk& w, const VcGrid& g, const int b[3], const d
         const int i = b[0] + w.di, j = b[1] + w.dj, k = b[2] + w.dk;
         if (i < 0 || i >= g.nb[0] || j < 0 || j >= g.nb[1] || k < 0 || k >= g.nb[2]) continue;
         const int64_t bk = (static_cast<int64_t>(i) * g.nb[1] + j) * g.nb[2] + k;
-        w.t = range[2 * bk];
-        w.t1 = range[2 * bk + 1];
+        const int2v rr = reinterpret_cast<const int2v*>(range)[bk];  // [beg, end) in one load
+        w.t = rr.x;
+        w.t1 = rr.y;
         return true;
     }
 }
 
-// Computes the cell of site `self` at (sx, sy, sz): the domain box cut by every kept site (site_cell >= 0)
-// closer than twice the cell's farthest vertex, scanned in shells of buckets of growing Chebyshev radius
-// until the unscanned buckets are all farther.  range[2 b], range[2 b + 1]: bucket b's sites in `sites`.
+// Computes the cell of site `self` at (sx, sy, sz): the domain box cut by every site of the buckets (the
+// kept sites) closer than twice the cell's farthest vertex, scanned in shells of buckets of growing
+// Chebyshev radius until the unscanned buckets are all farther.  range[2 b], range[2 b + 1] (8-byte
+// aligned): bucket b's sites in `sites`.
 // Each round first advances to the lane's next site whose plane cuts the cell, then cuts: on a wavefront,
 // whose lanes walk different buckets, the cuts of all lanes then run together instead of one lane's at
 // a time.  Returns VC_CUT, or VC_EMPTY / VC_OVERFLOW.
 KIWI_VHD int vc_compute(VCell& c, const double dom[6], const VcGrid& g, const int* range, const VcSite* sites,
-                        const int* site_cell, int self, double sx, double sy, double sz, double tol) {
+                        int self, double sx, double sy, double sz, double tol) {
     vc_init_box(c, dom, sx, sy, sz);
     double mr2 = vc_max_r2(c);
     const double p[3] = {sx, sy, sz};
@@ -454,7 +460,7 @@ KIWI_VHD int vc_compute(VCell& c, const double dom[6], const VcGrid& g, const in
         for (;;) {
             if (w.t < w.t1) {
                 const VcSite q = sites[w.t++];
-                if (q.idx == self || site_cell[q.idx] < 0) continue;
+                if (q.idx == self) continue;
                 px = q.x - sx;
                 py = q.y - sy;
                 pz = q.z - sz;

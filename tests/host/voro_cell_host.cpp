@@ -76,8 +76,7 @@ int vc_host_tessellate(int n, const double* x, const double* y, const double* z,
     for (int c = 0; c < nc; ++c) {
         const int i = G->cell_site[c];
         vc_bind(cell, store.data(), nc, c);
-        const int rc = vc_compute(cell, dom, g, range.data(), sites.data(), site_cell.data(), i, x[i], y[i], z[i],
-                                  VC_TOL_DEFAULT);
+        const int rc = vc_compute(cell, dom, g, range.data(), sites.data(), i, x[i], y[i], z[i], VC_TOL_DEFAULT);
         if (rc != VC_CUT && !status) status = rc;
         vc_save(cell);
     }
