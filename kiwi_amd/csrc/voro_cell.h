@@ -50,6 +50,7 @@ constexpr int64_t VC_BYTES = static_cast<int64_t>(VC_MAXV) * (4 * 8 + 2 + 1) +
 
 struct VCell {
     int nv, nf, st;
+    double mr2;  // the largest squared distance of a vertex from the site, kept by vc_init_box and vc_cut
     double *x, *y, *z;  // vertices, relative to the site
     uint16_t* fv;       // face f's cycle: fv[(f * VC_MAXFV + k) * st], k < fn[f]
     uint8_t* fn;
@@ -137,16 +138,12 @@ KIWI_VHD void vc_init_box(VCell& c, const double dom[6], double sx, double sy, d
         c.FN(f) = 4;
         c.NBR(f) = -1 - f;
     }
-}
-
-// largest squared distance of a vertex from the site
-KIWI_VHD double vc_max_r2(const VCell& c) {
     double m = 0.0;
-    for (int v = 0; v < c.nv; ++v) {
+    for (int v = 0; v < 8; ++v) {
         const double r2 = c.X(v) * c.X(v) + c.Y(v) * c.Y(v) + c.Z(v) * c.Z(v);
         m = r2 > m ? r2 : m;
     }
-    return m;
+    c.mr2 = m;
 }
 
 // the vertex where the cut's plane crosses edge (a, b), made once per edge; -1 past the limits
@@ -283,17 +280,22 @@ KIWI_VHD int vc_cut(VCell& c, double px, double py, double pz, int nb, double to
         ++nf_out;
     }
     c.nf = nf_out;
-    // drop the vertices no face keeps
+    // drop the vertices no face keeps; the farthest kept vertex on the way
     int nvn = 0;
+    double m = 0.0;
     for (int v = 0; v < c.nv; ++v) {
         if (c.REMAP(v)) {
             c.REMAP(v) = static_cast<uint16_t>(nvn);
-            c.X(nvn) = c.X(v);
-            c.Y(nvn) = c.Y(v);
-            c.Z(nvn) = c.Z(v);
+            const double x = c.X(v), y = c.Y(v), z = c.Z(v);
+            c.X(nvn) = x;
+            c.Y(nvn) = y;
+            c.Z(nvn) = z;
+            const double r2 = x * x + y * y + z * z;
+            m = r2 > m ? r2 : m;
             ++nvn;
         }
     }
+    c.mr2 = m;
     for (int f = 0; f < c.nf; ++f)
         for (int k = 0; k < c.FN(f); ++k) c.FV(f, k) = c.REMAP(c.FV(f, k));
     c.nv = nvn;
@@ -449,7 +451,6 @@ KIWI_VHD bool vc_next_bucket(VcWalk& w, const VcGrid& g, const int b[3], const d
 KIWI_VHD int vc_compute(VCell& c, const double dom[6], const VcGrid& g, const int* range, const VcSite* sites,
                         int self, double sx, double sy, double sz, double tol) {
     vc_init_box(c, dom, sx, sy, sz);
-    double mr2 = vc_max_r2(c);
     const double p[3] = {sx, sy, sz};
     const int b[3] = {vc_bucket_axis(g, 0, sx), vc_bucket_axis(g, 1, sy), vc_bucket_axis(g, 2, sz)};
     VcWalk w{0, 0, 0, 0, 0, 0, true};
@@ -464,20 +465,17 @@ KIWI_VHD int vc_compute(VCell& c, const double dom[6], const VcGrid& g, const in
                 px = q.x - sx;
                 py = q.y - sy;
                 pz = q.z - sz;
-                if (px * px + py * py + pz * pz > 4.0 * mr2) continue;  // its plane misses the cell
+                if (px * px + py * py + pz * pz > 4.0 * c.mr2) continue;  // its plane misses the cell
                 if (vc_misses(c, px, py, pz, tol)) continue;
                 nb = q.idx;
                 found = true;
                 break;
             }
-            if (!vc_next_bucket(w, g, b, p, mr2, range)) break;
+            if (!vc_next_bucket(w, g, b, p, c.mr2, range)) break;
         }
         if (!found) break;
-        const int rc = vc_cut(c, px, py, pz, nb, tol);
-        if (rc == VC_CUT)
-            mr2 = vc_max_r2(c);
-        else if (rc != VC_UNCHANGED)
-            return rc;
+        const int rc = vc_cut(c, px, py, pz, nb, tol);  // keeps c.mr2
+        if (rc != VC_CUT && rc != VC_UNCHANGED) return rc;
     }
     return VC_CUT;
 }
